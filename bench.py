@@ -1,0 +1,200 @@
+"""Benchmark of the matrix-free SIPG DG hot path on one MI355X.
+
+Workload (one "step" = one GridOperator::jacobian_apply over the whole vector):
+SIPG Poisson on the unit cube, 64^3 hexahedra, Q3 DG (64 DoF/cell, 16,777,216 DoF),
+alpha 3, FP64, applied to a seeded uniform[-1,1) vector resident in HBM.
+
+The JSON line also carries:
+  roofline      algorithmic bytes per apply (16 B/DoF: read z, write y) over the
+                HIP-event-measured average launch duration, against 8 TB/s HBM;
+  cpu_baseline  the oracle (CPU restatement of the reference quadrature-loop
+                algorithm) timed on this host's cores on the same apply;
+  solve         time to a 1e-10 relative residual: plain FP64 CG/Jacobi on the
+                manufactured exp problem, and for the lognormal-kappa problem plain
+                FP64 CG vs FP32-inner CG inside FP64 iterative refinement.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve] [--no-cpu]
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "dune-pdelab_amd")
+HBM_PEAK_GBS = 8000.0
+
+
+def import_package():
+    spec = importlib.util.spec_from_file_location("dune_pdelab_amd", os.path.join(PKG, "__init__.py"),
+                                                  submodule_search_locations=[PKG])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["dune_pdelab_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def splitmix_uniform(n, seed):
+    """uniform[-1,1) from splitmix64 (the counter generator the CPU oracle uses)."""
+    s = (np.arange(1, n + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15) + np.uint64(seed))
+    t = s.copy()
+    t ^= t >> np.uint64(30)
+    t *= np.uint64(0xBF58476D1CE4E5B9)
+    t ^= t >> np.uint64(27)
+    t *= np.uint64(0x94D049BB133111EB)
+    t ^= t >> np.uint64(31)
+    return 2.0 * ((t >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)) - 1.0
+
+
+def lognormal_kappa(ncells, seed):
+    rng = np.random.default_rng(seed)
+    return np.exp(rng.normal(0.0, 1.0, ncells))
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(cfg, min_seconds):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_ffi
+
+    cores = len(os.sched_getaffinity(0))
+    cores = max(1, min(cores, int(os.environ.get("OMP_NUM_THREADS", cores))))
+    o = oracle_ffi.Oracle(cfg)
+    sec, reps = o.time_apply(nthreads=cores, min_seconds=min_seconds)
+    n = oracle_ffi.cfgmod.ndofs(cfg)
+    return {"value": n / sec / 1e6, "unit": "MDoF/s", "cores": cores, "kind": "port",
+            "sample": f"{reps} full jacobian_apply of the 64^3 Q3 FP64 workload (oracle quadrature-loop "
+                      f"restatement, OpenMP over cells with two-sided faces), {sec * reps:.1f} s total"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--degree", type=int, default=3)
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
+              "(the domain-decomposition design is described in DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    pkg = import_package()
+    n, k = args.n, args.degree
+    cfg = pkg.make_config(n=n, degree=k, alpha=3.0, problem=pkg.PROBLEM_EXP)
+    log(f"bench: creating 64^3-style grid n={n} k={k}")
+    go = pkg.GridOperator(cfg)
+    N = go.N
+    z = go.vector(splitmix_uniform(N, 20261202))
+    y = go.vector()
+
+    # ---- timed region: K applies, inputs resident in HBM
+    for _ in range(args.warmup):
+        go.jacobian_apply(z, y)
+    go.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        go.jacobian_apply(z, y)
+    go.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = N * args.steps / (t1 - t0) / 1e6
+    log(f"bench: {args.steps} applies, {ms_per_step:.4f} ms/apply, {value:.1f} MDoF/s")
+
+    # ---- per-launch kernel time with HIP events on the handle's stream
+    kern_ms = go.time_apply(z, y, warmup=3, reps=max(20, args.steps))
+    alg_bytes = 16.0 * N
+    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
+    z32 = go.vector(splitmix_uniform(N, 20261202).astype(np.float32), dtype=np.float32)
+    y32 = go.vector(dtype=np.float32)
+    kern32_ms = go.time_apply(z32, y32, warmup=3, reps=max(20, args.steps))
+    log(f"bench: kernel avg {kern_ms * 1e3:.1f} us (fp64), {kern32_ms * 1e3:.1f} us (fp32)")
+    del z32, y32
+
+    out = {
+        "metric": "MDoF/s DG apply; solve time to 1e-10 rel err, mixed-prec vs FP64, 1 GPU",
+        "value": value,
+        "unit": "MDoF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"SIPG Poisson -div(grad u)=f, unit cube, YaspGrid-style {n}^3 hexahedra, "
+                               f"Q{k} DG ({(k + 1) ** 3} DoF/cell, {N} DoF), alpha 3, FP64 jacobian_apply "
+                               f"on a seeded uniform[-1,1) vector",
+                   "cells": n ** 3, "degree": k, "dofs": N},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "dg_apply_kernel<4,double,...>", "kernel_avg_us": kern_ms * 1e3,
+                     "algorithmic_bytes_per_launch": alg_bytes,
+                     "fp32_kernel_avg_us": kern32_ms * 1e3,
+                     "fp32_achieved_GBs": 8.0 * N / (kern32_ms * 1e-3) / 1e9},
+    }
+
+    if not args.no_solve:
+        solve = {}
+        # plain FP64 CG/Jacobi on the manufactured problem (64^3 Q3)
+        x = go.vector()
+        log("bench: FP64 CG/Jacobi solve (exp problem)")
+        res = go.linear_problem_solve(x, method=pkg.SOLVER_CG, reduction=1e-10)
+        solve["exp_fp64_cg"] = {"seconds": res["solver"]["elapsed"], "iterations": res["solver"]["iterations"],
+                                "converged": bool(res["solver"]["converged"]),
+                                "reduction": res["solver"]["reduction"]}
+        x.free()
+        # lognormal kappa: plain FP64 vs mixed precision
+        z.free()
+        y.free()
+        go.close()
+        cfg3 = pkg.make_config(n=n, degree=k, alpha=3.0, kappa_per_cell=True, problem=pkg.PROBLEM_CONSTANT,
+                               f0=1.0, g0=0.0)
+        kappa = lognormal_kappa(n ** 3, 31337)
+        go3 = pkg.GridOperator(cfg3, kappa)
+        x64 = go3.vector()
+        log("bench: FP64 CG/Jacobi solve (lognormal kappa)")
+        r64 = go3.linear_problem_solve(x64, method=pkg.SOLVER_CG, reduction=1e-10, maxit=100000)
+        xm = go3.vector()
+        log("bench: mixed-precision solve (lognormal kappa)")
+        rm = go3.solve_mixed(xm, inner_reduction=1e-4, outer_reduction=1e-10, inner_maxit=100000)
+        a64 = x64.download()
+        am = xm.download()
+        rel = float(np.linalg.norm(am - a64) / np.linalg.norm(a64))
+        t64 = r64["assembler_time"] + r64["linear_solver_time"]
+        solve["lognormal_fp64_cg"] = {"seconds": t64, "iterations": r64["solver"]["iterations"],
+                                      "converged": bool(r64["solver"]["converged"]),
+                                      "reduction": r64["solver"]["reduction"]}
+        solve["lognormal_mixed"] = {"seconds": rm["elapsed"], "outer": rm["outer_iterations"],
+                                    "inner_iterations": rm["inner_iterations"], "converged": bool(rm["converged"]),
+                                    "reduction": rm["reduction"], "rel_diff_vs_fp64_gpu": rel,
+                                    "speedup_vs_fp64": t64 / rm["elapsed"] if rm["elapsed"] > 0 else None}
+        out["solve"] = solve
+        for v in (x64, xm):
+            v.free()
+        go3.close()
+    else:
+        z.free()
+        y.free()
+        go.close()
+
+    if not args.no_cpu:
+        log("bench: CPU baseline (oracle)")
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

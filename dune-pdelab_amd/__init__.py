@@ -1,0 +1,289 @@
+"""dune-pdelab_amd -- Python handle on the gfx950 matrix-free SIPG DG hot path.
+
+This module is a thin ctypes layer over the C-ABI library ``lib/libpdelab_gpu.so``
+(declared in include/pdelab_gpu.h); all numerics run in the HIP kernels. There is no
+CPU fallback: if the library is missing or no HIP device is present, the calls
+raise.
+
+Import it under the name ``dune_pdelab_amd`` (the directory name has a hyphen)::
+
+    spec = importlib.util.spec_from_file_location(
+        "dune_pdelab_amd", "<repo>/dune-pdelab_amd/__init__.py",
+        submodule_search_locations=["<repo>/dune-pdelab_amd"])
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from . import config as cfgmod
+from .config import (  # noqa: F401  (re-exported)
+    BC_DIRICHLET, BC_NEUMANN, BC_NONE, BC_OUTFLOW, IIPG, NIPG, PREC_JACOBI, PREC_NONE,
+    PROBLEM_CONSTANT, PROBLEM_EXP, PROBLEM_GAUSSIANS, PROBLEM_QUADRATIC, PROBLEM_TABULATED,
+    PROBLEM_ZERO, SIPG, SOLVER_BICGSTAB, SOLVER_CG, WEIGHTS_OFF, WEIGHTS_ON, LinearProblemResult,
+    LinearSolverResult, MixedResult, PdelabConfig, make_config, ncells, ndofs)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "lib", "libpdelab_gpu.so")
+
+_lib = None
+_vp = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class PdelabError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{cfgmod.STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+def build():
+    """Compiles lib/libpdelab_gpu.so for gfx950 (hipcc; no GPU needed)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise RuntimeError(f"HIP library not built: {LIBPATH} (run __graft_entry__.build())")
+    L = ctypes.CDLL(LIBPATH)
+    C = ctypes.POINTER(cfgmod.PdelabConfig)
+    L.pdelab_gpu_default_config.argtypes = [C]
+    L.pdelab_gpu_default_config.restype = None
+    L.pdelab_gpu_create.argtypes = [C, _dp, ctypes.POINTER(_vp)]
+    L.pdelab_gpu_destroy.argtypes = [_vp]
+    L.pdelab_gpu_last_error.argtypes = [_vp]
+    L.pdelab_gpu_last_error.restype = ctypes.c_char_p
+    for fn in ("pdelab_gpu_size", "pdelab_gpu_num_cells"):
+        getattr(L, fn).argtypes = [_vp]
+        getattr(L, fn).restype = ctypes.c_size_t
+    L.pdelab_gpu_is_linear.argtypes = [_vp]
+    L.pdelab_gpu_quadrature_points.argtypes = [_vp]
+    L.pdelab_gpu_set_kappa.argtypes = [_vp, _dp]
+    L.pdelab_gpu_set_cell_source.argtypes = [_vp, _dp]
+    L.pdelab_gpu_set_tabulated_data.argtypes = [_vp, _dp, _dp]
+    L.pdelab_gpu_malloc.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.c_size_t]
+    L.pdelab_gpu_free.argtypes = [_vp, _vp]
+    for fn in ("pdelab_gpu_memcpy_htod", "pdelab_gpu_memcpy_dtoh", "pdelab_gpu_memcpy_dtod"):
+        getattr(L, fn).argtypes = [_vp, _vp, _vp, ctypes.c_size_t]
+    L.pdelab_gpu_memset_zero.argtypes = [_vp, _vp, ctypes.c_size_t]
+    L.pdelab_gpu_synchronize.argtypes = [_vp]
+    for fn in ("pdelab_gpu_residual", "pdelab_gpu_jacobian_apply", "pdelab_gpu_jacobian_apply_f32",
+               "pdelab_gpu_jacobian_apply_transpose", "pdelab_gpu_mass_apply", "pdelab_gpu_grad_source"):
+        getattr(L, fn).argtypes = [_vp, _vp, _vp]
+    L.pdelab_gpu_jacobian_apply_nonlinear.argtypes = [_vp, _vp, _vp, _vp]
+    L.pdelab_gpu_point_diagonal.argtypes = [_vp, _vp]
+    L.pdelab_gpu_dot.argtypes = [_vp, _vp, _vp, _dp]
+    L.pdelab_gpu_norm.argtypes = [_vp, _vp, _dp]
+    L.pdelab_gpu_axpy.argtypes = [_vp, ctypes.c_double, _vp, _vp]
+    for fn in ("pdelab_gpu_solve", "pdelab_gpu_solve_transpose"):
+        getattr(L, fn).argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _vp, _vp,
+                                   ctypes.POINTER(cfgmod.LinearSolverResult)]
+    L.pdelab_gpu_linear_problem_solve.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                                  ctypes.c_double, ctypes.c_int, _vp,
+                                                  ctypes.POINTER(cfgmod.LinearProblemResult)]
+    L.pdelab_gpu_solve_mixed.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                         ctypes.c_double, ctypes.c_int, _vp, ctypes.POINTER(cfgmod.MixedResult)]
+    L.pdelab_gpu_grad_logkappa.argtypes = [_vp, _vp, _vp, _vp]
+    L.pdelab_gpu_time_apply.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, _dp]
+    _lib = L
+    return L
+
+
+class DeviceVector:
+    """A device buffer owned by a GridOperator handle (float64 or float32)."""
+
+    def __init__(self, go, n, dtype=np.float64):
+        self.go = go
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        ptr = _vp()
+        go._check(lib().pdelab_gpu_malloc(go.h, ctypes.byref(ptr), self.nbytes))
+        self.ptr = ptr
+        self.zero()
+
+    @property
+    def p(self):
+        return self.ptr
+
+    def zero(self):
+        self.go._check(lib().pdelab_gpu_memset_zero(self.go.h, self.ptr, self.nbytes))
+        return self
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n, (a.size, self.n)
+        self.go._check(lib().pdelab_gpu_memcpy_htod(self.go.h, self.ptr, a.ctypes.data_as(_vp), self.nbytes))
+        return self
+
+    def download(self):
+        out = np.empty(self.n, dtype=self.dtype)
+        self.go._check(lib().pdelab_gpu_memcpy_dtoh(self.go.h, out.ctypes.data_as(_vp), self.ptr, self.nbytes))
+        return out
+
+    def copy_from(self, other):
+        assert other.nbytes == self.nbytes
+        self.go._check(lib().pdelab_gpu_memcpy_dtod(self.go.h, self.ptr, other.ptr, self.nbytes))
+        return self
+
+    def free(self):
+        if getattr(self, "ptr", None) is not None and self.go.h:
+            lib().pdelab_gpu_free(self.go.h, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class GridOperator:
+    """GridOperator + ISTL-backend handle of the HIP implementation.
+
+    Mirrors GridOperator::residual / jacobian_apply
+    (R:dune/pdelab/gridoperator/gridoperator.hh:176-205) and the solver backends'
+    apply(z, r, reduction) (R:dune/pdelab/backend/istl/seqistlsolverbackend.hh:231-252).
+    """
+
+    def __init__(self, cfg, kappa=None):
+        L = lib()
+        self.cfg = cfg
+        self._children = []
+        kp = None
+        if kappa is not None:
+            self._kappa = np.ascontiguousarray(kappa, dtype=np.float64)
+            kp = self._kappa.ctypes.data_as(_dp)
+        h = _vp()
+        st = L.pdelab_gpu_create(ctypes.byref(cfg), kp, ctypes.byref(h))
+        if st != 0:
+            raise PdelabError(st, L.pdelab_gpu_last_error(None).decode())
+        self.h = h
+        self.N = L.pdelab_gpu_size(h)
+        self.ncells = L.pdelab_gpu_num_cells(h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pdelab_gpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != 0:
+            raise PdelabError(st, lib().pdelab_gpu_last_error(self.h).decode())
+
+    # -- vectors
+    def vector(self, data=None, dtype=np.float64):
+        v = DeviceVector(self, self.N, dtype)
+        if data is not None:
+            v.upload(data)
+        return v
+
+    def cell_vector(self, data=None):
+        v = DeviceVector(self, self.ncells, np.float64)
+        if data is not None:
+            v.upload(data)
+        return v
+
+    def synchronize(self):
+        self._check(lib().pdelab_gpu_synchronize(self.h))
+
+    # -- parameters
+    def set_kappa(self, kappa):
+        k = np.ascontiguousarray(kappa, dtype=np.float64)
+        self._check(lib().pdelab_gpu_set_kappa(self.h, k.ctypes.data_as(_dp)))
+
+    def set_cell_source(self, s):
+        if s is None:
+            self._check(lib().pdelab_gpu_set_cell_source(self.h, None))
+        else:
+            s = np.ascontiguousarray(s, dtype=np.float64)
+            self._check(lib().pdelab_gpu_set_cell_source(self.h, s.ctypes.data_as(_dp)))
+
+    def set_tabulated(self, fq, gq):
+        fq = np.ascontiguousarray(fq, dtype=np.float64)
+        gq = np.ascontiguousarray(gq, dtype=np.float64)
+        self._check(lib().pdelab_gpu_set_tabulated_data(self.h, fq.ctypes.data_as(_dp), gq.ctypes.data_as(_dp)))
+
+    def quadrature_points(self):
+        return lib().pdelab_gpu_quadrature_points(self.h)
+
+    # -- GridOperator
+    def residual(self, x, r):
+        self._check(lib().pdelab_gpu_residual(self.h, x.p, r.p))
+
+    def jacobian_apply(self, z, y):
+        if z.dtype == np.float32:
+            self._check(lib().pdelab_gpu_jacobian_apply_f32(self.h, z.p, y.p))
+        else:
+            self._check(lib().pdelab_gpu_jacobian_apply(self.h, z.p, y.p))
+
+    def jacobian_apply_nonlinear(self, x, z, y):
+        self._check(lib().pdelab_gpu_jacobian_apply_nonlinear(self.h, x.p, z.p, y.p))
+
+    def jacobian_apply_transpose(self, z, y):
+        self._check(lib().pdelab_gpu_jacobian_apply_transpose(self.h, z.p, y.p))
+
+    def point_diagonal(self, d):
+        self._check(lib().pdelab_gpu_point_diagonal(self.h, d.p))
+
+    def mass_apply(self, z, y):
+        self._check(lib().pdelab_gpu_mass_apply(self.h, z.p, y.p))
+
+    # -- vector backend
+    def dot(self, x, y):
+        out = ctypes.c_double()
+        self._check(lib().pdelab_gpu_dot(self.h, x.p, y.p, ctypes.byref(out)))
+        return out.value
+
+    def norm(self, x):
+        out = ctypes.c_double()
+        self._check(lib().pdelab_gpu_norm(self.h, x.p, ctypes.byref(out)))
+        return out.value
+
+    def axpy(self, a, x, y):
+        self._check(lib().pdelab_gpu_axpy(self.h, a, x.p, y.p))
+
+    # -- solvers
+    def solve(self, z, r, method=SOLVER_CG, precond=PREC_JACOBI, reduction=1e-10, maxit=5000, transpose=False):
+        res = LinearSolverResult()
+        fn = lib().pdelab_gpu_solve_transpose if transpose else lib().pdelab_gpu_solve
+        self._check(fn(self.h, method, precond, reduction, maxit, z.p, r.p, ctypes.byref(res)))
+        return res.as_dict()
+
+    def linear_problem_solve(self, x, method=SOLVER_CG, precond=PREC_JACOBI, reduction=1e-10, min_defect=1e-99,
+                             maxit=5000):
+        res = LinearProblemResult()
+        self._check(lib().pdelab_gpu_linear_problem_solve(self.h, method, precond, reduction, min_defect, maxit, x.p,
+                                                          ctypes.byref(res)))
+        return res.as_dict()
+
+    def solve_mixed(self, x, method=SOLVER_CG, precond=PREC_JACOBI, inner_reduction=1e-4, inner_maxit=5000,
+                    outer_reduction=1e-10, outer_maxit=50):
+        res = MixedResult()
+        self._check(lib().pdelab_gpu_solve_mixed(self.h, method, precond, inner_reduction, inner_maxit,
+                                                 outer_reduction, outer_maxit, x.p, ctypes.byref(res)))
+        return res.as_dict()
+
+    # -- adjoint gradients
+    def grad_logkappa(self, u, lam, g):
+        self._check(lib().pdelab_gpu_grad_logkappa(self.h, u.p, lam.p, g.p))
+
+    def grad_source(self, lam, g):
+        self._check(lib().pdelab_gpu_grad_source(self.h, lam.p, g.p))
+
+    # -- measurement
+    def time_apply(self, z, y, warmup=3, reps=20):
+        prec = 32 if z.dtype == np.float32 else 64
+        out = ctypes.c_double()
+        self._check(lib().pdelab_gpu_time_apply(self.h, prec, z.p, y.p, warmup, reps, ctypes.byref(out)))
+        return out.value

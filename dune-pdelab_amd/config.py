@@ -1,0 +1,146 @@
+"""ctypes mirror of ``pdelab_config`` and the result structs of include/pdelab_gpu.h.
+
+The field order must match the C header exactly; ``tests/test_boundary.py`` checks
+the struct size against the C compiler's.
+"""
+import ctypes
+
+ABI_VERSION = 1
+MAX_GAUSSIANS = 16
+
+# ConvectionDiffusionDGMethod / Weights / BoundaryConditions
+# (R:dune/pdelab/localoperator/convectiondiffusiondg.hh:29-37,
+#  R:dune/pdelab/localoperator/convectiondiffusionparameter.hh:111-115)
+NIPG, SIPG, IIPG = 0, 1, 2
+WEIGHTS_ON, WEIGHTS_OFF = 0, 1
+BC_DIRICHLET, BC_NEUMANN, BC_OUTFLOW, BC_NONE = 1, -1, -2, -3
+
+PROBLEM_ZERO, PROBLEM_EXP, PROBLEM_CONSTANT, PROBLEM_GAUSSIANS, PROBLEM_QUADRATIC, PROBLEM_TABULATED = range(6)
+SOLVER_CG, SOLVER_BICGSTAB = 0, 1
+PREC_NONE, PREC_JACOBI = 0, 1
+
+STATUS = {
+    0: "PDELAB_OK",
+    1: "PDELAB_ERR_INVALID_ARG",
+    2: "PDELAB_ERR_LINEARITY_MISMATCH",
+    3: "PDELAB_ERR_HIP",
+    4: "PDELAB_ERR_UNSUPPORTED",
+    5: "PDELAB_ERR_SOLVER_ABORT",
+    6: "PDELAB_ERR_OUTFLOW_ON_INFLOW",
+    7: "PDELAB_ERR_NO_DEVICE",
+}
+
+
+class PdelabConfig(ctypes.Structure):
+    _fields_ = [
+        ("abi_version", ctypes.c_int),
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("lx", ctypes.c_double), ("ly", ctypes.c_double), ("lz", ctypes.c_double),
+        ("degree", ctypes.c_int),
+        ("method", ctypes.c_int),
+        ("weights", ctypes.c_int),
+        ("alpha", ctypes.c_double),
+        ("intorderadd", ctypes.c_int),
+        ("kappa_per_cell", ctypes.c_int),
+        ("kappa0", ctypes.c_double),
+        ("b", ctypes.c_double * 3),
+        ("c", ctypes.c_double),
+        ("bctype", ctypes.c_int * 6),
+        ("problem", ctypes.c_int),
+        ("f0", ctypes.c_double), ("g0", ctypes.c_double),
+        ("quad", ctypes.c_double * 7),
+        ("n_gaussians", ctypes.c_int),
+        ("gaussians", (ctypes.c_double * 5) * MAX_GAUSSIANS),
+    ]
+
+
+class LinearSolverResult(ctypes.Structure):
+    """LinearSolverResult<double>, R:dune/pdelab/backend/solver.hh:29-51"""
+    _fields_ = [
+        ("converged", ctypes.c_int),
+        ("iterations", ctypes.c_uint),
+        ("elapsed", ctypes.c_double),
+        ("reduction", ctypes.c_double),
+        ("conv_rate", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class LinearProblemResult(ctypes.Structure):
+    """StationaryLinearProblemSolverResult, R:dune/pdelab/stationary/linearproblem.hh:27-44"""
+    _fields_ = [
+        ("solver", LinearSolverResult),
+        ("first_defect", ctypes.c_double),
+        ("defect", ctypes.c_double),
+        ("assembler_time", ctypes.c_double),
+        ("linear_solver_time", ctypes.c_double),
+        ("linear_solver_iterations", ctypes.c_int),
+    ]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f != "solver"}
+        d["solver"] = self.solver.as_dict()
+        return d
+
+
+class MixedResult(ctypes.Structure):
+    _fields_ = [
+        ("converged", ctypes.c_int),
+        ("outer_iterations", ctypes.c_int),
+        ("inner_iterations", ctypes.c_int),
+        ("first_defect", ctypes.c_double),
+        ("defect", ctypes.c_double),
+        ("reduction", ctypes.c_double),
+        ("elapsed", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def make_config(n=4, degree=1, nx=None, ny=None, nz=None, lx=1.0, ly=1.0, lz=1.0,
+                method=SIPG, weights=WEIGHTS_ON, alpha=1.0, intorderadd=0,
+                kappa_per_cell=False, kappa0=1.0, b=(0.0, 0.0, 0.0), c=0.0,
+                bctype=(0, 0, 0, 0, 0, 0), problem=PROBLEM_ZERO, f0=0.0, g0=0.0,
+                quad=(0.0,) * 7, gaussians=()):
+    """Config with the reference defaults (SIPG, weightsOn, alpha=1,
+    convectiondiffusiondg.hh:85-90), overridable by keyword."""
+    cfg = PdelabConfig()
+    cfg.abi_version = ABI_VERSION
+    cfg.nx = nx if nx is not None else n
+    cfg.ny = ny if ny is not None else n
+    cfg.nz = nz if nz is not None else n
+    cfg.lx, cfg.ly, cfg.lz = lx, ly, lz
+    cfg.degree = degree
+    cfg.method = method
+    cfg.weights = weights
+    cfg.alpha = alpha
+    cfg.intorderadd = intorderadd
+    cfg.kappa_per_cell = 1 if kappa_per_cell else 0
+    cfg.kappa0 = kappa0
+    for i in range(3):
+        cfg.b[i] = b[i]
+    cfg.c = c
+    for i in range(6):
+        cfg.bctype[i] = bctype[i]
+    cfg.problem = problem
+    cfg.f0, cfg.g0 = f0, g0
+    for i in range(7):
+        cfg.quad[i] = quad[i]
+    if len(gaussians) > MAX_GAUSSIANS:
+        raise ValueError("too many gaussians")
+    cfg.n_gaussians = len(gaussians)
+    for i, gsn in enumerate(gaussians):
+        for j in range(5):
+            cfg.gaussians[i][j] = gsn[j]
+    return cfg
+
+
+def ncells(cfg):
+    return cfg.nx * cfg.ny * cfg.nz
+
+
+def ndofs(cfg):
+    return ncells(cfg) * (cfg.degree + 1) ** 3

@@ -1,0 +1,198 @@
+// blas_kernels.hh -- vector backend kernels (ISTL::BlockVector dot / two_norm / axpy,
+// R:dune/pdelab/backend/istl/vector.hh:249-282) and the fused Krylov updates.
+//
+// Reductions are deterministic: a fixed grid of kRedBlocks blocks walks the vector
+// with a fixed stride, each block reduces in a fixed tree (wave64 shuffles, then LDS)
+// and one block sums the per-block partials in order. Results are therefore
+// bitwise reproducible run to run.
+#ifndef PDELAB_BLAS_KERNELS_HH
+#define PDELAB_BLAS_KERNELS_HH
+
+#include <hip/hip_runtime.h>
+
+namespace pdelab_amd {
+
+constexpr int kRedBlocks = 1024;
+constexpr int kRedThreads = 256;
+
+template <typename A>
+__device__ __forceinline__ A wave_sum(A v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Block-wide sum of NR accumulators; result valid in thread 0.
+template <typename A, int NR>
+__device__ __forceinline__ void block_sum(A (&v)[NR]) {
+  __shared__ A red[NR][kRedThreads / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < NR; ++j) {
+    v[j] = wave_sum(v[j]);
+    if (lane == 0) red[j][wid] = v[j];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+      A s = red[j][0];
+      for (int w = 1; w < kRedThreads / 64; ++w) s += red[j][w];
+      v[j] = s;
+    }
+  }
+}
+
+// partial[b*NR + j] = block b's share of sum_i f_j(i)
+template <typename T, typename A>
+__global__ __launch_bounds__(kRedThreads) void dot_partial_kernel(const T* __restrict__ x, const T* __restrict__ y,
+                                                                  long long n, A* __restrict__ partial) {
+  A v[1] = {A(0)};
+  for (long long i = (long long)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (long long)kRedBlocks * kRedThreads)
+    v[0] += (A)x[i] * (A)y[i];
+  block_sum<A, 1>(v);
+  if (threadIdx.x == 0) partial[blockIdx.x] = v[0];
+}
+
+// out[slot + j] = sum_b partial[b*NR + j]   (single block of kRedThreads threads)
+template <typename A, int NR>
+__global__ __launch_bounds__(kRedThreads) void final_reduce_kernel(const A* __restrict__ partial, int nb,
+                                                                   A* __restrict__ out) {
+  A v[NR];
+#pragma unroll
+  for (int j = 0; j < NR; ++j) v[j] = A(0);
+  for (int b = threadIdx.x; b < nb; b += kRedThreads)
+#pragma unroll
+    for (int j = 0; j < NR; ++j) v[j] += partial[b * NR + j];
+  block_sum<A, NR>(v);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int j = 0; j < NR; ++j) out[j] = v[j];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void axpy_kernel(long long n, T a, const T* __restrict__ x, T* __restrict__ y) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    y[i] += a * x[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void scale_kernel(long long n, T a, T* __restrict__ x) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    x[i] *= a;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void fill_kernel(long long n, T a, T* __restrict__ x) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    x[i] = a;
+}
+
+template <typename S, typename D>
+__global__ __launch_bounds__(256) void convert_kernel(long long n, const S* __restrict__ x, D* __restrict__ y) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    y[i] = (D)x[i];
+}
+
+// x -= (double) z32   (refinement update)
+__global__ __launch_bounds__(256) void sub_f32_kernel(long long n, const float* __restrict__ z, double* __restrict__ x) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    x[i] -= (double)z[i];
+}
+
+// dinv = 1 / d
+template <typename T>
+__global__ __launch_bounds__(256) void invert_kernel(long long n, const double* __restrict__ d, T* __restrict__ dinv) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    dinv[i] = (T)(1.0 / d[i]);
+}
+
+// y = dinv .* x  (SeqJac with one step and relaxation 1 from a zero start, or the
+// identity when dinv == nullptr); also returns partials of y . x
+template <typename T, typename A>
+__global__ __launch_bounds__(kRedThreads) void prec_dot_kernel(long long n, const T* __restrict__ dinv,
+                                                               const T* __restrict__ x, T* __restrict__ y,
+                                                               A* __restrict__ partial) {
+  A v[1] = {A(0)};
+  for (long long i = (long long)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (long long)kRedBlocks * kRedThreads) {
+    const T xi = x[i];
+    const T yi = dinv ? dinv[i] * xi : xi;
+    y[i] = yi;
+    v[0] += (A)yi * (A)xi;
+  }
+  block_sum<A, 1>(v);
+  if (threadIdx.x == 0) partial[blockIdx.x] = v[0];
+}
+
+// CG step: lambda = rho/alpha (device scalars); x += lambda p; r -= lambda q;
+// partials of r.r and (dinv r).r
+template <typename T, typename A>
+__global__ __launch_bounds__(kRedThreads) void cg_update_kernel(long long n, T* __restrict__ x, const T* __restrict__ p,
+                                                                T* __restrict__ r, const T* __restrict__ q,
+                                                                const T* __restrict__ dinv, const A* __restrict__ scal,
+                                                                int slot_rho, int slot_alpha, A* __restrict__ partial) {
+  const T lambda = (T)(scal[slot_rho] / scal[slot_alpha]);
+  A v[2] = {A(0), A(0)};
+  for (long long i = (long long)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (long long)kRedBlocks * kRedThreads) {
+    x[i] += lambda * p[i];
+    const T ri = r[i] - lambda * q[i];
+    r[i] = ri;
+    const T zi = dinv ? dinv[i] * ri : ri;
+    v[0] += (A)ri * (A)ri;
+    v[1] += (A)zi * (A)ri;
+  }
+  block_sum<A, 2>(v);
+  if (threadIdx.x == 0) { partial[2 * blockIdx.x] = v[0]; partial[2 * blockIdx.x + 1] = v[1]; }
+}
+
+// CG direction update: p = dinv r + beta p
+template <typename T>
+__global__ __launch_bounds__(256) void cg_pupdate_kernel(long long n, T* __restrict__ p, const T* __restrict__ r,
+                                                         const T* __restrict__ dinv, T beta) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const T zi = dinv ? dinv[i] * r[i] : r[i];
+    p[i] = zi + beta * p[i];
+  }
+}
+
+// BiCGSTAB: p = r + beta (p - omega v)   (or p = r when first)
+template <typename T>
+__global__ __launch_bounds__(256) void bicg_p_kernel(long long n, T* __restrict__ p, const T* __restrict__ r,
+                                                     const T* __restrict__ v, T beta, T omega, int first) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = first ? r[i] : beta * (p[i] - omega * v[i]) + r[i];
+}
+
+// x += a y; r -= a v; partials of r.r
+template <typename T, typename A>
+__global__ __launch_bounds__(kRedThreads) void xr_update_kernel(long long n, T* __restrict__ x, const T* __restrict__ y,
+                                                                T* __restrict__ r, const T* __restrict__ v, T a,
+                                                                A* __restrict__ partial) {
+  A s[1] = {A(0)};
+  for (long long i = (long long)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (long long)kRedBlocks * kRedThreads) {
+    x[i] += a * y[i];
+    const T ri = r[i] - a * v[i];
+    r[i] = ri;
+    s[0] += (A)ri * (A)ri;
+  }
+  block_sum<A, 1>(s);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+// partials of t.r and t.t
+template <typename T, typename A>
+__global__ __launch_bounds__(kRedThreads) void dot2_kernel(long long n, const T* __restrict__ t, const T* __restrict__ r,
+                                                           A* __restrict__ partial) {
+  A s[2] = {A(0), A(0)};
+  for (long long i = (long long)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (long long)kRedBlocks * kRedThreads) {
+    const A ti = (A)t[i];
+    s[0] += ti * (A)r[i];
+    s[1] += ti * ti;
+  }
+  block_sum<A, 2>(s);
+  if (threadIdx.x == 0) { partial[2 * blockIdx.x] = s[0]; partial[2 * blockIdx.x + 1] = s[1]; }
+}
+
+}  // namespace pdelab_amd
+
+#endif

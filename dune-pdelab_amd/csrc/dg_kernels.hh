@@ -1,0 +1,352 @@
+// dg_kernels.hh -- gfx950 kernels of the SIPG/NIPG/IIPG DG operator (ConvectionDiffusionDG)
+// on a structured hexahedral grid, in tensor-product ("Kronecker") form.
+//
+// Operator structure (exact for affine axis-aligned cells, cell-constant kappa,
+// constant b and c; see dg_tables.hh for the 1D matrices):
+//
+//   J = sum_d  (M (x) M)_{tangential to d} (x) A_d  +  V c M (x) M (x) M
+//
+// where A_d is the 1D DG operator along grid lines in direction d: per cell the
+// volume part S_d*kappa*K - Sc_d*b_d*G and the face parts of
+// R:dune/pdelab/localoperator/convectiondiffusiondg.hh:271-471 (interior) and
+// :684-879 (boundary), S_d = V/h_d^2, Sc_d = V/h_d, V = hx*hy*hz. Every face term is
+// a rank-one combination of the traces e0/e1 and normal derivatives d0/d1, so a
+// line only needs its own values plus the trace value and trace derivative of the
+// two neighbouring lines. The kernel evaluates
+//
+//   y (+)= (M (x) M (x) M) * sum_d [ M^{-1} A_d z ]_d
+//
+// with the M^{-1}-premultiplied 1D tables, so each direction is a register-local
+// line operation and the tangential mass is applied once at the end.
+//
+// Work decomposition: a 256-thread workgroup handles CPB = 256/(k+1)^2 consecutive
+// cells; within a cell, thread (p,q) owns one grid line per phase (z-, y-, then
+// x-lines). The per-cell partial sums live in LDS. Global loads in each layout
+// touch whole 128-byte segments of the cell block.
+#ifndef PDELAB_DG_KERNELS_HH
+#define PDELAB_DG_KERNELS_HH
+
+#include <hip/hip_runtime.h>
+
+namespace pdelab_amd {
+
+enum { BC_DIRICHLET = 1, BC_NEUMANN = -1, BC_OUTFLOW = -2, BC_NONE = -3 };
+
+template <typename T>
+struct DGArgs {
+  T MiK[4][4], MiG[4][4], MiGt[4][4], M[4][4];
+  T Mie0[4], Mie1[4], Mid0[4], Mid1[4], d0[4], d1[4];
+  T Mdiag[4], Kdiag[4], Gdiag[4], e0d0[4], e1d1[4];
+  T S[3], Sc[3], b[3];
+  T V, c, theta, alphak, kappa0;
+  // constant-kappa face coefficients (interior: w of lower cell, w of upper cell,
+  // penalty; boundary penalty)
+  T wI_lo, wI_hi, PI, PB;
+  int n[3];
+  long long stride[3];
+  long long ncells;
+  int weightsOn;
+  int bc[6];
+};
+
+// Maps the hardware block index to a logical block so that each of the 8 XCDs
+// (blocks are dispatched round-robin over XCDs) works on one contiguous range of
+// cells: neighbouring lines in y and z then hit the same XCD's L2.
+__device__ __forceinline__ long long xcd_logical_block(long long bid, long long nb) {
+  const long long q = nb >> 3, r = nb & 7;
+  const long long x = bid & 7, i = bid >> 3;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+// Interior face coefficients, seen from the inside cell (upper, larger index) as the
+// reference assembler does (R:dune/pdelab/gridoperator/default/assembler.hh:181):
+// omega and harmonic average convectiondiffusiondg.hh:326-338, penalty :346.
+template <typename T>
+__device__ __forceinline__ void face_coeffs(const DGArgs<T>& a, T klo, T khi, T& wlo, T& whi, T& P) {
+  if (a.weightsOn) {
+    const T den = khi + klo + T(1e-20);
+    const T omega_s = klo / den, omega_n = khi / den;
+    whi = omega_s * khi;
+    wlo = omega_n * klo;
+    P = a.alphak * (T(2.0) * khi * klo / den);
+  } else {
+    whi = T(0.5) * khi;
+    wlo = T(0.5) * klo;
+    P = a.alphak;
+  }
+}
+
+template <int N1, int D>
+__device__ __forceinline__ int line_index(int p, int q, int e) {
+  constexpr int L = N1 * N1;
+  return D == 0 ? e + N1 * p + L * q : (D == 1 ? p + N1 * e + L * q : p + N1 * q + L * e);
+}
+
+// s = M^{-1} (A_d z) on one line of cell c (or M^{-1} (A_d^T z) for TRANS).
+template <int N1, int D, typename T, bool TRANS, bool CONV, bool REACT, bool VARK>
+__device__ __forceinline__ void line_op(const T* __restrict__ z, const T* __restrict__ kappa, const DGArgs<T>& a,
+                                        long long c, const int coord[3], int p, int q, T kc, T s[N1]) {
+  constexpr int NL = N1 * N1 * N1;
+  T u[N1];
+  const T* zc = z + c * NL;
+#pragma unroll
+  for (int e = 0; e < N1; ++e) u[e] = zc[line_index<N1, D>(p, q, e)];
+  const T val0 = u[0], val1 = u[N1 - 1];
+  T der0 = T(0), der1 = T(0);
+#pragma unroll
+  for (int e = 0; e < N1; ++e) { der0 += a.d0[e] * u[e]; der1 += a.d1[e] * u[e]; }
+  const T Sd = a.S[D], Sc = a.Sc[D], bd = a.b[D], theta = a.theta;
+  T ce0 = T(0), cd0 = T(0), ce1 = T(0), cd1 = T(0);
+  // ---- lower face
+  if (coord[D] > 0) {
+    const long long cl = c - a.stride[D];
+    const T* zl = z + cl * NL;
+    T ul[N1];
+#pragma unroll
+    for (int e = 0; e < N1; ++e) ul[e] = zl[line_index<N1, D>(p, q, e)];
+    const T v1l = ul[N1 - 1];
+    T d1l = T(0);
+#pragma unroll
+    for (int e = 0; e < N1; ++e) d1l += a.d1[e] * ul[e];
+    T wlo, whi, P;
+    if (VARK) face_coeffs<T>(a, kappa[cl], kc, wlo, whi, P);
+    else { wlo = a.wI_lo; whi = a.wI_hi; P = a.PI; }
+    const T J = v1l - val0;           // jump u_lo - u_hi
+    const T G = wlo * d1l + whi * der0;  // weighted average of the normal derivative
+    if (!TRANS) { ce0 += Sd * (G - P * J); cd0 += Sd * theta * whi * J; }
+    else { ce0 += Sd * (-P * J - theta * G); cd0 += -Sd * whi * J; }
+    if (CONV) {
+      if (!TRANS) ce0 += Sc * (-bd) * (bd > T(0) ? v1l : val0);
+      else if (bd <= T(0)) ce0 += Sc * (-bd) * (val0 - v1l);
+    }
+  } else {
+    const int bct = a.bc[2 * D];
+    if (bct == BC_DIRICHLET) {
+      const T Pb = VARK ? a.alphak * (a.weightsOn ? kc : T(1)) : a.PB;
+      if (!TRANS) { ce0 += Sd * (kc * der0 + Pb * val0); cd0 += -Sd * theta * kc * val0; }
+      else { ce0 += Sd * (Pb * val0 - theta * kc * der0); cd0 += Sd * kc * val0; }
+      if (CONV && bd <= T(0)) ce0 += Sc * (-bd) * val0;
+    } else if (bct == BC_OUTFLOW) {
+      if (CONV) ce0 += Sc * (-bd) * val0;
+    }
+  }
+  // ---- upper face
+  if (coord[D] < a.n[D] - 1) {
+    const long long cr = c + a.stride[D];
+    const T* zr = z + cr * NL;
+    T ur[N1];
+#pragma unroll
+    for (int e = 0; e < N1; ++e) ur[e] = zr[line_index<N1, D>(p, q, e)];
+    const T v0r = ur[0];
+    T d0r = T(0);
+#pragma unroll
+    for (int e = 0; e < N1; ++e) d0r += a.d0[e] * ur[e];
+    T wlo, whi, P;
+    if (VARK) face_coeffs<T>(a, kc, kappa[cr], wlo, whi, P);
+    else { wlo = a.wI_lo; whi = a.wI_hi; P = a.PI; }
+    const T J = val1 - v0r;
+    const T G = wlo * der1 + whi * d0r;
+    if (!TRANS) { ce1 += Sd * (P * J - G); cd1 += Sd * theta * wlo * J; }
+    else { ce1 += Sd * (P * J + theta * G); cd1 += -Sd * wlo * J; }
+    if (CONV) {
+      if (!TRANS) ce1 += Sc * bd * (bd > T(0) ? val1 : v0r);
+      else if (bd > T(0)) ce1 += Sc * bd * (val1 - v0r);
+    }
+  } else {
+    const int bct = a.bc[2 * D + 1];
+    if (bct == BC_DIRICHLET) {
+      const T Pb = VARK ? a.alphak * (a.weightsOn ? kc : T(1)) : a.PB;
+      if (!TRANS) { ce1 += Sd * (Pb * val1 - kc * der1); cd1 += Sd * theta * kc * val1; }
+      else { ce1 += Sd * (Pb * val1 + theta * kc * der1); cd1 += -Sd * kc * val1; }
+      if (CONV && bd >= T(0)) ce1 += Sc * bd * val1;
+    } else if (bct == BC_OUTFLOW) {
+      if (CONV) ce1 += Sc * bd * val1;
+    }
+  }
+  // ---- volume + assembly of the face terms
+  const T sk = Sd * kc;
+  const T sb = -Sc * bd;
+#pragma unroll
+  for (int i = 0; i < N1; ++i) {
+    T acc = ce0 * a.Mie0[i] + cd0 * a.Mid0[i] + ce1 * a.Mie1[i] + cd1 * a.Mid1[i];
+    T kv = T(0);
+#pragma unroll
+    for (int j = 0; j < N1; ++j) kv += a.MiK[i][j] * u[j];
+    acc += sk * kv;
+    if (CONV) {
+      T gv = T(0);
+#pragma unroll
+      for (int j = 0; j < N1; ++j) gv += (TRANS ? a.MiGt[i][j] : a.MiG[i][j]) * u[j];
+      acc += sb * gv;
+    }
+    if (REACT && D == 0) acc += a.V * a.c * u[i];
+    s[i] = acc;
+  }
+}
+
+// y (+)= J z   (or J^T z).  ACCUM: add into y (GridOperator::jacobian_apply semantics);
+// otherwise y = J z.
+template <int N1, typename T, bool TRANS, bool CONV, bool REACT, bool VARK, bool ACCUM>
+__global__ __launch_bounds__(256) void dg_apply_kernel(const T* __restrict__ z, T* __restrict__ y,
+                                                       const T* __restrict__ kappa, const DGArgs<T> a,
+                                                       long long nblocks) {
+  constexpr int L = N1 * N1, NL = L * N1, CPB = 256 / L;
+  __shared__ T S[CPB * NL];
+  const int t = threadIdx.x;
+  const int lc = t / L, l = t % L, p = l % N1, q = l / N1;
+  const long long blk = xcd_logical_block(blockIdx.x, nblocks);
+  const long long c = blk * CPB + lc;
+  const bool active = (lc < CPB) && (c < a.ncells);
+  int coord[3] = {0, 0, 0};
+  T kc = a.kappa0;
+  if (active) {
+    coord[0] = (int)(c % a.n[0]);
+    coord[1] = (int)((c / a.n[0]) % a.n[1]);
+    coord[2] = (int)(c / ((long long)a.n[0] * a.n[1]));
+    if (VARK) kc = kappa[c];
+  }
+  T* Sl = S + (active ? lc * NL : 0);
+  T s[N1];
+  if (active) {
+    line_op<N1, 2, T, TRANS, CONV, REACT, VARK>(z, kappa, a, c, coord, p, q, kc, s);
+#pragma unroll
+    for (int e = 0; e < N1; ++e) Sl[line_index<N1, 2>(p, q, e)] = s[e];
+  }
+  __syncthreads();
+  if (active) {
+    line_op<N1, 1, T, TRANS, CONV, REACT, VARK>(z, kappa, a, c, coord, p, q, kc, s);
+#pragma unroll
+    for (int e = 0; e < N1; ++e) Sl[line_index<N1, 1>(p, q, e)] += s[e];
+  }
+  __syncthreads();
+  if (active) {
+    line_op<N1, 0, T, TRANS, CONV, REACT, VARK>(z, kappa, a, c, coord, p, q, kc, s);
+    T v[N1];
+#pragma unroll
+    for (int e = 0; e < N1; ++e) v[e] = s[e] + Sl[line_index<N1, 0>(p, q, e)];
+#pragma unroll
+    for (int i = 0; i < N1; ++i) {
+      T acc = T(0);
+#pragma unroll
+      for (int j = 0; j < N1; ++j) acc += a.M[i][j] * v[j];
+      Sl[line_index<N1, 0>(p, q, i)] = acc;
+    }
+  }
+  __syncthreads();
+  if (active) {
+    T v[N1];
+#pragma unroll
+    for (int e = 0; e < N1; ++e) v[e] = Sl[line_index<N1, 1>(p, q, e)];
+#pragma unroll
+    for (int i = 0; i < N1; ++i) {
+      T acc = T(0);
+#pragma unroll
+      for (int j = 0; j < N1; ++j) acc += a.M[i][j] * v[j];
+      Sl[line_index<N1, 1>(p, q, i)] = acc;
+    }
+  }
+  __syncthreads();
+  if (active) {
+    T v[N1];
+#pragma unroll
+    for (int e = 0; e < N1; ++e) v[e] = Sl[line_index<N1, 2>(p, q, e)];
+    T* yc = y + c * NL;
+#pragma unroll
+    for (int i = 0; i < N1; ++i) {
+      T acc = T(0);
+#pragma unroll
+      for (int j = 0; j < N1; ++j) acc += a.M[i][j] * v[j];
+      const int idx = line_index<N1, 2>(p, q, i);
+      if (ACCUM) yc[idx] += acc;
+      else yc[idx] = acc;
+    }
+  }
+}
+
+// d += diag(J) (PointDiagonalLocalOperatorWrapper, R:dune/pdelab/localoperator/
+// pointdiagonalwrapper.hh:51-131): per DoF, sum over directions of the own-cell 1D
+// diagonal times the tangential mass diagonals. One thread per DoF.
+template <int N1, typename T, bool VARK>
+__global__ __launch_bounds__(256) void dg_diag_kernel(T* __restrict__ d, const T* __restrict__ kappa,
+                                                      const DGArgs<T> a) {
+  constexpr int L = N1 * N1, NL = L * N1;
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= a.ncells * NL) return;
+  const long long c = gid / NL;
+  const int li = (int)(gid % NL);
+  const int ii[3] = {li % N1, (li / N1) % N1, li / L};
+  const int coord[3] = {(int)(c % a.n[0]), (int)((c / a.n[0]) % a.n[1]), (int)(c / ((long long)a.n[0] * a.n[1]))};
+  const T kc = VARK ? kappa[c] : a.kappa0;
+  const T theta = a.theta;
+  T acc = T(0);
+  for (int D = 0; D < 3; ++D) {
+    const int i = ii[D];
+    const T Sd = a.S[D], Sc = a.Sc[D], bd = a.b[D];
+    const T e0 = (i == 0) ? T(1) : T(0), e1 = (i == N1 - 1) ? T(1) : T(0);
+    const T e0d0 = a.e0d0[i], e1d1 = a.e1d1[i];
+    T dd = Sd * kc * a.Kdiag[i] - Sc * bd * a.Gdiag[i];
+    if (coord[D] > 0) {
+      T wlo, whi, P;
+      if (VARK) face_coeffs<T>(a, kappa[c - a.stride[D]], kc, wlo, whi, P);
+      else { wlo = a.wI_lo; whi = a.wI_hi; P = a.PI; }
+      dd += Sd * (whi * e0d0 + P * e0 - theta * whi * e0d0);
+      if (bd <= T(0)) dd += Sc * (-bd) * e0;
+    } else {
+      const int bct = a.bc[2 * D];
+      if (bct == BC_DIRICHLET) {
+        const T Pb = VARK ? a.alphak * (a.weightsOn ? kc : T(1)) : a.PB;
+        dd += Sd * (kc * e0d0 + Pb * e0 - theta * kc * e0d0);
+        if (bd <= T(0)) dd += Sc * (-bd) * e0;
+      } else if (bct == BC_OUTFLOW) {
+        dd += Sc * (-bd) * e0;
+      }
+    }
+    if (coord[D] < a.n[D] - 1) {
+      T wlo, whi, P;
+      if (VARK) face_coeffs<T>(a, kc, kappa[c + a.stride[D]], wlo, whi, P);
+      else { wlo = a.wI_lo; whi = a.wI_hi; P = a.PI; }
+      dd += Sd * (-wlo * e1d1 + P * e1 + theta * wlo * e1d1);
+      if (bd > T(0)) dd += Sc * bd * e1;
+    } else {
+      const int bct = a.bc[2 * D + 1];
+      if (bct == BC_DIRICHLET) {
+        const T Pb = VARK ? a.alphak * (a.weightsOn ? kc : T(1)) : a.PB;
+        dd += Sd * (-kc * e1d1 + Pb * e1 + theta * kc * e1d1);
+        if (bd >= T(0)) dd += Sc * bd * e1;
+      } else if (bct == BC_OUTFLOW) {
+        dd += Sc * bd * e1;
+      }
+    }
+    T mt = T(1);
+    for (int e = 0; e < 3; ++e)
+      if (e != D) mt *= a.Mdiag[ii[e]];
+    acc += dd * mt;
+  }
+  acc += a.V * a.c * a.Mdiag[ii[0]] * a.Mdiag[ii[1]] * a.Mdiag[ii[2]];
+  d[gid] += acc;
+}
+
+// y += V (M (x) M (x) M) z, the DG mass matrix (ScalarL2, R:dune/pdelab/localoperator/l2.hh:45-83).
+template <int N1, typename T>
+__global__ __launch_bounds__(256) void dg_mass_kernel(const T* __restrict__ z, T* __restrict__ y, const DGArgs<T> a) {
+  constexpr int L = N1 * N1, NL = L * N1;
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= a.ncells * NL) return;
+  const long long c = gid / NL;
+  const int li = (int)(gid % NL);
+  const int i0 = li % N1, i1 = (li / N1) % N1, i2 = li / L;
+  const T* zc = z + c * NL;
+  T acc = T(0);
+  for (int j2 = 0; j2 < N1; ++j2)
+    for (int j1 = 0; j1 < N1; ++j1) {
+      T m12 = a.M[i1][j1] * a.M[i2][j2];
+      T row = T(0);
+      for (int j0 = 0; j0 < N1; ++j0) row += a.M[i0][j0] * zc[j0 + N1 * j1 + L * j2];
+      acc += m12 * row;
+    }
+  y[gid] += a.V * acc;
+}
+
+}  // namespace pdelab_amd
+
+#endif

@@ -1,0 +1,1011 @@
+// pdelab_gpu.hip -- C-ABI of the gfx950 matrix-free DG hot path (see include/pdelab_gpu.h).
+//
+// One handle = one GridOperator + its solver backends on one HIP device and one HIP
+// stream. All vectors are device pointers in PDELab's cell-blocked DoF layout.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pdelab_gpu.h"
+#include "blas_kernels.hh"
+#include "dg_kernels.hh"
+#include "dg_load.hh"
+#include "dg_tables.hh"
+
+using namespace pdelab_amd;
+
+namespace {
+thread_local std::string g_create_error;
+
+struct Workspace {
+  std::vector<void*> bufs;
+};
+}  // namespace
+
+struct pdelab_gpu_gridoperator {
+  pdelab_config cfg;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  Tables1D tab;
+  DGArgs<double> a64;
+  DGArgs<float> a32;
+  LoadArgs la;
+  GradArgs ga;
+  long long ncells = 0, nloc = 0, N = 0;
+  int n1 = 2;
+  bool conv = false, vark = false;
+  double* d_kappa = nullptr;
+  float* d_kappa32 = nullptr;
+  double* d_R0 = nullptr;
+  bool R0_valid = false;
+  double* d_src = nullptr;
+  double* d_ftab = nullptr;
+  double* d_gtab = nullptr;
+  double* d_diag = nullptr;
+  double* d_dinv = nullptr;
+  float* d_dinv32 = nullptr;
+  bool diag_valid = false;
+  double* w64[8] = {nullptr};
+  float* w32[8] = {nullptr};
+  double* d_part64 = nullptr;
+  float* d_part32 = nullptr;
+  double* d_scal64 = nullptr;
+  float* d_scal32 = nullptr;
+  double* h_scal64 = nullptr;
+  float* h_scal32 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+using H = pdelab_gpu_gridoperator;
+
+#define HIPCHK(h, expr)                                                              \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      (h)->err = std::string(#expr " failed: ") + hipGetErrorString(e_);             \
+      return PDELAB_ERR_HIP;                                                         \
+    }                                                                                \
+  } while (0)
+
+#define CHK(...)                     \
+  do {                               \
+    int s_ = (__VA_ARGS__);          \
+    if (s_ != PDELAB_OK) return s_;  \
+  } while (0)
+
+static int fail(H* h, int code, const std::string& msg) {
+  if (h) h->err = msg;
+  return code;
+}
+
+static int grid_for(long long n) {
+  long long b = (n + 255) / 256;
+  return (int)std::max(1LL, std::min(b, 4096LL * 8));
+}
+
+// ---------------------------------------------------------------------------
+// setup of the kernel argument blocks
+template <typename T>
+static void fill_args(H* h, DGArgs<T>& a) {
+  const Tables1D& t = h->tab;
+  const int n1 = t.n1;
+  std::memset(&a, 0, sizeof(a));
+  for (int i = 0; i < n1; ++i) {
+    for (int j = 0; j < n1; ++j) {
+      long double mk = 0, mg = 0, mgt = 0;
+      for (int l = 0; l < n1; ++l) {
+        mk += t.Minv[i][l] * t.K[l][j];
+        mg += t.Minv[i][l] * t.G[l][j];
+        mgt += t.Minv[i][l] * t.G[j][l];
+      }
+      a.MiK[i][j] = (T)mk;
+      a.MiG[i][j] = (T)mg;
+      a.MiGt[i][j] = (T)mgt;
+      a.M[i][j] = (T)t.M[i][j];
+    }
+    long double me0 = 0, me1 = 0, md0 = 0, md1 = 0;
+    for (int l = 0; l < n1; ++l) {
+      me0 += t.Minv[i][l] * t.e0[l];
+      me1 += t.Minv[i][l] * t.e1[l];
+      md0 += t.Minv[i][l] * t.d0[l];
+      md1 += t.Minv[i][l] * t.d1[l];
+    }
+    a.Mie0[i] = (T)me0; a.Mie1[i] = (T)me1; a.Mid0[i] = (T)md0; a.Mid1[i] = (T)md1;
+    a.d0[i] = (T)t.d0[i]; a.d1[i] = (T)t.d1[i];
+    a.Mdiag[i] = (T)t.M[i][i]; a.Kdiag[i] = (T)t.K[i][i]; a.Gdiag[i] = (T)t.G[i][i];
+    a.e0d0[i] = (T)(t.e0[i] * t.d0[i]);
+    a.e1d1[i] = (T)(t.e1[i] * t.d1[i]);
+  }
+  const pdelab_config& c = h->cfg;
+  const double hh[3] = {c.lx / c.nx, c.ly / c.ny, c.lz / c.nz};
+  const double V = hh[0] * hh[1] * hh[2];
+  for (int d = 0; d < 3; ++d) {
+    a.S[d] = (T)(V / (hh[d] * hh[d]));
+    a.Sc[d] = (T)(V / hh[d]);
+    a.b[d] = (T)c.b[d];
+  }
+  a.V = (T)V;
+  a.c = (T)c.c;
+  // convectiondiffusiondg.hh:99-101
+  double theta = 1.0;
+  if (c.method == PDELAB_SIPG) theta = -1.0;
+  if (c.method == PDELAB_IIPG) theta = 0.0;
+  a.theta = (T)theta;
+  const int k = c.degree;
+  a.alphak = (T)(c.alpha * (double)(k * (k + 3 - 1)));
+  a.kappa0 = (T)c.kappa0;
+  a.weightsOn = c.weights == PDELAB_WEIGHTS_ON ? 1 : 0;
+  // constant-kappa face coefficients, same formulas as face_coeffs()
+  {
+    const T klo = (T)c.kappa0, khi = (T)c.kappa0;
+    if (a.weightsOn) {
+      const T den = khi + klo + T(1e-20);
+      const T omega_s = klo / den, omega_n = khi / den;
+      a.wI_hi = omega_s * khi;
+      a.wI_lo = omega_n * klo;
+      a.PI = a.alphak * (T(2.0) * khi * klo / den);
+      a.PB = a.alphak * (T)c.kappa0;
+    } else {
+      a.wI_hi = T(0.5) * khi;
+      a.wI_lo = T(0.5) * klo;
+      a.PI = a.alphak;
+      a.PB = a.alphak;
+    }
+  }
+  a.n[0] = c.nx; a.n[1] = c.ny; a.n[2] = c.nz;
+  a.stride[0] = 1;
+  a.stride[1] = c.nx;
+  a.stride[2] = (long long)c.nx * c.ny;
+  a.ncells = h->ncells;
+  for (int s = 0; s < 6; ++s) a.bc[s] = c.bctype[s] == 0 ? BC_DIRICHLET : c.bctype[s];
+}
+
+static void fill_load_args(H* h) {
+  LoadArgs& a = h->la;
+  std::memset(&a, 0, sizeof(a));
+  const pdelab_config& c = h->cfg;
+  const Tables1D& t = h->tab;
+  a.problem = c.problem;
+  a.kappa0 = c.kappa0;
+  for (int d = 0; d < 3; ++d) a.b[d] = c.b[d];
+  a.c = c.c;
+  a.f0 = c.f0;
+  a.g0 = c.g0;
+  for (int i = 0; i < 7; ++i) a.quad[i] = c.quad[i];
+  a.n_gauss = c.n_gaussians;
+  for (int i = 0; i < c.n_gaussians && i < 16; ++i)
+    for (int j = 0; j < 5; ++j) a.gauss[i][j] = c.gaussians[i][j];
+  a.n[0] = c.nx; a.n[1] = c.ny; a.n[2] = c.nz;
+  a.ncells = h->ncells;
+  a.h[0] = c.lx / c.nx; a.h[1] = c.ly / c.ny; a.h[2] = c.lz / c.nz;
+  a.V = a.h[0] * a.h[1] * a.h[2];
+  a.m = t.m;
+  for (int q = 0; q < t.m; ++q) {
+    a.qx[q] = (double)t.qx[q];
+    a.qw[q] = (double)t.qw[q];
+    for (int i = 0; i < t.n1; ++i) a.P[q][i] = (double)t.P[q][i];
+  }
+  for (int i = 0; i < t.n1; ++i) {
+    a.e0[i] = (double)t.e0[i]; a.e1[i] = (double)t.e1[i];
+    a.d0[i] = (double)t.d0[i]; a.d1[i] = (double)t.d1[i];
+  }
+  a.theta = (double)h->a64.theta;
+  a.alphak = (double)h->a64.alphak;
+  a.weightsOn = h->a64.weightsOn;
+  for (int s = 0; s < 6; ++s) a.bc[s] = h->a64.bc[s];
+  a.bface_offset[0] = 0;
+  for (int s = 0; s < 6; ++s) {
+    int d = s / 2, t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+    a.bface_offset[s + 1] = a.bface_offset[s] + (long long)a.n[t1] * a.n[t2];
+  }
+  GradArgs& g = h->ga;
+  std::memset(&g, 0, sizeof(g));
+  for (int d = 0; d < 3; ++d) { g.n[d] = a.n[d]; g.stride[d] = h->a64.stride[d]; g.S[d] = (double)h->a64.S[d]; }
+  g.ncells = h->ncells;
+  for (int i = 0; i < t.n1; ++i) {
+    for (int j = 0; j < t.n1; ++j) { g.M[i][j] = (double)t.M[i][j]; g.K[i][j] = (double)t.K[i][j]; }
+    g.e0[i] = a.e0[i]; g.e1[i] = a.e1[i]; g.d0[i] = a.d0[i]; g.d1[i] = a.d1[i];
+  }
+  g.theta = a.theta;
+  g.alphak = a.alphak;
+  for (int s = 0; s < 6; ++s) g.bc[s] = a.bc[s];
+}
+
+// ---------------------------------------------------------------------------
+// kernel dispatch
+template <int N1, typename T, bool TRANS, bool CONV, bool VARK, bool ACCUM>
+static int launch_apply_t(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, bool react) {
+  constexpr int L = N1 * N1, CPB = 256 / L;
+  const long long nb = (h->ncells + CPB - 1) / CPB;
+  if (react)
+    hipLaunchKernelGGL((dg_apply_kernel<N1, T, TRANS, CONV, true, VARK, ACCUM>), dim3((unsigned)nb), dim3(256), 0,
+                       h->stream, z, y, kappa, a, nb);
+  else
+    hipLaunchKernelGGL((dg_apply_kernel<N1, T, TRANS, CONV, false, VARK, ACCUM>), dim3((unsigned)nb), dim3(256), 0,
+                       h->stream, z, y, kappa, a, nb);
+  HIPCHK(h, hipGetLastError());
+  return PDELAB_OK;
+}
+
+template <int N1, typename T, bool TRANS, bool CONV, bool VARK>
+static int launch_apply_a(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, bool react, bool accum) {
+  return accum ? launch_apply_t<N1, T, TRANS, CONV, VARK, true>(h, z, y, kappa, a, react)
+               : launch_apply_t<N1, T, TRANS, CONV, VARK, false>(h, z, y, kappa, a, react);
+}
+template <int N1, typename T, bool TRANS, bool CONV>
+static int launch_apply_v(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, bool react, bool accum) {
+  return h->vark ? launch_apply_a<N1, T, TRANS, CONV, true>(h, z, y, kappa, a, react, accum)
+                 : launch_apply_a<N1, T, TRANS, CONV, false>(h, z, y, kappa, a, react, accum);
+}
+template <int N1, typename T, bool TRANS>
+static int launch_apply_c(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, bool react, bool accum) {
+  return h->conv ? launch_apply_v<N1, T, TRANS, true>(h, z, y, kappa, a, react, accum)
+                 : launch_apply_v<N1, T, TRANS, false>(h, z, y, kappa, a, react, accum);
+}
+template <int N1, typename T>
+static int launch_apply_n(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, bool trans, bool accum) {
+  const bool react = h->cfg.c != 0.0;
+  return trans ? launch_apply_c<N1, T, true>(h, z, y, kappa, a, react, accum)
+               : launch_apply_c<N1, T, false>(h, z, y, kappa, a, react, accum);
+}
+
+static int check_outflow(H* h) {
+  for (int s = 0; s < 6; ++s) {
+    if (h->a64.bc[s] != BC_OUTFLOW) continue;
+    const int d = s / 2;
+    const double nf = (s % 2 ? 1.0 : -1.0) * h->cfg.b[d];
+    if (nf < -1e-30) return fail(h, PDELAB_ERR_OUTFLOW_ON_INFLOW, "Outflow boundary condition on inflow!");
+  }
+  return PDELAB_OK;
+}
+
+static int apply64(H* h, const double* z, double* y, bool trans, bool accum) {
+  CHK(check_outflow(h));
+  switch (h->n1) {
+    case 2: return launch_apply_n<2, double>(h, z, y, h->d_kappa, h->a64, trans, accum);
+    case 3: return launch_apply_n<3, double>(h, z, y, h->d_kappa, h->a64, trans, accum);
+    case 4: return launch_apply_n<4, double>(h, z, y, h->d_kappa, h->a64, trans, accum);
+  }
+  return fail(h, PDELAB_ERR_UNSUPPORTED, "degree");
+}
+static int apply32(H* h, const float* z, float* y, bool trans, bool accum) {
+  CHK(check_outflow(h));
+  switch (h->n1) {
+    case 2: return launch_apply_n<2, float>(h, z, y, h->d_kappa32, h->a32, trans, accum);
+    case 3: return launch_apply_n<3, float>(h, z, y, h->d_kappa32, h->a32, trans, accum);
+    case 4: return launch_apply_n<4, float>(h, z, y, h->d_kappa32, h->a32, trans, accum);
+  }
+  return fail(h, PDELAB_ERR_UNSUPPORTED, "degree");
+}
+template <typename T>
+static int apply_t(H* h, const T* z, T* y, bool trans, bool accum);
+template <>
+int apply_t<double>(H* h, const double* z, double* y, bool trans, bool accum) { return apply64(h, z, y, trans, accum); }
+template <>
+int apply_t<float>(H* h, const float* z, float* y, bool trans, bool accum) { return apply32(h, z, y, trans, accum); }
+
+static int compute_R0(H* h) {
+  if (h->R0_valid) return PDELAB_OK;
+  if (!h->d_R0) HIPCHK(h, hipMalloc(&h->d_R0, sizeof(double) * h->N));
+  HIPCHK(h, hipMemsetAsync(h->d_R0, 0, sizeof(double) * h->N, h->stream));
+  const long long n = h->N;
+  const int nb = (int)((n + 255) / 256);
+  switch (h->n1) {
+    case 2: hipLaunchKernelGGL(load_volume_kernel<2>, dim3(nb), dim3(256), 0, h->stream, h->d_R0, h->d_ftab, h->d_src, h->la); break;
+    case 3: hipLaunchKernelGGL(load_volume_kernel<3>, dim3(nb), dim3(256), 0, h->stream, h->d_R0, h->d_ftab, h->d_src, h->la); break;
+    case 4: hipLaunchKernelGGL(load_volume_kernel<4>, dim3(nb), dim3(256), 0, h->stream, h->d_R0, h->d_ftab, h->d_src, h->la); break;
+  }
+  HIPCHK(h, hipGetLastError());
+  const bool has_g = h->cfg.problem != PDELAB_PROBLEM_ZERO;
+  if (has_g) {
+    for (int s = 0; s < 6; ++s) {
+      if (h->a64.bc[s] != BC_DIRICHLET) continue;
+      const int d = s / 2, t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+      const long long nf = (long long)h->la.n[t1] * h->la.n[t2] * h->nloc;
+      const int gb = (int)((nf + 255) / 256);
+      switch (h->n1) {
+        case 2: hipLaunchKernelGGL(load_boundary_kernel<2>, dim3(gb), dim3(256), 0, h->stream, h->d_R0, h->d_gtab, h->d_kappa, h->la, s); break;
+        case 3: hipLaunchKernelGGL(load_boundary_kernel<3>, dim3(gb), dim3(256), 0, h->stream, h->d_R0, h->d_gtab, h->d_kappa, h->la, s); break;
+        case 4: hipLaunchKernelGGL(load_boundary_kernel<4>, dim3(gb), dim3(256), 0, h->stream, h->d_R0, h->d_gtab, h->d_kappa, h->la, s); break;
+      }
+      HIPCHK(h, hipGetLastError());
+    }
+  }
+  h->R0_valid = true;
+  return PDELAB_OK;
+}
+
+static int compute_diag(H* h) {
+  if (h->diag_valid) return PDELAB_OK;
+  const long long n = h->N;
+  if (!h->d_diag) HIPCHK(h, hipMalloc(&h->d_diag, sizeof(double) * n));
+  if (!h->d_dinv) HIPCHK(h, hipMalloc(&h->d_dinv, sizeof(double) * n));
+  if (!h->d_dinv32) HIPCHK(h, hipMalloc(&h->d_dinv32, sizeof(float) * n));
+  HIPCHK(h, hipMemsetAsync(h->d_diag, 0, sizeof(double) * n, h->stream));
+  const int nb = (int)((n + 255) / 256);
+#define DIAGL(N1_)                                                                                           \
+  if (h->vark) hipLaunchKernelGGL((dg_diag_kernel<N1_, double, true>), dim3(nb), dim3(256), 0, h->stream, h->d_diag, h->d_kappa, h->a64); \
+  else hipLaunchKernelGGL((dg_diag_kernel<N1_, double, false>), dim3(nb), dim3(256), 0, h->stream, h->d_diag, h->d_kappa, h->a64);
+  switch (h->n1) {
+    case 2: DIAGL(2); break;
+    case 3: DIAGL(3); break;
+    case 4: DIAGL(4); break;
+  }
+#undef DIAGL
+  HIPCHK(h, hipGetLastError());
+  hipLaunchKernelGGL(invert_kernel<double>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, h->d_diag, h->d_dinv);
+  hipLaunchKernelGGL(invert_kernel<float>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, h->d_diag, h->d_dinv32);
+  HIPCHK(h, hipGetLastError());
+  h->diag_valid = true;
+  return PDELAB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// reductions
+template <typename A>
+static A* partial_buf(H* h);
+template <>
+double* partial_buf<double>(H* h) { return h->d_part64; }
+template <>
+float* partial_buf<float>(H* h) { return h->d_part32; }
+template <typename A>
+static A* scal_buf(H* h);
+template <>
+double* scal_buf<double>(H* h) { return h->d_scal64; }
+template <>
+float* scal_buf<float>(H* h) { return h->d_scal32; }
+template <typename A>
+static A* hscal_buf(H* h);
+template <>
+double* hscal_buf<double>(H* h) { return h->h_scal64; }
+template <>
+float* hscal_buf<float>(H* h) { return h->h_scal32; }
+
+template <typename A, int NR>
+static int finish_reduce(H* h, int slot) {
+  hipLaunchKernelGGL((final_reduce_kernel<A, NR>), dim3(1), dim3(kRedThreads), 0, h->stream, partial_buf<A>(h),
+                     kRedBlocks, scal_buf<A>(h) + slot);
+  HIPCHK(h, hipGetLastError());
+  return PDELAB_OK;
+}
+
+template <typename A>
+static int read_scalars(H* h, A* out, int count) {
+  HIPCHK(h, hipMemcpyAsync(hscal_buf<A>(h), scal_buf<A>(h), sizeof(A) * count, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  std::memcpy(out, hscal_buf<A>(h), sizeof(A) * count);
+  return PDELAB_OK;
+}
+
+template <typename T, typename A>
+static int dot_dev(H* h, const T* x, const T* y, long long n, int slot) {
+  hipLaunchKernelGGL((dot_partial_kernel<T, A>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, x, y, n,
+                     partial_buf<A>(h));
+  HIPCHK(h, hipGetLastError());
+  return finish_reduce<A, 1>(h, slot);
+}
+
+// ---------------------------------------------------------------------------
+// dune-istl IterativeSolver::Iteration semantics
+struct Iter {
+  double def0 = 0, def = 0, red, i = 0;
+  bool converged = false, aborted = false;
+  explicit Iter(double r) : red(r) {}
+  bool step(double it, double d) {
+    if (!std::isfinite(d)) { aborted = true; def = d; i = it; converged = false; return true; }
+    if (it == 0) def0 = d;
+    def = d;
+    i = it;
+    converged = (d < def0 * red) || (d < 1e-30);
+    return converged;
+  }
+  void finalize(pdelab_linear_solver_result* r, double el) const {
+    r->converged = converged ? 1 : 0;
+    r->iterations = (unsigned)std::ceil(i);
+    r->reduction = def0 > 0 ? def / def0 : 0.0;
+    r->conv_rate = std::pow(r->reduction, 1.0 / std::max(1.0, std::ceil(i)));
+    r->elapsed = el;
+  }
+};
+
+template <typename T>
+static T** ws(H* h);
+template <>
+double** ws<double>(H* h) { return h->w64; }
+template <>
+float** ws<float>(H* h) { return h->w32; }
+
+template <typename T>
+static int ensure_ws(H* h, int count) {
+  T** w = ws<T>(h);
+  for (int i = 0; i < count; ++i)
+    if (!w[i]) HIPCHK(h, hipMalloc(&w[i], sizeof(T) * h->N));
+  return PDELAB_OK;
+}
+
+template <typename T>
+static const T* dinv_for(H* h, int precond);
+template <>
+const double* dinv_for<double>(H* h, int precond) { return precond == PDELAB_PREC_JACOBI ? h->d_dinv : nullptr; }
+template <>
+const float* dinv_for<float>(H* h, int precond) { return precond == PDELAB_PREC_JACOBI ? h->d_dinv32 : nullptr; }
+
+// CGSolver::apply (dune-istl): solves J z = b from the given z; b becomes the defect.
+// Workspace slots base, base+1 (p, q).
+template <typename T, typename A>
+static int cg_solve(H* h, T* x, T* b, double reduction, int maxit, int precond, bool trans, int base,
+                    pdelab_linear_solver_result* res) {
+  auto t0 = std::chrono::steady_clock::now();
+  const long long n = h->N;
+  CHK(ensure_ws<T>(h, base + 2));
+  T* p = ws<T>(h)[base];
+  T* q = ws<T>(h)[base + 1];
+  const T* dinv = dinv_for<T>(h, precond);
+  Iter it(reduction);
+  // b -= J x  (applyscaleadd(-1, x, b))
+  CHK(apply_t<T>(h, x, q, trans, false));
+  hipLaunchKernelGGL(axpy_kernel<T>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, T(-1), (const T*)q, b);
+  CHK(dot_dev<T, A>(h, b, b, n, 1));
+  A sc[8];
+  CHK(read_scalars<A>(h, sc, 2));
+  double def = std::sqrt((double)sc[1]);
+  if (!it.step(0, def)) {
+    // p = W^{-1} b; rho = p.b  -> slot 7
+    hipLaunchKernelGGL((prec_dot_kernel<T, A>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, n, dinv,
+                       (const T*)b, p, partial_buf<A>(h));
+    CHK(finish_reduce<A, 1>(h, 7));
+    A rho = 0;
+    for (int i = 1; i <= maxit; ++i) {
+      const int rslot = (i & 1) ? 7 : 5;   // slot holding the current rho
+      const int oslot = (i & 1) ? 4 : 6;   // (def^2, rho_new) written here
+      CHK(apply_t<T>(h, p, q, trans, false));
+      CHK(dot_dev<T, A>(h, p, q, n, 0));   // alpha -> slot 0
+      hipLaunchKernelGGL((cg_update_kernel<T, A>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, n, x,
+                         (const T*)p, b, (const T*)q, dinv, (const A*)scal_buf<A>(h), rslot, 0, partial_buf<A>(h));
+      CHK(finish_reduce<A, 2>(h, oslot));
+      CHK(read_scalars<A>(h, sc, 8));
+      def = std::sqrt((double)sc[oslot]);
+      if (it.step(i, def)) break;
+      const A rho_new = sc[oslot + 1];
+      rho = sc[rslot];
+      const T beta = (T)(rho_new / rho);
+      hipLaunchKernelGGL(cg_pupdate_kernel<T>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, p, (const T*)b, dinv,
+                         beta);
+      HIPCHK(h, hipGetLastError());
+    }
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  it.finalize(res, el);
+  if (it.aborted) return fail(h, PDELAB_ERR_SOLVER_ABORT, "CG: defect is infinite or NaN");
+  return PDELAB_OK;
+}
+
+// BiCGSTABSolver::apply (dune-istl) with half-step iteration counting.
+// Workspace slots base .. base+4 (p, v, t, y, rt).
+template <typename T, typename A>
+static int bicgstab_solve(H* h, T* x, T* r, double reduction, int maxit, int precond, bool trans, int base,
+                          pdelab_linear_solver_result* res) {
+  auto t0 = std::chrono::steady_clock::now();
+  const long long n = h->N;
+  CHK(ensure_ws<T>(h, base + 5));
+  T *p = ws<T>(h)[base], *v = ws<T>(h)[base + 1], *t = ws<T>(h)[base + 2], *y = ws<T>(h)[base + 3],
+    *rt = ws<T>(h)[base + 4];
+  const T* dinv = dinv_for<T>(h, precond);
+  const double EPSILON = 1e-80;
+  Iter iter(reduction);
+  int status = PDELAB_OK;
+  CHK(apply_t<T>(h, x, t, trans, false));
+  hipLaunchKernelGGL(axpy_kernel<T>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, T(-1), (const T*)t, r);
+  HIPCHK(h, hipMemcpyAsync(rt, r, sizeof(T) * n, hipMemcpyDeviceToDevice, h->stream));
+  CHK(dot_dev<T, A>(h, r, r, n, 0));
+  A sc[4];
+  CHK(read_scalars<A>(h, sc, 1));
+  double norm = std::sqrt((double)sc[0]);
+  double rho_new_next = (double)sc[0];  // rt.r with rt == r
+  if (!iter.step(0, norm)) {
+    double rho = 1, alpha = 1, omega = 1, rho_new, beta, hh;
+    for (double itc = 0.5; itc < maxit; itc += .5) {
+      rho_new = rho_new_next;
+      if (std::fabs(rho) <= EPSILON) { status = fail(h, PDELAB_ERR_SOLVER_ABORT, "breakdown in BiCGSTAB - rho"); break; }
+      if (std::fabs(omega) <= EPSILON) { status = fail(h, PDELAB_ERR_SOLVER_ABORT, "breakdown in BiCGSTAB - omega"); break; }
+      if (itc < 1) {
+        hipLaunchKernelGGL(bicg_p_kernel<T>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, p, (const T*)r, (const T*)v,
+                           T(0), T(0), 1);
+      } else {
+        beta = (norm == 0.0) ? 0.0 : (rho_new / rho) * (alpha / omega);
+        hipLaunchKernelGGL(bicg_p_kernel<T>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, p, (const T*)r, (const T*)v,
+                           (T)beta, (T)omega, 0);
+      }
+      // y = W^{-1} p ; v = J y ; h = rt.v
+      hipLaunchKernelGGL((prec_dot_kernel<T, A>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, n, dinv,
+                         (const T*)p, y, partial_buf<A>(h));
+      CHK(apply_t<T>(h, y, v, trans, false));
+      CHK(dot_dev<T, A>(h, rt, v, n, 0));
+      CHK(read_scalars<A>(h, sc, 1));
+      hh = (double)sc[0];
+      if (std::fabs(hh) < EPSILON) { status = fail(h, PDELAB_ERR_SOLVER_ABORT, "abs(h) < EPSILON in BiCGSTAB"); break; }
+      alpha = (norm == 0.0) ? 0.0 : rho_new / hh;
+      hipLaunchKernelGGL((xr_update_kernel<T, A>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, n, x,
+                         (const T*)y, r, (const T*)v, (T)alpha, partial_buf<A>(h));
+      CHK(finish_reduce<A, 1>(h, 0));
+      CHK(read_scalars<A>(h, sc, 1));
+      norm = std::sqrt((double)sc[0]);
+      if (iter.step(itc, norm)) break;
+      itc += .5;
+      // y = W^{-1} r ; t = J y ; omega = t.r / t.t
+      hipLaunchKernelGGL((prec_dot_kernel<T, A>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, n, dinv,
+                         (const T*)r, y, partial_buf<A>(h));
+      CHK(apply_t<T>(h, y, t, trans, false));
+      hipLaunchKernelGGL((dot2_kernel<T, A>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, n, (const T*)t,
+                         (const T*)r, partial_buf<A>(h));
+      CHK(finish_reduce<A, 2>(h, 0));
+      CHK(read_scalars<A>(h, sc, 2));
+      omega = (norm == 0.0) ? 0.0 : (double)sc[0] / (double)sc[1];
+      hipLaunchKernelGGL((xr_update_kernel<T, A>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, n, x,
+                         (const T*)y, r, (const T*)t, (T)omega, partial_buf<A>(h));
+      CHK(finish_reduce<A, 1>(h, 0));
+      CHK(dot_dev<T, A>(h, rt, r, n, 1));
+      CHK(read_scalars<A>(h, sc, 2));
+      rho = rho_new;
+      norm = std::sqrt((double)sc[0]);
+      rho_new_next = (double)sc[1];
+      if (iter.step(itc, norm)) break;
+    }
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  iter.finalize(res, el);
+  if (status != PDELAB_OK) { res->converged = 0; return status; }
+  if (iter.aborted) return fail(h, PDELAB_ERR_SOLVER_ABORT, "BiCGSTAB: defect is infinite or NaN");
+  return PDELAB_OK;
+}
+
+template <typename T, typename A>
+static int krylov(H* h, int method, int precond, double reduction, int maxit, bool trans, T* z, T* r, int base,
+                  pdelab_linear_solver_result* res) {
+  if (maxit <= 0) maxit = 5000;
+  if (precond == PDELAB_PREC_JACOBI) CHK(compute_diag(h));
+  else if (precond != PDELAB_PREC_NONE) return fail(h, PDELAB_ERR_INVALID_ARG, "unknown preconditioner");
+  if (method == PDELAB_SOLVER_CG) return cg_solve<T, A>(h, z, r, reduction, maxit, precond, trans, base, res);
+  if (method == PDELAB_SOLVER_BICGSTAB) return bicgstab_solve<T, A>(h, z, r, reduction, maxit, precond, trans, base, res);
+  return fail(h, PDELAB_ERR_INVALID_ARG, "unknown Krylov method");
+}
+
+static int validate(const pdelab_config* c, std::string& msg) {
+  if (!c) { msg = "null config"; return PDELAB_ERR_INVALID_ARG; }
+  if (c->abi_version != PDELAB_GPU_ABI_VERSION) { msg = "ABI version mismatch"; return PDELAB_ERR_INVALID_ARG; }
+  if (c->nx < 1 || c->ny < 1 || c->nz < 1) { msg = "cells per direction must be >= 1"; return PDELAB_ERR_INVALID_ARG; }
+  if (!(c->lx > 0 && c->ly > 0 && c->lz > 0)) { msg = "domain extents must be > 0"; return PDELAB_ERR_INVALID_ARG; }
+  if (c->degree < 1 || c->degree > 3) { msg = "degree must be 1, 2 or 3"; return PDELAB_ERR_UNSUPPORTED; }
+  if (c->method < 0 || c->method > 2) { msg = "unknown DG method"; return PDELAB_ERR_INVALID_ARG; }
+  if (c->weights < 0 || c->weights > 1) { msg = "unknown weights"; return PDELAB_ERR_INVALID_ARG; }
+  if (c->intorderadd < 0) { msg = "intorderadd must be >= 0"; return PDELAB_ERR_INVALID_ARG; }
+  if (c->problem < 0 || c->problem > PDELAB_PROBLEM_TABULATED) { msg = "unknown problem"; return PDELAB_ERR_INVALID_ARG; }
+  if (c->n_gaussians < 0 || c->n_gaussians > PDELAB_MAX_GAUSSIANS) { msg = "n_gaussians"; return PDELAB_ERR_INVALID_ARG; }
+  for (int s = 0; s < 6; ++s) {
+    int b = c->bctype[s];
+    if (!(b == 0 || b == PDELAB_BC_DIRICHLET || b == PDELAB_BC_NEUMANN || b == PDELAB_BC_OUTFLOW || b == PDELAB_BC_NONE)) {
+      msg = "unknown boundary condition type";
+      return PDELAB_ERR_INVALID_ARG;
+    }
+  }
+  if (c->kappa_per_cell == 0 && !(c->kappa0 > 0)) { msg = "kappa0 must be > 0"; return PDELAB_ERR_INVALID_ARG; }
+  return PDELAB_OK;
+}
+
+static int upload_kappa(H* h, const double* kappa_host) {
+  const size_t nc = (size_t)h->ncells;
+  for (size_t i = 0; i < nc; ++i)
+    if (!(kappa_host[i] > 0) || !std::isfinite(kappa_host[i]))
+      return fail(h, PDELAB_ERR_INVALID_ARG, "kappa must be finite and > 0");
+  if (!h->d_kappa) HIPCHK(h, hipMalloc(&h->d_kappa, sizeof(double) * nc));
+  if (!h->d_kappa32) HIPCHK(h, hipMalloc(&h->d_kappa32, sizeof(float) * nc));
+  HIPCHK(h, hipMemcpyAsync(h->d_kappa, kappa_host, sizeof(double) * nc, hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for((long long)nc)), dim3(256), 0, h->stream,
+                     (long long)nc, (const double*)h->d_kappa, h->d_kappa32);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->vark = true;
+  h->R0_valid = false;
+  h->diag_valid = false;
+  return PDELAB_OK;
+}
+
+// ===========================================================================
+extern "C" {
+
+void pdelab_gpu_default_config(pdelab_config* cfg) {
+  if (!cfg) return;
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->abi_version = PDELAB_GPU_ABI_VERSION;
+  cfg->nx = cfg->ny = cfg->nz = 4;
+  cfg->lx = cfg->ly = cfg->lz = 1.0;
+  cfg->degree = 1;
+  cfg->method = PDELAB_SIPG;
+  cfg->weights = PDELAB_WEIGHTS_ON;
+  cfg->alpha = 1.0;
+  cfg->intorderadd = 0;
+  cfg->kappa_per_cell = 0;
+  cfg->kappa0 = 1.0;
+  cfg->problem = PDELAB_PROBLEM_ZERO;
+}
+
+int pdelab_gpu_create(const pdelab_config* cfg, const double* kappa_host, pdelab_gpu_gridoperator** out) {
+  g_create_error.clear();
+  if (!out) { g_create_error = "null output pointer"; return PDELAB_ERR_INVALID_ARG; }
+  *out = nullptr;
+  std::string msg;
+  int st = validate(cfg, msg);
+  if (st) { g_create_error = msg; return st; }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) { g_create_error = "no HIP device"; return PDELAB_ERR_NO_DEVICE; }
+  H* h = new H;
+  h->cfg = *cfg;
+  if (!build_tables(cfg->degree, cfg->intorderadd, h->tab)) {
+    g_create_error = "table construction failed";
+    delete h;
+    return PDELAB_ERR_UNSUPPORTED;
+  }
+  h->n1 = cfg->degree + 1;
+  h->ncells = (long long)cfg->nx * cfg->ny * cfg->nz;
+  h->nloc = (long long)h->n1 * h->n1 * h->n1;
+  h->N = h->ncells * h->nloc;
+  h->conv = cfg->b[0] != 0.0 || cfg->b[1] != 0.0 || cfg->b[2] != 0.0;
+  (void)hipGetDevice(&h->device);
+  auto bail = [&](const std::string& m) {
+    g_create_error = m;
+    pdelab_gpu_destroy(h);
+    return PDELAB_ERR_HIP;
+  };
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return bail("hipStreamCreate failed");
+  fill_args<double>(h, h->a64);
+  fill_args<float>(h, h->a32);
+  fill_load_args(h);
+  if (hipMalloc(&h->d_part64, sizeof(double) * kRedBlocks * 4) != hipSuccess) return bail("hipMalloc failed");
+  if (hipMalloc(&h->d_part32, sizeof(float) * kRedBlocks * 4) != hipSuccess) return bail("hipMalloc failed");
+  if (hipMalloc(&h->d_scal64, sizeof(double) * 16) != hipSuccess) return bail("hipMalloc failed");
+  if (hipMalloc(&h->d_scal32, sizeof(float) * 16) != hipSuccess) return bail("hipMalloc failed");
+  if (hipHostMalloc(&h->h_scal64, sizeof(double) * 16) != hipSuccess) return bail("hipHostMalloc failed");
+  if (hipHostMalloc(&h->h_scal32, sizeof(float) * 16) != hipSuccess) return bail("hipHostMalloc failed");
+  if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) return bail("hipEventCreate failed");
+  if (cfg->kappa_per_cell) {
+    if (!kappa_host) { g_create_error = "kappa_per_cell requires a kappa array"; pdelab_gpu_destroy(h); return PDELAB_ERR_INVALID_ARG; }
+    if (upload_kappa(h, kappa_host) != PDELAB_OK) {
+      g_create_error = h->err;
+      pdelab_gpu_destroy(h);
+      return PDELAB_ERR_INVALID_ARG;
+    }
+  }
+  *out = h;
+  return PDELAB_OK;
+}
+
+int pdelab_gpu_destroy(pdelab_gpu_gridoperator* h) {
+  if (!h) return PDELAB_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  void* ptrs[] = {h->d_kappa, h->d_kappa32, h->d_R0, h->d_src, h->d_ftab, h->d_gtab, h->d_diag, h->d_dinv,
+                  h->d_dinv32, h->d_part64, h->d_part32, h->d_scal64, h->d_scal32};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (int i = 0; i < 8; ++i) {
+    if (h->w64[i]) (void)hipFree(h->w64[i]);
+    if (h->w32[i]) (void)hipFree(h->w32[i]);
+  }
+  if (h->h_scal64) (void)hipHostFree(h->h_scal64);
+  if (h->h_scal32) (void)hipHostFree(h->h_scal32);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return PDELAB_OK;
+}
+
+const char* pdelab_gpu_last_error(const pdelab_gpu_gridoperator* h) {
+  return h ? h->err.c_str() : g_create_error.c_str();
+}
+size_t pdelab_gpu_size(const pdelab_gpu_gridoperator* h) { return h ? (size_t)h->N : 0; }
+size_t pdelab_gpu_num_cells(const pdelab_gpu_gridoperator* h) { return h ? (size_t)h->ncells : 0; }
+int pdelab_gpu_is_linear(const pdelab_gpu_gridoperator* h) { return h ? 1 : 0; }
+int pdelab_gpu_quadrature_points(const pdelab_gpu_gridoperator* h) { return h ? h->tab.m : 0; }
+
+int pdelab_gpu_set_kappa(pdelab_gpu_gridoperator* h, const double* kappa_host) {
+  if (!h || !kappa_host) return PDELAB_ERR_INVALID_ARG;
+  return upload_kappa(h, kappa_host);
+}
+
+int pdelab_gpu_set_cell_source(pdelab_gpu_gridoperator* h, const double* src) {
+  if (!h) return PDELAB_ERR_INVALID_ARG;
+  if (!src) {
+    if (h->d_src) { (void)hipFree(h->d_src); h->d_src = nullptr; }
+  } else {
+    if (!h->d_src) HIPCHK(h, hipMalloc(&h->d_src, sizeof(double) * h->ncells));
+    HIPCHK(h, hipMemcpyAsync(h->d_src, src, sizeof(double) * h->ncells, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  }
+  h->R0_valid = false;
+  return PDELAB_OK;
+}
+
+int pdelab_gpu_set_tabulated_data(pdelab_gpu_gridoperator* h, const double* fq, const double* gq) {
+  if (!h) return PDELAB_ERR_INVALID_ARG;
+  if (h->cfg.problem != PDELAB_PROBLEM_TABULATED)
+    return fail(h, PDELAB_ERR_INVALID_ARG, "tabulated data needs problem PDELAB_PROBLEM_TABULATED");
+  const long long m = h->tab.m;
+  const long long nf = h->ncells * m * m * m;
+  const long long ng = h->la.bface_offset[6] * m * m;
+  if (fq) {
+    if (!h->d_ftab) HIPCHK(h, hipMalloc(&h->d_ftab, sizeof(double) * nf));
+    HIPCHK(h, hipMemcpyAsync(h->d_ftab, fq, sizeof(double) * nf, hipMemcpyHostToDevice, h->stream));
+  }
+  if (gq) {
+    if (!h->d_gtab) HIPCHK(h, hipMalloc(&h->d_gtab, sizeof(double) * ng));
+    HIPCHK(h, hipMemcpyAsync(h->d_gtab, gq, sizeof(double) * ng, hipMemcpyHostToDevice, h->stream));
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->R0_valid = false;
+  return PDELAB_OK;
+}
+
+int pdelab_gpu_malloc(pdelab_gpu_gridoperator* h, void** dptr, size_t bytes) {
+  if (!h || !dptr) return PDELAB_ERR_INVALID_ARG;
+  HIPCHK(h, hipMalloc(dptr, bytes ? bytes : 1));
+  return PDELAB_OK;
+}
+int pdelab_gpu_free(pdelab_gpu_gridoperator* h, void* dptr) {
+  if (!h) return PDELAB_ERR_INVALID_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (dptr) HIPCHK(h, hipFree(dptr));
+  return PDELAB_OK;
+}
+int pdelab_gpu_memcpy_htod(pdelab_gpu_gridoperator* h, void* dst, const void* src, size_t bytes) {
+  if (!h || (bytes && (!dst || !src))) return PDELAB_ERR_INVALID_ARG;
+  HIPCHK(h, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return PDELAB_OK;
+}
+int pdelab_gpu_memcpy_dtoh(pdelab_gpu_gridoperator* h, void* dst, const void* src, size_t bytes) {
+  if (!h || (bytes && (!dst || !src))) return PDELAB_ERR_INVALID_ARG;
+  HIPCHK(h, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return PDELAB_OK;
+}
+int pdelab_gpu_memcpy_dtod(pdelab_gpu_gridoperator* h, void* dst, const void* src, size_t bytes) {
+  if (!h || (bytes && (!dst || !src))) return PDELAB_ERR_INVALID_ARG;
+  HIPCHK(h, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, h->stream));
+  return PDELAB_OK;
+}
+int pdelab_gpu_memset_zero(pdelab_gpu_gridoperator* h, void* dptr, size_t bytes) {
+  if (!h || (bytes && !dptr)) return PDELAB_ERR_INVALID_ARG;
+  HIPCHK(h, hipMemsetAsync(dptr, 0, bytes, h->stream));
+  return PDELAB_OK;
+}
+int pdelab_gpu_synchronize(pdelab_gpu_gridoperator* h) {
+  if (!h) return PDELAB_ERR_INVALID_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return PDELAB_OK;
+}
+
+int pdelab_gpu_residual(pdelab_gpu_gridoperator* h, const double* x, double* r) {
+  if (!h || !x || !r) return PDELAB_ERR_INVALID_ARG;
+  CHK(compute_R0(h));
+  CHK(apply64(h, x, r, false, true));
+  hipLaunchKernelGGL(axpy_kernel<double>, dim3(grid_for(h->N)), dim3(256), 0, h->stream, h->N, 1.0,
+                     (const double*)h->d_R0, r);
+  HIPCHK(h, hipGetLastError());
+  return PDELAB_OK;
+}
+
+int pdelab_gpu_jacobian_apply(pdelab_gpu_gridoperator* h, const double* z, double* y) {
+  if (!h || !z || !y) return PDELAB_ERR_INVALID_ARG;
+  return apply64(h, z, y, false, true);
+}
+
+int pdelab_gpu_jacobian_apply_f32(pdelab_gpu_gridoperator* h, const float* z, float* y) {
+  if (!h || !z || !y) return PDELAB_ERR_INVALID_ARG;
+  return apply32(h, z, y, false, true);
+}
+
+int pdelab_gpu_jacobian_apply_nonlinear(pdelab_gpu_gridoperator* h, const double*, const double*, double*) {
+  if (!h) return PDELAB_ERR_INVALID_ARG;
+  return fail(h, PDELAB_ERR_LINEARITY_MISMATCH, "Your trying to use a non linear jacobian apply for a linear problem.");
+}
+
+int pdelab_gpu_jacobian_apply_transpose(pdelab_gpu_gridoperator* h, const double* z, double* y) {
+  if (!h || !z || !y) return PDELAB_ERR_INVALID_ARG;
+  return apply64(h, z, y, true, true);
+}
+
+int pdelab_gpu_point_diagonal(pdelab_gpu_gridoperator* h, double* d) {
+  if (!h || !d) return PDELAB_ERR_INVALID_ARG;
+  CHK(check_outflow(h));
+  CHK(compute_diag(h));
+  hipLaunchKernelGGL(axpy_kernel<double>, dim3(grid_for(h->N)), dim3(256), 0, h->stream, h->N, 1.0,
+                     (const double*)h->d_diag, d);
+  HIPCHK(h, hipGetLastError());
+  return PDELAB_OK;
+}
+
+int pdelab_gpu_mass_apply(pdelab_gpu_gridoperator* h, const double* z, double* y) {
+  if (!h || !z || !y) return PDELAB_ERR_INVALID_ARG;
+  const int nb = (int)((h->N + 255) / 256);
+  switch (h->n1) {
+    case 2: hipLaunchKernelGGL((dg_mass_kernel<2, double>), dim3(nb), dim3(256), 0, h->stream, z, y, h->a64); break;
+    case 3: hipLaunchKernelGGL((dg_mass_kernel<3, double>), dim3(nb), dim3(256), 0, h->stream, z, y, h->a64); break;
+    case 4: hipLaunchKernelGGL((dg_mass_kernel<4, double>), dim3(nb), dim3(256), 0, h->stream, z, y, h->a64); break;
+  }
+  HIPCHK(h, hipGetLastError());
+  return PDELAB_OK;
+}
+
+int pdelab_gpu_dot(pdelab_gpu_gridoperator* h, const double* x, const double* y, double* out) {
+  if (!h || !x || !y || !out) return PDELAB_ERR_INVALID_ARG;
+  CHK((dot_dev<double, double>(h, x, y, h->N, 0)));
+  return read_scalars<double>(h, out, 1);
+}
+int pdelab_gpu_norm(pdelab_gpu_gridoperator* h, const double* x, double* out) {
+  CHK(pdelab_gpu_dot(h, x, x, out));
+  *out = std::sqrt(*out);
+  return PDELAB_OK;
+}
+int pdelab_gpu_axpy(pdelab_gpu_gridoperator* h, double a, const double* x, double* y) {
+  if (!h || !x || !y) return PDELAB_ERR_INVALID_ARG;
+  hipLaunchKernelGGL(axpy_kernel<double>, dim3(grid_for(h->N)), dim3(256), 0, h->stream, h->N, a, x, y);
+  HIPCHK(h, hipGetLastError());
+  return PDELAB_OK;
+}
+
+int pdelab_gpu_solve(pdelab_gpu_gridoperator* h, int method, int precond, double reduction, int maxit, double* z,
+                     double* r, pdelab_linear_solver_result* res) {
+  if (!h || !z || !r || !res) return PDELAB_ERR_INVALID_ARG;
+  std::memset(res, 0, sizeof(*res));
+  CHK(check_outflow(h));
+  return krylov<double, double>(h, method, precond, reduction, maxit, false, z, r, 0, res);
+}
+
+int pdelab_gpu_solve_transpose(pdelab_gpu_gridoperator* h, int method, int precond, double reduction, int maxit,
+                               double* z, double* r, pdelab_linear_solver_result* res) {
+  if (!h || !z || !r || !res) return PDELAB_ERR_INVALID_ARG;
+  std::memset(res, 0, sizeof(*res));
+  CHK(check_outflow(h));
+  return krylov<double, double>(h, method, precond, reduction, maxit, true, z, r, 0, res);
+}
+
+int pdelab_gpu_linear_problem_solve(pdelab_gpu_gridoperator* h, int method, int precond, double reduction,
+                                    double min_defect, int maxit, double* x, pdelab_linear_problem_result* res) {
+  if (!h || !x || !res) return PDELAB_ERR_INVALID_ARG;
+  std::memset(res, 0, sizeof(*res));
+  const long long n = h->N;
+  CHK(ensure_ws<double>(h, 7));
+  double* r = h->w64[5];
+  double* z = h->w64[6];
+  auto t0 = std::chrono::steady_clock::now();
+  HIPCHK(h, hipMemsetAsync(r, 0, sizeof(double) * n, h->stream));
+  CHK(pdelab_gpu_residual(h, x, r));  // residual is additive
+  double defect = 0;
+  CHK(pdelab_gpu_norm(h, r, &defect));
+  res->assembler_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  HIPCHK(h, hipMemsetAsync(z, 0, sizeof(double) * n, h->stream));
+  const double red = std::max(reduction, min_defect / defect);
+  auto t1 = std::chrono::steady_clock::now();
+  int st = krylov<double, double>(h, method, precond, red, maxit, false, z, r, 0, &res->solver);
+  res->linear_solver_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
+  res->first_defect = defect;
+  res->defect = defect * res->solver.reduction;
+  res->linear_solver_iterations = (int)res->solver.iterations;
+  if (st) return st;
+  hipLaunchKernelGGL(axpy_kernel<double>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, -1.0, (const double*)z, x);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return PDELAB_OK;
+}
+
+int pdelab_gpu_solve_mixed(pdelab_gpu_gridoperator* h, int method, int precond, double inner_reduction,
+                           int inner_maxit, double outer_reduction, int outer_maxit, double* x,
+                           pdelab_mixed_result* res) {
+  if (!h || !x || !res) return PDELAB_ERR_INVALID_ARG;
+  std::memset(res, 0, sizeof(*res));
+  if (outer_maxit <= 0) outer_maxit = 50;
+  const long long n = h->N;
+  CHK(ensure_ws<double>(h, 6));
+  CHK(ensure_ws<float>(h, 2));
+  double* r64 = h->w64[5];
+  float* r32 = h->w32[0];
+  float* z32 = h->w32[1];
+  auto t0 = std::chrono::steady_clock::now();
+  CHK(compute_R0(h));
+  double def0 = 0, def = 0;
+  for (int outer = 0;; ++outer) {
+    // FP64 outer residual r = R(x) = J x + R(0)
+    CHK(apply64(h, x, r64, false, false));
+    hipLaunchKernelGGL(axpy_kernel<double>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, 1.0,
+                       (const double*)h->d_R0, r64);
+    CHK(pdelab_gpu_norm(h, r64, &def));
+    if (outer == 0) def0 = def;
+    res->outer_iterations = outer;
+    if (!std::isfinite(def)) {
+      res->defect = def;
+      return fail(h, PDELAB_ERR_SOLVER_ABORT, "refinement: residual is infinite or NaN");
+    }
+    if (def <= outer_reduction * def0 || def < 1e-30) { res->converged = 1; break; }
+    if (outer >= outer_maxit) break;
+    hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(n)), dim3(256), 0, h->stream, n,
+                       (const double*)r64, r32);
+    HIPCHK(h, hipMemsetAsync(z32, 0, sizeof(float) * n, h->stream));
+    pdelab_linear_solver_result ires;
+    CHK((krylov<float, float>(h, method, precond, inner_reduction, inner_maxit, false, z32, r32, 2, &ires)));
+    res->inner_iterations += (int)ires.iterations;
+    hipLaunchKernelGGL(sub_f32_kernel, dim3(grid_for(n)), dim3(256), 0, h->stream, n, (const float*)z32, x);
+    HIPCHK(h, hipGetLastError());
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  res->first_defect = def0;
+  res->defect = def;
+  res->reduction = def0 > 0 ? def / def0 : 0.0;
+  res->elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return PDELAB_OK;
+}
+
+int pdelab_gpu_grad_logkappa(pdelab_gpu_gridoperator* h, const double* u, const double* lambda, double* g) {
+  if (!h || !u || !lambda || !g) return PDELAB_ERR_INVALID_ARG;
+  if (h->cfg.weights != PDELAB_WEIGHTS_ON)
+    return fail(h, PDELAB_ERR_UNSUPPORTED, "kappa gradient implemented for weightsOn");
+  const int p = h->cfg.problem;
+  const bool g_zero = p == PDELAB_PROBLEM_ZERO || ((p == PDELAB_PROBLEM_CONSTANT || p == PDELAB_PROBLEM_GAUSSIANS) && h->cfg.g0 == 0.0);
+  if (!g_zero) return fail(h, PDELAB_ERR_UNSUPPORTED, "kappa gradient implemented for Dirichlet data g = 0");
+  if (!h->d_kappa) {
+    std::vector<double> k(h->ncells, h->cfg.kappa0);
+    CHK(upload_kappa(h, k.data()));
+  }
+  const int nb = (int)((h->ncells + 127) / 128);
+  switch (h->n1) {
+    case 2: hipLaunchKernelGGL(grad_logkappa_kernel<2>, dim3(nb), dim3(128), 0, h->stream, u, lambda, h->d_kappa, g, h->ga); break;
+    case 3: hipLaunchKernelGGL(grad_logkappa_kernel<3>, dim3(nb), dim3(128), 0, h->stream, u, lambda, h->d_kappa, g, h->ga); break;
+    case 4: hipLaunchKernelGGL(grad_logkappa_kernel<4>, dim3(nb), dim3(128), 0, h->stream, u, lambda, h->d_kappa, g, h->ga); break;
+  }
+  HIPCHK(h, hipGetLastError());
+  return PDELAB_OK;
+}
+
+int pdelab_gpu_grad_source(pdelab_gpu_gridoperator* h, const double* lambda, double* g) {
+  if (!h || !lambda || !g) return PDELAB_ERR_INVALID_ARG;
+  const Tables1D& t = h->tab;
+  double w[4] = {0, 0, 0, 0};
+  for (int i = 0; i < t.n1; ++i) w[i] = (double)t.wint[i];
+  const int nb = (int)((h->ncells + 255) / 256);
+  const double V = h->la.V;
+  switch (h->n1) {
+    case 2: hipLaunchKernelGGL(grad_source_kernel<2>, dim3(nb), dim3(256), 0, h->stream, lambda, g, h->ncells, V, nullptr, w[0], w[1], w[2], w[3]); break;
+    case 3: hipLaunchKernelGGL(grad_source_kernel<3>, dim3(nb), dim3(256), 0, h->stream, lambda, g, h->ncells, V, nullptr, w[0], w[1], w[2], w[3]); break;
+    case 4: hipLaunchKernelGGL(grad_source_kernel<4>, dim3(nb), dim3(256), 0, h->stream, lambda, g, h->ncells, V, nullptr, w[0], w[1], w[2], w[3]); break;
+  }
+  HIPCHK(h, hipGetLastError());
+  return PDELAB_OK;
+}
+
+int pdelab_gpu_time_apply(pdelab_gpu_gridoperator* h, int precision, const void* z, void* y, int warmup, int reps,
+                          double* avg_ms) {
+  if (!h || !z || !y || !avg_ms || reps < 1 || (precision != 64 && precision != 32)) return PDELAB_ERR_INVALID_ARG;
+  auto one = [&]() -> int {
+    if (precision == 64) return apply64(h, (const double*)z, (double*)y, false, false);
+    return apply32(h, (const float*)z, (float*)y, false, false);
+  };
+  for (int i = 0; i < warmup; ++i) CHK(one());
+  HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+  for (int i = 0; i < reps; ++i) CHK(one());
+  HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+  HIPCHK(h, hipEventSynchronize(h->ev1));
+  float ms = 0;
+  HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  *avg_ms = (double)ms / reps;
+  return PDELAB_OK;
+}
+
+}  // extern "C"

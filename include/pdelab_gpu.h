@@ -1,0 +1,257 @@
+/*
+ * pdelab_gpu.h -- C-ABI of the gfx950 matrix-free SIPG DG hot path.
+ *
+ * The entry points replace, one for one, the numeric calls that PDELab makes on
+ * the path  StationaryLinearProblemSolver -> ISTL solver backend -> OnTheFlyOperator
+ * -> GridOperator::jacobian_apply -> ConvectionDiffusionDG::alpha_* :
+ *
+ *   pdelab_gpu_residual              GridOperator::residual(x, r)
+ *                                    R:dune/pdelab/gridoperator/gridoperator.hh:176-181
+ *   pdelab_gpu_jacobian_apply        GridOperator::jacobian_apply(z, y)
+ *                                    R:dune/pdelab/gridoperator/gridoperator.hh:192-197
+ *   pdelab_gpu_jacobian_apply_nonlinear
+ *                                    GridOperator::jacobian_apply(x, z, y) (throws for a
+ *                                    linear operator) gridoperator.hh:200-205
+ *   pdelab_gpu_point_diagonal        residual of PointDiagonalLocalOperatorWrapper
+ *                                    R:dune/pdelab/localoperator/pointdiagonalwrapper.hh:51-131
+ *   pdelab_gpu_solve                 ISTLBackend_SEQ_{CG,BCGS}_Jac::apply /
+ *                                    ISTLBackend_SEQ_MatrixFree_Base::apply(z, r, reduction)
+ *                                    R:dune/pdelab/backend/istl/seqistlsolverbackend.hh:231-252
+ *                                    R:dune/pdelab/backend/istl/matrixfree/backends.hh:116-126
+ *   pdelab_gpu_linear_problem_solve  StationaryLinearProblemSolver::apply
+ *                                    R:dune/pdelab/stationary/linearproblem.hh:188-302
+ *   pdelab_gpu_dot / _norm / _axpy   ISTL::BlockVector dot / two_norm / axpy
+ *                                    R:dune/pdelab/backend/istl/vector.hh:249-282
+ *   pdelab_gpu_mass_apply            ScalarL2::jacobian_apply_volume
+ *                                    R:dune/pdelab/localoperator/l2.hh:45-90
+ *
+ * Extensions with no single reference counterpart (they compose the above):
+ *   pdelab_gpu_jacobian_apply_f32, pdelab_gpu_solve_mixed (FP32 Krylov inside FP64
+ *   iterative refinement), pdelab_gpu_jacobian_apply_transpose (adjoint operator),
+ *   pdelab_gpu_grad_logkappa / pdelab_gpu_grad_source (adjoint parameter gradients).
+ *
+ * Conventions
+ *   - Vectors are device pointers in PDELab's cell-blocked DG layout
+ *     index = cell * (k+1)^3 + ix + (k+1)*iy + (k+1)^2*iz, cell = i + nx*(j + ny*l)
+ *     (R:dune/pdelab/ordering/leaforderingbase.hh:97-141 with
+ *      R:dune/pdelab/finiteelement/qkdglagrange.hh:118-128).
+ *   - residual / jacobian_apply ACCUMULATE into their output (r += R(x), y += J z), as
+ *     the reference engines do.
+ *   - Every call is ordered on the handle's own HIP stream; calls that return host
+ *     scalars (dot, norm, solve) synchronise that stream before returning.
+ *   - No C++ exception crosses this boundary: each call returns a pdelab_status;
+ *     pdelab_gpu_last_error() returns the message of the last failure on the handle.
+ *   - Non-convergence of a Krylov solve is NOT an error: it is reported in
+ *     pdelab_linear_solver_result.converged (R:dune/pdelab/backend/solver.hh:29-51).
+ *   - Calls on one handle must be serialised by the caller; separate handles may be
+ *     used from separate host threads.
+ */
+#ifndef PDELAB_GPU_H
+#define PDELAB_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PDELAB_GPU_ABI_VERSION 1
+#define PDELAB_MAX_GAUSSIANS 16
+
+/* Status codes. */
+typedef enum pdelab_status {
+  PDELAB_OK = 0,
+  PDELAB_ERR_INVALID_ARG = 1,        /* bad config / null pointer / size */
+  PDELAB_ERR_LINEARITY_MISMATCH = 2, /* gridoperator.hh:194-195, 202-203 */
+  PDELAB_ERR_HIP = 3,                /* HIP runtime error */
+  PDELAB_ERR_UNSUPPORTED = 4,        /* configuration outside the implemented path */
+  PDELAB_ERR_SOLVER_ABORT = 5,       /* NaN/inf defect or BiCGSTAB breakdown (dune-istl SolverAbort) */
+  PDELAB_ERR_OUTFLOW_ON_INFLOW = 6,  /* convectiondiffusiondg.hh:802-806 */
+  PDELAB_ERR_NO_DEVICE = 7
+} pdelab_status;
+
+/* ConvectionDiffusionDGMethod::Type, R:dune/pdelab/localoperator/convectiondiffusiondg.hh:29-32 */
+enum { PDELAB_NIPG = 0, PDELAB_SIPG = 1, PDELAB_IIPG = 2 };
+/* ConvectionDiffusionDGWeights::Type, convectiondiffusiondg.hh:34-37 */
+enum { PDELAB_WEIGHTS_ON = 0, PDELAB_WEIGHTS_OFF = 1 };
+/* ConvectionDiffusionBoundaryConditions::Type,
+   R:dune/pdelab/localoperator/convectiondiffusionparameter.hh:111-115 */
+enum { PDELAB_BC_DIRICHLET = 1, PDELAB_BC_NEUMANN = -1, PDELAB_BC_OUTFLOW = -2, PDELAB_BC_NONE = -3 };
+
+/* Source term f and Dirichlet data g of the parameter class
+   (ConvectionDiffusionModelProblem::f / ::g, convectiondiffusionparameter.hh:174-194).
+   The built-in problems are evaluated on the device at the quadrature points. */
+enum {
+  PDELAB_PROBLEM_ZERO = 0,      /* f = 0, g = 0 */
+  PDELAB_PROBLEM_EXP = 1,       /* u = exp(-|x-(1/2,1/2,1/2)|^2); f = -kappa0*lap(u) + b.grad(u) + c u; g = u */
+  PDELAB_PROBLEM_CONSTANT = 2,  /* f = f0, g = g0 */
+  PDELAB_PROBLEM_GAUSSIANS = 3, /* f = sum_i a_i exp(-|x-x_i|^2/(2 s_i^2)); g = g0 */
+  PDELAB_PROBLEM_QUADRATIC = 4, /* u = q0 + q.x + sum_d p_d x_d^2; f = -kappa0*lap(u) + b.grad(u) + c u; g = u */
+  PDELAB_PROBLEM_TABULATED = 5  /* f, g given at the quadrature points, see pdelab_gpu_set_tabulated_data */
+};
+
+/* Krylov method and preconditioner of the ISTL backends. */
+enum { PDELAB_SOLVER_CG = 0, PDELAB_SOLVER_BICGSTAB = 1 };
+enum { PDELAB_PREC_NONE = 0, PDELAB_PREC_JACOBI = 1 };
+
+typedef struct pdelab_config {
+  int abi_version;            /* must be PDELAB_GPU_ABI_VERSION */
+  /* YaspGrid box [0,lx]x[0,ly]x[0,lz] with nx*ny*nz hexahedra */
+  int nx, ny, nz;
+  double lx, ly, lz;
+  /* QkDGLocalFiniteElementMap<.., k, 3> */
+  int degree;                 /* k, 1..3 */
+  /* ConvectionDiffusionDG(param, method, weights, alpha, intorderadd) */
+  int method;
+  int weights;
+  double alpha;
+  int intorderadd;            /* >= 0; affects only the load vector (f, g) quadrature */
+  /* parameter class: A = kappa * I (constant per cell), b constant, c constant */
+  int kappa_per_cell;         /* 0: kappa = kappa0 everywhere; 1: per-cell array given at create */
+  double kappa0;
+  double b[3];
+  double c;
+  int bctype[6];              /* per boundary side -x,+x,-y,+y,-z,+z; 0 means Dirichlet */
+  /* source / Dirichlet data */
+  int problem;
+  double f0, g0;              /* CONSTANT / GAUSSIANS g0 */
+  double quad[7];             /* QUADRATIC: q0, q_x, q_y, q_z, p_x, p_y, p_z */
+  int n_gaussians;
+  double gaussians[PDELAB_MAX_GAUSSIANS][5]; /* x, y, z, sigma, amplitude */
+} pdelab_config;
+
+/* LinearSolverResult<double>, R:dune/pdelab/backend/solver.hh:29-51 */
+typedef struct pdelab_linear_solver_result {
+  int converged;
+  unsigned int iterations;
+  double elapsed;             /* seconds */
+  double reduction;           /* final defect / first defect */
+  double conv_rate;           /* reduction^(1/iterations) */
+} pdelab_linear_solver_result;
+
+/* StationaryLinearProblemSolverResult<double>, R:dune/pdelab/stationary/linearproblem.hh:27-44 */
+typedef struct pdelab_linear_problem_result {
+  pdelab_linear_solver_result solver;
+  double first_defect;
+  double defect;
+  double assembler_time;
+  double linear_solver_time;
+  int linear_solver_iterations;
+} pdelab_linear_problem_result;
+
+/* Result of the mixed-precision solve (FP32 inner Krylov, FP64 refinement). */
+typedef struct pdelab_mixed_result {
+  int converged;
+  int outer_iterations;
+  int inner_iterations;       /* summed over all refinement steps */
+  double first_defect;        /* ||R(x0)||_2 in FP64 */
+  double defect;              /* final ||R(x)||_2 in FP64 */
+  double reduction;           /* defect / first_defect */
+  double elapsed;             /* seconds */
+} pdelab_mixed_result;
+
+typedef struct pdelab_gpu_gridoperator pdelab_gpu_gridoperator;
+
+/* ---- lifecycle ------------------------------------------------------------ */
+
+/* Fills *cfg with the reference defaults: SIPG, weightsOn, alpha 1, intorderadd 0
+   (convectiondiffusiondg.hh:85-90), model problem A=I, b=0, c=0, Dirichlet everywhere
+   (convectiondiffusionparameter.hh:134-209), unit cube, 4^3 cells, k=1. */
+void pdelab_gpu_default_config(pdelab_config* cfg);
+
+/* Creates a grid operator on the current HIP device. kappa_host (nx*ny*nz doubles,
+   host memory) is read when cfg->kappa_per_cell != 0, else ignored. */
+int pdelab_gpu_create(const pdelab_config* cfg, const double* kappa_host,
+                      pdelab_gpu_gridoperator** out);
+int pdelab_gpu_destroy(pdelab_gpu_gridoperator* h);
+/* Message of the last failed call on h (h may be NULL: last failed create). */
+const char* pdelab_gpu_last_error(const pdelab_gpu_gridoperator* h);
+/* Number of DoFs N = nx*ny*nz*(k+1)^3 and of cells. */
+size_t pdelab_gpu_size(const pdelab_gpu_gridoperator* h);
+size_t pdelab_gpu_num_cells(const pdelab_gpu_gridoperator* h);
+/* isLinear of the wrapped local operator (always 1 for ConvectionDiffusionDG,
+   convectiondiffusiondg.hh inherits LocalOperatorDefaultFlags::isLinear = true). */
+int pdelab_gpu_is_linear(const pdelab_gpu_gridoperator* h);
+/* Replaces the per-cell kappa (host array of nx*ny*nz doubles). */
+int pdelab_gpu_set_kappa(pdelab_gpu_gridoperator* h, const double* kappa_host);
+/* Per-cell additive source s_c (f -> f + s_c on cell c); host array or NULL to clear. */
+int pdelab_gpu_set_cell_source(pdelab_gpu_gridoperator* h, const double* source_host);
+/* PDELAB_PROBLEM_TABULATED data, host arrays:
+   f_q: per cell, volume quadrature points, x fastest: nx*ny*nz*m^3,
+   g_q: per boundary face (side-major: -x,+x,-y,+y,-z,+z; within a side faces
+        lexicographic in the two tangential cell indices), m^2 points each,
+   with m = (2k + intorderadd)/2 + 1 Gauss points per direction. */
+int pdelab_gpu_set_tabulated_data(pdelab_gpu_gridoperator* h, const double* f_q, const double* g_q);
+int pdelab_gpu_quadrature_points(const pdelab_gpu_gridoperator* h); /* m */
+
+/* ---- device memory helpers (ordered on the handle's stream) --------------- */
+int pdelab_gpu_malloc(pdelab_gpu_gridoperator* h, void** dptr, size_t bytes);
+int pdelab_gpu_free(pdelab_gpu_gridoperator* h, void* dptr);
+int pdelab_gpu_memcpy_htod(pdelab_gpu_gridoperator* h, void* dst, const void* src, size_t bytes);
+int pdelab_gpu_memcpy_dtoh(pdelab_gpu_gridoperator* h, void* dst, const void* src, size_t bytes);
+int pdelab_gpu_memcpy_dtod(pdelab_gpu_gridoperator* h, void* dst, const void* src, size_t bytes);
+int pdelab_gpu_memset_zero(pdelab_gpu_gridoperator* h, void* dptr, size_t bytes);
+int pdelab_gpu_synchronize(pdelab_gpu_gridoperator* h);
+
+/* ---- GridOperator --------------------------------------------------------- */
+/* r += R(x) = alpha_volume + alpha_skeleton + alpha_boundary + lambda_volume terms */
+int pdelab_gpu_residual(pdelab_gpu_gridoperator* h, const double* x, double* r);
+/* y += J z (linear operator: the Jacobian apply is the residual without the load terms) */
+int pdelab_gpu_jacobian_apply(pdelab_gpu_gridoperator* h, const double* z, double* y);
+/* FP32 operator apply, y += J z in single precision (arithmetic in FP32). */
+int pdelab_gpu_jacobian_apply_f32(pdelab_gpu_gridoperator* h, const float* z, float* y);
+/* Nonlinear form jacobian_apply(x, z, y): always PDELAB_ERR_LINEARITY_MISMATCH here,
+   mirroring gridoperator.hh:202-203. */
+int pdelab_gpu_jacobian_apply_nonlinear(pdelab_gpu_gridoperator* h, const double* x,
+                                        const double* z, double* y);
+/* y += J^T z (adjoint operator; convection becomes downwind). */
+int pdelab_gpu_jacobian_apply_transpose(pdelab_gpu_gridoperator* h, const double* z, double* y);
+/* d += diag(J) */
+int pdelab_gpu_point_diagonal(pdelab_gpu_gridoperator* h, double* d);
+/* y += M z, DG mass matrix (ScalarL2 with scaling 1). */
+int pdelab_gpu_mass_apply(pdelab_gpu_gridoperator* h, const double* z, double* y);
+
+/* ---- vector backend (FP64) ------------------------------------------------ */
+int pdelab_gpu_dot(pdelab_gpu_gridoperator* h, const double* x, const double* y, double* out);
+int pdelab_gpu_norm(pdelab_gpu_gridoperator* h, const double* x, double* out);
+int pdelab_gpu_axpy(pdelab_gpu_gridoperator* h, double a, const double* x, double* y);
+
+/* ---- solver backends ------------------------------------------------------ */
+/* ISTL backend apply(z, r, reduction): solves J z = r starting from the given z with
+   Krylov method `method` and preconditioner `precond`; r is overwritten by the final
+   defect, as dune-istl does with its right-hand side. maxit <= 0 means 5000. */
+int pdelab_gpu_solve(pdelab_gpu_gridoperator* h, int method, int precond, double reduction,
+                     int maxit, double* z, double* r, pdelab_linear_solver_result* res);
+/* Same on J^T (adjoint solve). */
+int pdelab_gpu_solve_transpose(pdelab_gpu_gridoperator* h, int method, int precond,
+                               double reduction, int maxit, double* z, double* r,
+                               pdelab_linear_solver_result* res);
+/* StationaryLinearProblemSolver::apply on x (device): r = R(x); solve J z = r to
+   max(reduction, min_defect/||r||); x -= z. */
+int pdelab_gpu_linear_problem_solve(pdelab_gpu_gridoperator* h, int method, int precond,
+                                    double reduction, double min_defect, int maxit, double* x,
+                                    pdelab_linear_problem_result* res);
+/* Mixed precision: FP64 refinement x <- x - J32^{-1} R(x) with the inner solve in FP32
+   to inner_reduction, until ||R(x)|| <= outer_reduction * ||R(x0)||. */
+int pdelab_gpu_solve_mixed(pdelab_gpu_gridoperator* h, int method, int precond,
+                           double inner_reduction, int inner_maxit, double outer_reduction,
+                           int outer_maxit, double* x, pdelab_mixed_result* res);
+
+/* ---- adjoint gradients ---------------------------------------------------- */
+/* g[c] = -lambda^T (dJ/dlog kappa_c) u for every cell (device arrays, g has nx*ny*nz). */
+int pdelab_gpu_grad_logkappa(pdelab_gpu_gridoperator* h, const double* u, const double* lambda,
+                             double* g);
+/* g[c] = lambda^T (d b / d s_c): the load-vector derivative for a per-cell source s_c. */
+int pdelab_gpu_grad_source(pdelab_gpu_gridoperator* h, const double* lambda, double* g);
+
+/* ---- measurement ---------------------------------------------------------- */
+/* Runs `reps` back-to-back jacobian applies (precision 64 or 32) after `warmup`
+   untimed ones and returns the average duration per launch in milliseconds,
+   measured with HIP events on the handle's stream. */
+int pdelab_gpu_time_apply(pdelab_gpu_gridoperator* h, int precision, const void* z, void* y,
+                          int warmup, int reps, double* avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PDELAB_GPU_H */

@@ -1,0 +1,293 @@
+"""GPU parity: the HIP path (through the C-ABI) against the oracle on identical inputs.
+
+Tolerances (FP64): operator apply within 1e-12 relative (2-norm), as the target
+states; solves at the solver tolerance. FP32 apply: within 1e-5 relative.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+APPLY_RTOL = 1e-12
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+CASES = [
+    ("sipg_q1_4x3x2", dict(nx=4, ny=3, nz=2, degree=1, alpha=3.0), None),
+    ("sipg_q2_5x4x3", dict(nx=5, ny=4, nz=3, degree=2, alpha=3.0), None),
+    ("sipg_q3_3x3x4", dict(nx=3, ny=3, nz=4, degree=3, alpha=3.0), None),
+    ("sipg_q2_var", dict(nx=4, ny=4, nz=4, degree=2, alpha=3.0), 1),
+    ("sipg_q3_var", dict(nx=3, ny=4, nz=5, degree=3, alpha=3.0), 2),
+    ("nipg_q2_var", dict(nx=3, ny=2, nz=4, degree=2, method=0, alpha=1.0), 3),
+    ("iipg_q1_var", dict(nx=5, ny=2, nz=3, degree=1, method=2, alpha=5.0), 4),
+    ("weights_off_q3", dict(nx=2, ny=3, nz=3, degree=3, weights=1, alpha=3.0), 5),
+    ("aniso_q2", dict(nx=4, ny=3, nz=2, degree=2, lx=1.0, ly=0.5, lz=2.0, alpha=3.0), 6),
+    ("conv_react_q2", dict(nx=4, ny=4, nz=3, degree=2, alpha=3.0, b=(1.0, 0.5, -0.25), c=0.7, kappa0=0.01), None),
+    ("conv_q3_var", dict(nx=3, ny=3, nz=3, degree=3, alpha=3.0, b=(-0.3, 0.0, 2.0)), 7),
+    ("outflow_q2", dict(nx=3, ny=2, nz=3, degree=2, alpha=3.0, b=(1.0, 0.0, 0.0), bctype=(0, -2, 0, -1, -3, 0)), None),
+    ("single_cell_q3", dict(nx=1, ny=1, nz=1, degree=3, alpha=3.0), None),
+    ("line_x_q2", dict(nx=7, ny=1, nz=1, degree=2, alpha=3.0), None),
+]
+
+
+def make(pkg, of, kw, seed, problem=None, **extra):
+    kw = dict(kw)
+    kw.update(extra)
+    if problem is not None:
+        kw["problem"] = problem
+    cfg = pkg.make_config(**kw)
+    kappa = None
+    if seed is not None:
+        rng = np.random.default_rng(seed)
+        kappa = np.exp(rng.normal(0.0, 1.0, pkg.ncells(cfg)))
+        cfg.kappa_per_cell = 1
+    return cfg, kappa, pkg.GridOperator(cfg, kappa), of.Oracle(cfg, kappa)
+
+
+@pytest.mark.parametrize("name,kw,seed", CASES, ids=[c[0] for c in CASES])
+def test_apply_diag_transpose_mass(pkg, of, name, kw, seed):
+    say(f"[gpu] apply parity case={name}")
+    cfg, kappa, go, o = make(pkg, of, kw, seed)
+    rng = np.random.default_rng(42)
+    z = rng.uniform(-1, 1, go.N)
+    y0 = rng.uniform(-1, 1, go.N)
+    zd, yd = go.vector(z), go.vector(y0)
+    go.jacobian_apply(zd, yd)
+    y_ref = o.jacobian_apply(z, y0.copy())
+    assert rel(yd.download(), y_ref) <= APPLY_RTOL
+    # transpose
+    yd.upload(y0)
+    go.jacobian_apply_transpose(zd, yd)
+    assert rel(yd.download(), y0 + o.jacobian_apply_transpose(z)) <= APPLY_RTOL
+    # point diagonal (accumulates)
+    dd = go.vector(np.ones(go.N))
+    go.point_diagonal(dd)
+    assert rel(dd.download(), 1.0 + o.point_diagonal()) <= APPLY_RTOL
+    # mass
+    yd.zero()
+    go.mass_apply(zd, yd)
+    assert rel(yd.download(), o.mass_apply(z)) <= APPLY_RTOL
+    # FP32 apply
+    z32 = go.vector(z.astype(np.float32), dtype=np.float32)
+    y32 = go.vector(np.zeros(go.N, np.float32), dtype=np.float32)
+    go.jacobian_apply(z32, y32)
+    assert rel(y32.download().astype(np.float64), o.jacobian_apply(z)) <= 2e-5
+
+
+PROBLEMS = [
+    ("exp_q2", dict(n=4, degree=2, alpha=3.0), None, 1),
+    ("quadratic_q3_var", dict(nx=3, ny=2, nz=4, degree=3, alpha=3.0, quad=(0.3, 1.0, 2.0, -1.0, 1.0, 1.0, -2.0)), 9, 4),
+    ("gauss_conv_q2", dict(n=4, degree=2, alpha=3.0, b=(1.0, 0.5, 0.25), kappa0=0.01,
+                           gaussians=((0.3, 0.4, 0.5, 0.1, 1.0), (0.7, 0.2, 0.6, 0.15, -0.5)), g0=0.2), None, 3),
+    ("constant_intorder_q1", dict(n=5, degree=1, alpha=2.0, f0=1.5, g0=-0.5, intorderadd=3), 8, 2),
+]
+
+
+@pytest.mark.parametrize("name,kw,seed,problem", PROBLEMS, ids=[p[0] for p in PROBLEMS])
+def test_residual(pkg, of, name, kw, seed, problem):
+    say(f"[gpu] residual parity case={name}")
+    cfg, kappa, go, o = make(pkg, of, kw, seed, problem=problem)
+    rng = np.random.default_rng(1)
+    x = rng.uniform(-1, 1, go.N)
+    xd, rd = go.vector(x), go.vector()
+    go.residual(xd, rd)
+    r_ref = o.residual(x)
+    assert rel(rd.download(), r_ref) <= APPLY_RTOL
+    # residual at zero: the load vector alone
+    rd.zero()
+    go.residual(go.vector(), rd)
+    assert rel(rd.download(), o.residual(np.zeros(go.N))) <= APPLY_RTOL
+
+
+def test_tabulated_problem(pkg, of):
+    say("[gpu] tabulated f/g")
+    cfg = pkg.make_config(nx=3, ny=4, nz=2, degree=2, alpha=3.0, problem=pkg.PROBLEM_TABULATED)
+    go, o = pkg.GridOperator(cfg), of.Oracle(cfg)
+    m = go.quadrature_points()
+    assert m == o.quadrature_points()
+    rng = np.random.default_rng(2)
+    nb = 2 * (4 * 2 + 3 * 2 + 3 * 4)
+    fq = rng.normal(size=go.ncells * m ** 3)
+    gq = rng.normal(size=nb * m ** 2)
+    go.set_tabulated(fq, gq)
+    o.set_tabulated(fq, gq)
+    rd = go.vector()
+    x = rng.normal(size=go.N)
+    go.residual(go.vector(x), rd)
+    assert rel(rd.download(), o.residual(x)) <= APPLY_RTOL
+
+
+def test_cell_source(pkg, of):
+    say("[gpu] cell source")
+    cfg = pkg.make_config(n=3, degree=2, alpha=3.0, problem=pkg.PROBLEM_CONSTANT, f0=1.0)
+    go, o = pkg.GridOperator(cfg), of.Oracle(cfg)
+    s = np.random.default_rng(3).normal(size=27)
+    go.set_cell_source(s)
+    o.set_cell_source(s)
+    rd = go.vector()
+    go.residual(go.vector(), rd)
+    assert rel(rd.download(), o.residual(np.zeros(go.N))) <= APPLY_RTOL
+
+
+def test_vector_backend_deterministic(pkg, of):
+    say("[gpu] dot/norm/axpy")
+    cfg = pkg.make_config(n=20, degree=2)
+    go = pkg.GridOperator(cfg)
+    rng = np.random.default_rng(4)
+    x, y = rng.normal(size=go.N), rng.normal(size=go.N)
+    xd, yd = go.vector(x), go.vector(y)
+    d1 = go.dot(xd, yd)
+    d2 = go.dot(xd, yd)
+    assert d1 == d2
+    assert abs(d1 - x @ y) <= 1e-12 * np.linalg.norm(x) * np.linalg.norm(y)
+    assert abs(go.norm(xd) - np.linalg.norm(x)) <= 1e-13 * np.linalg.norm(x)
+    go.axpy(0.5, xd, yd)
+    assert np.abs(yd.download() - (y + 0.5 * x)).max() <= 1e-15 * np.abs(y + 0.5 * x).max()
+
+
+def test_apply_deterministic(pkg, of):
+    say("[gpu] apply run-to-run bitwise")
+    cfg = pkg.make_config(n=12, degree=3, alpha=3.0, kappa_per_cell=True)
+    kappa = np.exp(np.random.default_rng(5).normal(size=12 ** 3))
+    go = pkg.GridOperator(cfg, kappa)
+    zd = go.vector(np.random.default_rng(6).uniform(-1, 1, go.N))
+    a, b = go.vector(), go.vector()
+    go.jacobian_apply(zd, a)
+    go.jacobian_apply(zd, b)
+    assert np.array_equal(a.download(), b.download())
+
+
+@pytest.mark.parametrize("method", [0, 1], ids=["cg", "bicgstab"])
+def test_linear_problem_solve_c1(pkg, of, method):
+    """16^3 Q2 SIPG Poisson with the manufactured exp solution, alpha 3, CG/BiCGSTAB +
+    point Jacobi to 1e-10: GPU solution vs oracle solution at the solver tolerance."""
+    say(f"[gpu] linear problem 16^3 Q2 method={method}")
+    cfg = pkg.make_config(n=16, degree=2, alpha=3.0, problem=pkg.PROBLEM_EXP)
+    go, o = pkg.GridOperator(cfg), of.Oracle(cfg)
+    o.set_threads(8)
+    xd = go.vector()
+    res = go.linear_problem_solve(xd, method=method, reduction=1e-10)
+    assert res["solver"]["converged"]
+    x_ref, res_ref = o.linear_problem_solve(method=method, reduction=1e-10)
+    x = xd.download()
+    # both solve the same system to 1e-10 relative residual
+    assert rel(x, x_ref) <= 1e-7
+    assert abs(res["first_defect"] - res_ref["first_defect"]) <= 1e-12 * res_ref["first_defect"]
+    assert abs(int(res["solver"]["iterations"]) - int(res_ref["solver"]["iterations"])) <= 2
+    assert o.l2_error_sq(x) < 1e-8
+    # the residual of the returned solution, in FP64 on the oracle
+    r = o.residual(x)
+    assert np.linalg.norm(r) <= 1.01e-10 * res["first_defect"]
+
+
+def test_solver_not_converged_and_errors(pkg, of):
+    say("[gpu] error paths")
+    cfg = pkg.make_config(n=4, degree=1, alpha=3.0, problem=pkg.PROBLEM_EXP)
+    go = pkg.GridOperator(cfg)
+    res = go.linear_problem_solve(go.vector(), reduction=1e-10, maxit=1)
+    assert not res["solver"]["converged"]
+    assert res["solver"]["iterations"] == 1
+    with pytest.raises(pkg.PdelabError) as ei:
+        go.jacobian_apply_nonlinear(go.vector(), go.vector(), go.vector())
+    assert ei.value.status == 2
+    # NaN defect -> solver abort (dune-istl SolverAbort)
+    x = np.zeros(go.N)
+    x[3] = np.nan
+    r = go.vector(x)
+    with pytest.raises(pkg.PdelabError) as ei:
+        go.solve(go.vector(), r)
+    assert ei.value.status == 5
+    # outflow on inflow (convectiondiffusiondg.hh:802-806)
+    cfg2 = pkg.make_config(n=2, degree=1, b=(-1.0, 0, 0), bctype=(0, -2, 0, 0, 0, 0))
+    go2 = pkg.GridOperator(cfg2)
+    with pytest.raises(pkg.PdelabError) as ei:
+        go2.jacobian_apply(go2.vector(), go2.vector())
+    assert ei.value.status == 6
+    bad = pkg.make_config(n=2, degree=5)
+    with pytest.raises(pkg.PdelabError):
+        pkg.GridOperator(bad)
+
+
+def test_mixed_precision_refinement(pkg, of):
+    say("[gpu] mixed precision 12^3 Q3 lognormal kappa")
+    n = 12
+    cfg = pkg.make_config(n=n, degree=3, alpha=3.0, kappa_per_cell=True, problem=pkg.PROBLEM_CONSTANT, f0=1.0, g0=0.0)
+    kappa = np.exp(np.random.default_rng(8).normal(size=n ** 3))
+    go, o = pkg.GridOperator(cfg, kappa), of.Oracle(cfg, kappa)
+    o.set_threads(8)
+    xd = go.vector()
+    res = go.solve_mixed(xd, inner_reduction=1e-4, outer_reduction=1e-10)
+    assert res["converged"], res
+    x = xd.download()
+    x_ref, _ = o.linear_problem_solve(reduction=1e-13, maxit=20000)
+    assert rel(x, x_ref) <= 1e-8
+    r = o.residual(x)
+    assert np.linalg.norm(r) <= 1.01e-10 * res["first_defect"]
+
+
+def test_adjoint_gradient_vs_oracle(pkg, of):
+    say("[gpu] grad logkappa 6^3 Q2")
+    n = 6
+    cfg = pkg.make_config(n=n, degree=2, alpha=3.0, kappa_per_cell=True, problem=pkg.PROBLEM_CONSTANT, f0=1.0)
+    rng = np.random.default_rng(9)
+    kappa = np.exp(rng.normal(0, 0.5, n ** 3))
+    go, o = pkg.GridOperator(cfg, kappa), of.Oracle(cfg, kappa)
+    u = rng.normal(size=go.N)
+    lam = rng.normal(size=go.N)
+    g = go.cell_vector()
+    go.grad_logkappa(go.vector(u), go.vector(lam), g)
+    g_ref = o.grad_logkappa(u, lam)
+    assert rel(g.download(), g_ref) <= 1e-12
+    gs = go.cell_vector()
+    go.grad_source(go.vector(lam), gs)
+    assert rel(gs.download(), o.grad_source(lam)) <= 1e-12
+
+
+@pytest.mark.parametrize("method", [0, 1], ids=["cg", "bicgstab"])
+def test_transpose_solve(pkg, of, method):
+    say(f"[gpu] transpose solve method={method}")
+    cfg = pkg.make_config(n=5, degree=2, alpha=3.0, b=(1.0, 0.5, 0.25), kappa0=0.1 if method else 1.0)
+    if method == 0:
+        cfg.b[0] = cfg.b[1] = cfg.b[2] = 0.0
+    go, o = pkg.GridOperator(cfg), of.Oracle(cfg)
+    rhs = np.random.default_rng(10).normal(size=go.N)
+    z, r = go.vector(), go.vector(rhs)
+    res = go.solve(z, r, method=method, reduction=1e-12, transpose=True)
+    assert res["converged"]
+    zt = z.download()
+    back = o.jacobian_apply_transpose(zt)
+    assert rel(back, rhs) <= 2e-12
+
+
+def test_large_apply_properties(pkg, of):
+    """64^3 Q3 (16.8M DoF): linearity and symmetry x^T J y == y^T J x of the SIPG
+    operator, sizes beyond the oracle's reach, plus parity on a 16^3 sub-problem."""
+    say("[gpu] 64^3 Q3 properties")
+    cfg = pkg.make_config(n=64, degree=3, alpha=3.0, problem=pkg.PROBLEM_EXP)
+    go = pkg.GridOperator(cfg)
+    rng = np.random.default_rng(11)
+    x = rng.uniform(-1, 1, go.N)
+    y = rng.uniform(-1, 1, go.N)
+    xd, yd = go.vector(x), go.vector(y)
+    jx, jy = go.vector(), go.vector()
+    go.jacobian_apply(xd, jx)
+    go.jacobian_apply(yd, jy)
+    a = go.dot(yd, jx)
+    b = go.dot(xd, jy)
+    assert abs(a - b) <= 1e-11 * (abs(a) + abs(b))
+    s = go.vector(x + 2.0 * y)
+    js = go.vector()
+    go.jacobian_apply(s, js)
+    lin = js.download() - (jx.download() + 2.0 * jy.download())
+    assert np.linalg.norm(lin) <= 1e-13 * np.linalg.norm(js.download())

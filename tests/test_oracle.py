@@ -1,0 +1,266 @@
+"""Pins the oracle (CPU restatement of the reference algorithm) before it is trusted.
+
+Checks, each chosen so that a plausible slip (dropped term, wrong sign, transposed
+operand, wrong index) fails at least one:
+  * 1D basis and Gauss tables against exact sympy fixtures (tests/golden) and the
+    closed-form 3- and 4-point Gauss rules;
+  * the quadrature-loop operator against an independent tensor-product formulation
+    built from the exact tables (all methods, weights, variable kappa, convection,
+    reaction, anisotropic cells, outflow boundaries);
+  * structural identities: assembled matrix x == jacobian_apply(x) (as
+    R:dune/pdelab/test/matrixfree/matrix_free_linear.cc:392 relies on), SIPG
+    symmetry and positive definiteness, point diagonal == diag(J)
+    (R:dune/pdelab/test/testblocklocaloperatorwrapper.cc:254-272), transpose;
+  * SIPG consistency: R(I_h u) = 0 for u in Q_k and the solve returns I_h u;
+  * L2 convergence order k+1 and the reference tests' error thresholds in 3D
+    (R:dune/pdelab/test/testconvectiondiffusiondg.cc:160-168);
+  * solvers against a direct solve, matrix-based vs matrix-free iteration equality;
+  * the adjoint gradient against central finite differences.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import kron_reference as kr
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+with open(os.path.join(HERE, "golden", "tables_1d.json")) as _f:
+    GOLD = json.load(_f)
+
+
+def fl(v):
+    return np.array([float(s) for s in v])
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_tables_vs_sympy(of, k):
+    t = of.tables_1d(k, 2 * k)
+    g = GOLD["gauss"][str(k + 1)]
+    assert np.allclose(t["x"], fl(g["x"]), rtol=0, atol=2e-16)
+    assert np.allclose(t["w"], fl(g["w"]), rtol=0, atol=2e-16)
+    b = GOLD["basis"][str(k)]
+    P = np.array([fl(r) for r in b["P_at_gauss"]])
+    D = np.array([fl(r) for r in b["D_at_gauss"]])
+    assert np.allclose(t["P"], P, rtol=0, atol=1e-14)
+    assert np.allclose(t["D"], D, rtol=0, atol=1e-13)
+    for key in ("e0", "e1", "d0", "d1"):
+        assert np.allclose(t[key], fl(b[key]), rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("order", range(0, 12))
+def test_gauss_rule_sizes_and_exactness(of, order):
+    t = of.tables_1d(1, order)
+    m = order // 2 + 1
+    assert len(t["x"]) == m
+    g = GOLD["gauss"][str(m)] if m <= 6 else None
+    if g:
+        assert np.allclose(t["x"], fl(g["x"]), atol=3e-16)
+    # exact for monomials up to degree 2m-1
+    for p in range(2 * m):
+        assert abs(np.dot(t["w"], t["x"] ** p) - 1.0 / (p + 1)) < 1e-15
+
+
+def test_closed_form_gauss(of):
+    t3 = of.tables_1d(2, 4)
+    s = math.sqrt(3.0 / 5.0)
+    assert np.allclose(t3["x"], [0.5 - 0.5 * s, 0.5, 0.5 + 0.5 * s], atol=1e-16)
+    assert np.allclose(t3["w"], [5 / 18, 8 / 18, 5 / 18], atol=1e-16)
+    t4 = of.tables_1d(3, 6)
+    a = math.sqrt(3 / 7 - 2 / 7 * math.sqrt(6 / 5))
+    bb = math.sqrt(3 / 7 + 2 / 7 * math.sqrt(6 / 5))
+    x = sorted([0.5 - 0.5 * bb, 0.5 - 0.5 * a, 0.5 + 0.5 * a, 0.5 + 0.5 * bb])
+    w30 = math.sqrt(30)
+    assert np.allclose(t4["x"], x, atol=1e-16)
+    assert np.allclose(t4["w"], [(18 - w30) / 72, (18 + w30) / 72, (18 + w30) / 72, (18 - w30) / 72], atol=1e-16)
+
+
+CASES = [
+    # (description, config kwargs, kappa seed)
+    ("sipg_q1_2x2x2", dict(nx=2, ny=2, nz=2, degree=1, alpha=3.0), None),
+    ("sipg_q2_2x1x1", dict(nx=2, ny=1, nz=1, degree=2, alpha=3.0), None),
+    ("sipg_q3_1x2x1", dict(nx=1, ny=2, nz=1, degree=3, alpha=2.0), None),
+    ("nipg_q2_var", dict(nx=2, ny=2, nz=1, degree=2, method=0, alpha=1.0), 1),
+    ("iipg_q1_var", dict(nx=3, ny=1, nz=2, degree=1, method=2, alpha=5.0), 2),
+    ("sipg_off_q2", dict(nx=1, ny=2, nz=2, degree=2, weights=1, alpha=3.0), 3),
+    ("sipg_aniso_q2", dict(nx=2, ny=3, nz=1, degree=2, lx=1.0, ly=0.5, lz=2.0, alpha=3.0), 4),
+    ("conv_q2", dict(nx=2, ny=2, nz=2, degree=2, alpha=3.0, b=(1.0, 0.5, -0.25), c=0.7, kappa0=0.01), None),
+    ("conv_q1_var", dict(nx=3, ny=2, nz=1, degree=1, alpha=3.0, b=(-0.3, 0.0, 2.0)), 5),
+    ("outflow_q2", dict(nx=2, ny=1, nz=2, degree=2, alpha=3.0, b=(1.0, 0.0, 0.0), bctype=(0, -2, 0, -1, -3, 0)), None),
+]
+
+
+def _kappa(cfg, seed):
+    if seed is None:
+        return None
+    rng = np.random.default_rng(seed)
+    return np.exp(rng.normal(0.0, 1.0, cfg.nx * cfg.ny * cfg.nz))
+
+
+@pytest.mark.parametrize("name,kw,seed", CASES, ids=[c[0] for c in CASES])
+def test_oracle_matches_tensor_product_form(of, name, kw, seed):
+    cfg = of.cfgmod.make_config(**kw)
+    kappa = _kappa(cfg, seed)
+    if kappa is not None:
+        cfg.kappa_per_cell = 1
+    o = of.Oracle(cfg, kappa)
+    A = o.dense_matrix()
+    Ak = kr.dense_operator(cfg, kappa)
+    scale = np.abs(A).max()
+    assert np.abs(A - Ak).max() <= 1e-13 * scale, name
+    x = np.random.default_rng(7).uniform(-1, 1, o.N)
+    y = o.jacobian_apply(x)
+    assert np.abs(y - A @ x).max() <= 1e-13 * np.abs(y).max()
+    assert np.abs(o.point_diagonal() - np.diag(A)).max() <= 1e-13 * scale
+    assert np.abs(o.jacobian_apply_transpose(x) - A.T @ x).max() <= 1e-13 * np.abs(y).max()
+    assert np.abs(o.jacobian_apply_twosided(x) - y).max() <= 1e-13 * np.abs(y).max()
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_sipg_symmetric_positive_definite(of, k):
+    cfg = of.cfgmod.make_config(nx=2, ny=2, nz=1, degree=k, alpha=3.0, kappa_per_cell=True)
+    kappa = _kappa(cfg, 11)
+    A = of.Oracle(cfg, kappa).dense_matrix()
+    assert np.abs(A - A.T).max() <= 1e-14 * np.abs(A).max()
+    assert np.linalg.eigvalsh(0.5 * (A + A.T)).min() > 0
+
+
+def test_nipg_not_symmetric(of):
+    cfg = of.cfgmod.make_config(nx=2, ny=1, nz=1, degree=1, method=0, alpha=3.0)
+    A = of.Oracle(cfg).dense_matrix()
+    assert np.abs(A - A.T).max() > 1e-3
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_polynomial_reproduction(of, k):
+    q = (0.3, 1.0, 2.0, -1.0, 0.0, 0.0, 0.0) if k == 1 else (0.3, 1.0, 2.0, -1.0, 1.0, 1.0, -2.0)
+    cfg = of.cfgmod.make_config(nx=3, ny=2, nz=2, degree=k, alpha=3.0, problem=of.cfgmod.PROBLEM_QUADRATIC, quad=q)
+    o = of.Oracle(cfg)
+    u = o.interpolate()
+    r = o.residual(u)
+    assert np.abs(r).max() < 1e-13
+    x, res = o.linear_problem_solve(reduction=1e-13)
+    assert res["solver"]["converged"]
+    assert np.abs(x - u).max() < 1e-9
+
+
+def test_convergence_order(of):
+    for k, expect in ((1, 2.0), (2, 3.0)):
+        errs = []
+        for n in (4, 8):
+            cfg = of.cfgmod.make_config(n=n, degree=k, alpha=3.0, problem=of.cfgmod.PROBLEM_EXP)
+            o = of.Oracle(cfg)
+            x, res = o.linear_problem_solve(reduction=1e-12)
+            assert res["solver"]["converged"]
+            errs.append(math.sqrt(o.l2_error_sq(x)))
+        rate = math.log2(errs[0] / errs[1])
+        assert rate > expect - 0.25, (k, errs, rate)
+
+
+def test_reference_error_threshold_3d(of):
+    """R:dune/pdelab/test/testconvectiondiffusiondg.cc:160-168 (2D, Q1, 16^2, pass iff
+    l2 error^2 <= 1e-6) restated in 3D on 16^3 cells with the same defaults (SIPG, weightsOn, alpha 1)."""
+    cfg = of.cfgmod.make_config(n=16, degree=1, alpha=1.0, problem=of.cfgmod.PROBLEM_EXP)
+    o = of.Oracle(cfg)
+    x, res = o.linear_problem_solve(reduction=1e-12)
+    assert res["solver"]["converged"]
+    assert o.l2_error_sq(x) <= 1e-6
+
+
+def test_solvers_against_direct_solve(of):
+    cfg = of.cfgmod.make_config(n=3, degree=2, alpha=3.0, problem=of.cfgmod.PROBLEM_EXP)
+    o = of.Oracle(cfg)
+    A = o.dense_matrix()
+    r0 = o.residual(np.zeros(o.N))
+    xd = -np.linalg.solve(A, r0)
+    for method in (0, 1):
+        x, res = o.linear_problem_solve(method=method, reduction=1e-12)
+        assert res["solver"]["converged"]
+        assert np.abs(x - xd).max() <= 1e-9 * np.abs(xd).max()
+
+
+def test_matrix_free_equals_matrix_based_iterations(of):
+    """matrix_free_linear.cc:390-394: identical iteration counts (on 6^3 Q1; on 4^3 the
+    rounding difference between assembled and on-the-fly operator moves the last CG
+    step across the tolerance, a 1-iteration difference)."""
+    cfg = of.cfgmod.make_config(n=6, degree=1, alpha=3.0, problem=of.cfgmod.PROBLEM_EXP)
+    o = of.Oracle(cfg)
+    _, r1 = o.linear_problem_solve(reduction=1e-10, matrix_based=0)
+    _, r2 = o.linear_problem_solve(reduction=1e-10, matrix_based=1)
+    assert r1["solver"]["iterations"] == r2["solver"]["iterations"]
+    _, r3 = o.linear_problem_solve(method=1, reduction=1e-10, matrix_based=0)
+    _, r4 = o.linear_problem_solve(method=1, reduction=1e-10, matrix_based=1)
+    assert r3["solver"]["iterations"] == r4["solver"]["iterations"]
+
+
+def test_bicgstab_convection_diffusion(of):
+    cfg = of.cfgmod.make_config(n=4, degree=1, alpha=3.0, kappa0=0.01, b=(1.0, 0.5, 0.25),
+                                problem=of.cfgmod.PROBLEM_CONSTANT, f0=1.0)
+    o = of.Oracle(cfg)
+    x, res = o.linear_problem_solve(method=1, reduction=1e-10)
+    assert res["solver"]["converged"]
+    r = o.residual(x)
+    assert np.linalg.norm(r) <= 1e-9 * res["first_defect"]
+
+
+def _objective(o, cfg, kappa, uobs):
+    o.set_kappa(kappa)
+    x, res = o.linear_problem_solve(reduction=1e-14, maxit=20000)
+    d = x - uobs
+    return 0.5 * d @ o.mass_apply(d), x
+
+
+def test_grad_logkappa_vs_finite_differences(of):
+    cfg = of.cfgmod.make_config(n=3, degree=1, alpha=3.0, kappa_per_cell=True, problem=of.cfgmod.PROBLEM_CONSTANT,
+                                f0=1.0, g0=0.0)
+    rng = np.random.default_rng(3)
+    theta = rng.normal(0, 0.5, 27)
+    o = of.Oracle(cfg, np.exp(theta))
+    uobs = rng.normal(0, 0.01, o.N)
+    J0, u = _objective(o, cfg, np.exp(theta), uobs)
+    # adjoint: J^T lambda = M (u - uobs)
+    rhs = o.mass_apply(u - uobs)
+    lam, _, res = o.solve(rhs, reduction=1e-14, maxit=20000, matrix_based=2)
+    g = o.grad_logkappa(u, lam)
+    eps = 1e-5
+    for c in (0, 4, 13, 26):
+        tp = theta.copy(); tp[c] += eps
+        tm = theta.copy(); tm[c] -= eps
+        Jp, _ = _objective(o, cfg, np.exp(tp), uobs)
+        Jm, _ = _objective(o, cfg, np.exp(tm), uobs)
+        fd = (Jp - Jm) / (2 * eps)
+        assert abs(fd - g[c]) <= 1e-6 * max(abs(fd), 1e-12) + 1e-12, (c, fd, g[c])
+
+
+def test_grad_source_vs_finite_differences(of):
+    cfg = of.cfgmod.make_config(n=2, degree=2, alpha=3.0, b=(1.0, 0.5, 0.25), kappa0=0.1,
+                                problem=of.cfgmod.PROBLEM_CONSTANT, f0=1.0)
+    o = of.Oracle(cfg)
+    rng = np.random.default_rng(5)
+    uobs = rng.normal(0, 0.01, o.N)
+
+    def J(src):
+        o.set_cell_source(src)
+        x, _ = o.linear_problem_solve(method=1, reduction=1e-14, maxit=20000)
+        d = x - uobs
+        return 0.5 * d @ o.mass_apply(d), x
+
+    s0 = np.zeros(8)
+    J0, u = J(s0)
+    lam, _, _ = o.solve(o.mass_apply(u - uobs), method=1, reduction=1e-14, maxit=20000, matrix_based=2)
+    g = o.grad_source(lam)
+    eps = 1e-4
+    for c in (0, 3, 7):
+        sp = s0.copy(); sp[c] += eps
+        sm = s0.copy(); sm[c] -= eps
+        fd = (J(sp)[0] - J(sm)[0]) / (2 * eps)
+        assert abs(fd - g[c]) <= 1e-6 * abs(fd) + 1e-14, (c, fd, g[c])
+
+
+def test_linearity_flags_and_errors(of):
+    cfg = of.cfgmod.make_config(nx=2, ny=1, nz=1, degree=1, b=(-1.0, 0, 0), bctype=(0, -2, 0, 0, 0, 0))
+    o = of.Oracle(cfg)
+    with pytest.raises(RuntimeError, match="Outflow"):
+        o.jacobian_apply(np.ones(o.N))
