@@ -49,7 +49,7 @@ def lib():
         L = ctypes.CDLL(LIB)
         vp = ctypes.c_void_p
         L.oracle_create.restype = vp
-        L.oracle_create.argtypes = [ctypes.POINTER(cfgmod.PdelabConfig), _dp]
+        L.oracle_create.argtypes = [vp, _dp]
         L.oracle_destroy.argtypes = [vp]
         L.oracle_last_error.restype = ctypes.c_char_p
         L.oracle_last_error.argtypes = [vp]
@@ -112,7 +112,8 @@ class Oracle:
             self._kappa = np.ascontiguousarray(kappa, dtype=np.float64)
             assert self._kappa.size == self.ncells
             kp = _p(self._kappa)
-        self.h = lib().oracle_create(ctypes.byref(cfg), kp)
+        # the config may come from the product package's ctypes class: pass its address
+        self.h = lib().oracle_create(ctypes.addressof(cfg), kp)
         if not self.h:
             raise ValueError("oracle_create rejected the config")
 

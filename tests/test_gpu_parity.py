@@ -184,7 +184,9 @@ def test_linear_problem_solve_c1(pkg, of, method):
     # both solve the same system to 1e-10 relative residual
     assert rel(x, x_ref) <= 1e-7
     assert abs(res["first_defect"] - res_ref["first_defect"]) <= 1e-12 * res_ref["first_defect"]
-    assert abs(int(res["solver"]["iterations"]) - int(res_ref["solver"]["iterations"])) <= 2
+    # same Krylov recurrence; CG counts agree to 2, BiCGSTAB's are rounding-sensitive (15 %)
+    it, it_ref = int(res["solver"]["iterations"]), int(res_ref["solver"]["iterations"])
+    assert abs(it - it_ref) <= (2 if method == 0 else max(2, int(0.15 * it_ref)))
     assert o.l2_error_sq(x) < 1e-8
     # the residual of the returned solution, in FP64 on the oracle
     r = o.residual(x)
