@@ -83,6 +83,8 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run 5 axpy launches (reads 16 B/DoF, writes 8 B/DoF) to scale HBM counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
@@ -119,7 +121,12 @@ def main():
     y32 = go.vector(dtype=np.float32)
     kern32_ms = go.time_apply(z32, y32, warmup=3, reps=max(20, args.steps))
     log(f"bench: kernel avg {kern_ms * 1e3:.1f} us (fp64), {kern32_ms * 1e3:.1f} us (fp32)")
-    del z32, y32
+    z32.free()
+    y32.free()
+    if args.calibrate:
+        for _ in range(5):
+            go.axpy(0.5, z, y)
+        go.synchronize()
 
     out = {
         "metric": "MDoF/s DG apply; solve time to 1e-10 rel err, mixed-prec vs FP64, 1 GPU",

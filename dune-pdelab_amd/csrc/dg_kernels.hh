@@ -19,10 +19,9 @@
 // with the M^{-1}-premultiplied 1D tables, so each direction is a register-local
 // line operation and the tangential mass is applied once at the end.
 //
-// Work decomposition: a 256-thread workgroup handles CPB = 256/(k+1)^2 consecutive
-// cells; within a cell, thread (p,q) owns one grid line per phase (z-, y-, then
-// x-lines). The per-cell partial sums live in LDS. Global loads in each layout
-// touch whole 128-byte segments of the cell block.
+// Work decomposition: a workgroup handles CPB cells (64 for Q1, 32 for Q2, 16 for Q3);
+// within a cell, thread (p,q) owns one grid line in each of the three layouts (z-,
+// y- and x-lines). The cell's values and the per-cell partial sums live in LDS.
 #ifndef PDELAB_DG_KERNELS_HH
 #define PDELAB_DG_KERNELS_HH
 
@@ -46,6 +45,7 @@ struct DGArgs {
   long long stride[3];
   long long ncells;
   int weightsOn;
+  int rowmode;  // nx % CPB == 0: workgroups are aligned x-row chunks, z-fastest order
   int bc[6];
 };
 
@@ -82,15 +82,21 @@ __device__ __forceinline__ int line_index(int p, int q, int e) {
   return D == 0 ? e + N1 * p + L * q : (D == 1 ? p + N1 * e + L * q : p + N1 * q + L * e);
 }
 
-// s = M^{-1} (A_d z) on one line of cell c (or M^{-1} (A_d^T z) for TRANS).
+// Per-cell apply geometry: CPB cells per workgroup, (k+1)^2 threads per cell.
+template <int N1>
+struct ApplyShape {
+  static constexpr int L = N1 * N1;
+  static constexpr int NL = L * N1;
+  static constexpr int CPB = N1 == 2 ? 64 : (N1 == 3 ? 32 : 16);
+  static constexpr int THREADS = CPB * L;  // 256 (Q1, Q3) or 288 (Q2)
+};
+
+// s = M^{-1} (A_d u) on one grid line of a cell (or M^{-1} A_d^T u for TRANS), from the
+// line values u, the neighbouring lines ul/ur (valid when has_lo/has_hi) and the
+// kappa of the cell and its two neighbours in direction D.
 template <int N1, int D, typename T, bool TRANS, bool CONV, bool REACT, bool VARK>
-__device__ __forceinline__ void line_op(const T* __restrict__ z, const T* __restrict__ kappa, const DGArgs<T>& a,
-                                        long long c, const int coord[3], int p, int q, T kc, T s[N1]) {
-  constexpr int NL = N1 * N1 * N1;
-  T u[N1];
-  const T* zc = z + c * NL;
-#pragma unroll
-  for (int e = 0; e < N1; ++e) u[e] = zc[line_index<N1, D>(p, q, e)];
+__device__ __forceinline__ void line_op(const DGArgs<T>& a, const T (&u)[N1], const T (&ul)[N1], const T (&ur)[N1],
+                                        bool has_lo, bool has_hi, T kc, T klo, T khi, T (&s)[N1]) {
   const T val0 = u[0], val1 = u[N1 - 1];
   T der0 = T(0), der1 = T(0);
 #pragma unroll
@@ -98,20 +104,15 @@ __device__ __forceinline__ void line_op(const T* __restrict__ z, const T* __rest
   const T Sd = a.S[D], Sc = a.Sc[D], bd = a.b[D], theta = a.theta;
   T ce0 = T(0), cd0 = T(0), ce1 = T(0), cd1 = T(0);
   // ---- lower face
-  if (coord[D] > 0) {
-    const long long cl = c - a.stride[D];
-    const T* zl = z + cl * NL;
-    T ul[N1];
-#pragma unroll
-    for (int e = 0; e < N1; ++e) ul[e] = zl[line_index<N1, D>(p, q, e)];
+  if (has_lo) {
     const T v1l = ul[N1 - 1];
     T d1l = T(0);
 #pragma unroll
     for (int e = 0; e < N1; ++e) d1l += a.d1[e] * ul[e];
     T wlo, whi, P;
-    if (VARK) face_coeffs<T>(a, kappa[cl], kc, wlo, whi, P);
+    if (VARK) face_coeffs<T>(a, klo, kc, wlo, whi, P);
     else { wlo = a.wI_lo; whi = a.wI_hi; P = a.PI; }
-    const T J = v1l - val0;           // jump u_lo - u_hi
+    const T J = v1l - val0;              // jump u_lo - u_hi
     const T G = wlo * d1l + whi * der0;  // weighted average of the normal derivative
     if (!TRANS) { ce0 += Sd * (G - P * J); cd0 += Sd * theta * whi * J; }
     else { ce0 += Sd * (-P * J - theta * G); cd0 += -Sd * whi * J; }
@@ -131,18 +132,13 @@ __device__ __forceinline__ void line_op(const T* __restrict__ z, const T* __rest
     }
   }
   // ---- upper face
-  if (coord[D] < a.n[D] - 1) {
-    const long long cr = c + a.stride[D];
-    const T* zr = z + cr * NL;
-    T ur[N1];
-#pragma unroll
-    for (int e = 0; e < N1; ++e) ur[e] = zr[line_index<N1, D>(p, q, e)];
+  if (has_hi) {
     const T v0r = ur[0];
     T d0r = T(0);
 #pragma unroll
     for (int e = 0; e < N1; ++e) d0r += a.d0[e] * ur[e];
     T wlo, whi, P;
-    if (VARK) face_coeffs<T>(a, kc, kappa[cr], wlo, whi, P);
+    if (VARK) face_coeffs<T>(a, kc, khi, wlo, whi, P);
     else { wlo = a.wI_lo; whi = a.wI_hi; P = a.PI; }
     const T J = val1 - v0r;
     const T G = wlo * der1 + whi * d0r;
@@ -184,46 +180,117 @@ __device__ __forceinline__ void line_op(const T* __restrict__ z, const T* __rest
   }
 }
 
+template <int N1, int D, typename T>
+__device__ __forceinline__ void load_line(const T* __restrict__ base, int off, int p, int q, T (&u)[N1]) {
+#pragma unroll
+  for (int e = 0; e < N1; ++e) u[e] = base[off + line_index<N1, D>(p, q, e)];
+}
+
 // y (+)= J z   (or J^T z).  ACCUM: add into y (GridOperator::jacobian_apply semantics);
 // otherwise y = J z.
-template <int N1, typename T, bool TRANS, bool CONV, bool REACT, bool VARK, bool ACCUM>
-__global__ __launch_bounds__(256) void dg_apply_kernel(const T* __restrict__ z, T* __restrict__ y,
-                                                       const T* __restrict__ kappa, const DGArgs<T> a,
-                                                       long long nblocks) {
-  constexpr int L = N1 * N1, NL = L * N1, CPB = 256 / L;
+//
+// ROW (nx % CPB == 0): the CPB cells of a workgroup are one aligned chunk of an x-row
+// and consecutive logical blocks walk z fastest, then y, then the x-chunk, so the y-
+// and z-neighbour lines a workgroup reads were fetched moments earlier by workgroups
+// on the same XCD (L2 hits). The chunk's cell coordinates and the base pointers of
+// the own and neighbouring cell blocks are then wave-uniform (scalar registers) and
+// each lane only adds a 32-bit offset. Without ROW a workgroup takes CPB consecutive
+// cells in lexicographic order.
+// All global loads of a workgroup are issued before the first wait: the own cell
+// (once, coalesced, then shared through LDS) and the six neighbouring lines.
+template <int N1, typename T, bool TRANS, bool CONV, bool REACT, bool VARK, bool ACCUM, bool ROW>
+__global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const T* __restrict__ z,
+                                                                           T* __restrict__ y,
+                                                                           const T* __restrict__ kappa,
+                                                                           const DGArgs<T> a, long long nblocks) {
+  using SH = ApplyShape<N1>;
+  constexpr int L = SH::L, NL = SH::NL, CPB = SH::CPB;
+  __shared__ T U[CPB * NL];
   __shared__ T S[CPB * NL];
   const int t = threadIdx.x;
   const int lc = t / L, l = t % L, p = l % N1, q = l / N1;
   const long long blk = xcd_logical_block(blockIdx.x, nblocks);
-  const long long c = blk * CPB + lc;
-  const bool active = (lc < CPB) && (c < a.ncells);
-  int coord[3] = {0, 0, 0};
-  T kc = a.kappa0;
-  if (active) {
-    coord[0] = (int)(c % a.n[0]);
-    coord[1] = (int)((c / a.n[0]) % a.n[1]);
-    coord[2] = (int)(c / ((long long)a.n[0] * a.n[1]));
-    if (VARK) kc = kappa[c];
+  // c0: first cell of the block (uniform); this lane's cell is c0 + lc
+  long long c0;
+  int coord[3];
+  bool active;
+  if (ROW) {
+    const unsigned nz = (unsigned)a.n[2], ny = (unsigned)a.n[1];
+    const unsigned ub = (unsigned)blk;
+    const unsigned kz = ub % nz, rest = ub / nz;
+    const unsigned jy = rest % ny, xc = rest / ny;
+    c0 = (long long)xc * CPB + a.stride[1] * jy + a.stride[2] * kz;
+    coord[0] = (int)(xc * CPB) + lc;
+    coord[1] = (int)jy;
+    coord[2] = (int)kz;
+    active = true;
+  } else {
+    c0 = blk * CPB;
+    const long long c = c0 + lc;
+    active = c < a.ncells;
+    const long long cc = active ? c : 0;
+    coord[0] = (int)(cc % a.n[0]);
+    coord[1] = (int)((cc / a.n[0]) % a.n[1]);
+    coord[2] = (int)(cc / ((long long)a.n[0] * a.n[1]));
   }
-  T* Sl = S + (active ? lc * NL : 0);
-  T s[N1];
-  if (active) {
-    line_op<N1, 2, T, TRANS, CONV, REACT, VARK>(z, kappa, a, c, coord, p, q, kc, s);
+  const int off = lc * NL;  // lane's cell block relative to c0
+  const T* zc = z + c0 * NL;
+  T* Ul = U + off;
+  T* Sl = S + off;
+  // ---- all global loads up front
+  T uz[N1], xl[N1], xr[N1], yl[N1], yr[N1], zl[N1], zr[N1];
+  bool lo[3], hi[3];
+  T kc = a.kappa0, klo[3], khi[3];
 #pragma unroll
-    for (int e = 0; e < N1; ++e) Sl[line_index<N1, 2>(p, q, e)] = s[e];
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = active && coord[d] > 0;
+    hi[d] = active && coord[d] < a.n[d] - 1;
+    klo[d] = khi[d] = a.kappa0;
+  }
+  if (active) {
+    load_line<N1, 2, T>(zc, off, p, q, uz);
+    if (lo[0]) load_line<N1, 0, T>(zc - a.stride[0] * NL, off, p, q, xl);
+    if (hi[0]) load_line<N1, 0, T>(zc + a.stride[0] * NL, off, p, q, xr);
+    if (lo[1]) load_line<N1, 1, T>(zc - a.stride[1] * NL, off, p, q, yl);
+    if (hi[1]) load_line<N1, 1, T>(zc + a.stride[1] * NL, off, p, q, yr);
+    if (lo[2]) load_line<N1, 2, T>(zc - a.stride[2] * NL, off, p, q, zl);
+    if (hi[2]) load_line<N1, 2, T>(zc + a.stride[2] * NL, off, p, q, zr);
+    if (VARK) {
+      const T* kb = kappa + c0;
+      kc = kb[lc];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (lo[d]) klo[d] = (kb - a.stride[d])[lc];
+        if (hi[d]) khi[d] = (kb + a.stride[d])[lc];
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < N1; ++e) Ul[line_index<N1, 2>(p, q, e)] = uz[e];
+  }
+  __syncthreads();
+  T sx[N1], sy[N1], sz[N1];
+  if (active) {
+    line_op<N1, 2, T, TRANS, CONV, REACT, VARK>(a, uz, zl, zr, lo[2], hi[2], kc, klo[2], khi[2], sz);
+    T uy[N1], ux[N1];
+#pragma unroll
+    for (int e = 0; e < N1; ++e) uy[e] = Ul[line_index<N1, 1>(p, q, e)];
+    line_op<N1, 1, T, TRANS, CONV, REACT, VARK>(a, uy, yl, yr, lo[1], hi[1], kc, klo[1], khi[1], sy);
+#pragma unroll
+    for (int e = 0; e < N1; ++e) ux[e] = Ul[line_index<N1, 0>(p, q, e)];
+    line_op<N1, 0, T, TRANS, CONV, REACT, VARK>(a, ux, xl, xr, lo[0], hi[0], kc, klo[0], khi[0], sx);
+#pragma unroll
+    for (int e = 0; e < N1; ++e) Sl[line_index<N1, 2>(p, q, e)] = sz[e];
   }
   __syncthreads();
   if (active) {
-    line_op<N1, 1, T, TRANS, CONV, REACT, VARK>(z, kappa, a, c, coord, p, q, kc, s);
 #pragma unroll
-    for (int e = 0; e < N1; ++e) Sl[line_index<N1, 1>(p, q, e)] += s[e];
+    for (int e = 0; e < N1; ++e) Sl[line_index<N1, 1>(p, q, e)] += sy[e];
   }
   __syncthreads();
   if (active) {
-    line_op<N1, 0, T, TRANS, CONV, REACT, VARK>(z, kappa, a, c, coord, p, q, kc, s);
     T v[N1];
 #pragma unroll
-    for (int e = 0; e < N1; ++e) v[e] = s[e] + Sl[line_index<N1, 0>(p, q, e)];
+    for (int e = 0; e < N1; ++e) v[e] = sx[e] + Sl[line_index<N1, 0>(p, q, e)];
 #pragma unroll
     for (int i = 0; i < N1; ++i) {
       T acc = T(0);
@@ -250,13 +317,13 @@ __global__ __launch_bounds__(256) void dg_apply_kernel(const T* __restrict__ z, 
     T v[N1];
 #pragma unroll
     for (int e = 0; e < N1; ++e) v[e] = Sl[line_index<N1, 2>(p, q, e)];
-    T* yc = y + c * NL;
+    T* yc = y + c0 * NL;
 #pragma unroll
     for (int i = 0; i < N1; ++i) {
       T acc = T(0);
 #pragma unroll
       for (int j = 0; j < N1; ++j) acc += a.M[i][j] * v[j];
-      const int idx = line_index<N1, 2>(p, q, i);
+      const int idx = off + line_index<N1, 2>(p, q, i);
       if (ACCUM) yc[idx] += acc;
       else yc[idx] = acc;
     }

@@ -164,6 +164,8 @@ static void fill_args(H* h, DGArgs<T>& a) {
   a.stride[1] = c.nx;
   a.stride[2] = (long long)c.nx * c.ny;
   a.ncells = h->ncells;
+  const int cpb = c.degree == 1 ? ApplyShape<2>::CPB : (c.degree == 2 ? ApplyShape<3>::CPB : ApplyShape<4>::CPB);
+  a.rowmode = (c.nx % cpb == 0) ? 1 : 0;
   for (int s = 0; s < 6; ++s) a.bc[s] = c.bctype[s] == 0 ? BC_DIRICHLET : c.bctype[s];
 }
 
@@ -222,14 +224,23 @@ static void fill_load_args(H* h) {
 // kernel dispatch
 template <int N1, typename T, bool TRANS, bool CONV, bool VARK, bool ACCUM>
 static int launch_apply_t(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, bool react) {
-  constexpr int L = N1 * N1, CPB = 256 / L;
-  const long long nb = (h->ncells + CPB - 1) / CPB;
-  if (react)
-    hipLaunchKernelGGL((dg_apply_kernel<N1, T, TRANS, CONV, true, VARK, ACCUM>), dim3((unsigned)nb), dim3(256), 0,
-                       h->stream, z, y, kappa, a, nb);
-  else
-    hipLaunchKernelGGL((dg_apply_kernel<N1, T, TRANS, CONV, false, VARK, ACCUM>), dim3((unsigned)nb), dim3(256), 0,
-                       h->stream, z, y, kappa, a, nb);
+  using SH = ApplyShape<N1>;
+  const long long nb = (h->ncells + SH::CPB - 1) / SH::CPB;
+  if (a.rowmode) {
+    if (react)
+      hipLaunchKernelGGL((dg_apply_kernel<N1, T, TRANS, CONV, true, VARK, ACCUM, true>), dim3((unsigned)nb),
+                         dim3(SH::THREADS), 0, h->stream, z, y, kappa, a, nb);
+    else
+      hipLaunchKernelGGL((dg_apply_kernel<N1, T, TRANS, CONV, false, VARK, ACCUM, true>), dim3((unsigned)nb),
+                         dim3(SH::THREADS), 0, h->stream, z, y, kappa, a, nb);
+  } else {
+    if (react)
+      hipLaunchKernelGGL((dg_apply_kernel<N1, T, TRANS, CONV, true, VARK, ACCUM, false>), dim3((unsigned)nb),
+                         dim3(SH::THREADS), 0, h->stream, z, y, kappa, a, nb);
+    else
+      hipLaunchKernelGGL((dg_apply_kernel<N1, T, TRANS, CONV, false, VARK, ACCUM, false>), dim3((unsigned)nb),
+                         dim3(SH::THREADS), 0, h->stream, z, y, kappa, a, nb);
+  }
   HIPCHK(h, hipGetLastError());
   return PDELAB_OK;
 }

@@ -1,0 +1,124 @@
+// 3D restatement of R:dune/pdelab/test/testconvectiondiffusiondg.cc on the HIP path,
+// written against the PDELab-shaped host classes: Poisson problem with the manufactured
+// solution u = exp(-|x - 1/2|^2), Q1 DG on 16^3 cells, ConvectionDiffusionDG with the
+// default SIPG / weightsOn / alpha 1, StationaryLinearProblemSolver with a matrix-free
+// CG + Jacobi backend to 1e-12; pass iff the squared L2 error <= 1e-6 (the reference's
+// threshold). Further checks: the nonlinear jacobian_apply throws for this linear
+// operator (gridoperator.hh:202-203), OnTheFlyOperator::apply equals jacobian_apply,
+// and the mixed-precision backend reaches 1e-10 on a lognormal-kappa problem.
+// Returns nonzero on failure, like the reference tests.
+#include <cmath>
+#include <cstdio>
+#include <random>
+
+#include "pdelab_amd.hh"
+
+using namespace pdelab_amd;
+
+struct PoissonProblem : ConvectionDiffusionModelProblem {
+  static double u(const std::array<double, 3>& x) {
+    double c = 0;
+    for (int d = 0; d < 3; ++d) c += (0.5 - x[d]) * (0.5 - x[d]);
+    return std::exp(-c);
+  }
+  double f(const Element& e, const std::array<double, 3>& xl) const {
+    auto x = e.geometry().global(xl);
+    double c = 0;
+    for (int d = 0; d < 3; ++d) c += (0.5 - x[d]) * (0.5 - x[d]);
+    return (6.0 - 4.0 * c) * std::exp(-c);  // -lap u in 3D
+  }
+  double g(const Element& e, const std::array<double, 3>& xl) const { return u(e.geometry().global(xl)); }
+};
+
+struct LognormalProblem : ConvectionDiffusionModelProblem {
+  std::vector<double> kappa;
+  double A(const Element& e, const std::array<double, 3>&) const { return kappa[e.index]; }
+  double f(const Element&, const std::array<double, 3>&) const { return 1.0; }
+  double g(const Element&, const std::array<double, 3>&) const { return 0.0; }
+};
+
+int main() {
+  int fails = 0;
+  try {
+    // --- testconvectiondiffusiondg.cc in 3D
+    StructuredGrid grid({1.0, 1.0, 1.0}, {16, 16, 16});
+    using FEM = QkDGLocalFiniteElementMap<double, double, 1, 3>;
+    FEM fem;
+    using GFS = GridFunctionSpace<FEM>;
+    GFS gfs(grid, fem);
+    PoissonProblem problem;
+    using LOP = ConvectionDiffusionDG<PoissonProblem, FEM>;
+    LOP lop(problem);
+    using GO = GridOperator<GFS, GFS, LOP>;
+    GO go(gfs, gfs, lop);
+    std::printf("gfs with %zu dofs generated\n", gfs.size());
+    auto x = go.makeVector();
+    x = 0.0;
+    using LS = ISTLBackend_SEQ_MatrixFree_CG_Jac<GO>;
+    LS ls(go, 5000, 0);
+    StationaryLinearProblemSolver<GO, LS, GO::Domain> solver(go, ls, x, 1e-12);
+    solver.apply();
+    const auto& res = solver.result();
+    const double err = l2_error_squared(gfs, x.download(), PoissonProblem::u, 10);
+    std::printf("l2errorsquared: %g (iterations %u, converged %d)\n", err, res.iterations, (int)res.converged);
+    if (std::isnan(err) || std::fabs(err) > 1e-6 || !res.converged) ++fails;
+
+    // --- linear / nonlinear jacobian_apply dispatch
+    bool threw = false;
+    try {
+      auto y = go.makeVector();
+      go.jacobian_apply(x, x, y);
+    } catch (const Exception&) {
+      threw = true;
+    }
+    std::printf("nonlinear jacobian_apply on linear operator throws: %d\n", (int)threw);
+    if (!threw) ++fails;
+
+    // --- OnTheFlyOperator::apply == jacobian_apply
+    OnTheFlyOperator<GO::Domain, GO::Range, GO> op(go);
+    auto y1 = go.makeVector();
+    auto y2 = go.makeVector();
+    op.apply(x, y1);
+    go.jacobian_apply(x, y2);
+    auto h1 = y1.download(), h2 = y2.download();
+    double diff = 0;
+    for (size_t i = 0; i < h1.size(); ++i) diff = std::max(diff, std::fabs(h1[i] - h2[i]));
+    std::printf("OnTheFlyOperator vs jacobian_apply max diff: %g\n", diff);
+    if (diff != 0.0) ++fails;
+
+    // --- mixed precision on a lognormal kappa field, Q3
+    StructuredGrid grid3({1.0, 1.0, 1.0}, {12, 12, 12});
+    using FEM3 = QkDGLocalFiniteElementMap<double, double, 3, 3>;
+    FEM3 fem3;
+    GridFunctionSpace<FEM3> gfs3(grid3, fem3);
+    LognormalProblem lp;
+    std::mt19937_64 rng(7);
+    std::normal_distribution<double> nd(0.0, 1.0);
+    for (size_t c = 0; c < grid3.size(); ++c) lp.kappa.push_back(std::exp(nd(rng)));
+    using LOP3 = ConvectionDiffusionDG<LognormalProblem, FEM3>;
+    LOP3 lop3(lp, ConvectionDiffusionDGMethod::SIPG, ConvectionDiffusionDGWeights::weightsOn, 3.0);
+    using GO3 = GridOperator<GridFunctionSpace<FEM3>, GridFunctionSpace<FEM3>, LOP3>;
+    GO3 go3(gfs3, gfs3, lop3);
+    auto xm = go3.makeVector();
+    MixedPrecisionSolver<GO3, Krylov::CG> mixed(go3, 1e-4);
+    auto mr = mixed.apply(xm, 1e-10);
+    std::printf("mixed precision: converged %d outer %d inner %d reduction %g\n", mr.converged, mr.outer_iterations,
+                mr.inner_iterations, mr.reduction);
+    if (!mr.converged || mr.reduction > 1e-10) ++fails;
+    // FP64 reference solve on the same operator
+    auto x64 = go3.makeVector();
+    ISTLBackend_SEQ_MatrixFree_CG_Jac<GO3> ls3(go3, 20000, 0);
+    StationaryLinearProblemSolver<GO3, decltype(ls3), GO3::Domain> s3(go3, ls3, x64, 1e-12);
+    s3.apply();
+    auto a = xm.download(), b = x64.download();
+    double num = 0, den = 0;
+    for (size_t i = 0; i < a.size(); ++i) { num += (a[i] - b[i]) * (a[i] - b[i]); den += b[i] * b[i]; }
+    std::printf("mixed vs FP64 relative difference: %g\n", std::sqrt(num / den));
+    if (std::sqrt(num / den) > 1e-8) ++fails;
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "error: %s\n", e.what());
+    return 1;
+  }
+  std::printf(fails ? "FAILED (%d)\n" : "PASSED\n", fails);
+  return fails ? 1 : 0;
+}
