@@ -83,13 +83,30 @@ __device__ __forceinline__ int line_index(int p, int q, int e) {
 }
 
 // Per-cell apply geometry: CPB cells per workgroup, (k+1)^2 threads per cell.
+// LDS keeps each cell in a rotated layout, slot(x,y,z) = ((x+z)%n) + n*((y+z)%n) + n^2*z:
+// in each of the three line layouts the n^2 lanes of a cell then touch n^2 distinct
+// slots modulo n^2 (conflict-free banks), and the padded cell stride NLP puts the
+// cells of one wave on disjoint bank ranges.
 template <int N1>
 struct ApplyShape {
   static constexpr int L = N1 * N1;
   static constexpr int NL = L * N1;
+  static constexpr int NLP = NL + (N1 == 4 ? 16 : 1);
   static constexpr int CPB = N1 == 2 ? 64 : (N1 == 3 ? 32 : 16);
   static constexpr int THREADS = CPB * L;  // 256 (Q1, Q3) or 288 (Q2)
 };
+
+template <int N1, int D>
+__device__ __forceinline__ int lds_index(int p, int q, int e) {
+  // (x, y, z) of element e of the lane's line in layout D
+  const int x = D == 0 ? e : p;
+  const int yy = D == 0 ? p : (D == 1 ? e : q);
+  const int zz = D == 2 ? e : q;
+  int xs = x + zz, ys = yy + zz;
+  if (xs >= N1) xs -= N1;
+  if (ys >= N1) ys -= N1;
+  return xs + N1 * ys + N1 * N1 * zz;
+}
 
 // s = M^{-1} (A_d u) on one grid line of a cell (or M^{-1} A_d^T u for TRANS), from the
 // line values u, the neighbouring lines ul/ur (valid when has_lo/has_hi) and the
@@ -186,48 +203,33 @@ __device__ __forceinline__ void load_line(const T* __restrict__ base, int off, i
   for (int e = 0; e < N1; ++e) u[e] = base[off + line_index<N1, D>(p, q, e)];
 }
 
-// y (+)= J z   (or J^T z).  ACCUM: add into y (GridOperator::jacobian_apply semantics);
-// otherwise y = J z.
+// y (+)= J z   (or J^T z), general grids.  ACCUM: add into y (GridOperator::jacobian_apply
+// semantics); otherwise y = J z. A workgroup takes CPB consecutive cells in lexicographic
+// order (the chunk may span x-rows).
 //
-// ROW (nx % CPB == 0): the CPB cells of a workgroup are one aligned chunk of an x-row
-// and consecutive logical blocks walk z fastest, then y, then the x-chunk, so the y-
-// and z-neighbour lines a workgroup reads were fetched moments earlier by workgroups
-// on the same XCD (L2 hits). The chunk's cell coordinates and the base pointers of
-// the own and neighbouring cell blocks are then wave-uniform (scalar registers) and
-// each lane only adds a 32-bit offset. Without ROW a workgroup takes CPB consecutive
-// cells in lexicographic order.
-// All global loads of a workgroup are issued before the first wait: the own cell
-// (once, coalesced, then shared through LDS) and the six neighbouring lines.
-template <int N1, typename T, bool TRANS, bool CONV, bool REACT, bool VARK, bool ACCUM, bool ROW>
+// Every global access is a coalesced z-line access (n^2 lanes of a cell read or write
+// n^2 consecutive values per instruction). Lines in x and y are served from LDS:
+// the own cells, the cells of the rows y-1 and y+1 (same x positions) and the two cells
+// beyond the ends of the chunk are staged there; x-neighbours inside the chunk are the
+// staged own cells. All global loads are issued before the first wait.
+template <int N1, typename T, bool TRANS, bool CONV, bool REACT, bool VARK, bool ACCUM>
 __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const T* __restrict__ z,
                                                                            T* __restrict__ y,
                                                                            const T* __restrict__ kappa,
                                                                            const DGArgs<T> a, long long nblocks) {
   using SH = ApplyShape<N1>;
-  constexpr int L = SH::L, NL = SH::NL, CPB = SH::CPB;
-  __shared__ T U[CPB * NL];
-  __shared__ T S[CPB * NL];
+  constexpr int L = SH::L, NL = SH::NL, NLP = SH::NLP, CPB = SH::CPB;
+  __shared__ T U[(CPB + 2) * NLP];  // [0] cell before the chunk, [1..CPB] own cells, [CPB+1] cell after
+  __shared__ T YM[CPB * NLP];       // row y-1, later the z partial sums and the x mass sweep
+  __shared__ T YP[CPB * NLP];       // row y+1, later the x partial sums
   const int t = threadIdx.x;
   const int lc = t / L, l = t % L, p = l % N1, q = l / N1;
   const long long blk = xcd_logical_block(blockIdx.x, nblocks);
-  // c0: first cell of the block (uniform); this lane's cell is c0 + lc
-  long long c0;
+  const long long c0 = blk * CPB;
   int coord[3];
-  bool active;
-  if (ROW) {
-    const unsigned nz = (unsigned)a.n[2], ny = (unsigned)a.n[1];
-    const unsigned ub = (unsigned)blk;
-    const unsigned kz = ub % nz, rest = ub / nz;
-    const unsigned jy = rest % ny, xc = rest / ny;
-    c0 = (long long)xc * CPB + a.stride[1] * jy + a.stride[2] * kz;
-    coord[0] = (int)(xc * CPB) + lc;
-    coord[1] = (int)jy;
-    coord[2] = (int)kz;
-    active = true;
-  } else {
-    c0 = blk * CPB;
-    const long long c = c0 + lc;
-    active = c < a.ncells;
+  const long long c = c0 + lc;
+  const bool active = c < a.ncells;
+  {
     const long long cc = active ? c : 0;
     coord[0] = (int)(cc % a.n[0]);
     coord[1] = (int)((cc / a.n[0]) % a.n[1]);
@@ -235,26 +237,27 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
   }
   const int off = lc * NL;  // lane's cell block relative to c0
   const T* zc = z + c0 * NL;
-  T* Ul = U + off;
-  T* Sl = S + off;
-  // ---- all global loads up front
-  T uz[N1], xl[N1], xr[N1], yl[N1], yr[N1], zl[N1], zr[N1];
   bool lo[3], hi[3];
-  T kc = a.kappa0, klo[3], khi[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     lo[d] = active && coord[d] > 0;
     hi[d] = active && coord[d] < a.n[d] - 1;
-    klo[d] = khi[d] = a.kappa0;
   }
+  T uz[N1], zl[N1], zr[N1], yl[N1], yr[N1], xe[N1];
+  T kc = a.kappa0, klo[3], khi[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) klo[d] = khi[d] = a.kappa0;
+  // the end-cell loads: lanes of cell 0 load the cell before the chunk, lanes of the
+  // last cell the cell after it (only where that cell exists and is an x-neighbour)
+  const bool first = lc == 0, last = lc == CPB - 1;
+  const bool ld_xe = active && ((first && lo[0]) || (last && hi[0]));
   if (active) {
     load_line<N1, 2, T>(zc, off, p, q, uz);
-    if (lo[0]) load_line<N1, 0, T>(zc - a.stride[0] * NL, off, p, q, xl);
-    if (hi[0]) load_line<N1, 0, T>(zc + a.stride[0] * NL, off, p, q, xr);
-    if (lo[1]) load_line<N1, 1, T>(zc - a.stride[1] * NL, off, p, q, yl);
-    if (hi[1]) load_line<N1, 1, T>(zc + a.stride[1] * NL, off, p, q, yr);
     if (lo[2]) load_line<N1, 2, T>(zc - a.stride[2] * NL, off, p, q, zl);
     if (hi[2]) load_line<N1, 2, T>(zc + a.stride[2] * NL, off, p, q, zr);
+    if (lo[1]) load_line<N1, 2, T>(zc - a.stride[1] * NL, off, p, q, yl);
+    if (hi[1]) load_line<N1, 2, T>(zc + a.stride[1] * NL, off, p, q, yr);
+    if (ld_xe) load_line<N1, 2, T>(zc + (first && lo[0] ? -NL : NL), off, p, q, xe);
     if (VARK) {
       const T* kb = kappa + c0;
       kc = kb[lc];
@@ -264,59 +267,251 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
         if (hi[d]) khi[d] = (kb + a.stride[d])[lc];
       }
     }
+  }
+  T* Ul = U + (lc + 1) * NLP;
+  T* YMl = YM + lc * NLP;
+  T* YPl = YP + lc * NLP;
+  if (active) {
 #pragma unroll
-    for (int e = 0; e < N1; ++e) Ul[line_index<N1, 2>(p, q, e)] = uz[e];
+    for (int e = 0; e < N1; ++e) {
+      const int sl = lds_index<N1, 2>(p, q, e);
+      Ul[sl] = uz[e];
+      if (lo[1]) YMl[sl] = yl[e];
+      if (hi[1]) YPl[sl] = yr[e];
+      if (ld_xe) (first && lo[0] ? U : U + (CPB + 1) * NLP)[sl] = xe[e];
+    }
   }
   __syncthreads();
   T sx[N1], sy[N1], sz[N1];
   if (active) {
     line_op<N1, 2, T, TRANS, CONV, REACT, VARK>(a, uz, zl, zr, lo[2], hi[2], kc, klo[2], khi[2], sz);
-    T uy[N1], ux[N1];
+    T u[N1], ul[N1], ur[N1];
 #pragma unroll
-    for (int e = 0; e < N1; ++e) uy[e] = Ul[line_index<N1, 1>(p, q, e)];
-    line_op<N1, 1, T, TRANS, CONV, REACT, VARK>(a, uy, yl, yr, lo[1], hi[1], kc, klo[1], khi[1], sy);
+    for (int e = 0; e < N1; ++e) {
+      const int sl = lds_index<N1, 1>(p, q, e);
+      u[e] = Ul[sl];
+      ul[e] = lo[1] ? YMl[sl] : T(0);
+      ur[e] = hi[1] ? YPl[sl] : T(0);
+    }
+    line_op<N1, 1, T, TRANS, CONV, REACT, VARK>(a, u, ul, ur, lo[1], hi[1], kc, klo[1], khi[1], sy);
+    // x-neighbours: staged cell lc-1 / lc+1 (U[0] and U[CPB+1] hold the end cells)
+    const T* xlo = Ul - NLP;
+    const T* xhi = Ul + NLP;
 #pragma unroll
-    for (int e = 0; e < N1; ++e) ux[e] = Ul[line_index<N1, 0>(p, q, e)];
-    line_op<N1, 0, T, TRANS, CONV, REACT, VARK>(a, ux, xl, xr, lo[0], hi[0], kc, klo[0], khi[0], sx);
+    for (int e = 0; e < N1; ++e) {
+      const int sl = lds_index<N1, 0>(p, q, e);
+      u[e] = Ul[sl];
+      ul[e] = lo[0] ? xlo[sl] : T(0);
+      ur[e] = hi[0] ? xhi[sl] : T(0);
+    }
+    line_op<N1, 0, T, TRANS, CONV, REACT, VARK>(a, u, ul, ur, lo[0], hi[0], kc, klo[0], khi[0], sx);
+  }
+  __syncthreads();  // all reads of the staged cells done
+  if (active) {
 #pragma unroll
-    for (int e = 0; e < N1; ++e) Sl[line_index<N1, 2>(p, q, e)] = sz[e];
+    for (int e = 0; e < N1; ++e) {
+      YMl[lds_index<N1, 2>(p, q, e)] = sz[e];
+      YPl[lds_index<N1, 0>(p, q, e)] = sx[e];
+    }
   }
   __syncthreads();
   if (active) {
-#pragma unroll
-    for (int e = 0; e < N1; ++e) Sl[line_index<N1, 1>(p, q, e)] += sy[e];
-  }
-  __syncthreads();
-  if (active) {
+    // y-layout: s = sy + sz + sx, then the mass along y
     T v[N1];
 #pragma unroll
-    for (int e = 0; e < N1; ++e) v[e] = sx[e] + Sl[line_index<N1, 0>(p, q, e)];
+    for (int e = 0; e < N1; ++e) {
+      const int sl = lds_index<N1, 1>(p, q, e);
+      v[e] = sy[e] + YMl[sl] + YPl[sl];
+    }
 #pragma unroll
     for (int i = 0; i < N1; ++i) {
       T acc = T(0);
 #pragma unroll
       for (int j = 0; j < N1; ++j) acc += a.M[i][j] * v[j];
-      Sl[line_index<N1, 0>(p, q, i)] = acc;
+      Ul[lds_index<N1, 1>(p, q, i)] = acc;
     }
   }
   __syncthreads();
   if (active) {
     T v[N1];
 #pragma unroll
-    for (int e = 0; e < N1; ++e) v[e] = Sl[line_index<N1, 1>(p, q, e)];
+    for (int e = 0; e < N1; ++e) v[e] = Ul[lds_index<N1, 0>(p, q, e)];
 #pragma unroll
     for (int i = 0; i < N1; ++i) {
       T acc = T(0);
 #pragma unroll
       for (int j = 0; j < N1; ++j) acc += a.M[i][j] * v[j];
-      Sl[line_index<N1, 1>(p, q, i)] = acc;
+      YMl[lds_index<N1, 0>(p, q, i)] = acc;
     }
   }
   __syncthreads();
   if (active) {
     T v[N1];
 #pragma unroll
-    for (int e = 0; e < N1; ++e) v[e] = Sl[line_index<N1, 2>(p, q, e)];
+    for (int e = 0; e < N1; ++e) v[e] = YMl[lds_index<N1, 2>(p, q, e)];
+    T* yc = y + c0 * NL;
+#pragma unroll
+    for (int i = 0; i < N1; ++i) {
+      T acc = T(0);
+#pragma unroll
+      for (int j = 0; j < N1; ++j) acc += a.M[i][j] * v[j];
+      const int idx = off + line_index<N1, 2>(p, q, i);
+      if (ACCUM) yc[idx] += acc;
+      else yc[idx] = acc;
+    }
+  }
+}
+
+#define PDELAB_LDS_AS __attribute__((address_space(3)))
+
+// Copies nbytes contiguous bytes from global memory to LDS with the full waves of the
+// workgroup through direct global->LDS loads (VB bytes per lane, no VGPR round trip).
+template <int VB>
+__device__ __forceinline__ void dma_to_lds(const char* __restrict__ src, char* dst, int nbytes, int wave, int nwaves,
+                                           int lane) {
+  for (int ch = wave; ch * 64 * VB < nbytes; ch += nwaves) {
+    const int byte = ch * 64 * VB + lane * VB;
+    if (byte < nbytes) {
+      if constexpr (VB == 16)
+        __builtin_amdgcn_global_load_lds((const void*)(src + byte), (PDELAB_LDS_AS void*)(dst + ch * 64 * VB), 16, 0, 0);
+      else
+        __builtin_amdgcn_global_load_lds((const void*)(src + byte), (PDELAB_LDS_AS void*)(dst + ch * 64 * VB), 4, 0, 0);
+    }
+  }
+}
+
+// y (+)= J z (or J^T z) on grids with nx % CPB == 0: the CPB cells of a workgroup are
+// one aligned chunk of an x-row, and consecutive logical blocks walk z fastest, then y,
+// then the x-chunk, so the y- and z-neighbour cells were fetched moments earlier by
+// workgroups on the same XCD (L2 hits). The chunk with its two end cells and the
+// chunks of the rows y-1 and y+1 are contiguous in memory; they are copied into LDS
+// with direct global->LDS loads, the z-neighbour lines are read coalesced into
+// registers, and all x- and y-lines are then read from LDS.
+template <int N1, typename T, bool TRANS, bool CONV, bool REACT, bool VARK, bool ACCUM>
+__global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_row_kernel(const T* __restrict__ z,
+                                                                               T* __restrict__ y,
+                                                                               const T* __restrict__ kappa,
+                                                                               const DGArgs<T> a, long long nblocks) {
+  using SH = ApplyShape<N1>;
+  constexpr int L = SH::L, NL = SH::NL, CPB = SH::CPB, THREADS = SH::THREADS;
+  constexpr int CB = NL * (int)sizeof(T);  // bytes per cell
+  constexpr int VB = (CB % 16 == 0) ? 16 : 4;
+  constexpr int NW = THREADS / 64;  // full waves
+  __shared__ __attribute__((aligned(16))) T OWN[(CPB + 2) * NL];  // cell c0-1, the chunk, cell c0+CPB
+  __shared__ __attribute__((aligned(16))) T YM[CPB * NL];         // row y-1, later z sums / x sweep
+  __shared__ __attribute__((aligned(16))) T YP[CPB * NL];         // row y+1, later x sums
+  const int t = threadIdx.x;
+  const int lc = t / L, l = t % L, p = l % N1, q = l / N1;
+  const int lane = t & 63, wave = t >> 6;
+  const long long blk = xcd_logical_block(blockIdx.x, nblocks);
+  const unsigned nzu = (unsigned)a.n[2], nyu = (unsigned)a.n[1];
+  const unsigned ub = (unsigned)blk;
+  const unsigned kz = ub % nzu, rest = ub / nzu;
+  const unsigned jy = rest % nyu, xc = rest / nyu;
+  const long long c0 = (long long)xc * CPB + a.stride[1] * jy + a.stride[2] * kz;
+  const bool active = lc < CPB;
+  const int gx = (int)(xc * CPB) + lc;
+  const bool xlo_ok = xc > 0, xhi_ok = (long long)(xc + 1) * CPB < a.n[0];
+  const bool ylo = jy > 0, yhi = (int)jy < a.n[1] - 1, zlo = kz > 0, zhi = (int)kz < a.n[2] - 1;
+  const T* zc = z + c0 * NL;
+  if (wave < NW) {
+    dma_to_lds<VB>((const char*)zc, (char*)(OWN + NL), CPB * CB, wave, NW, lane);
+    if (ylo) dma_to_lds<VB>((const char*)(zc - a.stride[1] * NL), (char*)YM, CPB * CB, (wave + 1) % NW, NW, lane);
+    if (yhi) dma_to_lds<VB>((const char*)(zc + a.stride[1] * NL), (char*)YP, CPB * CB, (wave + 2) % NW, NW, lane);
+    if (xlo_ok) dma_to_lds<VB>((const char*)(zc - NL), (char*)OWN, CB, (wave + 3) % NW, NW, lane);
+    if (xhi_ok) dma_to_lds<VB>((const char*)(zc + CPB * NL), (char*)(OWN + (CPB + 1) * NL), CB, wave, NW, lane);
+  }
+  const int off = lc * NL;
+  T zl[N1], zr[N1];
+  T kc = a.kappa0, klo[3], khi[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) klo[d] = khi[d] = a.kappa0;
+  const bool lox = active && gx > 0, hix = active && gx < a.n[0] - 1;
+  if (active) {
+    if (zlo) load_line<N1, 2, T>(zc - a.stride[2] * NL, off, p, q, zl);
+    if (zhi) load_line<N1, 2, T>(zc + a.stride[2] * NL, off, p, q, zr);
+    if (VARK) {
+      const T* kb = kappa + c0;
+      kc = kb[lc];
+      if (lox) klo[0] = kb[lc - 1];
+      if (hix) khi[0] = kb[lc + 1];
+      if (ylo) klo[1] = (kb - a.stride[1])[lc];
+      if (yhi) khi[1] = (kb + a.stride[1])[lc];
+      if (zlo) klo[2] = (kb - a.stride[2])[lc];
+      if (zhi) khi[2] = (kb + a.stride[2])[lc];
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  const T* Ol = OWN + (lc + 1) * NL;
+  T sx[N1], sy[N1], sz[N1];
+  if (active) {
+    T u[N1], ul[N1], ur[N1];
+#pragma unroll
+    for (int e = 0; e < N1; ++e) u[e] = Ol[line_index<N1, 2>(p, q, e)];
+    line_op<N1, 2, T, TRANS, CONV, REACT, VARK>(a, u, zl, zr, zlo, zhi, kc, klo[2], khi[2], sz);
+#pragma unroll
+    for (int e = 0; e < N1; ++e) {
+      const int sl = line_index<N1, 1>(p, q, e);
+      u[e] = Ol[sl];
+      ul[e] = ylo ? YM[off + sl] : T(0);
+      ur[e] = yhi ? YP[off + sl] : T(0);
+    }
+    line_op<N1, 1, T, TRANS, CONV, REACT, VARK>(a, u, ul, ur, ylo, yhi, kc, klo[1], khi[1], sy);
+#pragma unroll
+    for (int e = 0; e < N1; ++e) {
+      const int sl = line_index<N1, 0>(p, q, e);
+      u[e] = Ol[sl];
+      ul[e] = lox ? Ol[sl - NL] : T(0);
+      ur[e] = hix ? Ol[sl + NL] : T(0);
+    }
+    line_op<N1, 0, T, TRANS, CONV, REACT, VARK>(a, u, ul, ur, lox, hix, kc, klo[0], khi[0], sx);
+  }
+  __syncthreads();  // all reads of the staged cells done
+  T* YMl = YM + off;
+  T* YPl = YP + off;
+  T* Ow = OWN + (lc + 1) * NL;
+  if (active) {
+#pragma unroll
+    for (int e = 0; e < N1; ++e) {
+      YMl[line_index<N1, 2>(p, q, e)] = sz[e];
+      YPl[line_index<N1, 0>(p, q, e)] = sx[e];
+    }
+  }
+  __syncthreads();
+  if (active) {
+    T v[N1];
+#pragma unroll
+    for (int e = 0; e < N1; ++e) {
+      const int sl = line_index<N1, 1>(p, q, e);
+      v[e] = sy[e] + YMl[sl] + YPl[sl];
+    }
+#pragma unroll
+    for (int i = 0; i < N1; ++i) {
+      T acc = T(0);
+#pragma unroll
+      for (int j = 0; j < N1; ++j) acc += a.M[i][j] * v[j];
+      Ow[line_index<N1, 1>(p, q, i)] = acc;
+    }
+  }
+  __syncthreads();
+  if (active) {
+    T v[N1];
+#pragma unroll
+    for (int e = 0; e < N1; ++e) v[e] = Ow[line_index<N1, 0>(p, q, e)];
+#pragma unroll
+    for (int i = 0; i < N1; ++i) {
+      T acc = T(0);
+#pragma unroll
+      for (int j = 0; j < N1; ++j) acc += a.M[i][j] * v[j];
+      YMl[line_index<N1, 0>(p, q, i)] = acc;
+    }
+  }
+  __syncthreads();
+  if (active) {
+    T v[N1];
+#pragma unroll
+    for (int e = 0; e < N1; ++e) v[e] = YMl[line_index<N1, 2>(p, q, e)];
     T* yc = y + c0 * NL;
 #pragma unroll
     for (int i = 0; i < N1; ++i) {

@@ -228,17 +228,17 @@ static int launch_apply_t(H* h, const T* z, T* y, const T* kappa, const DGArgs<T
   const long long nb = (h->ncells + SH::CPB - 1) / SH::CPB;
   if (a.rowmode) {
     if (react)
-      hipLaunchKernelGGL((dg_apply_kernel<N1, T, TRANS, CONV, true, VARK, ACCUM, true>), dim3((unsigned)nb),
+      hipLaunchKernelGGL((dg_apply_row_kernel<N1, T, TRANS, CONV, true, VARK, ACCUM>), dim3((unsigned)nb),
                          dim3(SH::THREADS), 0, h->stream, z, y, kappa, a, nb);
     else
-      hipLaunchKernelGGL((dg_apply_kernel<N1, T, TRANS, CONV, false, VARK, ACCUM, true>), dim3((unsigned)nb),
+      hipLaunchKernelGGL((dg_apply_row_kernel<N1, T, TRANS, CONV, false, VARK, ACCUM>), dim3((unsigned)nb),
                          dim3(SH::THREADS), 0, h->stream, z, y, kappa, a, nb);
   } else {
     if (react)
-      hipLaunchKernelGGL((dg_apply_kernel<N1, T, TRANS, CONV, true, VARK, ACCUM, false>), dim3((unsigned)nb),
+      hipLaunchKernelGGL((dg_apply_kernel<N1, T, TRANS, CONV, true, VARK, ACCUM>), dim3((unsigned)nb),
                          dim3(SH::THREADS), 0, h->stream, z, y, kappa, a, nb);
     else
-      hipLaunchKernelGGL((dg_apply_kernel<N1, T, TRANS, CONV, false, VARK, ACCUM, false>), dim3((unsigned)nb),
+      hipLaunchKernelGGL((dg_apply_kernel<N1, T, TRANS, CONV, false, VARK, ACCUM>), dim3((unsigned)nb),
                          dim3(SH::THREADS), 0, h->stream, z, y, kappa, a, nb);
   }
   HIPCHK(h, hipGetLastError());

@@ -37,6 +37,13 @@ CASES = [
     ("conv_q3_var", dict(nx=3, ny=3, nz=3, degree=3, alpha=3.0, b=(-0.3, 0.0, 2.0)), 7),
     ("outflow_q2", dict(nx=3, ny=2, nz=3, degree=2, alpha=3.0, b=(1.0, 0.0, 0.0), bctype=(0, -2, 0, -1, -3, 0)), None),
     ("single_cell_q3", dict(nx=1, ny=1, nz=1, degree=3, alpha=3.0), None),
+    # nx a multiple of the workgroup chunk (16 cells Q3, 32 Q2, 64 Q1): aligned-row kernel
+    ("row_q3_32x3x3_var_conv", dict(nx=32, ny=3, nz=3, degree=3, alpha=3.0, b=(0.5, -1.0, 0.25)), 12),
+    ("row_q3_16x2x4", dict(nx=16, ny=2, nz=4, degree=3, alpha=3.0, c=0.3), None),
+    ("row_q2_32x3x2_var", dict(nx=32, ny=3, nz=2, degree=2, alpha=3.0), 13),
+    ("row_q2_64x2x2_nipg_conv", dict(nx=64, ny=2, nz=2, degree=2, method=0, alpha=2.0, b=(-1.0, 0.5, 0.0)), None),
+    ("row_q1_64x3x2_var", dict(nx=64, ny=3, nz=2, degree=1, alpha=3.0, b=(0.0, 0.0, 1.0)), 14),
+    ("row_q1_128x2x1_off", dict(nx=128, ny=2, nz=1, degree=1, weights=1, alpha=3.0), None),
     ("line_x_q2", dict(nx=7, ny=1, nz=1, degree=2, alpha=3.0), None),
 ]
 
