@@ -63,6 +63,7 @@ def lib():
     L.pdelab_gpu_is_linear.argtypes = [_vp]
     L.pdelab_gpu_quadrature_points.argtypes = [_vp]
     L.pdelab_gpu_set_kappa.argtypes = [_vp, _dp]
+    L.pdelab_gpu_set_log_kappa_device.argtypes = [_vp, _vp]
     L.pdelab_gpu_set_cell_source.argtypes = [_vp, _dp]
     L.pdelab_gpu_set_tabulated_data.argtypes = [_vp, _dp, _dp]
     L.pdelab_gpu_malloc.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.c_size_t]
@@ -201,6 +202,9 @@ class GridOperator:
     def set_kappa(self, kappa):
         k = np.ascontiguousarray(kappa, dtype=np.float64)
         self._check(lib().pdelab_gpu_set_kappa(self.h, k.ctypes.data_as(_dp)))
+
+    def set_log_kappa_device(self, logk):
+        self._check(lib().pdelab_gpu_set_log_kappa_device(self.h, logk.p))
 
     def set_cell_source(self, s):
         if s is None:

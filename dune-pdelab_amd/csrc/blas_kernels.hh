@@ -100,6 +100,16 @@ __global__ __launch_bounds__(256) void sub_f32_kernel(long long n, const float* 
     x[i] -= (double)z[i];
 }
 
+// kappa = exp(log kappa) in FP64 and FP32
+__global__ __launch_bounds__(256) void exp_kernel(long long n, const double* __restrict__ logk, double* __restrict__ k64,
+                                                  float* __restrict__ k32) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const double v = exp(logk[i]);
+    k64[i] = v;
+    k32[i] = (float)v;
+  }
+}
+
 // dinv = 1 / d
 template <typename T>
 __global__ __launch_bounds__(256) void invert_kernel(long long n, const double* __restrict__ d, T* __restrict__ dinv) {
@@ -130,7 +140,9 @@ template <typename T, typename A>
 __global__ __launch_bounds__(kRedThreads) void cg_update_kernel(long long n, T* __restrict__ x, const T* __restrict__ p,
                                                                 T* __restrict__ r, const T* __restrict__ q,
                                                                 const T* __restrict__ dinv, const A* __restrict__ scal,
-                                                                int slot_rho, int slot_alpha, A* __restrict__ partial) {
+                                                                int slot_rho, int slot_alpha, A* __restrict__ partial,
+                                                                const int* __restrict__ stop) {
+  if (stop && *stop) return;  // the device-side loop has stopped
   const T lambda = (T)(scal[slot_rho] / scal[slot_alpha]);
   A v[2] = {A(0), A(0)};
   for (long long i = (long long)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (long long)kRedBlocks * kRedThreads) {
@@ -143,6 +155,52 @@ __global__ __launch_bounds__(kRedThreads) void cg_update_kernel(long long n, T* 
   }
   block_sum<A, 2>(v);
   if (threadIdx.x == 0) { partial[2 * blockIdx.x] = v[0]; partial[2 * blockIdx.x + 1] = v[1]; }
+}
+
+// dune-istl IterativeSolver::Iteration::step on the device (single workgroup): reduces the
+// partials of (r.r, z.r) into out[0..1], records the defect and the iteration number of a
+// running loop and raises the stop flag on convergence (def < reduction*def0 or
+// def < 1e-30) or on a non-finite defect. state = {stopped, aborted, iteration};
+// dstate = {defect, def0, reduction}.
+template <typename A>
+__global__ __launch_bounds__(kRedThreads) void cg_check_kernel(const A* __restrict__ partial, int nb,
+                                                               A* __restrict__ out, int* __restrict__ state,
+                                                               double* __restrict__ dstate, int it) {
+  A v[2] = {A(0), A(0)};
+  for (int b = threadIdx.x; b < nb; b += kRedThreads) {
+    v[0] += partial[2 * b];
+    v[1] += partial[2 * b + 1];
+  }
+  block_sum<A, 2>(v);
+  if (threadIdx.x == 0) {
+    out[0] = v[0];
+    out[1] = v[1];
+    if (state[0] == 0) {
+      const double def = sqrt((double)v[0]);
+      state[2] = it;
+      dstate[0] = def;
+      if (!isfinite(def)) {
+        state[1] = 1;
+        state[0] = 1;
+      } else if (def < dstate[1] * dstate[2] || def < 1e-30) {
+        state[0] = 1;
+      }
+    }
+  }
+}
+
+// CG direction update with beta = scal[rnew] / scal[rold] from the device; skipped once
+// the device-side loop has stopped
+template <typename T, typename A>
+__global__ __launch_bounds__(256) void cg_pupdate_dev_kernel(long long n, T* __restrict__ p, const T* __restrict__ r,
+                                                             const T* __restrict__ dinv, const A* __restrict__ scal,
+                                                             int rnew, int rold, const int* __restrict__ stop) {
+  if (*stop) return;
+  const T beta = (T)(scal[rnew] / scal[rold]);
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const T zi = dinv ? dinv[i] * r[i] : r[i];
+    p[i] = zi + beta * p[i];
+  }
 }
 
 // CG direction update: p = dinv r + beta p

@@ -197,6 +197,27 @@ __device__ __forceinline__ void line_op(const DGArgs<T>& a, const T (&u)[N1], co
   }
 }
 
+
+// Deterministic sum over the workgroup of one value per thread (any size <= 512);
+// the result is valid in thread 0.
+template <int THREADS, typename T>
+__device__ __forceinline__ T block_sum_any(T v) {
+  constexpr int R = (THREADS + 63) / 64 * 64;
+  __shared__ T red[R];
+  const int t = threadIdx.x;
+  red[t] = v;
+  if (t < R - THREADS) red[THREADS + t] = T(0);
+  __syncthreads();
+  T s = T(0);
+  if (t < 64) {
+#pragma unroll
+    for (int j = 0; j < R / 64; ++j) s += red[t + 64 * j];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  }
+  return s;
+}
+
 template <int N1, int D, typename T>
 __device__ __forceinline__ void load_line(const T* __restrict__ base, int off, int p, int q, T (&u)[N1]) {
 #pragma unroll
@@ -212,11 +233,14 @@ __device__ __forceinline__ void load_line(const T* __restrict__ base, int off, i
 // the own cells, the cells of the rows y-1 and y+1 (same x positions) and the two cells
 // beyond the ends of the chunk are staged there; x-neighbours inside the chunk are the
 // staged own cells. All global loads are issued before the first wait.
-template <int N1, typename T, bool TRANS, bool CONV, bool REACT, bool VARK, bool ACCUM>
+template <int N1, typename T, bool TRANS, bool CONV, bool REACT, bool VARK, bool ACCUM, bool DOT>
 __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const T* __restrict__ z,
                                                                            T* __restrict__ y,
                                                                            const T* __restrict__ kappa,
-                                                                           const DGArgs<T> a, long long nblocks) {
+                                                                           const DGArgs<T> a, long long nblocks,
+                                                                           T* __restrict__ dotp,
+                                                                           const int* __restrict__ stop) {
+  if (stop && *stop) return;  // a device-side Krylov loop has already stopped
   using SH = ApplyShape<N1>;
   constexpr int L = SH::L, NL = SH::NL, NLP = SH::NLP, CPB = SH::CPB;
   __shared__ T U[(CPB + 2) * NLP];  // [0] cell before the chunk, [1..CPB] own cells, [CPB+1] cell after
@@ -307,6 +331,8 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
     line_op<N1, 0, T, TRANS, CONV, REACT, VARK>(a, u, ul, ur, lo[0], hi[0], kc, klo[0], khi[0], sx);
   }
   __syncthreads();  // all reads of the staged cells done
+  T pq = T(0);
+  const T* PZ = uz;
   if (active) {
 #pragma unroll
     for (int e = 0; e < N1; ++e) {
@@ -358,7 +384,13 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
       const int idx = off + line_index<N1, 2>(p, q, i);
       if (ACCUM) yc[idx] += acc;
       else yc[idx] = acc;
+      if (DOT) pq += PZ[i] * acc;
     }
+  }
+  if (DOT) {
+    // fused z . (J z) for the Krylov step length (one partial per workgroup)
+    const T bs = block_sum_any<SH::THREADS, T>(pq);
+    if (threadIdx.x == 0) dotp[blockIdx.x] = bs;
   }
 }
 
@@ -387,11 +419,14 @@ __device__ __forceinline__ void dma_to_lds(const char* __restrict__ src, char* d
 // chunks of the rows y-1 and y+1 are contiguous in memory; they are copied into LDS
 // with direct global->LDS loads, the z-neighbour lines are read coalesced into
 // registers, and all x- and y-lines are then read from LDS.
-template <int N1, typename T, bool TRANS, bool CONV, bool REACT, bool VARK, bool ACCUM>
+template <int N1, typename T, bool TRANS, bool CONV, bool REACT, bool VARK, bool ACCUM, bool DOT>
 __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_row_kernel(const T* __restrict__ z,
                                                                                T* __restrict__ y,
                                                                                const T* __restrict__ kappa,
-                                                                               const DGArgs<T> a, long long nblocks) {
+                                                                               const DGArgs<T> a, long long nblocks,
+                                                                               T* __restrict__ dotp,
+                                                                               const int* __restrict__ stop) {
+  if (stop && *stop) return;  // a device-side Krylov loop has already stopped
   using SH = ApplyShape<N1>;
   constexpr int L = SH::L, NL = SH::NL, CPB = SH::CPB, THREADS = SH::THREADS;
   constexpr int CB = NL * (int)sizeof(T);  // bytes per cell
@@ -444,11 +479,11 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_row_kernel(c
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   const T* Ol = OWN + (lc + 1) * NL;
-  T sx[N1], sy[N1], sz[N1];
+  T sx[N1], sy[N1], sz[N1], PZ[N1];
   if (active) {
     T u[N1], ul[N1], ur[N1];
 #pragma unroll
-    for (int e = 0; e < N1; ++e) u[e] = Ol[line_index<N1, 2>(p, q, e)];
+    for (int e = 0; e < N1; ++e) u[e] = PZ[e] = Ol[line_index<N1, 2>(p, q, e)];
     line_op<N1, 2, T, TRANS, CONV, REACT, VARK>(a, u, zl, zr, zlo, zhi, kc, klo[2], khi[2], sz);
 #pragma unroll
     for (int e = 0; e < N1; ++e) {
@@ -468,6 +503,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_row_kernel(c
     line_op<N1, 0, T, TRANS, CONV, REACT, VARK>(a, u, ul, ur, lox, hix, kc, klo[0], khi[0], sx);
   }
   __syncthreads();  // all reads of the staged cells done
+  T pq = T(0);
   T* YMl = YM + off;
   T* YPl = YP + off;
   T* Ow = OWN + (lc + 1) * NL;
@@ -521,7 +557,13 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_row_kernel(c
       const int idx = off + line_index<N1, 2>(p, q, i);
       if (ACCUM) yc[idx] += acc;
       else yc[idx] = acc;
+      if (DOT) pq += PZ[i] * acc;
     }
+  }
+  if (DOT) {
+    // fused z . (J z) for the Krylov step length (one partial per workgroup)
+    const T bs = block_sum_any<SH::THREADS, T>(pq);
+    if (threadIdx.x == 0) dotp[blockIdx.x] = bs;
   }
 }
 

@@ -61,6 +61,11 @@ struct pdelab_gpu_gridoperator {
   double* h_scal64 = nullptr;
   float* h_scal32 = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double* d_applydot = nullptr;  // per-workgroup partials of the fused apply dot
+  int* d_state = nullptr;        // device Krylov loop: {stopped, aborted, iteration, -}
+  double* d_dstate = nullptr;    // {defect, def0, reduction, -}
+  int* h_state = nullptr;        // pinned mirrors
+  double* h_dstate = nullptr;
 };
 
 using H = pdelab_gpu_gridoperator;
@@ -222,49 +227,56 @@ static void fill_load_args(H* h) {
 
 // ---------------------------------------------------------------------------
 // kernel dispatch
-template <int N1, typename T, bool TRANS, bool CONV, bool VARK, bool ACCUM>
-static int launch_apply_t(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, bool react) {
+template <int N1, typename T, bool TRANS, bool CONV, bool VARK, bool ACCUM, bool DOT>
+static int launch_apply_t(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, bool react, T* dotp,
+                          const int* stop) {
   using SH = ApplyShape<N1>;
   const long long nb = (h->ncells + SH::CPB - 1) / SH::CPB;
   if (a.rowmode) {
     if (react)
-      hipLaunchKernelGGL((dg_apply_row_kernel<N1, T, TRANS, CONV, true, VARK, ACCUM>), dim3((unsigned)nb),
-                         dim3(SH::THREADS), 0, h->stream, z, y, kappa, a, nb);
+      hipLaunchKernelGGL((dg_apply_row_kernel<N1, T, TRANS, CONV, true, VARK, ACCUM, DOT>), dim3((unsigned)nb),
+                         dim3(SH::THREADS), 0, h->stream, z, y, kappa, a, nb, dotp, stop);
     else
-      hipLaunchKernelGGL((dg_apply_row_kernel<N1, T, TRANS, CONV, false, VARK, ACCUM>), dim3((unsigned)nb),
-                         dim3(SH::THREADS), 0, h->stream, z, y, kappa, a, nb);
+      hipLaunchKernelGGL((dg_apply_row_kernel<N1, T, TRANS, CONV, false, VARK, ACCUM, DOT>), dim3((unsigned)nb),
+                         dim3(SH::THREADS), 0, h->stream, z, y, kappa, a, nb, dotp, stop);
   } else {
     if (react)
-      hipLaunchKernelGGL((dg_apply_kernel<N1, T, TRANS, CONV, true, VARK, ACCUM>), dim3((unsigned)nb),
-                         dim3(SH::THREADS), 0, h->stream, z, y, kappa, a, nb);
+      hipLaunchKernelGGL((dg_apply_kernel<N1, T, TRANS, CONV, true, VARK, ACCUM, DOT>), dim3((unsigned)nb),
+                         dim3(SH::THREADS), 0, h->stream, z, y, kappa, a, nb, dotp, stop);
     else
-      hipLaunchKernelGGL((dg_apply_kernel<N1, T, TRANS, CONV, false, VARK, ACCUM>), dim3((unsigned)nb),
-                         dim3(SH::THREADS), 0, h->stream, z, y, kappa, a, nb);
+      hipLaunchKernelGGL((dg_apply_kernel<N1, T, TRANS, CONV, false, VARK, ACCUM, DOT>), dim3((unsigned)nb),
+                         dim3(SH::THREADS), 0, h->stream, z, y, kappa, a, nb, dotp, stop);
   }
   HIPCHK(h, hipGetLastError());
   return PDELAB_OK;
 }
 
+// mode 0: y = J z; 1: y += J z; 2: y = J z with the fused z.(Jz) partials
 template <int N1, typename T, bool TRANS, bool CONV, bool VARK>
-static int launch_apply_a(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, bool react, bool accum) {
-  return accum ? launch_apply_t<N1, T, TRANS, CONV, VARK, true>(h, z, y, kappa, a, react)
-               : launch_apply_t<N1, T, TRANS, CONV, VARK, false>(h, z, y, kappa, a, react);
+static int launch_apply_a(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, bool react, int mode, T* dotp,
+                          const int* stop) {
+  if (mode == 1) return launch_apply_t<N1, T, TRANS, CONV, VARK, true, false>(h, z, y, kappa, a, react, nullptr, stop);
+  if (mode == 2) return launch_apply_t<N1, T, TRANS, CONV, VARK, false, true>(h, z, y, kappa, a, react, dotp, stop);
+  return launch_apply_t<N1, T, TRANS, CONV, VARK, false, false>(h, z, y, kappa, a, react, nullptr, stop);
 }
 template <int N1, typename T, bool TRANS, bool CONV>
-static int launch_apply_v(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, bool react, bool accum) {
-  return h->vark ? launch_apply_a<N1, T, TRANS, CONV, true>(h, z, y, kappa, a, react, accum)
-                 : launch_apply_a<N1, T, TRANS, CONV, false>(h, z, y, kappa, a, react, accum);
+static int launch_apply_v(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, bool react, int mode, T* dotp,
+                          const int* stop) {
+  return h->vark ? launch_apply_a<N1, T, TRANS, CONV, true>(h, z, y, kappa, a, react, mode, dotp, stop)
+                 : launch_apply_a<N1, T, TRANS, CONV, false>(h, z, y, kappa, a, react, mode, dotp, stop);
 }
 template <int N1, typename T, bool TRANS>
-static int launch_apply_c(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, bool react, bool accum) {
-  return h->conv ? launch_apply_v<N1, T, TRANS, true>(h, z, y, kappa, a, react, accum)
-                 : launch_apply_v<N1, T, TRANS, false>(h, z, y, kappa, a, react, accum);
+static int launch_apply_c(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, bool react, int mode, T* dotp,
+                          const int* stop) {
+  return h->conv ? launch_apply_v<N1, T, TRANS, true>(h, z, y, kappa, a, react, mode, dotp, stop)
+                 : launch_apply_v<N1, T, TRANS, false>(h, z, y, kappa, a, react, mode, dotp, stop);
 }
 template <int N1, typename T>
-static int launch_apply_n(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, bool trans, bool accum) {
+static int launch_apply_n(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, bool trans, int mode, T* dotp,
+                          const int* stop) {
   const bool react = h->cfg.c != 0.0;
-  return trans ? launch_apply_c<N1, T, true>(h, z, y, kappa, a, react, accum)
-               : launch_apply_c<N1, T, false>(h, z, y, kappa, a, react, accum);
+  return trans ? launch_apply_c<N1, T, true>(h, z, y, kappa, a, react, mode, dotp, stop)
+               : launch_apply_c<N1, T, false>(h, z, y, kappa, a, react, mode, dotp, stop);
 }
 
 static int check_outflow(H* h) {
@@ -277,30 +289,44 @@ static int check_outflow(H* h) {
   return PDELAB_OK;
 }
 
-static int apply64(H* h, const double* z, double* y, bool trans, bool accum) {
+template <typename T>
+static const T* kappa_for(H* h);
+template <>
+const double* kappa_for<double>(H* h) { return h->d_kappa; }
+template <>
+const float* kappa_for<float>(H* h) { return h->d_kappa32; }
+template <typename T>
+static const DGArgs<T>& args_for(H* h);
+template <>
+const DGArgs<double>& args_for<double>(H* h) { return h->a64; }
+template <>
+const DGArgs<float>& args_for<float>(H* h) { return h->a32; }
+
+template <typename T>
+static int apply_mode(H* h, const T* z, T* y, bool trans, int mode, T* dotp = nullptr, const int* stop = nullptr) {
   CHK(check_outflow(h));
   switch (h->n1) {
-    case 2: return launch_apply_n<2, double>(h, z, y, h->d_kappa, h->a64, trans, accum);
-    case 3: return launch_apply_n<3, double>(h, z, y, h->d_kappa, h->a64, trans, accum);
-    case 4: return launch_apply_n<4, double>(h, z, y, h->d_kappa, h->a64, trans, accum);
+    case 2: return launch_apply_n<2, T>(h, z, y, kappa_for<T>(h), args_for<T>(h), trans, mode, dotp, stop);
+    case 3: return launch_apply_n<3, T>(h, z, y, kappa_for<T>(h), args_for<T>(h), trans, mode, dotp, stop);
+    case 4: return launch_apply_n<4, T>(h, z, y, kappa_for<T>(h), args_for<T>(h), trans, mode, dotp, stop);
   }
   return fail(h, PDELAB_ERR_UNSUPPORTED, "degree");
+}
+static int apply64(H* h, const double* z, double* y, bool trans, bool accum) {
+  return apply_mode<double>(h, z, y, trans, accum ? 1 : 0);
 }
 static int apply32(H* h, const float* z, float* y, bool trans, bool accum) {
-  CHK(check_outflow(h));
-  switch (h->n1) {
-    case 2: return launch_apply_n<2, float>(h, z, y, h->d_kappa32, h->a32, trans, accum);
-    case 3: return launch_apply_n<3, float>(h, z, y, h->d_kappa32, h->a32, trans, accum);
-    case 4: return launch_apply_n<4, float>(h, z, y, h->d_kappa32, h->a32, trans, accum);
-  }
-  return fail(h, PDELAB_ERR_UNSUPPORTED, "degree");
+  return apply_mode<float>(h, z, y, trans, accum ? 1 : 0);
 }
 template <typename T>
-static int apply_t(H* h, const T* z, T* y, bool trans, bool accum);
-template <>
-int apply_t<double>(H* h, const double* z, double* y, bool trans, bool accum) { return apply64(h, z, y, trans, accum); }
-template <>
-int apply_t<float>(H* h, const float* z, float* y, bool trans, bool accum) { return apply32(h, z, y, trans, accum); }
+static int apply_t(H* h, const T* z, T* y, bool trans, bool accum) {
+  return apply_mode<T>(h, z, y, trans, accum ? 1 : 0);
+}
+// number of workgroups of one apply launch (= fused-dot partials)
+static long long apply_blocks(H* h) {
+  const int cpb = h->n1 == 2 ? ApplyShape<2>::CPB : (h->n1 == 3 ? ApplyShape<3>::CPB : ApplyShape<4>::CPB);
+  return (h->ncells + cpb - 1) / cpb;
+}
 
 static int compute_R0(H* h) {
   if (h->R0_valid) return PDELAB_OK;
@@ -450,6 +476,15 @@ const float* dinv_for<float>(H* h, int precond) { return precond == PDELAB_PREC_
 
 // CGSolver::apply (dune-istl): solves J z = b from the given z; b becomes the defect.
 // Workspace slots base, base+1 (p, q).
+//
+// The loop runs on the device: per iteration the apply (with the fused p.q partials),
+// the update of x and r (with the r.r and z.r partials), the Iteration::step check and the
+// direction update are enqueued without waiting; the check kernel raises a device flag
+// at the iteration where dune-istl's stopping rule fires and every later kernel of the
+// loop returns immediately, so the state is exactly the one at that iteration. The host
+// reads the flag back every kCheckEvery iterations.
+constexpr int kCheckEvery = 16;
+
 template <typename T, typename A>
 static int cg_solve(H* h, T* x, T* b, double reduction, int maxit, int precond, bool trans, int base,
                     pdelab_linear_solver_result* res) {
@@ -459,6 +494,9 @@ static int cg_solve(H* h, T* x, T* b, double reduction, int maxit, int precond, 
   T* p = ws<T>(h)[base];
   T* q = ws<T>(h)[base + 1];
   const T* dinv = dinv_for<T>(h, precond);
+  const long long nbapply = apply_blocks(h);
+  if (!h->d_applydot) HIPCHK(h, hipMalloc(&h->d_applydot, sizeof(double) * (size_t)std::max(nbapply, 1LL)));
+  T* dotp = reinterpret_cast<T*>(h->d_applydot);
   Iter it(reduction);
   // b -= J x  (applyscaleadd(-1, x, b))
   CHK(apply_t<T>(h, x, q, trans, false));
@@ -466,30 +504,57 @@ static int cg_solve(H* h, T* x, T* b, double reduction, int maxit, int precond, 
   CHK(dot_dev<T, A>(h, b, b, n, 1));
   A sc[8];
   CHK(read_scalars<A>(h, sc, 2));
-  double def = std::sqrt((double)sc[1]);
-  if (!it.step(0, def)) {
+  const double def0 = std::sqrt((double)sc[1]);
+  if (!it.step(0, def0)) {
     // p = W^{-1} b; rho = p.b  -> slot 7
     hipLaunchKernelGGL((prec_dot_kernel<T, A>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, n, dinv,
                        (const T*)b, p, partial_buf<A>(h));
     CHK(finish_reduce<A, 1>(h, 7));
-    A rho = 0;
-    for (int i = 1; i <= maxit; ++i) {
-      const int rslot = (i & 1) ? 7 : 5;   // slot holding the current rho
-      const int oslot = (i & 1) ? 4 : 6;   // (def^2, rho_new) written here
-      CHK(apply_t<T>(h, p, q, trans, false));
-      CHK(dot_dev<T, A>(h, p, q, n, 0));   // alpha -> slot 0
+    // device loop state
+    h->h_state[0] = h->h_state[1] = h->h_state[2] = h->h_state[3] = 0;
+    h->h_dstate[0] = def0;
+    h->h_dstate[1] = def0;
+    h->h_dstate[2] = reduction;
+    HIPCHK(h, hipMemcpyAsync(h->d_state, h->h_state, sizeof(int) * 4, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d_dstate, h->h_dstate, sizeof(double) * 4, hipMemcpyHostToDevice, h->stream));
+    A* scal = scal_buf<A>(h);
+    bool stopped = false;
+    int last = 0;
+    for (int i = 1; i <= maxit && !stopped; ++i) {
+      const int rslot = (i & 1) ? 7 : 5;  // slot holding the current rho
+      const int oslot = (i & 1) ? 4 : 6;  // (def^2, rho_new) written here
+      CHK(apply_mode<T>(h, p, q, trans, 2, dotp, h->d_state));
+      hipLaunchKernelGGL((final_reduce_kernel<A, 1>), dim3(1), dim3(kRedThreads), 0, h->stream, (const A*)dotp,
+                         (int)nbapply, scal + 0);  // alpha = p.q
       hipLaunchKernelGGL((cg_update_kernel<T, A>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, n, x,
-                         (const T*)p, b, (const T*)q, dinv, (const A*)scal_buf<A>(h), rslot, 0, partial_buf<A>(h));
-      CHK(finish_reduce<A, 2>(h, oslot));
-      CHK(read_scalars<A>(h, sc, 8));
-      def = std::sqrt((double)sc[oslot]);
-      if (it.step(i, def)) break;
-      const A rho_new = sc[oslot + 1];
-      rho = sc[rslot];
-      const T beta = (T)(rho_new / rho);
-      hipLaunchKernelGGL(cg_pupdate_kernel<T>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, p, (const T*)b, dinv,
-                         beta);
+                         (const T*)p, b, (const T*)q, dinv, (const A*)scal, rslot, 0, partial_buf<A>(h),
+                         (const int*)h->d_state);
+      hipLaunchKernelGGL((cg_check_kernel<A>), dim3(1), dim3(kRedThreads), 0, h->stream, (const A*)partial_buf<A>(h),
+                         kRedBlocks, scal + oslot, h->d_state, h->d_dstate, i);
+      hipLaunchKernelGGL((cg_pupdate_dev_kernel<T, A>), dim3(grid_for(n)), dim3(256), 0, h->stream, n, p,
+                         (const T*)b, dinv, (const A*)scal, oslot + 1, rslot, (const int*)h->d_state);
       HIPCHK(h, hipGetLastError());
+      last = i;
+      if (i % kCheckEvery == 0 || i == maxit) {
+        HIPCHK(h, hipMemcpyAsync(h->h_state, h->d_state, sizeof(int) * 4, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        stopped = h->h_state[0] != 0;
+      }
+    }
+    HIPCHK(h, hipMemcpyAsync(h->h_state, h->d_state, sizeof(int) * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->h_dstate, h->d_dstate, sizeof(double) * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const int iters = h->h_state[0] ? h->h_state[2] : last;
+    const double def = h->h_dstate[0];
+    if (h->h_state[1]) {
+      it.aborted = true;
+      it.def = def;
+      it.i = iters;
+      it.converged = false;
+    } else {
+      it.def = def;
+      it.i = iters;
+      it.converged = (def < it.def0 * it.red) || (def < 1e-30);
     }
   }
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -687,6 +752,10 @@ int pdelab_gpu_create(const pdelab_config* cfg, const double* kappa_host, pdelab
   if (hipHostMalloc(&h->h_scal64, sizeof(double) * 16) != hipSuccess) return bail("hipHostMalloc failed");
   if (hipHostMalloc(&h->h_scal32, sizeof(float) * 16) != hipSuccess) return bail("hipHostMalloc failed");
   if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) return bail("hipEventCreate failed");
+  if (hipMalloc(&h->d_state, sizeof(int) * 4) != hipSuccess) return bail("hipMalloc failed");
+  if (hipMalloc(&h->d_dstate, sizeof(double) * 4) != hipSuccess) return bail("hipMalloc failed");
+  if (hipHostMalloc(&h->h_state, sizeof(int) * 4) != hipSuccess) return bail("hipHostMalloc failed");
+  if (hipHostMalloc(&h->h_dstate, sizeof(double) * 4) != hipSuccess) return bail("hipHostMalloc failed");
   if (cfg->kappa_per_cell) {
     if (!kappa_host) { g_create_error = "kappa_per_cell requires a kappa array"; pdelab_gpu_destroy(h); return PDELAB_ERR_INVALID_ARG; }
     if (upload_kappa(h, kappa_host) != PDELAB_OK) {
@@ -703,7 +772,8 @@ int pdelab_gpu_destroy(pdelab_gpu_gridoperator* h) {
   if (!h) return PDELAB_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   void* ptrs[] = {h->d_kappa, h->d_kappa32, h->d_R0, h->d_src, h->d_ftab, h->d_gtab, h->d_diag, h->d_dinv,
-                  h->d_dinv32, h->d_part64, h->d_part32, h->d_scal64, h->d_scal32};
+                  h->d_dinv32, h->d_part64, h->d_part32, h->d_scal64, h->d_scal32, h->d_applydot,
+                  h->d_state, h->d_dstate};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int i = 0; i < 8; ++i) {
@@ -712,6 +782,8 @@ int pdelab_gpu_destroy(pdelab_gpu_gridoperator* h) {
   }
   if (h->h_scal64) (void)hipHostFree(h->h_scal64);
   if (h->h_scal32) (void)hipHostFree(h->h_scal32);
+  if (h->h_state) (void)hipHostFree(h->h_state);
+  if (h->h_dstate) (void)hipHostFree(h->h_dstate);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
   if (h->ev1) (void)hipEventDestroy(h->ev1);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -730,6 +802,20 @@ int pdelab_gpu_quadrature_points(const pdelab_gpu_gridoperator* h) { return h ? 
 int pdelab_gpu_set_kappa(pdelab_gpu_gridoperator* h, const double* kappa_host) {
   if (!h || !kappa_host) return PDELAB_ERR_INVALID_ARG;
   return upload_kappa(h, kappa_host);
+}
+
+int pdelab_gpu_set_log_kappa_device(pdelab_gpu_gridoperator* h, const double* logk) {
+  if (!h || !logk) return PDELAB_ERR_INVALID_ARG;
+  const long long nc = h->ncells;
+  if (!h->d_kappa) HIPCHK(h, hipMalloc(&h->d_kappa, sizeof(double) * nc));
+  if (!h->d_kappa32) HIPCHK(h, hipMalloc(&h->d_kappa32, sizeof(float) * nc));
+  hipLaunchKernelGGL(exp_kernel, dim3(grid_for(nc)), dim3(256), 0, h->stream, nc, logk, h->d_kappa, h->d_kappa32);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->vark = true;
+  h->R0_valid = false;
+  h->diag_valid = false;
+  return PDELAB_OK;
 }
 
 int pdelab_gpu_set_cell_source(pdelab_gpu_gridoperator* h, const double* src) {

@@ -174,6 +174,8 @@ size_t pdelab_gpu_num_cells(const pdelab_gpu_gridoperator* h);
 int pdelab_gpu_is_linear(const pdelab_gpu_gridoperator* h);
 /* Replaces the per-cell kappa (host array of nx*ny*nz doubles). */
 int pdelab_gpu_set_kappa(pdelab_gpu_gridoperator* h, const double* kappa_host);
+/* Same from a device array of log(kappa) (kappa = exp(log_kappa), computed on the device). */
+int pdelab_gpu_set_log_kappa_device(pdelab_gpu_gridoperator* h, const double* log_kappa_dev);
 /* Per-cell additive source s_c (f -> f + s_c on cell c); host array or NULL to clear. */
 int pdelab_gpu_set_cell_source(pdelab_gpu_gridoperator* h, const double* source_host);
 /* PDELAB_PROBLEM_TABULATED data, host arrays:
