@@ -10,6 +10,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "block_reduce.hh"
+
 namespace pdelab_amd {
 
 constexpr int kRedBlocks = 1024;
@@ -57,7 +59,9 @@ __global__ __launch_bounds__(kRedThreads) void dot_partial_kernel(const T* __res
 // out[slot + j] = sum_b partial[b*NR + j]   (single block of kRedThreads threads)
 template <typename A, int NR>
 __global__ __launch_bounds__(kRedThreads) void final_reduce_kernel(const A* __restrict__ partial, int nb,
-                                                                   A* __restrict__ out) {
+                                                                   A* __restrict__ out,
+                                                                   const int* __restrict__ stop) {
+  if (stop && *stop) return;  // a device-side Krylov loop has already stopped
   A v[NR];
 #pragma unroll
   for (int j = 0; j < NR; ++j) v[j] = A(0);
@@ -134,72 +138,118 @@ __global__ __launch_bounds__(kRedThreads) void prec_dot_kernel(long long n, cons
   if (threadIdx.x == 0) partial[blockIdx.x] = v[0];
 }
 
-// CG step: lambda = rho/alpha (device scalars); x += lambda p; r -= lambda q;
-// partials of r.r and (dinv r).r
+// Device-side CG iteration (dune-istl CGSolver::apply, R:dune/istl/solvers.hh), five
+// launches per iteration: the apply with the fused p.q partials, their reduction into
+// scal[0], cg_rupdate_kernel, cg_check_kernel and cg_xpupdate_kernel.
+// state = {stopped, aborted, stop iteration}; dstate = {defect, def0, reduction}. Every
+// kernel of an iteration after the one that stopped the loop returns at once.
+constexpr int kVecUnroll = 4;
+
+// r -= lambda q with lambda = scal[rho] / scal[0]; partials of r.r and (dinv r).r
 template <typename T, typename A>
-__global__ __launch_bounds__(kRedThreads) void cg_update_kernel(long long n, T* __restrict__ x, const T* __restrict__ p,
-                                                                T* __restrict__ r, const T* __restrict__ q,
-                                                                const T* __restrict__ dinv, const A* __restrict__ scal,
-                                                                int slot_rho, int slot_alpha, A* __restrict__ partial,
-                                                                const int* __restrict__ stop) {
-  if (stop && *stop) return;  // the device-side loop has stopped
-  const T lambda = (T)(scal[slot_rho] / scal[slot_alpha]);
+__global__ __launch_bounds__(kRedThreads) void cg_rupdate_kernel(long long n, T* __restrict__ r, const T* __restrict__ q,
+                                                                 const T* __restrict__ dinv, const A* __restrict__ scal,
+                                                                 int slot_rho, A* __restrict__ partial,
+                                                                 const int* __restrict__ state) {
+  if (state[0]) return;
+  const T lambda = (T)(scal[slot_rho] / scal[0]);
   A v[2] = {A(0), A(0)};
-  for (long long i = (long long)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (long long)kRedBlocks * kRedThreads) {
-    x[i] += lambda * p[i];
-    const T ri = r[i] - lambda * q[i];
-    r[i] = ri;
-    const T zi = dinv ? dinv[i] * ri : ri;
-    v[0] += (A)ri * (A)ri;
-    v[1] += (A)zi * (A)ri;
+  const long long S = (long long)gridDim.x * kRedThreads;
+  for (long long i0 = (long long)blockIdx.x * kRedThreads + threadIdx.x; i0 < n; i0 += kVecUnroll * S) {
+    T rv[kVecUnroll], qv[kVecUnroll], dv[kVecUnroll];
+#pragma unroll
+    for (int u = 0; u < kVecUnroll; ++u) {
+      const long long i = i0 + u * S;
+      rv[u] = qv[u] = T(0);
+      dv[u] = T(1);
+      if (i < n) {
+        rv[u] = r[i];
+        qv[u] = q[i];
+        if (dinv) dv[u] = dinv[i];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kVecUnroll; ++u) {
+      const long long i = i0 + u * S;
+      if (i < n) {
+        const T ri = rv[u] - lambda * qv[u];
+        r[i] = ri;
+        const T zi = dv[u] * ri;
+        v[0] += (A)ri * (A)ri;
+        v[1] += (A)zi * (A)ri;
+      }
+    }
   }
-  block_sum<A, 2>(v);
-  if (threadIdx.x == 0) { partial[2 * blockIdx.x] = v[0]; partial[2 * blockIdx.x + 1] = v[1]; }
+  block_sum_n<kRedThreads, A, 2>(v);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = v[0];
+    partial[2 * blockIdx.x + 1] = v[1];
+  }
 }
 
-// dune-istl IterativeSolver::Iteration::step on the device (single workgroup): reduces the
-// partials of (r.r, z.r) into out[0..1], records the defect and the iteration number of a
-// running loop and raises the stop flag on convergence (def < reduction*def0 or
-// def < 1e-30) or on a non-finite defect. state = {stopped, aborted, iteration};
-// dstate = {defect, def0, reduction}.
+// IterativeSolver::Iteration::step on the device (single workgroup): reduces the
+// partials of (r.r, z.r) into out[0..1], records the defect and the iteration number of
+// a running loop and raises the stop flag on convergence (def < reduction*def0 or
+// def < 1e-30) or on a non-finite defect.
 template <typename A>
 __global__ __launch_bounds__(kRedThreads) void cg_check_kernel(const A* __restrict__ partial, int nb,
                                                                A* __restrict__ out, int* __restrict__ state,
                                                                double* __restrict__ dstate, int it) {
+  if (state[0]) return;
   A v[2] = {A(0), A(0)};
   for (int b = threadIdx.x; b < nb; b += kRedThreads) {
     v[0] += partial[2 * b];
     v[1] += partial[2 * b + 1];
   }
-  block_sum<A, 2>(v);
+  block_sum_n<kRedThreads, A, 2>(v);
   if (threadIdx.x == 0) {
     out[0] = v[0];
     out[1] = v[1];
-    if (state[0] == 0) {
-      const double def = sqrt((double)v[0]);
-      state[2] = it;
-      dstate[0] = def;
-      if (!isfinite(def)) {
-        state[1] = 1;
-        state[0] = 1;
-      } else if (def < dstate[1] * dstate[2] || def < 1e-30) {
-        state[0] = 1;
-      }
+    const double def = sqrt((double)v[0]);
+    state[2] = it;
+    dstate[0] = def;
+    if (!isfinite(def)) {
+      state[1] = 1;
+      state[0] = 1;
+    } else if (def < dstate[1] * dstate[2] || def < 1e-30) {
+      state[0] = 1;
     }
   }
 }
 
-// CG direction update with beta = scal[rnew] / scal[rold] from the device; skipped once
-// the device-side loop has stopped
+// x += lambda p (lambda = scal[rho_old] / scal[0]), then p = dinv r + beta p with
+// beta = scal[rho_new] / scal[rho_old]; on the iteration that stopped the loop only x
+// is updated (dune-istl leaves the loop after the defect check).
 template <typename T, typename A>
-__global__ __launch_bounds__(256) void cg_pupdate_dev_kernel(long long n, T* __restrict__ p, const T* __restrict__ r,
-                                                             const T* __restrict__ dinv, const A* __restrict__ scal,
-                                                             int rnew, int rold, const int* __restrict__ stop) {
-  if (*stop) return;
-  const T beta = (T)(scal[rnew] / scal[rold]);
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    const T zi = dinv ? dinv[i] * r[i] : r[i];
-    p[i] = zi + beta * p[i];
+__global__ __launch_bounds__(256) void cg_xpupdate_kernel(long long n, T* __restrict__ x, T* __restrict__ p,
+                                                          const T* __restrict__ r, const T* __restrict__ dinv,
+                                                          const A* __restrict__ scal, int slot_rho_old,
+                                                          int slot_rho_new, const int* __restrict__ state, int it) {
+  if (state[0] && state[2] < it) return;
+  const bool pupd = state[0] == 0;
+  const T lambda = (T)(scal[slot_rho_old] / scal[0]);
+  const T beta = (T)(scal[slot_rho_new] / scal[slot_rho_old]);
+  const long long S = (long long)gridDim.x * blockDim.x;
+  for (long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += kVecUnroll * S) {
+    T pv[kVecUnroll], xv[kVecUnroll], zv[kVecUnroll];
+#pragma unroll
+    for (int u = 0; u < kVecUnroll; ++u) {
+      const long long i = i0 + u * S;
+      pv[u] = xv[u] = zv[u] = T(0);
+      if (i < n) {
+        pv[u] = p[i];
+        xv[u] = x[i];
+        if (pupd) zv[u] = dinv ? dinv[i] * r[i] : r[i];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kVecUnroll; ++u) {
+      const long long i = i0 + u * S;
+      if (i < n) {
+        x[i] = xv[u] + lambda * pv[u];
+        if (pupd) p[i] = zv[u] + beta * pv[u];
+      }
+    }
   }
 }
 

@@ -27,6 +27,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "block_reduce.hh"
+
 namespace pdelab_amd {
 
 enum { BC_DIRICHLET = 1, BC_NEUMANN = -1, BC_OUTFLOW = -2, BC_NONE = -3 };
@@ -62,7 +64,7 @@ __device__ __forceinline__ long long xcd_logical_block(long long bid, long long 
 // reference assembler does (R:dune/pdelab/gridoperator/default/assembler.hh:181):
 // omega and harmonic average convectiondiffusiondg.hh:326-338, penalty :346.
 template <typename T>
-__device__ __forceinline__ void face_coeffs(const DGArgs<T>& a, T klo, T khi, T& wlo, T& whi, T& P) {
+__host__ __device__ __forceinline__ void face_coeffs(const DGArgs<T>& a, T klo, T khi, T& wlo, T& whi, T& P) {
   if (a.weightsOn) {
     const T den = khi + klo + T(1e-20);
     const T omega_s = klo / den, omega_n = khi / den;
@@ -112,7 +114,7 @@ __device__ __forceinline__ int lds_index(int p, int q, int e) {
 // line values u, the neighbouring lines ul/ur (valid when has_lo/has_hi) and the
 // kappa of the cell and its two neighbours in direction D.
 template <int N1, int D, typename T, bool TRANS, bool CONV, bool REACT, bool VARK>
-__device__ __forceinline__ void line_op(const DGArgs<T>& a, const T (&u)[N1], const T (&ul)[N1], const T (&ur)[N1],
+__host__ __device__ __forceinline__ void line_op(const DGArgs<T>& a, const T (&u)[N1], const T (&ul)[N1], const T (&ur)[N1],
                                         bool has_lo, bool has_hi, T kc, T klo, T khi, T (&s)[N1]) {
   const T val0 = u[0], val1 = u[N1 - 1];
   T der0 = T(0), der1 = T(0);
@@ -198,24 +200,74 @@ __device__ __forceinline__ void line_op(const DGArgs<T>& a, const T (&u)[N1], co
 }
 
 
-// Deterministic sum over the workgroup of one value per thread (any size <= 512);
-// the result is valid in thread 0.
-template <int THREADS, typename T>
-__device__ __forceinline__ T block_sum_any(T v) {
-  constexpr int R = (THREADS + 63) / 64 * 64;
-  __shared__ T red[R];
-  const int t = threadIdx.x;
-  red[t] = v;
-  if (t < R - THREADS) red[THREADS + t] = T(0);
-  __syncthreads();
-  T s = T(0);
-  if (t < 64) {
+// Constant-coefficient form of line_op (kappa, b and c the same in every cell): per
+// direction d the line operator is linear in (u, ul, ur) with fixed matrices,
+//   s = A u + Pl ul[n-1] + Ql (d1 . ul) + Pr ur[0] + Qr (d0 . ur)      (interior faces)
+// and a boundary face replaces its neighbour term by the correction Dlo u (or Dhi u).
+// The matrices are extracted on the host from line_op itself (build_line_mats), so the
+// two forms are the same operator; this one needs ~40 instead of ~64 FMAs per line.
+template <typename T>
+struct LineMats {
+  T A[3][4][4], Dlo[3][4][4], Dhi[3][4][4];
+  T Pl[3][4], Ql[3][4], Pr[3][4], Qr[3][4];
+};
+
+template <int N1, int D, typename T>
+__device__ __forceinline__ void line_op_const(const LineMats<T>& m, const DGArgs<T>& a, const T (&u)[N1],
+                                              const T (&ul)[N1], const T (&ur)[N1], bool has_lo, bool has_hi,
+                                              T (&s)[N1]) {
 #pragma unroll
-    for (int j = 0; j < R / 64; ++j) s += red[t + 64 * j];
+  for (int i = 0; i < N1; ++i) {
+    T acc = T(0);
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    for (int j = 0; j < N1; ++j) acc += m.A[D][i][j] * u[j];
+    s[i] = acc;
   }
-  return s;
+  if (has_lo) {
+    const T vl = ul[N1 - 1];
+    T dl = T(0);
+#pragma unroll
+    for (int j = 0; j < N1; ++j) dl += a.d1[j] * ul[j];
+#pragma unroll
+    for (int i = 0; i < N1; ++i) s[i] += m.Pl[D][i] * vl + m.Ql[D][i] * dl;
+  } else {
+#pragma unroll
+    for (int i = 0; i < N1; ++i) {
+      T acc = T(0);
+#pragma unroll
+      for (int j = 0; j < N1; ++j) acc += m.Dlo[D][i][j] * u[j];
+      s[i] += acc;
+    }
+  }
+  if (has_hi) {
+    const T vr = ur[0];
+    T dr = T(0);
+#pragma unroll
+    for (int j = 0; j < N1; ++j) dr += a.d0[j] * ur[j];
+#pragma unroll
+    for (int i = 0; i < N1; ++i) s[i] += m.Pr[D][i] * vr + m.Qr[D][i] * dr;
+  } else {
+#pragma unroll
+    for (int i = 0; i < N1; ++i) {
+      T acc = T(0);
+#pragma unroll
+      for (int j = 0; j < N1; ++j) acc += m.Dhi[D][i][j] * u[j];
+      s[i] += acc;
+    }
+  }
+}
+
+// The line operator used by the apply kernels: the general form for cell-wise kappa and
+// for FP64 (whose per-direction matrices would not fit the scalar registers), the
+// precomputed constant-coefficient form for FP32 with constant coefficients.
+template <int N1, int D, typename T, bool TRANS, bool CONV, bool REACT, bool VARK>
+__device__ __forceinline__ void line_apply(const DGArgs<T>& a, const LineMats<T>* __restrict__ lm, const T (&u)[N1],
+                                           const T (&ul)[N1], const T (&ur)[N1], bool has_lo, bool has_hi, T kc,
+                                           T klo, T khi, T (&s)[N1]) {
+  if constexpr (VARK || sizeof(T) == 8)
+    line_op<N1, D, T, TRANS, CONV, REACT, VARK>(a, u, ul, ur, has_lo, has_hi, kc, klo, khi, s);
+  else
+    line_op_const<N1, D, T>(*lm, a, u, ul, ur, has_lo, has_hi, s);
 }
 
 template <int N1, int D, typename T>
@@ -237,7 +289,9 @@ template <int N1, typename T, bool TRANS, bool CONV, bool REACT, bool VARK, bool
 __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const T* __restrict__ z,
                                                                            T* __restrict__ y,
                                                                            const T* __restrict__ kappa,
-                                                                           const DGArgs<T> a, long long nblocks,
+                                                                           const DGArgs<T> a,
+                                                                           const LineMats<T>* __restrict__ lm,
+                                                                           long long nblocks,
                                                                            T* __restrict__ dotp,
                                                                            const int* __restrict__ stop) {
   if (stop && *stop) return;  // a device-side Krylov loop has already stopped
@@ -308,7 +362,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
   __syncthreads();
   T sx[N1], sy[N1], sz[N1];
   if (active) {
-    line_op<N1, 2, T, TRANS, CONV, REACT, VARK>(a, uz, zl, zr, lo[2], hi[2], kc, klo[2], khi[2], sz);
+    line_apply<N1, 2, T, TRANS, CONV, REACT, VARK>(a, lm, uz, zl, zr, lo[2], hi[2], kc, klo[2], khi[2], sz);
     T u[N1], ul[N1], ur[N1];
 #pragma unroll
     for (int e = 0; e < N1; ++e) {
@@ -317,7 +371,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
       ul[e] = lo[1] ? YMl[sl] : T(0);
       ur[e] = hi[1] ? YPl[sl] : T(0);
     }
-    line_op<N1, 1, T, TRANS, CONV, REACT, VARK>(a, u, ul, ur, lo[1], hi[1], kc, klo[1], khi[1], sy);
+    line_apply<N1, 1, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, lo[1], hi[1], kc, klo[1], khi[1], sy);
     // x-neighbours: staged cell lc-1 / lc+1 (U[0] and U[CPB+1] hold the end cells)
     const T* xlo = Ul - NLP;
     const T* xhi = Ul + NLP;
@@ -328,7 +382,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
       ul[e] = lo[0] ? xlo[sl] : T(0);
       ur[e] = hi[0] ? xhi[sl] : T(0);
     }
-    line_op<N1, 0, T, TRANS, CONV, REACT, VARK>(a, u, ul, ur, lo[0], hi[0], kc, klo[0], khi[0], sx);
+    line_apply<N1, 0, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, lo[0], hi[0], kc, klo[0], khi[0], sx);
   }
   __syncthreads();  // all reads of the staged cells done
   T pq = T(0);
@@ -389,8 +443,210 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
   }
   if (DOT) {
     // fused z . (J z) for the Krylov step length (one partial per workgroup)
-    const T bs = block_sum_any<SH::THREADS, T>(pq);
-    if (threadIdx.x == 0) dotp[blockIdx.x] = bs;
+    T v[1] = {pq};
+    block_sum_n<SH::THREADS, T, 1>(v);
+    if (threadIdx.x == 0) dotp[blockIdx.x] = v[0];
+  }
+}
+
+// y (+)= J z (or J^T z) on grids with nx % CPB == 0, software-pipelined along z.
+// A workgroup owns one aligned x-row chunk (CPB cells) at row jy and walks kz over a
+// segment [k0, k1). Per step it stages in LDS the chunk, its two x-end cells and the
+// chunks of the rows y-1 and y+1 (all read from registers loaded one step earlier, in
+// the coalesced z-line pattern), and it issues the loads of the next step before it
+// computes the current one, so the memory traffic of step kz+1 overlaps the arithmetic
+// of step kz. The z-neighbour lines are the previous and next steps' own z-lines,
+// kept in registers. Logical blocks run jy fastest, so the workgroups of rows y-1 and
+// y+1 of a chunk run at the same time on the same XCD and their loads meet in its L2.
+template <int N1, typename T, bool TRANS, bool CONV, bool REACT, bool VARK, bool ACCUM, bool DOT>
+__global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_zwalk_kernel(
+    const T* __restrict__ z, T* __restrict__ y, const T* __restrict__ kappa, const DGArgs<T> a,
+    const LineMats<T>* __restrict__ lm, long long nblocks, int seglen, T* __restrict__ dotp,
+    const int* __restrict__ stop) {
+  if (stop && *stop) return;  // a device-side Krylov loop has already stopped
+  using SH = ApplyShape<N1>;
+  constexpr int L = SH::L, NL = SH::NL, NLP = SH::NLP, CPB = SH::CPB;
+  __shared__ T U[(CPB + 2) * NLP];  // [0] cell before the chunk, [1..CPB] own cells, [CPB+1] cell after
+  __shared__ T YM[CPB * NLP];       // row y-1, later the z partial sums and the x mass sweep
+  __shared__ T YP[CPB * NLP];       // row y+1, later the x partial sums
+  const int t = threadIdx.x;
+  const int lc = t / L, l = t % L, p = l % N1, q = l / N1;
+  const unsigned ub = (unsigned)xcd_logical_block(blockIdx.x, nblocks);
+  const unsigned nyu = (unsigned)a.n[1], nxc = (unsigned)(a.n[0] / CPB);
+  const unsigned jy = ub % nyu, rest = ub / nyu;
+  const unsigned xc = rest % nxc, seg = rest / nxc;
+  const int nz = a.n[2];
+  const int k0 = (int)seg * seglen;
+  const int k1 = min(nz, k0 + seglen);
+  const long long sY = a.stride[1], sZ = a.stride[2];
+  const long long cb = (long long)xc * CPB + sY * jy;  // first cell of the chunk at kz = 0
+  const bool xlo_ok = xc > 0, xhi_ok = (xc + 1) * CPB < (unsigned)a.n[0];
+  const bool ylo = jy > 0, yhi = (int)jy < a.n[1] - 1;
+  const int gx = (int)xc * CPB + lc;
+  const bool lox = gx > 0, hix = gx < a.n[0] - 1;
+  const bool first = lc == 0, last = lc == CPB - 1;
+  const bool ld_xe = (first && xlo_ok) || (last && xhi_ok);
+  const int off = lc * NL;
+  // the z-line of this lane in the chunk at kz, shifted by dc cells
+  auto zline = [&](int kz, long long dc, T (&u)[N1]) {
+    const T* base = z + (cb + sZ * kz + dc) * NL + off;
+#pragma unroll
+    for (int e = 0; e < N1; ++e) u[e] = base[line_index<N1, 2>(p, q, e)];
+  };
+  T zp[N1], zc[N1], zn[N1], pm[N1], pp[N1], pe[N1];
+  T kc = a.kappa0, klo[3], khi[3], nk[6];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) klo[d] = khi[d] = a.kappa0;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) nk[i] = a.kappa0;
+  // kappa of the cell and its x/y neighbours at step kz (the z neighbours are the cells
+  // of the previous and next steps)
+  auto kload = [&](int kz) {
+    if (VARK) {
+      const T* kb = kappa + cb + sZ * kz + lc;
+      nk[0] = kb[0];
+      if (lox) nk[1] = kb[-1];
+      if (hix) nk[2] = kb[1];
+      if (ylo) nk[3] = kb[-sY];
+      if (yhi) nk[4] = kb[sY];
+      if (kz + 1 < nz) nk[5] = kb[sZ];
+    }
+  };
+  auto rows = [&](int kz) {
+    if (ylo) zline(kz, -sY, pm);
+    if (yhi) zline(kz, sY, pp);
+    if (ld_xe) zline(kz, first ? -1 - lc : CPB - lc, pe);
+    kload(kz);
+  };
+#pragma unroll
+  for (int e = 0; e < N1; ++e) zp[e] = zn[e] = T(0);
+  if (k0 < k1) {  // (an empty segment only takes part in the fused reduction)
+    if (k0 > 0) zline(k0 - 1, 0, zp);
+    zline(k0, 0, zc);
+    if (k0 + 1 < nz) zline(k0 + 1, 0, zn);
+    rows(k0);
+    if (VARK && k0 > 0) kc = kappa[cb + sZ * (k0 - 1) + lc];  // becomes klo[2] of the first step
+  }
+  T pq = T(0);
+  T* Ul = U + (lc + 1) * NLP;
+  T* YMl = YM + lc * NLP;
+  T* YPl = YP + lc * NLP;
+  for (int kz = k0; kz < k1; ++kz) {
+    const bool zlo = kz > 0, zhi = kz < nz - 1;
+    __syncthreads();  // the previous step's reads of the staged data are done
+#pragma unroll
+    for (int e = 0; e < N1; ++e) {
+      const int sl = lds_index<N1, 2>(p, q, e);
+      Ul[sl] = zc[e];
+      if (ylo) YMl[sl] = pm[e];
+      if (yhi) YPl[sl] = pp[e];
+      if (ld_xe) (first ? U : U + (CPB + 1) * NLP)[sl] = pe[e];
+    }
+    if (VARK) {
+      klo[2] = kc;
+      kc = nk[0];
+      klo[0] = nk[1];
+      khi[0] = nk[2];
+      klo[1] = nk[3];
+      khi[1] = nk[4];
+      khi[2] = nk[5];
+    }
+    // the loads of the next step are in flight while this one computes
+    T zf[N1];
+#pragma unroll
+    for (int e = 0; e < N1; ++e) zf[e] = T(0);
+    if (kz + 1 < k1) {
+      if (kz + 2 < nz) zline(kz + 2, 0, zf);
+      rows(kz + 1);
+    }
+    __syncthreads();
+    T sx[N1], sy[N1], sz[N1];
+    line_apply<N1, 2, T, TRANS, CONV, REACT, VARK>(a, lm, zc, zp, zn, zlo, zhi, kc, klo[2], khi[2], sz);
+    {
+      T u[N1], ul[N1], ur[N1];
+#pragma unroll
+      for (int e = 0; e < N1; ++e) {
+        const int sl = lds_index<N1, 1>(p, q, e);
+        u[e] = Ul[sl];
+        ul[e] = ylo ? YMl[sl] : T(0);
+        ur[e] = yhi ? YPl[sl] : T(0);
+      }
+      line_apply<N1, 1, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, ylo, yhi, kc, klo[1], khi[1], sy);
+      const T* xlo = Ul - NLP;
+      const T* xhi = Ul + NLP;
+#pragma unroll
+      for (int e = 0; e < N1; ++e) {
+        const int sl = lds_index<N1, 0>(p, q, e);
+        u[e] = Ul[sl];
+        ul[e] = lox ? xlo[sl] : T(0);
+        ur[e] = hix ? xhi[sl] : T(0);
+      }
+      line_apply<N1, 0, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, lox, hix, kc, klo[0], khi[0], sx);
+    }
+    __syncthreads();  // all reads of the staged cells done
+#pragma unroll
+    for (int e = 0; e < N1; ++e) {
+      YMl[lds_index<N1, 2>(p, q, e)] = sz[e];
+      YPl[lds_index<N1, 0>(p, q, e)] = sx[e];
+    }
+    __syncthreads();
+    {
+      T v[N1];
+#pragma unroll
+      for (int e = 0; e < N1; ++e) {
+        const int sl = lds_index<N1, 1>(p, q, e);
+        v[e] = sy[e] + YMl[sl] + YPl[sl];
+      }
+#pragma unroll
+      for (int i = 0; i < N1; ++i) {
+        T acc = T(0);
+#pragma unroll
+        for (int j = 0; j < N1; ++j) acc += a.M[i][j] * v[j];
+        Ul[lds_index<N1, 1>(p, q, i)] = acc;
+      }
+    }
+    __syncthreads();
+    {
+      T v[N1];
+#pragma unroll
+      for (int e = 0; e < N1; ++e) v[e] = Ul[lds_index<N1, 0>(p, q, e)];
+#pragma unroll
+      for (int i = 0; i < N1; ++i) {
+        T acc = T(0);
+#pragma unroll
+        for (int j = 0; j < N1; ++j) acc += a.M[i][j] * v[j];
+        YMl[lds_index<N1, 0>(p, q, i)] = acc;
+      }
+    }
+    __syncthreads();
+    {
+      T v[N1];
+#pragma unroll
+      for (int e = 0; e < N1; ++e) v[e] = YMl[lds_index<N1, 2>(p, q, e)];
+      T* yc = y + (cb + sZ * kz) * NL + off;
+#pragma unroll
+      for (int i = 0; i < N1; ++i) {
+        T acc = T(0);
+#pragma unroll
+        for (int j = 0; j < N1; ++j) acc += a.M[i][j] * v[j];
+        const int idx = line_index<N1, 2>(p, q, i);
+        if (ACCUM) yc[idx] += acc;
+        else yc[idx] = acc;
+        if (DOT) pq += zc[i] * acc;
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < N1; ++e) {
+      zp[e] = zc[e];
+      zc[e] = zn[e];
+      zn[e] = zf[e];
+    }
+  }
+  if (DOT) {
+    // fused z . (J z) for the Krylov step length (one partial per workgroup)
+    T v[1] = {pq};
+    block_sum_n<SH::THREADS, T, 1>(v);
+    if (threadIdx.x == 0) dotp[blockIdx.x] = v[0];
   }
 }
 
@@ -423,7 +679,9 @@ template <int N1, typename T, bool TRANS, bool CONV, bool REACT, bool VARK, bool
 __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_row_kernel(const T* __restrict__ z,
                                                                                T* __restrict__ y,
                                                                                const T* __restrict__ kappa,
-                                                                               const DGArgs<T> a, long long nblocks,
+                                                                               const DGArgs<T> a,
+                                                                               const LineMats<T>* __restrict__ lm,
+                                                                               long long nblocks,
                                                                                T* __restrict__ dotp,
                                                                                const int* __restrict__ stop) {
   if (stop && *stop) return;  // a device-side Krylov loop has already stopped
@@ -484,7 +742,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_row_kernel(c
     T u[N1], ul[N1], ur[N1];
 #pragma unroll
     for (int e = 0; e < N1; ++e) u[e] = PZ[e] = Ol[line_index<N1, 2>(p, q, e)];
-    line_op<N1, 2, T, TRANS, CONV, REACT, VARK>(a, u, zl, zr, zlo, zhi, kc, klo[2], khi[2], sz);
+    line_apply<N1, 2, T, TRANS, CONV, REACT, VARK>(a, lm, u, zl, zr, zlo, zhi, kc, klo[2], khi[2], sz);
 #pragma unroll
     for (int e = 0; e < N1; ++e) {
       const int sl = line_index<N1, 1>(p, q, e);
@@ -492,7 +750,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_row_kernel(c
       ul[e] = ylo ? YM[off + sl] : T(0);
       ur[e] = yhi ? YP[off + sl] : T(0);
     }
-    line_op<N1, 1, T, TRANS, CONV, REACT, VARK>(a, u, ul, ur, ylo, yhi, kc, klo[1], khi[1], sy);
+    line_apply<N1, 1, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, ylo, yhi, kc, klo[1], khi[1], sy);
 #pragma unroll
     for (int e = 0; e < N1; ++e) {
       const int sl = line_index<N1, 0>(p, q, e);
@@ -500,7 +758,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_row_kernel(c
       ul[e] = lox ? Ol[sl - NL] : T(0);
       ur[e] = hix ? Ol[sl + NL] : T(0);
     }
-    line_op<N1, 0, T, TRANS, CONV, REACT, VARK>(a, u, ul, ur, lox, hix, kc, klo[0], khi[0], sx);
+    line_apply<N1, 0, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, lox, hix, kc, klo[0], khi[0], sx);
   }
   __syncthreads();  // all reads of the staged cells done
   T pq = T(0);
@@ -562,8 +820,9 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_row_kernel(c
   }
   if (DOT) {
     // fused z . (J z) for the Krylov step length (one partial per workgroup)
-    const T bs = block_sum_any<SH::THREADS, T>(pq);
-    if (threadIdx.x == 0) dotp[blockIdx.x] = bs;
+    T v[1] = {pq};
+    block_sum_n<SH::THREADS, T, 1>(v);
+    if (threadIdx.x == 0) dotp[blockIdx.x] = v[0];
   }
 }
 

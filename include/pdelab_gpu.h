@@ -234,7 +234,8 @@ int pdelab_gpu_linear_problem_solve(pdelab_gpu_gridoperator* h, int method, int 
                                     double reduction, double min_defect, int maxit, double* x,
                                     pdelab_linear_problem_result* res);
 /* Mixed precision: FP64 refinement x <- x - J32^{-1} R(x) with the inner solve in FP32
-   to inner_reduction, until ||R(x)|| <= outer_reduction * ||R(x0)||. */
+   to inner_reduction (or, when weaker, to a quarter of the reduction still missing), until
+   ||R(x)|| <= outer_reduction * ||R(x0)||. */
 int pdelab_gpu_solve_mixed(pdelab_gpu_gridoperator* h, int method, int precond,
                            double inner_reduction, int inner_maxit, double outer_reduction,
                            int outer_maxit, double* x, pdelab_mixed_result* res);

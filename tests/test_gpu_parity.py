@@ -92,6 +92,58 @@ def test_apply_diag_transpose_mass(pkg, of, name, kw, seed):
     assert rel(y32.download().astype(np.float64), o.jacobian_apply(z)) <= 2e-5
 
 
+# Aligned-row grids through both aligned-row apply kernels: the z-walk kernel with segment
+# lengths that put segment boundaries inside the grid (and the default launch shape), and
+# the one-chunk-per-workgroup kernel. SPD cases also run CG, whose iterations use the
+# apply with the fused z.(Jz) partials.
+VARIANTS = [
+    ("zwalk_q3_16x3x5_seg2_var_conv", dict(nx=16, ny=3, nz=5, degree=3, alpha=3.0, b=(0.5, -1.0, 0.25), c=0.2),
+     21, "zwalk", 2),
+    ("zwalk_q3_16x2x6_seg4_react", dict(nx=16, ny=2, nz=6, degree=3, alpha=3.0, c=0.5), None, "zwalk", 4),
+    ("zwalk_q3_16x4x5_seg2_var", dict(nx=16, ny=4, nz=5, degree=3, alpha=3.0), 24, "zwalk", 2),
+    ("zwalk_q2_32x2x7_seg3", dict(nx=32, ny=2, nz=7, degree=2, alpha=3.0), None, "zwalk", 3),
+    ("zwalk_q2_64x3x3_seg1_nipg_var", dict(nx=64, ny=3, nz=3, degree=2, method=0, alpha=2.0), 22, "zwalk", 1),
+    ("zwalk_q1_64x3x4_seg1_var_conv", dict(nx=64, ny=3, nz=4, degree=1, alpha=3.0, b=(0.0, 1.0, -1.0)), 23,
+     "zwalk", 1),
+    ("zwalk_q3_32x1x3_neumann", dict(nx=32, ny=1, nz=3, degree=3, alpha=3.0, bctype=(0, -1, 0, 0, -1, 0)), None,
+     "zwalk", None),
+    ("zwalk_q1_64x1x1_off", dict(nx=64, ny=1, nz=1, degree=1, weights=1, alpha=3.0), None, "zwalk", None),
+    ("zwalk_q2_32x5x9_default", dict(nx=32, ny=5, nz=9, degree=2, alpha=3.0), None, "zwalk", None),
+    ("row_q3_16x3x5_var_conv", dict(nx=16, ny=3, nz=5, degree=3, alpha=3.0, b=(0.5, -1.0, 0.25)), 21, "row", None),
+    ("row_q2_32x2x3", dict(nx=32, ny=2, nz=3, degree=2, alpha=3.0), None, "row", None),
+]
+
+
+@pytest.mark.parametrize("name,kw,seed,variant,seglen", VARIANTS, ids=[v[0] for v in VARIANTS])
+def test_apply_kernel_variants(pkg, of, monkeypatch, name, kw, seed, variant, seglen):
+    say(f"[gpu] apply kernel variant case={name}")
+    monkeypatch.setenv("PDELAB_GPU_APPLY", variant)
+    if seglen:
+        monkeypatch.setenv("PDELAB_GPU_SEGLEN", str(seglen))
+    else:
+        monkeypatch.delenv("PDELAB_GPU_SEGLEN", raising=False)
+    cfg, kappa, go, o = make(pkg, of, kw, seed)
+    rng = np.random.default_rng(43)
+    z = rng.uniform(-1, 1, go.N)
+    y0 = rng.uniform(-1, 1, go.N)
+    zd, yd = go.vector(z), go.vector(y0)
+    go.jacobian_apply(zd, yd)
+    assert rel(yd.download(), o.jacobian_apply(z, y0.copy())) <= APPLY_RTOL
+    yd.upload(y0)
+    go.jacobian_apply_transpose(zd, yd)
+    assert rel(yd.download(), y0 + o.jacobian_apply_transpose(z)) <= APPLY_RTOL
+    z32 = go.vector(z.astype(np.float32), dtype=np.float32)
+    y32 = go.vector(np.zeros(go.N, np.float32), dtype=np.float32)
+    go.jacobian_apply(z32, y32)
+    assert rel(y32.download().astype(np.float64), o.jacobian_apply(z)) <= 2e-5
+    if kw.get("method", 1) == 1 and "b" not in kw:
+        rhs = rng.normal(size=go.N)
+        x, r = go.vector(), go.vector(rhs)
+        res = go.solve(x, r, method=0, reduction=1e-10, maxit=5000)
+        assert res["converged"]
+        assert rel(o.jacobian_apply(x.download()), rhs) <= 1e-9
+
+
 PROBLEMS = [
     ("exp_q2", dict(n=4, degree=2, alpha=3.0), None, 1),
     ("quadratic_q3_var", dict(nx=3, ny=2, nz=4, degree=3, alpha=3.0, quad=(0.3, 1.0, 2.0, -1.0, 1.0, 1.0, -2.0)), 9, 4),
