@@ -73,6 +73,72 @@ def cpu_baseline(cfg, min_seconds):
                       f"restatement, OpenMP over cells with two-sided faces), {sec * reps:.1f} s total"}
 
 
+APPLY_KERNEL = "dg_apply_row_kernel<4, double, false, false, false, false, false, false>"
+CALIB_KERNEL = "axpy_kernel<double>"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "apply_traffic.json")
+
+
+def _csv_rows(d, suffix):
+    import csv
+    import glob
+    rows = []
+    for f in glob.glob(os.path.join(d, "**", "*" + suffix), recursive=True):
+        with open(f) as fh:
+            rows += list(csv.DictReader(fh))
+    return rows
+
+
+def profile_summary(d, n, k):
+    """Summary (markdown) of the rocprofv3 runs of this benchmark kept under `d`:
+    d/trace (--kernel-trace --stats), d/fetch (--pmc FETCH_SIZE), d/write (--pmc WRITE_SIZE).
+    HBM bytes of the apply are the counters scaled by what they read for the axpy
+    calibration launches (16 B/DoF read, 8 B/DoF written); also writes TRAFFIC_FILE."""
+    N = n ** 3 * (k + 1) ** 3
+    stats = _csv_rows(os.path.join(d, "trace"), "_kernel_stats.csv")
+    stats.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    lines = ["| kernel | calls | average (us) | total (%) |", "|---|---|---|---|"]
+    for r in stats[:8]:
+        lines.append(f"| `{r['Name'][:110]}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "
+                     f"{float(r['Percentage']):.1f} |")
+    avg = {}
+    for name, sub, cnt in (("fetch", "fetch", "FETCH_SIZE"), ("write", "write", "WRITE_SIZE")):
+        for r in _csv_rows(os.path.join(d, sub), "_counter_collection.csv"):
+            if r["Counter_Name"] != cnt:
+                continue
+            for tag, pat in (("apply", APPLY_KERNEL), ("calib", CALIB_KERNEL)):
+                if pat in r["Kernel_Name"]:
+                    avg.setdefault((name, tag), []).append(float(r["Counter_Value"]) * 1024.0)
+    m = {key: sum(v) / len(v) for key, v in avg.items()}
+    ff = 16.0 * N / m[("fetch", "calib")]
+    fw = 8.0 * N / m[("write", "calib")]
+    fetch = m[("fetch", "apply")] * ff
+    write = m[("write", "apply")] * fw
+    apply_avg = [float(r["AverageNs"]) for r in stats if APPLY_KERNEL in r["Name"]]
+    out = {"kernel": APPLY_KERNEL, "n": n, "degree": k, "fetch_bytes": fetch, "write_bytes": write,
+           "traffic_bytes": fetch + write, "algorithmic_bytes": 16.0 * N, "fetch_factor": ff, "write_factor": fw,
+           "average_us": apply_avg[0] / 1e3 if apply_avg else None}
+    os.makedirs(os.path.dirname(TRAFFIC_FILE), exist_ok=True)
+    with open(TRAFFIC_FILE, "w") as fh:
+        json.dump(out, fh, indent=1)
+    md = [f"Kernel time per launch (rocprofv3 --kernel-trace --stats), {n}^3 Q{k}:", ""] + lines + [
+        "", f"HBM traffic of `{APPLY_KERNEL}` per launch (rocprofv3 --pmc, one counter per run):", "",
+        "| counter | raw (KiB/launch) | scale (axpy calibration) | bytes/launch | algorithmic |", "|---|---|---|---|---|",
+        f"| FETCH_SIZE | {m[('fetch', 'apply')] / 1024:.0f} | {ff:.3f} | {fetch:.4g} | {8.0 * N:.4g} |",
+        f"| WRITE_SIZE | {m[('write', 'apply')] / 1024:.0f} | {fw:.3f} | {write:.4g} | {8.0 * N:.4g} |",
+        "", f"traffic = {fetch + write:.4g} B/launch = {(fetch + write) / (16.0 * N):.3f} x algorithmic"]
+    return "\n".join(md)
+
+
+def measured_traffic(n, k):
+    """HBM bytes per apply launch from the committed rocprofv3 counter runs (or None)."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            t = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    return t["traffic_bytes"] if (t.get("n"), t.get("degree")) == (n, k) else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,7 +153,12 @@ def main():
                     help="FP32 inner Krylov reduction of the mixed-precision refinement")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run 5 axpy launches (reads 16 B/DoF, writes 8 B/DoF) to scale HBM counters")
+    ap.add_argument("--profile-summary", metavar="DIR",
+                    help="print the summary of the rocprofv3 runs of this benchmark kept under DIR and exit")
     args = ap.parse_args()
+    if args.profile_summary:
+        print(profile_summary(args.profile_summary, args.n, args.degree))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
               "(the domain-decomposition design is described in DESIGN.md).", file=sys.stderr)
@@ -148,7 +219,7 @@ def main():
                                f"on a seeded uniform[-1,1) vector",
                    "cells": n ** 3, "degree": k, "dofs": N},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(n, k),
                      "kernel": (f"dg_apply_row_kernel<{k + 1},double,...>" if n % {1: 64, 2: 32, 3: 16}[k] == 0
                                 else f"dg_apply_kernel<{k + 1},double,...>"), "kernel_avg_us": kern_ms * 1e3,
                      "algorithmic_bytes_per_launch": alg_bytes,
