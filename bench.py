@@ -73,7 +73,7 @@ def cpu_baseline(cfg, min_seconds):
                       f"restatement, OpenMP over cells with two-sided faces), {sec * reps:.1f} s total"}
 
 
-APPLY_KERNEL = "dg_apply_row_kernel<4, double, false, false, false, false, false, false>"
+APPLY_KERNEL = "dg_apply_rowreg_kernel<4, double, false, false, false, false, false, false>"
 CALIB_KERNEL = "axpy_kernel<double>"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "apply_traffic.json")
 
@@ -220,7 +220,7 @@ def main():
                    "cells": n ** 3, "degree": k, "dofs": N},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(n, k),
-                     "kernel": (f"dg_apply_row_kernel<{k + 1},double,...>" if n % {1: 64, 2: 32, 3: 16}[k] == 0
+                     "kernel": (f"dg_apply_rowreg_kernel<{k + 1},double,...>" if n % {1: 64, 2: 32, 3: 16}[k] == 0
                                 else f"dg_apply_kernel<{k + 1},double,...>"), "kernel_avg_us": kern_ms * 1e3,
                      "algorithmic_bytes_per_launch": alg_bytes,
                      "fp32_kernel_avg_us": kern32_ms * 1e3,

@@ -40,7 +40,7 @@ class PdelabError(RuntimeError):
 
 def build():
     """Compiles lib/libpdelab_gpu.so for gfx950 (hipcc; no GPU needed)."""
-    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    subprocess.run(["make", "-s", "-j4", "-C", HERE], check=True)
 
 
 def lib():

@@ -43,6 +43,11 @@ struct DGArgs {
   // constant-kappa face coefficients (interior: w of lower cell, w of upper cell,
   // penalty; boundary penalty)
   T wI_lo, wI_hi, PI, PB;
+  // constant kappa without convection: the line operator of direction d is S[d] times
+  // one centro-symmetric matrix Ah with neighbour couplings Ph/Qh ([0] J, [1] J^T);
+  // vc = V c is the reaction term of the x lines
+  T Ah[2][4][4], Ph[2][4], Qh[2][4];
+  T vc;
   int n[3];
   long long stride[3];
   long long ncells;
@@ -200,28 +205,37 @@ __host__ __device__ __forceinline__ void line_op(const DGArgs<T>& a, const T (&u
 }
 
 
-// Constant-coefficient form of line_op (kappa, b and c the same in every cell): per
-// direction d the line operator is linear in (u, ul, ur) with fixed matrices,
-//   s = A u + Pl ul[n-1] + Ql (d1 . ul) + Pr ur[0] + Qr (d0 . ur)      (interior faces)
-// and a boundary face replaces its neighbour term by the correction Dlo u (or Dhi u).
-// The matrices are extracted on the host from line_op itself (build_line_mats), so the
-// two forms are the same operator; this one needs ~40 instead of ~64 FMAs per line.
+// Constant-coefficient line matrices of direction d, extracted on the host from line_op
+// (build_line_mats): s = A u + Pl ul[n-1] + Ql (d1 . ul) + Pr ur[0] + Qr (d0 . ur) with
+// interior faces; a boundary face replaces its neighbour term by Dlo u (Dhi u).
 template <typename T>
 struct LineMats {
   T A[3][4][4], Dlo[3][4][4], Dhi[3][4][4];
   T Pl[3][4], Ql[3][4], Pr[3][4], Qr[3][4];
 };
 
-template <int N1, int D, typename T>
-__device__ __forceinline__ void line_op_const(const LineMats<T>& m, const DGArgs<T>& a, const T (&u)[N1],
-                                              const T (&ul)[N1], const T (&ur)[N1], bool has_lo, bool has_hi,
-                                              T (&s)[N1]) {
+// Constant kappa without convection (line_op with VARK = CONV = false): the operator of
+// direction d is S[d] (Ah u + Ph ul[n-1] + Qh (d1 . ul) + mirrored upper-neighbour
+// terms) with Ah centro-symmetric, Pr[i] = Ph[n-1-i], Qr[i] = -Qh[n-1-i] and
+// d0[j] = -d1[n-1-j]; a boundary face adds the correction Dlo u (Dhi u) of the line
+// matrices instead of the neighbour term. The host extracts all of it from line_op and
+// checks the structure (build_line_mats), so both forms are the same operator; this one
+// needs ~44 multiply-adds per line and few scalar registers.
+template <int N1, int D, typename T, bool TRANS, bool REACT>
+__device__ __forceinline__ void line_op_sym(const DGArgs<T>& a, const LineMats<T>* __restrict__ lm, const T (&u)[N1],
+                                            const T (&ul)[N1], const T (&ur)[N1], bool has_lo, bool has_hi,
+                                            T (&s)[N1]) {
+  constexpr int tr = TRANS ? 1 : 0;
+  T acc[N1];
 #pragma unroll
   for (int i = 0; i < N1; ++i) {
-    T acc = T(0);
+    T v = T(0);
 #pragma unroll
-    for (int j = 0; j < N1; ++j) acc += m.A[D][i][j] * u[j];
-    s[i] = acc;
+    for (int j = 0; j < N1; ++j) {
+      const bool keep = i * N1 + j <= (N1 - 1 - i) * N1 + (N1 - 1 - j);
+      v += a.Ah[tr][keep ? i : N1 - 1 - i][keep ? j : N1 - 1 - j] * u[j];
+    }
+    acc[i] = v;
   }
   if (has_lo) {
     const T vl = ul[N1 - 1];
@@ -229,45 +243,54 @@ __device__ __forceinline__ void line_op_const(const LineMats<T>& m, const DGArgs
 #pragma unroll
     for (int j = 0; j < N1; ++j) dl += a.d1[j] * ul[j];
 #pragma unroll
-    for (int i = 0; i < N1; ++i) s[i] += m.Pl[D][i] * vl + m.Ql[D][i] * dl;
-  } else {
-#pragma unroll
-    for (int i = 0; i < N1; ++i) {
-      T acc = T(0);
-#pragma unroll
-      for (int j = 0; j < N1; ++j) acc += m.Dlo[D][i][j] * u[j];
-      s[i] += acc;
-    }
+    for (int i = 0; i < N1; ++i) acc[i] += a.Ph[tr][i] * vl + a.Qh[tr][i] * dl;
   }
   if (has_hi) {
     const T vr = ur[0];
-    T dr = T(0);
+    T dr = T(0);  // -(d0 . ur)
 #pragma unroll
-    for (int j = 0; j < N1; ++j) dr += a.d0[j] * ur[j];
+    for (int j = 0; j < N1; ++j) dr += a.d1[N1 - 1 - j] * ur[j];
 #pragma unroll
-    for (int i = 0; i < N1; ++i) s[i] += m.Pr[D][i] * vr + m.Qr[D][i] * dr;
-  } else {
+    for (int i = 0; i < N1; ++i) acc[i] += a.Ph[tr][N1 - 1 - i] * vr + a.Qh[tr][N1 - 1 - i] * dr;
+  }
+  const T Sd = a.S[D];
+#pragma unroll
+  for (int i = 0; i < N1; ++i) s[i] = Sd * acc[i];
+  if (!has_lo) {
 #pragma unroll
     for (int i = 0; i < N1; ++i) {
-      T acc = T(0);
+      T v = T(0);
 #pragma unroll
-      for (int j = 0; j < N1; ++j) acc += m.Dhi[D][i][j] * u[j];
-      s[i] += acc;
+      for (int j = 0; j < N1; ++j) v += lm->Dlo[D][i][j] * u[j];
+      s[i] += v;
     }
+  }
+  if (!has_hi) {
+#pragma unroll
+    for (int i = 0; i < N1; ++i) {
+      T v = T(0);
+#pragma unroll
+      for (int j = 0; j < N1; ++j) v += lm->Dhi[D][i][j] * u[j];
+      s[i] += v;
+    }
+  }
+  if (REACT && D == 0) {
+#pragma unroll
+    for (int i = 0; i < N1; ++i) s[i] += a.vc * u[i];
   }
 }
 
-// The line operator used by the apply kernels: the general form for cell-wise kappa and
-// for FP64 (whose per-direction matrices would not fit the scalar registers), the
-// precomputed constant-coefficient form for FP32 with constant coefficients.
+// The line operator used by the apply kernels: the general form for cell-wise kappa or
+// convection (CONV also marks a constant-coefficient operator without the symmetric
+// structure), the symmetric constant-coefficient form otherwise.
 template <int N1, int D, typename T, bool TRANS, bool CONV, bool REACT, bool VARK>
 __device__ __forceinline__ void line_apply(const DGArgs<T>& a, const LineMats<T>* __restrict__ lm, const T (&u)[N1],
                                            const T (&ul)[N1], const T (&ur)[N1], bool has_lo, bool has_hi, T kc,
                                            T klo, T khi, T (&s)[N1]) {
-  if constexpr (VARK || sizeof(T) == 8)
+  if constexpr (VARK || CONV)
     line_op<N1, D, T, TRANS, CONV, REACT, VARK>(a, u, ul, ur, has_lo, has_hi, kc, klo, khi, s);
   else
-    line_op_const<N1, D, T>(*lm, a, u, ul, ur, has_lo, has_hi, s);
+    line_op_sym<N1, D, T, TRANS, REACT>(a, lm, u, ul, ur, has_lo, has_hi, s);
 }
 
 template <int N1, int D, typename T>
@@ -449,20 +472,21 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
   }
 }
 
-// y (+)= J z (or J^T z) on grids with nx % CPB == 0, software-pipelined along z.
-// A workgroup owns one aligned x-row chunk (CPB cells) at row jy and walks kz over a
-// segment [k0, k1). Per step it stages in LDS the chunk, its two x-end cells and the
-// chunks of the rows y-1 and y+1 (all read from registers loaded one step earlier, in
-// the coalesced z-line pattern), and it issues the loads of the next step before it
-// computes the current one, so the memory traffic of step kz+1 overlaps the arithmetic
-// of step kz. The z-neighbour lines are the previous and next steps' own z-lines,
-// kept in registers. Logical blocks run jy fastest, so the workgroups of rows y-1 and
-// y+1 of a chunk run at the same time on the same XCD and their loads meet in its L2.
+// y (+)= J z (or J^T z) on grids with nx % CPB == 0, staged through registers: the CPB
+// cells of a workgroup are one aligned chunk of an x-row (so the y and z boundary flags
+// are uniform over the workgroup), consecutive logical blocks walk z fastest, then y,
+// then the x-chunk, and every global access is a coalesced z-line access. The chunk,
+// its two end cells and the chunks of the rows y-1 and y+1 are written to LDS in the
+// rotated, conflict-free layout of lds_index; the z-neighbour lines stay in registers.
 template <int N1, typename T, bool TRANS, bool CONV, bool REACT, bool VARK, bool ACCUM, bool DOT>
-__global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_zwalk_kernel(
-    const T* __restrict__ z, T* __restrict__ y, const T* __restrict__ kappa, const DGArgs<T> a,
-    const LineMats<T>* __restrict__ lm, long long nblocks, int seglen, T* __restrict__ dotp,
-    const int* __restrict__ stop) {
+__global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kernel(const T* __restrict__ z,
+                                                                           T* __restrict__ y,
+                                                                           const T* __restrict__ kappa,
+                                                                           const DGArgs<T> a,
+                                                                           const LineMats<T>* __restrict__ lm,
+                                                                           long long nblocks,
+                                                                           T* __restrict__ dotp,
+                                                                           const int* __restrict__ stop) {
   if (stop && *stop) return;  // a device-side Krylov loop has already stopped
   using SH = ApplyShape<N1>;
   constexpr int L = SH::L, NL = SH::NL, NLP = SH::NLP, CPB = SH::CPB;
@@ -471,313 +495,100 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_zwalk_kernel
   __shared__ T YP[CPB * NLP];       // row y+1, later the x partial sums
   const int t = threadIdx.x;
   const int lc = t / L, l = t % L, p = l % N1, q = l / N1;
-  const unsigned ub = (unsigned)xcd_logical_block(blockIdx.x, nblocks);
-  const unsigned nyu = (unsigned)a.n[1], nxc = (unsigned)(a.n[0] / CPB);
-  const unsigned jy = ub % nyu, rest = ub / nyu;
-  const unsigned xc = rest % nxc, seg = rest / nxc;
-  const int nz = a.n[2];
-  const int k0 = (int)seg * seglen;
-  const int k1 = min(nz, k0 + seglen);
-  const long long sY = a.stride[1], sZ = a.stride[2];
-  const long long cb = (long long)xc * CPB + sY * jy;  // first cell of the chunk at kz = 0
-  const bool xlo_ok = xc > 0, xhi_ok = (xc + 1) * CPB < (unsigned)a.n[0];
-  const bool ylo = jy > 0, yhi = (int)jy < a.n[1] - 1;
-  const int gx = (int)xc * CPB + lc;
-  const bool lox = gx > 0, hix = gx < a.n[0] - 1;
-  const bool first = lc == 0, last = lc == CPB - 1;
-  const bool ld_xe = (first && xlo_ok) || (last && xhi_ok);
-  const int off = lc * NL;
-  // the z-line of this lane in the chunk at kz, shifted by dc cells
-  auto zline = [&](int kz, long long dc, T (&u)[N1]) {
-    const T* base = z + (cb + sZ * kz + dc) * NL + off;
-#pragma unroll
-    for (int e = 0; e < N1; ++e) u[e] = base[line_index<N1, 2>(p, q, e)];
-  };
-  T zp[N1], zc[N1], zn[N1], pm[N1], pp[N1], pe[N1];
-  T kc = a.kappa0, klo[3], khi[3], nk[6];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) klo[d] = khi[d] = a.kappa0;
-#pragma unroll
-  for (int i = 0; i < 6; ++i) nk[i] = a.kappa0;
-  // kappa of the cell and its x/y neighbours at step kz (the z neighbours are the cells
-  // of the previous and next steps)
-  auto kload = [&](int kz) {
-    if (VARK) {
-      const T* kb = kappa + cb + sZ * kz + lc;
-      nk[0] = kb[0];
-      if (lox) nk[1] = kb[-1];
-      if (hix) nk[2] = kb[1];
-      if (ylo) nk[3] = kb[-sY];
-      if (yhi) nk[4] = kb[sY];
-      if (kz + 1 < nz) nk[5] = kb[sZ];
-    }
-  };
-  auto rows = [&](int kz) {
-    if (ylo) zline(kz, -sY, pm);
-    if (yhi) zline(kz, sY, pp);
-    if (ld_xe) zline(kz, first ? -1 - lc : CPB - lc, pe);
-    kload(kz);
-  };
-#pragma unroll
-  for (int e = 0; e < N1; ++e) zp[e] = zn[e] = T(0);
-  if (k0 < k1) {  // (an empty segment only takes part in the fused reduction)
-    if (k0 > 0) zline(k0 - 1, 0, zp);
-    zline(k0, 0, zc);
-    if (k0 + 1 < nz) zline(k0 + 1, 0, zn);
-    rows(k0);
-    if (VARK && k0 > 0) kc = kappa[cb + sZ * (k0 - 1) + lc];  // becomes klo[2] of the first step
-  }
-  T pq = T(0);
-  T* Ul = U + (lc + 1) * NLP;
-  T* YMl = YM + lc * NLP;
-  T* YPl = YP + lc * NLP;
-  for (int kz = k0; kz < k1; ++kz) {
-    const bool zlo = kz > 0, zhi = kz < nz - 1;
-    __syncthreads();  // the previous step's reads of the staged data are done
-#pragma unroll
-    for (int e = 0; e < N1; ++e) {
-      const int sl = lds_index<N1, 2>(p, q, e);
-      Ul[sl] = zc[e];
-      if (ylo) YMl[sl] = pm[e];
-      if (yhi) YPl[sl] = pp[e];
-      if (ld_xe) (first ? U : U + (CPB + 1) * NLP)[sl] = pe[e];
-    }
-    if (VARK) {
-      klo[2] = kc;
-      kc = nk[0];
-      klo[0] = nk[1];
-      khi[0] = nk[2];
-      klo[1] = nk[3];
-      khi[1] = nk[4];
-      khi[2] = nk[5];
-    }
-    // the loads of the next step are in flight while this one computes
-    T zf[N1];
-#pragma unroll
-    for (int e = 0; e < N1; ++e) zf[e] = T(0);
-    if (kz + 1 < k1) {
-      if (kz + 2 < nz) zline(kz + 2, 0, zf);
-      rows(kz + 1);
-    }
-    __syncthreads();
-    T sx[N1], sy[N1], sz[N1];
-    line_apply<N1, 2, T, TRANS, CONV, REACT, VARK>(a, lm, zc, zp, zn, zlo, zhi, kc, klo[2], khi[2], sz);
-    {
-      T u[N1], ul[N1], ur[N1];
-#pragma unroll
-      for (int e = 0; e < N1; ++e) {
-        const int sl = lds_index<N1, 1>(p, q, e);
-        u[e] = Ul[sl];
-        ul[e] = ylo ? YMl[sl] : T(0);
-        ur[e] = yhi ? YPl[sl] : T(0);
-      }
-      line_apply<N1, 1, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, ylo, yhi, kc, klo[1], khi[1], sy);
-      const T* xlo = Ul - NLP;
-      const T* xhi = Ul + NLP;
-#pragma unroll
-      for (int e = 0; e < N1; ++e) {
-        const int sl = lds_index<N1, 0>(p, q, e);
-        u[e] = Ul[sl];
-        ul[e] = lox ? xlo[sl] : T(0);
-        ur[e] = hix ? xhi[sl] : T(0);
-      }
-      line_apply<N1, 0, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, lox, hix, kc, klo[0], khi[0], sx);
-    }
-    __syncthreads();  // all reads of the staged cells done
-#pragma unroll
-    for (int e = 0; e < N1; ++e) {
-      YMl[lds_index<N1, 2>(p, q, e)] = sz[e];
-      YPl[lds_index<N1, 0>(p, q, e)] = sx[e];
-    }
-    __syncthreads();
-    {
-      T v[N1];
-#pragma unroll
-      for (int e = 0; e < N1; ++e) {
-        const int sl = lds_index<N1, 1>(p, q, e);
-        v[e] = sy[e] + YMl[sl] + YPl[sl];
-      }
-#pragma unroll
-      for (int i = 0; i < N1; ++i) {
-        T acc = T(0);
-#pragma unroll
-        for (int j = 0; j < N1; ++j) acc += a.M[i][j] * v[j];
-        Ul[lds_index<N1, 1>(p, q, i)] = acc;
-      }
-    }
-    __syncthreads();
-    {
-      T v[N1];
-#pragma unroll
-      for (int e = 0; e < N1; ++e) v[e] = Ul[lds_index<N1, 0>(p, q, e)];
-#pragma unroll
-      for (int i = 0; i < N1; ++i) {
-        T acc = T(0);
-#pragma unroll
-        for (int j = 0; j < N1; ++j) acc += a.M[i][j] * v[j];
-        YMl[lds_index<N1, 0>(p, q, i)] = acc;
-      }
-    }
-    __syncthreads();
-    {
-      T v[N1];
-#pragma unroll
-      for (int e = 0; e < N1; ++e) v[e] = YMl[lds_index<N1, 2>(p, q, e)];
-      T* yc = y + (cb + sZ * kz) * NL + off;
-#pragma unroll
-      for (int i = 0; i < N1; ++i) {
-        T acc = T(0);
-#pragma unroll
-        for (int j = 0; j < N1; ++j) acc += a.M[i][j] * v[j];
-        const int idx = line_index<N1, 2>(p, q, i);
-        if (ACCUM) yc[idx] += acc;
-        else yc[idx] = acc;
-        if (DOT) pq += zc[i] * acc;
-      }
-    }
-#pragma unroll
-    for (int e = 0; e < N1; ++e) {
-      zp[e] = zc[e];
-      zc[e] = zn[e];
-      zn[e] = zf[e];
-    }
-  }
-  if (DOT) {
-    // fused z . (J z) for the Krylov step length (one partial per workgroup)
-    T v[1] = {pq};
-    block_sum_n<SH::THREADS, T, 1>(v);
-    if (threadIdx.x == 0) dotp[blockIdx.x] = v[0];
-  }
-}
-
-#define PDELAB_LDS_AS __attribute__((address_space(3)))
-
-// Copies nbytes contiguous bytes from global memory to LDS with the full waves of the
-// workgroup through direct global->LDS loads (VB bytes per lane, no VGPR round trip).
-template <int VB>
-__device__ __forceinline__ void dma_to_lds(const char* __restrict__ src, char* dst, int nbytes, int wave, int nwaves,
-                                           int lane) {
-  for (int ch = wave; ch * 64 * VB < nbytes; ch += nwaves) {
-    const int byte = ch * 64 * VB + lane * VB;
-    if (byte < nbytes) {
-      if constexpr (VB == 16)
-        __builtin_amdgcn_global_load_lds((const void*)(src + byte), (PDELAB_LDS_AS void*)(dst + ch * 64 * VB), 16, 0, 0);
-      else
-        __builtin_amdgcn_global_load_lds((const void*)(src + byte), (PDELAB_LDS_AS void*)(dst + ch * 64 * VB), 4, 0, 0);
-    }
-  }
-}
-
-// y (+)= J z (or J^T z) on grids with nx % CPB == 0: the CPB cells of a workgroup are
-// one aligned chunk of an x-row, and consecutive logical blocks walk z fastest, then y,
-// then the x-chunk, so the y- and z-neighbour cells were fetched moments earlier by
-// workgroups on the same XCD (L2 hits). The chunk with its two end cells and the
-// chunks of the rows y-1 and y+1 are contiguous in memory; they are copied into LDS
-// with direct global->LDS loads, the z-neighbour lines are read coalesced into
-// registers, and all x- and y-lines are then read from LDS.
-template <int N1, typename T, bool TRANS, bool CONV, bool REACT, bool VARK, bool ACCUM, bool DOT>
-__global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_row_kernel(const T* __restrict__ z,
-                                                                               T* __restrict__ y,
-                                                                               const T* __restrict__ kappa,
-                                                                               const DGArgs<T> a,
-                                                                               const LineMats<T>* __restrict__ lm,
-                                                                               long long nblocks,
-                                                                               T* __restrict__ dotp,
-                                                                               const int* __restrict__ stop) {
-  if (stop && *stop) return;  // a device-side Krylov loop has already stopped
-  using SH = ApplyShape<N1>;
-  constexpr int L = SH::L, NL = SH::NL, CPB = SH::CPB, THREADS = SH::THREADS;
-  constexpr int CB = NL * (int)sizeof(T);  // bytes per cell
-  constexpr int VB = (CB % 16 == 0) ? 16 : 4;
-  constexpr int NW = THREADS / 64;  // full waves
-  __shared__ __attribute__((aligned(16))) T OWN[(CPB + 2) * NL];  // cell c0-1, the chunk, cell c0+CPB
-  __shared__ __attribute__((aligned(16))) T YM[CPB * NL];         // row y-1, later z sums / x sweep
-  __shared__ __attribute__((aligned(16))) T YP[CPB * NL];         // row y+1, later x sums
-  const int t = threadIdx.x;
-  const int lc = t / L, l = t % L, p = l % N1, q = l / N1;
-  const int lane = t & 63, wave = t >> 6;
   const long long blk = xcd_logical_block(blockIdx.x, nblocks);
   const unsigned nzu = (unsigned)a.n[2], nyu = (unsigned)a.n[1];
   const unsigned ub = (unsigned)blk;
   const unsigned kz = ub % nzu, rest = ub / nzu;
   const unsigned jy = rest % nyu, xc = rest / nyu;
   const long long c0 = (long long)xc * CPB + a.stride[1] * jy + a.stride[2] * kz;
-  const bool active = lc < CPB;
-  const int gx = (int)(xc * CPB) + lc;
   const bool xlo_ok = xc > 0, xhi_ok = (long long)(xc + 1) * CPB < a.n[0];
-  const bool ylo = jy > 0, yhi = (int)jy < a.n[1] - 1, zlo = kz > 0, zhi = (int)kz < a.n[2] - 1;
+  const int gx = (int)(xc * CPB) + lc;
+  bool lo[3], hi[3];
+  lo[0] = gx > 0;
+  hi[0] = gx < a.n[0] - 1;
+  lo[1] = jy > 0;
+  hi[1] = (int)jy < a.n[1] - 1;
+  lo[2] = kz > 0;
+  hi[2] = (int)kz < a.n[2] - 1;
+  const int off = lc * NL;  // lane's cell block relative to c0
   const T* zc = z + c0 * NL;
-  if (wave < NW) {
-    dma_to_lds<VB>((const char*)zc, (char*)(OWN + NL), CPB * CB, wave, NW, lane);
-    if (ylo) dma_to_lds<VB>((const char*)(zc - a.stride[1] * NL), (char*)YM, CPB * CB, (wave + 1) % NW, NW, lane);
-    if (yhi) dma_to_lds<VB>((const char*)(zc + a.stride[1] * NL), (char*)YP, CPB * CB, (wave + 2) % NW, NW, lane);
-    if (xlo_ok) dma_to_lds<VB>((const char*)(zc - NL), (char*)OWN, CB, (wave + 3) % NW, NW, lane);
-    if (xhi_ok) dma_to_lds<VB>((const char*)(zc + CPB * NL), (char*)(OWN + (CPB + 1) * NL), CB, wave, NW, lane);
-  }
-  const int off = lc * NL;
-  T zl[N1], zr[N1];
+  T uz[N1], zl[N1], zr[N1], yl[N1], yr[N1], xe[N1];
   T kc = a.kappa0, klo[3], khi[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) klo[d] = khi[d] = a.kappa0;
-  const bool lox = active && gx > 0, hix = active && gx < a.n[0] - 1;
-  if (active) {
-    if (zlo) load_line<N1, 2, T>(zc - a.stride[2] * NL, off, p, q, zl);
-    if (zhi) load_line<N1, 2, T>(zc + a.stride[2] * NL, off, p, q, zr);
-    if (VARK) {
-      const T* kb = kappa + c0;
-      kc = kb[lc];
-      if (lox) klo[0] = kb[lc - 1];
-      if (hix) khi[0] = kb[lc + 1];
-      if (ylo) klo[1] = (kb - a.stride[1])[lc];
-      if (yhi) khi[1] = (kb + a.stride[1])[lc];
-      if (zlo) klo[2] = (kb - a.stride[2])[lc];
-      if (zhi) khi[2] = (kb + a.stride[2])[lc];
+  const bool first = lc == 0, last = lc == CPB - 1;
+  const bool ld_xe = (first && xlo_ok) || (last && xhi_ok);
+  load_line<N1, 2, T>(zc, off, p, q, uz);
+  if (lo[2]) load_line<N1, 2, T>(zc - a.stride[2] * NL, off, p, q, zl);
+  if (hi[2]) load_line<N1, 2, T>(zc + a.stride[2] * NL, off, p, q, zr);
+  if (lo[1]) load_line<N1, 2, T>(zc - a.stride[1] * NL, off, p, q, yl);
+  if (hi[1]) load_line<N1, 2, T>(zc + a.stride[1] * NL, off, p, q, yr);
+  if (ld_xe) load_line<N1, 2, T>(zc + (first ? -NL : NL), off, p, q, xe);
+  if (VARK) {
+    const T* kb = kappa + c0;
+    kc = kb[lc];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const long long sd = d == 0 ? 1 : a.stride[d];
+      if (lo[d]) klo[d] = kb[lc - sd];
+      if (hi[d]) khi[d] = kb[lc + sd];
     }
   }
-  __builtin_amdgcn_s_waitcnt(0);
+  T* Ul = U + (lc + 1) * NLP;
+  T* YMl = YM + lc * NLP;
+  T* YPl = YP + lc * NLP;
+  {
+#pragma unroll
+    for (int e = 0; e < N1; ++e) {
+      const int sl = lds_index<N1, 2>(p, q, e);
+      Ul[sl] = uz[e];
+      if (lo[1]) YMl[sl] = yl[e];
+      if (hi[1]) YPl[sl] = yr[e];
+      if (ld_xe) (first ? U : U + (CPB + 1) * NLP)[sl] = xe[e];
+    }
+  }
   __syncthreads();
-  const T* Ol = OWN + (lc + 1) * NL;
-  T sx[N1], sy[N1], sz[N1], PZ[N1];
-  if (active) {
+  T sx[N1], sy[N1], sz[N1];
+  {
+    line_apply<N1, 2, T, TRANS, CONV, REACT, VARK>(a, lm, uz, zl, zr, lo[2], hi[2], kc, klo[2], khi[2], sz);
     T u[N1], ul[N1], ur[N1];
 #pragma unroll
-    for (int e = 0; e < N1; ++e) u[e] = PZ[e] = Ol[line_index<N1, 2>(p, q, e)];
-    line_apply<N1, 2, T, TRANS, CONV, REACT, VARK>(a, lm, u, zl, zr, zlo, zhi, kc, klo[2], khi[2], sz);
+    for (int e = 0; e < N1; ++e) {
+      const int sl = lds_index<N1, 1>(p, q, e);
+      u[e] = Ul[sl];
+      ul[e] = lo[1] ? YMl[sl] : T(0);
+      ur[e] = hi[1] ? YPl[sl] : T(0);
+    }
+    line_apply<N1, 1, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, lo[1], hi[1], kc, klo[1], khi[1], sy);
+    // x-neighbours: staged cell lc-1 / lc+1 (U[0] and U[CPB+1] hold the end cells)
+    const T* xlo = Ul - NLP;
+    const T* xhi = Ul + NLP;
 #pragma unroll
     for (int e = 0; e < N1; ++e) {
-      const int sl = line_index<N1, 1>(p, q, e);
-      u[e] = Ol[sl];
-      ul[e] = ylo ? YM[off + sl] : T(0);
-      ur[e] = yhi ? YP[off + sl] : T(0);
+      const int sl = lds_index<N1, 0>(p, q, e);
+      u[e] = Ul[sl];
+      ul[e] = lo[0] ? xlo[sl] : T(0);
+      ur[e] = hi[0] ? xhi[sl] : T(0);
     }
-    line_apply<N1, 1, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, ylo, yhi, kc, klo[1], khi[1], sy);
-#pragma unroll
-    for (int e = 0; e < N1; ++e) {
-      const int sl = line_index<N1, 0>(p, q, e);
-      u[e] = Ol[sl];
-      ul[e] = lox ? Ol[sl - NL] : T(0);
-      ur[e] = hix ? Ol[sl + NL] : T(0);
-    }
-    line_apply<N1, 0, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, lox, hix, kc, klo[0], khi[0], sx);
+    line_apply<N1, 0, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, lo[0], hi[0], kc, klo[0], khi[0], sx);
   }
   __syncthreads();  // all reads of the staged cells done
   T pq = T(0);
-  T* YMl = YM + off;
-  T* YPl = YP + off;
-  T* Ow = OWN + (lc + 1) * NL;
-  if (active) {
+  const T* PZ = uz;
+  {
 #pragma unroll
     for (int e = 0; e < N1; ++e) {
-      YMl[line_index<N1, 2>(p, q, e)] = sz[e];
-      YPl[line_index<N1, 0>(p, q, e)] = sx[e];
+      YMl[lds_index<N1, 2>(p, q, e)] = sz[e];
+      YPl[lds_index<N1, 0>(p, q, e)] = sx[e];
     }
   }
   __syncthreads();
-  if (active) {
+  {
+    // y-layout: s = sy + sz + sx, then the mass along y
     T v[N1];
 #pragma unroll
     for (int e = 0; e < N1; ++e) {
-      const int sl = line_index<N1, 1>(p, q, e);
+      const int sl = lds_index<N1, 1>(p, q, e);
       v[e] = sy[e] + YMl[sl] + YPl[sl];
     }
 #pragma unroll
@@ -785,27 +596,27 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_row_kernel(c
       T acc = T(0);
 #pragma unroll
       for (int j = 0; j < N1; ++j) acc += a.M[i][j] * v[j];
-      Ow[line_index<N1, 1>(p, q, i)] = acc;
+      Ul[lds_index<N1, 1>(p, q, i)] = acc;
     }
   }
   __syncthreads();
-  if (active) {
+  {
     T v[N1];
 #pragma unroll
-    for (int e = 0; e < N1; ++e) v[e] = Ow[line_index<N1, 0>(p, q, e)];
+    for (int e = 0; e < N1; ++e) v[e] = Ul[lds_index<N1, 0>(p, q, e)];
 #pragma unroll
     for (int i = 0; i < N1; ++i) {
       T acc = T(0);
 #pragma unroll
       for (int j = 0; j < N1; ++j) acc += a.M[i][j] * v[j];
-      YMl[line_index<N1, 0>(p, q, i)] = acc;
+      YMl[lds_index<N1, 0>(p, q, i)] = acc;
     }
   }
   __syncthreads();
-  if (active) {
+  {
     T v[N1];
 #pragma unroll
-    for (int e = 0; e < N1; ++e) v[e] = YMl[line_index<N1, 2>(p, q, e)];
+    for (int e = 0; e < N1; ++e) v[e] = YMl[lds_index<N1, 2>(p, q, e)];
     T* yc = y + c0 * NL;
 #pragma unroll
     for (int i = 0; i < N1; ++i) {

@@ -1,0 +1,104 @@
+// handle.hh -- the state behind one pdelab_gpu_gridoperator handle (internal to the
+// library) and the per-degree apply launcher shared by its translation units.
+#ifndef PDELAB_HANDLE_HH
+#define PDELAB_HANDLE_HH
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+
+#include "../../include/pdelab_gpu.h"
+#include "dg_kernels.hh"
+#include "dg_load.hh"
+#include "dg_tables.hh"
+
+using namespace pdelab_amd;
+
+struct pdelab_gpu_gridoperator {
+  pdelab_config cfg;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  Tables1D tab;
+  DGArgs<double> a64;
+  DGArgs<float> a32;
+  LoadArgs la;
+  GradArgs ga;
+  long long ncells = 0, nloc = 0, N = 0;
+  int n1 = 2;
+  bool conv = false, vark = false;
+  double* d_kappa = nullptr;
+  float* d_kappa32 = nullptr;
+  double* d_R0 = nullptr;
+  bool R0_valid = false;
+  double* d_src = nullptr;
+  double* d_ftab = nullptr;
+  double* d_gtab = nullptr;
+  double* d_diag = nullptr;
+  double* d_dinv = nullptr;
+  float* d_dinv32 = nullptr;
+  bool diag_valid = false;
+  double* w64[8] = {nullptr};
+  float* w32[8] = {nullptr};
+  double* d_part64 = nullptr;
+  float* d_part32 = nullptr;
+  double* d_scal64 = nullptr;
+  float* d_scal32 = nullptr;
+  double* h_scal64 = nullptr;
+  float* h_scal32 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double* d_applydot = nullptr;  // per-workgroup partials of the fused apply dot
+  int* d_state = nullptr;        // device Krylov loop: {stopped, aborted, iteration, -}
+  double* d_dstate = nullptr;    // {defect, def0, reduction, -}
+  int* h_state = nullptr;        // pinned mirrors
+  double* h_dstate = nullptr;
+  // constant-coefficient line matrices: [0] J, [1] J^T
+  LineMats<double>* d_lm64 = nullptr;
+  LineMats<float>* d_lm32 = nullptr;
+  bool sym[2] = {false, false};  // J / J^T have the structure of line_op_sym
+  // apply kernel on aligned-row grids: 0 the aligned-row kernel, 1 the general kernel
+  int apply_variant = 0;
+};
+
+using H = pdelab_gpu_gridoperator;
+
+#define HIPCHK(h, expr)                                                              \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      (h)->err = std::string(#expr " failed: ") + hipGetErrorString(e_);             \
+      return PDELAB_ERR_HIP;                                                         \
+    }                                                                                \
+  } while (0)
+
+#define CHK(...)                     \
+  do {                               \
+    int s_ = (__VA_ARGS__);          \
+    if (s_ != PDELAB_OK) return s_;  \
+  } while (0)
+
+inline int fail(H* h, int code, const std::string& msg) {
+  if (h) h->err = msg;
+  return code;
+}
+
+inline int grid_for(long long n) {
+  long long b = (n + 255) / 256;
+  return (int)std::max(1LL, std::min(b, 4096LL * 8));
+}
+
+template <typename T>
+inline const LineMats<T>* line_mats(H* h, bool trans);
+template <>
+inline const LineMats<double>* line_mats<double>(H* h, bool trans) { return h->d_lm64 + (trans ? 1 : 0); }
+template <>
+inline const LineMats<float>* line_mats<float>(H* h, bool trans) { return h->d_lm32 + (trans ? 1 : 0); }
+
+// y (+)= J z (or J^T z) on the handle's stream for Q_{N1-1}: mode 0 y = J z, 1 y += J z,
+// 2 y = J z with the fused z.(Jz) partials in dotp. Defined in apply_q<N1-1>.hip.
+template <int N1, typename T>
+int launch_apply_n(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, bool trans, int mode, T* dotp,
+                   const int* stop);
+
+#endif

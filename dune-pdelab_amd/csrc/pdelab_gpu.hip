@@ -14,13 +14,8 @@
 #include <type_traits>
 #include <vector>
 
-#include "../../include/pdelab_gpu.h"
 #include "blas_kernels.hh"
-#include "dg_kernels.hh"
-#include "dg_load.hh"
-#include "dg_tables.hh"
-
-using namespace pdelab_amd;
+#include "handle.hh"
 
 namespace {
 thread_local std::string g_create_error;
@@ -29,82 +24,6 @@ struct Workspace {
   std::vector<void*> bufs;
 };
 }  // namespace
-
-struct pdelab_gpu_gridoperator {
-  pdelab_config cfg;
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
-  Tables1D tab;
-  DGArgs<double> a64;
-  DGArgs<float> a32;
-  LoadArgs la;
-  GradArgs ga;
-  long long ncells = 0, nloc = 0, N = 0;
-  int n1 = 2;
-  bool conv = false, vark = false;
-  double* d_kappa = nullptr;
-  float* d_kappa32 = nullptr;
-  double* d_R0 = nullptr;
-  bool R0_valid = false;
-  double* d_src = nullptr;
-  double* d_ftab = nullptr;
-  double* d_gtab = nullptr;
-  double* d_diag = nullptr;
-  double* d_dinv = nullptr;
-  float* d_dinv32 = nullptr;
-  bool diag_valid = false;
-  double* w64[8] = {nullptr};
-  float* w32[8] = {nullptr};
-  double* d_part64 = nullptr;
-  float* d_part32 = nullptr;
-  double* d_scal64 = nullptr;
-  float* d_scal32 = nullptr;
-  double* h_scal64 = nullptr;
-  float* h_scal32 = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  double* d_applydot = nullptr;  // per-workgroup partials of the fused apply dot
-  int* d_state = nullptr;        // device Krylov loop: {stopped, aborted, iteration, -}
-  double* d_dstate = nullptr;    // {defect, def0, reduction, -}
-  int* h_state = nullptr;        // pinned mirrors
-  double* h_dstate = nullptr;
-  // constant-coefficient line matrices: [0] J, [1] J^T
-  LineMats<double>* d_lm64 = nullptr;
-  LineMats<float>* d_lm32 = nullptr;
-  // apply kernel on aligned-row grids: 1 one aligned chunk per workgroup (default),
-  // 0 z-walk (pipelined along z), 2 the general kernel; z-walk segment length and
-  // workgroup count
-  int apply_variant = 0;
-  int seglen = 0;
-  long long zwalk_blocks = 0;
-};
-
-using H = pdelab_gpu_gridoperator;
-
-#define HIPCHK(h, expr)                                                              \
-  do {                                                                               \
-    hipError_t e_ = (expr);                                                          \
-    if (e_ != hipSuccess) {                                                          \
-      (h)->err = std::string(#expr " failed: ") + hipGetErrorString(e_);             \
-      return PDELAB_ERR_HIP;                                                         \
-    }                                                                                \
-  } while (0)
-
-#define CHK(...)                     \
-  do {                               \
-    int s_ = (__VA_ARGS__);          \
-    if (s_ != PDELAB_OK) return s_;  \
-  } while (0)
-
-static int fail(H* h, int code, const std::string& msg) {
-  if (h) h->err = msg;
-  return code;
-}
-
-static int grid_for(long long n) {
-  long long b = (n + 255) / 256;
-  return (int)std::max(1LL, std::min(b, 4096LL * 8));
-}
 
 // ---------------------------------------------------------------------------
 // setup of the kernel argument blocks
@@ -149,6 +68,7 @@ static void fill_args(H* h, DGArgs<T>& a) {
   }
   a.V = (T)V;
   a.c = (T)c.c;
+  a.vc = (T)(V * c.c);
   // convectiondiffusiondg.hh:99-101
   double theta = 1.0;
   if (c.method == PDELAB_SIPG) theta = -1.0;
@@ -259,6 +179,46 @@ static void cast_line_mats(const LineMats<LD>& m, LineMats<T>& o) {
   for (size_t i = 0; i < sizeof(LineMats<LD>) / sizeof(LD); ++i) dst[i] = (T)src[i];
 }
 
+// The structure used by line_op_sym: A_d = S_d Ah (+ V c I for d = 0), Pl_d = S_d Ph,
+// Ql_d = S_d Qh, Pr_d[i] = S_d Ph[n-1-i], Qr_d[i] = -S_d Qh[n-1-i], Ah centro-symmetric
+// and d0[j] = -d1[n-1-j]; it holds for constant kappa without convection (the
+// operator commutes with the reflection x -> 1-x of every direction).
+static bool symmetric_form(const DGArgs<LD>& al, const LineMats<LD>& m, int n, LD (&Ah)[4][4], LD (&Ph)[4],
+                           LD (&Qh)[4]) {
+  std::memset(Ah, 0, sizeof(Ah));
+  std::memset(Ph, 0, sizeof(Ph));
+  std::memset(Qh, 0, sizeof(Qh));
+  const LD S1 = al.S[1];
+  LD scale = 0, pscale = 0;
+  for (int i = 0; i < n; ++i) {
+    for (int j = 0; j < n; ++j) {
+      Ah[i][j] = m.A[1][i][j] / S1;
+      scale = std::max(scale, std::fabs(Ah[i][j]));
+    }
+    Ph[i] = m.Pl[1][i] / S1;
+    Qh[i] = m.Ql[1][i] / S1;
+    pscale = std::max(pscale, std::max(std::fabs(Ph[i]), std::fabs(Qh[i])));
+  }
+  const LD tol = 1e-12L;
+  auto close = [&](LD x, LD y, LD sc) { return std::fabs(x - y) <= tol * sc; };
+  for (int j = 0; j < n; ++j)
+    if (!close(al.d0[j], -al.d1[n - 1 - j], std::fabs(al.d1[n - 1 - j]) + 1)) return false;
+  for (int d = 0; d < 3; ++d) {
+    const LD Sd = al.S[d];
+    for (int i = 0; i < n; ++i) {
+      for (int j = 0; j < n; ++j) {
+        const LD a = m.A[d][i][j] - (d == 0 && i == j ? al.vc : 0);
+        if (!close(a, Sd * Ah[i][j], Sd * scale)) return false;
+        if (!close(Ah[i][j], Ah[n - 1 - i][n - 1 - j], scale)) return false;
+      }
+      if (!close(m.Pl[d][i], Sd * Ph[i], Sd * pscale) || !close(m.Ql[d][i], Sd * Qh[i], Sd * pscale)) return false;
+      if (!close(m.Pr[d][i], Sd * Ph[n - 1 - i], Sd * pscale) || !close(m.Qr[d][i], -Sd * Qh[n - 1 - i], Sd * pscale))
+        return false;
+    }
+  }
+  return true;
+}
+
 static int build_line_mats(H* h) {
   DGArgs<LD> al;
   fill_args<LD>(h, al);
@@ -270,6 +230,20 @@ static int build_line_mats(H* h) {
     default: ok = build_line_mats_n<4, false>(al, m[0]) && build_line_mats_n<4, true>(al, m[1]); break;
   }
   if (!ok) return fail(h, PDELAB_ERR_INVALID_ARG, "constant-coefficient line operator is not of rank-two face form");
+  for (int t = 0; t < 2; ++t) {
+    LD Ah[4][4], Ph[4], Qh[4];
+    h->sym[t] = symmetric_form(al, m[t], h->n1, Ah, Ph, Qh);
+    for (int i = 0; i < 4; ++i) {
+      h->a64.Ph[t][i] = (double)Ph[i];
+      h->a64.Qh[t][i] = (double)Qh[i];
+      h->a32.Ph[t][i] = (float)Ph[i];
+      h->a32.Qh[t][i] = (float)Qh[i];
+      for (int j = 0; j < 4; ++j) {
+        h->a64.Ah[t][i][j] = (double)Ah[i][j];
+        h->a32.Ah[t][i][j] = (float)Ah[i][j];
+      }
+    }
+  }
   LineMats<double> m64[2];
   LineMats<float> m32[2];
   for (int t = 0; t < 2; ++t) { cast_line_mats(m[t], m64[t]); cast_line_mats(m[t], m32[t]); }
@@ -333,74 +307,6 @@ static void fill_load_args(H* h) {
 
 // ---------------------------------------------------------------------------
 // kernel dispatch
-template <typename T>
-static const LineMats<T>* line_mats(H* h, bool trans);
-template <>
-const LineMats<double>* line_mats<double>(H* h, bool trans) { return h->d_lm64 + (trans ? 1 : 0); }
-template <>
-const LineMats<float>* line_mats<float>(H* h, bool trans) { return h->d_lm32 + (trans ? 1 : 0); }
-
-template <int N1, typename T, bool TRANS, bool CONV, bool REACT, bool VARK, bool ACCUM, bool DOT>
-static int launch_apply_k(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, T* dotp, const int* stop) {
-  using SH = ApplyShape<N1>;
-  const long long nb = (h->ncells + SH::CPB - 1) / SH::CPB;
-  const LineMats<T>* lm = line_mats<T>(h, TRANS);
-  if (a.rowmode && h->apply_variant == 0)
-    hipLaunchKernelGGL((dg_apply_zwalk_kernel<N1, T, TRANS, CONV, REACT, VARK, ACCUM, DOT>),
-                       dim3((unsigned)h->zwalk_blocks), dim3(SH::THREADS), 0, h->stream, z, y, kappa, a, lm,
-                       h->zwalk_blocks, h->seglen, dotp, stop);
-  else if (a.rowmode && h->apply_variant == 1)
-    hipLaunchKernelGGL((dg_apply_row_kernel<N1, T, TRANS, CONV, REACT, VARK, ACCUM, DOT>), dim3((unsigned)nb),
-                       dim3(SH::THREADS), 0, h->stream, z, y, kappa, a, lm, nb, dotp, stop);
-  else
-    hipLaunchKernelGGL((dg_apply_kernel<N1, T, TRANS, CONV, REACT, VARK, ACCUM, DOT>), dim3((unsigned)nb),
-                       dim3(SH::THREADS), 0, h->stream, z, y, kappa, a, lm, nb, dotp, stop);
-  HIPCHK(h, hipGetLastError());
-  return PDELAB_OK;
-}
-
-// Cell-wise kappa uses the general line operator (instantiated per convection/reaction
-// case); constant coefficients use the precomputed line matrices, which already hold
-// the convection and reaction terms.
-template <int N1, typename T, bool TRANS, bool CONV, bool VARK, bool ACCUM, bool DOT>
-static int launch_apply_t(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, bool react, T* dotp,
-                          const int* stop) {
-  if constexpr (!VARK && sizeof(T) == 4) {
-    return launch_apply_k<N1, T, TRANS, false, false, false, ACCUM, DOT>(h, z, y, kappa, a, dotp, stop);
-  } else {
-    if (react) return launch_apply_k<N1, T, TRANS, CONV, true, VARK, ACCUM, DOT>(h, z, y, kappa, a, dotp, stop);
-    return launch_apply_k<N1, T, TRANS, CONV, false, VARK, ACCUM, DOT>(h, z, y, kappa, a, dotp, stop);
-  }
-}
-
-// mode 0: y = J z; 1: y += J z; 2: y = J z with the fused z.(Jz) partials
-template <int N1, typename T, bool TRANS, bool CONV, bool VARK>
-static int launch_apply_a(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, bool react, int mode, T* dotp,
-                          const int* stop) {
-  if (mode == 1) return launch_apply_t<N1, T, TRANS, CONV, VARK, true, false>(h, z, y, kappa, a, react, nullptr, stop);
-  if (mode == 2) return launch_apply_t<N1, T, TRANS, CONV, VARK, false, true>(h, z, y, kappa, a, react, dotp, stop);
-  return launch_apply_t<N1, T, TRANS, CONV, VARK, false, false>(h, z, y, kappa, a, react, nullptr, stop);
-}
-template <int N1, typename T, bool TRANS, bool CONV>
-static int launch_apply_v(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, bool react, int mode, T* dotp,
-                          const int* stop) {
-  return h->vark ? launch_apply_a<N1, T, TRANS, CONV, true>(h, z, y, kappa, a, react, mode, dotp, stop)
-                 : launch_apply_a<N1, T, TRANS, CONV, false>(h, z, y, kappa, a, react, mode, dotp, stop);
-}
-template <int N1, typename T, bool TRANS>
-static int launch_apply_c(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, bool react, int mode, T* dotp,
-                          const int* stop) {
-  return h->conv ? launch_apply_v<N1, T, TRANS, true>(h, z, y, kappa, a, react, mode, dotp, stop)
-                 : launch_apply_v<N1, T, TRANS, false>(h, z, y, kappa, a, react, mode, dotp, stop);
-}
-template <int N1, typename T>
-static int launch_apply_n(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, bool trans, int mode, T* dotp,
-                          const int* stop) {
-  const bool react = h->cfg.c != 0.0;
-  return trans ? launch_apply_c<N1, T, true>(h, z, y, kappa, a, react, mode, dotp, stop)
-               : launch_apply_c<N1, T, false>(h, z, y, kappa, a, react, mode, dotp, stop);
-}
-
 static int check_outflow(H* h) {
   for (int s = 0; s < 6; ++s) {
     if (h->a64.bc[s] != BC_OUTFLOW) continue;
@@ -452,36 +358,15 @@ static int apply_cpb(int n1) {
 
 // number of workgroups (= fused-dot partials) of one apply launch
 static long long apply_blocks(H* h) {
-  if (h->a64.rowmode && h->apply_variant == 0) return h->zwalk_blocks;
   const int cpb = apply_cpb(h->n1);
   return (h->ncells + cpb - 1) / cpb;
 }
 
-// Launch shape of the z-walk kernel: the x-row chunks times z segments, with about
-// `per_cu` workgroups per compute unit so that one dispatch round covers the grid.
-// PDELAB_GPU_APPLY=zwalk selects the z-walk kernel on aligned-row grids (=general the
-// general kernel), and PDELAB_GPU_SEGLEN fixes its segment length.
+// PDELAB_GPU_APPLY=general runs the general apply kernel on aligned-row grids as well
+// (for comparison; the aligned-row kernel is the default there).
 static void choose_apply_shape(H* h) {
-  const pdelab_config& c = h->cfg;
   const char* v = std::getenv("PDELAB_GPU_APPLY");
-  h->apply_variant = 1;
-  if (v && std::strcmp(v, "zwalk") == 0) h->apply_variant = 0;
-  if (v && std::strcmp(v, "general") == 0) h->apply_variant = 2;
-  const long long ncols = (long long)(c.nx / apply_cpb(h->n1)) * c.ny;
-  int cus = 256;
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, h->device) == hipSuccess && prop.multiProcessorCount > 0)
-    cus = prop.multiProcessorCount;
-  const long long per_cu = 4;
-  long long nseg = std::max(1LL, (cus * per_cu + ncols - 1) / std::max(ncols, 1LL));
-  nseg = std::min<long long>(nseg, c.nz);
-  int seglen = (int)((c.nz + nseg - 1) / nseg);
-  if (const char* sl = std::getenv("PDELAB_GPU_SEGLEN")) {
-    const int v2 = std::atoi(sl);
-    if (v2 > 0) seglen = std::min(v2, c.nz);
-  }
-  h->seglen = seglen;
-  h->zwalk_blocks = ncols * ((c.nz + seglen - 1) / seglen);
+  h->apply_variant = (v && std::strcmp(v, "general") == 0) ? 1 : 0;
 }
 
 static int compute_R0(H* h) {

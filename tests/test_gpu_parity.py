@@ -92,36 +92,34 @@ def test_apply_diag_transpose_mass(pkg, of, name, kw, seed):
     assert rel(y32.download().astype(np.float64), o.jacobian_apply(z)) <= 2e-5
 
 
-# Aligned-row grids through both aligned-row apply kernels: the z-walk kernel with segment
-# lengths that put segment boundaries inside the grid (and the default launch shape), and
-# the one-chunk-per-workgroup kernel. SPD cases also run CG, whose iterations use the
-# apply with the fused z.(Jz) partials.
-VARIANTS = [
-    ("zwalk_q3_16x3x5_seg2_var_conv", dict(nx=16, ny=3, nz=5, degree=3, alpha=3.0, b=(0.5, -1.0, 0.25), c=0.2),
-     21, "zwalk", 2),
-    ("zwalk_q3_16x2x6_seg4_react", dict(nx=16, ny=2, nz=6, degree=3, alpha=3.0, c=0.5), None, "zwalk", 4),
-    ("zwalk_q3_16x4x5_seg2_var", dict(nx=16, ny=4, nz=5, degree=3, alpha=3.0), 24, "zwalk", 2),
-    ("zwalk_q2_32x2x7_seg3", dict(nx=32, ny=2, nz=7, degree=2, alpha=3.0), None, "zwalk", 3),
-    ("zwalk_q2_64x3x3_seg1_nipg_var", dict(nx=64, ny=3, nz=3, degree=2, method=0, alpha=2.0), 22, "zwalk", 1),
-    ("zwalk_q1_64x3x4_seg1_var_conv", dict(nx=64, ny=3, nz=4, degree=1, alpha=3.0, b=(0.0, 1.0, -1.0)), 23,
-     "zwalk", 1),
-    ("zwalk_q3_32x1x3_neumann", dict(nx=32, ny=1, nz=3, degree=3, alpha=3.0, bctype=(0, -1, 0, 0, -1, 0)), None,
-     "zwalk", None),
-    ("zwalk_q1_64x1x1_off", dict(nx=64, ny=1, nz=1, degree=1, weights=1, alpha=3.0), None, "zwalk", None),
-    ("zwalk_q2_32x5x9_default", dict(nx=32, ny=5, nz=9, degree=2, alpha=3.0), None, "zwalk", None),
-    ("row_q3_16x3x5_var_conv", dict(nx=16, ny=3, nz=5, degree=3, alpha=3.0, b=(0.5, -1.0, 0.25)), 21, "row", None),
-    ("row_q2_32x2x3", dict(nx=32, ny=2, nz=3, degree=2, alpha=3.0), None, "row", None),
+# Aligned-row grids (nx a multiple of the workgroup chunk: 16 cells Q3, 32 Q2, 64 Q1)
+# through the aligned-row kernel (default) and through the general kernel. SPD cases also
+# run CG, whose iterations use the apply with the fused z.(Jz) partials.
+ALIGNED = [
+    ("q3_16x3x5_var_conv", dict(nx=16, ny=3, nz=5, degree=3, alpha=3.0, b=(0.5, -1.0, 0.25), c=0.2), 21),
+    ("q3_16x2x6_react", dict(nx=16, ny=2, nz=6, degree=3, alpha=3.0, c=0.5), None),
+    ("q3_16x4x5_var", dict(nx=16, ny=4, nz=5, degree=3, alpha=3.0), 24),
+    ("q3_32x2x3", dict(nx=32, ny=2, nz=3, degree=3, alpha=3.0), None),
+    ("q3_32x1x3_neumann", dict(nx=32, ny=1, nz=3, degree=3, alpha=3.0, bctype=(0, -1, 0, 0, -1, 0)), None),
+    ("q3_16x3x2_aniso_iipg", dict(nx=16, ny=3, nz=2, degree=3, method=2, lx=2.0, ly=0.5, lz=1.0, alpha=4.0), None),
+    ("q2_32x2x7", dict(nx=32, ny=2, nz=7, degree=2, alpha=3.0), None),
+    ("q2_64x3x3_nipg_var", dict(nx=64, ny=3, nz=3, degree=2, method=0, alpha=2.0), 22),
+    ("q2_32x5x9_off", dict(nx=32, ny=5, nz=9, degree=2, weights=1, alpha=3.0), None),
+    ("q1_64x3x4_var_conv", dict(nx=64, ny=3, nz=4, degree=1, alpha=3.0, b=(0.0, 1.0, -1.0)), 23),
+    ("q1_64x1x1_off", dict(nx=64, ny=1, nz=1, degree=1, weights=1, alpha=3.0), None),
+    ("q1_64x2x3_react_outflow", dict(nx=64, ny=2, nz=3, degree=1, alpha=3.0, c=0.4, b=(1.0, 0.0, 0.0),
+                                     bctype=(0, -2, 0, 0, -1, 0)), None),
 ]
+VARIANTS = [(f"{k}_{name}", kw, seed, k) for name, kw, seed in ALIGNED for k in ("aligned", "general")]
 
 
-@pytest.mark.parametrize("name,kw,seed,variant,seglen", VARIANTS, ids=[v[0] for v in VARIANTS])
-def test_apply_kernel_variants(pkg, of, monkeypatch, name, kw, seed, variant, seglen):
+@pytest.mark.parametrize("name,kw,seed,variant", VARIANTS, ids=[v[0] for v in VARIANTS])
+def test_apply_kernel_variants(pkg, of, monkeypatch, name, kw, seed, variant):
     say(f"[gpu] apply kernel variant case={name}")
-    monkeypatch.setenv("PDELAB_GPU_APPLY", variant)
-    if seglen:
-        monkeypatch.setenv("PDELAB_GPU_SEGLEN", str(seglen))
+    if variant == "general":
+        monkeypatch.setenv("PDELAB_GPU_APPLY", "general")
     else:
-        monkeypatch.delenv("PDELAB_GPU_SEGLEN", raising=False)
+        monkeypatch.delenv("PDELAB_GPU_APPLY", raising=False)
     cfg, kappa, go, o = make(pkg, of, kw, seed)
     rng = np.random.default_rng(43)
     z = rng.uniform(-1, 1, go.N)
