@@ -104,7 +104,7 @@ struct ApplyShape {
 };
 
 template <int N1, int D>
-__device__ __forceinline__ int lds_index(int p, int q, int e) {
+__host__ __device__ constexpr int lds_index(int p, int q, int e) {
   // (x, y, z) of element e of the lane's line in layout D
   const int x = D == 0 ? e : p;
   const int yy = D == 0 ? p : (D == 1 ? e : q);
@@ -113,6 +113,34 @@ __device__ __forceinline__ int lds_index(int p, int q, int e) {
   if (xs >= N1) xs -= N1;
   if (ys >= N1) ys -= N1;
   return xs + N1 * ys + N1 * N1 * zz;
+}
+
+// lds_index of every lane of a cell: [layout D][lane l = p + N1 q][element e], read by the
+// aligned-row kernel instead of evaluating the rotation per access.
+template <int N1>
+struct SlotTable {
+  int s[3][N1 * N1][4];
+};
+template <int N1>
+constexpr SlotTable<N1> make_slot_table() {
+  SlotTable<N1> t{};
+  for (int l = 0; l < N1 * N1; ++l)
+    for (int e = 0; e < 4; ++e) {
+      const int p = l % N1, q = l / N1, ee = e < N1 ? e : 0;
+      t.s[0][l][e] = lds_index<N1, 0>(p, q, ee);
+      t.s[1][l][e] = lds_index<N1, 1>(p, q, ee);
+      t.s[2][l][e] = lds_index<N1, 2>(p, q, ee);
+    }
+  return t;
+}
+static __constant__ const SlotTable<2> kSlots2 = make_slot_table<2>();
+static __constant__ const SlotTable<3> kSlots3 = make_slot_table<3>();
+static __constant__ const SlotTable<4> kSlots4 = make_slot_table<4>();
+template <int N1>
+__device__ __forceinline__ const SlotTable<N1>& slot_table() {
+  if constexpr (N1 == 2) return kSlots2;
+  else if constexpr (N1 == 3) return kSlots3;
+  else return kSlots4;
 }
 
 // s = M^{-1} (A_d u) on one grid line of a cell (or M^{-1} A_d^T u for TRANS), from the
@@ -243,7 +271,10 @@ __device__ __forceinline__ void line_op_sym(const DGArgs<T>& a, const LineMats<T
 #pragma unroll
     for (int j = 0; j < N1; ++j) dl += a.d1[j] * ul[j];
 #pragma unroll
-    for (int i = 0; i < N1; ++i) acc[i] += a.Ph[tr][i] * vl + a.Qh[tr][i] * dl;
+    for (int i = 0; i < N1; ++i) {
+      acc[i] += a.Ph[tr][i] * vl;
+      acc[i] += a.Qh[tr][i] * dl;
+    }
   }
   if (has_hi) {
     const T vr = ur[0];
@@ -251,7 +282,10 @@ __device__ __forceinline__ void line_op_sym(const DGArgs<T>& a, const LineMats<T
 #pragma unroll
     for (int j = 0; j < N1; ++j) dr += a.d1[N1 - 1 - j] * ur[j];
 #pragma unroll
-    for (int i = 0; i < N1; ++i) acc[i] += a.Ph[tr][N1 - 1 - i] * vr + a.Qh[tr][N1 - 1 - i] * dr;
+    for (int i = 0; i < N1; ++i) {
+      acc[i] += a.Ph[tr][N1 - 1 - i] * vr;
+      acc[i] += a.Qh[tr][N1 - 1 - i] * dr;
+    }
   }
   const T Sd = a.S[D];
 #pragma unroll
@@ -476,23 +510,21 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
 // cells of a workgroup are one aligned chunk of an x-row (so the y and z boundary flags
 // are uniform over the workgroup), consecutive logical blocks walk z fastest, then y,
 // then the x-chunk, and every global access is a coalesced z-line access. The chunk,
-// its two end cells and the chunks of the rows y-1 and y+1 are written to LDS in the
-// rotated, conflict-free layout of lds_index; the z-neighbour lines stay in registers.
+// its two end cells and the chunks of the rows y-1 and y+1 are written to one LDS
+// array in the rotated, conflict-free cell layout of lds_index; each lane reads its
+// twelve slot offsets once from SlotTable, so every LDS access is a register plus an
+// immediate offset. The z-neighbour lines stay in registers.
 template <int N1, typename T, bool TRANS, bool CONV, bool REACT, bool VARK, bool ACCUM, bool DOT>
-__global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kernel(const T* __restrict__ z,
-                                                                           T* __restrict__ y,
-                                                                           const T* __restrict__ kappa,
-                                                                           const DGArgs<T> a,
-                                                                           const LineMats<T>* __restrict__ lm,
-                                                                           long long nblocks,
-                                                                           T* __restrict__ dotp,
-                                                                           const int* __restrict__ stop) {
+__global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kernel(
+    const T* __restrict__ z, T* __restrict__ y, const T* __restrict__ kappa, const DGArgs<T> a,
+    const LineMats<T>* __restrict__ lm, long long nblocks, T* __restrict__ dotp, const int* __restrict__ stop) {
   if (stop && *stop) return;  // a device-side Krylov loop has already stopped
   using SH = ApplyShape<N1>;
   constexpr int L = SH::L, NL = SH::NL, NLP = SH::NLP, CPB = SH::CPB;
-  __shared__ T U[(CPB + 2) * NLP];  // [0] cell before the chunk, [1..CPB] own cells, [CPB+1] cell after
-  __shared__ T YM[CPB * NLP];       // row y-1, later the z partial sums and the x mass sweep
-  __shared__ T YP[CPB * NLP];       // row y+1, later the x partial sums
+  // cell before the chunk, the CPB own cells, cell after the chunk (from UO - NLP on);
+  // the chunks of the rows y-1 (YMO) and y+1 (YPO), later the partial sums and sweeps
+  constexpr int UO = NLP, YMO = (CPB + 2) * NLP, YPO = (2 * CPB + 2) * NLP;
+  __shared__ T LS[(3 * CPB + 2) * NLP];
   const int t = threadIdx.x;
   const int lc = t / L, l = t % L, p = l % N1, q = l / N1;
   const long long blk = xcd_logical_block(blockIdx.x, nblocks);
@@ -534,17 +566,26 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
       if (hi[d]) khi[d] = kb[lc + sd];
     }
   }
-  T* Ul = U + (lc + 1) * NLP;
-  T* YMl = YM + lc * NLP;
-  T* YPl = YP + lc * NLP;
+  // slot of element e of the lane's x-, y- and z-line, relative to the cell region
+  int ax[N1], ay[N1], az[N1];
   {
+    const SlotTable<N1>& st = slot_table<N1>();
+    const int cb = lc * NLP;
 #pragma unroll
     for (int e = 0; e < N1; ++e) {
-      const int sl = lds_index<N1, 2>(p, q, e);
-      Ul[sl] = uz[e];
-      if (lo[1]) YMl[sl] = yl[e];
-      if (hi[1]) YPl[sl] = yr[e];
-      if (ld_xe) (first ? U : U + (CPB + 1) * NLP)[sl] = xe[e];
+      ax[e] = cb + st.s[0][l][e];
+      ay[e] = cb + st.s[1][l][e];
+      az[e] = cb + st.s[2][l][e];
+    }
+  }
+  {
+    const int xo = first ? UO - NLP : UO + NLP;
+#pragma unroll
+    for (int e = 0; e < N1; ++e) {
+      LS[UO + az[e]] = uz[e];
+      if (lo[1]) LS[YMO + az[e]] = yl[e];
+      if (hi[1]) LS[YPO + az[e]] = yr[e];
+      if (ld_xe) LS[xo + az[e]] = xe[e];
     }
   }
   __syncthreads();
@@ -554,69 +595,59 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
     T u[N1], ul[N1], ur[N1];
 #pragma unroll
     for (int e = 0; e < N1; ++e) {
-      const int sl = lds_index<N1, 1>(p, q, e);
-      u[e] = Ul[sl];
-      ul[e] = lo[1] ? YMl[sl] : T(0);
-      ur[e] = hi[1] ? YPl[sl] : T(0);
+      u[e] = LS[UO + ay[e]];
+      ul[e] = lo[1] ? LS[YMO + ay[e]] : T(0);
+      ur[e] = hi[1] ? LS[YPO + ay[e]] : T(0);
     }
     line_apply<N1, 1, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, lo[1], hi[1], kc, klo[1], khi[1], sy);
-    // x-neighbours: staged cell lc-1 / lc+1 (U[0] and U[CPB+1] hold the end cells)
-    const T* xlo = Ul - NLP;
-    const T* xhi = Ul + NLP;
+    // x-neighbours: the staged cells lc-1 and lc+1 (the end cells for the chunk ends)
 #pragma unroll
     for (int e = 0; e < N1; ++e) {
-      const int sl = lds_index<N1, 0>(p, q, e);
-      u[e] = Ul[sl];
-      ul[e] = lo[0] ? xlo[sl] : T(0);
-      ur[e] = hi[0] ? xhi[sl] : T(0);
+      u[e] = LS[UO + ax[e]];
+      ul[e] = lo[0] ? LS[UO - NLP + ax[e]] : T(0);
+      ur[e] = hi[0] ? LS[UO + NLP + ax[e]] : T(0);
     }
     line_apply<N1, 0, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, lo[0], hi[0], kc, klo[0], khi[0], sx);
   }
   __syncthreads();  // all reads of the staged cells done
-  T pq = T(0);
-  const T* PZ = uz;
-  {
 #pragma unroll
-    for (int e = 0; e < N1; ++e) {
-      YMl[lds_index<N1, 2>(p, q, e)] = sz[e];
-      YPl[lds_index<N1, 0>(p, q, e)] = sx[e];
-    }
+  for (int e = 0; e < N1; ++e) {
+    LS[YMO + az[e]] = sz[e];
+    LS[YPO + ax[e]] = sx[e];
   }
   __syncthreads();
   {
     // y-layout: s = sy + sz + sx, then the mass along y
     T v[N1];
 #pragma unroll
-    for (int e = 0; e < N1; ++e) {
-      const int sl = lds_index<N1, 1>(p, q, e);
-      v[e] = sy[e] + YMl[sl] + YPl[sl];
-    }
+    for (int e = 0; e < N1; ++e) v[e] = sy[e] + LS[YMO + ay[e]] + LS[YPO + ay[e]];
 #pragma unroll
     for (int i = 0; i < N1; ++i) {
       T acc = T(0);
 #pragma unroll
       for (int j = 0; j < N1; ++j) acc += a.M[i][j] * v[j];
-      Ul[lds_index<N1, 1>(p, q, i)] = acc;
+      LS[UO + ay[i]] = acc;
     }
   }
   __syncthreads();
   {
     T v[N1];
 #pragma unroll
-    for (int e = 0; e < N1; ++e) v[e] = Ul[lds_index<N1, 0>(p, q, e)];
+    for (int e = 0; e < N1; ++e) v[e] = LS[UO + ax[e]];
 #pragma unroll
     for (int i = 0; i < N1; ++i) {
       T acc = T(0);
 #pragma unroll
       for (int j = 0; j < N1; ++j) acc += a.M[i][j] * v[j];
-      YMl[lds_index<N1, 0>(p, q, i)] = acc;
+      LS[YMO + ax[i]] = acc;
     }
   }
   __syncthreads();
+  T pq = T(0);
   {
     T v[N1];
 #pragma unroll
-    for (int e = 0; e < N1; ++e) v[e] = YMl[lds_index<N1, 2>(p, q, e)];
+    for (int e = 0; e < N1; ++e) v[e] = LS[YMO + az[e]];
     T* yc = y + c0 * NL;
 #pragma unroll
     for (int i = 0; i < N1; ++i) {
@@ -626,7 +657,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
       const int idx = off + line_index<N1, 2>(p, q, i);
       if (ACCUM) yc[idx] += acc;
       else yc[idx] = acc;
-      if (DOT) pq += PZ[i] * acc;
+      if (DOT) pq += uz[i] * acc;
     }
   }
   if (DOT) {
