@@ -11,7 +11,11 @@ The JSON line also carries:
                 algorithm) timed on this host's cores on the same apply;
   solve         time to a 1e-10 relative residual: plain FP64 CG/Jacobi on the
                 manufactured exp problem, and for the lognormal-kappa problem plain
-                FP64 CG vs FP32-inner CG inside FP64 iterative refinement.
+                FP64 CG vs FP32-inner CG inside FP64 iterative refinement;
+  convection    96^3 Q2 convection-diffusion (kappa 1e-2, b = (1, .5, .25), eight seeded
+                Gaussian sources, g = 0): FP64 BiCGSTAB/Jacobi vs FP32-inner BiCGSTAB in
+                FP64 refinement to 1e-8, and the adjoint (J^T, downwind) gradient of
+                1/2 u^T M u with respect to a per-cell source.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve] [--no-cpu]
 """
@@ -59,6 +63,18 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def kron_flops_per_dof(k):
+    """FP64 flops per DoF of the implemented apply (constant kappa, no convection):
+    per line of n1 = k+1 values, A u (n1 mul + n1(n1-1) FMA), two neighbour couplings
+    (trace derivative: 1 mul + (n1-1) FMA, then 2 n1 FMA each), the S_d scaling (n1
+    mul), one mass sweep (n1 mul + n1(n1-1) FMA), three directions, plus 2 n1 adds
+    gathering the partial sums; an FMA counts 2 flops."""
+    n1 = k + 1
+    line = n1 + 2 * n1 * (n1 - 1) + 2 * (1 + 2 * (n1 - 1) + 4 * n1) + n1
+    mass = n1 + 2 * n1 * (n1 - 1)
+    return (3 * (line + mass) + 2 * n1) / n1
+
+
 def cpu_baseline(cfg, min_seconds):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_ffi
@@ -67,10 +83,58 @@ def cpu_baseline(cfg, min_seconds):
     cores = max(1, min(cores, int(os.environ.get("OMP_NUM_THREADS", cores))))
     o = oracle_ffi.Oracle(cfg)
     sec, reps = o.time_apply(nthreads=cores, min_seconds=min_seconds)
+    sec1, reps1 = o.time_apply(nthreads=1, min_seconds=min_seconds / 2)
     n = oracle_ffi.cfgmod.ndofs(cfg)
     return {"value": n / sec / 1e6, "unit": "MDoF/s", "cores": cores, "kind": "port",
             "sample": f"{reps} full jacobian_apply of the 64^3 Q3 FP64 workload (oracle quadrature-loop "
-                      f"restatement, OpenMP over cells with two-sided faces), {sec * reps:.1f} s total"}
+                      f"restatement, OpenMP over cells with two-sided faces), {sec * reps:.1f} s total",
+            "single_core": {"value": n / sec1 / 1e6, "unit": "MDoF/s", "cores": 1,
+                            "sample": f"{reps1} full jacobian_apply on one core, {sec1 * reps1:.1f} s total"}}
+
+
+def convection_leg(pkg):
+    """96^3 Q2 convection-diffusion solves and the source gradient (see the module doc)."""
+    rng = np.random.default_rng(96)
+    gauss = [(*rng.uniform(0.15, 0.85, 3), rng.uniform(0.05, 0.15), rng.uniform(0.5, 1.5)) for _ in range(8)]
+    cfg = pkg.make_config(n=96, degree=2, alpha=3.0, kappa0=1e-2, b=(1.0, 0.5, 0.25), problem=pkg.PROBLEM_GAUSSIANS,
+                          gaussians=gauss, g0=0.0)
+    go = pkg.GridOperator(cfg)
+    out = {"workload": "96^3 Q2 SIPG convection-diffusion, kappa 1e-2, b (1, 0.5, 0.25), 8 seeded Gaussian "
+                       "sources, Dirichlet g = 0, 23,887,872 DoF"}
+    x64 = go.vector()
+    log("bench: 96^3 Q2 FP64 BiCGSTAB/Jacobi")
+    r64 = go.linear_problem_solve(x64, method=pkg.SOLVER_BICGSTAB, reduction=1e-8, maxit=20000)
+    t64 = r64["assembler_time"] + r64["linear_solver_time"]
+    out["fp64_bicgstab"] = {"seconds": t64, "iterations": r64["solver"]["iterations"],
+                            "converged": bool(r64["solver"]["converged"]), "reduction": r64["solver"]["reduction"]}
+    xm = go.vector()
+    log("bench: 96^3 Q2 mixed-precision BiCGSTAB")
+    rm = go.solve_mixed(xm, method=pkg.SOLVER_BICGSTAB, inner_reduction=1e-3, outer_reduction=1e-8,
+                        inner_maxit=20000)
+    a64, am = x64.download(), xm.download()
+    out["mixed_bicgstab"] = {"seconds": rm["elapsed"], "outer": rm["outer_iterations"],
+                             "inner_iterations": rm["inner_iterations"], "converged": bool(rm["converged"]),
+                             "reduction": rm["reduction"],
+                             "rel_diff_vs_fp64_gpu": float(np.linalg.norm(am - a64) / np.linalg.norm(a64)),
+                             "speedup_vs_fp64": t64 / rm["elapsed"] if rm["elapsed"] > 0 else None}
+    # dJ/ds for J = 1/2 u^T M u: J^T lambda = M u, dJ/ds_c = lambda^T (int_c phi)
+    log("bench: 96^3 Q2 adjoint source gradient")
+    t0 = time.perf_counter()
+    rhs = go.vector()
+    go.mass_apply(x64, rhs)
+    lam = go.vector()
+    ra = go.solve(lam, rhs, method=pkg.SOLVER_BICGSTAB, reduction=1e-8, maxit=20000, transpose=True)
+    g = go.cell_vector()
+    go.grad_source(lam, g)
+    go.synchronize()
+    t1 = time.perf_counter()
+    gh = g.download()
+    out["adjoint_source_gradient"] = {"seconds": t1 - t0, "iterations": ra["iterations"],
+                                      "converged": bool(ra["converged"]), "grad_norm": float(np.linalg.norm(gh))}
+    for v in (x64, xm, rhs, lam, g):
+        v.free()
+    go.close()
+    return out
 
 
 APPLY_KERNEL = "dg_apply_rowreg_kernel<4, double, false, false, false, false, false, false>"
@@ -148,6 +212,7 @@ def main():
     ap.add_argument("--degree", type=int, default=3)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-convection", action="store_true", help="skip the 96^3 Q2 convection-diffusion leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--inner-reduction", type=float, default=1e-3,
                     help="FP32 inner Krylov reduction of the mixed-precision refinement")
@@ -193,7 +258,10 @@ def main():
     z32 = go.vector(splitmix_uniform(N, 20261202).astype(np.float32), dtype=np.float32)
     y32 = go.vector(dtype=np.float32)
     kern32_ms = go.time_apply(z32, y32, warmup=3, reps=max(20, args.steps))
-    log(f"bench: kernel avg {kern_ms * 1e3:.1f} us (fp64), {kern32_ms * 1e3:.1f} us (fp32)")
+    stream_ms = go.time_stream(z, y, warmup=3, reps=20)
+    stream_gbs = 24.0 * N / (stream_ms * 1e-3) / 1e9
+    log(f"bench: kernel avg {kern_ms * 1e3:.1f} us (fp64), {kern32_ms * 1e3:.1f} us (fp32); "
+        f"stream reference {stream_gbs:.0f} GB/s")
     z32.free()
     y32.free()
     if args.calibrate:
@@ -224,7 +292,10 @@ def main():
                                 else f"dg_apply_kernel<{k + 1},double,...>"), "kernel_avg_us": kern_ms * 1e3,
                      "algorithmic_bytes_per_launch": alg_bytes,
                      "fp32_kernel_avg_us": kern32_ms * 1e3,
-                     "fp32_achieved_GBs": 8.0 * N / (kern32_ms * 1e-3) / 1e9},
+                     "fp32_achieved_GBs": 8.0 * N / (kern32_ms * 1e-3) / 1e9,
+                     "stream_GBs": stream_gbs, "frac_of_stream": achieved / stream_gbs,
+                     "flops_per_dof": kron_flops_per_dof(k),
+                     "achieved_fp64_TFLOPs": kron_flops_per_dof(k) * N / (kern_ms * 1e-3) / 1e12},
     }
 
     if not args.no_solve:
@@ -267,6 +338,8 @@ def main():
         for v in (x64, xm):
             v.free()
         go3.close()
+        if not args.no_convection:
+            out["convection"] = convection_leg(pkg)
     else:
         z.free()
         y.free()

@@ -90,6 +90,7 @@ def lib():
                                          ctypes.c_double, ctypes.c_int, _vp, ctypes.POINTER(cfgmod.MixedResult)]
     L.pdelab_gpu_grad_logkappa.argtypes = [_vp, _vp, _vp, _vp]
     L.pdelab_gpu_time_apply.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int, _dp]
+    L.pdelab_gpu_time_stream.argtypes = [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _dp]
     _lib = L
     return L
 
@@ -286,6 +287,12 @@ class GridOperator:
         self._check(lib().pdelab_gpu_grad_source(self.h, lam.p, g.p))
 
     # -- measurement
+    def time_stream(self, x, y, warmup=3, reps=20):
+        """Average duration (ms) of y += 0.5 x over FP64 vectors of the handle's length."""
+        out = ctypes.c_double()
+        self._check(lib().pdelab_gpu_time_stream(self.h, x.p, y.p, warmup, reps, ctypes.byref(out)))
+        return out.value
+
     def time_apply(self, z, y, warmup=3, reps=20):
         prec = 32 if z.dtype == np.float32 else 64
         out = ctypes.c_double()

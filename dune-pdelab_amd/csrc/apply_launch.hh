@@ -5,56 +5,56 @@
 #include "handle.hh"
 
 template <int N1, typename T, bool TRANS, bool CONV, bool REACT, bool VARK, bool ACCUM, bool DOT>
-static int launch_apply_k(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, T* dotp, const int* stop) {
+static int launch_apply_k(H* h, const T* z, T* y, const T* fc, const DGArgs<T>& a, T* dotp, const int* stop) {
   using SH = ApplyShape<N1>;
   const long long nb = (h->ncells + SH::CPB - 1) / SH::CPB;
   const LineMats<T>* lm = line_mats<T>(h, TRANS);
   if (a.rowmode && h->apply_variant == 0)
     hipLaunchKernelGGL((dg_apply_rowreg_kernel<N1, T, TRANS, CONV, REACT, VARK, ACCUM, DOT>), dim3((unsigned)nb),
-                       dim3(SH::THREADS), 0, h->stream, z, y, kappa, a, lm, nb, dotp, stop);
+                       dim3(SH::THREADS), 0, h->stream, z, y, fc, a, lm, nb, dotp, stop);
   else
     hipLaunchKernelGGL((dg_apply_kernel<N1, T, TRANS, CONV, REACT, VARK, ACCUM, DOT>), dim3((unsigned)nb),
-                       dim3(SH::THREADS), 0, h->stream, z, y, kappa, a, lm, nb, dotp, stop);
+                       dim3(SH::THREADS), 0, h->stream, z, y, fc, a, lm, nb, dotp, stop);
   HIPCHK(h, hipGetLastError());
   return PDELAB_OK;
 }
 
 template <int N1, typename T, bool TRANS, bool CONV, bool VARK, bool ACCUM, bool DOT>
-static int launch_apply_t(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, bool react, T* dotp,
+static int launch_apply_t(H* h, const T* z, T* y, const T* fc, const DGArgs<T>& a, bool react, T* dotp,
                           const int* stop) {
-  if (react) return launch_apply_k<N1, T, TRANS, CONV, true, VARK, ACCUM, DOT>(h, z, y, kappa, a, dotp, stop);
-  return launch_apply_k<N1, T, TRANS, CONV, false, VARK, ACCUM, DOT>(h, z, y, kappa, a, dotp, stop);
+  if (react) return launch_apply_k<N1, T, TRANS, CONV, true, VARK, ACCUM, DOT>(h, z, y, fc, a, dotp, stop);
+  return launch_apply_k<N1, T, TRANS, CONV, false, VARK, ACCUM, DOT>(h, z, y, fc, a, dotp, stop);
 }
 
 // mode 0: y = J z; 1: y += J z; 2: y = J z with the fused z.(Jz) partials
 template <int N1, typename T, bool TRANS, bool CONV, bool VARK>
-static int launch_apply_a(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, bool react, int mode, T* dotp,
+static int launch_apply_a(H* h, const T* z, T* y, const T* fc, const DGArgs<T>& a, bool react, int mode, T* dotp,
                           const int* stop) {
-  if (mode == 1) return launch_apply_t<N1, T, TRANS, CONV, VARK, true, false>(h, z, y, kappa, a, react, nullptr, stop);
-  if (mode == 2) return launch_apply_t<N1, T, TRANS, CONV, VARK, false, true>(h, z, y, kappa, a, react, dotp, stop);
-  return launch_apply_t<N1, T, TRANS, CONV, VARK, false, false>(h, z, y, kappa, a, react, nullptr, stop);
+  if (mode == 1) return launch_apply_t<N1, T, TRANS, CONV, VARK, true, false>(h, z, y, fc, a, react, nullptr, stop);
+  if (mode == 2) return launch_apply_t<N1, T, TRANS, CONV, VARK, false, true>(h, z, y, fc, a, react, dotp, stop);
+  return launch_apply_t<N1, T, TRANS, CONV, VARK, false, false>(h, z, y, fc, a, react, nullptr, stop);
 }
 template <int N1, typename T, bool TRANS, bool CONV>
-static int launch_apply_v(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, bool react, int mode, T* dotp,
+static int launch_apply_v(H* h, const T* z, T* y, const T* fc, const DGArgs<T>& a, bool react, int mode, T* dotp,
                           const int* stop) {
-  return h->vark ? launch_apply_a<N1, T, TRANS, CONV, true>(h, z, y, kappa, a, react, mode, dotp, stop)
-                 : launch_apply_a<N1, T, TRANS, CONV, false>(h, z, y, kappa, a, react, mode, dotp, stop);
+  return h->vark ? launch_apply_a<N1, T, TRANS, CONV, true>(h, z, y, fc, a, react, mode, dotp, stop)
+                 : launch_apply_a<N1, T, TRANS, CONV, false>(h, z, y, fc, a, react, mode, dotp, stop);
 }
 template <int N1, typename T, bool TRANS>
-static int launch_apply_c(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, bool react, int mode, T* dotp,
+static int launch_apply_c(H* h, const T* z, T* y, const T* fc, const DGArgs<T>& a, bool react, int mode, T* dotp,
                           const int* stop) {
   // CONV selects the general line operator: convection, or constant coefficients
   // without the symmetric structure of line_op_sym
   const bool general = h->conv || (!h->vark && !h->sym[TRANS ? 1 : 0]);
-  return general ? launch_apply_v<N1, T, TRANS, true>(h, z, y, kappa, a, react, mode, dotp, stop)
-                 : launch_apply_v<N1, T, TRANS, false>(h, z, y, kappa, a, react, mode, dotp, stop);
+  return general ? launch_apply_v<N1, T, TRANS, true>(h, z, y, fc, a, react, mode, dotp, stop)
+                 : launch_apply_v<N1, T, TRANS, false>(h, z, y, fc, a, react, mode, dotp, stop);
 }
 template <int N1, typename T>
-int launch_apply_n(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, bool trans, int mode, T* dotp,
+int launch_apply_n(H* h, const T* z, T* y, const T* fc, const DGArgs<T>& a, bool trans, int mode, T* dotp,
                           const int* stop) {
   const bool react = h->cfg.c != 0.0;
-  return trans ? launch_apply_c<N1, T, true>(h, z, y, kappa, a, react, mode, dotp, stop)
-               : launch_apply_c<N1, T, false>(h, z, y, kappa, a, react, mode, dotp, stop);
+  return trans ? launch_apply_c<N1, T, true>(h, z, y, fc, a, react, mode, dotp, stop)
+               : launch_apply_c<N1, T, false>(h, z, y, fc, a, react, mode, dotp, stop);
 }
 
 

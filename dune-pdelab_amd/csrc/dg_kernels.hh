@@ -83,6 +83,37 @@ __host__ __device__ __forceinline__ void face_coeffs(const DGArgs<T>& a, T klo, 
   }
 }
 
+// Coefficients of one interior face as face_coeffs returns them: weights of the lower
+// and upper cell's normal derivative and the penalty.
+template <typename T>
+struct FaceCoef {
+  T wlo, whi, P;
+};
+
+// Per-cell face coefficients for cell-wise kappa, FC values per cell: kappa of the cell,
+// then (wlo, whi, P) of its lower face in x, y and z (zero where that face is a boundary
+// face); the upper face of a cell is the lower face of its upper neighbour.
+constexpr int FC = 10;
+
+template <typename T>
+__global__ __launch_bounds__(256) void face_coeff_kernel(const double* __restrict__ kappa, T* __restrict__ fc,
+                                                         const DGArgs<double> a) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.ncells) return;
+  const int coord[3] = {(int)(c % a.n[0]), (int)((c / a.n[0]) % a.n[1]), (int)(c / ((long long)a.n[0] * a.n[1]))};
+  const double kc = kappa[c];
+  T* f = fc + c * FC;
+  f[0] = (T)kc;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double wlo = 0, whi = 0, P = 0;
+    if (coord[d] > 0) face_coeffs<double>(a, kappa[c - a.stride[d]], kc, wlo, whi, P);
+    f[1 + 3 * d] = (T)wlo;
+    f[2 + 3 * d] = (T)whi;
+    f[3 + 3 * d] = (T)P;
+  }
+}
+
 template <int N1, int D>
 __device__ __forceinline__ int line_index(int p, int q, int e) {
   constexpr int L = N1 * N1;
@@ -144,11 +175,12 @@ __device__ __forceinline__ const SlotTable<N1>& slot_table() {
 }
 
 // s = M^{-1} (A_d u) on one grid line of a cell (or M^{-1} A_d^T u for TRANS), from the
-// line values u, the neighbouring lines ul/ur (valid when has_lo/has_hi) and the
-// kappa of the cell and its two neighbours in direction D.
+// line values u, the neighbouring lines ul/ur (valid when has_lo/has_hi), the kappa of
+// the cell and the coefficients of its lower and upper face in direction D.
 template <int N1, int D, typename T, bool TRANS, bool CONV, bool REACT, bool VARK>
 __host__ __device__ __forceinline__ void line_op(const DGArgs<T>& a, const T (&u)[N1], const T (&ul)[N1], const T (&ur)[N1],
-                                        bool has_lo, bool has_hi, T kc, T klo, T khi, T (&s)[N1]) {
+                                        bool has_lo, bool has_hi, T kc, const FaceCoef<T>& flo,
+                                        const FaceCoef<T>& fhi, T (&s)[N1]) {
   const T val0 = u[0], val1 = u[N1 - 1];
   T der0 = T(0), der1 = T(0);
 #pragma unroll
@@ -161,9 +193,7 @@ __host__ __device__ __forceinline__ void line_op(const DGArgs<T>& a, const T (&u
     T d1l = T(0);
 #pragma unroll
     for (int e = 0; e < N1; ++e) d1l += a.d1[e] * ul[e];
-    T wlo, whi, P;
-    if (VARK) face_coeffs<T>(a, klo, kc, wlo, whi, P);
-    else { wlo = a.wI_lo; whi = a.wI_hi; P = a.PI; }
+    const T wlo = flo.wlo, whi = flo.whi, P = flo.P;
     const T J = v1l - val0;              // jump u_lo - u_hi
     const T G = wlo * d1l + whi * der0;  // weighted average of the normal derivative
     if (!TRANS) { ce0 += Sd * (G - P * J); cd0 += Sd * theta * whi * J; }
@@ -189,9 +219,7 @@ __host__ __device__ __forceinline__ void line_op(const DGArgs<T>& a, const T (&u
     T d0r = T(0);
 #pragma unroll
     for (int e = 0; e < N1; ++e) d0r += a.d0[e] * ur[e];
-    T wlo, whi, P;
-    if (VARK) face_coeffs<T>(a, kc, khi, wlo, whi, P);
-    else { wlo = a.wI_lo; whi = a.wI_hi; P = a.PI; }
+    const T wlo = fhi.wlo, whi = fhi.whi, P = fhi.P;
     const T J = val1 - v0r;
     const T G = wlo * der1 + whi * d0r;
     if (!TRANS) { ce1 += Sd * (P * J - G); cd1 += Sd * theta * wlo * J; }
@@ -320,9 +348,9 @@ __device__ __forceinline__ void line_op_sym(const DGArgs<T>& a, const LineMats<T
 template <int N1, int D, typename T, bool TRANS, bool CONV, bool REACT, bool VARK>
 __device__ __forceinline__ void line_apply(const DGArgs<T>& a, const LineMats<T>* __restrict__ lm, const T (&u)[N1],
                                            const T (&ul)[N1], const T (&ur)[N1], bool has_lo, bool has_hi, T kc,
-                                           T klo, T khi, T (&s)[N1]) {
+                                           const FaceCoef<T>& flo, const FaceCoef<T>& fhi, T (&s)[N1]) {
   if constexpr (VARK || CONV)
-    line_op<N1, D, T, TRANS, CONV, REACT, VARK>(a, u, ul, ur, has_lo, has_hi, kc, klo, khi, s);
+    line_op<N1, D, T, TRANS, CONV, REACT, VARK>(a, u, ul, ur, has_lo, has_hi, kc, flo, fhi, s);
   else
     line_op_sym<N1, D, T, TRANS, REACT>(a, lm, u, ul, ur, has_lo, has_hi, s);
 }
@@ -345,7 +373,7 @@ __device__ __forceinline__ void load_line(const T* __restrict__ base, int off, i
 template <int N1, typename T, bool TRANS, bool CONV, bool REACT, bool VARK, bool ACCUM, bool DOT>
 __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const T* __restrict__ z,
                                                                            T* __restrict__ y,
-                                                                           const T* __restrict__ kappa,
+                                                                           const T* __restrict__ fc,
                                                                            const DGArgs<T> a,
                                                                            const LineMats<T>* __restrict__ lm,
                                                                            long long nblocks,
@@ -379,9 +407,10 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
     hi[d] = active && coord[d] < a.n[d] - 1;
   }
   T uz[N1], zl[N1], zr[N1], yl[N1], yr[N1], xe[N1];
-  T kc = a.kappa0, klo[3], khi[3];
+  T kc = a.kappa0;
+  FaceCoef<T> flo[3], fhi[3];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) klo[d] = khi[d] = a.kappa0;
+  for (int d = 0; d < 3; ++d) flo[d] = fhi[d] = FaceCoef<T>{a.wI_lo, a.wI_hi, a.PI};
   // the end-cell loads: lanes of cell 0 load the cell before the chunk, lanes of the
   // last cell the cell after it (only where that cell exists and is an x-neighbour)
   const bool first = lc == 0, last = lc == CPB - 1;
@@ -394,12 +423,15 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
     if (hi[1]) load_line<N1, 2, T>(zc + a.stride[1] * NL, off, p, q, yr);
     if (ld_xe) load_line<N1, 2, T>(zc + (first && lo[0] ? -NL : NL), off, p, q, xe);
     if (VARK) {
-      const T* kb = kappa + c0;
-      kc = kb[lc];
+      const T* fb = fc + (c0 + lc) * FC;
+      kc = fb[0];
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        if (lo[d]) klo[d] = (kb - a.stride[d])[lc];
-        if (hi[d]) khi[d] = (kb + a.stride[d])[lc];
+        if (lo[d]) flo[d] = FaceCoef<T>{fb[1 + 3 * d], fb[2 + 3 * d], fb[3 + 3 * d]};
+        if (hi[d]) {
+          const T* fn = fb + a.stride[d] * FC;
+          fhi[d] = FaceCoef<T>{fn[1 + 3 * d], fn[2 + 3 * d], fn[3 + 3 * d]};
+        }
       }
     }
   }
@@ -419,7 +451,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
   __syncthreads();
   T sx[N1], sy[N1], sz[N1];
   if (active) {
-    line_apply<N1, 2, T, TRANS, CONV, REACT, VARK>(a, lm, uz, zl, zr, lo[2], hi[2], kc, klo[2], khi[2], sz);
+    line_apply<N1, 2, T, TRANS, CONV, REACT, VARK>(a, lm, uz, zl, zr, lo[2], hi[2], kc, flo[2], fhi[2], sz);
     T u[N1], ul[N1], ur[N1];
 #pragma unroll
     for (int e = 0; e < N1; ++e) {
@@ -428,7 +460,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
       ul[e] = lo[1] ? YMl[sl] : T(0);
       ur[e] = hi[1] ? YPl[sl] : T(0);
     }
-    line_apply<N1, 1, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, lo[1], hi[1], kc, klo[1], khi[1], sy);
+    line_apply<N1, 1, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, lo[1], hi[1], kc, flo[1], fhi[1], sy);
     // x-neighbours: staged cell lc-1 / lc+1 (U[0] and U[CPB+1] hold the end cells)
     const T* xlo = Ul - NLP;
     const T* xhi = Ul + NLP;
@@ -439,7 +471,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
       ul[e] = lo[0] ? xlo[sl] : T(0);
       ur[e] = hi[0] ? xhi[sl] : T(0);
     }
-    line_apply<N1, 0, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, lo[0], hi[0], kc, klo[0], khi[0], sx);
+    line_apply<N1, 0, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, lo[0], hi[0], kc, flo[0], fhi[0], sx);
   }
   __syncthreads();  // all reads of the staged cells done
   T pq = T(0);
@@ -516,7 +548,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
 // immediate offset. The z-neighbour lines stay in registers.
 template <int N1, typename T, bool TRANS, bool CONV, bool REACT, bool VARK, bool ACCUM, bool DOT>
 __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kernel(
-    const T* __restrict__ z, T* __restrict__ y, const T* __restrict__ kappa, const DGArgs<T> a,
+    const T* __restrict__ z, T* __restrict__ y, const T* __restrict__ fc, const DGArgs<T> a,
     const LineMats<T>* __restrict__ lm, long long nblocks, T* __restrict__ dotp, const int* __restrict__ stop) {
   if (stop && *stop) return;  // a device-side Krylov loop has already stopped
   using SH = ApplyShape<N1>;
@@ -545,9 +577,10 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
   const int off = lc * NL;  // lane's cell block relative to c0
   const T* zc = z + c0 * NL;
   T uz[N1], zl[N1], zr[N1], yl[N1], yr[N1], xe[N1];
-  T kc = a.kappa0, klo[3], khi[3];
+  T kc = a.kappa0;
+  FaceCoef<T> flo[3], fhi[3];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) klo[d] = khi[d] = a.kappa0;
+  for (int d = 0; d < 3; ++d) flo[d] = fhi[d] = FaceCoef<T>{a.wI_lo, a.wI_hi, a.PI};
   const bool first = lc == 0, last = lc == CPB - 1;
   const bool ld_xe = (first && xlo_ok) || (last && xhi_ok);
   load_line<N1, 2, T>(zc, off, p, q, uz);
@@ -557,13 +590,15 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
   if (hi[1]) load_line<N1, 2, T>(zc + a.stride[1] * NL, off, p, q, yr);
   if (ld_xe) load_line<N1, 2, T>(zc + (first ? -NL : NL), off, p, q, xe);
   if (VARK) {
-    const T* kb = kappa + c0;
-    kc = kb[lc];
+    const T* fb = fc + (c0 + lc) * FC;
+    kc = fb[0];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const long long sd = d == 0 ? 1 : a.stride[d];
-      if (lo[d]) klo[d] = kb[lc - sd];
-      if (hi[d]) khi[d] = kb[lc + sd];
+      if (lo[d]) flo[d] = FaceCoef<T>{fb[1 + 3 * d], fb[2 + 3 * d], fb[3 + 3 * d]};
+      if (hi[d]) {
+        const T* fn = fb + a.stride[d] * FC;
+        fhi[d] = FaceCoef<T>{fn[1 + 3 * d], fn[2 + 3 * d], fn[3 + 3 * d]};
+      }
     }
   }
   // slot of element e of the lane's x-, y- and z-line, relative to the cell region
@@ -591,7 +626,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
   __syncthreads();
   T sx[N1], sy[N1], sz[N1];
   {
-    line_apply<N1, 2, T, TRANS, CONV, REACT, VARK>(a, lm, uz, zl, zr, lo[2], hi[2], kc, klo[2], khi[2], sz);
+    line_apply<N1, 2, T, TRANS, CONV, REACT, VARK>(a, lm, uz, zl, zr, lo[2], hi[2], kc, flo[2], fhi[2], sz);
     T u[N1], ul[N1], ur[N1];
 #pragma unroll
     for (int e = 0; e < N1; ++e) {
@@ -599,7 +634,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
       ul[e] = lo[1] ? LS[YMO + ay[e]] : T(0);
       ur[e] = hi[1] ? LS[YPO + ay[e]] : T(0);
     }
-    line_apply<N1, 1, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, lo[1], hi[1], kc, klo[1], khi[1], sy);
+    line_apply<N1, 1, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, lo[1], hi[1], kc, flo[1], fhi[1], sy);
     // x-neighbours: the staged cells lc-1 and lc+1 (the end cells for the chunk ends)
 #pragma unroll
     for (int e = 0; e < N1; ++e) {
@@ -607,7 +642,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
       ul[e] = lo[0] ? LS[UO - NLP + ax[e]] : T(0);
       ur[e] = hi[0] ? LS[UO + NLP + ax[e]] : T(0);
     }
-    line_apply<N1, 0, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, lo[0], hi[0], kc, klo[0], khi[0], sx);
+    line_apply<N1, 0, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, lo[0], hi[0], kc, flo[0], fhi[0], sx);
   }
   __syncthreads();  // all reads of the staged cells done
 #pragma unroll

@@ -30,6 +30,8 @@ struct pdelab_gpu_gridoperator {
   bool conv = false, vark = false;
   double* d_kappa = nullptr;
   float* d_kappa32 = nullptr;
+  double* d_fc64 = nullptr;  // per-cell face coefficients (FC values per cell) of kappa
+  float* d_fc32 = nullptr;
   double* d_R0 = nullptr;
   bool R0_valid = false;
   double* d_src = nullptr;
@@ -98,7 +100,7 @@ inline const LineMats<float>* line_mats<float>(H* h, bool trans) { return h->d_l
 // y (+)= J z (or J^T z) on the handle's stream for Q_{N1-1}: mode 0 y = J z, 1 y += J z,
 // 2 y = J z with the fused z.(Jz) partials in dotp. Defined in apply_q<N1-1>.hip.
 template <int N1, typename T>
-int launch_apply_n(H* h, const T* z, T* y, const T* kappa, const DGArgs<T>& a, bool trans, int mode, T* dotp,
+int launch_apply_n(H* h, const T* z, T* y, const T* fc, const DGArgs<T>& a, bool trans, int mode, T* dotp,
                    const int* stop);
 
 #endif

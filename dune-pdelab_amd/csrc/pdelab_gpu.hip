@@ -115,9 +115,10 @@ template <int N1, bool TRANS>
 static void line_eval(const DGArgs<LD>& al, int D, const LD (&u)[N1], const LD (&ul)[N1], const LD (&ur)[N1],
                       bool lo, bool hi, LD (&s)[N1]) {
   const LD k = al.kappa0;
-  if (D == 0) line_op<N1, 0, LD, TRANS, true, true, false>(al, u, ul, ur, lo, hi, k, k, k, s);
-  else if (D == 1) line_op<N1, 1, LD, TRANS, true, true, false>(al, u, ul, ur, lo, hi, k, k, k, s);
-  else line_op<N1, 2, LD, TRANS, true, true, false>(al, u, ul, ur, lo, hi, k, k, k, s);
+  const FaceCoef<LD> f{al.wI_lo, al.wI_hi, al.PI};
+  if (D == 0) line_op<N1, 0, LD, TRANS, true, true, false>(al, u, ul, ur, lo, hi, k, f, f, s);
+  else if (D == 1) line_op<N1, 1, LD, TRANS, true, true, false>(al, u, ul, ur, lo, hi, k, f, f, s);
+  else line_op<N1, 2, LD, TRANS, true, true, false>(al, u, ul, ur, lo, hi, k, f, f, s);
 }
 
 template <int N1, bool TRANS>
@@ -318,11 +319,11 @@ static int check_outflow(H* h) {
 }
 
 template <typename T>
-static const T* kappa_for(H* h);
+static const T* fc_for(H* h);
 template <>
-const double* kappa_for<double>(H* h) { return h->d_kappa; }
+const double* fc_for<double>(H* h) { return h->d_fc64; }
 template <>
-const float* kappa_for<float>(H* h) { return h->d_kappa32; }
+const float* fc_for<float>(H* h) { return h->d_fc32; }
 template <typename T>
 static const DGArgs<T>& args_for(H* h);
 template <>
@@ -335,9 +336,9 @@ static int apply_mode(H* h, const T* z, T* y, bool trans, int mode, T* dotp = nu
                       const int* stop = nullptr) {
   CHK(check_outflow(h));
   switch (h->n1) {
-    case 2: return launch_apply_n<2, T>(h, z, y, kappa_for<T>(h), args_for<T>(h), trans, mode, dotp, stop);
-    case 3: return launch_apply_n<3, T>(h, z, y, kappa_for<T>(h), args_for<T>(h), trans, mode, dotp, stop);
-    case 4: return launch_apply_n<4, T>(h, z, y, kappa_for<T>(h), args_for<T>(h), trans, mode, dotp, stop);
+    case 2: return launch_apply_n<2, T>(h, z, y, fc_for<T>(h), args_for<T>(h), trans, mode, dotp, stop);
+    case 3: return launch_apply_n<3, T>(h, z, y, fc_for<T>(h), args_for<T>(h), trans, mode, dotp, stop);
+    case 4: return launch_apply_n<4, T>(h, z, y, fc_for<T>(h), args_for<T>(h), trans, mode, dotp, stop);
   }
   return fail(h, PDELAB_ERR_UNSUPPORTED, "degree");
 }
@@ -720,6 +721,20 @@ static int validate(const pdelab_config* c, std::string& msg) {
   return PDELAB_OK;
 }
 
+// per-cell face coefficients of the current kappa (FP64 and FP32) for the apply kernels
+static int update_face_coeffs(H* h) {
+  const long long nc = h->ncells;
+  if (!h->d_fc64) HIPCHK(h, hipMalloc(&h->d_fc64, sizeof(double) * FC * nc));
+  if (!h->d_fc32) HIPCHK(h, hipMalloc(&h->d_fc32, sizeof(float) * FC * nc));
+  const int nb = (int)((nc + 255) / 256);
+  hipLaunchKernelGGL(face_coeff_kernel<double>, dim3(nb), dim3(256), 0, h->stream, (const double*)h->d_kappa,
+                     h->d_fc64, h->a64);
+  hipLaunchKernelGGL(face_coeff_kernel<float>, dim3(nb), dim3(256), 0, h->stream, (const double*)h->d_kappa,
+                     h->d_fc32, h->a64);
+  HIPCHK(h, hipGetLastError());
+  return PDELAB_OK;
+}
+
 static int upload_kappa(H* h, const double* kappa_host) {
   const size_t nc = (size_t)h->ncells;
   for (size_t i = 0; i < nc; ++i)
@@ -731,6 +746,7 @@ static int upload_kappa(H* h, const double* kappa_host) {
   hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for((long long)nc)), dim3(256), 0, h->stream,
                      (long long)nc, (const double*)h->d_kappa, h->d_kappa32);
   HIPCHK(h, hipGetLastError());
+  CHK(update_face_coeffs(h));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->vark = true;
   h->R0_valid = false;
@@ -823,7 +839,7 @@ int pdelab_gpu_destroy(pdelab_gpu_gridoperator* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   void* ptrs[] = {h->d_kappa, h->d_kappa32, h->d_R0, h->d_src, h->d_ftab, h->d_gtab, h->d_diag, h->d_dinv,
                   h->d_dinv32, h->d_part64, h->d_part32, h->d_scal64, h->d_scal32, h->d_applydot,
-                  h->d_state, h->d_dstate, h->d_lm64, h->d_lm32};
+                  h->d_state, h->d_dstate, h->d_lm64, h->d_lm32, h->d_fc64, h->d_fc32};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int i = 0; i < 8; ++i) {
@@ -861,6 +877,7 @@ int pdelab_gpu_set_log_kappa_device(pdelab_gpu_gridoperator* h, const double* lo
   if (!h->d_kappa32) HIPCHK(h, hipMalloc(&h->d_kappa32, sizeof(float) * nc));
   hipLaunchKernelGGL(exp_kernel, dim3(grid_for(nc)), dim3(256), 0, h->stream, nc, logk, h->d_kappa, h->d_kappa32);
   HIPCHK(h, hipGetLastError());
+  CHK(update_face_coeffs(h));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->vark = true;
   h->R0_valid = false;
@@ -1147,6 +1164,26 @@ int pdelab_gpu_time_apply(pdelab_gpu_gridoperator* h, int precision, const void*
   auto one = [&]() -> int {
     if (precision == 64) return apply64(h, (const double*)z, (double*)y, false, false);
     return apply32(h, (const float*)z, (float*)y, false, false);
+  };
+  for (int i = 0; i < warmup; ++i) CHK(one());
+  HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+  for (int i = 0; i < reps; ++i) CHK(one());
+  HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+  HIPCHK(h, hipEventSynchronize(h->ev1));
+  float ms = 0;
+  HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  *avg_ms = (double)ms / reps;
+  return PDELAB_OK;
+}
+
+int pdelab_gpu_time_stream(pdelab_gpu_gridoperator* h, const double* x, double* y, int warmup, int reps,
+                           double* avg_ms) {
+  if (!h || !x || !y || !avg_ms || reps < 1) return PDELAB_ERR_INVALID_ARG;
+  const long long n = h->N;
+  auto one = [&]() -> int {
+    hipLaunchKernelGGL(axpy_kernel<double>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, 0.5, x, y);
+    HIPCHK(h, hipGetLastError());
+    return PDELAB_OK;
   };
   for (int i = 0; i < warmup; ++i) CHK(one());
   HIPCHK(h, hipEventRecord(h->ev0, h->stream));

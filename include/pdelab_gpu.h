@@ -253,6 +253,12 @@ int pdelab_gpu_grad_source(pdelab_gpu_gridoperator* h, const double* lambda, dou
    measured with HIP events on the handle's stream. */
 int pdelab_gpu_time_apply(pdelab_gpu_gridoperator* h, int precision, const void* z, void* y,
                           int warmup, int reps, double* avg_ms);
+/* Stream reference of the same box: `reps` back-to-back y += 0.5 x over the handle's
+   vector length (FP64: 16 B read and 8 B written per entry) after `warmup` untimed ones;
+   average duration per launch in milliseconds (HIP events, the handle's stream). No
+   reference counterpart: it is the measured bandwidth ceiling the apply is compared with. */
+int pdelab_gpu_time_stream(pdelab_gpu_gridoperator* h, const double* x, double* y, int warmup, int reps,
+                           double* avg_ms);
 
 #ifdef __cplusplus
 }

@@ -295,6 +295,26 @@ def test_mixed_precision_refinement(pkg, of):
     assert np.linalg.norm(r) <= 1.01e-10 * res["first_defect"]
 
 
+def test_mixed_precision_bicgstab_convection(pkg, of):
+    """The 96^3 Q2 convection-diffusion bench problem at 8^3: FP32 BiCGSTAB inside FP64
+    refinement to 1e-8 against the oracle's FP64 BiCGSTAB solve to 1e-12."""
+    say("[gpu] mixed precision BiCGSTAB 8^3 Q2 convection")
+    gauss = ((0.3, 0.4, 0.5, 0.1, 1.0), (0.7, 0.2, 0.6, 0.15, -0.5), (0.5, 0.8, 0.3, 0.08, 0.8))
+    cfg = pkg.make_config(n=8, degree=2, alpha=3.0, kappa0=1e-2, b=(1.0, 0.5, 0.25), problem=pkg.PROBLEM_GAUSSIANS,
+                          gaussians=gauss, g0=0.0)
+    go, o = pkg.GridOperator(cfg), of.Oracle(cfg)
+    o.set_threads(8)
+    xd = go.vector()
+    res = go.solve_mixed(xd, method=pkg.SOLVER_BICGSTAB, inner_reduction=1e-3, outer_reduction=1e-8,
+                         inner_maxit=20000)
+    assert res["converged"], res
+    x = xd.download()
+    x_ref, _ = o.linear_problem_solve(method=1, reduction=1e-12, maxit=20000)
+    r = o.residual(x)
+    assert np.linalg.norm(r) <= 1.01e-8 * res["first_defect"]
+    assert rel(x, x_ref) <= 1e-6
+
+
 def test_adjoint_gradient_vs_oracle(pkg, of):
     say("[gpu] grad logkappa 6^3 Q2")
     n = 6
