@@ -11,7 +11,7 @@ The JSON line also carries:
                 algorithm) timed on this host's cores on the same apply;
   solve         time to a 1e-10 relative residual: plain FP64 CG/Jacobi on the
                 manufactured exp problem, and for the lognormal-kappa problem plain
-                FP64 CG vs FP32-inner CG inside FP64 iterative refinement;
+                FP64 CG vs FP32 CG with FP64 residual replacement (reliable updates);
   convection    96^3 Q2 convection-diffusion (kappa 1e-2, b = (1, .5, .25), eight seeded
                 Gaussian sources, g = 0): FP64 BiCGSTAB/Jacobi vs FP32-inner BiCGSTAB in
                 FP64 refinement to 1e-8, and the adjoint (J^T, downwind) gradient of
@@ -214,8 +214,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-convection", action="store_true", help="skip the 96^3 Q2 convection-diffusion leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--inner-reduction", type=float, default=1e-3,
-                    help="FP32 inner Krylov reduction of the mixed-precision refinement")
+    ap.add_argument("--inner-reduction", type=float, default=0.1,
+                    help="mixed-precision CG: FP32 residual reduction between FP64 residual replacements")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run 5 axpy launches (reads 16 B/DoF, writes 8 B/DoF) to scale HBM counters")
     ap.add_argument("--profile-summary", metavar="DIR",

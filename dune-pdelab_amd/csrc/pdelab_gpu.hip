@@ -527,6 +527,77 @@ const float* dinv_for<float>(H* h, int precond) { return precond == PDELAB_PREC_
 // reads the flag back every kCheckEvery iterations.
 constexpr int kCheckEvery = 16;
 
+// Outcome of one run of the device CG loop.
+struct CgLoopOut {
+  int iters = 0;          // iterations done (the stopping one included)
+  double def = 0;         // defect at the last iteration
+  bool stopped = false;   // the stopping rule fired (else the iteration limit was reached)
+  bool aborted = false;   // non-finite defect
+  int rho_slot = 7;       // scalar slot of the rho the last iteration used (beta's denominator)
+};
+
+// The device CG loop from iteration 1: p holds the search direction and scal[7] the rho
+// of iteration 1. Stops where dune-istl's rule fires against def_ref:
+// def < def_ref * red || def < 1e-30.
+template <typename T, typename A>
+static int cg_loop(H* h, T* x, T* b, T* p, T* q, const T* dinv, bool trans, int maxit, double def_ref, double red,
+                   CgLoopOut& out) {
+  const long long n = h->N;
+  const long long nbapply = apply_blocks(h);
+  if (!h->d_applydot) HIPCHK(h, hipMalloc(&h->d_applydot, sizeof(double) * (size_t)std::max(nbapply, 1LL)));
+  static_assert(std::is_same<T, A>::value, "the fused apply dot is accumulated in the vector type");
+  A* scal = scal_buf<A>(h);
+  T* dotp = reinterpret_cast<T*>(h->d_applydot);
+  h->h_state[0] = h->h_state[1] = h->h_state[2] = h->h_state[3] = 0;
+  h->h_dstate[0] = def_ref;
+  h->h_dstate[1] = def_ref;
+  h->h_dstate[2] = red;
+  HIPCHK(h, hipMemcpyAsync(h->d_state, h->h_state, sizeof(int) * 4, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->d_dstate, h->h_dstate, sizeof(double) * 4, hipMemcpyHostToDevice, h->stream));
+  bool stopped = false;
+  int last = 0;
+  for (int i = 1; i <= maxit && !stopped; ++i) {
+    const int rslot = (i & 1) ? 7 : 5;  // slot holding the current rho
+    const int oslot = (i & 1) ? 4 : 6;  // (def^2, rho_new) written here
+    // q = J p with the p.q partials -> scal[0]; r -= lambda q with the (r.r, z.r)
+    // partials -> scal[oslot..] and the defect check; x += lambda p, p = z + beta p
+    CHK(apply_mode<T>(h, p, q, trans, 2, dotp, h->d_state));
+    hipLaunchKernelGGL((final_reduce_kernel<A, 1>), dim3(1), dim3(kRedThreads), 0, h->stream, (const A*)dotp,
+                       (int)nbapply, scal + 0, (const int*)h->d_state);
+    hipLaunchKernelGGL((cg_rupdate_kernel<T, A>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, n, b,
+                       (const T*)q, dinv, (const A*)scal, rslot, partial_buf<A>(h), (const int*)h->d_state);
+    hipLaunchKernelGGL((cg_check_kernel<A>), dim3(1), dim3(kRedThreads), 0, h->stream, (const A*)partial_buf<A>(h),
+                       kRedBlocks, scal + oslot, h->d_state, h->d_dstate, i);
+    hipLaunchKernelGGL((cg_xpupdate_kernel<T, A>), dim3(grid_for(n)), dim3(256), 0, h->stream, n, x, p,
+                       (const T*)b, dinv, (const A*)scal, rslot, oslot + 1, (const int*)h->d_state, i);
+    HIPCHK(h, hipGetLastError());
+    last = i;
+    if (i % kCheckEvery == 0 || i == maxit) {
+      HIPCHK(h, hipMemcpyAsync(h->h_state, h->d_state, sizeof(int) * 4, hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(h, hipStreamSynchronize(h->stream));
+      stopped = h->h_state[0] != 0;
+    }
+  }
+  HIPCHK(h, hipMemcpyAsync(h->h_state, h->d_state, sizeof(int) * 4, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->h_dstate, h->d_dstate, sizeof(double) * 4, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  out.stopped = h->h_state[0] != 0;
+  out.aborted = h->h_state[1] != 0;
+  out.iters = out.stopped ? h->h_state[2] : last;
+  out.def = h->h_dstate[0];
+  out.rho_slot = (out.iters & 1) ? 7 : 5;
+  return PDELAB_OK;
+}
+
+// CGSolver::apply (dune-istl): solves J z = b from the given z; b becomes the defect.
+// Workspace slots base, base+1 (p, q).
+//
+// The loop runs on the device: per iteration the apply (with the fused p.q partials),
+// the update of r (with the r.r and z.r partials), the Iteration::step check and the
+// update of x and the direction are enqueued without waiting; the check kernel raises a
+// device flag at the iteration where dune-istl's stopping rule fires and every later
+// kernel of the loop returns immediately, so the state is exactly the one at that
+// iteration. The host reads the flag back every kCheckEvery iterations.
 template <typename T, typename A>
 static int cg_solve(H* h, T* x, T* b, double reduction, int maxit, int precond, bool trans, int base,
                     pdelab_linear_solver_result* res) {
@@ -536,11 +607,6 @@ static int cg_solve(H* h, T* x, T* b, double reduction, int maxit, int precond, 
   T* p = ws<T>(h)[base];
   T* q = ws<T>(h)[base + 1];
   const T* dinv = dinv_for<T>(h, precond);
-  const long long nbapply = apply_blocks(h);
-  if (!h->d_applydot) HIPCHK(h, hipMalloc(&h->d_applydot, sizeof(double) * (size_t)std::max(nbapply, 1LL)));
-  static_assert(std::is_same<T, A>::value, "the fused apply dot is accumulated in the vector type");
-  A* scal = scal_buf<A>(h);
-  T* dotp = reinterpret_cast<T*>(h->d_applydot);
   Iter it(reduction);
   // b -= J x  (applyscaleadd(-1, x, b))
   CHK(apply_t<T>(h, x, q, trans, false));
@@ -554,52 +620,12 @@ static int cg_solve(H* h, T* x, T* b, double reduction, int maxit, int precond, 
     hipLaunchKernelGGL((prec_dot_kernel<T, A>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, n, dinv,
                        (const T*)b, p, partial_buf<A>(h));
     CHK(finish_reduce<A, 1>(h, 7));
-    // device loop state
-    h->h_state[0] = h->h_state[1] = h->h_state[2] = h->h_state[3] = 0;
-    h->h_dstate[0] = def0;
-    h->h_dstate[1] = def0;
-    h->h_dstate[2] = reduction;
-    HIPCHK(h, hipMemcpyAsync(h->d_state, h->h_state, sizeof(int) * 4, hipMemcpyHostToDevice, h->stream));
-    HIPCHK(h, hipMemcpyAsync(h->d_dstate, h->h_dstate, sizeof(double) * 4, hipMemcpyHostToDevice, h->stream));
-    bool stopped = false;
-    int last = 0;
-    for (int i = 1; i <= maxit && !stopped; ++i) {
-      const int rslot = (i & 1) ? 7 : 5;  // slot holding the current rho
-      const int oslot = (i & 1) ? 4 : 6;  // (def^2, rho_new) written here
-      // q = J p with the p.q partials -> scal[0]; r -= lambda q with the (r.r, z.r)
-      // partials -> scal[oslot..] and the defect check; x += lambda p, p = z + beta p
-      CHK(apply_mode<T>(h, p, q, trans, 2, dotp, h->d_state));
-      hipLaunchKernelGGL((final_reduce_kernel<A, 1>), dim3(1), dim3(kRedThreads), 0, h->stream, (const A*)dotp,
-                         (int)nbapply, scal + 0, (const int*)h->d_state);
-      hipLaunchKernelGGL((cg_rupdate_kernel<T, A>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, n, b,
-                         (const T*)q, dinv, (const A*)scal, rslot, partial_buf<A>(h), (const int*)h->d_state);
-      hipLaunchKernelGGL((cg_check_kernel<A>), dim3(1), dim3(kRedThreads), 0, h->stream, (const A*)partial_buf<A>(h),
-                         kRedBlocks, scal + oslot, h->d_state, h->d_dstate, i);
-      hipLaunchKernelGGL((cg_xpupdate_kernel<T, A>), dim3(grid_for(n)), dim3(256), 0, h->stream, n, x, p,
-                         (const T*)b, dinv, (const A*)scal, rslot, oslot + 1, (const int*)h->d_state, i);
-      HIPCHK(h, hipGetLastError());
-      last = i;
-      if (i % kCheckEvery == 0 || i == maxit) {
-        HIPCHK(h, hipMemcpyAsync(h->h_state, h->d_state, sizeof(int) * 4, hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(h, hipStreamSynchronize(h->stream));
-        stopped = h->h_state[0] != 0;
-      }
-    }
-    HIPCHK(h, hipMemcpyAsync(h->h_state, h->d_state, sizeof(int) * 4, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipMemcpyAsync(h->h_dstate, h->d_dstate, sizeof(double) * 4, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
-    const int iters = h->h_state[0] ? h->h_state[2] : last;
-    const double def = h->h_dstate[0];
-    if (h->h_state[1]) {
-      it.aborted = true;
-      it.def = def;
-      it.i = iters;
-      it.converged = false;
-    } else {
-      it.def = def;
-      it.i = iters;
-      it.converged = (def < it.def0 * it.red) || (def < 1e-30);
-    }
+    CgLoopOut lo;
+    CHK((cg_loop<T, A>(h, x, b, p, q, dinv, trans, maxit, def0, reduction, lo)));
+    it.def = lo.def;
+    it.i = lo.iters;
+    it.aborted = lo.aborted;
+    it.converged = !lo.aborted && ((lo.def < it.def0 * it.red) || (lo.def < 1e-30));
   }
   HIPCHK(h, hipStreamSynchronize(h->stream));
   const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -755,6 +781,94 @@ static int upload_kappa(H* h, const double* kappa_host) {
 }
 
 // ===========================================================================
+// Mixed-precision CG with reliable updates (residual replacement): one FP32 CG run on
+// J z = R(x) whose recursive residual is replaced by the FP64 residual whenever it has
+// dropped by inner_reduction (or by the reduction still missing). At each replacement
+// the FP32 correction is subtracted from x in FP64, R(x) is recomputed in FP64 and
+// converted, and the search direction is kept (beta from the replaced residual), so
+// the Krylov space is not rebuilt as a restarted refinement would rebuild it.
+static int cg_mixed_reliable(H* h, int precond, double inner_reduction, int inner_maxit, double outer_reduction,
+                             int outer_maxit, double* x, pdelab_mixed_result* res) {
+  const long long n = h->N;
+  CHK(ensure_ws<double>(h, 6));
+  CHK(ensure_ws<float>(h, 4));
+  double* r64 = h->w64[5];
+  float *r32 = h->w32[0], *x32 = h->w32[1], *p32 = h->w32[2], *q32 = h->w32[3];
+  if (precond == PDELAB_PREC_JACOBI) CHK(compute_diag(h));
+  else if (precond != PDELAB_PREC_NONE) return fail(h, PDELAB_ERR_INVALID_ARG, "unknown preconditioner");
+  const float* dinv = dinv_for<float>(h, precond);
+  auto residual64 = [&](double& d) -> int {
+    CHK(apply64(h, x, r64, false, false));
+    hipLaunchKernelGGL(axpy_kernel<double>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, 1.0,
+                       (const double*)h->d_R0, r64);
+    CHK(pdelab_gpu_norm(h, r64, &d));
+    if (!std::isfinite(d)) return fail(h, PDELAB_ERR_SOLVER_ABORT, "refinement: residual is infinite or NaN");
+    return PDELAB_OK;
+  };
+  auto to_f32 = [&]() -> int {
+    hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(n)), dim3(256), 0, h->stream, n,
+                       (const double*)r64, r32);
+    HIPCHK(h, hipGetLastError());
+    return PDELAB_OK;
+  };
+  double def0 = 0, def = 0;
+  CHK(residual64(def0));
+  def = def0;
+  res->first_defect = def0;
+  int outer = 0, inner = 0;
+  bool conv = def0 < 1e-30;
+  if (!conv) {
+    CHK(to_f32());
+    hipLaunchKernelGGL(fill_kernel<float>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, 0.0f, x32);
+    // p = W^{-1} r; rho -> slot 7
+    hipLaunchKernelGGL((prec_dot_kernel<float, float>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, n, dinv,
+                       (const float*)r32, p32, partial_buf<float>(h));
+    CHK((finish_reduce<float, 1>(h, 7)));
+    for (;;) {
+      const double thr = std::max(inner_reduction * def, 0.5 * outer_reduction * def0);
+      CgLoopOut lo;
+      CHK((cg_loop<float, float>(h, x32, r32, p32, q32, dinv, false, std::max(1, inner_maxit - inner), def, thr / def,
+                                 lo)));
+      inner += lo.iters;
+      if (lo.aborted) {
+        res->inner_iterations = inner;
+        return fail(h, PDELAB_ERR_SOLVER_ABORT, "mixed CG: FP32 defect is infinite or NaN");
+      }
+      // residual replacement: x -= z32, r = R(x) in FP64
+      hipLaunchKernelGGL(sub_f32_kernel, dim3(grid_for(n)), dim3(256), 0, h->stream, n, (const float*)x32, x);
+      hipLaunchKernelGGL(fill_kernel<float>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, 0.0f, x32);
+      HIPCHK(h, hipGetLastError());
+      CHK(residual64(def));
+      ++outer;
+      if (def <= outer_reduction * def0 || def < 1e-30) {
+        conv = true;
+        break;
+      }
+      if (outer >= outer_maxit || inner >= inner_maxit) break;
+      // continue the direction: p = W^{-1} r + (rho_new / rho_old) p
+      float sc[8];
+      CHK((read_scalars<float>(h, sc, 8)));
+      const double rho_old = sc[lo.rho_slot];
+      CHK(to_f32());
+      hipLaunchKernelGGL((prec_dot_kernel<float, float>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, n, dinv,
+                         (const float*)r32, q32, partial_buf<float>(h));
+      CHK((finish_reduce<float, 1>(h, 7)));
+      CHK((read_scalars<float>(h, sc, 8)));
+      const float beta = rho_old != 0 ? (float)(sc[7] / rho_old) : 0.0f;
+      hipLaunchKernelGGL(cg_pupdate_kernel<float>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, p32,
+                         (const float*)r32, dinv, beta);
+      HIPCHK(h, hipGetLastError());
+    }
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  res->converged = conv ? 1 : 0;
+  res->outer_iterations = outer;
+  res->inner_iterations = inner;
+  res->defect = def;
+  res->reduction = def0 > 0 ? def / def0 : 0.0;
+  return PDELAB_OK;
+}
+
 extern "C" {
 
 void pdelab_gpu_default_config(pdelab_config* cfg) {
@@ -1077,6 +1191,14 @@ int pdelab_gpu_solve_mixed(pdelab_gpu_gridoperator* h, int method, int precond, 
   if (!h || !x || !res) return PDELAB_ERR_INVALID_ARG;
   std::memset(res, 0, sizeof(*res));
   if (outer_maxit <= 0) outer_maxit = 50;
+  if (inner_maxit <= 0) inner_maxit = 5000;
+  if (method == PDELAB_SOLVER_CG) {
+    auto t0 = std::chrono::steady_clock::now();
+    CHK(compute_R0(h));
+    CHK(cg_mixed_reliable(h, precond, inner_reduction, inner_maxit, outer_reduction, outer_maxit, x, res));
+    res->elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return PDELAB_OK;
+  }
   const long long n = h->N;
   CHK(ensure_ws<double>(h, 6));
   CHK(ensure_ws<float>(h, 2));

@@ -504,12 +504,15 @@ using ISTLBackend_SEQ_MatrixFree_CG_Jac = ISTLBackend_SEQ_MatrixFree_Jac<GO, Kry
 template <class GO>
 using ISTLBackend_SEQ_MatrixFree_BCGS_Jac = ISTLBackend_SEQ_MatrixFree_Jac<GO, Krylov::BiCGSTAB>;
 
-// Mixed precision backend: FP32 inner Krylov inside FP64 iterative refinement; used in
-// place of the linear problem solver (the refinement owns the residual evaluations).
+// Mixed precision backend: FP32 inner Krylov inside FP64 refinement (CG: reliable
+// updates, inner_reduction = FP32 residual reduction between FP64 residual replacements;
+// BiCGSTAB: restarted inner solves to inner_reduction); used in place of the linear
+// problem solver (the refinement owns the residual evaluations).
 template <class GO, Krylov method>
 class MixedPrecisionSolver {
  public:
-  MixedPrecisionSolver(const GO& go, double inner_reduction = 1e-4, unsigned inner_maxit = 5000)
+  MixedPrecisionSolver(const GO& go, double inner_reduction = method == Krylov::CG ? 0.1 : 1e-3,
+                       unsigned inner_maxit = 20000)
       : go_(go), inner_(inner_reduction), inner_maxit_(inner_maxit) {}
   pdelab_mixed_result apply(typename GO::Traits::Domain& x, double reduction, int outer_maxit = 50) {
     pdelab_mixed_result r;

@@ -233,9 +233,13 @@ int pdelab_gpu_solve_transpose(pdelab_gpu_gridoperator* h, int method, int preco
 int pdelab_gpu_linear_problem_solve(pdelab_gpu_gridoperator* h, int method, int precond,
                                     double reduction, double min_defect, int maxit, double* x,
                                     pdelab_linear_problem_result* res);
-/* Mixed precision: FP64 refinement x <- x - J32^{-1} R(x) with the inner solve in FP32
-   to inner_reduction (or, when weaker, to a quarter of the reduction still missing), until
-   ||R(x)|| <= outer_reduction * ||R(x0)||. */
+/* Mixed precision: FP32 Krylov inside FP64 refinement until ||R(x)|| <= outer_reduction *
+   ||R(x0)||. BiCGSTAB: x <- x - J32^{-1} R(x), each inner solve to inner_reduction (or, when
+   weaker, to a quarter of the reduction still missing). CG: one FP32 CG run with reliable
+   updates; whenever its recursive residual has dropped by inner_reduction, the FP32
+   correction is subtracted from x in FP64 and the residual replaced by the FP64 R(x), with
+   the search direction kept (0.1 is a good interval; much smaller ones let the FP32
+   recursion drift and stall). outer_iterations counts the FP64 residual replacements. */
 int pdelab_gpu_solve_mixed(pdelab_gpu_gridoperator* h, int method, int precond,
                            double inner_reduction, int inner_maxit, double outer_reduction,
                            int outer_maxit, double* x, pdelab_mixed_result* res);

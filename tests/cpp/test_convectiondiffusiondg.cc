@@ -100,7 +100,7 @@ int main() {
     using GO3 = GridOperator<GridFunctionSpace<FEM3>, GridFunctionSpace<FEM3>, LOP3>;
     GO3 go3(gfs3, gfs3, lop3);
     auto xm = go3.makeVector();
-    MixedPrecisionSolver<GO3, Krylov::CG> mixed(go3, 1e-4);
+    MixedPrecisionSolver<GO3, Krylov::CG> mixed(go3, 0.1);
     auto mr = mixed.apply(xm, 1e-10);
     std::printf("mixed precision: converged %d outer %d inner %d reduction %g\n", mr.converged, mr.outer_iterations,
                 mr.inner_iterations, mr.reduction);

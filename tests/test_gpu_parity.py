@@ -286,7 +286,7 @@ def test_mixed_precision_refinement(pkg, of):
     go, o = pkg.GridOperator(cfg, kappa), of.Oracle(cfg, kappa)
     o.set_threads(8)
     xd = go.vector()
-    res = go.solve_mixed(xd, inner_reduction=1e-4, outer_reduction=1e-10)
+    res = go.solve_mixed(xd, inner_reduction=0.1, outer_reduction=1e-10, inner_maxit=20000)
     assert res["converged"], res
     x = xd.download()
     x_ref, _ = o.linear_problem_solve(reduction=1e-13, maxit=20000)
