@@ -720,7 +720,10 @@ static int krylov(H* h, int method, int precond, double reduction, int maxit, bo
   if (maxit <= 0) maxit = 5000;
   if (precond == PDELAB_PREC_JACOBI) CHK(compute_diag(h));
   else if (precond != PDELAB_PREC_NONE) return fail(h, PDELAB_ERR_INVALID_ARG, "unknown preconditioner");
-  if (method == PDELAB_SOLVER_CG) return cg_solve<T, A>(h, z, r, reduction, maxit, precond, trans, base, res);
+  if (method == PDELAB_SOLVER_CG) {
+    if constexpr (std::is_same<T, A>::value) return cg_solve<T, A>(h, z, r, reduction, maxit, precond, trans, base, res);
+    else return fail(h, PDELAB_ERR_UNSUPPORTED, "CG accumulates in its vector type");
+  }
   if (method == PDELAB_SOLVER_BICGSTAB) return bicgstab_solve<T, A>(h, z, r, reduction, maxit, precond, trans, base, res);
   return fail(h, PDELAB_ERR_INVALID_ARG, "unknown Krylov method");
 }
@@ -1199,6 +1202,7 @@ int pdelab_gpu_solve_mixed(pdelab_gpu_gridoperator* h, int method, int precond, 
     res->elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return PDELAB_OK;
   }
+
   const long long n = h->N;
   CHK(ensure_ws<double>(h, 6));
   CHK(ensure_ws<float>(h, 2));
