@@ -19,7 +19,7 @@ import numpy as np
 
 from . import config as cfgmod
 from .config import (  # noqa: F401  (re-exported)
-    BC_DIRICHLET, BC_NEUMANN, BC_NONE, BC_OUTFLOW, IIPG, NIPG, PREC_JACOBI, PREC_NONE,
+    BC_DIRICHLET, BC_NEUMANN, BC_NONE, BC_OUTFLOW, IIPG, NIPG, PREC_BLOCK_JACOBI, PREC_JACOBI, PREC_NONE,
     PROBLEM_CONSTANT, PROBLEM_EXP, PROBLEM_GAUSSIANS, PROBLEM_QUADRATIC, PROBLEM_TABULATED,
     PROBLEM_ZERO, SIPG, SOLVER_BICGSTAB, SOLVER_CG, WEIGHTS_OFF, WEIGHTS_ON, LinearProblemResult,
     LinearSolverResult, MixedResult, PdelabConfig, make_config, ncells, ndofs)
@@ -77,6 +77,9 @@ def lib():
         getattr(L, fn).argtypes = [_vp, _vp, _vp]
     L.pdelab_gpu_jacobian_apply_nonlinear.argtypes = [_vp, _vp, _vp, _vp]
     L.pdelab_gpu_point_diagonal.argtypes = [_vp, _vp]
+    L.pdelab_gpu_block_diagonal.argtypes = [_vp, ctypes.c_int, _vp]
+    L.pdelab_gpu_block_jacobi_apply.argtypes = [_vp, _vp, _vp]
+    L.pdelab_gpu_block_jacobi_apply_f32.argtypes = [_vp, _vp, _vp]
     L.pdelab_gpu_dot.argtypes = [_vp, _vp, _vp, _dp]
     L.pdelab_gpu_norm.argtypes = [_vp, _vp, _dp]
     L.pdelab_gpu_axpy.argtypes = [_vp, ctypes.c_double, _vp, _vp]
@@ -240,6 +243,17 @@ class GridOperator:
 
     def point_diagonal(self, d):
         self._check(lib().pdelab_gpu_point_diagonal(self.h, d.p))
+
+    def block_diagonal(self, blocks, transpose=False):
+        """blocks (ncells * nloc^2 doubles) = the diagonal blocks of J (J^T), row-major per cell."""
+        self._check(lib().pdelab_gpu_block_diagonal(self.h, 1 if transpose else 0, blocks.p))
+
+    def block_jacobi_apply(self, z, y):
+        """y += D^{-1} z with D the block diagonal of J (FP32 for float32 vectors)."""
+        if z.dtype == np.float32:
+            self._check(lib().pdelab_gpu_block_jacobi_apply_f32(self.h, z.p, y.p))
+        else:
+            self._check(lib().pdelab_gpu_block_jacobi_apply(self.h, z.p, y.p))
 
     def mass_apply(self, z, y):
         self._check(lib().pdelab_gpu_mass_apply(self.h, z.p, y.p))

@@ -14,9 +14,6 @@
 
 namespace pdelab_amd {
 
-constexpr int kRedBlocks = 1024;
-constexpr int kRedThreads = 256;
-
 template <typename A>
 __device__ __forceinline__ A wave_sum(A v) {
 #pragma unroll
@@ -217,13 +214,15 @@ __global__ __launch_bounds__(kRedThreads) void cg_check_kernel(const A* __restri
   }
 }
 
-// x += lambda p (lambda = scal[rho_old] / scal[0]), then p = dinv r + beta p with
-// beta = scal[rho_new] / scal[rho_old]; on the iteration that stopped the loop only x
-// is updated (dune-istl leaves the loop after the defect check).
+// x += lambda p (lambda = scal[rho_old] / scal[0]), then p = z + beta p with
+// beta = scal[rho_new] / scal[rho_old] and z = W^{-1} r: the given z (block Jacobi), else
+// dinv r or r; on the iteration that stopped the loop only x is updated (dune-istl leaves
+// the loop after the defect check).
 template <typename T, typename A>
 __global__ __launch_bounds__(256) void cg_xpupdate_kernel(long long n, T* __restrict__ x, T* __restrict__ p,
                                                           const T* __restrict__ r, const T* __restrict__ dinv,
-                                                          const A* __restrict__ scal, int slot_rho_old,
+                                                          const T* __restrict__ z, const A* __restrict__ scal,
+                                                          int slot_rho_old,
                                                           int slot_rho_new, const int* __restrict__ state, int it) {
   if (state[0] && state[2] < it) return;
   const bool pupd = state[0] == 0;
@@ -239,7 +238,7 @@ __global__ __launch_bounds__(256) void cg_xpupdate_kernel(long long n, T* __rest
       if (i < n) {
         pv[u] = p[i];
         xv[u] = x[i];
-        if (pupd) zv[u] = dinv ? dinv[i] * r[i] : r[i];
+        if (pupd) zv[u] = z ? z[i] : (dinv ? dinv[i] * r[i] : r[i]);
       }
     }
 #pragma unroll

@@ -1,0 +1,509 @@
+// block_jacobi.hh -- per-cell block Jacobi preconditioner of the DG operator.
+//
+// The diagonal block of cell c is D_c = (M (x) M (x) M) K_c with the Kronecker sum
+// K_c = I(x)I(x)L_x + I(x)L_y(x)I + L_z(x)I(x)I of the cell's 1D self operators
+// L_d = M^{-1} A_d, the operator of one line with both neighbour traces set to zero (what
+// line_op computes for a unit vector). The reference inverts D_c by LU with partial
+// pivoting (AssembledBlockJacobiPreconditionerLocalOperator,
+// R:dune/pdelab/backend/istl/matrixfree/assembledblockjacobipreconditioner.hh:96-242;
+// the matrix-based SeqJac over Blocking::fixed, R:dune/pdelab/backend/istl/seqistlsolverbackend.hh:240-243).
+// Two exact representations of D_c^{-1} are used here:
+//
+//  * fast diagonalisation (symmetric A_d: SIPG without convection): with the generalised
+//    eigenvectors A_d V_d = M V_d diag(lam_d), V_d^T M V_d = I,
+//        D_c^{-1} = (V_z (x) V_y (x) V_x) diag(1 / (lam_x + lam_y + lam_z)) (V_z^T (x) V_y^T (x) V_x^T),
+//    stored as (V_d, lam_d) per direction: 3 (n^2 + n) values per cell, or per boundary type
+//    for constant coefficients;
+//  * dense inverse blocks (any operator): D_c^{-1} = K_c^{-1} (M^{-1} (x) M^{-1} (x) M^{-1}),
+//    stored transposed, one block per cell or one per boundary type.
+//
+// Boundary type of a cell in direction d: 0 interior, 1 lower face on the boundary, 2 upper
+// face on the boundary, 3 both. With constant coefficients D_c depends only on these.
+#ifndef PDELAB_BLOCK_JACOBI_HH
+#define PDELAB_BLOCK_JACOBI_HH
+
+#include "block_reduce.hh"
+#include "dg_kernels.hh"
+
+namespace pdelab_amd {
+
+enum { BJ_PREC = 0, BJ_ADD = 1, BJ_CGR = 2 };
+
+__host__ __device__ inline int bj_type(int i, int n) { return (i == 0 ? 1 : 0) | (i == n - 1 ? 2 : 0); }
+
+struct BJGeom {
+  int n[3];
+  long long stride[3];
+  long long ncells;
+  int per_cell;  // data per cell instead of per boundary type
+  int vark;      // cell-wise kappa: coefficients from the per-cell face coefficients
+};
+
+// 1D mass matrix data: Ci = C^{-1} for the Cholesky factor M = C C^T, and M^{-1}.
+struct BJMass {
+  double Ci[4][4];
+  double Minv[4][4];
+};
+
+template <int N1>
+struct BJShape {
+  static constexpr int NL = N1 * N1 * N1;
+  static constexpr int THREADS = 256;
+  static constexpr int CPG = THREADS / NL;  // cells per workgroup: 32 (Q1), 9 (Q2), 4 (Q3)
+  static constexpr int USED = CPG * NL;
+  static constexpr int ES = N1 * N1 + N1;   // fast-diagonalisation entry: V (row-major), lam
+};
+
+// ---------------------------------------------------------------------------
+// 1D self operator of direction D for a cell of boundary type t:
+// L[i][j] = (line_op e_j)[i] with zero neighbour traces.
+template <int N1, int D, bool TRANS, bool CONV, bool VARK>
+__device__ inline void bj_line_d(const DGArgs<double>& a, int t, double kc, const FaceCoef<double>& flo,
+                                 const FaceCoef<double>& fhi, double (&L)[N1][N1]) {
+  double zero[N1], e[N1], s[N1];
+#pragma unroll
+  for (int i = 0; i < N1; ++i) zero[i] = 0.0;
+#pragma unroll
+  for (int j = 0; j < N1; ++j) {
+#pragma unroll
+    for (int i = 0; i < N1; ++i) e[i] = (i == j) ? 1.0 : 0.0;
+    line_op<N1, D, double, TRANS, CONV, true, VARK>(a, e, zero, zero, !(t & 1), !(t & 2), kc, flo, fhi, s);
+#pragma unroll
+    for (int i = 0; i < N1; ++i) L[i][j] = s[i];
+  }
+}
+
+template <int N1, bool TRANS, bool CONV, bool VARK>
+__device__ inline void bj_line(const DGArgs<double>& a, int d, int t, double kc, const FaceCoef<double>& flo,
+                               const FaceCoef<double>& fhi, double (&L)[N1][N1]) {
+  if (d == 0) bj_line_d<N1, 0, TRANS, CONV, VARK>(a, t, kc, flo, fhi, L);
+  else if (d == 1) bj_line_d<N1, 1, TRANS, CONV, VARK>(a, t, kc, flo, fhi, L);
+  else bj_line_d<N1, 2, TRANS, CONV, VARK>(a, t, kc, flo, fhi, L);
+}
+
+// Cell coefficients of direction d for line_op: the cell's kappa and its two faces.
+__device__ inline void bj_cell_coeffs(const DGArgs<double>& a, const double* __restrict__ fc, const BJGeom& g,
+                                      long long c, int d, int coord, double& kc, FaceCoef<double>& flo,
+                                      FaceCoef<double>& fhi) {
+  if (!g.vark) {
+    kc = a.kappa0;
+    flo = FaceCoef<double>{a.wI_lo, a.wI_hi, a.PI};
+    fhi = flo;
+    return;
+  }
+  const double* f = fc + c * FC;
+  kc = f[0];
+  flo = FaceCoef<double>{f[1 + 3 * d], f[2 + 3 * d], f[3 + 3 * d]};
+  fhi = FaceCoef<double>{0.0, 0.0, 0.0};
+  if (coord < g.n[d] - 1) {
+    const double* fu = fc + (c + g.stride[d]) * FC;
+    fhi = FaceCoef<double>{fu[1 + 3 * d], fu[2 + 3 * d], fu[3 + 3 * d]};
+  }
+}
+
+__device__ inline int bj_coord(const BJGeom& g, long long c, int d) {
+  return d == 0 ? (int)(c % g.n[0]) : (d == 1 ? (int)((c / g.n[0]) % g.n[1]) : (int)(c / ((long long)g.n[0] * g.n[1])));
+}
+
+// Symmetric eigen-decomposition S = Q diag(w) Q^T by cyclic Jacobi rotations
+// (Golub & Van Loan, Matrix Computations, 4th ed., algorithm 8.5.2).
+template <int N>
+__device__ inline void sym_eig_jacobi(double (&S)[N][N], double (&Q)[N][N], double (&w)[N]) {
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) Q[i][j] = (i == j) ? 1.0 : 0.0;
+  double fro = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) fro += S[i][j] * S[i][j];
+  for (int sweep = 0; sweep < 32; ++sweep) {
+    double off = 0.0;
+#pragma unroll
+    for (int p = 0; p < N; ++p)
+#pragma unroll
+      for (int q = p + 1; q < N; ++q) off += S[p][q] * S[p][q];
+    if (off <= 1e-34 * fro) break;
+#pragma unroll
+    for (int p = 0; p < N; ++p) {
+#pragma unroll
+      for (int q = p + 1; q < N; ++q) {
+        const double apq = S[p][q];
+        if (apq == 0.0) continue;
+        // rotation angle that annihilates S[p][q]
+        const double tau = (S[q][q] - S[p][p]) / (2.0 * apq);
+        const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+        const double cs = 1.0 / sqrt(1.0 + t * t), sn = t * cs;
+#pragma unroll
+        for (int k = 0; k < N; ++k) {  // S <- S J (columns p, q)
+          const double skp = S[k][p], skq = S[k][q];
+          S[k][p] = cs * skp - sn * skq;
+          S[k][q] = sn * skp + cs * skq;
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k) {  // S <- J^T S (rows p, q)
+          const double spk = S[p][k], sqk = S[q][k];
+          S[p][k] = cs * spk - sn * sqk;
+          S[q][k] = sn * spk + cs * sqk;
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k) {  // Q <- Q J
+          const double qkp = Q[k][p], qkq = Q[k][q];
+          Q[k][p] = cs * qkp - sn * qkq;
+          Q[k][q] = sn * qkp + cs * qkq;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) w[i] = S[i][i];
+}
+
+// Fast-diagonalisation data, one thread per (entry, direction): entries d*4 + t for
+// boundary types (constant coefficients) or c*3 + d per cell.
+template <int N1, bool TRANS, bool VARK>
+__global__ __launch_bounds__(256) void bj_fdm_setup_kernel(const DGArgs<double> a, const double* __restrict__ fc,
+                                                           const BJGeom g, const BJMass ch, long long nentries,
+                                                           double* __restrict__ out64, float* __restrict__ out32) {
+  using S = BJShape<N1>;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nentries) return;
+  long long c = 0;
+  int d, t;
+  if (g.per_cell) {
+    c = e / 3;
+    d = (int)(e - 3 * c);
+    t = bj_type(bj_coord(g, c, d), g.n[d]);
+  } else {
+    d = (int)(e / 4);
+    t = (int)(e % 4);
+  }
+  double kc;
+  FaceCoef<double> flo, fhi;
+  bj_cell_coeffs(a, fc, g, c, d, g.per_cell ? bj_coord(g, c, d) : 0, kc, flo, fhi);
+  double L[N1][N1], A[N1][N1], T1[N1][N1], Sm[N1][N1], Q[N1][N1], w[N1];
+  bj_line<N1, TRANS, false, VARK>(a, d, t, kc, flo, fhi, L);
+  // A = M L (symmetric), then Sm = Ci A Ci^T
+#pragma unroll
+  for (int i = 0; i < N1; ++i)
+#pragma unroll
+    for (int j = 0; j < N1; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int l = 0; l < N1; ++l) s += a.M[i][l] * L[l][j];
+      A[i][j] = s;
+    }
+#pragma unroll
+  for (int i = 0; i < N1; ++i)
+#pragma unroll
+    for (int j = 0; j < N1; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int l = 0; l < N1; ++l) s += ch.Ci[i][l] * A[l][j];
+      T1[i][j] = s;
+    }
+#pragma unroll
+  for (int i = 0; i < N1; ++i)
+#pragma unroll
+    for (int j = 0; j < N1; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int l = 0; l < N1; ++l) s += T1[i][l] * ch.Ci[j][l];
+      Sm[i][j] = s;
+    }
+#pragma unroll
+  for (int i = 0; i < N1; ++i)
+#pragma unroll
+    for (int j = i + 1; j < N1; ++j) {
+      const double m = 0.5 * (Sm[i][j] + Sm[j][i]);
+      Sm[i][j] = m;
+      Sm[j][i] = m;
+    }
+  sym_eig_jacobi<N1>(Sm, Q, w);
+  // V = Ci^T Q
+  const long long base = e * S::ES;
+#pragma unroll
+  for (int l = 0; l < N1; ++l)
+#pragma unroll
+    for (int i = 0; i < N1; ++i) {
+      double s = 0.0;
+#pragma unroll
+      for (int m = 0; m < N1; ++m) s += ch.Ci[m][l] * Q[m][i];
+      if (out64) out64[base + l * N1 + i] = s;
+      if (out32) out32[base + l * N1 + i] = (float)s;
+    }
+#pragma unroll
+  for (int i = 0; i < N1; ++i) {
+    if (out64) out64[base + N1 * N1 + i] = w[i];
+    if (out32) out32[base + N1 * N1 + i] = (float)w[i];
+  }
+}
+
+// Dense blocks, one workgroup per entry (boundary-type triple tx + 4 ty + 16 tz, or cell).
+// out_mode 0: D^{-1} transposed (DT[j][i] = D^{-1}[i][j]) into out64/out32;
+// out_mode 1: the block D itself, row-major, into out64 (per cell only).
+template <int N1, bool TRANS, bool CONV, bool VARK>
+__global__ __launch_bounds__(256) void bj_dense_setup_kernel(const DGArgs<double> a, const double* __restrict__ fc,
+                                                             const BJGeom g, const BJMass ch, int out_mode,
+                                                             double* __restrict__ out64, float* __restrict__ out32) {
+  constexpr int NL = N1 * N1 * N1;
+  __shared__ double Ls[3][N1][N1];
+  __shared__ double K[NL][NL + 1];
+  __shared__ double fcol[NL];
+  __shared__ int perm[NL];
+  __shared__ int pivrow;
+  const long long e = blockIdx.x;
+  const int tid = threadIdx.x;
+  long long c = 0;
+  int t[3];
+  if (g.per_cell) {
+    c = e;
+    for (int d = 0; d < 3; ++d) t[d] = bj_type(bj_coord(g, c, d), g.n[d]);
+  } else {
+    t[0] = (int)(e % 4);
+    t[1] = (int)((e / 4) % 4);
+    t[2] = (int)(e / 16);
+  }
+  if (tid < 3) {
+    double kc;
+    FaceCoef<double> flo, fhi;
+    bj_cell_coeffs(a, fc, g, c, tid, g.per_cell ? bj_coord(g, c, tid) : 0, kc, flo, fhi);
+    double L[N1][N1];
+    bj_line<N1, TRANS, CONV, VARK>(a, tid, t[tid], kc, flo, fhi, L);
+    for (int i = 0; i < N1; ++i)
+      for (int j = 0; j < N1; ++j) Ls[tid][i][j] = L[i][j];
+  }
+  __syncthreads();
+  // Kronecker sum, index i = ix + N1 iy + N1^2 iz
+  for (int ij = tid; ij < NL * NL; ij += blockDim.x) {
+    const int i = ij / NL, j = ij - i * NL;
+    const int ix = i % N1, iy = (i / N1) % N1, iz = i / (N1 * N1);
+    const int jx = j % N1, jy = (j / N1) % N1, jz = j / (N1 * N1);
+    double s = 0.0;
+    if (iy == jy && iz == jz) s += Ls[0][ix][jx];
+    if (ix == jx && iz == jz) s += Ls[1][iy][jy];
+    if (ix == jx && iy == jy) s += Ls[2][iz][jz];
+    K[i][j] = s;
+  }
+  __syncthreads();
+  if (out_mode == 1) {
+    // D = (M (x) M (x) M) K
+    for (int ij = tid; ij < NL * NL; ij += blockDim.x) {
+      const int i = ij / NL, j = ij - i * NL;
+      const int ix = i % N1, iy = (i / N1) % N1, iz = i / (N1 * N1);
+      double s = 0.0;
+      for (int l = 0; l < NL; ++l) {
+        const int lx = l % N1, ly = (l / N1) % N1, lz = l / (N1 * N1);
+        s += a.M[ix][lx] * a.M[iy][ly] * a.M[iz][lz] * K[l][j];
+      }
+      out64[e * NL * NL + ij] = s;
+    }
+    return;
+  }
+  // in-place Gauss-Jordan inversion of K with partial (row) pivoting
+  for (int k = 0; k < NL; ++k) {
+    if (tid < 64) {  // pivot search by the first wave
+      double best = -1.0;
+      int br = k;
+      for (int i = k + tid; i < NL; i += 64) {
+        const double v = fabs(K[i][k]);
+        if (v > best) { best = v; br = i; }
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_down(best, off, 64);
+        const int orow = __shfl_down(br, off, 64);
+        if (ob > best || (ob == best && orow < br)) { best = ob; br = orow; }
+      }
+      if (tid == 0) { pivrow = br; perm[k] = br; }
+    }
+    __syncthreads();
+    const int p = pivrow;
+    if (p != k)
+      for (int j = tid; j < NL; j += blockDim.x) {
+        const double tmp = K[k][j];
+        K[k][j] = K[p][j];
+        K[p][j] = tmp;
+      }
+    __syncthreads();
+    const double piv = K[k][k];
+    for (int i = tid; i < NL; i += blockDim.x) fcol[i] = K[i][k];
+    __syncthreads();
+    // row k: K[k][j] /= piv (j != k), K[k][k] = 1 / piv
+    for (int j = tid; j < NL; j += blockDim.x) K[k][j] = (j == k) ? 1.0 / piv : K[k][j] / piv;
+    __syncthreads();
+    for (int ij = tid; ij < NL * NL; ij += blockDim.x) {
+      const int i = ij / NL, j = ij - i * NL;
+      if (i == k) continue;
+      if (j == k) K[i][j] = -fcol[i] * K[k][k];
+      else K[i][j] -= fcol[i] * K[k][j];
+    }
+    __syncthreads();
+  }
+  // undo the row interchanges as column interchanges, last first
+  for (int k = NL - 1; k >= 0; --k) {
+    const int p = perm[k];
+    if (p != k)
+      for (int i = tid; i < NL; i += blockDim.x) {
+        const double tmp = K[i][k];
+        K[i][k] = K[i][p];
+        K[i][p] = tmp;
+      }
+    __syncthreads();
+  }
+  // D^{-1} = K^{-1} (M^{-1} (x) M^{-1} (x) M^{-1}), written transposed
+  for (int ij = tid; ij < NL * NL; ij += blockDim.x) {
+    const int j = ij / NL, i = ij - j * NL;
+    const int jx = j % N1, jy = (j / N1) % N1, jz = j / (N1 * N1);
+    double s = 0.0;
+    for (int l = 0; l < NL; ++l) {
+      const int lx = l % N1, ly = (l / N1) % N1, lz = l / (N1 * N1);
+      s += K[i][l] * ch.Minv[lx][jx] * ch.Minv[ly][jy] * ch.Minv[lz][jz];
+    }
+    if (out64) out64[e * NL * NL + ij] = s;
+    if (out32) out32[e * NL * NL + ij] = (float)s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Application. One workgroup takes CPG cells at a time (grid-stride over cell groups);
+// each lane owns one DoF, loaded and stored coalesced.
+//   BJ_PREC: out = W^{-1} in, partials of out.in (one per workgroup)
+//   BJ_ADD : out += W^{-1} in
+//   BJ_CGR : r -= lambda q (lambda = scal[slot_rho] / scal[0]), out = W^{-1} r, partials
+//            of (r.r, out.r) (two per workgroup); returns at once when state[0] is set
+template <typename T, typename A>
+struct BJVec {
+  const T* in;
+  T* out;
+  T* r;
+  const T* q;
+  const A* scal;
+  int slot_rho;
+  A* partial;
+  const int* state;
+};
+
+template <int N1, int D, typename T>
+__device__ __forceinline__ T bj_sweep(const T* __restrict__ src, const T (&w)[N1], int ix, int iy, int iz) {
+  T s = T(0);
+#pragma unroll
+  for (int l = 0; l < N1; ++l) {
+    const int idx = D == 0 ? l + N1 * iy + N1 * N1 * iz : (D == 1 ? ix + N1 * l + N1 * N1 * iz : ix + N1 * iy + N1 * N1 * l);
+    s += w[l] * src[idx];
+  }
+  return s;
+}
+
+template <int N1, typename T, typename A, int MODE, bool FDM>
+__global__ __launch_bounds__(256) void bj_apply_kernel(const BJVec<T, A> v, const T* __restrict__ data,
+                                                       const BJGeom g) {
+  using S = BJShape<N1>;
+  constexpr int NL = S::NL, CPG = S::CPG;
+  if (MODE == BJ_CGR && v.state[0]) return;
+  __shared__ T buf[2][CPG * NL];
+  const int tid = threadIdx.x;
+  const bool lane = tid < S::USED;  // Q2: the last 13 lanes own no DoF
+  const int cl = lane ? tid / NL : 0, l = lane ? tid - cl * NL : 0;
+  const int ix = l % N1, iy = (l / N1) % N1, iz = l / (N1 * N1);
+  T lambda = T(0);
+  if (MODE == BJ_CGR) lambda = (T)(v.scal[v.slot_rho] / v.scal[0]);
+  A acc0 = A(0), acc1 = A(0);
+  const long long ngroups = (g.ncells + CPG - 1) / CPG;
+  int parity = 0;
+  for (long long grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const long long c = grp * CPG + cl;
+    const bool act = lane && c < g.ncells;
+    const long long cc = act ? c : 0;
+    const long long idx = cc * NL + l;
+    T u = T(0);
+    if (act) {
+      u = v.in[idx];
+      if (MODE == BJ_CGR) {
+        u -= lambda * v.q[idx];
+        v.r[idx] = u;
+        acc0 += (A)u * (A)u;
+      }
+    }
+    T res = T(0);
+    if constexpr (FDM) {
+      // per-direction entries of this lane's cell
+      const T* E[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const int co = bj_coord(g, cc, d);
+        const long long ent = g.per_cell ? cc * 3 + d : (long long)(d * 4 + bj_type(co, g.n[d]));
+        E[d] = data + ent * S::ES;
+      }
+      const int ii[3] = {ix, iy, iz};
+      T vc[3][N1], vr[3][N1];  // column ii[d] and row ii[d] of V_d
+      T lam = T(0);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+#pragma unroll
+        for (int m = 0; m < N1; ++m) {
+          vc[d][m] = E[d][m * N1 + ii[d]];
+          vr[d][m] = E[d][ii[d] * N1 + m];
+        }
+        lam += E[d][N1 * N1 + ii[d]];
+      }
+      T* b0 = buf[0] + cl * NL;
+      T* b1 = buf[1] + cl * NL;
+      if (lane) b0[l] = u;
+      __syncthreads();
+      T t1 = bj_sweep<N1, 0, T>(b0, vc[0], ix, iy, iz);
+      if (lane) b1[l] = t1;
+      __syncthreads();
+      T t2 = bj_sweep<N1, 1, T>(b1, vc[1], ix, iy, iz);
+      if (lane) b0[l] = t2;
+      __syncthreads();
+      T t3 = bj_sweep<N1, 2, T>(b0, vc[2], ix, iy, iz) / lam;
+      if (lane) b1[l] = t3;
+      __syncthreads();
+      T s1 = bj_sweep<N1, 0, T>(b1, vr[0], ix, iy, iz);
+      if (lane) b0[l] = s1;
+      __syncthreads();
+      T s2 = bj_sweep<N1, 1, T>(b0, vr[1], ix, iy, iz);
+      if (lane) b1[l] = s2;
+      __syncthreads();
+      res = bj_sweep<N1, 2, T>(b1, vr[2], ix, iy, iz);
+    } else {
+      long long ent;
+      if (g.per_cell) ent = cc;
+      else
+        ent = bj_type(bj_coord(g, cc, 0), g.n[0]) + 4 * bj_type(bj_coord(g, cc, 1), g.n[1]) +
+              16 * bj_type(bj_coord(g, cc, 2), g.n[2]);
+      const T* DT = data + ent * NL * NL;
+      T* b = buf[parity] + cl * NL;
+      if (lane) b[l] = u;
+      __syncthreads();
+      T s = T(0);
+#pragma unroll 8
+      for (int j = 0; j < NL; ++j) s += DT[j * NL + l] * b[j];
+      res = s;
+      parity ^= 1;
+    }
+    if (act) {
+      if (MODE == BJ_ADD) v.out[idx] += res;
+      else v.out[idx] = res;
+      if (MODE == BJ_PREC) acc0 += (A)res * (A)u;
+      if (MODE == BJ_CGR) acc1 += (A)res * (A)u;
+    }
+  }
+  if (MODE == BJ_PREC) {
+    A s[1] = {acc0};
+    block_sum_n<256, A, 1>(s);
+    if (tid == 0) v.partial[blockIdx.x] = s[0];
+  } else if (MODE == BJ_CGR) {
+    A s[2] = {acc0, acc1};
+    block_sum_n<256, A, 2>(s);
+    if (tid == 0) {
+      v.partial[2 * blockIdx.x] = s[0];
+      v.partial[2 * blockIdx.x + 1] = s[1];
+    }
+  }
+}
+
+}  // namespace pdelab_amd
+
+#endif

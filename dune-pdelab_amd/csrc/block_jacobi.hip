@@ -1,0 +1,206 @@
+// block_jacobi.hip -- set-up and launches of the per-cell block Jacobi preconditioner.
+#include <cmath>
+#include <cstring>
+
+#include "handle.hh"
+
+namespace {
+
+// Cholesky factor of the 1D mass matrix and its inverse, and M^{-1}, from the long
+// double tables.
+BJMass mass_data(const H* h) {
+  using LD = long double;
+  const int n = h->n1;
+  LD C[4][4] = {}, Ci[4][4] = {};
+  for (int j = 0; j < n; ++j) {
+    LD s = h->tab.M[j][j];
+    for (int k = 0; k < j; ++k) s -= C[j][k] * C[j][k];
+    C[j][j] = std::sqrt(s);
+    for (int i = j + 1; i < n; ++i) {
+      LD t = h->tab.M[i][j];
+      for (int k = 0; k < j; ++k) t -= C[i][k] * C[j][k];
+      C[i][j] = t / C[j][j];
+    }
+  }
+  for (int j = 0; j < n; ++j) {  // forward substitution C Ci = I, column by column
+    for (int i = 0; i < n; ++i) {
+      LD s = (i == j) ? 1 : 0;
+      for (int k = 0; k < i; ++k) s -= C[i][k] * Ci[k][j];
+      Ci[i][j] = s / C[i][i];
+    }
+  }
+  BJMass m;
+  std::memset(&m, 0, sizeof(m));
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      m.Ci[i][j] = (double)Ci[i][j];
+      m.Minv[i][j] = (double)h->tab.Minv[i][j];
+    }
+  return m;
+}
+
+BJGeom geom(const H* h, bool per_cell) {
+  BJGeom g;
+  g.vark = h->vark ? 1 : 0;
+  for (int d = 0; d < 3; ++d) {
+    g.n[d] = h->a64.n[d];
+    g.stride[d] = h->a64.stride[d];
+  }
+  g.ncells = h->ncells;
+  g.per_cell = per_cell ? 1 : 0;
+  return g;
+}
+
+template <int N1, bool TRANS, bool VARK>
+int fdm_setup(H* h, const BJGeom& g, long long entries, double* o64, float* o32) {
+  const int nb = (int)((entries + 255) / 256);
+  hipLaunchKernelGGL((bj_fdm_setup_kernel<N1, TRANS, VARK>), dim3(nb), dim3(256), 0, h->stream, h->a64,
+                     (const double*)h->d_fc64, g, mass_data(h), entries, o64, o32);
+  HIPCHK(h, hipGetLastError());
+  return PDELAB_OK;
+}
+
+template <int N1, bool TRANS, bool CONV, bool VARK>
+int dense_setup(H* h, const BJGeom& g, long long entries, int out_mode, double* o64, float* o32) {
+  // one workgroup per block
+  hipLaunchKernelGGL((bj_dense_setup_kernel<N1, TRANS, CONV, VARK>), dim3((unsigned)entries), dim3(256), 0, h->stream,
+                     h->a64, (const double*)h->d_fc64, g, mass_data(h), out_mode, o64, o32);
+  HIPCHK(h, hipGetLastError());
+  return PDELAB_OK;
+}
+
+template <int N1, bool TRANS>
+int setup_n(H* h, const BlockJacobiData& b, const BJGeom& g, double* o64, float* o32) {
+  if (b.fdm) {
+    return h->vark ? fdm_setup<N1, TRANS, true>(h, g, b.entries, o64, o32)
+                   : fdm_setup<N1, TRANS, false>(h, g, b.entries, o64, o32);
+  }
+  if (h->conv)
+    return h->vark ? dense_setup<N1, TRANS, true, true>(h, g, b.entries, 0, o64, o32)
+                   : dense_setup<N1, TRANS, true, false>(h, g, b.entries, 0, o64, o32);
+  return h->vark ? dense_setup<N1, TRANS, false, true>(h, g, b.entries, 0, o64, o32)
+                 : dense_setup<N1, TRANS, false, false>(h, g, b.entries, 0, o64, o32);
+}
+
+template <bool TRANS>
+int setup_t(H* h, const BlockJacobiData& b, const BJGeom& g, double* o64, float* o32) {
+  switch (h->n1) {
+    case 2: return setup_n<2, TRANS>(h, b, g, o64, o32);
+    case 3: return setup_n<3, TRANS>(h, b, g, o64, o32);
+    default: return setup_n<4, TRANS>(h, b, g, o64, o32);
+  }
+}
+
+template <int N1, typename T, typename A, int MODE>
+int launch_n(H* h, const BlockJacobiData& b, const T* data, const BJVec<T, A>& v) {
+  const BJGeom g = geom(h, b.per_cell);
+  const long long ngroups = (h->ncells + BJShape<N1>::CPG - 1) / BJShape<N1>::CPG;
+  // the reducing modes write one partial per workgroup for kRedBlocks workgroups
+  const int nb = MODE == BJ_ADD ? (int)std::min<long long>(ngroups, 8192) : kRedBlocks;
+  if (b.fdm)
+    hipLaunchKernelGGL((bj_apply_kernel<N1, T, A, MODE, true>), dim3(nb), dim3(256), 0, h->stream, v, data, g);
+  else
+    hipLaunchKernelGGL((bj_apply_kernel<N1, T, A, MODE, false>), dim3(nb), dim3(256), 0, h->stream, v, data, g);
+  HIPCHK(h, hipGetLastError());
+  return PDELAB_OK;
+}
+
+template <typename T, typename A, int MODE>
+int launch_m(H* h, const BlockJacobiData& b, const T* data, const BJVec<T, A>& v) {
+  switch (h->n1) {
+    case 2: return launch_n<2, T, A, MODE>(h, b, data, v);
+    case 3: return launch_n<3, T, A, MODE>(h, b, data, v);
+    default: return launch_n<4, T, A, MODE>(h, b, data, v);
+  }
+}
+
+template <typename T>
+const T* bj_data(const BlockJacobiData& b);
+template <>
+const double* bj_data<double>(const BlockJacobiData& b) { return b.d64; }
+template <>
+const float* bj_data<float>(const BlockJacobiData& b) { return b.d32; }
+
+}  // namespace
+
+void bj_invalidate(H* h) {
+  for (auto& b : h->bj) b.valid64 = b.valid32 = false;
+}
+
+void bj_free(H* h) {
+  for (auto& b : h->bj) {
+    if (b.d64) (void)hipFree(b.d64);
+    if (b.d32) (void)hipFree(b.d32);
+    b = BlockJacobiData();
+  }
+}
+
+int bj_prepare(H* h, bool trans, int precision) {
+  BlockJacobiData& b = h->bj[trans ? 1 : 0];
+  const bool fdm = !h->conv && h->cfg.method == PDELAB_SIPG;
+  const bool per_cell = h->vark;
+  const long long nl = h->nloc;
+  const long long entries = fdm ? (per_cell ? 3 * h->ncells : 12) : (per_cell ? h->ncells : 64);
+  const long long esize = fdm ? (long long)(h->n1 * h->n1 + h->n1) : nl * nl;
+  if (b.fdm != fdm || b.per_cell != per_cell || b.entries != entries || b.esize != esize) {
+    if (b.d64) (void)hipFree(b.d64);
+    if (b.d32) (void)hipFree(b.d32);
+    b = BlockJacobiData();
+    b.fdm = fdm;
+    b.per_cell = per_cell;
+    b.entries = entries;
+    b.esize = esize;
+  }
+  const bool need64 = precision == 64 && !b.valid64;
+  const bool need32 = precision == 32 && !b.valid32;
+  if (!need64 && !need32) return PDELAB_OK;
+  const size_t count = (size_t)(entries * esize);
+  if (need64 && !b.d64) HIPCHK(h, hipMalloc(&b.d64, sizeof(double) * count));
+  if (need32 && !b.d32) HIPCHK(h, hipMalloc(&b.d32, sizeof(float) * count));
+  const BJGeom g = geom(h, per_cell);
+  double* o64 = need64 ? b.d64 : nullptr;
+  float* o32 = need32 ? b.d32 : nullptr;
+  CHK(trans ? setup_t<true>(h, b, g, o64, o32) : setup_t<false>(h, b, g, o64, o32));
+  if (need64) b.valid64 = true;
+  if (need32) b.valid32 = true;
+  return PDELAB_OK;
+}
+
+template <typename T, typename A>
+int bj_launch(H* h, bool trans, int mode, const BJVec<T, A>& v) {
+  CHK(bj_prepare(h, trans, sizeof(T) == 8 ? 64 : 32));
+  const BlockJacobiData& b = h->bj[trans ? 1 : 0];
+  const T* data = bj_data<T>(b);
+  if (mode == BJ_PREC) return launch_m<T, A, BJ_PREC>(h, b, data, v);
+  if (mode == BJ_ADD) return launch_m<T, A, BJ_ADD>(h, b, data, v);
+  return launch_m<T, A, BJ_CGR>(h, b, data, v);
+}
+
+template int bj_launch<double, double>(H*, bool, int, const BJVec<double, double>&);
+template int bj_launch<float, float>(H*, bool, int, const BJVec<float, float>&);
+
+int bj_block_diagonal(H* h, bool trans, double* blocks) {
+  const BJGeom g = geom(h, true);
+  BlockJacobiData b;
+  b.entries = h->ncells;
+  auto run = [&](auto n1tag) -> int {
+    constexpr int N1 = decltype(n1tag)::value;
+    if (trans) {
+      if (h->conv)
+        return h->vark ? dense_setup<N1, true, true, true>(h, g, b.entries, 1, blocks, nullptr)
+                       : dense_setup<N1, true, true, false>(h, g, b.entries, 1, blocks, nullptr);
+      return h->vark ? dense_setup<N1, true, false, true>(h, g, b.entries, 1, blocks, nullptr)
+                     : dense_setup<N1, true, false, false>(h, g, b.entries, 1, blocks, nullptr);
+    }
+    if (h->conv)
+      return h->vark ? dense_setup<N1, false, true, true>(h, g, b.entries, 1, blocks, nullptr)
+                     : dense_setup<N1, false, true, false>(h, g, b.entries, 1, blocks, nullptr);
+    return h->vark ? dense_setup<N1, false, false, true>(h, g, b.entries, 1, blocks, nullptr)
+                   : dense_setup<N1, false, false, false>(h, g, b.entries, 1, blocks, nullptr);
+  };
+  switch (h->n1) {
+    case 2: return run(std::integral_constant<int, 2>());
+    case 3: return run(std::integral_constant<int, 3>());
+    default: return run(std::integral_constant<int, 4>());
+  }
+}

@@ -10,6 +10,11 @@
 
 namespace pdelab_amd {
 
+// Grid of the reducing kernels: kRedBlocks workgroups of kRedThreads write one partial
+// (or NR partials) each, summed in order by one workgroup.
+constexpr int kRedBlocks = 1024;
+constexpr int kRedThreads = 256;
+
 // Sum over the workgroup of NR values per thread (THREADS <= 1024, any multiple of
 // 32); the sums are valid in thread 0. Padded to whole waves with zeros in LDS, then
 // reduced by wave 0 in a fixed order.

@@ -9,11 +9,22 @@
 #include <string>
 
 #include "../../include/pdelab_gpu.h"
+#include "block_jacobi.hh"
 #include "dg_kernels.hh"
 #include "dg_load.hh"
 #include "dg_tables.hh"
 
 using namespace pdelab_amd;
+
+// Block Jacobi data of one operator (J or J^T): fast-diagonalisation entries or dense
+// transposed inverse blocks, per cell or per boundary type.
+struct BlockJacobiData {
+  bool fdm = false, per_cell = false;
+  bool valid64 = false, valid32 = false;
+  long long entries = 0, esize = 0;
+  double* d64 = nullptr;
+  float* d32 = nullptr;
+};
 
 struct pdelab_gpu_gridoperator {
   pdelab_config cfg;
@@ -61,6 +72,7 @@ struct pdelab_gpu_gridoperator {
   bool sym[2] = {false, false};  // J / J^T have the structure of line_op_sym
   // apply kernel on aligned-row grids: 0 the aligned-row kernel, 1 the general kernel
   int apply_variant = 0;
+  BlockJacobiData bj[2];  // [0] J, [1] J^T
 };
 
 using H = pdelab_gpu_gridoperator;
@@ -102,5 +114,15 @@ inline const LineMats<float>* line_mats<float>(H* h, bool trans) { return h->d_l
 template <int N1, typename T>
 int launch_apply_n(H* h, const T* z, T* y, const T* fc, const DGArgs<T>& a, bool trans, int mode, T* dotp,
                    const int* stop);
+
+// Block Jacobi (block_jacobi.hip): data set-up for the given precision (64 or 32),
+// invalidation when kappa changes, the launches (BJ_PREC / BJ_ADD / BJ_CGR, see
+// block_jacobi.hh) and the diagonal blocks themselves (ncells * nloc^2, row-major).
+int bj_prepare(H* h, bool trans, int precision);
+void bj_invalidate(H* h);
+void bj_free(H* h);
+template <typename T, typename A>
+int bj_launch(H* h, bool trans, int mode, const BJVec<T, A>& v);
+int bj_block_diagonal(H* h, bool trans, double* blocks);
 
 #endif

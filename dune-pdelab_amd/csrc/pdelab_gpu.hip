@@ -516,6 +516,41 @@ const double* dinv_for<double>(H* h, int precond) { return precond == PDELAB_PRE
 template <>
 const float* dinv_for<float>(H* h, int precond) { return precond == PDELAB_PREC_JACOBI ? h->d_dinv32 : nullptr; }
 
+// Preconditioner W of a Krylov solve: none, point Jacobi (dinv) or per-cell block Jacobi
+// of J (or of J^T for the transposed operator).
+template <typename T>
+struct Prec {
+  int kind = PDELAB_PREC_NONE;
+  const T* dinv = nullptr;
+  bool trans = false;
+  bool block() const { return kind == PDELAB_PREC_BLOCK_JACOBI; }
+};
+
+// Checks the preconditioner and prepares its data (point diagonal or block data).
+template <typename T>
+static int make_prec(H* h, int precond, bool trans, Prec<T>& P) {
+  P.kind = precond;
+  P.trans = trans;
+  if (precond == PDELAB_PREC_JACOBI) CHK(compute_diag(h));
+  else if (precond == PDELAB_PREC_BLOCK_JACOBI) CHK(bj_prepare(h, trans, sizeof(T) == 8 ? 64 : 32));
+  else if (precond != PDELAB_PREC_NONE) return fail(h, PDELAB_ERR_INVALID_ARG, "unknown preconditioner");
+  P.dinv = dinv_for<T>(h, precond);
+  return PDELAB_OK;
+}
+
+// out = W^{-1} in, with the partials of out.in in partial_buf (kRedBlocks values)
+template <typename T, typename A>
+static int prec_dot(H* h, const Prec<T>& P, const T* in, T* out) {
+  if (P.block()) {
+    const BJVec<T, A> v{in, out, nullptr, nullptr, nullptr, 0, partial_buf<A>(h), nullptr};
+    return bj_launch<T, A>(h, P.trans, BJ_PREC, v);
+  }
+  hipLaunchKernelGGL((prec_dot_kernel<T, A>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, h->N, P.dinv, in, out,
+                     partial_buf<A>(h));
+  HIPCHK(h, hipGetLastError());
+  return PDELAB_OK;
+}
+
 // CGSolver::apply (dune-istl): solves J z = b from the given z; b becomes the defect.
 // Workspace slots base, base+1 (p, q).
 //
@@ -540,8 +575,8 @@ struct CgLoopOut {
 // of iteration 1. Stops where dune-istl's rule fires against def_ref:
 // def < def_ref * red || def < 1e-30.
 template <typename T, typename A>
-static int cg_loop(H* h, T* x, T* b, T* p, T* q, const T* dinv, bool trans, int maxit, double def_ref, double red,
-                   CgLoopOut& out) {
+static int cg_loop(H* h, T* x, T* b, T* p, T* q, T* z, const Prec<T>& P, bool trans, int maxit, double def_ref,
+                   double red, CgLoopOut& out) {
   const long long n = h->N;
   const long long nbapply = apply_blocks(h);
   if (!h->d_applydot) HIPCHK(h, hipMalloc(&h->d_applydot, sizeof(double) * (size_t)std::max(nbapply, 1LL)));
@@ -564,12 +599,19 @@ static int cg_loop(H* h, T* x, T* b, T* p, T* q, const T* dinv, bool trans, int 
     CHK(apply_mode<T>(h, p, q, trans, 2, dotp, h->d_state));
     hipLaunchKernelGGL((final_reduce_kernel<A, 1>), dim3(1), dim3(kRedThreads), 0, h->stream, (const A*)dotp,
                        (int)nbapply, scal + 0, (const int*)h->d_state);
-    hipLaunchKernelGGL((cg_rupdate_kernel<T, A>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, n, b,
-                       (const T*)q, dinv, (const A*)scal, rslot, partial_buf<A>(h), (const int*)h->d_state);
+    if (P.block()) {
+      // r -= lambda q and z = W^{-1} r in one pass over the cells (z is kept for the p update)
+      const BJVec<T, A> v{b, z, b, q, scal, rslot, partial_buf<A>(h), h->d_state};
+      CHK((bj_launch<T, A>(h, P.trans, BJ_CGR, v)));
+    } else {
+      hipLaunchKernelGGL((cg_rupdate_kernel<T, A>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, n, b,
+                         (const T*)q, P.dinv, (const A*)scal, rslot, partial_buf<A>(h), (const int*)h->d_state);
+    }
     hipLaunchKernelGGL((cg_check_kernel<A>), dim3(1), dim3(kRedThreads), 0, h->stream, (const A*)partial_buf<A>(h),
                        kRedBlocks, scal + oslot, h->d_state, h->d_dstate, i);
     hipLaunchKernelGGL((cg_xpupdate_kernel<T, A>), dim3(grid_for(n)), dim3(256), 0, h->stream, n, x, p,
-                       (const T*)b, dinv, (const A*)scal, rslot, oslot + 1, (const int*)h->d_state, i);
+                       (const T*)b, P.dinv, (const T*)(P.block() ? z : nullptr), (const A*)scal, rslot, oslot + 1,
+                       (const int*)h->d_state, i);
     HIPCHK(h, hipGetLastError());
     last = i;
     if (i % kCheckEvery == 0 || i == maxit) {
@@ -603,10 +645,12 @@ static int cg_solve(H* h, T* x, T* b, double reduction, int maxit, int precond, 
                     pdelab_linear_solver_result* res) {
   auto t0 = std::chrono::steady_clock::now();
   const long long n = h->N;
-  CHK(ensure_ws<T>(h, base + 2));
+  Prec<T> P;
+  CHK(make_prec<T>(h, precond, trans, P));
+  CHK(ensure_ws<T>(h, base + (P.block() ? 3 : 2)));
   T* p = ws<T>(h)[base];
   T* q = ws<T>(h)[base + 1];
-  const T* dinv = dinv_for<T>(h, precond);
+  T* z = P.block() ? ws<T>(h)[base + 2] : nullptr;
   Iter it(reduction);
   // b -= J x  (applyscaleadd(-1, x, b))
   CHK(apply_t<T>(h, x, q, trans, false));
@@ -617,11 +661,10 @@ static int cg_solve(H* h, T* x, T* b, double reduction, int maxit, int precond, 
   const double def0 = std::sqrt((double)sc[1]);
   if (!it.step(0, def0)) {
     // p = W^{-1} b; rho = p.b  -> slot 7
-    hipLaunchKernelGGL((prec_dot_kernel<T, A>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, n, dinv,
-                       (const T*)b, p, partial_buf<A>(h));
+    CHK((prec_dot<T, A>(h, P, b, p)));
     CHK(finish_reduce<A, 1>(h, 7));
     CgLoopOut lo;
-    CHK((cg_loop<T, A>(h, x, b, p, q, dinv, trans, maxit, def0, reduction, lo)));
+    CHK((cg_loop<T, A>(h, x, b, p, q, z, P, trans, maxit, def0, reduction, lo)));
     it.def = lo.def;
     it.i = lo.iters;
     it.aborted = lo.aborted;
@@ -644,7 +687,8 @@ static int bicgstab_solve(H* h, T* x, T* r, double reduction, int maxit, int pre
   CHK(ensure_ws<T>(h, base + 5));
   T *p = ws<T>(h)[base], *v = ws<T>(h)[base + 1], *t = ws<T>(h)[base + 2], *y = ws<T>(h)[base + 3],
     *rt = ws<T>(h)[base + 4];
-  const T* dinv = dinv_for<T>(h, precond);
+  Prec<T> P;
+  CHK(make_prec<T>(h, precond, trans, P));
   const double EPSILON = 1e-80;
   Iter iter(reduction);
   int status = PDELAB_OK;
@@ -671,8 +715,7 @@ static int bicgstab_solve(H* h, T* x, T* r, double reduction, int maxit, int pre
                            (T)beta, (T)omega, 0);
       }
       // y = W^{-1} p ; v = J y ; h = rt.v
-      hipLaunchKernelGGL((prec_dot_kernel<T, A>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, n, dinv,
-                         (const T*)p, y, partial_buf<A>(h));
+      CHK((prec_dot<T, A>(h, P, p, y)));
       CHK(apply_t<T>(h, y, v, trans, false));
       CHK(dot_dev<T, A>(h, rt, v, n, 0));
       CHK(read_scalars<A>(h, sc, 1));
@@ -687,8 +730,7 @@ static int bicgstab_solve(H* h, T* x, T* r, double reduction, int maxit, int pre
       if (iter.step(itc, norm)) break;
       itc += .5;
       // y = W^{-1} r ; t = J y ; omega = t.r / t.t
-      hipLaunchKernelGGL((prec_dot_kernel<T, A>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, n, dinv,
-                         (const T*)r, y, partial_buf<A>(h));
+      CHK((prec_dot<T, A>(h, P, r, y)));
       CHK(apply_t<T>(h, y, t, trans, false));
       hipLaunchKernelGGL((dot2_kernel<T, A>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, n, (const T*)t,
                          (const T*)r, partial_buf<A>(h));
@@ -718,8 +760,6 @@ template <typename T, typename A>
 static int krylov(H* h, int method, int precond, double reduction, int maxit, bool trans, T* z, T* r, int base,
                   pdelab_linear_solver_result* res) {
   if (maxit <= 0) maxit = 5000;
-  if (precond == PDELAB_PREC_JACOBI) CHK(compute_diag(h));
-  else if (precond != PDELAB_PREC_NONE) return fail(h, PDELAB_ERR_INVALID_ARG, "unknown preconditioner");
   if (method == PDELAB_SOLVER_CG) {
     if constexpr (std::is_same<T, A>::value) return cg_solve<T, A>(h, z, r, reduction, maxit, precond, trans, base, res);
     else return fail(h, PDELAB_ERR_UNSUPPORTED, "CG accumulates in its vector type");
@@ -761,6 +801,7 @@ static int update_face_coeffs(H* h) {
   hipLaunchKernelGGL(face_coeff_kernel<float>, dim3(nb), dim3(256), 0, h->stream, (const double*)h->d_kappa,
                      h->d_fc32, h->a64);
   HIPCHK(h, hipGetLastError());
+  bj_invalidate(h);
   return PDELAB_OK;
 }
 
@@ -793,13 +834,12 @@ static int upload_kappa(H* h, const double* kappa_host) {
 static int cg_mixed_reliable(H* h, int precond, double inner_reduction, int inner_maxit, double outer_reduction,
                              int outer_maxit, double* x, pdelab_mixed_result* res) {
   const long long n = h->N;
+  Prec<float> P;
+  CHK(make_prec<float>(h, precond, false, P));
   CHK(ensure_ws<double>(h, 6));
-  CHK(ensure_ws<float>(h, 4));
+  CHK(ensure_ws<float>(h, 5));
   double* r64 = h->w64[5];
-  float *r32 = h->w32[0], *x32 = h->w32[1], *p32 = h->w32[2], *q32 = h->w32[3];
-  if (precond == PDELAB_PREC_JACOBI) CHK(compute_diag(h));
-  else if (precond != PDELAB_PREC_NONE) return fail(h, PDELAB_ERR_INVALID_ARG, "unknown preconditioner");
-  const float* dinv = dinv_for<float>(h, precond);
+  float *r32 = h->w32[0], *x32 = h->w32[1], *p32 = h->w32[2], *q32 = h->w32[3], *z32 = h->w32[4];
   auto residual64 = [&](double& d) -> int {
     CHK(apply64(h, x, r64, false, false));
     hipLaunchKernelGGL(axpy_kernel<double>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, 1.0,
@@ -824,14 +864,13 @@ static int cg_mixed_reliable(H* h, int precond, double inner_reduction, int inne
     CHK(to_f32());
     hipLaunchKernelGGL(fill_kernel<float>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, 0.0f, x32);
     // p = W^{-1} r; rho -> slot 7
-    hipLaunchKernelGGL((prec_dot_kernel<float, float>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, n, dinv,
-                       (const float*)r32, p32, partial_buf<float>(h));
+    CHK((prec_dot<float, float>(h, P, r32, p32)));
     CHK((finish_reduce<float, 1>(h, 7)));
     for (;;) {
       const double thr = std::max(inner_reduction * def, 0.5 * outer_reduction * def0);
       CgLoopOut lo;
-      CHK((cg_loop<float, float>(h, x32, r32, p32, q32, dinv, false, std::max(1, inner_maxit - inner), def, thr / def,
-                                 lo)));
+      CHK((cg_loop<float, float>(h, x32, r32, p32, q32, z32, P, false, std::max(1, inner_maxit - inner), def,
+                                 thr / def, lo)));
       inner += lo.iters;
       if (lo.aborted) {
         res->inner_iterations = inner;
@@ -853,13 +892,13 @@ static int cg_mixed_reliable(H* h, int precond, double inner_reduction, int inne
       CHK((read_scalars<float>(h, sc, 8)));
       const double rho_old = sc[lo.rho_slot];
       CHK(to_f32());
-      hipLaunchKernelGGL((prec_dot_kernel<float, float>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, n, dinv,
-                         (const float*)r32, q32, partial_buf<float>(h));
+      CHK((prec_dot<float, float>(h, P, r32, q32)));
       CHK((finish_reduce<float, 1>(h, 7)));
       CHK((read_scalars<float>(h, sc, 8)));
       const float beta = rho_old != 0 ? (float)(sc[7] / rho_old) : 0.0f;
+      // p = q + beta p with q = W^{-1} r
       hipLaunchKernelGGL(cg_pupdate_kernel<float>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, p32,
-                         (const float*)r32, dinv, beta);
+                         (const float*)q32, (const float*)nullptr, beta);
       HIPCHK(h, hipGetLastError());
     }
   }
@@ -959,6 +998,7 @@ int pdelab_gpu_destroy(pdelab_gpu_gridoperator* h) {
                   h->d_state, h->d_dstate, h->d_lm64, h->d_lm32, h->d_fc64, h->d_fc32};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  bj_free(h);
   for (int i = 0; i < 8; ++i) {
     if (h->w64[i]) (void)hipFree(h->w64[i]);
     if (h->w32[i]) (void)hipFree(h->w32[i]);
@@ -1112,6 +1152,26 @@ int pdelab_gpu_point_diagonal(pdelab_gpu_gridoperator* h, double* d) {
                      (const double*)h->d_diag, d);
   HIPCHK(h, hipGetLastError());
   return PDELAB_OK;
+}
+
+int pdelab_gpu_block_diagonal(pdelab_gpu_gridoperator* h, int transpose, double* blocks) {
+  if (!h || !blocks) return PDELAB_ERR_INVALID_ARG;
+  CHK(check_outflow(h));
+  return bj_block_diagonal(h, transpose != 0, blocks);
+}
+
+int pdelab_gpu_block_jacobi_apply(pdelab_gpu_gridoperator* h, const double* z, double* y) {
+  if (!h || !z || !y) return PDELAB_ERR_INVALID_ARG;
+  CHK(check_outflow(h));
+  const BJVec<double, double> v{z, y, nullptr, nullptr, nullptr, 0, nullptr, nullptr};
+  return bj_launch<double, double>(h, false, BJ_ADD, v);
+}
+
+int pdelab_gpu_block_jacobi_apply_f32(pdelab_gpu_gridoperator* h, const float* z, float* y) {
+  if (!h || !z || !y) return PDELAB_ERR_INVALID_ARG;
+  CHK(check_outflow(h));
+  const BJVec<float, float> v{z, y, nullptr, nullptr, nullptr, 0, nullptr, nullptr};
+  return bj_launch<float, float>(h, false, BJ_ADD, v);
 }
 
 int pdelab_gpu_mass_apply(pdelab_gpu_gridoperator* h, const double* z, double* y) {

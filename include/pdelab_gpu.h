@@ -14,6 +14,12 @@
  *                                    linear operator) gridoperator.hh:200-205
  *   pdelab_gpu_point_diagonal        residual of PointDiagonalLocalOperatorWrapper
  *                                    R:dune/pdelab/localoperator/pointdiagonalwrapper.hh:51-131
+ *   pdelab_gpu_block_diagonal        jacobian of BlockDiagonalLocalOperatorWrapper
+ *                                    R:dune/pdelab/localoperator/blockdiagonalwrapper.hh:79-160
+ *   pdelab_gpu_block_jacobi_apply    GridOperatorPreconditioner::apply over
+ *                                    AssembledBlockJacobiPreconditionerLocalOperator
+ *                                    R:dune/pdelab/backend/istl/matrixfree/gridoperatorpreconditioner.hh:78-84,
+ *                                    assembledblockjacobipreconditioner.hh:193-227
  *   pdelab_gpu_solve                 ISTLBackend_SEQ_{CG,BCGS}_Jac::apply /
  *                                    ISTLBackend_SEQ_MatrixFree_Base::apply(z, r, reduction)
  *                                    R:dune/pdelab/backend/istl/seqistlsolverbackend.hh:231-252
@@ -92,7 +98,12 @@ enum {
 
 /* Krylov method and preconditioner of the ISTL backends. */
 enum { PDELAB_SOLVER_CG = 0, PDELAB_SOLVER_BICGSTAB = 1 };
-enum { PDELAB_PREC_NONE = 0, PDELAB_PREC_JACOBI = 1 };
+/* PDELAB_PREC_JACOBI: point Jacobi (Blocking::none); PDELAB_PREC_BLOCK_JACOBI: per-cell
+   block Jacobi with exact block inverses (Blocking::fixed SeqJac,
+   R:dune/pdelab/backend/istl/seqistlsolverbackend.hh:240-243, and the matrix-free
+   AssembledBlockJacobiPreconditionerLocalOperator,
+   R:dune/pdelab/backend/istl/matrixfree/assembledblockjacobipreconditioner.hh:96-242). */
+enum { PDELAB_PREC_NONE = 0, PDELAB_PREC_JACOBI = 1, PDELAB_PREC_BLOCK_JACOBI = 2 };
 
 typedef struct pdelab_config {
   int abi_version;            /* must be PDELAB_GPU_ABI_VERSION */
@@ -210,6 +221,16 @@ int pdelab_gpu_jacobian_apply_nonlinear(pdelab_gpu_gridoperator* h, const double
 int pdelab_gpu_jacobian_apply_transpose(pdelab_gpu_gridoperator* h, const double* z, double* y);
 /* d += diag(J) */
 int pdelab_gpu_point_diagonal(pdelab_gpu_gridoperator* h, double* d);
+/* blocks[c][i][j] = dR_i/dx_j of cell c within cell c: the diagonal blocks of J (of J^T
+   if transpose), ncells * (k+1)^6 doubles, row-major per cell, device memory
+   (overwritten). */
+int pdelab_gpu_block_diagonal(pdelab_gpu_gridoperator* h, int transpose, double* blocks);
+/* y += D^{-1} z with D the block diagonal of J: the block Jacobi preconditioner as the
+   Krylov solvers apply it with PDELAB_PREC_BLOCK_JACOBI. The blocks are inverted
+   once (and again after the next kappa change). */
+int pdelab_gpu_block_jacobi_apply(pdelab_gpu_gridoperator* h, const double* z, double* y);
+/* FP32 form of pdelab_gpu_block_jacobi_apply (inverse blocks rounded to FP32). */
+int pdelab_gpu_block_jacobi_apply_f32(pdelab_gpu_gridoperator* h, const float* z, float* y);
 /* y += M z, DG mass matrix (ScalarL2 with scaling 1). */
 int pdelab_gpu_mass_apply(pdelab_gpu_gridoperator* h, const double* z, double* y);
 

@@ -208,6 +208,30 @@ struct Oracle {
     }
   }
 
+  // BlockDiagonalLocalOperatorWrapper jacobian: jacobian_volume, both sides of every
+  // interior face keeping the ss block, and the boundary faces,
+  // R:dune/pdelab/localoperator/blockdiagonalwrapper.hh:79-160. blocks[c][i][j] row-major.
+  void block_diagonal(double* blocks) const {
+    const size_t n2 = nloc * nloc;
+    std::vector<double> mss(n2), o1(n2), o2(n2), o3(n2);
+    for (size_t c = 0; c < grid.ncells; ++c) {
+      double* m = blocks + c * n2;
+      std::fill(m, m + n2, 0.0);
+      lop.jacobian_volume(c, m);
+      for (int f = 0; f < 6; ++f) {
+        size_t cn;
+        if (grid.neighbor(c, f, cn)) {
+          std::fill(mss.begin(), mss.end(), 0.0);
+          std::fill(o1.begin(), o1.end(), 0.0); std::fill(o2.begin(), o2.end(), 0.0); std::fill(o3.begin(), o3.end(), 0.0);
+          lop.jacobian_skeleton(c, f, cn, mss.data(), o1.data(), o2.data(), o3.data());
+          for (size_t i = 0; i < n2; ++i) m[i] += mss[i];
+        } else {
+          lop.jacobian_boundary(c, f, m);
+        }
+      }
+    }
+  }
+
   void mass_apply(const double* z, double* y) const {
     std::vector<double> yl(nloc);
     for (size_t c = 0; c < grid.ncells; ++c) {
@@ -421,9 +445,47 @@ static void bicgstab(size_t n, const OpFn& A, const OpFn& prec, double reduction
 
 using oracle::Oracle;
 
+// LU factorisation with partial pivoting of one diagonal block (in place, row-major) and
+// its solve: what AssembledBlockJacobiPreconditionerLocalOperator does with Eigen's
+// PartialPivLU, R:dune/pdelab/backend/istl/matrixfree/assembledblockjacobipreconditioner.hh:170-227.
+static void lu_factor(size_t n, double* A, int* piv) {
+  for (size_t k = 0; k < n; ++k) {
+    size_t p = k;
+    for (size_t i = k + 1; i < n; ++i)
+      if (std::fabs(A[i * n + k]) > std::fabs(A[p * n + k])) p = i;
+    piv[k] = (int)p;
+    if (p != k)
+      for (size_t j = 0; j < n; ++j) std::swap(A[k * n + j], A[p * n + j]);
+    const double d = A[k * n + k];
+    for (size_t i = k + 1; i < n; ++i) {
+      const double l = A[i * n + k] / d;
+      A[i * n + k] = l;
+      for (size_t j = k + 1; j < n; ++j) A[i * n + j] -= l * A[k * n + j];
+    }
+  }
+}
+
+static void lu_solve(size_t n, const double* LU, const int* piv, const double* b, double* x) {
+  for (size_t i = 0; i < n; ++i) x[i] = b[i];
+  for (size_t k = 0; k < n; ++k)
+    if ((size_t)piv[k] != k) std::swap(x[k], x[piv[k]]);
+  for (size_t i = 0; i < n; ++i) {  // L has ones on the diagonal
+    double s = x[i];
+    for (size_t j = 0; j < i; ++j) s -= LU[i * n + j] * x[j];
+    x[i] = s;
+  }
+  for (size_t i = n; i-- > 0;) {
+    double s = x[i];
+    for (size_t j = i + 1; j < n; ++j) s -= LU[i * n + j] * x[j];
+    x[i] = s / LU[i * n + i];
+  }
+}
+
 struct oracle_handle {
   std::unique_ptr<Oracle> o;
   std::vector<double> diag;  // cached point diagonal
+  std::vector<double> blu[2];  // LU factors of the diagonal blocks of J and J^T
+  std::vector<int> bpiv[2];
   std::vector<double> blocks;
   std::vector<long long> cols;
   int threads = 1;
@@ -456,6 +518,7 @@ int oracle_set_kappa(oracle_handle* h, const double* kappa) {
   h->o->param.kappa.assign(kappa, kappa + h->o->grid.ncells);
   h->diag.clear();
   h->blocks.clear();
+  for (int t = 0; t < 2; ++t) { h->blu[t].clear(); h->bpiv[t].clear(); }
   return 0;
 }
 int oracle_set_cell_source(oracle_handle* h, const double* s) {
@@ -491,6 +554,9 @@ int oracle_point_diagonal(oracle_handle* h, double* d) {
 int oracle_assemble(oracle_handle* h, double* blocks, long long* cols) {
   return guard(h, [&] { h->o->assemble(blocks, cols); });
 }
+int oracle_block_diagonal(oracle_handle* h, double* blocks) {
+  return guard(h, [&] { h->o->block_diagonal(blocks); });
+}
 int oracle_mass_apply(oracle_handle* h, const double* z, double* y) {
   return guard(h, [&] { h->o->mass_apply(z, y); });
 }
@@ -517,6 +583,20 @@ int oracle_solve(oracle_handle* h, int method, int precond, double reduction, in
     if (precond == PDELAB_PREC_JACOBI && h->diag.empty()) {
       h->diag.assign(n, 0.0);
       o.point_diagonal(h->diag.data());
+    }
+    const int bt = matrix_based == 2 ? 1 : 0;  // blocks of J^T for the transposed operator
+    if (precond == PDELAB_PREC_BLOCK_JACOBI && h->blu[bt].empty()) {
+      std::vector<double>& B = h->blu[bt];
+      B.resize(o.grid.ncells * n2);
+      h->bpiv[bt].resize(n);
+      o.block_diagonal(B.data());
+      for (size_t c = 0; c < o.grid.ncells; ++c) {
+        double* A = &B[c * n2];
+        if (bt)
+          for (size_t i = 0; i < nloc; ++i)
+            for (size_t j = i + 1; j < nloc; ++j) std::swap(A[i * nloc + j], A[j * nloc + i]);
+        lu_factor(nloc, A, &h->bpiv[bt][c * nloc]);
+      }
     }
     oracle::OpFn A;
     if (matrix_based == 1) {
@@ -554,6 +634,9 @@ int oracle_solve(oracle_handle* h, int method, int precond, double reduction, in
     oracle::OpFn P = [&](const double* d, double* v) {
       if (precond == PDELAB_PREC_JACOBI)
         for (size_t i = 0; i < n; ++i) v[i] = d[i] / h->diag[i];
+      else if (precond == PDELAB_PREC_BLOCK_JACOBI)
+        for (size_t c = 0; c < o.grid.ncells; ++c)
+          lu_solve(nloc, &h->blu[bt][c * n2], &h->bpiv[bt][c * nloc], d + c * nloc, v + c * nloc);
       else
         std::copy(d, d + n, v);
     };

@@ -56,7 +56,8 @@ def lib():
         L.oracle_size.restype = ctypes.c_size_t
         L.oracle_size.argtypes = [vp]
         L.oracle_set_threads.argtypes = [vp, ctypes.c_int]
-        for fn in ("oracle_set_kappa", "oracle_set_cell_source", "oracle_point_diagonal", "oracle_interpolate"):
+        for fn in ("oracle_set_kappa", "oracle_set_cell_source", "oracle_point_diagonal", "oracle_interpolate",
+                   "oracle_block_diagonal"):
             getattr(L, fn).argtypes = [vp, _dp]
         L.oracle_set_tabulated.argtypes = [vp, _dp, _dp]
         L.oracle_quadrature_points.argtypes = [vp]
@@ -182,6 +183,13 @@ class Oracle:
         d = np.zeros(self.N)
         self._check(lib().oracle_point_diagonal(self.h, _p(d)))
         return d
+
+    def block_diagonal(self):
+        """Diagonal blocks of J, shape (ncells, nloc, nloc), row-major (BlockDiagonalLocalOperatorWrapper)."""
+        nloc = (self.cfg.degree + 1) ** 3
+        b = np.zeros(self.ncells * nloc * nloc)
+        self._check(lib().oracle_block_diagonal(self.h, _p(b)))
+        return b.reshape(self.ncells, nloc, nloc)
 
     def assemble(self):
         nloc = (self.cfg.degree + 1) ** 3

@@ -15,6 +15,10 @@ operand, wrong index) fails at least one:
   * L2 convergence order k+1 and the reference tests' error thresholds in 3D
     (R:dune/pdelab/test/testconvectiondiffusiondg.cc:160-168);
   * solvers against a direct solve, matrix-based vs matrix-free iteration equality;
+  * block Jacobi: block diagonal == the assembled diagonal blocks, exact on one cell,
+    iteration counts equal to an independent numpy PCG with inverted blocks, and the
+    matrix-free == matrix-based iteration equality with block Jacobi
+    (R:dune/pdelab/test/matrixfree/matrix_free_linear.cc:253-277,390-394);
   * the adjoint gradient against central finite differences.
 """
 import json
@@ -193,6 +197,118 @@ def test_matrix_free_equals_matrix_based_iterations(of):
     _, r3 = o.linear_problem_solve(method=1, reduction=1e-10, matrix_based=0)
     _, r4 = o.linear_problem_solve(method=1, reduction=1e-10, matrix_based=1)
     assert r3["solver"]["iterations"] == r4["solver"]["iterations"]
+
+
+BJ_CASES = [
+    ("sipg_q2", dict(nx=3, ny=2, nz=4, degree=2, alpha=3.0), None),
+    ("sipg_q3_var", dict(nx=2, ny=3, nz=2, degree=3, alpha=3.0), 1),
+    ("nipg_q1_var", dict(nx=4, ny=3, nz=2, degree=1, method=0, alpha=1.0), 2),
+    ("conv_react_q2", dict(nx=3, ny=3, nz=2, degree=2, alpha=3.0, b=(1.0, -0.5, 0.25), c=0.7, kappa0=0.01), None),
+    ("outflow_q1", dict(nx=3, ny=2, nz=3, degree=1, alpha=3.0, b=(1.0, 0.0, 0.0), bctype=(0, -2, 0, -1, -3, 0)), None),
+]
+
+
+def _oracle_case(of, kw, seed):
+    cfg = of.cfgmod.make_config(**kw)
+    kappa = None
+    if seed is not None:
+        kappa = np.exp(np.random.default_rng(seed).normal(size=of.cfgmod.ncells(cfg)))
+        cfg.kappa_per_cell = 1
+    return cfg, of.Oracle(cfg, kappa)
+
+
+@pytest.mark.parametrize("name,kw,seed", BJ_CASES, ids=[c[0] for c in BJ_CASES])
+def test_block_diagonal_equals_assembled_blocks(of, name, kw, seed):
+    """BlockDiagonalLocalOperatorWrapper (two-sided skeleton, ss part) against slot 0 of the
+    assembled matrix (one-sided skeleton, ss and nn parts): the same diagonal blocks."""
+    _, o = _oracle_case(of, kw, seed)
+    blocks, cols = o.assemble()
+    assert (cols[:, 0] == np.arange(o.ncells)).all()
+    bd = o.block_diagonal()
+    assert np.abs(bd - blocks[:, 0]).max() <= 1e-13 * np.abs(blocks[:, 0]).max()
+
+
+def test_block_jacobi_exact_on_one_cell(of):
+    """On a single cell the block diagonal is the whole matrix: block-Jacobi CG and
+    BiCGSTAB converge in one step (BiCGSTAB: its first half step, counted as 1) to the
+    direct solution."""
+    cfg = of.cfgmod.make_config(n=1, degree=3, alpha=3.0, problem=of.cfgmod.PROBLEM_EXP)
+    o = of.Oracle(cfg)
+    A = o.dense_matrix()
+    r0 = o.residual(np.zeros(o.N))
+    xd = -np.linalg.solve(A, r0)
+    for method, its in ((0, 1), (1, 1)):
+        x, res = o.linear_problem_solve(method=method, precond=of.cfgmod.PREC_BLOCK_JACOBI, reduction=1e-10)
+        assert res["solver"]["converged"]
+        assert res["solver"]["iterations"] == its
+        assert np.abs(x - xd).max() <= 1e-12 * np.abs(xd).max()
+
+
+def _numpy_pcg(A, b, prec, reduction, maxit=10000):
+    """Textbook preconditioned CG (x0 = 0) with dune-istl's stopping rule ||r|| < red ||r0||."""
+    x = np.zeros_like(b)
+    r = b.copy()
+    z = prec(r)
+    p = z.copy()
+    rho = r @ z
+    r0 = np.linalg.norm(r)
+    for it in range(1, maxit + 1):
+        q = A @ p
+        alpha = rho / (p @ q)
+        x += alpha * p
+        r -= alpha * q
+        if np.linalg.norm(r) < reduction * r0:
+            return x, it
+        z = prec(r)
+        rho_new = r @ z
+        p = z + (rho_new / rho) * p
+        rho = rho_new
+    return x, -1
+
+
+@pytest.mark.parametrize("seed", [None, 3], ids=["const", "var"])
+def test_block_jacobi_cg_vs_numpy_pcg(of, seed):
+    """Block-Jacobi CG of the oracle against a numpy PCG whose preconditioner inverts the
+    assembled diagonal blocks with numpy.linalg.inv: same iteration count, same solution.
+    The count is far below point Jacobi's (the blocks carry the Q3 coupling)."""
+    cfg, o = _oracle_case(of, dict(n=4, degree=3, alpha=3.0), seed)
+    blocks, _ = o.assemble()
+    inv = np.linalg.inv(blocks[:, 0])
+    nloc = blocks.shape[-1]
+    A = o.dense_matrix()
+    rhs = np.random.default_rng(5).normal(size=o.N)
+    x_np, it_np = _numpy_pcg(A, rhs, lambda r: np.einsum("cij,cj->ci", inv, r.reshape(-1, nloc)).ravel(), 1e-10)
+    z, _, res = o.solve(rhs.copy(), precond=of.cfgmod.PREC_BLOCK_JACOBI, reduction=1e-10)
+    assert res["converged"]
+    assert abs(res["iterations"] - it_np) <= 1
+    assert np.abs(z - x_np).max() <= 1e-8 * np.abs(x_np).max()
+    _, _, res_pj = o.solve(rhs.copy(), precond=of.cfgmod.PREC_JACOBI, reduction=1e-10)
+    assert res["iterations"] < 0.8 * res_pj["iterations"]
+
+
+def test_block_jacobi_matrix_free_equals_matrix_based_iterations(of):
+    """matrix_free_linear.cc:253-277,390-394 with the partially matrix-free preconditioner:
+    BiCGSTAB with block Jacobi takes the same iteration count matrix-free (on-the-fly
+    operator) and matrix-based (assembled operator)."""
+    cfg = of.cfgmod.make_config(n=6, degree=1, alpha=3.0, problem=of.cfgmod.PROBLEM_EXP)
+    o = of.Oracle(cfg)
+    bj = of.cfgmod.PREC_BLOCK_JACOBI
+    _, r1 = o.linear_problem_solve(method=1, precond=bj, reduction=1e-12, matrix_based=0)
+    _, r2 = o.linear_problem_solve(method=1, precond=bj, reduction=1e-12, matrix_based=1)
+    assert r1["solver"]["converged"] and r2["solver"]["converged"]
+    assert r1["solver"]["iterations"] == r2["solver"]["iterations"]
+
+
+def test_block_jacobi_transposed_operator(of):
+    """Transposed solve (matrix_based=2) with block Jacobi of J^T on a convection problem:
+    the solution satisfies J^T z = r."""
+    cfg = of.cfgmod.make_config(n=3, degree=2, alpha=3.0, kappa0=0.05, b=(1.0, 0.5, -0.25))
+    o = of.Oracle(cfg)
+    rhs = np.random.default_rng(6).normal(size=o.N)
+    z, _, res = o.solve(rhs.copy(), method=1, precond=of.cfgmod.PREC_BLOCK_JACOBI, reduction=1e-12, matrix_based=2)
+    assert res["converged"]
+    At = o.dense_matrix().T
+    assert np.linalg.norm(At @ z - rhs) <= 1e-11 * np.linalg.norm(rhs)
 
 
 def test_bicgstab_convection_diffusion(of):
