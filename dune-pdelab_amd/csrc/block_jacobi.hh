@@ -45,13 +45,31 @@ struct BJMass {
   double Minv[4][4];
 };
 
+// Dense application: one lane per DoF, NLP lanes per cell (a power of two so that a cell
+// never straddles two waves).
 template <int N1>
 struct BJShape {
   static constexpr int NL = N1 * N1 * N1;
-  static constexpr int THREADS = 256;
-  static constexpr int CPG = THREADS / NL;  // cells per workgroup: 32 (Q1), 9 (Q2), 4 (Q3)
-  static constexpr int USED = CPG * NL;
-  static constexpr int ES = N1 * N1 + N1;   // fast-diagonalisation entry: V (row-major), lam
+  static constexpr int NLP = NL <= 8 ? 8 : (NL <= 32 ? 32 : 64);
+  static constexpr int CPW = 64 / NLP;       // cells per wave: 8 (Q1), 2 (Q2), 1 (Q3)
+  static constexpr int WAVES = 16;
+  static constexpr int THREADS = 64 * WAVES;
+  static constexpr int CPG = CPW * WAVES;    // cells per workgroup: 128 (Q1), 32 (Q2), 16 (Q3)
+  static constexpr int ES = N1 * N1 + N1;    // fast-diagonalisation entry: V (row-major), lam
+};
+
+// Fast-diagonalisation application: one lane per z-line (N1 DoFs in registers), N1^2
+// lanes per cell, as many whole cells per wave as fit.
+template <int N1>
+struct BJFdmShape {
+  static constexpr int NL = N1 * N1 * N1;
+  static constexpr int LPC = N1 * N1;        // lanes per cell
+  static constexpr int CPW = 64 / LPC;       // cells per wave: 16 (Q1), 7 (Q2), 4 (Q3)
+  static constexpr int WAVES = 8;
+  static constexpr int THREADS = 64 * WAVES;
+  static constexpr int CPG = CPW * WAVES;    // cells per workgroup: 128 (Q1), 56 (Q2), 32 (Q3)
+  static constexpr int ES = N1 * N1 + N1;
+  static constexpr int EW = CPW * 3 * ES;    // per-cell entry values staged per wave
 };
 
 // ---------------------------------------------------------------------------
@@ -101,9 +119,21 @@ __device__ inline void bj_cell_coeffs(const DGArgs<double>& a, const double* __r
   }
 }
 
+// Coordinate d of cell c with 32-bit divisions (the host keeps ncells below 2^31).
 __device__ inline int bj_coord(const BJGeom& g, long long c, int d) {
-  return d == 0 ? (int)(c % g.n[0]) : (d == 1 ? (int)((c / g.n[0]) % g.n[1]) : (int)(c / ((long long)g.n[0] * g.n[1])));
+  const unsigned cu = (unsigned)c, n0 = (unsigned)g.n[0], n1 = (unsigned)g.n[1];
+  const unsigned q = cu / n0;
+  return d == 0 ? (int)(cu - q * n0) : (d == 1 ? (int)(q % n1) : (int)(q / n1));
 }
+
+// 1/x: the hardware reciprocal refined by two Newton steps (FP64), or the hardware
+// reciprocal (FP32, 1 ulp).
+__device__ __forceinline__ double bj_rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(fma(-x, r, 1.0), r, r);
+  return fma(fma(-x, r, 1.0), r, r);
+}
+__device__ __forceinline__ float bj_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // Symmetric eigen-decomposition S = Q diag(w) Q^T by cyclic Jacobi rotations
 // (Golub & Van Loan, Matrix Computations, 4th ed., algorithm 8.5.2).
@@ -366,12 +396,13 @@ __global__ __launch_bounds__(256) void bj_dense_setup_kernel(const DGArgs<double
 }
 
 // ---------------------------------------------------------------------------
-// Application. One workgroup takes CPG cells at a time (grid-stride over cell groups);
-// each lane owns one DoF, loaded and stored coalesced.
-//   BJ_PREC: out = W^{-1} in, partials of out.in (one per workgroup)
+// Application. A workgroup takes CPG consecutive cells; every cell lies inside one wave
+// (NLP lanes, NL of them own a DoF), so the 1D sweeps exchange values through the wave's
+// own LDS region with no workgroup barrier. Vectors are loaded and stored coalesced.
+//   BJ_PREC: out = W^{-1} in, one partial of out.in per workgroup
 //   BJ_ADD : out += W^{-1} in
 //   BJ_CGR : r -= lambda q (lambda = scal[slot_rho] / scal[0]), out = W^{-1} r, partials
-//            of (r.r, out.r) (two per workgroup); returns at once when state[0] is set
+//            of (r.r, out.r) per workgroup; returns at once when state[0] is set
 template <typename T, typename A>
 struct BJVec {
   const T* in;
@@ -384,6 +415,14 @@ struct BJVec {
   const int* state;
 };
 
+// Orders the LDS accesses of one wave (the wave executes them in program order; the
+// fences keep the compiler from moving them across), as rocPRIM's wave_barrier does.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int N1, int D, typename T>
 __device__ __forceinline__ T bj_sweep(const T* __restrict__ src, const T (&w)[N1], int ix, int iy, int iz) {
   T s = T(0);
@@ -395,111 +434,233 @@ __device__ __forceinline__ T bj_sweep(const T* __restrict__ src, const T (&w)[N1
   return s;
 }
 
-template <int N1, typename T, typename A, int MODE, bool FDM>
-__global__ __launch_bounds__(256) void bj_apply_kernel(const BJVec<T, A> v, const T* __restrict__ data,
-                                                       const BJGeom g) {
+// Dense inverse blocks: lane l of a cell computes row l of D^{-1} r_c.
+template <int N1, typename T, typename A, int MODE>
+__global__ __launch_bounds__(BJShape<N1>::THREADS) void bj_dense_kernel(const BJVec<T, A> v, const T* __restrict__ data,
+                                                                       const BJGeom g) {
   using S = BJShape<N1>;
-  constexpr int NL = S::NL, CPG = S::CPG;
+  constexpr int NL = S::NL, NLP = S::NLP, CPW = S::CPW;
   if (MODE == BJ_CGR && v.state[0]) return;
-  __shared__ T buf[2][CPG * NL];
+  __shared__ T buf[S::WAVES][64];
   const int tid = threadIdx.x;
-  const bool lane = tid < S::USED;  // Q2: the last 13 lanes own no DoF
-  const int cl = lane ? tid / NL : 0, l = lane ? tid - cl * NL : 0;
-  const int ix = l % N1, iy = (l / N1) % N1, iz = l / (N1 * N1);
+  const int w = tid >> 6, lw = tid & 63;
+  const int cw = lw / NLP, l = lw - cw * NLP;
+  const long long c = (long long)blockIdx.x * S::CPG + (long long)w * CPW + cw;
+  const bool act = l < NL && c < g.ncells;
+  const long long cc = c < g.ncells ? c : g.ncells - 1;
+  const long long idx = cc * NL + (l < NL ? l : 0);
   T lambda = T(0);
   if (MODE == BJ_CGR) lambda = (T)(v.scal[v.slot_rho] / v.scal[0]);
   A acc0 = A(0), acc1 = A(0);
-  const long long ngroups = (g.ncells + CPG - 1) / CPG;
-  int parity = 0;
-  for (long long grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
-    const long long c = grp * CPG + cl;
-    const bool act = lane && c < g.ncells;
-    const long long cc = act ? c : 0;
-    const long long idx = cc * NL + l;
-    T u = T(0);
-    if (act) {
-      u = v.in[idx];
-      if (MODE == BJ_CGR) {
-        u -= lambda * v.q[idx];
-        v.r[idx] = u;
-        acc0 += (A)u * (A)u;
-      }
+  T u = T(0);
+  if (act) {
+    u = v.in[idx];
+    if (MODE == BJ_CGR) {
+      u -= lambda * v.q[idx];
+      v.r[idx] = u;
+      acc0 += (A)u * (A)u;
     }
-    T res = T(0);
-    if constexpr (FDM) {
-      // per-direction entries of this lane's cell
-      const T* E[3];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const int co = bj_coord(g, cc, d);
-        const long long ent = g.per_cell ? cc * 3 + d : (long long)(d * 4 + bj_type(co, g.n[d]));
-        E[d] = data + ent * S::ES;
-      }
-      const int ii[3] = {ix, iy, iz};
-      T vc[3][N1], vr[3][N1];  // column ii[d] and row ii[d] of V_d
-      T lam = T(0);
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-#pragma unroll
-        for (int m = 0; m < N1; ++m) {
-          vc[d][m] = E[d][m * N1 + ii[d]];
-          vr[d][m] = E[d][ii[d] * N1 + m];
-        }
-        lam += E[d][N1 * N1 + ii[d]];
-      }
-      T* b0 = buf[0] + cl * NL;
-      T* b1 = buf[1] + cl * NL;
-      if (lane) b0[l] = u;
-      __syncthreads();
-      T t1 = bj_sweep<N1, 0, T>(b0, vc[0], ix, iy, iz);
-      if (lane) b1[l] = t1;
-      __syncthreads();
-      T t2 = bj_sweep<N1, 1, T>(b1, vc[1], ix, iy, iz);
-      if (lane) b0[l] = t2;
-      __syncthreads();
-      T t3 = bj_sweep<N1, 2, T>(b0, vc[2], ix, iy, iz) / lam;
-      if (lane) b1[l] = t3;
-      __syncthreads();
-      T s1 = bj_sweep<N1, 0, T>(b1, vr[0], ix, iy, iz);
-      if (lane) b0[l] = s1;
-      __syncthreads();
-      T s2 = bj_sweep<N1, 1, T>(b0, vr[1], ix, iy, iz);
-      if (lane) b1[l] = s2;
-      __syncthreads();
-      res = bj_sweep<N1, 2, T>(b1, vr[2], ix, iy, iz);
-    } else {
-      long long ent;
-      if (g.per_cell) ent = cc;
-      else
-        ent = bj_type(bj_coord(g, cc, 0), g.n[0]) + 4 * bj_type(bj_coord(g, cc, 1), g.n[1]) +
-              16 * bj_type(bj_coord(g, cc, 2), g.n[2]);
-      const T* DT = data + ent * NL * NL;
-      T* b = buf[parity] + cl * NL;
-      if (lane) b[l] = u;
-      __syncthreads();
-      T s = T(0);
-#pragma unroll 8
-      for (int j = 0; j < NL; ++j) s += DT[j * NL + l] * b[j];
-      res = s;
-      parity ^= 1;
-    }
-    if (act) {
-      if (MODE == BJ_ADD) v.out[idx] += res;
-      else v.out[idx] = res;
-      if (MODE == BJ_PREC) acc0 += (A)res * (A)u;
-      if (MODE == BJ_CGR) acc1 += (A)res * (A)u;
-    }
+  }
+  long long e;
+  if (g.per_cell) e = cc;
+  else
+    e = bj_type(bj_coord(g, cc, 0), g.n[0]) + 4 * bj_type(bj_coord(g, cc, 1), g.n[1]) +
+        16 * bj_type(bj_coord(g, cc, 2), g.n[2]);
+  const T* DT = data + e * NL * NL + (l < NL ? l : 0);
+  T* b0 = buf[w] + cw * NLP;
+  b0[l] = u;
+  wave_lds_sync();
+  T res = T(0);
+#pragma unroll 9
+  for (int j = 0; j < NL; ++j) res += DT[j * NL] * b0[j];
+  if (act) {
+    if (MODE == BJ_ADD) v.out[idx] += res;
+    else v.out[idx] = res;
+    if (MODE == BJ_PREC) acc0 += (A)res * (A)u;
+    if (MODE == BJ_CGR) acc1 += (A)res * (A)u;
   }
   if (MODE == BJ_PREC) {
     A s[1] = {acc0};
-    block_sum_n<256, A, 1>(s);
+    block_sum_n<S::THREADS, A, 1>(s);
     if (tid == 0) v.partial[blockIdx.x] = s[0];
   } else if (MODE == BJ_CGR) {
     A s[2] = {acc0, acc1};
-    block_sum_n<256, A, 2>(s);
+    block_sum_n<S::THREADS, A, 2>(s);
     if (tid == 0) {
       v.partial[2 * blockIdx.x] = s[0];
       v.partial[2 * blockIdx.x + 1] = s[1];
+    }
+  }
+}
+
+// Fast diagonalisation: lane (ix, iy) of a cell holds the z-line u(ix, iy, 0..N1-1). The
+// x and y sweeps exchange lines through the wave's LDS region; the z sweeps stay in
+// registers. D^{-1} = (Vz (x) Vy (x) Vx) diag(1/(lx + ly + lz)) (Vz^T (x) Vy^T (x) Vx^T).
+template <int N1, typename T, typename A, int MODE, bool PER_CELL>
+__global__ __launch_bounds__(BJFdmShape<N1>::THREADS) void bj_fdm_kernel(const BJVec<T, A> v,
+                                                                        const T* __restrict__ data, const BJGeom g) {
+  using S = BJFdmShape<N1>;
+  constexpr int NL = S::NL, LPC = S::LPC, CPW = S::CPW, ES = S::ES;
+  if (MODE == BJ_CGR && v.state[0]) return;
+  // lines of the wave's cells, value k of lane l at X[w][k][l]: the sweep reads of a wave
+  // touch every LDS bank at most twice (64-bit) or once (32-bit)
+  __shared__ T X[S::WAVES][N1][64];
+  // per-cell entries of each wave's cells, or the whole table of the 12 boundary-type entries
+  __shared__ T ent[PER_CELL ? S::WAVES : 1][PER_CELL ? S::EW : 12 * S::ES];
+  const int tid = threadIdx.x;
+  const int w = tid >> 6, lw = tid & 63;
+  const int cw = lw / LPC, l2 = lw - cw * LPC;
+  const int ix = l2 % N1, iy = l2 / N1;
+  const long long cbase = (long long)blockIdx.x * S::CPG + (long long)w * CPW;
+  const long long c = cbase + cw;
+  const bool act = cw < CPW && c < g.ncells;
+  const long long cc = c < g.ncells ? c : g.ncells - 1;
+  const int cs = cw < CPW ? cw : 0;  // LDS cell slot (idle lanes alias cell 0 and write nothing)
+  const long long base = cc * NL + l2;
+  // stage the coefficients in LDS: per-cell entries (V_d, lam_d) of the wave's cells, value
+  // j = (cell, direction, offset), or the boundary-type table for the workgroup
+  if constexpr (PER_CELL) {
+#pragma unroll
+    for (int j0 = 0; j0 < S::EW; j0 += 64) {
+      const int j = j0 + lw;
+      if (j < S::EW) {
+        long long c2 = cbase + j / (3 * ES);
+        if (c2 >= g.ncells) c2 = g.ncells - 1;
+        ent[w][j] = data[c2 * 3 * ES + j % (3 * ES)];
+      }
+    }
+  } else {
+    for (int j = tid; j < 12 * ES; j += S::THREADS) ent[0][j] = data[j];
+    __syncthreads();
+  }
+  T lambda = T(0);
+  if (MODE == BJ_CGR) lambda = (T)(v.scal[v.slot_rho] / v.scal[0]);
+  A acc0 = A(0);
+  T u[N1];
+#pragma unroll
+  for (int k = 0; k < N1; ++k) {
+    u[k] = T(0);
+    if (act) {
+      const long long idx = base + LPC * k;
+      T x = v.in[idx];
+      if (MODE == BJ_CGR) {
+        x -= lambda * v.q[idx];
+        v.r[idx] = x;
+        acc0 += (A)x * (A)x;
+      }
+      u[k] = x;
+    }
+  }
+  // this lane's coefficients (LDS): the cell's staged entries or its boundary types' entries
+  const T* E[3];
+  if constexpr (PER_CELL) {
+    wave_lds_sync();
+#pragma unroll
+    for (int d = 0; d < 3; ++d) E[d] = ent[w] + (cs * 3 + d) * ES;
+  } else {
+    const unsigned cu = (unsigned)cc, n0 = (unsigned)g.n[0], n1 = (unsigned)g.n[1];
+    const unsigned q = cu / n0, qy = q / n1;
+    const int co[3] = {(int)(cu - q * n0), (int)(q - qy * n1), (int)qy};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) E[d] = ent[0] + (d * 4 + bj_type(co[d], g.n[d])) * ES;
+  }
+  T cx[N1], rx[N1], cy[N1], ry[N1], Vz[N1][N1], lz[N1];
+#pragma unroll
+  for (int m = 0; m < N1; ++m) {
+    cx[m] = E[0][m * N1 + ix];
+    rx[m] = E[0][ix * N1 + m];
+    cy[m] = E[1][m * N1 + iy];
+    ry[m] = E[1][iy * N1 + m];
+    lz[m] = E[2][N1 * N1 + m];
+#pragma unroll
+    for (int k = 0; k < N1; ++k) Vz[m][k] = E[2][m * N1 + k];
+  }
+  const T lxy = E[0][N1 * N1 + ix] + E[1][N1 * N1 + iy];
+  const int xo = cs * LPC;  // the cell's lines: value k of line (ix, iy) at X[w][k][xo + ix + N1 iy]
+  T t[N1];
+  // forward x: t = sum_m Vx[m][ix] u(m, iy, .)
+#pragma unroll
+  for (int k = 0; k < N1; ++k) X[w][k][lw] = u[k];
+  wave_lds_sync();
+#pragma unroll
+  for (int k = 0; k < N1; ++k) t[k] = T(0);
+#pragma unroll
+  for (int m = 0; m < N1; ++m)
+#pragma unroll
+    for (int k = 0; k < N1; ++k) t[k] += cx[m] * X[w][k][xo + m + N1 * iy];
+  wave_lds_sync();
+  // forward y
+#pragma unroll
+  for (int k = 0; k < N1; ++k) X[w][k][lw] = t[k];
+  wave_lds_sync();
+#pragma unroll
+  for (int k = 0; k < N1; ++k) u[k] = T(0);
+#pragma unroll
+  for (int m = 0; m < N1; ++m)
+#pragma unroll
+    for (int k = 0; k < N1; ++k) u[k] += cy[m] * X[w][k][xo + ix + N1 * m];
+  // forward z, scaling, backward z (registers); with w = V^T r, r.(D^{-1} r) is the
+  // sum of w^2 / lam over the transformed values
+  A dot = A(0);
+#pragma unroll
+  for (int k = 0; k < N1; ++k) {
+    T s = T(0);
+#pragma unroll
+    for (int m = 0; m < N1; ++m) s += Vz[m][k] * u[m];
+    t[k] = s * bj_rcp(lxy + lz[k]);
+    dot += (A)s * (A)t[k];
+  }
+#pragma unroll
+  for (int k = 0; k < N1; ++k) {
+    T s = T(0);
+#pragma unroll
+    for (int m = 0; m < N1; ++m) s += Vz[k][m] * t[m];
+    u[k] = s;
+  }
+  wave_lds_sync();
+  // backward y: t = sum_m Vy[iy][m] u(ix, m, .)
+#pragma unroll
+  for (int k = 0; k < N1; ++k) X[w][k][lw] = u[k];
+  wave_lds_sync();
+#pragma unroll
+  for (int k = 0; k < N1; ++k) t[k] = T(0);
+#pragma unroll
+  for (int m = 0; m < N1; ++m)
+#pragma unroll
+    for (int k = 0; k < N1; ++k) t[k] += ry[m] * X[w][k][xo + ix + N1 * m];
+  wave_lds_sync();
+  // backward x
+#pragma unroll
+  for (int k = 0; k < N1; ++k) X[w][k][lw] = t[k];
+  wave_lds_sync();
+#pragma unroll
+  for (int k = 0; k < N1; ++k) u[k] = T(0);
+#pragma unroll
+  for (int m = 0; m < N1; ++m)
+#pragma unroll
+    for (int k = 0; k < N1; ++k) u[k] += rx[m] * X[w][k][xo + m + N1 * iy];
+  if (act) {
+#pragma unroll
+    for (int k = 0; k < N1; ++k) {
+      const long long idx = base + LPC * k;
+      if (MODE == BJ_ADD) v.out[idx] += u[k];
+      else v.out[idx] = u[k];
+    }
+  }
+  if (MODE == BJ_PREC || MODE == BJ_CGR) {
+    // out.in (for BJ_CGR out.r with the updated r); idle lanes hold zeros
+    const A d = dot;
+    if (MODE == BJ_PREC) {
+      A s[1] = {d};
+      block_sum_n<S::THREADS, A, 1>(s);
+      if (tid == 0) v.partial[blockIdx.x] = s[0];
+    } else {
+      A s[2] = {acc0, d};
+      block_sum_n<S::THREADS, A, 2>(s);
+      if (tid == 0) {
+        v.partial[2 * blockIdx.x] = s[0];
+        v.partial[2 * blockIdx.x + 1] = s[1];
+      }
     }
   }
 }

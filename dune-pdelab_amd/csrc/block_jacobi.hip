@@ -39,6 +39,14 @@ BJMass mass_data(const H* h) {
   return m;
 }
 
+// workgroups of one application (CPG cells each)
+template <template <int> class SH>
+long long groups_of(const H* h) {
+  const int cpg = h->n1 == 2 ? SH<2>::CPG : (h->n1 == 3 ? SH<3>::CPG : SH<4>::CPG);
+  return (h->ncells + cpg - 1) / cpg;
+}
+long long bj_groups(const H* h, bool fdm) { return fdm ? groups_of<BJFdmShape>(h) : groups_of<BJShape>(h); }
+
 BJGeom geom(const H* h, bool per_cell) {
   BJGeom g;
   g.vark = h->vark ? 1 : 0;
@@ -94,13 +102,16 @@ int setup_t(H* h, const BlockJacobiData& b, const BJGeom& g, double* o64, float*
 template <int N1, typename T, typename A, int MODE>
 int launch_n(H* h, const BlockJacobiData& b, const T* data, const BJVec<T, A>& v) {
   const BJGeom g = geom(h, b.per_cell);
-  const long long ngroups = (h->ncells + BJShape<N1>::CPG - 1) / BJShape<N1>::CPG;
-  // the reducing modes write one partial per workgroup for kRedBlocks workgroups
-  const int nb = MODE == BJ_ADD ? (int)std::min<long long>(ngroups, 8192) : kRedBlocks;
-  if (b.fdm)
-    hipLaunchKernelGGL((bj_apply_kernel<N1, T, A, MODE, true>), dim3(nb), dim3(256), 0, h->stream, v, data, g);
+  const long long nb = bj_groups(h, b.fdm);
+  if (b.fdm && b.per_cell)
+    hipLaunchKernelGGL((bj_fdm_kernel<N1, T, A, MODE, true>), dim3(nb), dim3(BJFdmShape<N1>::THREADS), 0, h->stream, v,
+                       data, g);
+  else if (b.fdm)
+    hipLaunchKernelGGL((bj_fdm_kernel<N1, T, A, MODE, false>), dim3(nb), dim3(BJFdmShape<N1>::THREADS), 0, h->stream, v,
+                       data, g);
   else
-    hipLaunchKernelGGL((bj_apply_kernel<N1, T, A, MODE, false>), dim3(nb), dim3(256), 0, h->stream, v, data, g);
+    hipLaunchKernelGGL((bj_dense_kernel<N1, T, A, MODE>), dim3(nb), dim3(BJShape<N1>::THREADS), 0, h->stream, v, data,
+                       g);
   HIPCHK(h, hipGetLastError());
   return PDELAB_OK;
 }
@@ -133,9 +144,12 @@ void bj_free(H* h) {
     if (b.d32) (void)hipFree(b.d32);
     b = BlockJacobiData();
   }
+  if (h->d_bjpart) (void)hipFree(h->d_bjpart);
+  h->d_bjpart = nullptr;
 }
 
 int bj_prepare(H* h, bool trans, int precision) {
+  if (h->ncells >= (1LL << 31)) return fail(h, PDELAB_ERR_UNSUPPORTED, "block Jacobi: more than 2^31 - 1 cells");
   BlockJacobiData& b = h->bj[trans ? 1 : 0];
   const bool fdm = !h->conv && h->cfg.method == PDELAB_SIPG;
   const bool per_cell = h->vark;
@@ -150,6 +164,10 @@ int bj_prepare(H* h, bool trans, int precision) {
     b.per_cell = per_cell;
     b.entries = entries;
     b.esize = esize;
+  }
+  if (!h->d_bjpart) {
+    const size_t ng = (size_t)std::max(bj_groups(h, true), bj_groups(h, false));
+    HIPCHK(h, hipMalloc(&h->d_bjpart, sizeof(double) * 2 * ng));
   }
   const bool need64 = precision == 64 && !b.valid64;
   const bool need32 = precision == 32 && !b.valid32;
@@ -167,19 +185,22 @@ int bj_prepare(H* h, bool trans, int precision) {
 }
 
 template <typename T, typename A>
-int bj_launch(H* h, bool trans, int mode, const BJVec<T, A>& v) {
+int bj_launch(H* h, bool trans, int mode, BJVec<T, A> v, int* nparts) {
   CHK(bj_prepare(h, trans, sizeof(T) == 8 ? 64 : 32));
   const BlockJacobiData& b = h->bj[trans ? 1 : 0];
   const T* data = bj_data<T>(b);
+  v.partial = static_cast<A*>(h->d_bjpart);
+  if (nparts) *nparts = (int)bj_groups(h, b.fdm);
   if (mode == BJ_PREC) return launch_m<T, A, BJ_PREC>(h, b, data, v);
   if (mode == BJ_ADD) return launch_m<T, A, BJ_ADD>(h, b, data, v);
   return launch_m<T, A, BJ_CGR>(h, b, data, v);
 }
 
-template int bj_launch<double, double>(H*, bool, int, const BJVec<double, double>&);
-template int bj_launch<float, float>(H*, bool, int, const BJVec<float, float>&);
+template int bj_launch<double, double>(H*, bool, int, BJVec<double, double>, int*);
+template int bj_launch<float, float>(H*, bool, int, BJVec<float, float>, int*);
 
 int bj_block_diagonal(H* h, bool trans, double* blocks) {
+  if (h->ncells >= (1LL << 31)) return fail(h, PDELAB_ERR_UNSUPPORTED, "block diagonal: more than 2^31 - 1 cells");
   const BJGeom g = geom(h, true);
   BlockJacobiData b;
   b.entries = h->ncells;

@@ -73,6 +73,7 @@ struct pdelab_gpu_gridoperator {
   // apply kernel on aligned-row grids: 0 the aligned-row kernel, 1 the general kernel
   int apply_variant = 0;
   BlockJacobiData bj[2];  // [0] J, [1] J^T
+  void* d_bjpart = nullptr;  // per-workgroup partials of the block Jacobi application
 };
 
 using H = pdelab_gpu_gridoperator;
@@ -117,12 +118,13 @@ int launch_apply_n(H* h, const T* z, T* y, const T* fc, const DGArgs<T>& a, bool
 
 // Block Jacobi (block_jacobi.hip): data set-up for the given precision (64 or 32),
 // invalidation when kappa changes, the launches (BJ_PREC / BJ_ADD / BJ_CGR, see
-// block_jacobi.hh) and the diagonal blocks themselves (ncells * nloc^2, row-major).
+// block_jacobi.hh; the partials go to d_bjpart, their count to *nparts) and the
+// diagonal blocks themselves (ncells * nloc^2, row-major).
 int bj_prepare(H* h, bool trans, int precision);
 void bj_invalidate(H* h);
 void bj_free(H* h);
 template <typename T, typename A>
-int bj_launch(H* h, bool trans, int mode, const BJVec<T, A>& v);
+int bj_launch(H* h, bool trans, int mode, BJVec<T, A> v, int* nparts);
 int bj_block_diagonal(H* h, bool trans, double* blocks);
 
 #endif

@@ -538,16 +538,23 @@ static int make_prec(H* h, int precond, bool trans, Prec<T>& P) {
   return PDELAB_OK;
 }
 
-// out = W^{-1} in, with the partials of out.in in partial_buf (kRedBlocks values)
+// out = W^{-1} in; out.in into scal[slot] unless slot < 0
 template <typename T, typename A>
-static int prec_dot(H* h, const Prec<T>& P, const T* in, T* out) {
+static int prec_dot(H* h, const Prec<T>& P, const T* in, T* out, int slot) {
   if (P.block()) {
-    const BJVec<T, A> v{in, out, nullptr, nullptr, nullptr, 0, partial_buf<A>(h), nullptr};
-    return bj_launch<T, A>(h, P.trans, BJ_PREC, v);
+    const BJVec<T, A> v{in, out, nullptr, nullptr, nullptr, 0, nullptr, nullptr};
+    int nb = 0;
+    CHK((bj_launch<T, A>(h, P.trans, BJ_PREC, v, &nb)));
+    if (slot >= 0)
+      hipLaunchKernelGGL((final_reduce_kernel<A, 1>), dim3(1), dim3(kRedThreads), 0, h->stream,
+                         (const A*)h->d_bjpart, nb, scal_buf<A>(h) + slot, (const int*)nullptr);
+    HIPCHK(h, hipGetLastError());
+    return PDELAB_OK;
   }
   hipLaunchKernelGGL((prec_dot_kernel<T, A>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, h->N, P.dinv, in, out,
                      partial_buf<A>(h));
   HIPCHK(h, hipGetLastError());
+  if (slot >= 0) CHK((finish_reduce<A, 1>(h, slot)));
   return PDELAB_OK;
 }
 
@@ -599,16 +606,19 @@ static int cg_loop(H* h, T* x, T* b, T* p, T* q, T* z, const Prec<T>& P, bool tr
     CHK(apply_mode<T>(h, p, q, trans, 2, dotp, h->d_state));
     hipLaunchKernelGGL((final_reduce_kernel<A, 1>), dim3(1), dim3(kRedThreads), 0, h->stream, (const A*)dotp,
                        (int)nbapply, scal + 0, (const int*)h->d_state);
+    const A* parts = partial_buf<A>(h);
+    int nparts = kRedBlocks;
     if (P.block()) {
       // r -= lambda q and z = W^{-1} r in one pass over the cells (z is kept for the p update)
-      const BJVec<T, A> v{b, z, b, q, scal, rslot, partial_buf<A>(h), h->d_state};
-      CHK((bj_launch<T, A>(h, P.trans, BJ_CGR, v)));
+      const BJVec<T, A> v{b, z, b, q, scal, rslot, nullptr, h->d_state};
+      CHK((bj_launch<T, A>(h, P.trans, BJ_CGR, v, &nparts)));
+      parts = static_cast<const A*>(h->d_bjpart);
     } else {
       hipLaunchKernelGGL((cg_rupdate_kernel<T, A>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, n, b,
                          (const T*)q, P.dinv, (const A*)scal, rslot, partial_buf<A>(h), (const int*)h->d_state);
     }
-    hipLaunchKernelGGL((cg_check_kernel<A>), dim3(1), dim3(kRedThreads), 0, h->stream, (const A*)partial_buf<A>(h),
-                       kRedBlocks, scal + oslot, h->d_state, h->d_dstate, i);
+    hipLaunchKernelGGL((cg_check_kernel<A>), dim3(1), dim3(kRedThreads), 0, h->stream, parts, nparts, scal + oslot,
+                       h->d_state, h->d_dstate, i);
     hipLaunchKernelGGL((cg_xpupdate_kernel<T, A>), dim3(grid_for(n)), dim3(256), 0, h->stream, n, x, p,
                        (const T*)b, P.dinv, (const T*)(P.block() ? z : nullptr), (const A*)scal, rslot, oslot + 1,
                        (const int*)h->d_state, i);
@@ -661,8 +671,7 @@ static int cg_solve(H* h, T* x, T* b, double reduction, int maxit, int precond, 
   const double def0 = std::sqrt((double)sc[1]);
   if (!it.step(0, def0)) {
     // p = W^{-1} b; rho = p.b  -> slot 7
-    CHK((prec_dot<T, A>(h, P, b, p)));
-    CHK(finish_reduce<A, 1>(h, 7));
+    CHK((prec_dot<T, A>(h, P, b, p, 7)));
     CgLoopOut lo;
     CHK((cg_loop<T, A>(h, x, b, p, q, z, P, trans, maxit, def0, reduction, lo)));
     it.def = lo.def;
@@ -715,7 +724,7 @@ static int bicgstab_solve(H* h, T* x, T* r, double reduction, int maxit, int pre
                            (T)beta, (T)omega, 0);
       }
       // y = W^{-1} p ; v = J y ; h = rt.v
-      CHK((prec_dot<T, A>(h, P, p, y)));
+      CHK((prec_dot<T, A>(h, P, p, y, -1)));
       CHK(apply_t<T>(h, y, v, trans, false));
       CHK(dot_dev<T, A>(h, rt, v, n, 0));
       CHK(read_scalars<A>(h, sc, 1));
@@ -730,7 +739,7 @@ static int bicgstab_solve(H* h, T* x, T* r, double reduction, int maxit, int pre
       if (iter.step(itc, norm)) break;
       itc += .5;
       // y = W^{-1} r ; t = J y ; omega = t.r / t.t
-      CHK((prec_dot<T, A>(h, P, r, y)));
+      CHK((prec_dot<T, A>(h, P, r, y, -1)));
       CHK(apply_t<T>(h, y, t, trans, false));
       hipLaunchKernelGGL((dot2_kernel<T, A>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, n, (const T*)t,
                          (const T*)r, partial_buf<A>(h));
@@ -864,8 +873,7 @@ static int cg_mixed_reliable(H* h, int precond, double inner_reduction, int inne
     CHK(to_f32());
     hipLaunchKernelGGL(fill_kernel<float>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, 0.0f, x32);
     // p = W^{-1} r; rho -> slot 7
-    CHK((prec_dot<float, float>(h, P, r32, p32)));
-    CHK((finish_reduce<float, 1>(h, 7)));
+    CHK((prec_dot<float, float>(h, P, r32, p32, 7)));
     for (;;) {
       const double thr = std::max(inner_reduction * def, 0.5 * outer_reduction * def0);
       CgLoopOut lo;
@@ -892,8 +900,7 @@ static int cg_mixed_reliable(H* h, int precond, double inner_reduction, int inne
       CHK((read_scalars<float>(h, sc, 8)));
       const double rho_old = sc[lo.rho_slot];
       CHK(to_f32());
-      CHK((prec_dot<float, float>(h, P, r32, q32)));
-      CHK((finish_reduce<float, 1>(h, 7)));
+      CHK((prec_dot<float, float>(h, P, r32, q32, 7)));
       CHK((read_scalars<float>(h, sc, 8)));
       const float beta = rho_old != 0 ? (float)(sc[7] / rho_old) : 0.0f;
       // p = q + beta p with q = W^{-1} r
@@ -1164,14 +1171,14 @@ int pdelab_gpu_block_jacobi_apply(pdelab_gpu_gridoperator* h, const double* z, d
   if (!h || !z || !y) return PDELAB_ERR_INVALID_ARG;
   CHK(check_outflow(h));
   const BJVec<double, double> v{z, y, nullptr, nullptr, nullptr, 0, nullptr, nullptr};
-  return bj_launch<double, double>(h, false, BJ_ADD, v);
+  return bj_launch<double, double>(h, false, BJ_ADD, v, nullptr);
 }
 
 int pdelab_gpu_block_jacobi_apply_f32(pdelab_gpu_gridoperator* h, const float* z, float* y) {
   if (!h || !z || !y) return PDELAB_ERR_INVALID_ARG;
   CHK(check_outflow(h));
   const BJVec<float, float> v{z, y, nullptr, nullptr, nullptr, 0, nullptr, nullptr};
-  return bj_launch<float, float>(h, false, BJ_ADD, v);
+  return bj_launch<float, float>(h, false, BJ_ADD, v, nullptr);
 }
 
 int pdelab_gpu_mass_apply(pdelab_gpu_gridoperator* h, const double* z, double* y) {
