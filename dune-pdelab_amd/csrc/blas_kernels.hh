@@ -53,19 +53,40 @@ __global__ __launch_bounds__(kRedThreads) void dot_partial_kernel(const T* __res
   if (threadIdx.x == 0) partial[blockIdx.x] = v[0];
 }
 
-// out[slot + j] = sum_b partial[b*NR + j]   (single block of kRedThreads threads)
+// Single-workgroup sums of per-workgroup partials (up to tens of thousands of them):
+// kFinalThreads threads, each with four independent accumulators so that four loads
+// per thread are in flight, then the workgroup tree. The order is fixed.
+constexpr int kFinalThreads = 1024;
+
 template <typename A, int NR>
-__global__ __launch_bounds__(kRedThreads) void final_reduce_kernel(const A* __restrict__ partial, int nb,
-                                                                   A* __restrict__ out,
-                                                                   const int* __restrict__ stop) {
+__device__ __forceinline__ void strided_partial_sum(const A* __restrict__ partial, int nb, A (&v)[NR]) {
+  A a[4][NR];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int j = 0; j < NR; ++j) a[u][j] = A(0);
+  int b = threadIdx.x;
+  for (; b + 3 * kFinalThreads < nb; b += 4 * kFinalThreads)
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int j = 0; j < NR; ++j) a[u][j] += partial[(b + u * kFinalThreads) * NR + j];
+  for (; b < nb; b += kFinalThreads)
+#pragma unroll
+    for (int j = 0; j < NR; ++j) a[0][j] += partial[b * NR + j];
+#pragma unroll
+  for (int j = 0; j < NR; ++j) v[j] = (a[0][j] + a[1][j]) + (a[2][j] + a[3][j]);
+  block_sum_n<kFinalThreads, A, NR>(v);
+}
+
+// out[j] = sum_b partial[b*NR + j]   (one workgroup of kFinalThreads threads)
+template <typename A, int NR>
+__global__ __launch_bounds__(kFinalThreads) void final_reduce_kernel(const A* __restrict__ partial, int nb,
+                                                                     A* __restrict__ out,
+                                                                     const int* __restrict__ stop) {
   if (stop && *stop) return;  // a device-side Krylov loop has already stopped
   A v[NR];
-#pragma unroll
-  for (int j = 0; j < NR; ++j) v[j] = A(0);
-  for (int b = threadIdx.x; b < nb; b += kRedThreads)
-#pragma unroll
-    for (int j = 0; j < NR; ++j) v[j] += partial[b * NR + j];
-  block_sum<A, NR>(v);
+  strided_partial_sum<A, NR>(partial, nb, v);
   if (threadIdx.x == 0)
 #pragma unroll
     for (int j = 0; j < NR; ++j) out[j] = v[j];
@@ -189,16 +210,12 @@ __global__ __launch_bounds__(kRedThreads) void cg_rupdate_kernel(long long n, T*
 // a running loop and raises the stop flag on convergence (def < reduction*def0 or
 // def < 1e-30) or on a non-finite defect.
 template <typename A>
-__global__ __launch_bounds__(kRedThreads) void cg_check_kernel(const A* __restrict__ partial, int nb,
+__global__ __launch_bounds__(kFinalThreads) void cg_check_kernel(const A* __restrict__ partial, int nb,
                                                                A* __restrict__ out, int* __restrict__ state,
                                                                double* __restrict__ dstate, int it) {
   if (state[0]) return;
-  A v[2] = {A(0), A(0)};
-  for (int b = threadIdx.x; b < nb; b += kRedThreads) {
-    v[0] += partial[2 * b];
-    v[1] += partial[2 * b + 1];
-  }
-  block_sum_n<kRedThreads, A, 2>(v);
+  A v[2];
+  strided_partial_sum<A, 2>(partial, nb, v);
   if (threadIdx.x == 0) {
     out[0] = v[0];
     out[1] = v[1];

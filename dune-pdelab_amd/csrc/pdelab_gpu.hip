@@ -449,7 +449,7 @@ float* hscal_buf<float>(H* h) { return h->h_scal32; }
 
 template <typename A, int NR>
 static int finish_reduce(H* h, int slot) {
-  hipLaunchKernelGGL((final_reduce_kernel<A, NR>), dim3(1), dim3(kRedThreads), 0, h->stream, partial_buf<A>(h),
+  hipLaunchKernelGGL((final_reduce_kernel<A, NR>), dim3(1), dim3(kFinalThreads), 0, h->stream, partial_buf<A>(h),
                      kRedBlocks, scal_buf<A>(h) + slot, (const int*)nullptr);
   HIPCHK(h, hipGetLastError());
   return PDELAB_OK;
@@ -546,7 +546,7 @@ static int prec_dot(H* h, const Prec<T>& P, const T* in, T* out, int slot) {
     int nb = 0;
     CHK((bj_launch<T, A>(h, P.trans, BJ_PREC, v, &nb)));
     if (slot >= 0)
-      hipLaunchKernelGGL((final_reduce_kernel<A, 1>), dim3(1), dim3(kRedThreads), 0, h->stream,
+      hipLaunchKernelGGL((final_reduce_kernel<A, 1>), dim3(1), dim3(kFinalThreads), 0, h->stream,
                          (const A*)h->d_bjpart, nb, scal_buf<A>(h) + slot, (const int*)nullptr);
     HIPCHK(h, hipGetLastError());
     return PDELAB_OK;
@@ -604,7 +604,7 @@ static int cg_loop(H* h, T* x, T* b, T* p, T* q, T* z, const Prec<T>& P, bool tr
     // q = J p with the p.q partials -> scal[0]; r -= lambda q with the (r.r, z.r)
     // partials -> scal[oslot..] and the defect check; x += lambda p, p = z + beta p
     CHK(apply_mode<T>(h, p, q, trans, 2, dotp, h->d_state));
-    hipLaunchKernelGGL((final_reduce_kernel<A, 1>), dim3(1), dim3(kRedThreads), 0, h->stream, (const A*)dotp,
+    hipLaunchKernelGGL((final_reduce_kernel<A, 1>), dim3(1), dim3(kFinalThreads), 0, h->stream, (const A*)dotp,
                        (int)nbapply, scal + 0, (const int*)h->d_state);
     const A* parts = partial_buf<A>(h);
     int nparts = kRedBlocks;
@@ -617,7 +617,7 @@ static int cg_loop(H* h, T* x, T* b, T* p, T* q, T* z, const Prec<T>& P, bool tr
       hipLaunchKernelGGL((cg_rupdate_kernel<T, A>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, n, b,
                          (const T*)q, P.dinv, (const A*)scal, rslot, partial_buf<A>(h), (const int*)h->d_state);
     }
-    hipLaunchKernelGGL((cg_check_kernel<A>), dim3(1), dim3(kRedThreads), 0, h->stream, parts, nparts, scal + oslot,
+    hipLaunchKernelGGL((cg_check_kernel<A>), dim3(1), dim3(kFinalThreads), 0, h->stream, parts, nparts, scal + oslot,
                        h->d_state, h->d_dstate, i);
     hipLaunchKernelGGL((cg_xpupdate_kernel<T, A>), dim3(grid_for(n)), dim3(256), 0, h->stream, n, x, p,
                        (const T*)b, P.dinv, (const T*)(P.block() ? z : nullptr), (const A*)scal, rslot, oslot + 1,
