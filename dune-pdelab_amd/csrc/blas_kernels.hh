@@ -122,6 +122,20 @@ __global__ __launch_bounds__(256) void sub_f32_kernel(long long n, const float* 
     x[i] -= (double)z[i];
 }
 
+// y32 = (float)(s x64): the refinement residual handed to the FP32 solve at unit norm
+__global__ __launch_bounds__(256) void scale_to_f32_kernel(long long n, double s, const double* __restrict__ x,
+                                                           float* __restrict__ y) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    y[i] = (float)(s * x[i]);
+}
+
+// x -= s (double) z32   (refinement update of a correction computed at scale 1/s)
+__global__ __launch_bounds__(256) void sub_scaled_f32_kernel(long long n, double s, const float* __restrict__ z,
+                                                             double* __restrict__ x) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    x[i] -= s * (double)z[i];
+}
+
 // kappa = exp(log kappa) in FP64 and FP32
 __global__ __launch_bounds__(256) void exp_kernel(long long n, const double* __restrict__ logk, double* __restrict__ k64,
                                                   float* __restrict__ k32) {

@@ -1279,6 +1279,7 @@ int pdelab_gpu_solve_mixed(pdelab_gpu_gridoperator* h, int method, int precond, 
   auto t0 = std::chrono::steady_clock::now();
   CHK(compute_R0(h));
   double def0 = 0, def = 0;
+  int breakdowns = 0;
   for (int outer = 0;; ++outer) {
     // FP64 outer residual r = R(x) = J x + R(0)
     CHK(apply64(h, x, r64, false, false));
@@ -1293,7 +1294,9 @@ int pdelab_gpu_solve_mixed(pdelab_gpu_gridoperator* h, int method, int precond, 
     }
     if (def <= outer_reduction * def0 || def < 1e-30) { res->converged = 1; break; }
     if (outer >= outer_maxit) break;
-    hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(n)), dim3(256), 0, h->stream, n,
+    // the FP32 solve sees the residual at unit norm (its dot products then stay far from
+    // the FP32 underflow threshold however small the outer residual has become)
+    hipLaunchKernelGGL(scale_to_f32_kernel, dim3(grid_for(n)), dim3(256), 0, h->stream, n, 1.0 / def,
                        (const double*)r64, r32);
     HIPCHK(h, hipMemsetAsync(z32, 0, sizeof(float) * n, h->stream));
     // the inner solve reduces by inner_reduction, or by a quarter of the reduction
@@ -1301,9 +1304,17 @@ int pdelab_gpu_solve_mixed(pdelab_gpu_gridoperator* h, int method, int precond, 
     const double need = def0 > 0 ? outer_reduction * def0 / def : 1.0;
     const double ired = std::min(0.5, std::max(inner_reduction, 0.25 * need));
     pdelab_linear_solver_result ires;
-    CHK((krylov<float, float>(h, method, precond, ired, inner_maxit, false, z32, r32, 2, &ires)));
+    const int st = krylov<float, float>(h, method, precond, ired, inner_maxit, false, z32, r32, 2, &ires);
     res->inner_iterations += (int)ires.iterations;
-    hipLaunchKernelGGL(sub_f32_kernel, dim3(grid_for(n)), dim3(256), 0, h->stream, n, (const float*)z32, x);
+    if (st == PDELAB_ERR_SOLVER_ABORT) {
+      // an FP32 breakdown ends this inner solve: its iterate so far is still a correction
+      // (a non-finite one makes the next outer residual non-finite, which aborts)
+      if (++breakdowns > 3) return fail(h, PDELAB_ERR_SOLVER_ABORT, "refinement: repeated FP32 BiCGSTAB breakdown");
+      h->err.clear();
+    } else {
+      CHK(st);
+    }
+    hipLaunchKernelGGL(sub_scaled_f32_kernel, dim3(grid_for(n)), dim3(256), 0, h->stream, n, def, (const float*)z32, x);
     HIPCHK(h, hipGetLastError());
   }
   HIPCHK(h, hipStreamSynchronize(h->stream));
