@@ -77,6 +77,7 @@ def lib():
         getattr(L, fn).argtypes = [_vp, _vp, _vp]
     L.pdelab_gpu_jacobian_apply_nonlinear.argtypes = [_vp, _vp, _vp, _vp]
     L.pdelab_gpu_point_diagonal.argtypes = [_vp, _vp]
+    L.pdelab_gpu_operator_apply.argtypes = [_vp, _vp, _vp]
     L.pdelab_gpu_block_diagonal.argtypes = [_vp, ctypes.c_int, _vp]
     L.pdelab_gpu_block_jacobi_apply.argtypes = [_vp, _vp, _vp]
     L.pdelab_gpu_block_jacobi_apply_f32.argtypes = [_vp, _vp, _vp]
@@ -234,6 +235,10 @@ class GridOperator:
             self._check(lib().pdelab_gpu_jacobian_apply_f32(self.h, z.p, y.p))
         else:
             self._check(lib().pdelab_gpu_jacobian_apply(self.h, z.p, y.p))
+
+    def operator_apply(self, z, y):
+        """y = J z (OnTheFlyOperator::apply: y = 0, then jacobian_apply)."""
+        self._check(lib().pdelab_gpu_operator_apply(self.h, z.p, y.p))
 
     def jacobian_apply_nonlinear(self, x, z, y):
         self._check(lib().pdelab_gpu_jacobian_apply_nonlinear(self.h, x.p, z.p, y.p))

@@ -1136,6 +1136,11 @@ int pdelab_gpu_jacobian_apply(pdelab_gpu_gridoperator* h, const double* z, doubl
   return apply64(h, z, y, false, true);
 }
 
+int pdelab_gpu_operator_apply(pdelab_gpu_gridoperator* h, const double* z, double* y) {
+  if (!h || !z || !y) return PDELAB_ERR_INVALID_ARG;
+  return apply64(h, z, y, false, false);
+}
+
 int pdelab_gpu_jacobian_apply_f32(pdelab_gpu_gridoperator* h, const float* z, float* y) {
   if (!h || !z || !y) return PDELAB_ERR_INVALID_ARG;
   return apply32(h, z, y, false, true);

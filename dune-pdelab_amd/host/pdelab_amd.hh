@@ -446,9 +446,11 @@ class OnTheFlyOperator {
   explicit OnTheFlyOperator(const GO& go) : go_(go) {}
   void setLinearizationPoint(const X& u) { u_ = &u; }
   void apply(const X& x, Y& y) const {
-    y = 0.0;
-    if (isLinear) go_.jacobian_apply(x, y);
-    else {
+    if (isLinear) {
+      // y = 0; jacobian_apply(x, y) in one pass
+      check(pdelab_gpu_operator_apply(go_.handle(), x.data(), y.data()), go_.handle());
+    } else {
+      y = 0.0;
       if (!u_) throw InvalidStateException("You seem to apply a nonlinear operator without setting the linearization point first!");
       go_.jacobian_apply(*u_, x, y);
     }
@@ -471,10 +473,37 @@ class OnTheFlyOperator {
 
 enum class Krylov { CG = PDELAB_SOLVER_CG, BiCGSTAB = PDELAB_SOLVER_BICGSTAB };
 
-// Matrix-free Krylov backend with point-Jacobi preconditioning on the device:
-// the role of ISTLBackend_SEQ_MatrixFree_Base<GO, PrecGO, Solver> with a point-diagonal
-// preconditioner grid operator, and of ISTLBackend_SEQ_{CG,BCGS}_Jac on the same matrix.
-template <class GO, Krylov method>
+// Preconditioner of the device Krylov backends: point Jacobi (Blocking::none) or per-cell
+// block Jacobi with exact block inverses (Blocking::fixed SeqJac; the matrix-free
+// AssembledBlockJacobiPreconditionerLocalOperator).
+enum class Preconditioner { Jacobi = PDELAB_PREC_JACOBI, BlockJacobi = PDELAB_PREC_BLOCK_JACOBI };
+
+// GridOperatorPreconditioner over a grid operator of
+// AssembledBlockJacobiPreconditionerLocalOperator<BlockDiagonalLocalOperatorWrapper<LOP>>
+// (R:dune/pdelab/backend/istl/matrixfree/gridoperatorpreconditioner.hh:18-91): apply(v, d)
+// adds D^{-1} d, D the block diagonal of the grid operator's Jacobian. The blocks are
+// inverted on first use (pre() is a no-op) and again after the next kappa change.
+template <class GO>
+class AssembledBlockJacobiPreconditioner {
+  using V = typename GO::Traits::Domain;
+  using W = typename GO::Traits::Range;
+
+ public:
+  static constexpr bool isLinear = true;
+  explicit AssembledBlockJacobiPreconditioner(const GO& go) : go_(go) {}
+  void pre(V&, W&) {}
+  void apply(V& v, const W& d) { check(pdelab_gpu_block_jacobi_apply(go_.handle(), d.data(), v.data()), go_.handle()); }
+  void post(V&) {}
+
+ private:
+  const GO& go_;
+};
+
+// Matrix-free Krylov backend with device preconditioning: the role of
+// ISTLBackend_SEQ_MatrixFree_Base<GO, PrecGO, Solver> with a point-diagonal or
+// assembled-block-Jacobi preconditioner grid operator, and of ISTLBackend_SEQ_{CG,BCGS}_Jac
+// on the same matrix (Blocking::none / Blocking::fixed).
+template <class GO, Krylov method, Preconditioner prec = Preconditioner::Jacobi>
 class ISTLBackend_SEQ_MatrixFree_Jac : public SequentialNorm, public LinearResultStorage {
   using V = typename GO::Traits::Domain;
   using W = typename GO::Traits::Range;
@@ -485,8 +514,7 @@ class ISTLBackend_SEQ_MatrixFree_Jac : public SequentialNorm, public LinearResul
   // solves J z = r to the given relative reduction; r becomes the defect
   void apply(V& z, W& r, double reduction) {
     pdelab_linear_solver_result st;
-    check(pdelab_gpu_solve(go_.handle(), (int)method, PDELAB_PREC_JACOBI, reduction, (int)maxiter_, z.data(),
-                           r.data(), &st),
+    check(pdelab_gpu_solve(go_.handle(), (int)method, (int)prec, reduction, (int)maxiter_, z.data(), r.data(), &st),
           go_.handle());
     store(st);
   }
@@ -503,12 +531,17 @@ template <class GO>
 using ISTLBackend_SEQ_MatrixFree_CG_Jac = ISTLBackend_SEQ_MatrixFree_Jac<GO, Krylov::CG>;
 template <class GO>
 using ISTLBackend_SEQ_MatrixFree_BCGS_Jac = ISTLBackend_SEQ_MatrixFree_Jac<GO, Krylov::BiCGSTAB>;
+template <class GO>
+using ISTLBackend_SEQ_MatrixFree_CG_BlockJac = ISTLBackend_SEQ_MatrixFree_Jac<GO, Krylov::CG, Preconditioner::BlockJacobi>;
+template <class GO>
+using ISTLBackend_SEQ_MatrixFree_BCGS_BlockJac =
+    ISTLBackend_SEQ_MatrixFree_Jac<GO, Krylov::BiCGSTAB, Preconditioner::BlockJacobi>;
 
 // Mixed precision backend: FP32 inner Krylov inside FP64 refinement (CG: reliable
 // updates, inner_reduction = FP32 residual reduction between FP64 residual replacements;
 // BiCGSTAB: restarted inner solves to inner_reduction); used in place of the linear
 // problem solver (the refinement owns the residual evaluations).
-template <class GO, Krylov method>
+template <class GO, Krylov method, Preconditioner prec = Preconditioner::Jacobi>
 class MixedPrecisionSolver {
  public:
   MixedPrecisionSolver(const GO& go, double inner_reduction = method == Krylov::CG ? 0.1 : 1e-3,
@@ -516,7 +549,7 @@ class MixedPrecisionSolver {
       : go_(go), inner_(inner_reduction), inner_maxit_(inner_maxit) {}
   pdelab_mixed_result apply(typename GO::Traits::Domain& x, double reduction, int outer_maxit = 50) {
     pdelab_mixed_result r;
-    check(pdelab_gpu_solve_mixed(go_.handle(), (int)method, PDELAB_PREC_JACOBI, inner_, (int)inner_maxit_, reduction,
+    check(pdelab_gpu_solve_mixed(go_.handle(), (int)method, (int)prec, inner_, (int)inner_maxit_, reduction,
                                  outer_maxit, x.data(), &r),
           go_.handle());
     return r;

@@ -9,6 +9,9 @@
  *                                    R:dune/pdelab/gridoperator/gridoperator.hh:176-181
  *   pdelab_gpu_jacobian_apply        GridOperator::jacobian_apply(z, y)
  *                                    R:dune/pdelab/gridoperator/gridoperator.hh:192-197
+ *   pdelab_gpu_operator_apply        OnTheFlyOperator::apply(x, y): y = 0, then
+ *                                    jacobian_apply, fused into one pass
+ *                                    R:dune/pdelab/backend/istl/seqistlsolverbackend.hh:66-76
  *   pdelab_gpu_jacobian_apply_nonlinear
  *                                    GridOperator::jacobian_apply(x, z, y) (throws for a
  *                                    linear operator) gridoperator.hh:200-205
@@ -211,6 +214,8 @@ int pdelab_gpu_synchronize(pdelab_gpu_gridoperator* h);
 int pdelab_gpu_residual(pdelab_gpu_gridoperator* h, const double* x, double* r);
 /* y += J z (linear operator: the Jacobian apply is the residual without the load terms) */
 int pdelab_gpu_jacobian_apply(pdelab_gpu_gridoperator* h, const double* z, double* y);
+/* y = J z (overwrites y): the operator apply of the Krylov solvers, OnTheFlyOperator::apply. */
+int pdelab_gpu_operator_apply(pdelab_gpu_gridoperator* h, const double* z, double* y);
 /* FP32 operator apply, y += J z in single precision (arithmetic in FP32). */
 int pdelab_gpu_jacobian_apply_f32(pdelab_gpu_gridoperator* h, const float* z, float* y);
 /* Nonlinear form jacobian_apply(x, z, y): always PDELAB_ERR_LINEARITY_MISMATCH here,

@@ -73,6 +73,10 @@ def test_apply_diag_transpose_mass(pkg, of, name, kw, seed):
     go.jacobian_apply(zd, yd)
     y_ref = o.jacobian_apply(z, y0.copy())
     assert rel(yd.download(), y_ref) <= APPLY_RTOL
+    # OnTheFlyOperator::apply: y = J z, whatever y held
+    yd.upload(y0)
+    go.operator_apply(zd, yd)
+    assert rel(yd.download(), o.jacobian_apply(z)) <= APPLY_RTOL
     # transpose
     yd.upload(y0)
     go.jacobian_apply_transpose(zd, yd)
