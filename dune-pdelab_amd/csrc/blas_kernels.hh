@@ -14,13 +14,6 @@
 
 namespace pdelab_amd {
 
-template <typename A>
-__device__ __forceinline__ A wave_sum(A v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  return v;
-}
-
 // Block-wide sum of NR accumulators; result valid in thread 0.
 template <typename A, int NR>
 __device__ __forceinline__ void block_sum(A (&v)[NR]) {

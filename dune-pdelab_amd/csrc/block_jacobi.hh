@@ -479,16 +479,17 @@ __global__ __launch_bounds__(BJShape<N1>::THREADS) void bj_dense_kernel(const BJ
     if (MODE == BJ_PREC) acc0 += (A)res * (A)u;
     if (MODE == BJ_CGR) acc1 += (A)res * (A)u;
   }
+  // one partial (BJ_PREC) or pair (BJ_CGR) per workgroup, reduced through buf
   if (MODE == BJ_PREC) {
-    A s[1] = {acc0};
-    block_sum_n<S::THREADS, A, 1>(s);
-    if (tid == 0) v.partial[blockIdx.x] = s[0];
+    A r[1] = {acc0};
+    block_sum_reuse<S::WAVES, A, 1>(r, &buf[0][0]);
+    if (tid == 0) v.partial[blockIdx.x] = r[0];
   } else if (MODE == BJ_CGR) {
-    A s[2] = {acc0, acc1};
-    block_sum_n<S::THREADS, A, 2>(s);
+    A r[2] = {acc0, acc1};
+    block_sum_reuse<S::WAVES, A, 2>(r, &buf[0][0]);
     if (tid == 0) {
-      v.partial[2 * blockIdx.x] = s[0];
-      v.partial[2 * blockIdx.x + 1] = s[1];
+      v.partial[2 * blockIdx.x] = r[0];
+      v.partial[2 * blockIdx.x + 1] = r[1];
     }
   }
 }
@@ -650,16 +651,17 @@ __global__ __launch_bounds__(BJFdmShape<N1>::THREADS) void bj_fdm_kernel(const B
   if (MODE == BJ_PREC || MODE == BJ_CGR) {
     // out.in (for BJ_CGR out.r with the updated r); idle lanes hold zeros
     const A d = dot;
+    // one partial (BJ_PREC) or pair (BJ_CGR) per workgroup, reduced through X
     if (MODE == BJ_PREC) {
-      A s[1] = {d};
-      block_sum_n<S::THREADS, A, 1>(s);
-      if (tid == 0) v.partial[blockIdx.x] = s[0];
+      A r[1] = {d};
+      block_sum_reuse<S::WAVES, A, 1>(r, &X[0][0][0]);
+      if (tid == 0) v.partial[blockIdx.x] = r[0];
     } else {
-      A s[2] = {acc0, d};
-      block_sum_n<S::THREADS, A, 2>(s);
+      A r[2] = {acc0, d};
+      block_sum_reuse<S::WAVES, A, 2>(r, &X[0][0][0]);
       if (tid == 0) {
-        v.partial[2 * blockIdx.x] = s[0];
-        v.partial[2 * blockIdx.x + 1] = s[1];
+        v.partial[2 * blockIdx.x] = r[0];
+        v.partial[2 * blockIdx.x + 1] = r[1];
       }
     }
   }

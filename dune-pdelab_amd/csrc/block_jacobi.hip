@@ -46,6 +46,8 @@ long long groups_of(const H* h) {
   return (h->ncells + cpg - 1) / cpg;
 }
 long long bj_groups(const H* h, bool fdm) { return fdm ? groups_of<BJFdmShape>(h) : groups_of<BJShape>(h); }
+// partials of the reducing modes: one (or one pair) per workgroup
+long long bj_partials(const H* h, bool fdm) { return bj_groups(h, fdm); }
 
 BJGeom geom(const H* h, bool per_cell) {
   BJGeom g;
@@ -166,8 +168,8 @@ int bj_prepare(H* h, bool trans, int precision) {
     b.esize = esize;
   }
   if (!h->d_bjpart) {
-    const size_t ng = (size_t)std::max(bj_groups(h, true), bj_groups(h, false));
-    HIPCHK(h, hipMalloc(&h->d_bjpart, sizeof(double) * 2 * ng));
+    const size_t np = (size_t)std::max(bj_partials(h, true), bj_partials(h, false));
+    HIPCHK(h, hipMalloc(&h->d_bjpart, sizeof(double) * 2 * np));
   }
   const bool need64 = precision == 64 && !b.valid64;
   const bool need32 = precision == 32 && !b.valid32;
@@ -190,7 +192,7 @@ int bj_launch(H* h, bool trans, int mode, BJVec<T, A> v, int* nparts) {
   const BlockJacobiData& b = h->bj[trans ? 1 : 0];
   const T* data = bj_data<T>(b);
   v.partial = static_cast<A*>(h->d_bjpart);
-  if (nparts) *nparts = (int)bj_groups(h, b.fdm);
+  if (nparts) *nparts = (int)bj_partials(h, b.fdm);
   if (mode == BJ_PREC) return launch_m<T, A, BJ_PREC>(h, b, data, v);
   if (mode == BJ_ADD) return launch_m<T, A, BJ_ADD>(h, b, data, v);
   return launch_m<T, A, BJ_CGR>(h, b, data, v);

@@ -132,6 +132,7 @@ struct ApplyShape {
   static constexpr int NLP = NL + (N1 == 4 ? 16 : 1);
   static constexpr int CPB = N1 == 2 ? 64 : (N1 == 3 ? 32 : 16);
   static constexpr int THREADS = CPB * L;  // 256 (Q1, Q3) or 288 (Q2)
+  static constexpr int WAVES = (THREADS + 63) / 64;
 };
 
 template <int N1, int D>
@@ -531,9 +532,10 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
     }
   }
   if (DOT) {
-    // fused z . (J z) for the Krylov step length (one partial per workgroup)
+    // fused z . (J z) for the Krylov step length, one partial per workgroup, reduced
+    // through the kernel's own LDS array (a separate array would cost a workgroup per CU)
     T v[1] = {pq};
-    block_sum_n<SH::THREADS, T, 1>(v);
+    block_sum_reuse<SH::WAVES, T, 1>(v, U);
     if (threadIdx.x == 0) dotp[blockIdx.x] = v[0];
   }
 }
@@ -696,9 +698,10 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
     }
   }
   if (DOT) {
-    // fused z . (J z) for the Krylov step length (one partial per workgroup)
+    // fused z . (J z) for the Krylov step length, one partial per workgroup, reduced
+    // through the kernel's own LDS array (a separate array would cost a workgroup per CU)
     T v[1] = {pq};
-    block_sum_n<SH::THREADS, T, 1>(v);
+    block_sum_reuse<SH::WAVES, T, 1>(v, LS);
     if (threadIdx.x == 0) dotp[blockIdx.x] = v[0];
   }
 }
