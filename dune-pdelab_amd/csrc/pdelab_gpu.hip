@@ -622,7 +622,8 @@ static int cg_loop(H* h, T* x, T* b, T* p, T* q, T* z, const Prec<T>& P, bool tr
     }
     hipLaunchKernelGGL((cg_check_kernel<A>), dim3(1), dim3(kFinalThreads), 0, h->stream, parts, nparts, scal + oslot,
                        h->d_state, h->d_dstate, i);
-    hipLaunchKernelGGL((cg_xpupdate_kernel<T, A>), dim3(grid_for(n)), dim3(256), 0, h->stream, n, x, p,
+    hipLaunchKernelGGL((cg_xpupdate_kernel<T, A>), dim3((unsigned)((n + 256LL * xp_ept<T>() - 1) / (256LL * xp_ept<T>()))),
+                       dim3(256), 0, h->stream, n, x, p,
                        (const T*)b, P.dinv, (const T*)(P.block() ? z : nullptr), (const A*)scal, rslot, oslot + 1,
                        (const int*)h->d_state, i);
     HIPCHK(h, hipGetLastError());
