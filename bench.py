@@ -17,7 +17,9 @@ The JSON line also carries:
   convection    96^3 Q2 convection-diffusion (kappa 1e-2, b = (1, .5, .25), eight seeded
                 Gaussian sources, g = 0): FP64 BiCGSTAB/Jacobi vs FP32-inner BiCGSTAB in
                 FP64 refinement to 1e-8, and the adjoint (J^T, downwind) gradient of
-                1/2 u^T M u with respect to a per-cell source.
+                1/2 u^T M u with respect to a per-cell source;
+  matrix_based  32^3 Q2: the assembled Jacobian (7-point block pattern) and its block-matrix
+                apply against the matrix-free apply.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve] [--no-cpu]
 """
@@ -134,6 +136,42 @@ def convection_leg(pkg):
     out["adjoint_source_gradient"] = {"seconds": t1 - t0, "iterations": ra["iterations"],
                                       "converged": bool(ra["converged"]), "grad_norm": float(np.linalg.norm(gh))}
     for v in (x64, xm, rhs, lam, g):
+        v.free()
+    go.close()
+    return out
+
+
+def matrix_based_leg(pkg):
+    """32^3 Q2 SIPG Poisson: the assembled Jacobian (7-point block pattern, GridOperator::jacobian)
+    and its block-matrix apply y = A z against the matrix-free operator apply."""
+    cfg = pkg.make_config(n=32, degree=2, alpha=3.0, problem=pkg.PROBLEM_EXP)
+    go = pkg.GridOperator(cfg)
+    N, nloc = go.N, 27
+    z = go.vector(splitmix_uniform(N, 7))
+    y = go.vector()
+    blocks, cols = go.jacobian()  # first use loads the kernels; the second is timed
+    blocks.free()
+    cols.free()
+    go.synchronize()
+    t0 = time.perf_counter()
+    blocks, cols = go.jacobian()
+    go.synchronize()
+    t_asm = time.perf_counter() - t0
+    reps = 20
+    go.block_matrix_apply(blocks, cols, z, y)
+    go.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        go.block_matrix_apply(blocks, cols, z, y)
+    go.synchronize()
+    t_mb = (time.perf_counter() - t0) / reps
+    t_mf = go.time_apply(z, y, warmup=3, reps=reps) * 1e-3
+    mat_bytes = 7.0 * go.ncells * nloc * nloc * 8
+    out = {"workload": "32^3 Q2 SIPG Poisson, 884,736 DoF", "assembly_seconds": t_asm,
+           "matrix_bytes": mat_bytes, "block_matrix_apply_us": t_mb * 1e6,
+           "block_matrix_apply_GBs": (mat_bytes + 16.0 * N) / t_mb / 1e9,
+           "matrix_free_apply_us": t_mf * 1e6, "matrix_free_speedup": t_mb / t_mf}
+    for v in (z, y, blocks, cols):
         v.free()
     go.close()
     return out
@@ -371,6 +409,8 @@ def main():
         go3.close()
         if not args.no_convection:
             out["convection"] = convection_leg(pkg)
+        log("bench: 32^3 Q2 assembled Jacobian and block-matrix apply")
+        out["matrix_based"] = matrix_based_leg(pkg)
     else:
         z.free()
         y.free()

@@ -79,6 +79,8 @@ def lib():
     L.pdelab_gpu_point_diagonal.argtypes = [_vp, _vp]
     L.pdelab_gpu_operator_apply.argtypes = [_vp, _vp, _vp]
     L.pdelab_gpu_block_diagonal.argtypes = [_vp, ctypes.c_int, _vp]
+    L.pdelab_gpu_jacobian.argtypes = [_vp, ctypes.c_int, _vp, _vp]
+    L.pdelab_gpu_block_matrix_apply.argtypes = [_vp, _vp, _vp, _vp, _vp]
     L.pdelab_gpu_block_jacobi_apply.argtypes = [_vp, _vp, _vp]
     L.pdelab_gpu_block_jacobi_apply_f32.argtypes = [_vp, _vp, _vp]
     L.pdelab_gpu_dot.argtypes = [_vp, _vp, _vp, _dp]
@@ -252,6 +254,19 @@ class GridOperator:
     def block_diagonal(self, blocks, transpose=False):
         """blocks (ncells * nloc^2 doubles) = the diagonal blocks of J (J^T), row-major per cell."""
         self._check(lib().pdelab_gpu_block_diagonal(self.h, 1 if transpose else 0, blocks.p))
+
+    def jacobian(self, transpose=False):
+        """Assembled Jacobian (7-point block pattern) as device vectors (blocks, cols); cols holds
+        int64 values in a float64-sized buffer (download with .download().view(np.int64))."""
+        nloc = (self.cfg.degree + 1) ** 3
+        blocks = DeviceVector(self, 7 * self.ncells * nloc * nloc)
+        cols = DeviceVector(self, 7 * self.ncells)
+        self._check(lib().pdelab_gpu_jacobian(self.h, 1 if transpose else 0, blocks.p, cols.p))
+        return blocks, cols
+
+    def block_matrix_apply(self, blocks, cols, z, y):
+        """y = A z for a block matrix from jacobian()."""
+        self._check(lib().pdelab_gpu_block_matrix_apply(self.h, blocks.p, cols.p, z.p, y.p))
 
     def block_jacobi_apply(self, z, y):
         """y += D^{-1} z with D the block diagonal of J (FP32 for float32 vectors)."""

@@ -1,8 +1,10 @@
-// block_jacobi.hip -- set-up and launches of the per-cell block Jacobi preconditioner.
+// block_jacobi.hip -- set-up and launches of the per-cell block Jacobi preconditioner, and
+// the assembled Jacobian in the 7-point block pattern.
 #include <cmath>
 #include <cstring>
 
 #include "handle.hh"
+#include "jacobian.hh"
 
 namespace {
 
@@ -200,6 +202,54 @@ int bj_launch(H* h, bool trans, int mode, BJVec<T, A> v, int* nparts) {
 
 template int bj_launch<double, double>(H*, bool, int, BJVec<double, double>, int*);
 template int bj_launch<float, float>(H*, bool, int, BJVec<float, float>, int*);
+
+template <int N1, bool TRANS>
+static int jac_launch_t(H* h, const BJGeom& g, double* blocks, long long* cols) {
+  const dim3 grid((unsigned)h->ncells), block(256);
+  if (h->conv) {
+    if (h->vark)
+      hipLaunchKernelGGL((jacobian_kernel<N1, TRANS, true, true>), grid, block, 0, h->stream, h->a64,
+                         (const double*)h->d_fc64, g, blocks, cols);
+    else
+      hipLaunchKernelGGL((jacobian_kernel<N1, TRANS, true, false>), grid, block, 0, h->stream, h->a64,
+                         (const double*)h->d_fc64, g, blocks, cols);
+  } else if (h->vark) {
+    hipLaunchKernelGGL((jacobian_kernel<N1, TRANS, false, true>), grid, block, 0, h->stream, h->a64,
+                       (const double*)h->d_fc64, g, blocks, cols);
+  } else {
+    hipLaunchKernelGGL((jacobian_kernel<N1, TRANS, false, false>), grid, block, 0, h->stream, h->a64,
+                       (const double*)h->d_fc64, g, blocks, cols);
+  }
+  HIPCHK(h, hipGetLastError());
+  return PDELAB_OK;
+}
+
+int jac_assemble(H* h, bool trans, double* blocks, long long* cols) {
+  if (h->ncells >= (1LL << 31)) return fail(h, PDELAB_ERR_UNSUPPORTED, "jacobian: more than 2^31 - 1 cells");
+  const BJGeom g = geom(h, true);
+  switch (h->n1) {
+    case 2: return trans ? jac_launch_t<2, true>(h, g, blocks, cols) : jac_launch_t<2, false>(h, g, blocks, cols);
+    case 3: return trans ? jac_launch_t<3, true>(h, g, blocks, cols) : jac_launch_t<3, false>(h, g, blocks, cols);
+    default: return trans ? jac_launch_t<4, true>(h, g, blocks, cols) : jac_launch_t<4, false>(h, g, blocks, cols);
+  }
+}
+
+int jac_block_apply(H* h, const double* blocks, const long long* cols, const double* z, double* y) {
+  const dim3 grid((unsigned)h->ncells), block(256);
+  switch (h->n1) {
+    case 2:
+      hipLaunchKernelGGL(block_matrix_apply_kernel<2>, grid, block, 0, h->stream, blocks, cols, z, y, h->ncells);
+      break;
+    case 3:
+      hipLaunchKernelGGL(block_matrix_apply_kernel<3>, grid, block, 0, h->stream, blocks, cols, z, y, h->ncells);
+      break;
+    default:
+      hipLaunchKernelGGL(block_matrix_apply_kernel<4>, grid, block, 0, h->stream, blocks, cols, z, y, h->ncells);
+      break;
+  }
+  HIPCHK(h, hipGetLastError());
+  return PDELAB_OK;
+}
 
 int bj_block_diagonal(H* h, bool trans, double* blocks) {
   if (h->ncells >= (1LL << 31)) return fail(h, PDELAB_ERR_UNSUPPORTED, "block diagonal: more than 2^31 - 1 cells");

@@ -126,5 +126,8 @@ void bj_free(H* h);
 template <typename T, typename A>
 int bj_launch(H* h, bool trans, int mode, BJVec<T, A> v, int* nparts);
 int bj_block_diagonal(H* h, bool trans, double* blocks);
+// Assembled Jacobian in the 7-point block pattern (jacobian.hh) and its apply y = A z.
+int jac_assemble(H* h, bool trans, double* blocks, long long* cols);
+int jac_block_apply(H* h, const double* blocks, const long long* cols, const double* z, double* y);
 
 #endif

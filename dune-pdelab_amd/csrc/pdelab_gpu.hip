@@ -1176,6 +1176,18 @@ int pdelab_gpu_block_diagonal(pdelab_gpu_gridoperator* h, int transpose, double*
   return bj_block_diagonal(h, transpose != 0, blocks);
 }
 
+int pdelab_gpu_jacobian(pdelab_gpu_gridoperator* h, int transpose, double* blocks, long long* cols) {
+  if (!h || !blocks || !cols) return PDELAB_ERR_INVALID_ARG;
+  CHK(check_outflow(h));
+  return jac_assemble(h, transpose != 0, blocks, cols);
+}
+
+int pdelab_gpu_block_matrix_apply(pdelab_gpu_gridoperator* h, const double* blocks, const long long* cols,
+                                  const double* z, double* y) {
+  if (!h || !blocks || !cols || !z || !y) return PDELAB_ERR_INVALID_ARG;
+  return jac_block_apply(h, blocks, cols, z, y);
+}
+
 int pdelab_gpu_block_jacobi_apply(pdelab_gpu_gridoperator* h, const double* z, double* y) {
   if (!h || !z || !y) return PDELAB_ERR_INVALID_ARG;
   CHK(check_outflow(h));

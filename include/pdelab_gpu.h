@@ -17,6 +17,12 @@
  *                                    linear operator) gridoperator.hh:200-205
  *   pdelab_gpu_point_diagonal        residual of PointDiagonalLocalOperatorWrapper
  *                                    R:dune/pdelab/localoperator/pointdiagonalwrapper.hh:51-131
+ *   pdelab_gpu_jacobian              GridOperator::jacobian(x, A) into the BCRS matrix of
+ *                                    the 7-point block pattern
+ *                                    R:dune/pdelab/gridoperator/gridoperator.hh:184-189
+ *   pdelab_gpu_block_matrix_apply    MatrixAdapter::apply over that matrix (y = A z), the
+ *                                    operator of the matrix-based ISTL backends
+ *                                    R:dune/pdelab/backend/istl/seqistlsolverbackend.hh:237-241
  *   pdelab_gpu_block_diagonal        jacobian of BlockDiagonalLocalOperatorWrapper
  *                                    R:dune/pdelab/localoperator/blockdiagonalwrapper.hh:79-160
  *   pdelab_gpu_block_jacobi_apply    GridOperatorPreconditioner::apply over
@@ -230,6 +236,15 @@ int pdelab_gpu_point_diagonal(pdelab_gpu_gridoperator* h, double* d);
    if transpose), ncells * (k+1)^6 doubles, row-major per cell, device memory
    (overwritten). */
 int pdelab_gpu_block_diagonal(pdelab_gpu_gridoperator* h, int transpose, double* blocks);
+/* The Jacobian (of J^T if transpose) as a block matrix with the 7-point block pattern,
+   device arrays: blocks[(7c + s) nloc^2 + i nloc + j] = dR_i(cell c) / dx_j(cell cols[7c + s]),
+   nloc = (k+1)^3; slot 0 is cell c itself, slot 1 + f its neighbour across face f = 2d + side
+   (x-, x+, y-, y+, z-, z+), cols = -1 and a zero block where that face is on the boundary.
+   blocks holds 7 ncells nloc^2 doubles, cols 7 ncells entries; both are overwritten. */
+int pdelab_gpu_jacobian(pdelab_gpu_gridoperator* h, int transpose, double* blocks, long long* cols);
+/* y = A z for a block matrix in the layout of pdelab_gpu_jacobian (device arrays). */
+int pdelab_gpu_block_matrix_apply(pdelab_gpu_gridoperator* h, const double* blocks, const long long* cols,
+                                  const double* z, double* y);
 /* y += D^{-1} z with D the block diagonal of J: the block Jacobi preconditioner as the
    Krylov solvers apply it with PDELAB_PREC_BLOCK_JACOBI. The blocks are inverted
    once (and again after the next kappa change). */
