@@ -19,7 +19,8 @@ The JSON line also carries:
                 FP64 refinement to 1e-8, and the adjoint (J^T, downwind) gradient of
                 1/2 u^T M u with respect to a per-cell source;
   matrix_based  32^3 Q2: the assembled Jacobian (7-point block pattern) and its block-matrix
-                apply against the matrix-free apply.
+                apply against the matrix-free apply;
+  instationary  three Alexander2 steps of the heat equation at 64^3 Q3 (OneStepMethod).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve] [--no-cpu]
 """
@@ -172,6 +173,27 @@ def matrix_based_leg(pkg):
            "block_matrix_apply_GBs": (mat_bytes + 16.0 * N) / t_mb / 1e9,
            "matrix_free_apply_us": t_mf * 1e6, "matrix_free_speedup": t_mb / t_mf}
     for v in (z, y, blocks, cols):
+        v.free()
+    go.close()
+    return out
+
+
+def instationary_leg(pkg, n, k):
+    """Three Alexander2 steps (dt = 1e-3) of u_t - div(grad u) = f on the 64^3 Q3 exp problem from
+    u = 0, each stage a CG/block-Jacobi solve to 1e-10 (OneStepMethod::apply)."""
+    cfg = pkg.make_config(n=n, degree=k, alpha=3.0, problem=pkg.PROBLEM_EXP)
+    go = pkg.GridOperator(cfg)
+    xo, xn = go.vector(), go.vector()
+    steps = []
+    for _ in range(3):
+        r = go.onestep_step(xo, xn, scheme=pkg.TS_ALEXANDER2, dt=1e-3, method=pkg.SOLVER_CG,
+                            precond=pkg.PREC_BLOCK_JACOBI, reduction=1e-10)
+        steps.append({"seconds": r["elapsed"], "linear_solver_iterations": r["linear_solver_iterations"],
+                      "converged": bool(r["converged"])})
+        xo.copy_from(xn)
+    out = {"workload": f"{n}^3 Q{k} SIPG heat equation, Alexander2 (2 stages), dt 1e-3, CG/block Jacobi to 1e-10 "
+                       f"per stage", "steps": steps}
+    for v in (xo, xn):
         v.free()
     go.close()
     return out
@@ -411,6 +433,8 @@ def main():
             out["convection"] = convection_leg(pkg)
         log("bench: 32^3 Q2 assembled Jacobian and block-matrix apply")
         out["matrix_based"] = matrix_based_leg(pkg)
+        log("bench: instationary Alexander2 steps")
+        out["instationary"] = instationary_leg(pkg, n, k)
     else:
         z.free()
         y.free()

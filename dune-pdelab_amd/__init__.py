@@ -21,8 +21,9 @@ from . import config as cfgmod
 from .config import (  # noqa: F401  (re-exported)
     BC_DIRICHLET, BC_NEUMANN, BC_NONE, BC_OUTFLOW, IIPG, NIPG, PREC_BLOCK_JACOBI, PREC_JACOBI, PREC_NONE,
     PROBLEM_CONSTANT, PROBLEM_EXP, PROBLEM_GAUSSIANS, PROBLEM_QUADRATIC, PROBLEM_TABULATED,
-    PROBLEM_ZERO, SIPG, SOLVER_BICGSTAB, SOLVER_CG, WEIGHTS_OFF, WEIGHTS_ON, LinearProblemResult,
-    LinearSolverResult, MixedResult, PdelabConfig, make_config, ncells, ndofs)
+    PROBLEM_ZERO, SIPG, SOLVER_BICGSTAB, SOLVER_CG, TS_ALEXANDER2, TS_CRANK_NICOLSON, TS_IMPLICIT_EULER,
+    WEIGHTS_OFF, WEIGHTS_ON, LinearProblemResult, LinearSolverResult, MixedResult, OneStepResult, PdelabConfig,
+    make_config, ncells, ndofs)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "lib", "libpdelab_gpu.so")
@@ -80,6 +81,10 @@ def lib():
     L.pdelab_gpu_operator_apply.argtypes = [_vp, _vp, _vp]
     L.pdelab_gpu_block_diagonal.argtypes = [_vp, ctypes.c_int, _vp]
     L.pdelab_gpu_jacobian.argtypes = [_vp, ctypes.c_int, _vp, _vp]
+    L.pdelab_gpu_onestep_jacobian_apply.argtypes = [_vp, ctypes.c_double, ctypes.c_double, _vp, _vp]
+    L.pdelab_gpu_onestep_step.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_int, _vp, _vp,
+                                          ctypes.POINTER(cfgmod.OneStepResult)]
     L.pdelab_gpu_block_matrix_apply.argtypes = [_vp, _vp, _vp, _vp, _vp]
     L.pdelab_gpu_block_jacobi_apply.argtypes = [_vp, _vp, _vp]
     L.pdelab_gpu_block_jacobi_apply_f32.argtypes = [_vp, _vp, _vp]
@@ -267,6 +272,19 @@ class GridOperator:
     def block_matrix_apply(self, blocks, cols, z, y):
         """y = A z for a block matrix from jacobian()."""
         self._check(lib().pdelab_gpu_block_matrix_apply(self.h, blocks.p, cols.p, z.p, y.p))
+
+    # -- instationary
+    def onestep_jacobian_apply(self, a_mass, b_dt, z, y):
+        """y += b_dt J z + a_mass M z (a stage of OneStepGridOperator)."""
+        self._check(lib().pdelab_gpu_onestep_jacobian_apply(self.h, a_mass, b_dt, z.p, y.p))
+
+    def onestep_step(self, xold, xnew, scheme=TS_ALEXANDER2, dt=0.1, method=SOLVER_CG, precond=PREC_JACOBI,
+                     reduction=1e-10, maxit=5000):
+        """One step xold -> xnew of OneStepMethod::apply (xnew holds the first stage's initial guess)."""
+        res = cfgmod.OneStepResult()
+        self._check(lib().pdelab_gpu_onestep_step(self.h, scheme, method, precond, dt, reduction, maxit, xold.p,
+                                                  xnew.p, ctypes.byref(res)))
+        return res.as_dict()
 
     def block_jacobi_apply(self, z, y):
         """y += D^{-1} z with D the block diagonal of J (FP32 for float32 vectors)."""

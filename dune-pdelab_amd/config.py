@@ -85,6 +85,22 @@ class LinearProblemResult(ctypes.Structure):
         return d
 
 
+TS_IMPLICIT_EULER, TS_CRANK_NICOLSON, TS_ALEXANDER2 = 0, 1, 2
+
+
+class OneStepResult(ctypes.Structure):
+    _fields_ = [
+        ("converged", ctypes.c_int),
+        ("stages", ctypes.c_int),
+        ("linear_solver_iterations", ctypes.c_int),
+        ("linear_solver_time", ctypes.c_double),
+        ("elapsed", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class MixedResult(ctypes.Structure):
     _fields_ = [
         ("converged", ctypes.c_int),

@@ -52,7 +52,7 @@ static int launch_apply_c(H* h, const T* z, T* y, const T* fc, const DGArgs<T>& 
 template <int N1, typename T>
 int launch_apply_n(H* h, const T* z, T* y, const T* fc, const DGArgs<T>& a, bool trans, int mode, T* dotp,
                           const int* stop) {
-  const bool react = h->cfg.c != 0.0;
+  const bool react = h->cfg.c + h->shift != 0.0;
   return trans ? launch_apply_c<N1, T, true>(h, z, y, fc, a, react, mode, dotp, stop)
                : launch_apply_c<N1, T, false>(h, z, y, fc, a, react, mode, dotp, stop);
 }

@@ -73,6 +73,10 @@ struct pdelab_gpu_gridoperator {
   // apply kernel on aligned-row grids: 0 the aligned-row kernel, 1 the general kernel
   int apply_variant = 0;
   BlockJacobiData bj[2];  // [0] J, [1] J^T
+  // reaction shift of the one-step stage operators: the kernels apply J + shift M
+  double shift = 0.0;
+  double prec_shift = 0.0;  // shift for which the cached diagonal and block Jacobi data hold
+  double* os[4] = {nullptr, nullptr, nullptr, nullptr};  // one-step work vectors
   void* d_bjpart = nullptr;  // per-workgroup partials of the block Jacobi application
 };
 

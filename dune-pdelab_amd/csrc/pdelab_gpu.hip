@@ -529,9 +529,19 @@ struct Prec {
   bool block() const { return kind == PDELAB_PREC_BLOCK_JACOBI; }
 };
 
+// The cached point diagonal and block Jacobi data belong to the operator with the current
+// reaction shift (one-step stage operators); dropped when the shift has changed since.
+static void sync_prec_cache(H* h) {
+  if (h->prec_shift == h->shift) return;
+  h->diag_valid = false;
+  bj_invalidate(h);
+  h->prec_shift = h->shift;
+}
+
 // Checks the preconditioner and prepares its data (point diagonal or block data).
 template <typename T>
 static int make_prec(H* h, int precond, bool trans, Prec<T>& P) {
+  sync_prec_cache(h);
   P.kind = precond;
   P.trans = trans;
   if (precond == PDELAB_PREC_JACOBI) CHK(compute_diag(h));
@@ -1009,6 +1019,8 @@ int pdelab_gpu_destroy(pdelab_gpu_gridoperator* h) {
                   h->d_state, h->d_dstate, h->d_lm64, h->d_lm32, h->d_fc64, h->d_fc32};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  for (double* p : h->os)
+    if (p) (void)hipFree(p);
   bj_free(h);
   for (int i = 0; i < 8; ++i) {
     if (h->w64[i]) (void)hipFree(h->w64[i]);
@@ -1163,6 +1175,7 @@ int pdelab_gpu_jacobian_apply_transpose(pdelab_gpu_gridoperator* h, const double
 int pdelab_gpu_point_diagonal(pdelab_gpu_gridoperator* h, double* d) {
   if (!h || !d) return PDELAB_ERR_INVALID_ARG;
   CHK(check_outflow(h));
+  sync_prec_cache(h);
   CHK(compute_diag(h));
   hipLaunchKernelGGL(axpy_kernel<double>, dim3(grid_for(h->N)), dim3(256), 0, h->stream, h->N, 1.0,
                      (const double*)h->d_diag, d);
@@ -1191,6 +1204,7 @@ int pdelab_gpu_block_matrix_apply(pdelab_gpu_gridoperator* h, const double* bloc
 int pdelab_gpu_block_jacobi_apply(pdelab_gpu_gridoperator* h, const double* z, double* y) {
   if (!h || !z || !y) return PDELAB_ERR_INVALID_ARG;
   CHK(check_outflow(h));
+  sync_prec_cache(h);
   const BJVec<double, double> v{z, y, nullptr, nullptr, nullptr, 0, nullptr, nullptr};
   return bj_launch<double, double>(h, false, BJ_ADD, v, nullptr);
 }
@@ -1198,6 +1212,7 @@ int pdelab_gpu_block_jacobi_apply(pdelab_gpu_gridoperator* h, const double* z, d
 int pdelab_gpu_block_jacobi_apply_f32(pdelab_gpu_gridoperator* h, const float* z, float* y) {
   if (!h || !z || !y) return PDELAB_ERR_INVALID_ARG;
   CHK(check_outflow(h));
+  sync_prec_cache(h);
   const BJVec<float, float> v{z, y, nullptr, nullptr, nullptr, 0, nullptr, nullptr};
   return bj_launch<float, float>(h, false, BJ_ADD, v, nullptr);
 }
@@ -1342,6 +1357,153 @@ int pdelab_gpu_solve_mixed(pdelab_gpu_gridoperator* h, int method, int precond, 
   res->first_defect = def0;
   res->defect = def;
   res->reduction = def0 > 0 ? def / def0 : 0.0;
+  res->elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return PDELAB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// One-step methods. A stage operator a M + b J equals b (J + (a/b) M), and J + s M is J with
+// the reaction coefficient c + s: the reaction term c u v is the DG mass on the same
+// quadrature. The stage solves therefore run the apply kernels with a shifted reaction.
+static void set_reaction_shift(H* h, double s) {
+  if (s == h->shift) return;
+  h->shift = s;
+  const double c = h->cfg.c + s;
+  h->a64.c = c;
+  h->a64.vc = h->a64.V * c;
+  h->a32.c = (float)c;
+  h->a32.vc = (float)(h->a64.V * c);
+}
+
+// Coefficients a(r, i) = A[r-1][i], b(r, i) = B[r-1][i] of the schemes
+// (R:dune/pdelab/instationary/onestepparameter.hh: ImplicitEulerParameter :193,
+// OneStepThetaParameter :96-101 with theta = 1/2, Alexander2Parameter :449-461).
+struct StepScheme {
+  int s;
+  double A[2][3], B[2][3];
+};
+
+static bool step_scheme(int scheme, StepScheme& p) {
+  std::memset(&p, 0, sizeof(p));
+  if (scheme == PDELAB_TS_IMPLICIT_EULER) {
+    p.s = 1;
+    p.A[0][0] = -1.0; p.A[0][1] = 1.0;
+    p.B[0][0] = 0.0; p.B[0][1] = 1.0;
+  } else if (scheme == PDELAB_TS_CRANK_NICOLSON) {
+    p.s = 1;
+    p.A[0][0] = -1.0; p.A[0][1] = 1.0;
+    p.B[0][0] = 0.5; p.B[0][1] = 0.5;
+  } else if (scheme == PDELAB_TS_ALEXANDER2) {
+    const double alpha = 1.0 - 0.5 * std::sqrt(2.0);
+    p.s = 2;
+    p.A[0][0] = -1.0; p.A[0][1] = 1.0; p.A[0][2] = 0.0;
+    p.A[1][0] = -1.0; p.A[1][1] = 0.0; p.A[1][2] = 1.0;
+    p.B[0][0] = 0.0; p.B[0][1] = alpha; p.B[0][2] = 0.0;
+    p.B[1][0] = 0.0; p.B[1][1] = 1.0 - alpha; p.B[1][2] = alpha;
+  } else {
+    return false;
+  }
+  return true;
+}
+
+static int ensure_os(H* h) {
+  for (int i = 0; i < 4; ++i)
+    if (!h->os[i]) HIPCHK(h, hipMalloc(&h->os[i], sizeof(double) * h->N));
+  return PDELAB_OK;
+}
+
+// y += M z with the DG mass
+static int mass_add(H* h, const double* z, double* y) { return pdelab_gpu_mass_apply(h, z, y); }
+
+int pdelab_gpu_onestep_jacobian_apply(pdelab_gpu_gridoperator* h, double a_mass, double b_dt, const double* z,
+                                      double* y) {
+  if (!h || !z || !y) return PDELAB_ERR_INVALID_ARG;
+  CHK(check_outflow(h));
+  CHK(ensure_os(h));
+  const long long n = h->N;
+  double* t = h->os[3];
+  if (b_dt != 0.0) {
+    // t = (J + (a/b) M) z, y += b t
+    set_reaction_shift(h, a_mass / b_dt);
+    const int st = apply64(h, z, t, false, false);
+    set_reaction_shift(h, 0.0);
+    CHK(st);
+    hipLaunchKernelGGL(axpy_kernel<double>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, b_dt, (const double*)t, y);
+  } else if (a_mass != 0.0) {
+    HIPCHK(h, hipMemsetAsync(t, 0, sizeof(double) * n, h->stream));
+    CHK(mass_add(h, z, t));
+    hipLaunchKernelGGL(axpy_kernel<double>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, a_mass, (const double*)t, y);
+  }
+  HIPCHK(h, hipGetLastError());
+  return PDELAB_OK;
+}
+
+int pdelab_gpu_onestep_step(pdelab_gpu_gridoperator* h, int scheme, int method, int precond, double dt,
+                            double reduction, int maxit, const double* xold, double* xnew,
+                            pdelab_onestep_result* res) {
+  if (!h || !xold || !xnew || !res || !(dt > 0)) return PDELAB_ERR_INVALID_ARG;
+  std::memset(res, 0, sizeof(*res));
+  StepScheme P;
+  if (!step_scheme(scheme, P)) return fail(h, PDELAB_ERR_INVALID_ARG, "unknown one-step scheme");
+  CHK(check_outflow(h));
+  auto t0 = std::chrono::steady_clock::now();
+  const long long n = h->N;
+  set_reaction_shift(h, 0.0);
+  CHK(compute_R0(h));
+  CHK(ensure_os(h));
+  CHK(ensure_ws<double>(h, 7));
+  double *C = h->os[0], *r = h->os[1], *stage = h->os[2], *t = h->os[3], *z = h->w64[6];
+  const dim3 gv(grid_for(n)), bv(256);
+  // adds w R(u) = w (J u + R(0)) and w_m M u to y
+  auto add_terms = [&](const double* u, double w, double wm, double* y) -> int {
+    if (w != 0.0) {
+      CHK(apply64(h, u, t, false, false));
+      hipLaunchKernelGGL(axpy_kernel<double>, gv, bv, 0, h->stream, n, w, (const double*)t, y);
+      hipLaunchKernelGGL(axpy_kernel<double>, gv, bv, 0, h->stream, n, w, (const double*)h->d_R0, y);
+    }
+    if (wm != 0.0) {
+      HIPCHK(h, hipMemsetAsync(t, 0, sizeof(double) * n, h->stream));
+      CHK(mass_add(h, u, t));
+      hipLaunchKernelGGL(axpy_kernel<double>, gv, bv, 0, h->stream, n, wm, (const double*)t, y);
+    }
+    HIPCHK(h, hipGetLastError());
+    return PDELAB_OK;
+  };
+  res->converged = 1;
+  res->stages = P.s;
+  for (int rr = 1; rr <= P.s; ++rr) {
+    const double* A = P.A[rr - 1];
+    const double* B = P.B[rr - 1];
+    // constant part of the stage residual from the earlier stages
+    HIPCHK(h, hipMemsetAsync(C, 0, sizeof(double) * n, h->stream));
+    for (int j = 0; j < rr; ++j) CHK(add_terms(j == 0 ? xold : stage, B[j] * dt, A[j], C));
+    // the stage unknown and its initial guess (implicitonestep.hh:170-183)
+    double* x = rr == P.s ? xnew : stage;
+    if (rr > 1) HIPCHK(h, hipMemcpyAsync(x, stage, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream));
+    else if (x != xnew) HIPCHK(h, hipMemcpyAsync(x, xnew, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream));
+    const double bdt = B[rr] * dt, am = A[rr];
+    if (!(bdt > 0.0)) return fail(h, PDELAB_ERR_UNSUPPORTED, "one-step: explicit stage");
+    // StationaryLinearProblemSolver::apply on the stage operator am M + bdt J
+    HIPCHK(h, hipMemcpyAsync(r, C, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream));
+    CHK(add_terms(x, bdt, am, r));
+    double defect = 0;
+    CHK(pdelab_gpu_norm(h, r, &defect));
+    const double red = std::max(reduction, 1e-99 / defect);
+    hipLaunchKernelGGL(scale_kernel<double>, gv, bv, 0, h->stream, n, 1.0 / bdt, r);
+    HIPCHK(h, hipMemsetAsync(z, 0, sizeof(double) * n, h->stream));
+    set_reaction_shift(h, am / bdt);
+    pdelab_linear_solver_result ls;
+    auto t1 = std::chrono::steady_clock::now();
+    const int st = krylov<double, double>(h, method, precond, red, maxit, false, z, r, 0, &ls);
+    res->linear_solver_time += std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
+    set_reaction_shift(h, 0.0);
+    res->linear_solver_iterations += (int)ls.iterations;
+    if (!ls.converged) res->converged = 0;
+    CHK(st);
+    hipLaunchKernelGGL(axpy_kernel<double>, gv, bv, 0, h->stream, n, -1.0, (const double*)z, x);
+    HIPCHK(h, hipGetLastError());
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
   res->elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return PDELAB_OK;
 }

@@ -509,8 +509,11 @@ class ISTLBackend_SEQ_MatrixFree_Jac : public SequentialNorm, public LinearResul
   using W = typename GO::Traits::Range;
 
  public:
+  static constexpr int krylov_method = (int)method;
+  static constexpr int preconditioner = (int)prec;
   explicit ISTLBackend_SEQ_MatrixFree_Jac(const GO& go, unsigned maxiter = 5000, int verbose = 1)
       : go_(go), maxiter_(maxiter), verbose_(verbose) {}
+  unsigned maxiter() const { return maxiter_; }
   // solves J z = r to the given relative reduction; r becomes the defect
   void apply(V& z, W& r, double reduction) {
     pdelab_linear_solver_result st;
@@ -605,6 +608,9 @@ class StationaryLinearProblemSolver {
     check(pdelab_gpu_synchronize(x_->handle()), x_->handle());
   }
   const StationaryLinearProblemSolverResult<double>& result() const { return res_; }
+  double reduction() const { return reduction_; }
+  const LS& linearSolver() const { return ls_; }
+  using LinearSolver = LS;
 
  private:
   const GO& go_;
@@ -613,6 +619,85 @@ class StationaryLinearProblemSolver {
   double reduction_, min_defect_;
   int verbose_;
   StationaryLinearProblemSolverResult<double> res_;
+};
+
+// ---- instationary ------------------------------------------------------------------------
+// Implicit one-step schemes (R:dune/pdelab/instationary/onestepparameter.hh).
+template <class R>
+struct ImplicitEulerParameter {
+  int scheme() const { return PDELAB_TS_IMPLICIT_EULER; }
+  unsigned s() const { return 1; }
+};
+template <class R>
+struct Alexander2Parameter {
+  int scheme() const { return PDELAB_TS_ALEXANDER2; }
+  unsigned s() const { return 2; }
+};
+// theta = 1 (implicit Euler) and theta = 1/2 (Crank-Nicolson) are implemented
+template <class R>
+class OneStepThetaParameter {
+ public:
+  explicit OneStepThetaParameter(R theta) : theta_(theta) {
+    if (theta != R(1) && theta != R(0.5)) throw NotImplemented("OneStepThetaParameter: theta must be 1 or 1/2");
+  }
+  int scheme() const { return theta_ == R(1) ? PDELAB_TS_IMPLICIT_EULER : PDELAB_TS_CRANK_NICOLSON; }
+  unsigned s() const { return 1; }
+
+ private:
+  R theta_;
+};
+
+// The DG mass local operator (ScalarL2 / L2FastDG with the operator's quadrature).
+struct L2 {
+  explicit L2(int intorder = 2) : intorder_(intorder) {}
+  int intorder_;
+};
+
+// OneStepGridOperator<GO0, GO1> (R:dune/pdelab/gridoperator/onestep.hh): the spatial grid
+// operator and the DG mass. On the device the mass is the handle's own; GO1 only names it.
+template <class GO0, class GO1>
+class OneStepGridOperator {
+ public:
+  using Traits = typename GO0::Traits;
+  OneStepGridOperator(const GO0& go0, const GO1&) : go0_(go0) {}
+  pdelab_gpu_gridoperator* handle() const { return go0_.handle(); }
+  const GO0& spatialGridOperator() const { return go0_; }
+  typename Traits::Domain makeVector() const { return go0_.makeVector(); }
+  // y += b_dt J z + a_mass M z (the stage Jacobian apply)
+  void jacobian_apply(double a_mass, double b_dt, const typename Traits::Domain& z, typename Traits::Range& y) const {
+    check(pdelab_gpu_onestep_jacobian_apply(handle(), a_mass, b_dt, z.data(), y.data()), handle());
+  }
+
+ private:
+  const GO0& go0_;
+};
+
+// OneStepMethod<R, IGO, PDESOLVER, TrlV, TstV>::apply (R:dune/pdelab/instationary/implicitonestep.hh:144-230)
+// for time-independent problems: each stage one StationaryLinearProblemSolver step with the
+// pde solver's reduction and its backend's Krylov method and preconditioner.
+template <class R, class IGO, class PDESOLVER, class TrlV, class TstV = TrlV>
+class OneStepMethod {
+ public:
+  template <class Method>
+  OneStepMethod(const Method& method, const IGO& igo, PDESOLVER& pdesolver)
+      : scheme_(method.scheme()), igo_(igo), pdesolver_(pdesolver) {}
+  R apply(R time, R dt, TrlV& xold, TrlV& xnew) {
+    using LS = typename PDESOLVER::LinearSolver;
+    pdelab_onestep_result r;
+    check(pdelab_gpu_onestep_step(igo_.handle(), scheme_, LS::krylov_method, LS::preconditioner, (double)dt,
+                                  pdesolver_.reduction(), (int)pdesolver_.linearSolver().maxiter(), xold.data(),
+                                  xnew.data(), &r),
+          igo_.handle());
+    last_ = r;
+    return time + dt;
+  }
+  const pdelab_onestep_result& result() const { return last_; }
+
+ private:
+  int scheme_;
+  const IGO& igo_;
+  PDESOLVER& pdesolver_;
+  pdelab_onestep_result last_{};
 };
 
 // ---- error integration ------------------------------------------------------------------

@@ -37,6 +37,10 @@
  *                                    R:dune/pdelab/stationary/linearproblem.hh:188-302
  *   pdelab_gpu_dot / _norm / _axpy   ISTL::BlockVector dot / two_norm / axpy
  *                                    R:dune/pdelab/backend/istl/vector.hh:249-282
+ *   pdelab_gpu_onestep_jacobian_apply, pdelab_gpu_onestep_step
+ *                                    OneStepGridOperator / OneStepMethod::apply
+ *                                    R:dune/pdelab/gridoperator/onestep.hh,
+ *                                    R:dune/pdelab/instationary/implicitonestep.hh:144-230
  *   pdelab_gpu_mass_apply            ScalarL2::jacobian_apply_volume
  *                                    R:dune/pdelab/localoperator/l2.hh:45-90
  *
@@ -170,6 +174,20 @@ typedef struct pdelab_mixed_result {
   double elapsed;             /* seconds */
 } pdelab_mixed_result;
 
+/* Implicit one-step schemes (R:dune/pdelab/instationary/onestepparameter.hh): implicit Euler
+   (ImplicitEulerParameter), Crank-Nicolson (OneStepThetaParameter, theta = 1/2), Alexander's
+   two-stage order-2 DIRK (Alexander2Parameter). */
+enum { PDELAB_TS_IMPLICIT_EULER = 0, PDELAB_TS_CRANK_NICOLSON = 1, PDELAB_TS_ALEXANDER2 = 2 };
+
+/* Result of one time step (OneStepMethodPartialResult). */
+typedef struct pdelab_onestep_result {
+  int converged;                /* every stage solve converged */
+  int stages;
+  int linear_solver_iterations; /* summed over the stages */
+  double linear_solver_time;    /* seconds */
+  double elapsed;               /* seconds, whole step */
+} pdelab_onestep_result;
+
 typedef struct pdelab_gpu_gridoperator pdelab_gpu_gridoperator;
 
 /* ---- lifecycle ------------------------------------------------------------ */
@@ -284,6 +302,22 @@ int pdelab_gpu_linear_problem_solve(pdelab_gpu_gridoperator* h, int method, int 
 int pdelab_gpu_solve_mixed(pdelab_gpu_gridoperator* h, int method, int precond,
                            double inner_reduction, int inner_maxit, double outer_reduction,
                            int outer_maxit, double* x, pdelab_mixed_result* res);
+
+/* ---- instationary (OneStepGridOperator over the operator and the DG mass) ---- */
+/* y += b_dt J z + a_mass M z: the Jacobian apply of a stage of OneStepGridOperator<GO, GO_mass>
+   (R:dune/pdelab/gridoperator/onestep.hh; weights b_rr dt on the spatial operator and a_rr on
+   the mass, the MultiplyOperator0ByDT mode of gridoperator/onestep/localassembler.hh:122-126),
+   with M the ScalarL2 mass of pdelab_gpu_mass_apply. */
+int pdelab_gpu_onestep_jacobian_apply(pdelab_gpu_gridoperator* h, double a_mass, double b_dt, const double* z,
+                                      double* y);
+/* One step xold -> xnew of OneStepMethod::apply (R:dune/pdelab/instationary/implicitonestep.hh:144-230)
+   for the time-independent problem of the handle: each stage r solves
+     sum_{j<r} (b_rj dt R(u_j) + a_rj M u_j) + b_rr dt R(u_r) + a_rr M u_r = 0
+   with one StationaryLinearProblemSolver step (Krylov method, preconditioner, reduction).
+   Stage 1 starts from xnew as given, later stages from the previous stage. Device vectors. */
+int pdelab_gpu_onestep_step(pdelab_gpu_gridoperator* h, int scheme, int method, int precond, double dt,
+                            double reduction, int maxit, const double* xold, double* xnew,
+                            pdelab_onestep_result* res);
 
 /* ---- adjoint gradients ---------------------------------------------------- */
 /* g[c] = -lambda^T (dJ/dlog kappa_c) u for every cell (device arrays, g has nx*ny*nz). */

@@ -645,6 +645,146 @@ int oracle_solve(oracle_handle* h, int method, int precond, double reduction, in
   });
 }
 
+// ---------------------------------------------------------------------------
+// Instationary: OneStepGridOperator over the operator J and the DG mass M
+// (R:dune/pdelab/gridoperator/onestep.hh, weights of gridoperator/onestep/localassembler.hh:122-126
+// in MultiplyOperator0ByDT mode: b dt on J, a on M) and OneStepMethod::apply
+// (R:dune/pdelab/instationary/implicitonestep.hh:144-230).
+
+// cell mass matrix (the same for every cell of the uniform grid), row-major
+static std::vector<double> cell_mass(oracle::Oracle& o) {
+  const size_t nl = o.nloc;
+  std::vector<double> m(nl * nl), e(nl), y(nl);
+  for (size_t j = 0; j < nl; ++j) {
+    std::fill(e.begin(), e.end(), 0.0);
+    std::fill(y.begin(), y.end(), 0.0);
+    e[j] = 1.0;
+    oracle::mass_volume(o.grid, o.tab, 0, e.data(), y.data());
+    for (size_t i = 0; i < nl; ++i) m[i * nl + j] = y[i];
+  }
+  return m;
+}
+
+// solves (b_dt J + a_mass M) z = r (OnTheFlyOperator over the one-step grid operator) with
+// point or block Jacobi of that operator
+static void onestep_solve(oracle_handle* h, int method, int precond, double a_mass, double b_dt, double reduction,
+                          int maxit, double* z, double* r, pdelab_linear_solver_result* res) {
+  auto& o = *h->o;
+  const size_t n = o.N, nloc = o.nloc, n2 = nloc * nloc;
+  const std::vector<double> Mc = cell_mass(o);
+  std::vector<double> tmp(n);
+  oracle::OpFn A = [&](const double* x, double* y) {
+    std::fill(y, y + n, 0.0);
+    o.jacobian_apply(x, y);
+    for (size_t i = 0; i < n; ++i) y[i] *= b_dt;
+    std::fill(tmp.begin(), tmp.end(), 0.0);
+    o.mass_apply(x, tmp.data());
+    for (size_t i = 0; i < n; ++i) y[i] += a_mass * tmp[i];
+  };
+  std::vector<double> diag;
+  std::vector<double> blu;
+  std::vector<int> piv;
+  if (precond == PDELAB_PREC_JACOBI) {
+    diag.assign(n, 0.0);
+    o.point_diagonal(diag.data());
+    for (size_t i = 0; i < n; ++i) diag[i] = b_dt * diag[i] + a_mass * Mc[(i % nloc) * nloc + (i % nloc)];
+  } else if (precond == PDELAB_PREC_BLOCK_JACOBI) {
+    blu.resize(o.grid.ncells * n2);
+    piv.resize(n);
+    o.block_diagonal(blu.data());
+    for (size_t c = 0; c < o.grid.ncells; ++c) {
+      double* B = &blu[c * n2];
+      for (size_t k = 0; k < n2; ++k) B[k] = b_dt * B[k] + a_mass * Mc[k];
+      lu_factor(nloc, B, &piv[c * nloc]);
+    }
+  }
+  oracle::OpFn P = [&](const double* d, double* v) {
+    if (precond == PDELAB_PREC_JACOBI)
+      for (size_t i = 0; i < n; ++i) v[i] = d[i] / diag[i];
+    else if (precond == PDELAB_PREC_BLOCK_JACOBI)
+      for (size_t c = 0; c < o.grid.ncells; ++c) lu_solve(nloc, &blu[c * n2], &piv[c * nloc], d + c * nloc, v + c * nloc);
+    else
+      std::copy(d, d + n, v);
+  };
+  if (maxit <= 0) maxit = 5000;
+  if (method == PDELAB_SOLVER_BICGSTAB) oracle::bicgstab(n, A, P, reduction, maxit, z, r, res);
+  else oracle::cg(n, A, P, reduction, maxit, z, r, res);
+}
+
+int oracle_onestep_jacobian_apply(oracle_handle* h, double a_mass, double b_dt, const double* z, double* y) {
+  return guard(h, [&] {
+    auto& o = *h->o;
+    std::vector<double> t(o.N, 0.0);
+    o.jacobian_apply(z, t.data());
+    for (size_t i = 0; i < o.N; ++i) y[i] += b_dt * t[i];
+    std::fill(t.begin(), t.end(), 0.0);
+    o.mass_apply(z, t.data());
+    for (size_t i = 0; i < o.N; ++i) y[i] += a_mass * t[i];
+  });
+}
+
+// coefficients a(r, i) = A[r-1][i], b(r, i) = B[r-1][i],
+// R:dune/pdelab/instationary/onestepparameter.hh (ImplicitEuler :193, OneStepTheta :96-101,
+// Alexander2 :449-461)
+int oracle_onestep_step(oracle_handle* h, int scheme, int method, int precond, double dt, double reduction, int maxit,
+                        const double* xold, double* xnew, pdelab_onestep_result* res) {
+  return guard(h, [&] {
+    auto& o = *h->o;
+    const size_t n = o.N;
+    int s = 1;
+    double A[2][3] = {{-1, 1, 0}, {0, 0, 0}}, B[2][3] = {{0, 1, 0}, {0, 0, 0}};
+    if (scheme == PDELAB_TS_CRANK_NICOLSON) {
+      B[0][0] = 0.5;
+      B[0][1] = 0.5;
+    } else if (scheme == PDELAB_TS_ALEXANDER2) {
+      const double alpha = 1.0 - 0.5 * std::sqrt(2.0);
+      s = 2;
+      A[1][0] = -1; A[1][1] = 0; A[1][2] = 1;
+      B[0][0] = 0; B[0][1] = alpha; B[0][2] = 0;
+      B[1][0] = 0; B[1][1] = 1 - alpha; B[1][2] = alpha;
+    } else if (scheme != PDELAB_TS_IMPLICIT_EULER) {
+      throw std::runtime_error("unknown one-step scheme");
+    }
+    std::memset(res, 0, sizeof(*res));
+    res->converged = 1;
+    res->stages = s;
+    std::vector<std::vector<double>> u(s + 1, std::vector<double>(n));
+    std::copy(xold, xold + n, u[0].begin());
+    std::vector<double> C(n), r(n), z(n), t(n);
+    // y += w R(v) + wm M v
+    auto add_terms = [&](const std::vector<double>& v, double w, double wm, std::vector<double>& y) {
+      if (w != 0.0) {
+        std::fill(t.begin(), t.end(), 0.0);
+        o.residual(v.data(), t.data());
+        for (size_t i = 0; i < n; ++i) y[i] += w * t[i];
+      }
+      if (wm != 0.0) {
+        std::fill(t.begin(), t.end(), 0.0);
+        o.mass_apply(v.data(), t.data());
+        for (size_t i = 0; i < n; ++i) y[i] += wm * t[i];
+      }
+    };
+    for (int rr = 1; rr <= s; ++rr) {
+      std::fill(C.begin(), C.end(), 0.0);
+      for (int j = 0; j < rr; ++j) add_terms(u[j], B[rr - 1][j] * dt, A[rr - 1][j], C);
+      if (rr == 1) std::copy(xnew, xnew + n, u[1].begin());
+      else u[rr] = u[rr - 1];
+      const double bdt = B[rr - 1][rr] * dt, am = A[rr - 1][rr];
+      r = C;
+      add_terms(u[rr], bdt, am, r);
+      const double defect = oracle::norm2(n, r.data());
+      const double red = std::max(reduction, 1e-99 / defect);
+      std::fill(z.begin(), z.end(), 0.0);
+      pdelab_linear_solver_result ls;
+      onestep_solve(h, method, precond, am, bdt, red, maxit, z.data(), r.data(), &ls);
+      res->linear_solver_iterations += (int)ls.iterations;
+      if (!ls.converged) res->converged = 0;
+      for (size_t i = 0; i < n; ++i) u[rr][i] -= z[i];
+    }
+    std::copy(u[s].begin(), u[s].end(), xnew);
+  });
+}
+
 // StationaryLinearProblemSolver::apply, R:dune/pdelab/stationary/linearproblem.hh:188-302
 int oracle_linear_problem_solve(oracle_handle* h, int method, int precond, double reduction, double min_defect,
                                 int maxit, double* x, pdelab_linear_problem_result* res, int matrix_based) {

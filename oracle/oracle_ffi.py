@@ -77,6 +77,10 @@ def lib():
         L.oracle_time_apply.restype = ctypes.c_double
         L.oracle_time_apply.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_int)]
         L.oracle_tables_1d.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
+        L.oracle_onestep_jacobian_apply.argtypes = [vp, ctypes.c_double, ctypes.c_double, _dp, _dp]
+        L.oracle_onestep_step.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_int, _dp, _dp,
+                                          ctypes.POINTER(cfgmod.OneStepResult)]
         _lib = L
     return _lib
 
@@ -183,6 +187,23 @@ class Oracle:
         d = np.zeros(self.N)
         self._check(lib().oracle_point_diagonal(self.h, _p(d)))
         return d
+
+    def onestep_jacobian_apply(self, a_mass, b_dt, z, y=None):
+        """y += b_dt J z + a_mass M z (returns y)."""
+        z = np.ascontiguousarray(z, dtype=np.float64)
+        y = np.zeros(self.N) if y is None else np.array(y, dtype=np.float64)
+        self._check(lib().oracle_onestep_jacobian_apply(self.h, a_mass, b_dt, _p(z), _p(y)))
+        return y
+
+    def onestep_step(self, xold, scheme, dt, xnew=None, method=cfgmod.SOLVER_CG, precond=cfgmod.PREC_JACOBI,
+                     reduction=1e-10, maxit=5000):
+        """One step of OneStepMethod::apply; returns (xnew, result dict)."""
+        xold = np.ascontiguousarray(xold, dtype=np.float64)
+        xnew = np.zeros(self.N) if xnew is None else np.array(xnew, dtype=np.float64)
+        res = cfgmod.OneStepResult()
+        self._check(lib().oracle_onestep_step(self.h, scheme, method, precond, dt, reduction, maxit, _p(xold),
+                                              _p(xnew), ctypes.byref(res)))
+        return xnew, res.as_dict()
 
     def block_diagonal(self):
         """Diagonal blocks of J, shape (ncells, nloc, nloc), row-major (BlockDiagonalLocalOperatorWrapper)."""

