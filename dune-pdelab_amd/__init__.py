@@ -81,6 +81,8 @@ def lib():
     L.pdelab_gpu_operator_apply.argtypes = [_vp, _vp, _vp]
     L.pdelab_gpu_block_diagonal.argtypes = [_vp, ctypes.c_int, _vp]
     L.pdelab_gpu_jacobian.argtypes = [_vp, ctypes.c_int, _vp, _vp]
+    L.pdelab_gpu_l2_error_squared.argtypes = [_vp, _vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    L.pdelab_gpu_sample.argtypes = [_vp, _vp, ctypes.c_int, _vp]
     L.pdelab_gpu_onestep_jacobian_apply.argtypes = [_vp, ctypes.c_double, ctypes.c_double, _vp, _vp]
     L.pdelab_gpu_onestep_step.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                           ctypes.c_double, ctypes.c_int, _vp, _vp,
@@ -272,6 +274,23 @@ class GridOperator:
     def block_matrix_apply(self, blocks, cols, z, y):
         """y = A z for a block matrix from jacobian()."""
         self._check(lib().pdelab_gpu_block_matrix_apply(self.h, blocks.p, cols.p, z.p, y.p))
+
+    # -- output
+    def l2_error_squared(self, x, order=10):
+        """integral of (u_h - u)^2 for the built-in problem's exact solution u."""
+        out = ctypes.c_double()
+        self._check(lib().pdelab_gpu_l2_error_squared(self.h, x.p, order, ctypes.byref(out)))
+        return out.value
+
+    def sample(self, x, intervals):
+        """u_h at the (intervals+1)^3 equidistant local points of every cell, shape (ncells, L+1, L+1, L+1)
+        indexed [cell, z, y, x]."""
+        L1 = intervals + 1
+        v = DeviceVector(self, self.ncells * L1 ** 3)
+        self._check(lib().pdelab_gpu_sample(self.h, x.p, intervals, v.p))
+        out = v.download().reshape(self.ncells, L1, L1, L1)
+        v.free()
+        return out
 
     # -- instationary
     def onestep_jacobian_apply(self, a_mass, b_dt, z, y):

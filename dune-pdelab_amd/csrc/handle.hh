@@ -13,6 +13,7 @@
 #include "dg_kernels.hh"
 #include "dg_load.hh"
 #include "dg_tables.hh"
+#include "output.hh"
 
 using namespace pdelab_amd;
 

@@ -1361,6 +1361,85 @@ int pdelab_gpu_solve_mixed(pdelab_gpu_gridoperator* h, int method, int precond, 
   return PDELAB_OK;
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Output: values at tensor sets of local points, and the L2 error integral.
+static PointSet point_set(const H* h, int m, const long double* xi, const long double* w) {
+  PointSet ps;
+  std::memset(&ps, 0, sizeof(ps));
+  ps.m = m;
+  const int k = h->cfg.degree;
+  for (int q = 0; q < m; ++q) {
+    ps.xi[q] = (double)xi[q];
+    ps.w[q] = w ? (double)w[q] : 0.0;
+    for (int a = 0; a <= k; ++a) {  // QkDG Lagrange basis, nodes j/k
+      long double r = 1;
+      for (int j = 0; j <= k; ++j)
+        if (j != a) r *= ((long double)k * xi[q] - j) / (long double)(a - j);
+      ps.P[q][a] = (double)r;
+    }
+  }
+  return ps;
+}
+
+static CellGeom cell_geom(const H* h) {
+  CellGeom g;
+  g.n[0] = h->cfg.nx; g.n[1] = h->cfg.ny; g.n[2] = h->cfg.nz;
+  g.h[0] = h->cfg.lx / h->cfg.nx; g.h[1] = h->cfg.ly / h->cfg.ny; g.h[2] = h->cfg.lz / h->cfg.nz;
+  g.ncells = h->ncells;
+  return g;
+}
+
+template <int MODE>
+static int launch_points(H* h, const double* x, const PointSet& ps, double* out) {
+  const dim3 grid((unsigned)h->ncells), block(256);
+  const CellGeom g = cell_geom(h);
+  switch (h->n1) {
+    case 2: hipLaunchKernelGGL((cell_points_kernel<2, MODE>), grid, block, 0, h->stream, x, ps, g, h->la, out); break;
+    case 3: hipLaunchKernelGGL((cell_points_kernel<3, MODE>), grid, block, 0, h->stream, x, ps, g, h->la, out); break;
+    default: hipLaunchKernelGGL((cell_points_kernel<4, MODE>), grid, block, 0, h->stream, x, ps, g, h->la, out); break;
+  }
+  HIPCHK(h, hipGetLastError());
+  return PDELAB_OK;
+}
+
+extern "C" {
+
+int pdelab_gpu_l2_error_squared(pdelab_gpu_gridoperator* h, const double* x, int order, double* err) {
+  if (!h || !x || !err || order < 0) return PDELAB_ERR_INVALID_ARG;
+  const int m = order / 2 + 1;
+  if (m > kMaxPts) return fail(h, PDELAB_ERR_INVALID_ARG, "l2 error: quadrature order above 23");
+  if (h->cfg.problem != PDELAB_PROBLEM_EXP && h->cfg.problem != PDELAB_PROBLEM_QUADRATIC)
+    return fail(h, PDELAB_ERR_UNSUPPORTED, "l2 error: the problem has no built-in exact solution");
+  if (h->ncells >= (1LL << 31)) return fail(h, PDELAB_ERR_UNSUPPORTED, "l2 error: more than 2^31 - 1 cells");
+  long double xi[kMaxPts], w[kMaxPts];
+  gauss01(m, xi, w);
+  double* part = nullptr;
+  HIPCHK(h, hipMallocAsync((void**)&part, sizeof(double) * (h->ncells + 1), h->stream));
+  int st = launch_points<1>(h, x, point_set(h, m, xi, w), part);
+  if (st == PDELAB_OK) {
+    hipLaunchKernelGGL((final_reduce_kernel<double, 1>), dim3(1), dim3(kFinalThreads), 0, h->stream,
+                       (const double*)part, (int)h->ncells, part + h->ncells, (const int*)nullptr);
+    double v = 0;
+    if (hipMemcpyAsync(&v, part + h->ncells, sizeof(double), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+      st = fail(h, PDELAB_ERR_HIP, "l2 error: copy of the result failed");
+    *err = v;
+  }
+  (void)hipFreeAsync(part, h->stream);
+  return st;
+}
+
+int pdelab_gpu_sample(pdelab_gpu_gridoperator* h, const double* x, int intervals, double* values) {
+  if (!h || !x || !values || intervals < 1) return PDELAB_ERR_INVALID_ARG;
+  if (intervals + 1 > kMaxPts) return fail(h, PDELAB_ERR_INVALID_ARG, "sample: at most 11 intervals");
+  if (h->ncells >= (1LL << 31)) return fail(h, PDELAB_ERR_UNSUPPORTED, "sample: more than 2^31 - 1 cells");
+  long double xi[kMaxPts];
+  for (int q = 0; q <= intervals; ++q) xi[q] = (long double)q / intervals;
+  return launch_points<0>(h, x, point_set(h, intervals + 1, xi, nullptr), values);
+}
+
 // ---------------------------------------------------------------------------
 // One-step methods. A stage operator a M + b J equals b (J + (a/b) M), and J + s M is J with
 // the reaction coefficient c + s: the reaction term c u v is the DG mass on the same

@@ -41,6 +41,10 @@
  *                                    OneStepGridOperator / OneStepMethod::apply
  *                                    R:dune/pdelab/gridoperator/onestep.hh,
  *                                    R:dune/pdelab/instationary/implicitonestep.hh:144-230
+ *   pdelab_gpu_l2_error_squared      integrateGridFunction(DifferenceSquaredAdapter(u, u_h))
+ *                                    R:dune/pdelab/common/functionutilities.hh:51-83
+ *   pdelab_gpu_sample                DiscreteGridFunction values at the SubsamplingVTKWriter
+ *                                    vertices, R:dune/pdelab/common/vtk.hh:655-690
  *   pdelab_gpu_mass_apply            ScalarL2::jacobian_apply_volume
  *                                    R:dune/pdelab/localoperator/l2.hh:45-90
  *
@@ -318,6 +322,18 @@ int pdelab_gpu_onestep_jacobian_apply(pdelab_gpu_gridoperator* h, double a_mass,
 int pdelab_gpu_onestep_step(pdelab_gpu_gridoperator* h, int scheme, int method, int precond, double dt,
                             double reduction, int maxit, const double* xold, double* xnew,
                             pdelab_onestep_result* res);
+
+/* ---- output and error integration ----------------------------------------- */
+/* *err = integral over the grid of (u_h - u)^2, u the exact solution of the built-in problem
+   (PDELAB_PROBLEM_EXP or _QUADRATIC), with the tensor Gauss rule of the given order (at most
+   23): integrateGridFunction(DifferenceSquaredAdapter(u, u_h), order),
+   R:dune/pdelab/common/functionutilities.hh:51-83, as testconvectiondiffusiondg.cc:160-166 uses it. */
+int pdelab_gpu_l2_error_squared(pdelab_gpu_gridoperator* h, const double* x, int order, double* err);
+/* values[c (L+1)^3 + i + (L+1) j + (L+1)^2 l] = u_h(i/L, j/L, l/L) in cell c, L = intervals >= 1
+   (at most 11): the DiscreteGridFunction values (gridfunctionspaceutilities.hh:54) at the
+   vertices of the refined cells that SubsamplingVTKWriter with refinementIntervals(L) writes
+   (R:dune/pdelab/common/vtk.hh:655-690). values is a device array of ncells (L+1)^3 doubles. */
+int pdelab_gpu_sample(pdelab_gpu_gridoperator* h, const double* x, int intervals, double* values);
 
 /* ---- adjoint gradients ---------------------------------------------------- */
 /* g[c] = -lambda^T (dJ/dlog kappa_c) u for every cell (device arrays, g has nx*ny*nz). */
