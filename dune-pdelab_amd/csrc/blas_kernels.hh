@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include "block_reduce.hh"
+#include "stream_io.hh"
 
 namespace pdelab_amd {
 
@@ -187,9 +188,9 @@ __global__ __launch_bounds__(kRedThreads) void cg_rupdate_kernel(long long n, T*
       const long long i = i0 + u * S;
       rv[u] = qv[u] = T(0);
       dv[u] = T(1);
-      if (i < n) {
-        rv[u] = r[i];
-        qv[u] = q[i];
+      if (i < n) {  // r and q stream in; r is stored cached for the x/p update that follows
+        rv[u] = ld_stream<T>(r + i);
+        qv[u] = ld_stream<T>(q + i);
         if (dinv) dv[u] = dinv[i];
       }
     }
@@ -262,16 +263,18 @@ __global__ __launch_bounds__(256) void cg_xpupdate_kernel(long long n, T* __rest
   const bool pupd = state[0] == 0;
   const T lambda = (T)(scal[slot_rho_old] / scal[0]);
   const T beta = pupd ? (T)(scal[slot_rho_new] / scal[slot_rho_old]) : T(0);
+  // x and z (or r) stream (non-temporal); p, reread by the next apply, stays cached
   if constexpr (E == 2) {
     if (i0 + 1 < n) {  // two contiguous floats as one 8-byte access per stream
       typedef T v2 __attribute__((ext_vector_type(2)));
       const v2 pv = *reinterpret_cast<const v2*>(p + i0);
-      *reinterpret_cast<v2*>(x + i0) = *reinterpret_cast<const v2*>(x + i0) + lambda * pv;
+      v2* xv = reinterpret_cast<v2*>(x + i0);
+      st_stream<v2>(ld_stream<v2>(xv) + lambda * pv, xv);
       if (pupd) {
         v2 zv;
-        if (z) zv = *reinterpret_cast<const v2*>(z + i0);
-        else if (dinv) zv = *reinterpret_cast<const v2*>(dinv + i0) * *reinterpret_cast<const v2*>(r + i0);
-        else zv = *reinterpret_cast<const v2*>(r + i0);
+        if (z) zv = ld_stream<v2>(reinterpret_cast<const v2*>(z + i0));
+        else if (dinv) zv = *reinterpret_cast<const v2*>(dinv + i0) * ld_stream<v2>(reinterpret_cast<const v2*>(r + i0));
+        else zv = ld_stream<v2>(reinterpret_cast<const v2*>(r + i0));
         *reinterpret_cast<v2*>(p + i0) = zv + beta * pv;
       }
       return;
@@ -282,9 +285,9 @@ __global__ __launch_bounds__(256) void cg_xpupdate_kernel(long long n, T* __rest
     const long long i = i0 + e;
     if (i < n) {
       const T pv = p[i];
-      x[i] += lambda * pv;
+      st_stream<T>(ld_stream<T>(x + i) + lambda * pv, x + i);
       if (pupd) {
-        const T zv = z ? z[i] : (dinv ? dinv[i] * r[i] : r[i]);
+        const T zv = z ? ld_stream<T>(z + i) : (dinv ? dinv[i] * ld_stream<T>(r + i) : ld_stream<T>(r + i));
         p[i] = zv + beta * pv;
       }
     }

@@ -23,6 +23,7 @@
 #define PDELAB_BLOCK_JACOBI_HH
 
 #include "block_reduce.hh"
+#include "stream_io.hh"
 #include "dg_kernels.hh"
 
 namespace pdelab_amd {
@@ -497,7 +498,10 @@ __global__ __launch_bounds__(BJShape<N1>::THREADS) void bj_dense_kernel(const BJ
 // Fast diagonalisation: lane (ix, iy) of a cell holds the z-line u(ix, iy, 0..N1-1). The
 // x and y sweeps exchange lines through the wave's LDS region; the z sweeps stay in
 // registers. D^{-1} = (Vz (x) Vy (x) Vx) diag(1/(lx + ly + lz)) (Vz^T (x) Vy^T (x) Vx^T).
-template <int N1, typename T, typename A, int MODE, bool PER_CELL>
+// ZU (boundary-type table only, nx ny a multiple of CPW): every cell of a wave lies in one
+// z-plane, so the z entries (V_z, lam_z) are wave-uniform and are read from the table in
+// global memory by scalar loads instead of being held per lane (40 fewer VGPRs in FP64).
+template <int N1, typename T, typename A, int MODE, bool PER_CELL, bool ZU = false>
 __global__ __launch_bounds__(BJFdmShape<N1>::THREADS) void bj_fdm_kernel(const BJVec<T, A> v,
                                                                         const T* __restrict__ data, const BJGeom g) {
   using S = BJFdmShape<N1>;
@@ -543,13 +547,15 @@ __global__ __launch_bounds__(BJFdmShape<N1>::THREADS) void bj_fdm_kernel(const B
     u[k] = T(0);
     if (act) {
       const long long idx = base + LPC * k;
-      T x = v.in[idx];
       if (MODE == BJ_CGR) {
-        x -= lambda * v.q[idx];
-        v.r[idx] = x;
+        // r (reread next iteration) and q (dead after this step) stream past the caches
+        const T x = ld_stream<T>(v.in + idx) - lambda * ld_stream<T>(v.q + idx);
+        st_stream<T>(x, v.r + idx);
         acc0 += (A)x * (A)x;
+        u[k] = x;
+      } else {
+        u[k] = v.in[idx];
       }
-      u[k] = x;
     }
   }
   // this lane's coefficients (LDS): the cell's staged entries or its boundary types' entries
@@ -566,15 +572,23 @@ __global__ __launch_bounds__(BJFdmShape<N1>::THREADS) void bj_fdm_kernel(const B
     for (int d = 0; d < 3; ++d) E[d] = ent[0] + (d * 4 + bj_type(co[d], g.n[d])) * ES;
   }
   T cx[N1], rx[N1], cy[N1], ry[N1], Vz[N1][N1], lz[N1];
+  const T* Ez = E[2];
+  if constexpr (ZU && !PER_CELL) {
+    const unsigned w0 = (unsigned)__builtin_amdgcn_readfirstlane(tid >> 6);
+    unsigned c0 = (unsigned)blockIdx.x * S::CPG + w0 * CPW;
+    if (c0 >= (unsigned)g.ncells) c0 = (unsigned)g.ncells - 1;
+    const unsigned kz = c0 / ((unsigned)g.n[0] * (unsigned)g.n[1]);
+    Ez = data + (2 * 4 + bj_type((int)kz, g.n[2])) * ES;
+  }
 #pragma unroll
   for (int m = 0; m < N1; ++m) {
     cx[m] = E[0][m * N1 + ix];
     rx[m] = E[0][ix * N1 + m];
     cy[m] = E[1][m * N1 + iy];
     ry[m] = E[1][iy * N1 + m];
-    lz[m] = E[2][N1 * N1 + m];
+    lz[m] = Ez[N1 * N1 + m];
 #pragma unroll
-    for (int k = 0; k < N1; ++k) Vz[m][k] = E[2][m * N1 + k];
+    for (int k = 0; k < N1; ++k) Vz[m][k] = Ez[m * N1 + k];
   }
   const T lxy = E[0][N1 * N1 + ix] + E[1][N1 * N1 + iy];
   const int xo = cs * LPC;  // the cell's lines: value k of line (ix, iy) at X[w][k][xo + ix + N1 iy]
@@ -649,8 +663,10 @@ __global__ __launch_bounds__(BJFdmShape<N1>::THREADS) void bj_fdm_kernel(const B
     }
   }
   if (MODE == BJ_PREC || MODE == BJ_CGR) {
-    // out.in (for BJ_CGR out.r with the updated r); idle lanes hold zeros
-    const A d = dot;
+    // out.in (for BJ_CGR out.r with the updated r) from the active lanes only: a lane
+    // beyond the wave's cells (lane 63 at Q2) aliases cell 0 in the sweeps and holds its
+    // transformed values, which must not be counted twice
+    const A d = act ? dot : A(0);
     // one partial (BJ_PREC) or pair (BJ_CGR) per workgroup, reduced through X
     if (MODE == BJ_PREC) {
       A r[1] = {d};

@@ -110,6 +110,9 @@ int launch_n(H* h, const BlockJacobiData& b, const T* data, const BJVec<T, A>& v
   if (b.fdm && b.per_cell)
     hipLaunchKernelGGL((bj_fdm_kernel<N1, T, A, MODE, true>), dim3(nb), dim3(BJFdmShape<N1>::THREADS), 0, h->stream, v,
                        data, g);
+  else if (b.fdm && ((long long)g.n[0] * g.n[1]) % BJFdmShape<N1>::CPW == 0)
+    hipLaunchKernelGGL((bj_fdm_kernel<N1, T, A, MODE, false, true>), dim3(nb), dim3(BJFdmShape<N1>::THREADS), 0,
+                       h->stream, v, data, g);
   else if (b.fdm)
     hipLaunchKernelGGL((bj_fdm_kernel<N1, T, A, MODE, false>), dim3(nb), dim3(BJFdmShape<N1>::THREADS), 0, h->stream, v,
                        data, g);

@@ -548,13 +548,23 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
 // array in the rotated, conflict-free cell layout of lds_index; each lane reads its
 // twelve slot offsets once from SlotTable, so every LDS access is a register plus an
 // immediate offset. The z-neighbour lines stay in registers.
+//
+// FP64 Q3 stores the cells unpadded (64 doubles) and separates neighbouring cells' banks
+// by their layout parity instead: cell lc (lc = -1 .. CPB, in every region) keeps slot s
+// at s ^ (16 (lc & 1)), which moves odd cells by half of the 64 four-byte banks, as the
+// padded stride of 80 doubles does. A 64-lane b64 access is served in two passes of 32
+// lanes (two cells of opposite parity), so the banks of each pass are those of the padded
+// layout, and the 25,600 B of LDS fit six workgroups per CU instead of five.
 template <int N1, typename T, bool TRANS, bool CONV, bool REACT, bool VARK, bool ACCUM, bool DOT>
 __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kernel(
     const T* __restrict__ z, T* __restrict__ y, const T* __restrict__ fc, const DGArgs<T> a,
     const LineMats<T>* __restrict__ lm, long long nblocks, T* __restrict__ dotp, const int* __restrict__ stop) {
   if (stop && *stop) return;  // a device-side Krylov loop has already stopped
   using SH = ApplyShape<N1>;
-  constexpr int L = SH::L, NL = SH::NL, NLP = SH::NLP, CPB = SH::CPB;
+  constexpr int L = SH::L, NL = SH::NL, CPB = SH::CPB;
+  // parity-swizzled unpadded cells (FP64 Q3) or the padded stride
+  constexpr bool XS = sizeof(T) == 8 && N1 == 4;
+  constexpr int NLP = XS ? NL : SH::NLP;
   // cell before the chunk, the CPB own cells, cell after the chunk (from UO - NLP on);
   // the chunks of the rows y-1 (YMO) and y+1 (YPO), later the partial sums and sweeps
   constexpr int UO = NLP, YMO = (CPB + 2) * NLP, YPO = (2 * CPB + 2) * NLP;
@@ -604,15 +614,17 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
     }
   }
   // slot of element e of the lane's x-, y- and z-line, relative to the cell region
+  // (an x-neighbour, of the other parity, keeps the same element at the own slot ^ XP)
+  constexpr int XP = XS ? 16 : 0;
   int ax[N1], ay[N1], az[N1];
   {
     const SlotTable<N1>& st = slot_table<N1>();
-    const int cb = lc * NLP;
+    const int cb = lc * NLP, px = (lc & 1) * XP;
 #pragma unroll
     for (int e = 0; e < N1; ++e) {
-      ax[e] = cb + st.s[0][l][e];
-      ay[e] = cb + st.s[1][l][e];
-      az[e] = cb + st.s[2][l][e];
+      ax[e] = cb + (st.s[0][l][e] ^ px);
+      ay[e] = cb + (st.s[1][l][e] ^ px);
+      az[e] = cb + (st.s[2][l][e] ^ px);
     }
   }
   {
@@ -622,7 +634,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
       LS[UO + az[e]] = uz[e];
       if (lo[1]) LS[YMO + az[e]] = yl[e];
       if (hi[1]) LS[YPO + az[e]] = yr[e];
-      if (ld_xe) LS[xo + az[e]] = xe[e];
+      if (ld_xe) LS[xo + (az[e] ^ XP)] = xe[e];
     }
   }
   __syncthreads();
@@ -641,8 +653,9 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
 #pragma unroll
     for (int e = 0; e < N1; ++e) {
       u[e] = LS[UO + ax[e]];
-      ul[e] = lo[0] ? LS[UO - NLP + ax[e]] : T(0);
-      ur[e] = hi[0] ? LS[UO + NLP + ax[e]] : T(0);
+      const int xn = ax[e] ^ XP;
+      ul[e] = lo[0] ? LS[UO - NLP + xn] : T(0);
+      ur[e] = hi[0] ? LS[UO + NLP + xn] : T(0);
     }
     line_apply<N1, 0, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, lo[0], hi[0], kc, flo[0], fhi[0], sx);
   }

@@ -69,6 +69,31 @@ def test_block_jacobi_cg(pkg, of, seed):
     assert it < 0.85 * int(rp["solver"]["iterations"])
 
 
+# nx ny a multiple of the cells per wave of the block-Jacobi kernel (4 Q3, 7 Q2, 16 Q1)
+# takes the kernel whose z entries are wave-uniform; the other grids the general one
+GRIDS = [("q3_5x5x3_general", (5, 5, 3), 3), ("q3_4x3x4_uniform_z", (4, 3, 4), 3),
+         ("q2_7x3x2_uniform_z", (7, 3, 2), 2), ("q2_4x4x3_general", (4, 4, 3), 2),
+         ("q1_8x2x3_uniform_z", (8, 2, 3), 1), ("q1_5x3x4_general", (5, 3, 4), 1)]
+
+
+@pytest.mark.parametrize("name,dims,k", GRIDS, ids=[g[0] for g in GRIDS])
+def test_block_jacobi_cg_grids(pkg, of, name, dims, k):
+    say(f"[gpu] block jacobi CG {name}")
+    kw = dict(nx=dims[0], ny=dims[1], nz=dims[2], degree=k, alpha=3.0, problem=pkg.PROBLEM_EXP)
+    cfg, kappa, go, o = make(pkg, of, kw, None)
+    xd = go.vector()
+    res = go.linear_problem_solve(xd, method=0, precond=BJ, reduction=1e-10)
+    assert res["solver"]["converged"]
+    x_ref, res_ref = o.linear_problem_solve(method=0, precond=BJ, reduction=1e-10)
+    assert abs(int(res["solver"]["iterations"]) - int(res_ref["solver"]["iterations"])) <= 2
+    assert rel(xd.download(), x_ref) <= 1e-7
+    # the FP32 kernels of the same grid, inside the mixed-precision refinement
+    xm = go.vector()
+    rm = go.solve_mixed(xm, precond=BJ, inner_reduction=0.1, outer_reduction=1e-10)
+    assert rm["converged"]
+    assert rel(xm.download(), x_ref) <= 1e-7
+
+
 def test_block_jacobi_bicgstab_convection(pkg, of):
     say("[gpu] block jacobi BiCGSTAB 6^3 Q2 convection")
     kw = dict(n=6, degree=2, alpha=3.0, kappa0=0.01, b=(1.0, 0.5, 0.25), problem=pkg.PROBLEM_CONSTANT, f0=1.0)
