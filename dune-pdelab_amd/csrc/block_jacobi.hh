@@ -571,7 +571,9 @@ __global__ __launch_bounds__(BJFdmShape<N1>::THREADS) void bj_fdm_kernel(const B
 #pragma unroll
     for (int d = 0; d < 3; ++d) E[d] = ent[0] + (d * 4 + bj_type(co[d], g.n[d])) * ES;
   }
-  T cx[N1], rx[N1], cy[N1], ry[N1], Vz[N1][N1], lz[N1];
+  // each coefficient set is read after the wave fence that precedes its sweep (a fence
+  // keeps the reads from being hoisted), so only one set is live at a time
+  T cf[N1], Vz[N1][N1], lz[N1];
   const T* Ez = E[2];
   if constexpr (ZU && !PER_CELL) {
     const unsigned w0 = (unsigned)__builtin_amdgcn_readfirstlane(tid >> 6);
@@ -580,17 +582,6 @@ __global__ __launch_bounds__(BJFdmShape<N1>::THREADS) void bj_fdm_kernel(const B
     const unsigned kz = c0 / ((unsigned)g.n[0] * (unsigned)g.n[1]);
     Ez = data + (2 * 4 + bj_type((int)kz, g.n[2])) * ES;
   }
-#pragma unroll
-  for (int m = 0; m < N1; ++m) {
-    cx[m] = E[0][m * N1 + ix];
-    rx[m] = E[0][ix * N1 + m];
-    cy[m] = E[1][m * N1 + iy];
-    ry[m] = E[1][iy * N1 + m];
-    lz[m] = Ez[N1 * N1 + m];
-#pragma unroll
-    for (int k = 0; k < N1; ++k) Vz[m][k] = Ez[m * N1 + k];
-  }
-  const T lxy = E[0][N1 * N1 + ix] + E[1][N1 * N1 + iy];
   const int xo = cs * LPC;  // the cell's lines: value k of line (ix, iy) at X[w][k][xo + ix + N1 iy]
   T t[N1];
   // forward x: t = sum_m Vx[m][ix] u(m, iy, .)
@@ -598,22 +589,33 @@ __global__ __launch_bounds__(BJFdmShape<N1>::THREADS) void bj_fdm_kernel(const B
   for (int k = 0; k < N1; ++k) X[w][k][lw] = u[k];
   wave_lds_sync();
 #pragma unroll
+  for (int m = 0; m < N1; ++m) cf[m] = E[0][m * N1 + ix];
+#pragma unroll
   for (int k = 0; k < N1; ++k) t[k] = T(0);
 #pragma unroll
   for (int m = 0; m < N1; ++m)
 #pragma unroll
-    for (int k = 0; k < N1; ++k) t[k] += cx[m] * X[w][k][xo + m + N1 * iy];
+    for (int k = 0; k < N1; ++k) t[k] += cf[m] * X[w][k][xo + m + N1 * iy];
   wave_lds_sync();
   // forward y
 #pragma unroll
   for (int k = 0; k < N1; ++k) X[w][k][lw] = t[k];
   wave_lds_sync();
 #pragma unroll
+  for (int m = 0; m < N1; ++m) cf[m] = E[1][m * N1 + iy];
+#pragma unroll
   for (int k = 0; k < N1; ++k) u[k] = T(0);
 #pragma unroll
   for (int m = 0; m < N1; ++m)
 #pragma unroll
-    for (int k = 0; k < N1; ++k) u[k] += cy[m] * X[w][k][xo + ix + N1 * m];
+    for (int k = 0; k < N1; ++k) u[k] += cf[m] * X[w][k][xo + ix + N1 * m];
+#pragma unroll
+  for (int m = 0; m < N1; ++m) {
+    lz[m] = Ez[N1 * N1 + m];
+#pragma unroll
+    for (int k = 0; k < N1; ++k) Vz[m][k] = Ez[m * N1 + k];
+  }
+  const T lxy = E[0][N1 * N1 + ix] + E[1][N1 * N1 + iy];
   // forward z, scaling, backward z (registers); with w = V^T r, r.(D^{-1} r) is the
   // sum of w^2 / lam over the transformed values
   A dot = A(0);
@@ -638,22 +640,26 @@ __global__ __launch_bounds__(BJFdmShape<N1>::THREADS) void bj_fdm_kernel(const B
   for (int k = 0; k < N1; ++k) X[w][k][lw] = u[k];
   wave_lds_sync();
 #pragma unroll
+  for (int m = 0; m < N1; ++m) cf[m] = E[1][iy * N1 + m];
+#pragma unroll
   for (int k = 0; k < N1; ++k) t[k] = T(0);
 #pragma unroll
   for (int m = 0; m < N1; ++m)
 #pragma unroll
-    for (int k = 0; k < N1; ++k) t[k] += ry[m] * X[w][k][xo + ix + N1 * m];
+    for (int k = 0; k < N1; ++k) t[k] += cf[m] * X[w][k][xo + ix + N1 * m];
   wave_lds_sync();
   // backward x
 #pragma unroll
   for (int k = 0; k < N1; ++k) X[w][k][lw] = t[k];
   wave_lds_sync();
 #pragma unroll
+  for (int m = 0; m < N1; ++m) cf[m] = E[0][ix * N1 + m];
+#pragma unroll
   for (int k = 0; k < N1; ++k) u[k] = T(0);
 #pragma unroll
   for (int m = 0; m < N1; ++m)
 #pragma unroll
-    for (int k = 0; k < N1; ++k) u[k] += rx[m] * X[w][k][xo + m + N1 * iy];
+    for (int k = 0; k < N1; ++k) u[k] += cf[m] * X[w][k][xo + m + N1 * iy];
   if (act) {
 #pragma unroll
     for (int k = 0; k < N1; ++k) {
