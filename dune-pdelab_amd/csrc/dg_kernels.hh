@@ -601,17 +601,19 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
   if (lo[1]) load_line<N1, 2, T>(zc - a.stride[1] * NL, off, p, q, yl);
   if (hi[1]) load_line<N1, 2, T>(zc + a.stride[1] * NL, off, p, q, yr);
   if (ld_xe) load_line<N1, 2, T>(zc + (first ? -NL : NL), off, p, q, xe);
-  if (VARK) {
-    const T* fb = fc + (c0 + lc) * FC;
-    kc = fb[0];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      if (lo[d]) flo[d] = FaceCoef<T>{fb[1 + 3 * d], fb[2 + 3 * d], fb[3 + 3 * d]};
-      if (hi[d]) {
-        const T* fn = fb + a.stride[d] * FC;
-        fhi[d] = FaceCoef<T>{fn[1 + 3 * d], fn[2 + 3 * d], fn[3 + 3 * d]};
-      }
+  // cell-wise kappa: the cell's kappa and the z faces now, the y and x faces after the
+  // staging barrier (fewer registers live across the loads and the staging)
+  const T* fb = fc + (c0 + lc) * FC;
+  auto load_faces = [&](int d) {
+    if (lo[d]) flo[d] = FaceCoef<T>{fb[1 + 3 * d], fb[2 + 3 * d], fb[3 + 3 * d]};
+    if (hi[d]) {
+      const T* fn = fb + a.stride[d] * FC;
+      fhi[d] = FaceCoef<T>{fn[1 + 3 * d], fn[2 + 3 * d], fn[3 + 3 * d]};
     }
+  };
+  if (VARK) {
+    kc = fb[0];
+    load_faces(2);
   }
   // slot of element e of the lane's x-, y- and z-line, relative to the cell region
   // (an x-neighbour, of the other parity, keeps the same element at the own slot ^ XP)
@@ -638,6 +640,10 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
     }
   }
   __syncthreads();
+  if (VARK) {
+    load_faces(1);
+    load_faces(0);
+  }
   T sx[N1], sy[N1], sz[N1];
   {
     line_apply<N1, 2, T, TRANS, CONV, REACT, VARK>(a, lm, uz, zl, zr, lo[2], hi[2], kc, flo[2], fhi[2], sz);
