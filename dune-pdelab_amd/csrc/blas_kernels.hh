@@ -166,19 +166,36 @@ __global__ __launch_bounds__(kRedThreads) void prec_dot_kernel(long long n, cons
 
 // Device-side CG iteration (dune-istl CGSolver::apply, R:dune/istl/solvers.hh), five
 // launches per iteration: the apply with the fused p.q partials, their reduction into
-// scal[0], cg_rupdate_kernel, cg_check_kernel and cg_xpupdate_kernel.
+// scal[0] (cg_pq_kernel), cg_rupdate_kernel, cg_check_kernel and cg_xpupdate_kernel.
 // state = {stopped, aborted, stop iteration}; dstate = {defect, def0, reduction}. Every
 // kernel of an iteration after the one that stopped the loop returns at once.
 constexpr int kVecUnroll = 4;
+// Scalar slots of the step lengths: the single-workgroup kernels divide once, and the
+// vector kernels read the quotients (an FP64 division per wave cost ~10 us per launch).
+constexpr int kSlotLambda = 8, kSlotBeta = 9;
 
-// r -= lambda q with lambda = scal[rho] / scal[0]; partials of r.r and (dinv r).r
+// scal[0] = p.q from the apply's partials, scal[kSlotLambda] = lambda = scal[slot_rho] / p.q
+template <typename A>
+__global__ __launch_bounds__(kFinalThreads) void cg_pq_kernel(const A* __restrict__ partial, int nb,
+                                                            A* __restrict__ scal, int slot_rho,
+                                                            const int* __restrict__ stop) {
+  if (*stop) return;
+  A v[1];
+  strided_partial_sum<A, 1>(partial, nb, v);
+  if (threadIdx.x == 0) {
+    scal[0] = v[0];
+    scal[kSlotLambda] = scal[slot_rho] / v[0];
+  }
+}
+
+// r -= lambda q with lambda = scal[kSlotLambda]; partials of r.r and (dinv r).r
 template <typename T, typename A>
 __global__ __launch_bounds__(kRedThreads) void cg_rupdate_kernel(long long n, T* __restrict__ r, const T* __restrict__ q,
                                                                  const T* __restrict__ dinv, const A* __restrict__ scal,
-                                                                 int slot_rho, A* __restrict__ partial,
+                                                                 A* __restrict__ partial,
                                                                  const int* __restrict__ state) {
   if (state[0]) return;
-  const T lambda = (T)(scal[slot_rho] / scal[0]);
+  const T lambda = (T)scal[kSlotLambda];
   A v[2] = {A(0), A(0)};
   const long long S = (long long)gridDim.x * kRedThreads;
   for (long long i0 = (long long)blockIdx.x * kRedThreads + threadIdx.x; i0 < n; i0 += kVecUnroll * S) {
@@ -214,19 +231,22 @@ __global__ __launch_bounds__(kRedThreads) void cg_rupdate_kernel(long long n, T*
 }
 
 // IterativeSolver::Iteration::step on the device (single workgroup): reduces the
-// partials of (r.r, z.r) into out[0..1], records the defect and the iteration number of
-// a running loop and raises the stop flag on convergence (def < reduction*def0 or
-// def < 1e-30) or on a non-finite defect.
+// partials of (r.r, z.r) into out[0..1], writes beta = z.r / rho_old (when beta is not
+// null), records the defect and the iteration number of a running loop and raises the
+// stop flag on convergence (def < reduction*def0 or def < 1e-30) or on a non-finite defect.
 template <typename A>
 __global__ __launch_bounds__(kFinalThreads) void cg_check_kernel(const A* __restrict__ partial, int nb,
                                                                A* __restrict__ out, int* __restrict__ state,
-                                                               double* __restrict__ dstate, int it) {
+                                                               double* __restrict__ dstate, int it,
+                                                               const A* __restrict__ rho_old = nullptr,
+                                                               A* __restrict__ beta = nullptr) {
   if (state[0]) return;
   A v[2];
   strided_partial_sum<A, 2>(partial, nb, v);
   if (threadIdx.x == 0) {
     out[0] = v[0];
     out[1] = v[1];
+    if (beta) *beta = v[1] / *rho_old;
     const double def = sqrt((double)v[0]);
     state[2] = it;
     dstate[0] = def;
@@ -239,8 +259,8 @@ __global__ __launch_bounds__(kFinalThreads) void cg_check_kernel(const A* __rest
   }
 }
 
-// x += lambda p (lambda = scal[rho_old] / scal[0]), then p = z + beta p with
-// beta = scal[rho_new] / scal[rho_old] and z = W^{-1} r: the given z (block Jacobi), else
+// x += lambda p (lambda = scal[kSlotLambda]), then p = z + beta p with
+// beta = scal[kSlotBeta] and z = W^{-1} r: the given z (block Jacobi), else
 // dinv r or r; on the iteration that stopped the loop only x is updated (dune-istl leaves
 // the loop after the defect check).
 // Elements per thread of the x/p update: one double, or two contiguous floats (8 B per
@@ -252,7 +272,6 @@ template <typename T, typename A>
 __global__ __launch_bounds__(256) void cg_xpupdate_kernel(long long n, T* __restrict__ x, T* __restrict__ p,
                                                           const T* __restrict__ r, const T* __restrict__ dinv,
                                                           const T* __restrict__ z, const A* __restrict__ scal,
-                                                          int slot_rho_old, int slot_rho_new,
                                                           const int* __restrict__ state, int it) {
   // xp_ept<T>() contiguous elements per thread over an exact grid: the fastest of the
   // access patterns measured for this five-stream update
@@ -261,8 +280,8 @@ __global__ __launch_bounds__(256) void cg_xpupdate_kernel(long long n, T* __rest
   const long long i0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * E;
   if (i0 >= n) return;
   const bool pupd = state[0] == 0;
-  const T lambda = (T)(scal[slot_rho_old] / scal[0]);
-  const T beta = pupd ? (T)(scal[slot_rho_new] / scal[slot_rho_old]) : T(0);
+  const T lambda = (T)scal[kSlotLambda];
+  const T beta = pupd ? (T)scal[kSlotBeta] : T(0);
   // x and z (or r) stream (non-temporal); p, reread by the next apply, stays cached
   if constexpr (E == 2) {
     if (i0 + 1 < n) {  // two contiguous floats as one 8-byte access per stream

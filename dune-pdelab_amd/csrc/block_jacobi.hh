@@ -402,7 +402,7 @@ __global__ __launch_bounds__(256) void bj_dense_setup_kernel(const DGArgs<double
 // own LDS region with no workgroup barrier. Vectors are loaded and stored coalesced.
 //   BJ_PREC: out = W^{-1} in, one partial of out.in per workgroup
 //   BJ_ADD : out += W^{-1} in
-//   BJ_CGR : r -= lambda q (lambda = scal[slot_rho] / scal[0]), out = W^{-1} r, partials
+//   BJ_CGR : r -= lambda q (lambda = *lam), out = W^{-1} r, partials
 //            of (r.r, out.r) per workgroup; returns at once when state[0] is set
 template <typename T, typename A>
 struct BJVec {
@@ -410,8 +410,7 @@ struct BJVec {
   T* out;
   T* r;
   const T* q;
-  const A* scal;
-  int slot_rho;
+  const A* lam;
   A* partial;
   const int* state;
 };
@@ -451,7 +450,7 @@ __global__ __launch_bounds__(BJShape<N1>::THREADS) void bj_dense_kernel(const BJ
   const long long cc = c < g.ncells ? c : g.ncells - 1;
   const long long idx = cc * NL + (l < NL ? l : 0);
   T lambda = T(0);
-  if (MODE == BJ_CGR) lambda = (T)(v.scal[v.slot_rho] / v.scal[0]);
+  if (MODE == BJ_CGR) lambda = (T)*v.lam;
   A acc0 = A(0), acc1 = A(0);
   T u = T(0);
   if (act) {
@@ -539,7 +538,7 @@ __global__ __launch_bounds__(BJFdmShape<N1>::THREADS) void bj_fdm_kernel(const B
     __syncthreads();
   }
   T lambda = T(0);
-  if (MODE == BJ_CGR) lambda = (T)(v.scal[v.slot_rho] / v.scal[0]);
+  if (MODE == BJ_CGR) lambda = (T)*v.lam;
   A acc0 = A(0);
   T u[N1];
 #pragma unroll

@@ -555,7 +555,7 @@ static int make_prec(H* h, int precond, bool trans, Prec<T>& P) {
 template <typename T, typename A>
 static int prec_dot(H* h, const Prec<T>& P, const T* in, T* out, int slot) {
   if (P.block()) {
-    const BJVec<T, A> v{in, out, nullptr, nullptr, nullptr, 0, nullptr, nullptr};
+    const BJVec<T, A> v{in, out, nullptr, nullptr, nullptr, nullptr, nullptr};
     int nb = 0;
     CHK((bj_launch<T, A>(h, P.trans, BJ_PREC, v, &nb)));
     if (slot >= 0)
@@ -617,24 +617,24 @@ static int cg_loop(H* h, T* x, T* b, T* p, T* q, T* z, const Prec<T>& P, bool tr
     // q = J p with the p.q partials -> scal[0]; r -= lambda q with the (r.r, z.r)
     // partials -> scal[oslot..] and the defect check; x += lambda p, p = z + beta p
     CHK(apply_mode<T>(h, p, q, trans, 2, dotp, h->d_state));
-    hipLaunchKernelGGL((final_reduce_kernel<A, 1>), dim3(1), dim3(kFinalThreads), 0, h->stream, (const A*)dotp,
-                       (int)nbapply, scal + 0, (const int*)h->d_state);
+    hipLaunchKernelGGL((cg_pq_kernel<A>), dim3(1), dim3(kFinalThreads), 0, h->stream, (const A*)dotp, (int)nbapply,
+                       scal, rslot, (const int*)h->d_state);
     const A* parts = partial_buf<A>(h);
     int nparts = kRedBlocks;
     if (P.block()) {
       // r -= lambda q and z = W^{-1} r in one pass over the cells (z is kept for the p update)
-      const BJVec<T, A> v{b, z, b, q, scal, rslot, nullptr, h->d_state};
+      const BJVec<T, A> v{b, z, b, q, scal + kSlotLambda, nullptr, h->d_state};
       CHK((bj_launch<T, A>(h, P.trans, BJ_CGR, v, &nparts)));
       parts = static_cast<const A*>(h->d_bjpart);
     } else {
       hipLaunchKernelGGL((cg_rupdate_kernel<T, A>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, n, b,
-                         (const T*)q, P.dinv, (const A*)scal, rslot, partial_buf<A>(h), (const int*)h->d_state);
+                         (const T*)q, P.dinv, (const A*)scal, partial_buf<A>(h), (const int*)h->d_state);
     }
     hipLaunchKernelGGL((cg_check_kernel<A>), dim3(1), dim3(kFinalThreads), 0, h->stream, parts, nparts, scal + oslot,
-                       h->d_state, h->d_dstate, i);
+                       h->d_state, h->d_dstate, i, (const A*)(scal + rslot), scal + kSlotBeta);
     hipLaunchKernelGGL((cg_xpupdate_kernel<T, A>), dim3((unsigned)((n + 256LL * xp_ept<T>() - 1) / (256LL * xp_ept<T>()))),
                        dim3(256), 0, h->stream, n, x, p,
-                       (const T*)b, P.dinv, (const T*)(P.block() ? z : nullptr), (const A*)scal, rslot, oslot + 1,
+                       (const T*)b, P.dinv, (const T*)(P.block() ? z : nullptr), (const A*)scal,
                        (const int*)h->d_state, i);
     HIPCHK(h, hipGetLastError());
     last = i;
@@ -1205,7 +1205,7 @@ int pdelab_gpu_block_jacobi_apply(pdelab_gpu_gridoperator* h, const double* z, d
   if (!h || !z || !y) return PDELAB_ERR_INVALID_ARG;
   CHK(check_outflow(h));
   sync_prec_cache(h);
-  const BJVec<double, double> v{z, y, nullptr, nullptr, nullptr, 0, nullptr, nullptr};
+  const BJVec<double, double> v{z, y, nullptr, nullptr, nullptr, nullptr, nullptr};
   return bj_launch<double, double>(h, false, BJ_ADD, v, nullptr);
 }
 
@@ -1213,7 +1213,7 @@ int pdelab_gpu_block_jacobi_apply_f32(pdelab_gpu_gridoperator* h, const float* z
   if (!h || !z || !y) return PDELAB_ERR_INVALID_ARG;
   CHK(check_outflow(h));
   sync_prec_cache(h);
-  const BJVec<float, float> v{z, y, nullptr, nullptr, nullptr, 0, nullptr, nullptr};
+  const BJVec<float, float> v{z, y, nullptr, nullptr, nullptr, nullptr, nullptr};
   return bj_launch<float, float>(h, false, BJ_ADD, v, nullptr);
 }
 
