@@ -18,6 +18,9 @@ The JSON line also carries:
                 Gaussian sources, g = 0): FP64 BiCGSTAB/Jacobi vs FP32-inner BiCGSTAB in
                 FP64 refinement to 1e-8, and the adjoint (J^T, downwind) gradient of
                 1/2 u^T M u with respect to a per-cell source;
+  adjoint_logkappa  32^3 Q2, log-kappa ~ N(0, 0.5^2) per cell: forward solve and backward
+                (recomputed forward, adjoint solve, per-cell gradient kernel) of the gradient
+                of 1/2 ||u - u_obs||_M^2, as the libtorch autograd Function runs them;
   matrix_based  32^3 Q2: the assembled Jacobian (7-point block pattern) and its block-matrix
                 apply against the matrix-free apply;
   instationary  three Alexander2 steps of the heat equation at 64^3 Q3 (OneStepMethod).
@@ -137,6 +140,55 @@ def convection_leg(pkg):
     out["adjoint_source_gradient"] = {"seconds": t1 - t0, "iterations": ra["iterations"],
                                       "converged": bool(ra["converged"]), "grad_norm": float(np.linalg.norm(gh))}
     for v in (x64, xm, rhs, lam, g):
+        v.free()
+    go.close()
+    return out
+
+
+def adjoint_leg(pkg):
+    """32^3 Q2 SIPG, log-kappa per cell N(0, 0.5^2): the gradient of
+    J = 1/2 (u - u_obs)^T M (u - u_obs) with respect to log-kappa, u_obs the solution at a
+    second seeded field plus 1% Gaussian noise. Times the native calls of the autograd
+    Function (dune-pdelab_amd/torch_ext/adjoint.cpp): the forward solve, then the backward
+    (forward recomputed, adjoint solve J^T lambda = M (u - u_obs), per-cell gradient kernel),
+    CG to 1e-12 with block and with point Jacobi (the Function's default)."""
+    n, k = 32, 2
+    rng = np.random.default_rng(32)
+    theta, theta_b = rng.normal(0.0, 0.5, n ** 3), rng.normal(0.0, 0.5, n ** 3)
+    cfg = pkg.make_config(n=n, degree=k, alpha=3.0, kappa_per_cell=True, problem=pkg.PROBLEM_CONSTANT, f0=1.0,
+                          g0=0.0)
+    go = pkg.GridOperator(cfg, np.exp(theta_b))
+    ub = go.vector()
+    go.linear_problem_solve(ub, precond=pkg.PREC_BLOCK_JACOBI, reduction=1e-12, maxit=20000)
+    uh = ub.download()
+    uobs = go.vector(uh + 0.01 * np.sqrt(np.mean(uh * uh)) * rng.normal(size=go.N))
+    go.set_kappa(np.exp(theta))
+    out = {"workload": "32^3 Q2 SIPG, log-kappa ~ N(0, 0.25) per cell (32,768 entries), J = 1/2 ||u - u_obs||_M^2, "
+                       "u_obs from a second field plus 1% noise, CG to 1e-12"}
+    for name, prec in (("block_jacobi", pkg.PREC_BLOCK_JACOBI), ("point_jacobi", pkg.PREC_JACOBI)):
+        u, d, rhs, lam, g = go.vector(), go.vector(), go.vector(), go.vector(), go.cell_vector()
+        go.synchronize()
+        t0 = time.perf_counter()
+        rf = go.linear_problem_solve(u, precond=prec, reduction=1e-12, maxit=20000)
+        go.synchronize()
+        t1 = time.perf_counter()
+        u.free()
+        u = go.vector()
+        rb = go.linear_problem_solve(u, precond=prec, reduction=1e-12, maxit=20000)  # recomputed
+        d.copy_from(u)
+        go.axpy(-1.0, uobs, d)
+        go.mass_apply(d, rhs)
+        ra = go.solve(lam, rhs, precond=prec, reduction=1e-12, maxit=20000, transpose=True)
+        go.grad_logkappa(u, lam, g)
+        go.synchronize()
+        t2 = time.perf_counter()
+        out[name] = {"forward_seconds": t1 - t0, "backward_seconds": t2 - t1,
+                     "forward_iterations": rf["solver"]["iterations"], "adjoint_iterations": ra["iterations"],
+                     "converged": bool(rf["solver"]["converged"] and rb["solver"]["converged"] and ra["converged"]),
+                     "grad_norm": float(np.linalg.norm(g.download()))}
+        for v in (u, d, rhs, lam, g):
+            v.free()
+    for v in (ub, uobs):
         v.free()
     go.close()
     return out
@@ -431,6 +483,8 @@ def main():
         go3.close()
         if not args.no_convection:
             out["convection"] = convection_leg(pkg)
+        log("bench: 32^3 Q2 adjoint log-kappa gradient")
+        out["adjoint_logkappa"] = adjoint_leg(pkg)
         log("bench: 32^3 Q2 assembled Jacobian and block-matrix apply")
         out["matrix_based"] = matrix_based_leg(pkg)
         log("bench: instationary Alexander2 steps")

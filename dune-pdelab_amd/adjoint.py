@@ -40,8 +40,9 @@ def load():
     return mod
 
 
-def dg_solve(log_kappa, go, reduction=1e-12, maxit=20000):
+def dg_solve(log_kappa, go, reduction=1e-12, maxit=20000, precond=1):
     """u(log_kappa) for the grid operator ``go`` (a dune_pdelab_amd.GridOperator created with
-    kappa_per_cell); differentiable with respect to log_kappa."""
+    kappa_per_cell); differentiable with respect to log_kappa. Forward and adjoint solves are
+    CG with ``precond`` (0 none, 1 point Jacobi, 2 block Jacobi)."""
     mod = load()
-    return mod.dg_solve(log_kappa, int(go.h.value), reduction, maxit)
+    return mod.dg_solve(log_kappa, int(go.h.value), reduction, maxit, precond)

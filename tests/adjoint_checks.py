@@ -53,19 +53,20 @@ def check_oracle():
     import oracle_ffi as of
     cfg, go, theta, rng = setup(6, 2, 21)
     uobs = rng.normal(0.0, 0.01, go.N)
-    t = torch.tensor(theta, dtype=torch.float64, device="cuda", requires_grad=True)
-    u = adjoint.dg_solve(t, go, reduction=1e-13)
-    L = mass_quadratic(go)(u - torch.tensor(uobs, device="cuda"))
-    L.backward()
-    g = t.grad.detach().cpu().numpy()
     o = of.Oracle(cfg, np.exp(theta))
     x, _ = o.linear_problem_solve(reduction=1e-14, maxit=20000)
     lam, _, _ = o.solve(o.mass_apply(x - uobs), reduction=1e-14, maxit=20000, matrix_based=2)
     g_ref = o.grad_logkappa(x, lam)
-    eu = np.linalg.norm(u.detach().cpu().numpy() - x) / np.linalg.norm(x)
-    eg = np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref)
-    print(f"oracle check 6^3 Q2: u rel err {eu:.3e}, gradient rel err {eg:.3e}", flush=True)
-    assert eu <= 1e-10 and eg <= 1e-8
+    for precond, name in ((1, "point Jacobi"), (2, "block Jacobi")):
+        t = torch.tensor(theta, dtype=torch.float64, device="cuda", requires_grad=True)
+        u = adjoint.dg_solve(t, go, reduction=1e-13, precond=precond)
+        L = mass_quadratic(go)(u - torch.tensor(uobs, device="cuda"))
+        L.backward()
+        g = t.grad.detach().cpu().numpy()
+        eu = np.linalg.norm(u.detach().cpu().numpy() - x) / np.linalg.norm(x)
+        eg = np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref)
+        print(f"oracle check 6^3 Q2, {name}: u rel err {eu:.3e}, gradient rel err {eg:.3e}", flush=True)
+        assert eu <= 1e-10 and eg <= 1e-8
 
 
 def check_fd(n=32, ncheck=20):
