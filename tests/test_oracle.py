@@ -19,6 +19,10 @@ operand, wrong index) fails at least one:
     iteration counts equal to an independent numpy PCG with inverted blocks, and the
     matrix-free == matrix-based iteration equality with block Jacobi
     (R:dune/pdelab/test/matrixfree/matrix_free_linear.cc:253-277,390-394);
+  * the Krylov solvers (dune-istl CGSolver / BiCGSTABSolver restated with IterativeSolver's
+    stopping rule) against independent numpy textbook PCG and right-preconditioned BiCGSTAB
+    (van der Vorst 1992): iteration counts (BiCGSTAB within the rounding spread of the
+    textbook counts) and solutions;
   * the adjoint gradient against central finite differences.
 """
 import json
@@ -312,6 +316,84 @@ def test_block_jacobi_transposed_operator(of):
     assert res["converged"]
     At = o.dense_matrix().T
     assert np.linalg.norm(At @ z - rhs) <= 1e-11 * np.linalg.norm(rhs)
+
+
+def _numpy_bicgstab(A, b, prec, reduction, maxit=10000):
+    """Textbook right-preconditioned BiCGSTAB (van der Vorst, SIAM J. Sci. Stat. Comput. 13,
+    1992; x0 = 0, shadow residual r0), with the defect tested after each half step (on s) and
+    each full step (on r) against reduction * ||r0||. Returns (x, i) with i the outer step at
+    which it stopped, which is dune-istl's half-step count rounded up."""
+    x = np.zeros_like(b)
+    r = b.copy()
+    rt = r.copy()
+    r0 = np.linalg.norm(r)
+    rho_old = alpha = omega = 1.0
+    p = np.zeros_like(b)
+    v = np.zeros_like(b)
+    for i in range(1, maxit + 1):
+        rho = rt @ r
+        beta = (rho / rho_old) * (alpha / omega)
+        p = r + beta * (p - omega * v)
+        y = prec(p)
+        v = A @ y
+        alpha = rho / (rt @ v)
+        x = x + alpha * y
+        s = r - alpha * v
+        if np.linalg.norm(s) < reduction * r0:
+            return x, i
+        z = prec(s)
+        t = A @ z
+        omega = (t @ s) / (t @ t)
+        x = x + omega * z
+        r = s - omega * t
+        rho_old = rho
+        if np.linalg.norm(r) < reduction * r0:
+            return x, i
+    return x, -1
+
+
+@pytest.mark.parametrize("precond", [0, 1], ids=["none", "point_jacobi"])
+@pytest.mark.parametrize("k", [1, 2])
+def test_bicgstab_vs_numpy_textbook(of, precond, k):
+    """The oracle's BiCGSTAB (dune-istl BiCGSTABSolver restated, half-step counting) against
+    an independent numpy textbook BiCGSTAB on a convection-diffusion operator. BiCGSTAB's
+    count is sensitive to rounding (on the Q1 case with point Jacobi, right-hand sides that
+    differ by 1e-15 relative take 62 to 70 steps), so the oracle's count must lie within the
+    spread of the textbook counts over such perturbations, to one; the solutions agree to
+    the solver tolerance."""
+    cfg = of.cfgmod.make_config(n=4, degree=k, alpha=3.0, kappa0=0.05, b=(1.0, 0.5, -0.25))
+    o = of.Oracle(cfg)
+    A = o.dense_matrix()
+    rhs = np.random.default_rng(11 + k).normal(size=o.N)
+    dinv = 1.0 / np.diag(A)
+    pre = (lambda r: r.copy()) if precond == 0 else (lambda r: dinv * r)
+    x_np, it_np = _numpy_bicgstab(A, rhs, pre, 1e-10)
+    counts = [it_np] + [_numpy_bicgstab(A, rhs * (1 + 1e-15 * np.random.default_rng(100 + s).normal(size=o.N)),
+                                        pre, 1e-10)[1] for s in range(6)]
+    assert min(counts) > 0
+    z, _, res = o.solve(rhs.copy(), method=1, precond=precond, reduction=1e-10)
+    assert res["converged"]
+    assert min(counts) - 1 <= res["iterations"] <= max(counts) + 1, (res["iterations"], counts)
+    assert np.linalg.norm(A @ z - rhs) <= 1e-10 * np.linalg.norm(rhs)
+    assert np.abs(z - x_np).max() <= 1e-6 * np.abs(x_np).max()
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_point_jacobi_cg_vs_numpy_pcg(of, k):
+    """The oracle's CG with point Jacobi (diag(J) from the point-diagonal wrapper) against
+    the numpy textbook PCG with the diagonal of the assembled matrix: the same iteration
+    count (to one) and the same solution."""
+    cfg = of.cfgmod.make_config(n=4, degree=k, alpha=3.0)
+    o = of.Oracle(cfg)
+    A = o.dense_matrix()
+    rhs = np.random.default_rng(21 + k).normal(size=o.N)
+    dinv = 1.0 / np.diag(A)
+    x_np, it_np = _numpy_pcg(A, rhs, lambda r: dinv * r, 1e-10)
+    assert it_np > 0
+    z, _, res = o.solve(rhs.copy(), method=0, precond=of.cfgmod.PREC_JACOBI, reduction=1e-10)
+    assert res["converged"]
+    assert abs(res["iterations"] - it_np) <= 1
+    assert np.abs(z - x_np).max() <= 1e-7 * np.abs(x_np).max()
 
 
 def test_bicgstab_convection_diffusion(of):
