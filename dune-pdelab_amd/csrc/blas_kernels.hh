@@ -268,7 +268,12 @@ __global__ __launch_bounds__(kFinalThreads) void cg_check_kernel(const A* __rest
 template <typename T>
 constexpr int xp_ept() { return sizeof(T) == 4 ? 2 : 1; }
 
-template <typename T, typename A>
+// Source of z: the given vector (block Jacobi), dinv r (point Jacobi) or r (none), fixed at
+// compile time (alone at 2^24 doubles the run-time choice cost 104.5 against 96.8 us; inside
+// the CG loop the two are within noise).
+enum { XP_Z = 0, XP_DINV_R = 1, XP_R = 2 };
+
+template <typename T, typename A, int ZSRC>
 __global__ __launch_bounds__(256) void cg_xpupdate_kernel(long long n, T* __restrict__ x, T* __restrict__ p,
                                                           const T* __restrict__ r, const T* __restrict__ dinv,
                                                           const T* __restrict__ z, const A* __restrict__ scal,
@@ -291,8 +296,9 @@ __global__ __launch_bounds__(256) void cg_xpupdate_kernel(long long n, T* __rest
       st_stream<v2>(ld_stream<v2>(xv) + lambda * pv, xv);
       if (pupd) {
         v2 zv;
-        if (z) zv = ld_stream<v2>(reinterpret_cast<const v2*>(z + i0));
-        else if (dinv) zv = *reinterpret_cast<const v2*>(dinv + i0) * ld_stream<v2>(reinterpret_cast<const v2*>(r + i0));
+        if constexpr (ZSRC == XP_Z) zv = ld_stream<v2>(reinterpret_cast<const v2*>(z + i0));
+        else if constexpr (ZSRC == XP_DINV_R)
+          zv = *reinterpret_cast<const v2*>(dinv + i0) * ld_stream<v2>(reinterpret_cast<const v2*>(r + i0));
         else zv = ld_stream<v2>(reinterpret_cast<const v2*>(r + i0));
         *reinterpret_cast<v2*>(p + i0) = zv + beta * pv;
       }
@@ -306,7 +312,10 @@ __global__ __launch_bounds__(256) void cg_xpupdate_kernel(long long n, T* __rest
       const T pv = p[i];
       st_stream<T>(ld_stream<T>(x + i) + lambda * pv, x + i);
       if (pupd) {
-        const T zv = z ? ld_stream<T>(z + i) : (dinv ? dinv[i] * ld_stream<T>(r + i) : ld_stream<T>(r + i));
+        T zv;
+        if constexpr (ZSRC == XP_Z) zv = ld_stream<T>(z + i);
+        else if constexpr (ZSRC == XP_DINV_R) zv = dinv[i] * ld_stream<T>(r + i);
+        else zv = ld_stream<T>(r + i);
         p[i] = zv + beta * pv;
       }
     }

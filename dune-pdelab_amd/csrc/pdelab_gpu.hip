@@ -632,10 +632,19 @@ static int cg_loop(H* h, T* x, T* b, T* p, T* q, T* z, const Prec<T>& P, bool tr
     }
     hipLaunchKernelGGL((cg_check_kernel<A>), dim3(1), dim3(kFinalThreads), 0, h->stream, parts, nparts, scal + oslot,
                        h->d_state, h->d_dstate, i, (const A*)(scal + rslot), scal + kSlotBeta);
-    hipLaunchKernelGGL((cg_xpupdate_kernel<T, A>), dim3((unsigned)((n + 256LL * xp_ept<T>() - 1) / (256LL * xp_ept<T>()))),
-                       dim3(256), 0, h->stream, n, x, p,
-                       (const T*)b, P.dinv, (const T*)(P.block() ? z : nullptr), (const A*)scal,
-                       (const int*)h->d_state, i);
+    {
+      const dim3 grid((unsigned)((n + 256LL * xp_ept<T>() - 1) / (256LL * xp_ept<T>())));
+      const T* zsrc = P.block() ? z : nullptr;
+      if (P.block())
+        hipLaunchKernelGGL((cg_xpupdate_kernel<T, A, XP_Z>), grid, dim3(256), 0, h->stream, n, x, p, (const T*)b, P.dinv,
+                           zsrc, (const A*)scal, (const int*)h->d_state, i);
+      else if (P.dinv)
+        hipLaunchKernelGGL((cg_xpupdate_kernel<T, A, XP_DINV_R>), grid, dim3(256), 0, h->stream, n, x, p, (const T*)b,
+                           P.dinv, zsrc, (const A*)scal, (const int*)h->d_state, i);
+      else
+        hipLaunchKernelGGL((cg_xpupdate_kernel<T, A, XP_R>), grid, dim3(256), 0, h->stream, n, x, p, (const T*)b, P.dinv,
+                           zsrc, (const A*)scal, (const int*)h->d_state, i);
+    }
     HIPCHK(h, hipGetLastError());
     last = i;
     if (i % kCheckEvery == 0 || i == maxit) {
