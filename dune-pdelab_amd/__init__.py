@@ -196,6 +196,27 @@ class GridOperator:
         if st != 0:
             raise PdelabError(st, lib().pdelab_gpu_last_error(self.h).decode())
 
+    def _vec(self, v, n=None, dtype=np.float64):
+        """Device pointer of v after checking that it is a live DeviceVector of n elements
+        (default: the DoF count) and the given dtype. The library takes raw pointers and
+        cannot check them; a shorter or narrower buffer would be read and written past its
+        end."""
+        if not isinstance(v, DeviceVector) or v.ptr is None:
+            raise TypeError("expected a live DeviceVector")
+        if v.dtype != np.dtype(dtype):
+            raise TypeError(f"expected a {np.dtype(dtype)} vector, got {v.dtype}")
+        want = self.N if n is None else n
+        if v.n != want:
+            raise ValueError(f"expected {want} elements, got {v.n}")
+        return v.ptr
+
+    def _pair(self, z, y):
+        """Pointers of two DoF vectors of one dtype (float64 or float32) and that dtype."""
+        dt = z.dtype if isinstance(z, DeviceVector) else np.dtype(np.float64)
+        if dt not in (np.dtype(np.float64), np.dtype(np.float32)):
+            raise TypeError(f"unsupported vector dtype {dt}")
+        return self._vec(z, dtype=dt), self._vec(y, dtype=dt), dt
+
     # -- vectors
     def vector(self, data=None, dtype=np.float64):
         v = DeviceVector(self, self.N, dtype)
@@ -218,7 +239,7 @@ class GridOperator:
         self._check(lib().pdelab_gpu_set_kappa(self.h, k.ctypes.data_as(_dp)))
 
     def set_log_kappa_device(self, logk):
-        self._check(lib().pdelab_gpu_set_log_kappa_device(self.h, logk.p))
+        self._check(lib().pdelab_gpu_set_log_kappa_device(self.h, self._vec(logk, self.ncells)))
 
     def set_cell_source(self, s):
         if s is None:
@@ -237,30 +258,32 @@ class GridOperator:
 
     # -- GridOperator
     def residual(self, x, r):
-        self._check(lib().pdelab_gpu_residual(self.h, x.p, r.p))
+        self._check(lib().pdelab_gpu_residual(self.h, self._vec(x), self._vec(r)))
 
     def jacobian_apply(self, z, y):
-        if z.dtype == np.float32:
-            self._check(lib().pdelab_gpu_jacobian_apply_f32(self.h, z.p, y.p))
+        zp, yp, dt = self._pair(z, y)
+        if dt == np.float32:
+            self._check(lib().pdelab_gpu_jacobian_apply_f32(self.h, zp, yp))
         else:
-            self._check(lib().pdelab_gpu_jacobian_apply(self.h, z.p, y.p))
+            self._check(lib().pdelab_gpu_jacobian_apply(self.h, zp, yp))
 
     def operator_apply(self, z, y):
         """y = J z (OnTheFlyOperator::apply: y = 0, then jacobian_apply)."""
-        self._check(lib().pdelab_gpu_operator_apply(self.h, z.p, y.p))
+        self._check(lib().pdelab_gpu_operator_apply(self.h, self._vec(z), self._vec(y)))
 
     def jacobian_apply_nonlinear(self, x, z, y):
-        self._check(lib().pdelab_gpu_jacobian_apply_nonlinear(self.h, x.p, z.p, y.p))
+        self._check(lib().pdelab_gpu_jacobian_apply_nonlinear(self.h, self._vec(x), self._vec(z), self._vec(y)))
 
     def jacobian_apply_transpose(self, z, y):
-        self._check(lib().pdelab_gpu_jacobian_apply_transpose(self.h, z.p, y.p))
+        self._check(lib().pdelab_gpu_jacobian_apply_transpose(self.h, self._vec(z), self._vec(y)))
 
     def point_diagonal(self, d):
-        self._check(lib().pdelab_gpu_point_diagonal(self.h, d.p))
+        self._check(lib().pdelab_gpu_point_diagonal(self.h, self._vec(d)))
 
     def block_diagonal(self, blocks, transpose=False):
         """blocks (ncells * nloc^2 doubles) = the diagonal blocks of J (J^T), row-major per cell."""
-        self._check(lib().pdelab_gpu_block_diagonal(self.h, 1 if transpose else 0, blocks.p))
+        self._check(lib().pdelab_gpu_block_diagonal(self.h, 1 if transpose else 0,
+                                                    self._vec(blocks, self.ncells * ((self.cfg.degree + 1) ** 6))))
 
     def jacobian(self, transpose=False):
         """Assembled Jacobian (7-point block pattern) as device vectors (blocks, cols); cols holds
@@ -273,13 +296,15 @@ class GridOperator:
 
     def block_matrix_apply(self, blocks, cols, z, y):
         """y = A z for a block matrix from jacobian()."""
-        self._check(lib().pdelab_gpu_block_matrix_apply(self.h, blocks.p, cols.p, z.p, y.p))
+        self._check(lib().pdelab_gpu_block_matrix_apply(self.h, self._vec(blocks, 7 * self.ncells * ((self.cfg.degree + 1) ** 6)),
+                                                        self._vec(cols, 7 * self.ncells), self._vec(z),
+                                                        self._vec(y)))
 
     # -- output
     def l2_error_squared(self, x, order=10):
         """integral of (u_h - u)^2 for the built-in problem's exact solution u."""
         out = ctypes.c_double()
-        self._check(lib().pdelab_gpu_l2_error_squared(self.h, x.p, order, ctypes.byref(out)))
+        self._check(lib().pdelab_gpu_l2_error_squared(self.h, self._vec(x), order, ctypes.byref(out)))
         return out.value
 
     def sample(self, x, intervals):
@@ -287,7 +312,7 @@ class GridOperator:
         indexed [cell, z, y, x]."""
         L1 = intervals + 1
         v = DeviceVector(self, self.ncells * L1 ** 3)
-        self._check(lib().pdelab_gpu_sample(self.h, x.p, intervals, v.p))
+        self._check(lib().pdelab_gpu_sample(self.h, self._vec(x), intervals, v.p))
         out = v.download().reshape(self.ncells, L1, L1, L1)
         v.free()
         return out
@@ -295,52 +320,53 @@ class GridOperator:
     # -- instationary
     def onestep_jacobian_apply(self, a_mass, b_dt, z, y):
         """y += b_dt J z + a_mass M z (a stage of OneStepGridOperator)."""
-        self._check(lib().pdelab_gpu_onestep_jacobian_apply(self.h, a_mass, b_dt, z.p, y.p))
+        self._check(lib().pdelab_gpu_onestep_jacobian_apply(self.h, a_mass, b_dt, self._vec(z), self._vec(y)))
 
     def onestep_step(self, xold, xnew, scheme=TS_ALEXANDER2, dt=0.1, method=SOLVER_CG, precond=PREC_JACOBI,
                      reduction=1e-10, maxit=5000):
         """One step xold -> xnew of OneStepMethod::apply (xnew holds the first stage's initial guess)."""
         res = cfgmod.OneStepResult()
-        self._check(lib().pdelab_gpu_onestep_step(self.h, scheme, method, precond, dt, reduction, maxit, xold.p,
-                                                  xnew.p, ctypes.byref(res)))
+        self._check(lib().pdelab_gpu_onestep_step(self.h, scheme, method, precond, dt, reduction, maxit,
+                                                  self._vec(xold), self._vec(xnew), ctypes.byref(res)))
         return res.as_dict()
 
     def block_jacobi_apply(self, z, y):
         """y += D^{-1} z with D the block diagonal of J (FP32 for float32 vectors)."""
-        if z.dtype == np.float32:
-            self._check(lib().pdelab_gpu_block_jacobi_apply_f32(self.h, z.p, y.p))
+        zp, yp, dt = self._pair(z, y)
+        if dt == np.float32:
+            self._check(lib().pdelab_gpu_block_jacobi_apply_f32(self.h, zp, yp))
         else:
-            self._check(lib().pdelab_gpu_block_jacobi_apply(self.h, z.p, y.p))
+            self._check(lib().pdelab_gpu_block_jacobi_apply(self.h, zp, yp))
 
     def mass_apply(self, z, y):
-        self._check(lib().pdelab_gpu_mass_apply(self.h, z.p, y.p))
+        self._check(lib().pdelab_gpu_mass_apply(self.h, self._vec(z), self._vec(y)))
 
     # -- vector backend
     def dot(self, x, y):
         out = ctypes.c_double()
-        self._check(lib().pdelab_gpu_dot(self.h, x.p, y.p, ctypes.byref(out)))
+        self._check(lib().pdelab_gpu_dot(self.h, self._vec(x), self._vec(y), ctypes.byref(out)))
         return out.value
 
     def norm(self, x):
         out = ctypes.c_double()
-        self._check(lib().pdelab_gpu_norm(self.h, x.p, ctypes.byref(out)))
+        self._check(lib().pdelab_gpu_norm(self.h, self._vec(x), ctypes.byref(out)))
         return out.value
 
     def axpy(self, a, x, y):
-        self._check(lib().pdelab_gpu_axpy(self.h, a, x.p, y.p))
+        self._check(lib().pdelab_gpu_axpy(self.h, a, self._vec(x), self._vec(y)))
 
     # -- solvers
     def solve(self, z, r, method=SOLVER_CG, precond=PREC_JACOBI, reduction=1e-10, maxit=5000, transpose=False):
         res = LinearSolverResult()
         fn = lib().pdelab_gpu_solve_transpose if transpose else lib().pdelab_gpu_solve
-        self._check(fn(self.h, method, precond, reduction, maxit, z.p, r.p, ctypes.byref(res)))
+        self._check(fn(self.h, method, precond, reduction, maxit, self._vec(z), self._vec(r), ctypes.byref(res)))
         return res.as_dict()
 
     def linear_problem_solve(self, x, method=SOLVER_CG, precond=PREC_JACOBI, reduction=1e-10, min_defect=1e-99,
                              maxit=5000):
         res = LinearProblemResult()
-        self._check(lib().pdelab_gpu_linear_problem_solve(self.h, method, precond, reduction, min_defect, maxit, x.p,
-                                                          ctypes.byref(res)))
+        self._check(lib().pdelab_gpu_linear_problem_solve(self.h, method, precond, reduction, min_defect, maxit,
+                                                          self._vec(x), ctypes.byref(res)))
         return res.as_dict()
 
     def solve_mixed(self, x, method=SOLVER_CG, precond=PREC_JACOBI, inner_reduction=None, inner_maxit=5000,
@@ -352,25 +378,26 @@ class GridOperator:
             inner_reduction = 0.1 if method == SOLVER_CG else 1e-2
         res = MixedResult()
         self._check(lib().pdelab_gpu_solve_mixed(self.h, method, precond, inner_reduction, inner_maxit,
-                                                 outer_reduction, outer_maxit, x.p, ctypes.byref(res)))
+                                                 outer_reduction, outer_maxit, self._vec(x), ctypes.byref(res)))
         return res.as_dict()
 
     # -- adjoint gradients
     def grad_logkappa(self, u, lam, g):
-        self._check(lib().pdelab_gpu_grad_logkappa(self.h, u.p, lam.p, g.p))
+        self._check(lib().pdelab_gpu_grad_logkappa(self.h, self._vec(u), self._vec(lam), self._vec(g, self.ncells)))
 
     def grad_source(self, lam, g):
-        self._check(lib().pdelab_gpu_grad_source(self.h, lam.p, g.p))
+        self._check(lib().pdelab_gpu_grad_source(self.h, self._vec(lam), self._vec(g, self.ncells)))
 
     # -- measurement
     def time_stream(self, x, y, warmup=3, reps=20):
         """Average duration (ms) of y += 0.5 x over FP64 vectors of the handle's length."""
         out = ctypes.c_double()
-        self._check(lib().pdelab_gpu_time_stream(self.h, x.p, y.p, warmup, reps, ctypes.byref(out)))
+        self._check(lib().pdelab_gpu_time_stream(self.h, self._vec(x), self._vec(y), warmup, reps, ctypes.byref(out)))
         return out.value
 
     def time_apply(self, z, y, warmup=3, reps=20):
-        prec = 32 if z.dtype == np.float32 else 64
+        zp, yp, dt = self._pair(z, y)
+        prec = 32 if dt == np.float32 else 64
         out = ctypes.c_double()
-        self._check(lib().pdelab_gpu_time_apply(self.h, prec, z.p, y.p, warmup, reps, ctypes.byref(out)))
+        self._check(lib().pdelab_gpu_time_apply(self.h, prec, zp, yp, warmup, reps, ctypes.byref(out)))
         return out.value

@@ -598,8 +598,11 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
   load_line<N1, 2, T>(zc, off, p, q, uz);
   if (lo[2]) load_line<N1, 2, T>(zc - a.stride[2] * NL, off, p, q, zl);
   if (hi[2]) load_line<N1, 2, T>(zc + a.stride[2] * NL, off, p, q, zr);
-  if (lo[1]) load_line<N1, 2, T>(zc - a.stride[1] * NL, off, p, q, yl);
-  if (hi[1]) load_line<N1, 2, T>(zc + a.stride[1] * NL, off, p, q, yr);
+  // the y-neighbours' lines are loaded in y-layout (the lane's y-line at x = p, z = q), so
+  // the y line operation takes them from registers: 16 fewer LDS accesses per lane than
+  // staging them in z-layout, for 32-byte rather than 128-byte segments per cell and load
+  if (lo[1]) load_line<N1, 1, T>(zc - a.stride[1] * NL, off, p, q, yl);
+  if (hi[1]) load_line<N1, 1, T>(zc + a.stride[1] * NL, off, p, q, yr);
   if (ld_xe) load_line<N1, 2, T>(zc + (first ? -NL : NL), off, p, q, xe);
   // cell-wise kappa: the cell's kappa and the z faces now, the y and x faces after the
   // staging barrier (fewer registers live across the loads and the staging)
@@ -634,8 +637,6 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
 #pragma unroll
     for (int e = 0; e < N1; ++e) {
       LS[UO + az[e]] = uz[e];
-      if (lo[1]) LS[YMO + az[e]] = yl[e];
-      if (hi[1]) LS[YPO + az[e]] = yr[e];
       if (ld_xe) LS[xo + (az[e] ^ XP)] = xe[e];
     }
   }
@@ -651,8 +652,8 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
 #pragma unroll
     for (int e = 0; e < N1; ++e) {
       u[e] = LS[UO + ay[e]];
-      ul[e] = lo[1] ? LS[YMO + ay[e]] : T(0);
-      ur[e] = hi[1] ? LS[YPO + ay[e]] : T(0);
+      ul[e] = lo[1] ? yl[e] : T(0);
+      ur[e] = hi[1] ? yr[e] : T(0);
     }
     line_apply<N1, 1, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, lo[1], hi[1], kc, flo[1], fhi[1], sy);
     // x-neighbours: the staged cells lc-1 and lc+1 (the end cells for the chunk ends)

@@ -374,3 +374,23 @@ def test_large_apply_properties(pkg, of):
     go.jacobian_apply(s, js)
     lin = js.download() - (jx.download() + 2.0 * jy.download())
     assert np.linalg.norm(lin) <= 1e-13 * np.linalg.norm(js.download())
+
+
+def test_vector_arguments_checked_before_the_call(pkg):
+    """The C-ABI takes raw pointers; the binding rejects a vector of the wrong dtype or length
+    on the host, before the library could read or write past the buffer."""
+    say("[gpu] binding argument checks")
+    go = pkg.GridOperator(pkg.make_config(nx=16, ny=3, nz=4, degree=3, alpha=3.0))
+    z32, y32 = go.vector(dtype=np.float32), go.vector(dtype=np.float32)
+    with pytest.raises(TypeError):
+        go.operator_apply(z32, y32)          # FP64-only entry point
+    with pytest.raises(TypeError):
+        go.jacobian_apply(go.vector(), y32)  # mixed dtypes
+    short = pkg.DeviceVector(go, go.N - 1)
+    with pytest.raises(ValueError):
+        go.residual(short, go.vector())
+    with pytest.raises(ValueError):
+        go.grad_source(go.vector(), go.vector())  # needs one value per cell
+    go.jacobian_apply(z32, y32)              # the FP32 entry point accepts them
+    go.synchronize()
+    go.close()
