@@ -543,11 +543,11 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
 // y (+)= J z (or J^T z) on grids with nx % CPB == 0, staged through registers: the CPB
 // cells of a workgroup are one aligned chunk of an x-row (so the y and z boundary flags
 // are uniform over the workgroup), consecutive logical blocks walk z fastest, then y,
-// then the x-chunk, and every global access is a coalesced z-line access. The chunk,
-// its two end cells and the chunks of the rows y-1 and y+1 are written to one LDS
-// array in the rotated, conflict-free cell layout of lds_index; each lane reads its
-// twelve slot offsets once from SlotTable, so every LDS access is a register plus an
-// immediate offset. The z-neighbour lines stay in registers.
+// then the x-chunk. The chunk and its two end cells are written to one LDS array in the
+// rotated, conflict-free cell layout of lds_index; each lane reads its twelve slot
+// offsets once from SlotTable, so every LDS access is a register plus an immediate
+// offset. The z-neighbour lines (z-layout) and the y-neighbour lines (y-layout) are
+// loaded straight into registers; the regions after the chunk hold the partial sums.
 //
 // FP64 Q3 stores the cells unpadded (64 doubles) and separates neighbouring cells' banks
 // by their layout parity instead: cell lc (lc = -1 .. CPB, in every region) keeps slot s
@@ -566,7 +566,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
   constexpr bool XS = sizeof(T) == 8 && N1 == 4;
   constexpr int NLP = XS ? NL : SH::NLP;
   // cell before the chunk, the CPB own cells, cell after the chunk (from UO - NLP on);
-  // the chunks of the rows y-1 (YMO) and y+1 (YPO), later the partial sums and sweeps
+  // two chunk-sized regions (YMO, YPO) for the partial sums and the mass sweeps
   constexpr int UO = NLP, YMO = (CPB + 2) * NLP, YPO = (2 * CPB + 2) * NLP;
   __shared__ T LS[(3 * CPB + 2) * NLP];
   const int t = threadIdx.x;
