@@ -141,7 +141,7 @@ def convection_leg(pkg):
                             "converged": bool(r64["solver"]["converged"]), "reduction": r64["solver"]["reduction"]}
     xm = go.vector()
     log("bench: 96^3 Q2 mixed-precision BiCGSTAB")
-    rm = go.solve_mixed(xm, method=pkg.SOLVER_BICGSTAB, inner_reduction=1e-2, outer_reduction=1e-8,
+    rm = go.solve_mixed(xm, method=pkg.SOLVER_BICGSTAB, inner_reduction=1e-4, outer_reduction=1e-8,
                         inner_maxit=20000)
     a64, am = x64.download(), xm.download()
     out["mixed_bicgstab"] = {"seconds": rm["elapsed"], "outer": rm["outer_iterations"],

@@ -372,10 +372,10 @@ class GridOperator:
     def solve_mixed(self, x, method=SOLVER_CG, precond=PREC_JACOBI, inner_reduction=None, inner_maxit=5000,
                     outer_reduction=1e-10, outer_maxit=50):
         """FP32 Krylov inside FP64 refinement. inner_reduction defaults to 0.1 for CG (the
-        FP32 residual reduction between FP64 residual replacements) and 1e-2 for BiCGSTAB
+        FP32 residual reduction between FP64 residual replacements) and 1e-4 for BiCGSTAB
         (the reduction of each restarted inner solve)."""
         if inner_reduction is None:
-            inner_reduction = 0.1 if method == SOLVER_CG else 1e-2
+            inner_reduction = 0.1 if method == SOLVER_CG else 1e-4
         res = MixedResult()
         self._check(lib().pdelab_gpu_solve_mixed(self.h, method, precond, inner_reduction, inner_maxit,
                                                  outer_reduction, outer_maxit, self._vec(x), ctypes.byref(res)))

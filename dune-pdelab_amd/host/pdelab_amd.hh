@@ -547,7 +547,7 @@ using ISTLBackend_SEQ_MatrixFree_BCGS_BlockJac =
 template <class GO, Krylov method, Preconditioner prec = Preconditioner::Jacobi>
 class MixedPrecisionSolver {
  public:
-  MixedPrecisionSolver(const GO& go, double inner_reduction = method == Krylov::CG ? 0.1 : 1e-3,
+  MixedPrecisionSolver(const GO& go, double inner_reduction = method == Krylov::CG ? 0.1 : 1e-4,
                        unsigned inner_maxit = 20000)
       : go_(go), inner_(inner_reduction), inner_maxit_(inner_maxit) {}
   pdelab_mixed_result apply(typename GO::Traits::Domain& x, double reduction, int outer_maxit = 50) {
