@@ -571,15 +571,7 @@ static int prec_dot(H* h, const Prec<T>& P, const T* in, T* out, int slot) {
   return PDELAB_OK;
 }
 
-// CGSolver::apply (dune-istl): solves J z = b from the given z; b becomes the defect.
-// Workspace slots base, base+1 (p, q).
-//
-// The loop runs on the device: per iteration the apply (with the fused p.q partials),
-// the update of x and r (with the r.r and z.r partials), the Iteration::step check and the
-// direction update are enqueued without waiting; the check kernel raises a device flag
-// at the iteration where dune-istl's stopping rule fires and every later kernel of the
-// loop returns immediately, so the state is exactly the one at that iteration. The host
-// reads the flag back every kCheckEvery iterations.
+// Iterations between host reads of the device CG loop's stop flag.
 constexpr int kCheckEvery = 16;
 
 // Outcome of one run of the device CG loop.
