@@ -9,6 +9,17 @@ static int launch_apply_k(H* h, const T* z, T* y, const T* fc, const DGArgs<T>& 
   using SH = ApplyShape<N1>;
   const long long nb = (h->ncells + SH::CPB - 1) / SH::CPB;
   const LineMats<T>* lm = line_mats<T>(h, TRANS);
+  const int zw = apply_walk(h);
+  if constexpr (!VARK && !CONV && (N1 == 3 || N1 == 4)) {
+    constexpr int ZW = N1 == 4 ? 2 : 4;
+    if (zw == ZW) {
+      hipLaunchKernelGGL((dg_apply_rowreg_kernel<N1, T, TRANS, CONV, REACT, VARK, ACCUM, DOT, ZW>),
+                         dim3((unsigned)(nb / ZW)), dim3(SH::THREADS), 0, h->stream, z, y, fc, a, lm, nb / ZW, dotp,
+                         stop);
+      HIPCHK(h, hipGetLastError());
+      return PDELAB_OK;
+    }
+  }
   if (a.rowmode && h->apply_variant == 0)
     hipLaunchKernelGGL((dg_apply_rowreg_kernel<N1, T, TRANS, CONV, REACT, VARK, ACCUM, DOT>), dim3((unsigned)nb),
                        dim3(SH::THREADS), 0, h->stream, z, y, fc, a, lm, nb, dotp, stop);

@@ -555,7 +555,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
 // padded stride of 80 doubles does. A 64-lane b64 access is served in two passes of 32
 // lanes (two cells of opposite parity), so the banks of each pass are those of the padded
 // layout, and the 25,600 B of LDS fit six workgroups per CU instead of five.
-template <int N1, typename T, bool TRANS, bool CONV, bool REACT, bool VARK, bool ACCUM, bool DOT>
+template <int N1, typename T, bool TRANS, bool CONV, bool REACT, bool VARK, bool ACCUM, bool DOT, int ZW = 1>
 __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kernel(
     const T* __restrict__ z, T* __restrict__ y, const T* __restrict__ fc, const DGArgs<T> a,
     const LineMats<T>* __restrict__ lm, long long nblocks, T* __restrict__ dotp, const int* __restrict__ stop) {
@@ -574,8 +574,16 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
   const long long blk = xcd_logical_block(blockIdx.x, nblocks);
   const unsigned nzu = (unsigned)a.n[2], nyu = (unsigned)a.n[1];
   const unsigned ub = (unsigned)blk;
-  const unsigned kz = ub % nzu, rest = ub / nzu;
+  // ZW > 1: the workgroup walks ZW consecutive z-chunks (nz % ZW == 0), carrying the
+  // z-lines: a step's own lines are the next step's lower neighbours, its upper neighbours
+  // the next step's own lines
+  const unsigned nzw = nzu / ZW;
+  const unsigned kzw = ub % nzw, rest = ub / nzw;
   const unsigned jy = rest % nyu, xc = rest / nyu;
+  T uz[N1], zl[N1], zr[N1];
+  T pq = T(0);
+  for (int s = 0; s < ZW; ++s) {
+  const unsigned kz = kzw * ZW + s;
   const long long c0 = (long long)xc * CPB + a.stride[1] * jy + a.stride[2] * kz;
   const bool xlo_ok = xc > 0, xhi_ok = (long long)(xc + 1) * CPB < a.n[0];
   const int gx = (int)(xc * CPB) + lc;
@@ -588,15 +596,24 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
   hi[2] = (int)kz < a.n[2] - 1;
   const int off = lc * NL;  // lane's cell block relative to c0
   const T* zc = z + c0 * NL;
-  T uz[N1], zl[N1], zr[N1], yl[N1], yr[N1], xe[N1];
+  T yl[N1], yr[N1], xe[N1];
   T kc = a.kappa0;
   FaceCoef<T> flo[3], fhi[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) flo[d] = fhi[d] = FaceCoef<T>{a.wI_lo, a.wI_hi, a.PI};
   const bool first = lc == 0, last = lc == CPB - 1;
   const bool ld_xe = (first && xlo_ok) || (last && xhi_ok);
-  load_line<N1, 2, T>(zc, off, p, q, uz);
-  if (lo[2]) load_line<N1, 2, T>(zc - a.stride[2] * NL, off, p, q, zl);
+  if (s == 0) {
+    load_line<N1, 2, T>(zc, off, p, q, uz);
+    if (lo[2]) load_line<N1, 2, T>(zc - a.stride[2] * NL, off, p, q, zl);
+  } else {
+#pragma unroll
+    for (int e = 0; e < N1; ++e) {
+      zl[e] = uz[e];
+      uz[e] = zr[e];
+    }
+    __syncthreads();  // the previous step's reads of the LDS array are done
+  }
   if (hi[2]) load_line<N1, 2, T>(zc + a.stride[2] * NL, off, p, q, zr);
   // the y-neighbours' lines are loaded in y-layout (the lane's y-line at x = p, z = q), so
   // the y line operation takes them from registers: 16 fewer LDS accesses per lane than
@@ -700,7 +717,6 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
     }
   }
   __syncthreads();
-  T pq = T(0);
   {
     T v[N1];
 #pragma unroll
@@ -717,6 +733,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
       if (DOT) pq += uz[i] * acc;
     }
   }
+  }  // z-walk step
   if (DOT) {
     // fused z . (J z) for the Krylov step length, one partial per workgroup, reduced
     // through the kernel's own LDS array (a separate array would cost a workgroup per CU)

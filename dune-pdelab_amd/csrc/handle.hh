@@ -98,6 +98,15 @@ using H = pdelab_gpu_gridoperator;
     if (s_ != PDELAB_OK) return s_;  \
   } while (0)
 
+// z-chunks walked by one workgroup of the aligned-row apply: 2 (Q3) or 4 (Q2) for
+// constant-coefficient symmetric operators (measured faster there; slower with cell-wise
+// kappa), else 1. The launch and the count of fused-dot partials both use it.
+inline int apply_walk(const H* h) {
+  if (!h->a64.rowmode || h->apply_variant != 0 || h->vark || h->conv || !h->sym[0] || !h->sym[1]) return 1;
+  const int zw = h->n1 == 4 ? 2 : (h->n1 == 3 ? 4 : 1);
+  return h->a64.n[2] % zw == 0 ? zw : 1;
+}
+
 inline int fail(H* h, int code, const std::string& msg) {
   if (h) h->err = msg;
   return code;

@@ -363,8 +363,9 @@ static long long apply_blocks(H* h) {
   return (h->ncells + cpb - 1) / cpb;
 }
 
-// partials of the fused apply dot: one per apply workgroup
-static long long apply_partials(H* h) { return apply_blocks(h); }
+// partials of the fused apply dot: one per apply workgroup (a walking workgroup covers
+// apply_walk(h) chunks)
+static long long apply_partials(H* h) { return apply_blocks(h) / apply_walk(h); }
 
 // PDELAB_GPU_APPLY=general runs the general apply kernel on aligned-row grids as well
 // (for comparison; the aligned-row kernel is the default there).
@@ -591,7 +592,7 @@ static int cg_loop(H* h, T* x, T* b, T* p, T* q, T* z, const Prec<T>& P, bool tr
                    double red, CgLoopOut& out) {
   const long long n = h->N;
   const long long nbapply = apply_partials(h);
-  if (!h->d_applydot) HIPCHK(h, hipMalloc(&h->d_applydot, sizeof(double) * (size_t)std::max(nbapply, 1LL)));
+  if (!h->d_applydot) HIPCHK(h, hipMalloc(&h->d_applydot, sizeof(double) * (size_t)std::max(apply_blocks(h), 1LL)));
   static_assert(std::is_same<T, A>::value, "the fused apply dot is accumulated in the vector type");
   A* scal = scal_buf<A>(h);
   T* dotp = reinterpret_cast<T*>(h->d_applydot);

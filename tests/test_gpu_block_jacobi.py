@@ -73,7 +73,9 @@ def test_block_jacobi_cg(pkg, of, seed):
 # takes the kernel whose z entries are wave-uniform; the other grids the general one
 GRIDS = [("q3_5x5x3_general", (5, 5, 3), 3), ("q3_4x3x4_uniform_z", (4, 3, 4), 3),
          ("q2_7x3x2_uniform_z", (7, 3, 2), 2), ("q2_4x4x3_general", (4, 4, 3), 2),
-         ("q1_8x2x3_uniform_z", (8, 2, 3), 1), ("q1_5x3x4_general", (5, 3, 4), 1)]
+         ("q1_8x2x3_uniform_z", (8, 2, 3), 1), ("q1_5x3x4_general", (5, 3, 4), 1),
+         # aligned rows whose apply walks 2 (Q3) or 4 (Q2) z-chunks per workgroup
+         ("q3_16x3x4_walk", (16, 3, 4), 3), ("q2_32x2x4_walk", (32, 2, 4), 2)]
 
 
 @pytest.mark.parametrize("name,dims,k", GRIDS, ids=[g[0] for g in GRIDS])

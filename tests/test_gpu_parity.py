@@ -45,6 +45,9 @@ CASES = [
     ("row_q1_64x3x2_var", dict(nx=64, ny=3, nz=2, degree=1, alpha=3.0, b=(0.0, 0.0, 1.0)), 14),
     ("row_q1_128x2x1_off", dict(nx=128, ny=2, nz=1, degree=1, weights=1, alpha=3.0), None),
     ("line_x_q2", dict(nx=7, ny=1, nz=1, degree=2, alpha=3.0), None),
+    # aligned rows with nz a multiple of the z-walk (2 chunks per workgroup at Q3, 4 at Q2)
+    ("walk_q2_32x2x4", dict(nx=32, ny=2, nz=4, degree=2, alpha=3.0), None),
+    ("walk_q3_16x3x6_react", dict(nx=16, ny=3, nz=6, degree=3, alpha=3.0, c=0.5), None),
 ]
 
 
