@@ -8,8 +8,8 @@ alpha 3, FP64, applied to a seeded uniform[-1,1) vector resident in HBM.
 The JSON line also carries:
   roofline      algorithmic bytes per apply (16 B/DoF: read z, write y) over the
                 HIP-event-measured average launch duration of the timed region, against
-                8 TB/s HBM; burst_* give the same for 10 launches after a second of idle
-                (the GPU clock drops under sustained FP64 load);
+                8 TB/s HBM; burst_* give the same for 10 launches after a second of idle and
+                two warm-up launches (the GPU clock drops under sustained FP64 load);
   cpu_baseline  the oracle (CPU restatement of the reference quadrature-loop
                 algorithm) timed on this host's cores on the same apply, and on one core;
                 a projected single-core time of the full CG/Jacobi solve (GPU iteration
@@ -277,7 +277,8 @@ def instationary_leg(pkg, n, k):
     return out
 
 
-APPLY_KERNEL = "dg_apply_rowreg_kernel<4, double, false, false, false, false, false, false>"
+# the 64^3 Q3 FP64 apply y = J z (constant kappa: the z-walk instance, template argument 2)
+APPLY_KERNEL = "dg_apply_rowreg_kernel<4, double, false, false, false, false, false, false, 2>"
 CALIB_KERNEL = "axpy_kernel<double>"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "apply_traffic.json")
 
@@ -388,7 +389,7 @@ def main():
     # a burst of 10 launches after a second of idle, for context: sustained FP64 load lowers
     # the clock, and the timed region below (the reported figure) is the sustained rate
     time.sleep(1.0)
-    burst_ms = go.time_apply(z, y, warmup=0, reps=10)
+    burst_ms = go.time_apply(z, y, warmup=2, reps=10)  # the warm-up launches absorb the wake-up
     go.synchronize()
     t0 = time.perf_counter()
     kern_ms = go.time_apply(z, y, warmup=0, reps=args.steps)
