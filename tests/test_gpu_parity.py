@@ -379,6 +379,60 @@ def test_large_apply_properties(pkg, of):
     assert np.linalg.norm(lin) <= 1e-13 * np.linalg.norm(js.download())
 
 
+def test_large_convection_properties(pkg):
+    """96^3 Q2 convection-diffusion (23.9M DoF, the FP32-inner BiCGSTAB workload): duality
+    <y, J x> == <J^T y, x>, linearity, and the FP32 apply against the FP64 one."""
+    say("[gpu] 96^3 Q2 convection properties")
+    cfg = pkg.make_config(n=96, degree=2, alpha=3.0, kappa0=1e-2, b=(1.0, 0.5, 0.25))
+    go = pkg.GridOperator(cfg)
+    rng = np.random.default_rng(12)
+    x = rng.uniform(-1, 1, go.N)
+    y = rng.uniform(-1, 1, go.N)
+    xd, yd = go.vector(x), go.vector(y)
+    jx, jty = go.vector(), go.vector()
+    go.jacobian_apply(xd, jx)
+    go.jacobian_apply_transpose(yd, jty)
+    a = go.dot(yd, jx)
+    b = go.dot(jty, xd)
+    assert abs(a - b) <= 1e-11 * (abs(a) + abs(b))
+    s = go.vector(x - 0.5 * y)
+    js, jy = go.vector(), go.vector()
+    go.jacobian_apply(s, js)
+    go.jacobian_apply(yd, jy)
+    jxh = jx.download()
+    lin = js.download() - (jxh - 0.5 * jy.download())
+    assert np.linalg.norm(lin) <= 1e-13 * np.linalg.norm(jxh)
+    x32, j32 = go.vector(x.astype(np.float32), dtype=np.float32), go.vector(dtype=np.float32)
+    go.jacobian_apply(x32, j32)
+    assert rel(j32.download().astype(np.float64), jxh) <= 1e-5
+    go.close()
+
+
+def test_large_lognormal_properties(pkg):
+    """64^3 Q3 with cell-wise lognormal kappa (the mixed-precision workload): symmetry and
+    linearity of the SIPG operator at full size."""
+    say("[gpu] 64^3 Q3 lognormal kappa properties")
+    n = 64
+    cfg = pkg.make_config(n=n, degree=3, alpha=3.0, kappa_per_cell=True, problem=pkg.PROBLEM_CONSTANT, f0=1.0, g0=0.0)
+    go = pkg.GridOperator(cfg, np.exp(np.random.default_rng(13).normal(size=n ** 3)))
+    rng = np.random.default_rng(14)
+    x = rng.uniform(-1, 1, go.N)
+    y = rng.uniform(-1, 1, go.N)
+    xd, yd = go.vector(x), go.vector(y)
+    jx, jy = go.vector(), go.vector()
+    go.jacobian_apply(xd, jx)
+    go.jacobian_apply(yd, jy)
+    a = go.dot(yd, jx)
+    b = go.dot(xd, jy)
+    assert abs(a - b) <= 1e-11 * (abs(a) + abs(b))
+    s = go.vector(2.0 * x + y)
+    js = go.vector()
+    go.jacobian_apply(s, js)
+    lin = js.download() - (2.0 * jx.download() + jy.download())
+    assert np.linalg.norm(lin) <= 1e-13 * np.linalg.norm(js.download())
+    go.close()
+
+
 def test_vector_arguments_checked_before_the_call(pkg):
     """The C-ABI takes raw pointers; the binding rejects a vector of the wrong dtype or length
     on the host, before the library could read or write past the buffer."""
