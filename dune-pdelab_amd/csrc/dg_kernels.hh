@@ -662,9 +662,17 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
     load_faces(1);
     load_faces(0);
   }
-  T sx[N1], sy[N1], sz[N1];
+  // the partial sums of z and x go to the YMO / YPO regions as soon as they are formed:
+  // those regions hold nothing of this step before (the step's first barrier ordered the
+  // previous step's reads of them), so the writes need no barrier after the UO reads
+  T sy[N1];
   {
-    line_apply<N1, 2, T, TRANS, CONV, REACT, VARK>(a, lm, uz, zl, zr, lo[2], hi[2], kc, flo[2], fhi[2], sz);
+    {
+      T sz[N1];
+      line_apply<N1, 2, T, TRANS, CONV, REACT, VARK>(a, lm, uz, zl, zr, lo[2], hi[2], kc, flo[2], fhi[2], sz);
+#pragma unroll
+      for (int e = 0; e < N1; ++e) LS[YMO + az[e]] = sz[e];
+    }
     T u[N1], ul[N1], ur[N1];
 #pragma unroll
     for (int e = 0; e < N1; ++e) {
@@ -681,13 +689,10 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
       ul[e] = lo[0] ? LS[UO - NLP + xn] : T(0);
       ur[e] = hi[0] ? LS[UO + NLP + xn] : T(0);
     }
+    T sx[N1];
     line_apply<N1, 0, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, lo[0], hi[0], kc, flo[0], fhi[0], sx);
-  }
-  __syncthreads();  // all reads of the staged cells done
 #pragma unroll
-  for (int e = 0; e < N1; ++e) {
-    LS[YMO + az[e]] = sz[e];
-    LS[YPO + ax[e]] = sx[e];
+    for (int e = 0; e < N1; ++e) LS[YPO + ax[e]] = sx[e];
   }
   __syncthreads();
   {
