@@ -7,7 +7,7 @@
 template <int N1, typename T, bool TRANS, bool CONV, bool REACT, bool VARK, bool ACCUM, bool DOT>
 static int launch_apply_k(H* h, const T* z, T* y, const T* fc, const DGArgs<T>& a, T* dotp, const int* stop) {
   using SH = ApplyShape<N1>;
-  const long long nb = (h->ncells + SH::CPB - 1) / SH::CPB;
+  const long long nb = apply_blocks(h);
   const LineMats<T>* lm = line_mats<T>(h, TRANS);
   const int zw = apply_walk(h);
   if constexpr (!VARK && !CONV && (N1 == 3 || N1 == 4)) {
@@ -20,7 +20,7 @@ static int launch_apply_k(H* h, const T* z, T* y, const T* fc, const DGArgs<T>& 
       return PDELAB_OK;
     }
   }
-  if (a.rowmode && h->apply_variant == 0)
+  if (row_kernel(h))
     hipLaunchKernelGGL((dg_apply_rowreg_kernel<N1, T, TRANS, CONV, REACT, VARK, ACCUM, DOT>), dim3((unsigned)nb),
                        dim3(SH::THREADS), 0, h->stream, z, y, fc, a, lm, nb, dotp, stop);
   else

@@ -52,7 +52,6 @@ struct DGArgs {
   long long stride[3];
   long long ncells;
   int weightsOn;
-  int rowmode;  // nx % CPB == 0: workgroups are aligned x-row chunks, z-fastest order
   int bc[6];
 };
 
@@ -148,7 +147,7 @@ __host__ __device__ constexpr int lds_index(int p, int q, int e) {
 }
 
 // lds_index of every lane of a cell: [layout D][lane l = p + N1 q][element e], read by the
-// aligned-row kernel instead of evaluating the rotation per access.
+// row kernel instead of evaluating the rotation per access.
 template <int N1>
 struct SlotTable {
   int s[3][N1 * N1][4];
@@ -540,10 +539,10 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
   }
 }
 
-// y (+)= J z (or J^T z) on grids with nx % CPB == 0, staged through registers: the CPB
-// cells of a workgroup are one aligned chunk of an x-row (so the y and z boundary flags
-// are uniform over the workgroup), consecutive logical blocks walk z fastest, then y,
-// then the x-chunk. The chunk and its two end cells are written to one LDS array in the
+// y (+)= J z (or J^T z), staged through registers: the CPB cells of a workgroup are one
+// chunk of an x-row (so the y and z boundary flags are uniform over the workgroup; the
+// last chunk of a row is partial when CPB does not divide nx), consecutive logical
+// blocks walk z fastest, then y, then the x-chunk. The chunk and its two end cells are written to one LDS array in the
 // rotated, conflict-free cell layout of lds_index; each lane reads its twelve slot
 // offsets once from SlotTable, so every LDS access is a register plus an immediate
 // offset. The z-neighbour lines (z-layout) and the y-neighbour lines (y-layout) are
@@ -587,6 +586,11 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
   const long long c0 = (long long)xc * CPB + a.stride[1] * jy + a.stride[2] * kz;
   const bool xlo_ok = xc > 0, xhi_ok = (long long)(xc + 1) * CPB < a.n[0];
   const int gx = (int)(xc * CPB) + lc;
+  // a partial last chunk of a row (CPB does not divide nx): lanes past the row end load
+  // the row's last cell (in bounds; the row's own cells never read those slots, as the
+  // last cell has no upper x-neighbour) and store nothing
+  const bool own = gx < a.n[0];
+  const int lcc = own ? lc : a.n[0] - 1 - (int)(xc * CPB);
   bool lo[3], hi[3];
   lo[0] = gx > 0;
   hi[0] = gx < a.n[0] - 1;
@@ -594,7 +598,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
   hi[1] = (int)jy < a.n[1] - 1;
   lo[2] = kz > 0;
   hi[2] = (int)kz < a.n[2] - 1;
-  const int off = lc * NL;  // lane's cell block relative to c0
+  const int off = lcc * NL;  // lane's cell block relative to c0
   const T* zc = z + c0 * NL;
   T yl[N1], yr[N1], xe[N1];
   T kc = a.kappa0;
@@ -623,7 +627,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
   if (ld_xe) load_line<N1, 2, T>(zc + (first ? -NL : NL), off, p, q, xe);
   // cell-wise kappa: the cell's kappa and the z faces now, the y and x faces after the
   // staging barrier (fewer registers live across the loads and the staging)
-  const T* fb = fc + (c0 + lc) * FC;
+  const T* fb = fc + (c0 + lcc) * FC;
   auto load_faces = [&](int d) {
     if (lo[d]) flo[d] = FaceCoef<T>{fb[1 + 3 * d], fb[2 + 3 * d], fb[3 + 3 * d]};
     if (hi[d]) {
@@ -733,9 +737,11 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
 #pragma unroll
       for (int j = 0; j < N1; ++j) acc += a.M[i][j] * v[j];
       const int idx = off + line_index<N1, 2>(p, q, i);
-      if (ACCUM) yc[idx] += acc;
-      else yc[idx] = acc;
-      if (DOT) pq += uz[i] * acc;
+      if (own) {
+        if (ACCUM) yc[idx] += acc;
+        else yc[idx] = acc;
+        if (DOT) pq += uz[i] * acc;
+      }
     }
   }
   }  // z-walk step

@@ -71,7 +71,7 @@ struct pdelab_gpu_gridoperator {
   LineMats<double>* d_lm64 = nullptr;
   LineMats<float>* d_lm32 = nullptr;
   bool sym[2] = {false, false};  // J / J^T have the structure of line_op_sym
-  // apply kernel on aligned-row grids: 0 the aligned-row kernel, 1 the general kernel
+  // apply kernel: 0 the row kernel (x-row chunks), 1 the general kernel (chunks of consecutive cells)
   int apply_variant = 0;
   BlockJacobiData bj[2];  // [0] J, [1] J^T
   // reaction shift of the one-step stage operators: the kernels apply J + shift M
@@ -98,11 +98,39 @@ using H = pdelab_gpu_gridoperator;
     if (s_ != PDELAB_OK) return s_;  \
   } while (0)
 
-// z-chunks walked by one workgroup of the aligned-row apply: 2 (Q3) or 4 (Q2) for
+inline long long apply_cpb(const H* h) {
+  return h->n1 == 2 ? ApplyShape<2>::CPB : (h->n1 == 3 ? ApplyShape<3>::CPB : ApplyShape<4>::CPB);
+}
+
+// Whether an apply launch takes the row kernel (chunks of x-rows) rather than the general
+// kernel (chunks of consecutive cells). Rows that split into whole chunks always do. A
+// partial last chunk leaves lanes idle, so such grids do only for the constant-coefficient
+// symmetric operator, where the row kernel is fastest (0.78 of the general kernel's time
+// at Q3, 0.90 at Q2), with at most 1/8 (Q3) or 1/16 (Q2) of the row's lanes idle. Decided
+// per launch: set_kappa may change the operator after creation.
+inline bool row_kernel(const H* h) {
+  if (h->apply_variant != 0) return false;
+  const long long cpb = apply_cpb(h), nx = h->a64.n[0];
+  const long long span = (nx + cpb - 1) / cpb * cpb;
+  if (span == nx) return true;
+  if (h->n1 == 2 || h->vark || h->conv || !h->sym[0] || !h->sym[1]) return false;
+  return (span - nx) * (h->n1 == 4 ? 8 : 16) <= span;
+}
+
+// Chunks of one apply launch: the row kernel takes ceil(nx / CPB) chunks of every x-row,
+// the general kernel ceil(ncells / CPB) chunks of consecutive cells.
+inline long long apply_blocks_of(const H* h, bool row) {
+  const long long cpb = apply_cpb(h);
+  if (!row) return (h->ncells + cpb - 1) / cpb;
+  return (h->a64.n[0] + cpb - 1) / cpb * h->a64.n[1] * h->a64.n[2];
+}
+inline long long apply_blocks(const H* h) { return apply_blocks_of(h, row_kernel(h)); }
+
+// z-chunks walked by one workgroup of the row apply: 2 (Q3) or 4 (Q2) for
 // constant-coefficient symmetric operators (measured faster there; slower with cell-wise
 // kappa), else 1. The launch and the count of fused-dot partials both use it.
 inline int apply_walk(const H* h) {
-  if (!h->a64.rowmode || h->apply_variant != 0 || h->vark || h->conv || !h->sym[0] || !h->sym[1]) return 1;
+  if (!row_kernel(h) || h->vark || h->conv || !h->sym[0] || !h->sym[1]) return 1;
   const int zw = h->n1 == 4 ? 2 : (h->n1 == 3 ? 4 : 1);
   return h->a64.n[2] % zw == 0 ? zw : 1;
 }

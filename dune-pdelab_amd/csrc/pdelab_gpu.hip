@@ -100,8 +100,6 @@ static void fill_args(H* h, DGArgs<T>& a) {
   a.stride[1] = c.nx;
   a.stride[2] = (long long)c.nx * c.ny;
   a.ncells = h->ncells;
-  const int cpb = c.degree == 1 ? ApplyShape<2>::CPB : (c.degree == 2 ? ApplyShape<3>::CPB : ApplyShape<4>::CPB);
-  a.rowmode = (c.nx % cpb == 0) ? 1 : 0;
   for (int s = 0; s < 6; ++s) a.bc[s] = c.bctype[s] == 0 ? BC_DIRICHLET : c.bctype[s];
 }
 
@@ -352,23 +350,12 @@ template <typename T>
 static int apply_t(H* h, const T* z, T* y, bool trans, bool accum) {
   return apply_mode<T>(h, z, y, trans, accum ? 1 : 0);
 }
-// number of workgroups of one apply launch (= fused-dot partials)
-static int apply_cpb(int n1) {
-  return n1 == 2 ? ApplyShape<2>::CPB : (n1 == 3 ? ApplyShape<3>::CPB : ApplyShape<4>::CPB);
-}
-
-// number of workgroups (= fused-dot partials) of one apply launch
-static long long apply_blocks(H* h) {
-  const int cpb = apply_cpb(h->n1);
-  return (h->ncells + cpb - 1) / cpb;
-}
-
 // partials of the fused apply dot: one per apply workgroup (a walking workgroup covers
 // apply_walk(h) chunks)
 static long long apply_partials(H* h) { return apply_blocks(h) / apply_walk(h); }
 
-// PDELAB_GPU_APPLY=general runs the general apply kernel on aligned-row grids as well
-// (for comparison; the aligned-row kernel is the default there).
+// PDELAB_GPU_APPLY=general runs the general apply kernel on every grid (for comparison;
+// by default row_kernel() chooses per launch).
 static void choose_apply_shape(H* h) {
   const char* v = std::getenv("PDELAB_GPU_APPLY");
   h->apply_variant = (v && std::strcmp(v, "general") == 0) ? 1 : 0;
@@ -592,7 +579,11 @@ static int cg_loop(H* h, T* x, T* b, T* p, T* q, T* z, const Prec<T>& P, bool tr
                    double red, CgLoopOut& out) {
   const long long n = h->N;
   const long long nbapply = apply_partials(h);
-  if (!h->d_applydot) HIPCHK(h, hipMalloc(&h->d_applydot, sizeof(double) * (size_t)std::max(apply_blocks(h), 1LL)));
+  // sized for either kernel: the choice can change with set_kappa
+  if (!h->d_applydot) {
+    const long long nmax = std::max(std::max(apply_blocks_of(h, true), apply_blocks_of(h, false)), 1LL);
+    HIPCHK(h, hipMalloc(&h->d_applydot, sizeof(double) * (size_t)nmax));
+  }
   static_assert(std::is_same<T, A>::value, "the fused apply dot is accumulated in the vector type");
   A* scal = scal_buf<A>(h);
   T* dotp = reinterpret_cast<T*>(h->d_applydot);

@@ -37,7 +37,7 @@ CASES = [
     ("conv_q3_var", dict(nx=3, ny=3, nz=3, degree=3, alpha=3.0, b=(-0.3, 0.0, 2.0)), 7),
     ("outflow_q2", dict(nx=3, ny=2, nz=3, degree=2, alpha=3.0, b=(1.0, 0.0, 0.0), bctype=(0, -2, 0, -1, -3, 0)), None),
     ("single_cell_q3", dict(nx=1, ny=1, nz=1, degree=3, alpha=3.0), None),
-    # nx a multiple of the workgroup chunk (16 cells Q3, 32 Q2, 64 Q1): aligned-row kernel
+    # nx a multiple of the workgroup chunk (16 cells Q3, 32 Q2, 64 Q1): full row chunks only
     ("row_q3_32x3x3_var_conv", dict(nx=32, ny=3, nz=3, degree=3, alpha=3.0, b=(0.5, -1.0, 0.25)), 12),
     ("row_q3_16x2x4", dict(nx=16, ny=2, nz=4, degree=3, alpha=3.0, c=0.3), None),
     ("row_q2_32x3x2_var", dict(nx=32, ny=3, nz=2, degree=2, alpha=3.0), 13),
@@ -99,9 +99,10 @@ def test_apply_diag_transpose_mass(pkg, of, name, kw, seed):
     assert rel(y32.download().astype(np.float64), o.jacobian_apply(z)) <= 2e-5
 
 
-# Aligned-row grids (nx a multiple of the workgroup chunk: 16 cells Q3, 32 Q2, 64 Q1)
-# through the aligned-row kernel (default) and through the general kernel. SPD cases also
-# run CG, whose iterations use the apply with the fused z.(Jz) partials.
+# Grids through the row kernel (default) and through the general kernel: aligned rows (nx a
+# multiple of the workgroup chunk: 16 cells Q3, 32 Q2, 64 Q1) and rows whose last chunk is
+# partial. SPD cases also run CG, whose iterations use the apply with the fused z.(Jz)
+# partials.
 ALIGNED = [
     ("q3_16x3x5_var_conv", dict(nx=16, ny=3, nz=5, degree=3, alpha=3.0, b=(0.5, -1.0, 0.25), c=0.2), 21),
     ("q3_16x2x6_react", dict(nx=16, ny=2, nz=6, degree=3, alpha=3.0, c=0.5), None),
@@ -116,6 +117,18 @@ ALIGNED = [
     ("q1_64x1x1_off", dict(nx=64, ny=1, nz=1, degree=1, weights=1, alpha=3.0), None),
     ("q1_64x2x3_react_outflow", dict(nx=64, ny=2, nz=3, degree=1, alpha=3.0, c=0.4, b=(1.0, 0.0, 0.0),
                                      bctype=(0, -2, 0, 0, -1, 0)), None),
+    # partial last chunks. The row kernel takes them for constant kappa without convection
+    # when at most 1/8 (Q3) or 1/16 (Q2) of the row's lanes idle (nz % 2 at Q3, nz % 4 at Q2
+    # adds the z-walk); the other grids go to the general kernel in both runs
+    ("ragged_q3_15x3x3", dict(nx=15, ny=3, nz=3, degree=3, alpha=3.0), None),
+    ("ragged_q3_30x2x4_walk", dict(nx=30, ny=2, nz=4, degree=3, alpha=3.0), None),
+    ("ragged_q3_46x2x2_walk_react", dict(nx=46, ny=2, nz=2, degree=3, alpha=3.0, c=0.3), None),
+    ("ragged_q2_31x2x4_walk", dict(nx=31, ny=2, nz=4, degree=2, alpha=3.0), None),
+    ("ragged_q2_62x2x3", dict(nx=62, ny=2, nz=3, degree=2, alpha=3.0), None),
+    ("ragged_q3_5x3x4_react_general", dict(nx=5, ny=3, nz=4, degree=3, alpha=3.0, c=0.3), None),
+    ("ragged_q3_17x2x3_var_conv_general", dict(nx=17, ny=2, nz=3, degree=3, alpha=3.0, b=(0.5, -1.0, 0.25)), 25),
+    ("ragged_q2_31x3x3_var_general", dict(nx=31, ny=3, nz=3, degree=2, alpha=3.0), 26),
+    ("ragged_q1_70x2x2_general", dict(nx=70, ny=2, nz=2, degree=1, alpha=3.0), None),
 ]
 VARIANTS = [(f"{k}_{name}", kw, seed, k) for name, kw, seed in ALIGNED for k in ("aligned", "general")]
 
