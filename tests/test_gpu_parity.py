@@ -162,6 +162,32 @@ def test_apply_kernel_variants(pkg, of, monkeypatch, name, kw, seed, variant):
         assert rel(o.jacobian_apply(x.download()), rhs) <= 1e-9
 
 
+def test_apply_kernel_follows_set_kappa(pkg, of):
+    """A partial-chunk grid with constant kappa takes the row kernel with the z-walk (12
+    fused-dot partials); set_kappa makes the operator cell-wise, which sends the same grid
+    to the general kernel (23 partials, more than the first solve's walk needed). Both
+    CG solves (block Jacobi) follow the oracle. The iteration counts may differ by 1% on
+    the lognormal operator, whose CG iteration count moves with rounding at the 1e-12 level
+    (point Jacobi there: 864 against the oracle's 858 over ~860 iterations)."""
+    cfg = pkg.make_config(nx=30, ny=3, nz=4, degree=3, alpha=3.0, problem=pkg.PROBLEM_EXP)
+    go, o = pkg.GridOperator(cfg), of.Oracle(cfg)
+    o.set_threads(8)
+    k = np.exp(np.random.default_rng(27).normal(size=go.ncells))
+    for step in ("constant kappa", "cell-wise kappa"):
+        say(f"[gpu] apply kernel choice 30x3x4 Q3, {step}")
+        xd = go.vector()
+        res = go.linear_problem_solve(xd, method=0, precond=2, reduction=1e-10)
+        assert res["solver"]["converged"]
+        x_ref, res_ref = o.linear_problem_solve(method=0, precond=2, reduction=1e-10)
+        it, it_ref = int(res["solver"]["iterations"]), int(res_ref["solver"]["iterations"])
+        assert abs(it - it_ref) <= max(2, 0.01 * it_ref), (step, it, it_ref)
+        x = xd.download()
+        assert rel(x, x_ref) <= 1e-7
+        assert np.linalg.norm(o.residual(x)) <= 1.01e-10 * res["first_defect"]
+        go.set_kappa(k)
+        o.set_kappa(k)
+
+
 PROBLEMS = [
     ("exp_q2", dict(n=4, degree=2, alpha=3.0), None, 1),
     ("quadratic_q3_var", dict(nx=3, ny=2, nz=4, degree=3, alpha=3.0, quad=(0.3, 1.0, 2.0, -1.0, 1.0, 1.0, -2.0)), 9, 4),
