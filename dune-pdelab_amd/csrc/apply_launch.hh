@@ -4,21 +4,29 @@
 
 #include "handle.hh"
 
+// the row kernel walking ZW z-chunks per workgroup (nz % ZW == 0, checked by apply_walk)
+template <int N1, typename T, bool TRANS, bool REACT, bool ACCUM, bool DOT, int ZW>
+static int launch_apply_walk(H* h, const T* z, T* y, const T* fc, const DGArgs<T>& a, const LineMats<T>* lm,
+                             long long nb, T* dotp, const int* stop) {
+  hipLaunchKernelGGL((dg_apply_rowreg_kernel<N1, T, TRANS, false, REACT, false, ACCUM, DOT, ZW>),
+                     dim3((unsigned)(nb / ZW)), dim3(ApplyShape<N1>::THREADS), 0, h->stream, z, y, fc, a, lm, nb / ZW,
+                     dotp, stop);
+  HIPCHK(h, hipGetLastError());
+  return PDELAB_OK;
+}
+
 template <int N1, typename T, bool TRANS, bool CONV, bool REACT, bool VARK, bool ACCUM, bool DOT>
 static int launch_apply_k(H* h, const T* z, T* y, const T* fc, const DGArgs<T>& a, T* dotp, const int* stop) {
   using SH = ApplyShape<N1>;
   const long long nb = apply_blocks(h);
   const LineMats<T>* lm = line_mats<T>(h, TRANS);
   const int zw = apply_walk(h);
-  if constexpr (!VARK && !CONV && (N1 == 3 || N1 == 4)) {
-    constexpr int ZW = N1 == 4 ? 2 : 4;
-    if (zw == ZW) {
-      hipLaunchKernelGGL((dg_apply_rowreg_kernel<N1, T, TRANS, CONV, REACT, VARK, ACCUM, DOT, ZW>),
-                         dim3((unsigned)(nb / ZW)), dim3(SH::THREADS), 0, h->stream, z, y, fc, a, lm, nb / ZW, dotp,
-                         stop);
-      HIPCHK(h, hipGetLastError());
-      return PDELAB_OK;
-    }
+  if constexpr (!VARK && !CONV && N1 == 4) {
+    if (zw == 2) return launch_apply_walk<N1, T, TRANS, REACT, ACCUM, DOT, 2>(h, z, y, fc, a, lm, nb, dotp, stop);
+  }
+  if constexpr (!VARK && !CONV && N1 == 3) {
+    if (zw == 8) return launch_apply_walk<N1, T, TRANS, REACT, ACCUM, DOT, 8>(h, z, y, fc, a, lm, nb, dotp, stop);
+    if (zw == 4) return launch_apply_walk<N1, T, TRANS, REACT, ACCUM, DOT, 4>(h, z, y, fc, a, lm, nb, dotp, stop);
   }
   if (row_kernel(h))
     hipLaunchKernelGGL((dg_apply_rowreg_kernel<N1, T, TRANS, CONV, REACT, VARK, ACCUM, DOT>), dim3((unsigned)nb),

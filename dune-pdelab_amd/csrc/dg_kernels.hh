@@ -579,18 +579,20 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
   const unsigned nzw = nzu / ZW;
   const unsigned kzw = ub % nzw, rest = ub / nzw;
   const unsigned jy = rest % nyu, xc = rest / nyu;
+  // x0: first cell of the chunk. A partial last chunk of a row (CPB does not divide nx;
+  // the launch ensures nx > CPB) is moved back to end at the row end, so every lane has a
+  // cell; its lanes over cells of the previous chunk store nothing. All of it is uniform
+  // over the workgroup, and full chunks store unmasked.
+  const int xs = (int)(xc * CPB);
+  const bool full = xs + CPB <= a.n[0];
+  const int x0 = full ? xs : a.n[0] - CPB;
   T uz[N1], zl[N1], zr[N1];
   T pq = T(0);
   for (int s = 0; s < ZW; ++s) {
   const unsigned kz = kzw * ZW + s;
-  const long long c0 = (long long)xc * CPB + a.stride[1] * jy + a.stride[2] * kz;
-  const bool xlo_ok = xc > 0, xhi_ok = (long long)(xc + 1) * CPB < a.n[0];
-  const int gx = (int)(xc * CPB) + lc;
-  // a partial last chunk of a row (CPB does not divide nx): lanes past the row end load
-  // the row's last cell (in bounds; the row's own cells never read those slots, as the
-  // last cell has no upper x-neighbour) and store nothing
-  const bool own = gx < a.n[0];
-  const int lcc = own ? lc : a.n[0] - 1 - (int)(xc * CPB);
+  const long long c0 = (long long)x0 + a.stride[1] * jy + a.stride[2] * kz;
+  const bool xlo_ok = x0 > 0, xhi_ok = x0 + CPB < a.n[0];
+  const int gx = x0 + lc;
   bool lo[3], hi[3];
   lo[0] = gx > 0;
   hi[0] = gx < a.n[0] - 1;
@@ -598,7 +600,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
   hi[1] = (int)jy < a.n[1] - 1;
   lo[2] = kz > 0;
   hi[2] = (int)kz < a.n[2] - 1;
-  const int off = lcc * NL;  // lane's cell block relative to c0
+  const int off = lc * NL;  // lane's cell block relative to c0
   const T* zc = z + c0 * NL;
   T yl[N1], yr[N1], xe[N1];
   T kc = a.kappa0;
@@ -627,7 +629,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
   if (ld_xe) load_line<N1, 2, T>(zc + (first ? -NL : NL), off, p, q, xe);
   // cell-wise kappa: the cell's kappa and the z faces now, the y and x faces after the
   // staging barrier (fewer registers live across the loads and the staging)
-  const T* fb = fc + (c0 + lcc) * FC;
+  const T* fb = fc + (c0 + lc) * FC;
   auto load_faces = [&](int d) {
     if (lo[d]) flo[d] = FaceCoef<T>{fb[1 + 3 * d], fb[2 + 3 * d], fb[3 + 3 * d]};
     if (hi[d]) {
@@ -731,18 +733,22 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
 #pragma unroll
     for (int e = 0; e < N1; ++e) v[e] = LS[YMO + az[e]];
     T* yc = y + c0 * NL;
+    auto store = [&](bool own) {
 #pragma unroll
-    for (int i = 0; i < N1; ++i) {
-      T acc = T(0);
+      for (int i = 0; i < N1; ++i) {
+        T acc = T(0);
 #pragma unroll
-      for (int j = 0; j < N1; ++j) acc += a.M[i][j] * v[j];
-      const int idx = off + line_index<N1, 2>(p, q, i);
-      if (own) {
-        if (ACCUM) yc[idx] += acc;
-        else yc[idx] = acc;
-        if (DOT) pq += uz[i] * acc;
+        for (int j = 0; j < N1; ++j) acc += a.M[i][j] * v[j];
+        const int idx = off + line_index<N1, 2>(p, q, i);
+        if (own) {
+          if (ACCUM) yc[idx] += acc;
+          else yc[idx] = acc;
+          if (DOT) pq += uz[i] * acc;
+        }
       }
-    }
+    };
+    if (full) store(true);
+    else store(gx >= xs);
   }
   }  // z-walk step
   if (DOT) {

@@ -103,17 +103,18 @@ inline long long apply_cpb(const H* h) {
 }
 
 // Whether an apply launch takes the row kernel (chunks of x-rows) rather than the general
-// kernel (chunks of consecutive cells). Rows that split into whole chunks always do. A
-// partial last chunk leaves lanes idle, so such grids do only for the constant-coefficient
-// symmetric operator, where the row kernel is fastest (0.78 of the general kernel's time
-// at Q3, 0.90 at Q2), with at most 1/8 (Q3) or 1/16 (Q2) of the row's lanes idle. Decided
-// per launch: set_kappa may change the operator after creation.
+// kernel (chunks of consecutive cells). Rows that split into whole chunks always do. The
+// partial last chunk of a row is moved back to overlap the previous one (nx > CPB), and
+// its overlapping lanes do work that is not stored, so such grids take the row kernel only
+// for the constant-coefficient symmetric operator, where it is fastest (0.78 of the
+// general kernel's time at Q3, 0.90 at Q2), with at most 1/8 (Q3) or 1/16 (Q2) of the
+// row's lanes wasted. Decided per launch: set_kappa may change the operator after creation.
 inline bool row_kernel(const H* h) {
   if (h->apply_variant != 0) return false;
   const long long cpb = apply_cpb(h), nx = h->a64.n[0];
   const long long span = (nx + cpb - 1) / cpb * cpb;
   if (span == nx) return true;
-  if (h->n1 == 2 || h->vark || h->conv || !h->sym[0] || !h->sym[1]) return false;
+  if (nx < cpb || h->n1 == 2 || h->vark || h->conv || !h->sym[0] || !h->sym[1]) return false;
   return (span - nx) * (h->n1 == 4 ? 8 : 16) <= span;
 }
 
@@ -126,13 +127,16 @@ inline long long apply_blocks_of(const H* h, bool row) {
 }
 inline long long apply_blocks(const H* h) { return apply_blocks_of(h, row_kernel(h)); }
 
-// z-chunks walked by one workgroup of the row apply: 2 (Q3) or 4 (Q2) for
-// constant-coefficient symmetric operators (measured faster there; slower with cell-wise
-// kappa), else 1. The launch and the count of fused-dot partials both use it.
+// z-chunks walked by one workgroup of the row apply, for constant-coefficient symmetric
+// operators (measured faster there; slower with cell-wise kappa): 2 at Q3; 8 at Q2, or 4
+// where nz is a multiple of 4 but not of 8; else 1. The launch and the count of fused-dot
+// partials both use it.
 inline int apply_walk(const H* h) {
   if (!row_kernel(h) || h->vark || h->conv || !h->sym[0] || !h->sym[1]) return 1;
-  const int zw = h->n1 == 4 ? 2 : (h->n1 == 3 ? 4 : 1);
-  return h->a64.n[2] % zw == 0 ? zw : 1;
+  const long long nz = h->a64.n[2];
+  if (h->n1 == 4) return nz % 2 == 0 ? 2 : 1;
+  if (h->n1 == 3) return nz % 8 == 0 ? 8 : (nz % 4 == 0 ? 4 : 1);
+  return 1;
 }
 
 inline int fail(H* h, int code, const std::string& msg) {

@@ -74,10 +74,11 @@ def test_block_jacobi_cg(pkg, of, seed):
 GRIDS = [("q3_5x5x3_general", (5, 5, 3), 3), ("q3_4x3x4_uniform_z", (4, 3, 4), 3),
          ("q2_7x3x2_uniform_z", (7, 3, 2), 2), ("q2_4x4x3_general", (4, 4, 3), 2),
          ("q1_8x2x3_uniform_z", (8, 2, 3), 1), ("q1_5x3x4_general", (5, 3, 4), 1),
-         # aligned rows whose apply walks 2 (Q3) or 4 (Q2) z-chunks per workgroup
+         # aligned rows whose apply walks 2 (Q3) or 4 or 8 (Q2) z-chunks per workgroup
          ("q3_16x3x4_walk", (16, 3, 4), 3), ("q2_32x2x4_walk", (32, 2, 4), 2),
          # and rows whose last chunk is partial (row kernel), with and without the walk
-         ("q3_30x3x4_walk_ragged", (30, 3, 4), 3), ("q2_62x2x3_ragged", (62, 2, 3), 2)]
+         ("q3_30x3x4_walk_ragged", (30, 3, 4), 3), ("q2_62x2x3_ragged", (62, 2, 3), 2),
+         ("q2_32x2x8_walk8", (32, 2, 8), 2)]
 
 
 @pytest.mark.parametrize("name,dims,k", GRIDS, ids=[g[0] for g in GRIDS])

@@ -45,9 +45,10 @@ CASES = [
     ("row_q1_64x3x2_var", dict(nx=64, ny=3, nz=2, degree=1, alpha=3.0, b=(0.0, 0.0, 1.0)), 14),
     ("row_q1_128x2x1_off", dict(nx=128, ny=2, nz=1, degree=1, weights=1, alpha=3.0), None),
     ("line_x_q2", dict(nx=7, ny=1, nz=1, degree=2, alpha=3.0), None),
-    # aligned rows with nz a multiple of the z-walk (2 chunks per workgroup at Q3, 4 at Q2)
+    # aligned rows with nz a multiple of the z-walk (2 chunks per workgroup at Q3, 8 or 4 at Q2)
     ("walk_q2_32x2x4", dict(nx=32, ny=2, nz=4, degree=2, alpha=3.0), None),
     ("walk_q3_16x3x6_react", dict(nx=16, ny=3, nz=6, degree=3, alpha=3.0, c=0.5), None),
+    ("walk8_q2_32x2x8_react", dict(nx=32, ny=2, nz=8, degree=2, alpha=3.0, c=0.4), None),
 ]
 
 
@@ -117,14 +118,15 @@ ALIGNED = [
     ("q1_64x1x1_off", dict(nx=64, ny=1, nz=1, degree=1, weights=1, alpha=3.0), None),
     ("q1_64x2x3_react_outflow", dict(nx=64, ny=2, nz=3, degree=1, alpha=3.0, c=0.4, b=(1.0, 0.0, 0.0),
                                      bctype=(0, -2, 0, 0, -1, 0)), None),
-    # partial last chunks. The row kernel takes them for constant kappa without convection
-    # when at most 1/8 (Q3) or 1/16 (Q2) of the row's lanes idle (nz % 2 at Q3, nz % 4 at Q2
-    # adds the z-walk); the other grids go to the general kernel in both runs
-    ("ragged_q3_15x3x3", dict(nx=15, ny=3, nz=3, degree=3, alpha=3.0), None),
+    # partial last chunks. The row kernel takes them (nx > chunk) for constant kappa without
+    # convection when at most 1/8 (Q3) or 1/16 (Q2) of the row's lanes are wasted (nz % 2 at Q3, nz % 4 or
+    # nz % 8 at Q2 adds the z-walk); the other grids go to the general kernel in both runs
+    ("ragged_q3_31x3x3", dict(nx=31, ny=3, nz=3, degree=3, alpha=3.0), None),
     ("ragged_q3_30x2x4_walk", dict(nx=30, ny=2, nz=4, degree=3, alpha=3.0), None),
     ("ragged_q3_46x2x2_walk_react", dict(nx=46, ny=2, nz=2, degree=3, alpha=3.0, c=0.3), None),
-    ("ragged_q2_31x2x4_walk", dict(nx=31, ny=2, nz=4, degree=2, alpha=3.0), None),
+    ("ragged_q2_62x2x4_walk", dict(nx=62, ny=2, nz=4, degree=2, alpha=3.0), None),
     ("ragged_q2_62x2x3", dict(nx=62, ny=2, nz=3, degree=2, alpha=3.0), None),
+    ("ragged_q2_94x2x8_walk8", dict(nx=94, ny=2, nz=8, degree=2, alpha=3.0), None),
     ("ragged_q3_5x3x4_react_general", dict(nx=5, ny=3, nz=4, degree=3, alpha=3.0, c=0.3), None),
     ("ragged_q3_17x2x3_var_conv_general", dict(nx=17, ny=2, nz=3, degree=3, alpha=3.0, b=(0.5, -1.0, 0.25)), 25),
     ("ragged_q2_31x3x3_var_general", dict(nx=31, ny=3, nz=3, degree=2, alpha=3.0), 26),
