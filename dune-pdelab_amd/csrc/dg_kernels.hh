@@ -540,12 +540,12 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
 }
 
 // y (+)= J z (or J^T z), staged through registers: the CPB cells of a workgroup are one
-// chunk of an x-row (so the y and z boundary flags are uniform over the workgroup; the
-// last chunk of a row is partial when CPB does not divide nx), consecutive logical
-// blocks walk z fastest, then y, then the x-chunk. The chunk and its two end cells are written to one LDS array in the
-// rotated, conflict-free cell layout of lds_index; each lane reads its twelve slot
-// offsets once from SlotTable, so every LDS access is a register plus an immediate
-// offset. The z-neighbour lines (z-layout) and the y-neighbour lines (y-layout) are
+// chunk of an x-row (so the y and z boundary flags are uniform over the workgroup; when
+// CPB does not divide nx, the last chunk of a row is moved back to end at the row end),
+// consecutive logical blocks walk z fastest, then y, then the x-chunk. The chunk and its
+// two end cells are written to one LDS array in the rotated, conflict-free cell layout
+// of lds_index; each lane reads its twelve slot offsets once from SlotTable, so every
+// LDS access is a register plus an immediate offset. The z-neighbour lines (z-layout) and the y-neighbour lines (y-layout) are
 // loaded straight into registers; the regions after the chunk hold the partial sums.
 //
 // FP64 Q3 stores the cells unpadded (64 doubles) and separates neighbouring cells' banks
