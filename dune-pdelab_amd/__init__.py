@@ -14,6 +14,7 @@ Import it under the name ``dune_pdelab_amd`` (the directory name has a hyphen)::
 import ctypes
 import os
 import subprocess
+import weakref
 
 import numpy as np
 
@@ -119,6 +120,7 @@ class DeviceVector:
         ptr = _vp()
         go._check(lib().pdelab_gpu_malloc(go.h, ctypes.byref(ptr), self.nbytes))
         self.ptr = ptr
+        go._children.add(self)  # close() frees the buffers still alive
         self.zero()
 
     @property
@@ -168,7 +170,7 @@ class GridOperator:
     def __init__(self, cfg, kappa=None):
         L = lib()
         self.cfg = cfg
-        self._children = []
+        self._children = weakref.WeakSet()  # live DeviceVectors of this handle
         kp = None
         if kappa is not None:
             self._kappa = np.ascontiguousarray(kappa, dtype=np.float64)
@@ -182,7 +184,11 @@ class GridOperator:
         self.ncells = L.pdelab_gpu_num_cells(h)
 
     def close(self):
+        """Frees the device buffers of the vectors still alive, then the handle
+        (pdelab_gpu_destroy does not free user allocations)."""
         if getattr(self, "h", None):
+            for v in list(self._children):
+                v.free()
             lib().pdelab_gpu_destroy(self.h)
             self.h = None
 

@@ -23,9 +23,13 @@ torch::Tensor solve_forward(pdelab_gpu_gridoperator* h, const torch::Tensor& log
   TORCH_CHECK(logk.is_cuda() && logk.scalar_type() == torch::kFloat64 && logk.is_contiguous(),
               "log kappa must be a contiguous float64 GPU tensor");
   TORCH_CHECK((size_t)logk.numel() == pdelab_gpu_num_cells(h), "log kappa must have one entry per cell");
+  // The library runs on its own stream, which does not wait for torch's: torch's work on
+  // the inputs is finished here, and every buffer the library reads is allocated without a
+  // torch kernel (empty) and cleared on the library's stream.
   torch::cuda::synchronize();
   check(pdelab_gpu_set_log_kappa_device(h, logk.data_ptr<double>()), h);
-  auto u = torch::zeros({(int64_t)pdelab_gpu_size(h)}, logk.options());
+  auto u = torch::empty({(int64_t)pdelab_gpu_size(h)}, logk.options());
+  check(pdelab_gpu_memset_zero(h, u.data_ptr<double>(), sizeof(double) * (size_t)u.numel()), h);
   pdelab_linear_problem_result res;
   check(pdelab_gpu_linear_problem_solve(h, PDELAB_SOLVER_CG, (int)precond, reduction, 1e-99, (int)maxit,
                                         u.data_ptr<double>(), &res),
@@ -59,16 +63,18 @@ class DGSolve : public torch::autograd::Function<DGSolve> {
     auto u = solve_forward(h, logk, reduction, maxit, precond);
     // adjoint solve J^T lambda = dL/du
     auto rhs = grads[0].contiguous().to(torch::kFloat64).clone();
-    auto lam = torch::zeros_like(rhs);
-    torch::cuda::synchronize();
+    auto lam = torch::empty_like(rhs);
+    auto g = torch::empty_like(logk);
+    torch::cuda::synchronize();  // rhs is complete before the library's stream reads it
+    check(pdelab_gpu_memset_zero(h, lam.data_ptr<double>(), sizeof(double) * (size_t)lam.numel()), h);
+    check(pdelab_gpu_memset_zero(h, g.data_ptr<double>(), sizeof(double) * (size_t)g.numel()), h);
     pdelab_linear_solver_result res;
     check(pdelab_gpu_solve_transpose(h, PDELAB_SOLVER_CG, (int)precond, reduction, (int)maxit,
                                      lam.data_ptr<double>(), rhs.data_ptr<double>(), &res),
           h);
     TORCH_CHECK(res.converged, "adjoint solve did not converge");
-    auto g = torch::zeros_like(logk);
     check(pdelab_gpu_grad_logkappa(h, u.data_ptr<double>(), lam.data_ptr<double>(), g.data_ptr<double>()), h);
-    check(pdelab_gpu_synchronize(h), h);
+    check(pdelab_gpu_synchronize(h), h);  // g is complete before torch's stream uses it
     return {g, torch::Tensor(), torch::Tensor(), torch::Tensor(), torch::Tensor()};
   }
 };

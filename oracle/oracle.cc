@@ -117,6 +117,88 @@ struct Oracle {
     }
   }
 
+  // r += R(x) with two-sided skeleton visits, parallel over cells: the residual engine's
+  // terms of residual() (lambda_volume, alpha_volume, alpha_skeleton, boundary with the
+  // data terms), each cell writing only its own block. Same operator as residual() up to
+  // rounding (pinned against it); it makes full-size checks affordable on the host.
+  void residual_twosided(const double* x, double* r, int nthreads) const {
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel num_threads(nthreads)
+#endif
+    {
+      std::vector<double> rl(nloc), dummy(nloc);
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+      for (long long cc = 0; cc < (long long)grid.ncells; ++cc) {
+        const size_t c = (size_t)cc;
+        std::fill(rl.begin(), rl.end(), 0.0);
+        const double* xc = x + c * nloc;
+        lop.lambda_volume(c, rl.data());
+        lop.alpha_volume(c, xc, rl.data());
+        for (int f = 0; f < 6; ++f) {
+          size_t cn;
+          if (grid.neighbor(c, f, cn))
+            lop.alpha_skeleton(c, f, cn, xc, x + cn * nloc, rl.data(), dummy.data());
+          else
+            lop.boundary_integral(c, f, xc, rl.data(), false);
+        }
+        for (size_t i = 0; i < nloc; ++i) r[c * nloc + i] += rl[i];
+      }
+    }
+  }
+
+  // y += J^T z from the same local Jacobian blocks as jacobian_apply_transpose (each face
+  // taken from its higher-index cell, assembler.hh:181), but gathered per column cell:
+  // cell c collects (J_cc)^T z_c and (J_{n,c})^T z_n of its face neighbours n, so cells run
+  // in parallel. Equal to jacobian_apply_transpose up to the summation order.
+  void jacobian_apply_transpose_twosided(const double* z, double* y, int nthreads) const {
+    const size_t n2 = nloc * nloc;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel num_threads(nthreads)
+#endif
+    {
+      std::vector<double> m(n2), mss(n2), msn(n2), mns(n2), mnn(n2), yl(nloc);
+      auto addT = [&](const std::vector<double>& A, const double* x) {
+        for (size_t i = 0; i < nloc; ++i)
+          for (size_t j = 0; j < nloc; ++j) yl[j] += A[i * nloc + j] * x[i];
+      };
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+      for (long long cc = 0; cc < (long long)grid.ncells; ++cc) {
+        const size_t c = (size_t)cc;
+        std::fill(yl.begin(), yl.end(), 0.0);
+        std::fill(m.begin(), m.end(), 0.0);
+        lop.jacobian_volume(c, m.data());
+        addT(m, z + c * nloc);
+        for (int f = 0; f < 6; ++f) {
+          size_t cn;
+          if (grid.neighbor(c, f, cn)) {
+            std::fill(mss.begin(), mss.end(), 0.0); std::fill(msn.begin(), msn.end(), 0.0);
+            std::fill(mns.begin(), mns.end(), 0.0); std::fill(mnn.begin(), mnn.end(), 0.0);
+            if (c > cn) {  // c inside: dR_c/du_c = mss, dR_cn/du_c = mns
+              lop.jacobian_skeleton(c, f, cn, mss.data(), msn.data(), mns.data(), mnn.data());
+              addT(mss, z + c * nloc);
+              addT(mns, z + cn * nloc);
+            } else {       // cn inside: dR_c/du_c = mnn, dR_cn/du_c = msn
+              lop.jacobian_skeleton(cn, f ^ 1, c, mss.data(), msn.data(), mns.data(), mnn.data());
+              addT(mnn, z + c * nloc);
+              addT(msn, z + cn * nloc);
+            }
+          } else {
+            std::fill(m.begin(), m.end(), 0.0);
+            lop.jacobian_boundary(c, f, m.data());
+            addT(m, z + c * nloc);
+          }
+        }
+        for (size_t i = 0; i < nloc; ++i) y[c * nloc + i] += yl[i];
+      }
+    }
+  }
+
   // y += J^T z from the local Jacobian blocks (jacobian engine traversal)
   void jacobian_apply_transpose(const double* z, double* y) const {
     const size_t n2 = nloc * nloc;
@@ -547,6 +629,12 @@ int oracle_jacobian_apply(oracle_handle* h, const double* z, double* y) {
 }
 int oracle_jacobian_apply_twosided(oracle_handle* h, const double* z, double* y, int nthreads) {
   return guard(h, [&] { h->o->jacobian_apply_twosided(z, y, nthreads); });
+}
+int oracle_residual_twosided(oracle_handle* h, const double* x, double* r, int nthreads) {
+  return guard(h, [&] { h->o->residual_twosided(x, r, nthreads); });
+}
+int oracle_jacobian_apply_transpose_twosided(oracle_handle* h, const double* z, double* y, int nthreads) {
+  return guard(h, [&] { h->o->jacobian_apply_transpose_twosided(z, y, nthreads); });
 }
 int oracle_jacobian_apply_transpose(oracle_handle* h, const double* z, double* y) {
   return guard(h, [&] { h->o->jacobian_apply_transpose(z, y); });

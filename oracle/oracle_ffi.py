@@ -65,6 +65,8 @@ def lib():
                    "oracle_mass_apply", "oracle_grad_source"):
             getattr(L, fn).argtypes = [vp, _dp, _dp]
         L.oracle_jacobian_apply_twosided.argtypes = [vp, _dp, _dp, ctypes.c_int]
+        L.oracle_residual_twosided.argtypes = [vp, _dp, _dp, ctypes.c_int]
+        L.oracle_jacobian_apply_transpose_twosided.argtypes = [vp, _dp, _dp, ctypes.c_int]
         L.oracle_grad_logkappa.argtypes = [vp, _dp, _dp, _dp]
         L.oracle_assemble.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_longlong)]
         L.oracle_l2_error_sq.restype = ctypes.c_double
@@ -159,6 +161,13 @@ class Oracle:
         self._check(lib().oracle_residual(self.h, _p(x), _p(r)))
         return r
 
+    def residual_twosided(self, x, nthreads=0):
+        """R(x) with two-sided face visits, parallel over cells (nthreads <= 0: all)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        r = np.zeros(self.N)
+        self._check(lib().oracle_residual_twosided(self.h, _p(x), _p(r), nthreads))
+        return r
+
     def jacobian_apply(self, z, y=None):
         z = np.ascontiguousarray(z, dtype=np.float64)
         y = np.zeros(self.N) if y is None else y
@@ -175,6 +184,13 @@ class Oracle:
         z = np.ascontiguousarray(z, dtype=np.float64)
         y = np.zeros(self.N)
         self._check(lib().oracle_jacobian_apply_transpose(self.h, _p(z), _p(y)))
+        return y
+
+    def jacobian_apply_transpose_twosided(self, z, nthreads=0):
+        """J^T z from the same local blocks, gathered per column cell, parallel over cells."""
+        z = np.ascontiguousarray(z, dtype=np.float64)
+        y = np.zeros(self.N)
+        self._check(lib().oracle_jacobian_apply_transpose_twosided(self.h, _p(z), _p(y), nthreads))
         return y
 
     def mass_apply(self, z):

@@ -48,6 +48,8 @@ CASES = [
     # aligned rows with nz a multiple of the z-walk (2 chunks per workgroup at Q3, 8 or 4 at Q2)
     ("walk_q2_32x2x4", dict(nx=32, ny=2, nz=4, degree=2, alpha=3.0), None),
     ("walk_q3_16x3x6_react", dict(nx=16, ny=3, nz=6, degree=3, alpha=3.0, c=0.5), None),
+    # the bench's kernel instance (Q3, aligned rows, z-walk of 2, no reaction, y = J z)
+    ("walk_q3_32x2x4", dict(nx=32, ny=2, nz=4, degree=3, alpha=3.0), None),
     ("walk8_q2_32x2x8_react", dict(nx=32, ny=2, nz=8, degree=2, alpha=3.0, c=0.4), None),
 ]
 
@@ -399,7 +401,7 @@ def test_transpose_solve(pkg, of, method):
 
 def test_large_apply_properties(pkg, of):
     """64^3 Q3 (16.8M DoF): linearity and symmetry x^T J y == y^T J x of the SIPG
-    operator, sizes beyond the oracle's reach, plus parity on a 16^3 sub-problem."""
+    operator (the oracle comparison at this size is in test_gpu_fullsize.py)."""
     say("[gpu] 64^3 Q3 properties")
     cfg = pkg.make_config(n=64, degree=3, alpha=3.0, problem=pkg.PROBLEM_EXP)
     go = pkg.GridOperator(cfg)

@@ -124,6 +124,24 @@ def test_oracle_matches_tensor_product_form(of, name, kw, seed):
     assert np.abs(o.point_diagonal() - np.diag(A)).max() <= 1e-13 * scale
     assert np.abs(o.jacobian_apply_transpose(x) - A.T @ x).max() <= 1e-13 * np.abs(y).max()
     assert np.abs(o.jacobian_apply_twosided(x) - y).max() <= 1e-13 * np.abs(y).max()
+    assert np.abs(o.jacobian_apply_transpose_twosided(x, 2) - A.T @ x).max() <= 1e-13 * np.abs(y).max()
+
+
+@pytest.mark.parametrize("problem", [1, 3, 4], ids=["exp", "gaussians", "quadratic"])
+def test_residual_twosided_equals_residual(of, problem):
+    """The threaded two-sided residual (used for full-size checks) is the residual engine's
+    operator: same R(x) as the one-sided traversal, for data with f and g nonzero."""
+    kw = dict(nx=3, ny=2, nz=4, degree=2, alpha=3.0, problem=problem, b=(0.5, -0.25, 1.0), c=0.3,
+              bctype=(0, -1, 0, 0, 0, -2))
+    if problem == 3:
+        kw.update(gaussians=[(0.3, 0.4, 0.5, 0.2, 1.0), (0.7, 0.2, 0.6, 0.1, -2.0)], g0=0.5)
+    cfg = of.cfgmod.make_config(**kw)
+    kappa = _kappa(cfg, 9)
+    cfg.kappa_per_cell = 1
+    o = of.Oracle(cfg, kappa)
+    x = np.random.default_rng(8).uniform(-1, 1, o.N)
+    r = o.residual(x)
+    assert np.abs(o.residual_twosided(x, 3) - r).max() <= 1e-13 * np.abs(r).max()
 
 
 @pytest.mark.parametrize("k", [1, 2, 3])
