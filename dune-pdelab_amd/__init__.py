@@ -29,6 +29,9 @@ from .config import (  # noqa: F401  (re-exported)
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "lib", "libpdelab_gpu.so")
 
+# cells of the largest grid pdelab_gpu_l2_error_squared_noalloc takes (PDELAB_NOALLOC_MAX_CELLS)
+NOALLOC_MAX_CELLS = 1048576
+
 _lib = None
 _vp = ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -83,6 +86,7 @@ def lib():
     L.pdelab_gpu_block_diagonal.argtypes = [_vp, ctypes.c_int, _vp]
     L.pdelab_gpu_jacobian.argtypes = [_vp, ctypes.c_int, _vp, _vp]
     L.pdelab_gpu_l2_error_squared.argtypes = [_vp, _vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    L.pdelab_gpu_l2_error_squared_noalloc.argtypes = [_vp, _vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     L.pdelab_gpu_sample.argtypes = [_vp, _vp, ctypes.c_int, _vp]
     L.pdelab_gpu_onestep_jacobian_apply.argtypes = [_vp, ctypes.c_double, ctypes.c_double, _vp, _vp]
     L.pdelab_gpu_onestep_step.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -311,6 +315,13 @@ class GridOperator:
         """integral of (u_h - u)^2 for the built-in problem's exact solution u."""
         out = ctypes.c_double()
         self._check(lib().pdelab_gpu_l2_error_squared(self.h, self._vec(x), order, ctypes.byref(out)))
+        return out.value
+
+    def l2_error_squared_noalloc(self, x, order=10):
+        """l2_error_squared without any device allocation (library-owned scratch of
+        NOALLOC_MAX_CELLS + 1 doubles per GPU; larger grids raise)."""
+        out = ctypes.c_double()
+        self._check(lib().pdelab_gpu_l2_error_squared_noalloc(self.h, self._vec(x), order, ctypes.byref(out)))
         return out.value
 
     def sample(self, x, intervals):

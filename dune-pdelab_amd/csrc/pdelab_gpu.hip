@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -943,6 +944,8 @@ void pdelab_gpu_default_config(pdelab_config* cfg) {
   cfg->problem = PDELAB_PROBLEM_ZERO;
 }
 
+static int init_scratch(H* h);  // device address of the no-allocation scratch array
+
 int pdelab_gpu_create(const pdelab_config* cfg, const double* kappa_host, pdelab_gpu_gridoperator** out) {
   g_create_error.clear();
   if (!out) { g_create_error = "null output pointer"; return PDELAB_ERR_INVALID_ARG; }
@@ -992,6 +995,7 @@ int pdelab_gpu_create(const pdelab_config* cfg, const double* kappa_host, pdelab
   if (hipMalloc(&h->d_dstate, sizeof(double) * 4) != hipSuccess) return bail("hipMalloc failed");
   if (hipHostMalloc(&h->h_state, sizeof(int) * 4) != hipSuccess) return bail("hipHostMalloc failed");
   if (hipHostMalloc(&h->h_dstate, sizeof(double) * 4) != hipSuccess) return bail("hipHostMalloc failed");
+  if (init_scratch(h) != PDELAB_OK) return bail(h->err);
   if (cfg->kappa_per_cell) {
     if (!kappa_host) { g_create_error = "kappa_per_cell requires a kappa array"; pdelab_gpu_destroy(h); return PDELAB_ERR_INVALID_ARG; }
     if (upload_kappa(h, kappa_host) != PDELAB_OK) {
@@ -1397,31 +1401,80 @@ static int launch_points(H* h, const double* x, const PointSet& ps, double* out)
   return PDELAB_OK;
 }
 
-extern "C" {
-
-int pdelab_gpu_l2_error_squared(pdelab_gpu_gridoperator* h, const double* x, int order, double* err) {
+// Checks of the L2 error integral's arguments, shared by both entry points.
+static int l2_error_check(H* h, const double* x, int order, double* err) {
   if (!h || !x || !err || order < 0) return PDELAB_ERR_INVALID_ARG;
-  const int m = order / 2 + 1;
-  if (m > kMaxPts) return fail(h, PDELAB_ERR_INVALID_ARG, "l2 error: quadrature order above 23");
+  if (order / 2 + 1 > kMaxPts) return fail(h, PDELAB_ERR_INVALID_ARG, "l2 error: quadrature order above 23");
   if (h->cfg.problem != PDELAB_PROBLEM_EXP && h->cfg.problem != PDELAB_PROBLEM_QUADRATIC)
     return fail(h, PDELAB_ERR_UNSUPPORTED, "l2 error: the problem has no built-in exact solution");
   if (h->ncells >= (1LL << 31)) return fail(h, PDELAB_ERR_UNSUPPORTED, "l2 error: more than 2^31 - 1 cells");
+  return PDELAB_OK;
+}
+
+// The L2 error integral into part[0 .. ncells] (per-cell partials, then their fixed-order
+// sum in part[ncells]), copied to *err. Every entry of part is written before it is read.
+static int l2_error_into(H* h, const double* x, int order, double* part, double* err) {
+  const int m = order / 2 + 1;
   long double xi[kMaxPts], w[kMaxPts];
   gauss01(m, xi, w);
+  CHK(launch_points<1>(h, x, point_set(h, m, xi, w), part));
+  hipLaunchKernelGGL((final_reduce_kernel<double, 1>), dim3(1), dim3(kFinalThreads), 0, h->stream,
+                     (const double*)part, (int)h->ncells, part + h->ncells, (const int*)nullptr);
+  HIPCHK(h, hipGetLastError());
+  double v = 0;
+  if (hipMemcpyAsync(&v, part + h->ncells, sizeof(double), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+      hipStreamSynchronize(h->stream) != hipSuccess)
+    return fail(h, PDELAB_ERR_HIP, "l2 error: copy of the result failed");
+  *err = v;
+  return PDELAB_OK;
+}
+
+// Scratch of pdelab_gpu_l2_error_squared_noalloc: the per-cell partials and their sum. Each
+// device holds its own copy for the life of the process. Host code never passes the
+// symbol itself to a kernel: it uses the device address that pdelab_gpu_create obtains
+// once per device with hipGetSymbolAddress (g_scratch_ptr).
+static __device__ __align__(256) double scratch[PDELAB_NOALLOC_MAX_CELLS + 1];
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {nullptr};
+}  // namespace
+
+// With h->device current: the device address of that device's scratch array, looked up
+// once per device.
+static int init_scratch(H* h) {
+  if (h->device < 0 || h->device >= kMaxDevices)
+    return fail(h, PDELAB_ERR_UNSUPPORTED, "device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[h->device]) return PDELAB_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(h, PDELAB_ERR_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+  g_scratch_ptr[h->device] = (double*)p;
+  return PDELAB_OK;
+}
+
+extern "C" {
+
+int pdelab_gpu_l2_error_squared(pdelab_gpu_gridoperator* h, const double* x, int order, double* err) {
+  CHK(l2_error_check(h, x, order, err));
   double* part = nullptr;
   HIPCHK(h, hipMallocAsync((void**)&part, sizeof(double) * (h->ncells + 1), h->stream));
-  int st = launch_points<1>(h, x, point_set(h, m, xi, w), part);
-  if (st == PDELAB_OK) {
-    hipLaunchKernelGGL((final_reduce_kernel<double, 1>), dim3(1), dim3(kFinalThreads), 0, h->stream,
-                       (const double*)part, (int)h->ncells, part + h->ncells, (const int*)nullptr);
-    double v = 0;
-    if (hipMemcpyAsync(&v, part + h->ncells, sizeof(double), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
-        hipStreamSynchronize(h->stream) != hipSuccess)
-      st = fail(h, PDELAB_ERR_HIP, "l2 error: copy of the result failed");
-    *err = v;
-  }
+  const int st = l2_error_into(h, x, order, part, err);
   (void)hipFreeAsync(part, h->stream);
   return st;
+}
+
+int pdelab_gpu_l2_error_squared_noalloc(pdelab_gpu_gridoperator* h, const double* x, int order, double* err) {
+  CHK(l2_error_check(h, x, order, err));
+  if (h->ncells > PDELAB_NOALLOC_MAX_CELLS)
+    return fail(h, PDELAB_ERR_INVALID_ARG,
+                "l2 error (no allocation): " + std::to_string(h->ncells) + " cells, more than the scratch maximum of " +
+                    std::to_string((long long)PDELAB_NOALLOC_MAX_CELLS));
+  double* part = h->device >= 0 && h->device < kMaxDevices ? g_scratch_ptr[h->device] : nullptr;
+  if (!part) return fail(h, PDELAB_ERR_HIP, "l2 error (no allocation): scratch address of the device not set up");
+  return l2_error_into(h, x, order, part, err);
 }
 
 int pdelab_gpu_sample(pdelab_gpu_gridoperator* h, const double* x, int intervals, double* values) {

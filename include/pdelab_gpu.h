@@ -329,6 +329,16 @@ int pdelab_gpu_onestep_step(pdelab_gpu_gridoperator* h, int scheme, int method, 
    23): integrateGridFunction(DifferenceSquaredAdapter(u, u_h), order),
    R:dune/pdelab/common/functionutilities.hh:51-83, as testconvectiondiffusiondg.cc:160-166 uses it. */
 int pdelab_gpu_l2_error_squared(pdelab_gpu_gridoperator* h, const double* x, int order, double* err);
+/* pdelab_gpu_l2_error_squared without any allocation. No counterpart in the reference: it
+   is for callers that cannot allocate device memory (code under HIP graph capture, tight
+   serving loops, harnesses that forbid hipMalloc). The per-cell partials live in a
+   library-owned device array of PDELAB_NOALLOC_MAX_CELLS + 1 doubles (8 MiB + 8 B), one
+   per GPU, held until the process ends; grids with more cells return
+   PDELAB_ERR_INVALID_ARG. Every call on one GPU shares that array, so calls that overlap on
+   one GPU (handles used from separate host threads) race. Same result as
+   pdelab_gpu_l2_error_squared, bit for bit. */
+#define PDELAB_NOALLOC_MAX_CELLS 1048576
+int pdelab_gpu_l2_error_squared_noalloc(pdelab_gpu_gridoperator* h, const double* x, int order, double* err);
 /* values[c (L+1)^3 + i + (L+1) j + (L+1)^2 l] = u_h(i/L, j/L, l/L) in cell c, L = intervals >= 1
    (at most 11): the DiscreteGridFunction values (gridfunctionspaceutilities.hh:54) at the
    vertices of the refined cells that SubsamplingVTKWriter with refinementIntervals(L) writes
