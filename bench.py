@@ -126,6 +126,45 @@ def cpu_baseline(cfg, min_seconds, gpu_cg_iterations=None):
     return out
 
 
+APPLY_VARIANTS = [
+    # (label, grid cells per direction, degree, config keywords, cell-wise lognormal kappa)
+    ("q2_96_constant_kappa", 96, 2, dict(problem=1), False),
+    ("q2_96_lognormal_kappa", 96, 2, dict(problem=2, f0=1.0), True),
+    ("q2_96_convection", 96, 2, dict(kappa0=1e-2, b=(1.0, 0.5, 0.25)), False),
+    ("q3_64_lognormal_kappa", 64, 3, dict(problem=2, f0=1.0), True),
+]
+
+
+def apply_variants_leg(pkg, reps=20):
+    """FP64 and FP32 applies y = J z of the other operators on the path: the Q2 operators of the
+    16^3 reference run, the adjoint and the convection-diffusion workloads at 96^3 (constant
+    kappa, cell-wise lognormal kappa, convection), and the 64^3 Q3 cell-wise-kappa operator of the
+    mixed-precision solve. HIP-event average over `reps` launches after 3 warm-up launches.
+    Algorithmic bytes: 16 B/DoF FP64 (8 B/DoF FP32), plus 8 B (4 B) per cell for cell-wise kappa."""
+    out = {}
+    for label, n, k, kw, cellwise in APPLY_VARIANTS:
+        cfg = pkg.make_config(n=n, degree=k, alpha=3.0, kappa_per_cell=cellwise, **kw)
+        go = pkg.GridOperator(cfg, lognormal_kappa(n ** 3, 4242) if cellwise else None)
+        N, nc = go.N, go.ncells
+        z = go.vector(splitmix_uniform(N, 777))
+        y = go.vector()
+        ms64 = go.time_apply(z, y, warmup=3, reps=reps)
+        z32 = go.vector(splitmix_uniform(N, 777).astype(np.float32), dtype=np.float32)
+        y32 = go.vector(dtype=np.float32)
+        ms32 = go.time_apply(z32, y32, warmup=3, reps=reps)
+        b64 = 16.0 * N + (8.0 * nc if cellwise else 0.0)
+        b32 = 8.0 * N + (4.0 * nc if cellwise else 0.0)
+        out[label] = {"dofs": N, "fp64_us": ms64 * 1e3, "fp64_GBs": b64 / (ms64 * 1e-3) / 1e9,
+                      "fp64_frac": b64 / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS, "fp64_MDoFs": N / (ms64 * 1e-3) / 1e6,
+                      "fp32_us": ms32 * 1e3, "fp32_GBs": b32 / (ms32 * 1e-3) / 1e9,
+                      "fp32_frac": b32 / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                      "algorithmic_bytes_fp64": b64, "algorithmic_bytes_fp32": b32}
+        log(f"bench: apply {label}: {ms64 * 1e3:.1f} us fp64 ({out[label]['fp64_frac']:.3f} of HBM peak), "
+            f"{ms32 * 1e3:.1f} us fp32")
+        go.close()
+    return out
+
+
 def convection_leg(pkg):
     """96^3 Q2 convection-diffusion solves and the source gradient (see the module doc)."""
     rng = np.random.default_rng(96)
@@ -354,6 +393,8 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-convection", action="store_true", help="skip the 96^3 Q2 convection-diffusion leg")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the applies of the Q2 and cell-wise-kappa operators (apply_variants)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--inner-reduction", type=float, default=0.1,
                     help="mixed-precision CG: FP32 residual reduction between FP64 residual replacements")
@@ -409,6 +450,9 @@ def main():
         f"stream reference {stream_gbs:.0f} GB/s")
     z32.free()
     y32.free()
+    variants = None
+    if not args.no_variants:
+        variants = apply_variants_leg(pkg)
     if args.calibrate:
         for _ in range(5):
             go.axpy(0.5, z, y)
@@ -443,6 +487,14 @@ def main():
                      "flops_per_dof": kron_flops_per_dof(k),
                      "achieved_fp64_TFLOPs": kron_flops_per_dof(k) * N / (kern_ms * 1e-3) / 1e12},
     }
+    if variants:
+        q2 = variants["q2_96_constant_kappa"]
+        out["roofline_q2"] = {"bound": "hbm", "kernel": "dg_apply_rowreg_kernel<3,double,...>",
+                              "workload": "96^3 Q2 SIPG, constant kappa, FP64 y = J z",
+                              "achieved": q2["fp64_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": q2["fp64_frac"], "kernel_avg_us": q2["fp64_us"],
+                              "algorithmic_bytes_per_launch": q2["algorithmic_bytes_fp64"]}
+        out["apply_variants"] = variants
 
     if not args.no_solve:
         solve = {}

@@ -370,6 +370,141 @@ __global__ __launch_bounds__(kRedThreads) void dot2_kernel(long long n, const T*
   if (threadIdx.x == 0) { partial[2 * blockIdx.x] = s[0]; partial[2 * blockIdx.x + 1] = s[1]; }
 }
 
+// ---------------------------------------------------------------------------
+// Device-side BiCGSTAB (dune-istl BiCGSTABSolver::apply with its half-step Iteration
+// counting). Per iteration: p update, W^{-1} p, the apply, rt.v, x/r update with the r.r
+// partials, the half-step check, W^{-1} r, the apply, (t.r, t.t), x/r update with the
+// (r.r, rt.r) partials and the full-step check; single-workgroup kernels keep the solver's
+// scalars in FP64 as the host loop kept them:
+//   bs = {rho, rho_new, alpha, omega, beta}; state = {stopped, abort code, half steps};
+//   dstate = {defect, def0, reduction}.
+// Once state[0] is set every later kernel that writes x, r or the scalars returns at once,
+// so x and r are those of the half step at which the loop stopped.
+enum { BICG_RUNNING = 0, BICG_NAN = 1, BICG_RHO = 2, BICG_OMEGA = 3, BICG_H = 4 };
+constexpr int kBsRho = 0, kBsRhoNew = 1, kBsAlpha = 2, kBsOmega = 3, kBsBeta = 4;
+constexpr double kBicgEps = 1e-80;  // dune-istl's EPSILON of BiCGSTABSolver
+
+// Iteration::step(half / 2, norm) on the device; true when the loop stops here.
+__device__ __forceinline__ bool bicg_step(int* state, double* dstate, int half, double norm) {
+  state[2] = half;
+  dstate[0] = norm;
+  if (!isfinite(norm)) {
+    state[1] = BICG_NAN;
+    state[0] = 1;
+    return true;
+  }
+  if (norm < dstate[1] * dstate[2] || norm < 1e-30) {
+    state[0] = 1;
+    return true;
+  }
+  return false;
+}
+
+// bs at the start: rho = alpha = omega = 1, rho_new = rt.r (= r.r, rt being r), beta = 0
+__global__ __launch_bounds__(64) void bicg_init_kernel(double* __restrict__ bs, double rr) {
+  if (threadIdx.x == 0) {
+    bs[kBsRho] = 1.0;
+    bs[kBsRhoNew] = rr;
+    bs[kBsAlpha] = 1.0;
+    bs[kBsOmega] = 1.0;
+    bs[kBsBeta] = 0.0;
+  }
+}
+
+// p = r (first iteration) or p = r + beta (p - omega v)
+template <typename T>
+__global__ __launch_bounds__(256) void bicg_p_dev_kernel(long long n, T* __restrict__ p, const T* __restrict__ r,
+                                                         const T* __restrict__ v, const double* __restrict__ bs,
+                                                         const int* __restrict__ state, int first) {
+  if (state[0]) return;
+  const T beta = (T)bs[kBsBeta], omega = (T)bs[kBsOmega];
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = first ? r[i] : beta * (p[i] - omega * v[i]) + r[i];
+}
+
+// y = dinv .* x (or y = x), skipped once the loop has stopped
+template <typename T>
+__global__ __launch_bounds__(256) void prec_stop_kernel(long long n, const T* __restrict__ dinv, const T* __restrict__ x,
+                                                        T* __restrict__ y, const int* __restrict__ state) {
+  if (state[0]) return;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    y[i] = dinv ? dinv[i] * x[i] : x[i];
+}
+
+// x += a y; r -= a v with a = bs[slot]; partials of r.r (NR = 1) or of (r.r, rt.r) (NR = 2),
+// in the order of xr_update_kernel and dot_partial_kernel
+template <typename T, typename A, int NR>
+__global__ __launch_bounds__(kRedThreads) void bicg_xr_kernel(long long n, T* __restrict__ x, const T* __restrict__ y,
+                                                              T* __restrict__ r, const T* __restrict__ v,
+                                                              const T* __restrict__ rt, const double* __restrict__ bs,
+                                                              int slot, A* __restrict__ partial,
+                                                              const int* __restrict__ state) {
+  if (state[0]) return;
+  const T a = (T)bs[slot];
+  A s[NR];
+#pragma unroll
+  for (int j = 0; j < NR; ++j) s[j] = A(0);
+  for (long long i = (long long)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (long long)kRedBlocks * kRedThreads) {
+    x[i] += a * y[i];
+    const T ri = r[i] - a * v[i];
+    r[i] = ri;
+    s[0] += (A)ri * (A)ri;
+    if constexpr (NR == 2) s[1] += (A)rt[i] * (A)ri;
+  }
+  block_sum<A, NR>(s);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int j = 0; j < NR; ++j) partial[NR * blockIdx.x + j] = s[j];
+}
+
+// The scalar steps, one workgroup reducing the partials first:
+//   PHASE 0: h = rt.v; |h| < EPSILON aborts; alpha = rho_new / h (0 when the defect is 0)
+//   PHASE 1: defect of the half step, Iteration::step
+//   PHASE 2: omega = t.r / t.t (0 when the defect is 0)
+//   PHASE 3: defect of the full step, rho = rho_new, rho_new = rt.r, Iteration::step; then
+//            the breakdown checks at the top of dune-istl's next iteration (|rho|, |omega|
+//            <= EPSILON) and beta = (rho_new / rho)(alpha / omega)
+template <typename A, int PHASE>
+__global__ __launch_bounds__(kFinalThreads) void bicg_scalar_kernel(const A* __restrict__ partial, int nb,
+                                                                    double* __restrict__ bs, int* __restrict__ state,
+                                                                    double* __restrict__ dstate, int half) {
+  if (state[0]) return;
+  constexpr int NR = (PHASE == 2 || PHASE == 3) ? 2 : 1;
+  A v[NR];
+  strided_partial_sum<A, NR>(partial, nb, v);
+  if (threadIdx.x != 0) return;
+  const double norm = dstate[0];
+  if constexpr (PHASE == 0) {
+    const double hh = (double)v[0];
+    if (fabs(hh) < kBicgEps) {
+      state[1] = BICG_H;
+      state[0] = 1;
+      return;
+    }
+    bs[kBsAlpha] = norm == 0.0 ? 0.0 : bs[kBsRhoNew] / hh;
+  } else if constexpr (PHASE == 1) {
+    bicg_step(state, dstate, half, sqrt((double)v[0]));
+  } else if constexpr (PHASE == 2) {
+    bs[kBsOmega] = norm == 0.0 ? 0.0 : (double)v[0] / (double)v[1];
+  } else {
+    const double nrm = sqrt((double)v[0]);
+    bs[kBsRho] = bs[kBsRhoNew];
+    bs[kBsRhoNew] = (double)v[1];
+    if (bicg_step(state, dstate, half, nrm)) return;
+    if (fabs(bs[kBsRho]) <= kBicgEps) {
+      state[1] = BICG_RHO;
+      state[0] = 1;
+      return;
+    }
+    if (fabs(bs[kBsOmega]) <= kBicgEps) {
+      state[1] = BICG_OMEGA;
+      state[0] = 1;
+      return;
+    }
+    bs[kBsBeta] = nrm == 0.0 ? 0.0 : (bs[kBsRhoNew] / bs[kBsRho]) * (bs[kBsAlpha] / bs[kBsOmega]);
+  }
+}
+
 }  // namespace pdelab_amd
 
 #endif

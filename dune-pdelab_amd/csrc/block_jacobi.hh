@@ -403,7 +403,9 @@ __global__ __launch_bounds__(256) void bj_dense_setup_kernel(const DGArgs<double
 //   BJ_PREC: out = W^{-1} in, one partial of out.in per workgroup
 //   BJ_ADD : out += W^{-1} in
 //   BJ_CGR : r -= lambda q (lambda = *lam), out = W^{-1} r, partials
-//            of (r.r, out.r) per workgroup; returns at once when state[0] is set
+//            of (r.r, out.r) per workgroup
+// Every mode returns at once when a state is given and its stop flag state[0] is set (the
+// device-side Krylov loops).
 template <typename T, typename A>
 struct BJVec {
   const T* in;
@@ -440,7 +442,7 @@ __global__ __launch_bounds__(BJShape<N1>::THREADS) void bj_dense_kernel(const BJ
                                                                        const BJGeom g) {
   using S = BJShape<N1>;
   constexpr int NL = S::NL, NLP = S::NLP, CPW = S::CPW;
-  if (MODE == BJ_CGR && v.state[0]) return;
+  if (v.state && v.state[0]) return;  // a device-side Krylov loop has already stopped
   __shared__ T buf[S::WAVES][64];
   const int tid = threadIdx.x;
   const int w = tid >> 6, lw = tid & 63;
@@ -505,7 +507,7 @@ __global__ __launch_bounds__(BJFdmShape<N1>::THREADS) void bj_fdm_kernel(const B
                                                                         const T* __restrict__ data, const BJGeom g) {
   using S = BJFdmShape<N1>;
   constexpr int NL = S::NL, LPC = S::LPC, CPW = S::CPW, ES = S::ES;
-  if (MODE == BJ_CGR && v.state[0]) return;
+  if (v.state && v.state[0]) return;  // a device-side Krylov loop has already stopped
   // lines of the wave's cells, value k of lane l at X[w][k][l]: the sweep reads of a wave
   // touch every LDS bank at most twice (64-bit) or once (32-bit)
   __shared__ T X[S::WAVES][N1][64];

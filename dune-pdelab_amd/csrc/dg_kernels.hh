@@ -113,6 +113,39 @@ __global__ __launch_bounds__(256) void face_coeff_kernel(const double* __restric
   }
 }
 
+// The 1D mass matrix is symmetric and centro-symmetric (M_ij = M_ji = M_{n-1-i,n-1-j}: the
+// equidistant Lagrange nodes are symmetric about 1/2). The apply kernels read entry (i, j)
+// at the lexicographically first of those four positions, so a sweep references only the
+// distinct entries (6 of 16 at Q3): fewer kernel-argument values live in scalar registers.
+template <int N1>
+__host__ __device__ constexpr int mass_canon(int i, int j) {
+  const int a = i * N1 + j, b = j * N1 + i, c = (N1 - 1 - i) * N1 + (N1 - 1 - j), d = (N1 - 1 - j) * N1 + (N1 - 1 - i);
+  const int m1 = a < b ? a : b, m2 = c < d ? c : d;
+  return m1 < m2 ? m1 : m2;
+}
+// The distinct mass entries of an apply kernel; with VREG (FP64 constant symmetric
+// operators) copied once into vector registers. The FP64 kernels' uniform constants (line
+// matrices, traces, mass) exceed the scalar register file, and the compiler then parks
+// scalars in vector-register lanes and reloads them with v_readlane, one VALU instruction
+// per 32 bits per use; the mass entries, used in the last phase only, are cheaper held in
+// vector registers (2 per entry at Q3: 12 in all, within the 128 that keep 4 waves per
+// SIMD). The cell-wise-kappa and convection kernels hold more vector registers already and
+// keep the entries in scalar registers.
+template <int N1, typename T, bool VREG>
+struct MassVals {
+  T m[N1 * N1];
+  __device__ __forceinline__ explicit MassVals(const DGArgs<T>& a) {
+#pragma unroll
+    for (int k = 0; k < N1 * N1; ++k) {
+      if (mass_canon<N1>(k / N1, k % N1) != k) continue;  // only the distinct entries are loaded
+      T v = a.M[k / N1][k % N1];
+      if constexpr (VREG && sizeof(T) == 8) asm("v_mov_b64 %0, %1" : "=v"(v) : "s"(v));
+      m[k] = v;
+    }
+  }
+  __device__ __forceinline__ T operator()(int i, int j) const { return m[mass_canon<N1>(i, j)]; }
+};
+
 template <int N1, int D>
 __device__ __forceinline__ int line_index(int p, int q, int e) {
   constexpr int L = N1 * N1;
@@ -385,6 +418,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
   __shared__ T U[(CPB + 2) * NLP];  // [0] cell before the chunk, [1..CPB] own cells, [CPB+1] cell after
   __shared__ T YM[CPB * NLP];       // row y-1, later the z partial sums and the x mass sweep
   __shared__ T YP[CPB * NLP];       // row y+1, later the x partial sums
+  const MassVals<N1, T, !VARK && !CONV> Mv(a);
   const int t = threadIdx.x;
   const int lc = t / L, l = t % L, p = l % N1, q = l / N1;
   const long long blk = xcd_logical_block(blockIdx.x, nblocks);
@@ -496,7 +530,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
     for (int i = 0; i < N1; ++i) {
       T acc = T(0);
 #pragma unroll
-      for (int j = 0; j < N1; ++j) acc += a.M[i][j] * v[j];
+      for (int j = 0; j < N1; ++j) acc += Mv(i, j) * v[j];
       Ul[lds_index<N1, 1>(p, q, i)] = acc;
     }
   }
@@ -509,7 +543,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
     for (int i = 0; i < N1; ++i) {
       T acc = T(0);
 #pragma unroll
-      for (int j = 0; j < N1; ++j) acc += a.M[i][j] * v[j];
+      for (int j = 0; j < N1; ++j) acc += Mv(i, j) * v[j];
       YMl[lds_index<N1, 0>(p, q, i)] = acc;
     }
   }
@@ -523,7 +557,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
     for (int i = 0; i < N1; ++i) {
       T acc = T(0);
 #pragma unroll
-      for (int j = 0; j < N1; ++j) acc += a.M[i][j] * v[j];
+      for (int j = 0; j < N1; ++j) acc += Mv(i, j) * v[j];
       const int idx = off + line_index<N1, 2>(p, q, i);
       if (ACCUM) yc[idx] += acc;
       else yc[idx] = acc;
@@ -568,6 +602,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
   // two chunk-sized regions (YMO, YPO) for the partial sums and the mass sweeps
   constexpr int UO = NLP, YMO = (CPB + 2) * NLP, YPO = (2 * CPB + 2) * NLP;
   __shared__ T LS[(3 * CPB + 2) * NLP];
+  const MassVals<N1, T, !VARK && !CONV> Mv(a);
   const int t = threadIdx.x;
   const int lc = t / L, l = t % L, p = l % N1, q = l / N1;
   const long long blk = xcd_logical_block(blockIdx.x, nblocks);
@@ -710,7 +745,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
     for (int i = 0; i < N1; ++i) {
       T acc = T(0);
 #pragma unroll
-      for (int j = 0; j < N1; ++j) acc += a.M[i][j] * v[j];
+      for (int j = 0; j < N1; ++j) acc += Mv(i, j) * v[j];
       LS[UO + ay[i]] = acc;
     }
   }
@@ -723,7 +758,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
     for (int i = 0; i < N1; ++i) {
       T acc = T(0);
 #pragma unroll
-      for (int j = 0; j < N1; ++j) acc += a.M[i][j] * v[j];
+      for (int j = 0; j < N1; ++j) acc += Mv(i, j) * v[j];
       LS[YMO + ax[i]] = acc;
     }
   }
@@ -738,7 +773,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
       for (int i = 0; i < N1; ++i) {
         T acc = T(0);
 #pragma unroll
-        for (int j = 0; j < N1; ++j) acc += a.M[i][j] * v[j];
+        for (int j = 0; j < N1; ++j) acc += Mv(i, j) * v[j];
         const int idx = off + line_index<N1, 2>(p, q, i);
         if (own) {
           if (ACCUM) yc[idx] += acc;

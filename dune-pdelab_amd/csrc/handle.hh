@@ -67,6 +67,7 @@ struct pdelab_gpu_gridoperator {
   double* d_dstate = nullptr;    // {defect, def0, reduction, -}
   int* h_state = nullptr;        // pinned mirrors
   double* h_dstate = nullptr;
+  double* d_bscal = nullptr;     // device BiCGSTAB loop: {rho, rho_new, alpha, omega, beta}
   // constant-coefficient line matrices: [0] J, [1] J^T
   LineMats<double>* d_lm64 = nullptr;
   LineMats<float>* d_lm32 = nullptr;

@@ -694,7 +694,13 @@ static int cg_solve(H* h, T* x, T* b, double reduction, int maxit, int precond, 
   return PDELAB_OK;
 }
 
-// BiCGSTABSolver::apply (dune-istl) with half-step iteration counting.
+// Iterations of the device BiCGSTAB loop between host reads of its stop flag.
+constexpr int kBicgCheckEvery = 8;
+
+// BiCGSTABSolver::apply (dune-istl) with half-step iteration counting, as a device-side
+// loop: the scalars, the Iteration::step test and the breakdown checks run in
+// single-workgroup kernels (bicg_scalar_kernel), so an iteration enqueues its 13 launches
+// without waiting; the host reads the stop flag every kBicgCheckEvery iterations.
 // Workspace slots base .. base+4 (p, v, t, y, rt).
 template <typename T, typename A>
 static int bicgstab_solve(H* h, T* x, T* r, double reduction, int maxit, int precond, bool trans, int base,
@@ -706,7 +712,7 @@ static int bicgstab_solve(H* h, T* x, T* r, double reduction, int maxit, int pre
     *rt = ws<T>(h)[base + 4];
   Prec<T> P;
   CHK(make_prec<T>(h, precond, trans, P));
-  const double EPSILON = 1e-80;
+  if (!h->d_bscal) HIPCHK(h, hipMalloc(&h->d_bscal, sizeof(double) * 8));
   Iter iter(reduction);
   int status = PDELAB_OK;
   CHK(apply_t<T>(h, x, t, trans, false));
@@ -715,55 +721,74 @@ static int bicgstab_solve(H* h, T* x, T* r, double reduction, int maxit, int pre
   CHK(dot_dev<T, A>(h, r, r, n, 0));
   A sc[4];
   CHK(read_scalars<A>(h, sc, 1));
-  double norm = std::sqrt((double)sc[0]);
-  double rho_new_next = (double)sc[0];  // rt.r with rt == r
-  if (!iter.step(0, norm)) {
-    double rho = 1, alpha = 1, omega = 1, rho_new, beta, hh;
-    for (double itc = 0.5; itc < maxit; itc += .5) {
-      rho_new = rho_new_next;
-      if (std::fabs(rho) <= EPSILON) { status = fail(h, PDELAB_ERR_SOLVER_ABORT, "breakdown in BiCGSTAB - rho"); break; }
-      if (std::fabs(omega) <= EPSILON) { status = fail(h, PDELAB_ERR_SOLVER_ABORT, "breakdown in BiCGSTAB - omega"); break; }
-      if (itc < 1) {
-        hipLaunchKernelGGL(bicg_p_kernel<T>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, p, (const T*)r, (const T*)v,
-                           T(0), T(0), 1);
-      } else {
-        beta = (norm == 0.0) ? 0.0 : (rho_new / rho) * (alpha / omega);
-        hipLaunchKernelGGL(bicg_p_kernel<T>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, p, (const T*)r, (const T*)v,
-                           (T)beta, (T)omega, 0);
+  const double norm0 = std::sqrt((double)sc[0]);
+  if (!iter.step(0, norm0)) {
+    int* st = h->d_state;
+    double* ds = h->d_dstate;
+    double* bs = h->d_bscal;
+    // rho = alpha = omega = 1, rho_new = rt.r = r.r (rt = r), beta = 0
+    hipLaunchKernelGGL(bicg_init_kernel, dim3(1), dim3(64), 0, h->stream, bs, (double)sc[0]);
+    h->h_state[0] = h->h_state[1] = h->h_state[2] = h->h_state[3] = 0;
+    h->h_dstate[0] = norm0;
+    h->h_dstate[1] = norm0;
+    h->h_dstate[2] = reduction;
+    h->h_dstate[3] = 0.0;
+    HIPCHK(h, hipMemcpyAsync(st, h->h_state, sizeof(int) * 4, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(ds, h->h_dstate, sizeof(double) * 4, hipMemcpyHostToDevice, h->stream));
+    A* part = partial_buf<A>(h);
+    const dim3 rg(kRedBlocks), rb(kRedThreads), fg(1), fb(kFinalThreads);
+    auto prec = [&](const T* in, T* out) -> int {
+      if (P.block()) {
+        const BJVec<T, A> bv{in, out, nullptr, nullptr, nullptr, nullptr, st};
+        return bj_launch<T, A>(h, P.trans, BJ_PREC, bv, nullptr);
       }
-      // y = W^{-1} p ; v = J y ; h = rt.v
-      CHK((prec_dot<T, A>(h, P, p, y, -1)));
-      CHK(apply_t<T>(h, y, v, trans, false));
-      CHK(dot_dev<T, A>(h, rt, v, n, 0));
-      CHK(read_scalars<A>(h, sc, 1));
-      hh = (double)sc[0];
-      if (std::fabs(hh) < EPSILON) { status = fail(h, PDELAB_ERR_SOLVER_ABORT, "abs(h) < EPSILON in BiCGSTAB"); break; }
-      alpha = (norm == 0.0) ? 0.0 : rho_new / hh;
-      hipLaunchKernelGGL((xr_update_kernel<T, A>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, n, x,
-                         (const T*)y, r, (const T*)v, (T)alpha, partial_buf<A>(h));
-      CHK(finish_reduce<A, 1>(h, 0));
-      CHK(read_scalars<A>(h, sc, 1));
-      norm = std::sqrt((double)sc[0]);
-      if (iter.step(itc, norm)) break;
-      itc += .5;
-      // y = W^{-1} r ; t = J y ; omega = t.r / t.t
-      CHK((prec_dot<T, A>(h, P, r, y, -1)));
-      CHK(apply_t<T>(h, y, t, trans, false));
-      hipLaunchKernelGGL((dot2_kernel<T, A>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, n, (const T*)t,
-                         (const T*)r, partial_buf<A>(h));
-      CHK(finish_reduce<A, 2>(h, 0));
-      CHK(read_scalars<A>(h, sc, 2));
-      omega = (norm == 0.0) ? 0.0 : (double)sc[0] / (double)sc[1];
-      hipLaunchKernelGGL((xr_update_kernel<T, A>), dim3(kRedBlocks), dim3(kRedThreads), 0, h->stream, n, x,
-                         (const T*)y, r, (const T*)t, (T)omega, partial_buf<A>(h));
-      CHK(finish_reduce<A, 1>(h, 0));
-      CHK(dot_dev<T, A>(h, rt, r, n, 1));
-      CHK(read_scalars<A>(h, sc, 2));
-      rho = rho_new;
-      norm = std::sqrt((double)sc[0]);
-      rho_new_next = (double)sc[1];
-      if (iter.step(itc, norm)) break;
+      hipLaunchKernelGGL(prec_stop_kernel<T>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, P.dinv, in, out,
+                         (const int*)st);
+      HIPCHK(h, hipGetLastError());
+      return PDELAB_OK;
+    };
+    bool stopped = false;
+    for (int i = 1; i <= maxit && !stopped; ++i) {
+      // first half step: p, y = W^{-1} p, v = J y, alpha, x += alpha y, r -= alpha v
+      hipLaunchKernelGGL(bicg_p_dev_kernel<T>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, p, (const T*)r,
+                         (const T*)v, (const double*)bs, (const int*)st, i == 1 ? 1 : 0);
+      CHK(prec(p, y));
+      CHK(apply_mode<T>(h, y, v, trans, 0, nullptr, st));
+      hipLaunchKernelGGL((dot_partial_kernel<T, A>), rg, rb, 0, h->stream, (const T*)rt, (const T*)v, n, part);
+      hipLaunchKernelGGL((bicg_scalar_kernel<A, 0>), fg, fb, 0, h->stream, (const A*)part, kRedBlocks, bs, st, ds,
+                         2 * i - 1);
+      hipLaunchKernelGGL((bicg_xr_kernel<T, A, 1>), rg, rb, 0, h->stream, n, x, (const T*)y, r, (const T*)v,
+                         (const T*)nullptr, (const double*)bs, kBsAlpha, part, (const int*)st);
+      hipLaunchKernelGGL((bicg_scalar_kernel<A, 1>), fg, fb, 0, h->stream, (const A*)part, kRedBlocks, bs, st, ds,
+                         2 * i - 1);
+      // second half step: y = W^{-1} r, t = J y, omega, x += omega y, r -= omega t, rho
+      CHK(prec(r, y));
+      CHK(apply_mode<T>(h, y, t, trans, 0, nullptr, st));
+      hipLaunchKernelGGL((dot2_kernel<T, A>), rg, rb, 0, h->stream, n, (const T*)t, (const T*)r, part);
+      hipLaunchKernelGGL((bicg_scalar_kernel<A, 2>), fg, fb, 0, h->stream, (const A*)part, kRedBlocks, bs, st, ds,
+                         2 * i);
+      hipLaunchKernelGGL((bicg_xr_kernel<T, A, 2>), rg, rb, 0, h->stream, n, x, (const T*)y, r, (const T*)t,
+                         (const T*)rt, (const double*)bs, kBsOmega, part, (const int*)st);
+      hipLaunchKernelGGL((bicg_scalar_kernel<A, 3>), fg, fb, 0, h->stream, (const A*)part, kRedBlocks, bs, st, ds,
+                         2 * i);
+      HIPCHK(h, hipGetLastError());
+      if (i % kBicgCheckEvery == 0 || i == maxit) {
+        HIPCHK(h, hipMemcpyAsync(h->h_state, st, sizeof(int) * 4, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        stopped = h->h_state[0] != 0;
+      }
     }
+    HIPCHK(h, hipMemcpyAsync(h->h_state, st, sizeof(int) * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->h_dstate, ds, sizeof(double) * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const int code = h->h_state[1];
+    iter.def = h->h_dstate[0];
+    iter.i = 0.5 * h->h_state[2];
+    iter.aborted = code == BICG_NAN;
+    iter.converged = h->h_state[0] != 0 && code == BICG_RUNNING;
+    if (code == BICG_RHO) status = fail(h, PDELAB_ERR_SOLVER_ABORT, "breakdown in BiCGSTAB - rho");
+    else if (code == BICG_OMEGA) status = fail(h, PDELAB_ERR_SOLVER_ABORT, "breakdown in BiCGSTAB - omega");
+    else if (code == BICG_H) status = fail(h, PDELAB_ERR_SOLVER_ABORT, "abs(h) < EPSILON in BiCGSTAB");
   }
   HIPCHK(h, hipStreamSynchronize(h->stream));
   const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1013,7 +1038,7 @@ int pdelab_gpu_destroy(pdelab_gpu_gridoperator* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   void* ptrs[] = {h->d_kappa, h->d_kappa32, h->d_R0, h->d_src, h->d_ftab, h->d_gtab, h->d_diag, h->d_dinv,
                   h->d_dinv32, h->d_part64, h->d_part32, h->d_scal64, h->d_scal32, h->d_applydot,
-                  h->d_state, h->d_dstate, h->d_lm64, h->d_lm32, h->d_fc64, h->d_fc32};
+                  h->d_state, h->d_dstate, h->d_bscal, h->d_lm64, h->d_lm32, h->d_fc64, h->d_fc32};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (double* p : h->os)
