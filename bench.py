@@ -10,8 +10,13 @@ The JSON line also carries:
                 HIP-event-measured average launch duration of the timed region, against
                 8 TB/s HBM; burst_* give the same for 10 launches after a second of idle and
                 two warm-up launches (the GPU clock drops under sustained FP64 load);
+  roofline_q2, apply_variants
+                the same HIP-event measurement for the 96^3 Q2 operators (constant kappa,
+                cell-wise lognormal kappa, convection) and the 64^3 Q3 cell-wise-kappa
+                operator, FP64 and FP32;
   cpu_baseline  the oracle (CPU restatement of the reference quadrature-loop
-                algorithm) timed on this host's cores on the same apply, and on one core;
+                algorithm, its -O3 build with FP contraction) timed on this host's cores on
+                the same apply, and on one core;
                 a projected single-core time of the full CG/Jacobi solve (GPU iteration
                 count times the single-core apply, labelled projected); and the 16^3 Q2
                 reference run (exp problem, CG/Jacobi to 1e-10) solved on the host cores,
@@ -19,7 +24,9 @@ The JSON line also carries:
   solve         time to a 1e-10 relative residual: plain FP64 CG on the manufactured
                 exp problem, and for the lognormal-kappa problem plain FP64 CG vs FP32
                 CG with FP64 residual replacement (reliable updates), each with point
-                Jacobi and with per-cell block Jacobi;
+                Jacobi and with per-cell block Jacobi, and the stated configuration (restarted
+                refinement, each FP32 CG solve to 1e-4); the oracle's FP64 residual of the
+                64^3 solutions and of the 96^3 convection solutions (cpu_baseline leg);
   convection    96^3 Q2 convection-diffusion (kappa 1e-2, b = (1, .5, .25), eight seeded
                 Gaussian sources, g = 0): FP64 BiCGSTAB/Jacobi vs FP32-inner BiCGSTAB in
                 FP64 refinement to 1e-8, and the adjoint (J^T, downwind) gradient of
@@ -89,19 +96,23 @@ def kron_flops_per_dof(k):
     return (3 * (line + mass) + 2 * n1) / n1
 
 
-def cpu_baseline(cfg, min_seconds, gpu_cg_iterations=None):
+def cpu_baseline(cfg, min_seconds, gpu_cg_iterations=None, checks=None):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_ffi
 
     cores = len(os.sched_getaffinity(0))
     cores = max(1, min(cores, int(os.environ.get("OMP_NUM_THREADS", cores))))
-    o = oracle_ffi.Oracle(cfg)
+    # timed with the -O3 build that allows FP contraction (the checker build keeps it off)
+    build = os.path.basename(oracle_ffi.fast_lib_path())
+    o = oracle_ffi.Oracle(cfg, timing=True)
     sec, reps = o.time_apply(nthreads=cores, min_seconds=min_seconds)
     sec1, reps1 = o.time_apply(nthreads=1, min_seconds=min_seconds / 2)
     n = oracle_ffi.cfgmod.ndofs(cfg)
     out = {"value": n / sec / 1e6, "unit": "MDoF/s", "cores": cores, "kind": "port",
            "sample": f"{reps} full jacobian_apply of the 64^3 Q3 FP64 workload (oracle quadrature-loop "
                      f"restatement, OpenMP over cells with two-sided faces), {sec * reps:.1f} s total",
+           "build": f"{build}: g++ -O3 -ffp-contract=fast, -march=x86-64-v4 (AVX-512) or x86-64-v3, "
+                    f"the widest this host supports",
            "single_core": {"value": n / sec1 / 1e6, "unit": "MDoF/s", "cores": 1,
                            "sample": f"{reps1} full jacobian_apply on one core, {sec1 * reps1:.1f} s total"}}
     if gpu_cg_iterations:
@@ -113,7 +124,7 @@ def cpu_baseline(cfg, min_seconds, gpu_cg_iterations=None):
                     "vector operations excluded"}
     # the 16^3 Q2 reference run (exp problem, CG/Jacobi to 1e-10) on the host cores
     c1 = oracle_ffi.cfgmod.make_config(n=16, degree=2, alpha=3.0, problem=oracle_ffi.cfgmod.PROBLEM_EXP)
-    o1 = oracle_ffi.Oracle(c1)
+    o1 = oracle_ffi.Oracle(c1, timing=True)
     o1.set_threads(cores)
     t0 = time.perf_counter()
     x1, r1 = o1.linear_problem_solve(reduction=1e-10, maxit=5000)
@@ -123,6 +134,18 @@ def cpu_baseline(cfg, min_seconds, gpu_cg_iterations=None):
                        "seconds": t1 - t0, "cores": cores, "iterations": r1["solver"]["iterations"],
                        "converged": bool(r1["solver"]["converged"]),
                        "l2_error": float(np.sqrt(o1.l2_error_sq(x1)))}
+    # error of the GPU solutions against the FP64 oracle: the oracle's residual of each returned
+    # solution relative to its residual at x = 0 (the solves' first defect)
+    for name, c in (checks or {}).items():
+        oc = oracle_ffi.Oracle(c["cfg"], c.get("kappa"))
+        t0 = time.perf_counter()
+        r0 = float(np.linalg.norm(oc.residual_twosided(np.zeros(oc.N), cores)))
+        res = {k: float(np.linalg.norm(oc.residual_twosided(v, cores))) / r0 for k, v in c["solutions"].items()}
+        out[f"oracle_residual_{name}"] = {
+            "note": f"||R_oracle(x)|| / ||R_oracle(0)|| of the GPU solutions of the {c['workload']} "
+                    f"(target {c['target']:g}); R_oracle is the oracle's FP64 residual (two-sided faces, host cores)",
+            "relative_residual": res, "gpu_first_defect_vs_oracle": abs(c["first_defect"] - r0) / r0,
+            "seconds": time.perf_counter() - t0}
     return out
 
 
@@ -165,7 +188,7 @@ def apply_variants_leg(pkg, reps=20):
     return out
 
 
-def convection_leg(pkg):
+def convection_leg(pkg, checks):
     """96^3 Q2 convection-diffusion solves and the source gradient (see the module doc)."""
     rng = np.random.default_rng(96)
     gauss = [(*rng.uniform(0.15, 0.85, 3), rng.uniform(0.05, 0.15), rng.uniform(0.5, 1.5)) for _ in range(8)]
@@ -185,6 +208,9 @@ def convection_leg(pkg):
     rm = go.solve_mixed(xm, method=pkg.SOLVER_BICGSTAB, inner_reduction=1e-4, outer_reduction=1e-8,
                         inner_maxit=20000)
     a64, am = x64.download(), xm.download()
+    checks["convection"] = {"cfg": cfg, "first_defect": rm["first_defect"], "target": 1e-8,
+                            "workload": "96^3 Q2 convection-diffusion problem",
+                            "solutions": {"fp64_bicgstab": a64, "mixed_bicgstab": am}}
     out["mixed_bicgstab"] = {"seconds": rm["elapsed"], "outer": rm["outer_iterations"],
                              "inner_iterations": rm["inner_iterations"], "converged": bool(rm["converged"]),
                              "reduction": rm["reduction"],
@@ -496,6 +522,7 @@ def main():
                               "algorithmic_bytes_per_launch": q2["algorithmic_bytes_fp64"]}
         out["apply_variants"] = variants
 
+    checks = {}  # solutions whose oracle residual the cpu_baseline leg evaluates
     if not args.no_solve:
         solve = {}
         # plain FP64 CG/Jacobi on the manufactured problem (64^3 Q3)
@@ -539,8 +566,13 @@ def main():
         xm = go3.vector()
         log("bench: mixed-precision solve (lognormal kappa)")
         rm = go3.solve_mixed(xm, inner_reduction=args.inner_reduction, outer_reduction=1e-10, inner_maxit=100000)
+        xr = go3.vector()
+        log("bench: mixed-precision solve, restarted refinement with FP32 CG to 1e-4 (lognormal kappa)")
+        rr = go3.solve_mixed(xr, variant=pkg.MIXED_RESTARTED, inner_reduction=1e-4, outer_reduction=1e-10,
+                             inner_maxit=100000)
         a64 = x64.download()
         am = xm.download()
+        ar = xr.download()
         rel = float(np.linalg.norm(am - a64) / np.linalg.norm(a64))
         t64 = r64["assembler_time"] + r64["linear_solver_time"]
         solve["lognormal_fp64_cg"] = {"seconds": t64, "iterations": r64["solver"]["iterations"],
@@ -551,6 +583,16 @@ def main():
                                     "inner_reduction": args.inner_reduction, "converged": bool(rm["converged"]),
                                     "reduction": rm["reduction"], "rel_diff_vs_fp64_gpu": rel,
                                     "speedup_vs_fp64": t64 / rm["elapsed"] if rm["elapsed"] > 0 else None}
+        solve["lognormal_mixed_restarted"] = {
+            "seconds": rr["elapsed"], "outer": rr["outer_iterations"], "inner_iterations": rr["inner_iterations"],
+            "inner_reduction": 1e-4, "converged": bool(rr["converged"]), "reduction": rr["reduction"],
+            "rel_diff_vs_fp64_gpu": float(np.linalg.norm(ar - a64) / np.linalg.norm(a64)),
+            "speedup_vs_fp64": t64 / rr["elapsed"] if rr["elapsed"] > 0 else None,
+            "note": "the stated configuration: each FP32 CG solve to 1e-4, FP64 refinement to 1e-10"}
+        # the oracle's residual of these solutions is evaluated in the cpu_baseline leg
+        checks["lognormal"] = {"cfg": cfg3, "kappa": kappa, "first_defect": rm["first_defect"], "target": 1e-10,
+                               "workload": f"{n}^3 Q{k} lognormal-kappa problem",
+                               "solutions": {"fp64_cg": a64, "mixed_reliable": am, "mixed_restarted": ar}}
         # the same pair with per-cell block Jacobi (Blocking::fixed in the reference)
         bj = pkg.PREC_BLOCK_JACOBI
         x64b = go3.vector()
@@ -574,11 +616,11 @@ def main():
             "speedup_vs_fp64_block_jacobi": t64b / rmb["elapsed"] if rmb["elapsed"] > 0 else None,
             "speedup_vs_fp64_point_jacobi": t64 / rmb["elapsed"] if rmb["elapsed"] > 0 else None}
         out["solve"] = solve
-        for v in (x64, xm, x64b, xmb):
+        for v in (x64, xm, xr, x64b, xmb):
             v.free()
         go3.close()
         if not args.no_convection:
-            out["convection"] = convection_leg(pkg)
+            out["convection"] = convection_leg(pkg, checks)
         log("bench: 32^3 Q2 adjoint log-kappa gradient")
         out["adjoint_logkappa"] = adjoint_leg(pkg)
         log("bench: 32^3 Q2 assembled Jacobian and block-matrix apply")
@@ -593,7 +635,7 @@ def main():
     if not args.no_cpu:
         log("bench: CPU baseline (oracle)")
         its = out.get("solve", {}).get("exp_fp64_cg", {}).get("iterations")
-        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds, its)
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds, its, checks)
     print(json.dumps(out), flush=True)
 
 

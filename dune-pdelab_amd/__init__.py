@@ -20,7 +20,8 @@ import numpy as np
 
 from . import config as cfgmod
 from .config import (  # noqa: F401  (re-exported)
-    BC_DIRICHLET, BC_NEUMANN, BC_NONE, BC_OUTFLOW, IIPG, NIPG, PREC_BLOCK_JACOBI, PREC_JACOBI, PREC_NONE,
+    BC_DIRICHLET, BC_NEUMANN, BC_NONE, BC_OUTFLOW, IIPG, MIXED_FLEXIBLE, MIXED_RESTARTED, NIPG, PREC_BLOCK_JACOBI,
+    PREC_JACOBI, PREC_NONE,
     PROBLEM_CONSTANT, PROBLEM_EXP, PROBLEM_GAUSSIANS, PROBLEM_QUADRATIC, PROBLEM_TABULATED,
     PROBLEM_ZERO, SIPG, SOLVER_BICGSTAB, SOLVER_CG, TS_ALEXANDER2, TS_CRANK_NICOLSON, TS_IMPLICIT_EULER,
     WEIGHTS_OFF, WEIGHTS_ON, LinearProblemResult, LinearSolverResult, MixedResult, OneStepResult, PdelabConfig,
@@ -387,14 +388,21 @@ class GridOperator:
         return res.as_dict()
 
     def solve_mixed(self, x, method=SOLVER_CG, precond=PREC_JACOBI, inner_reduction=None, inner_maxit=5000,
-                    outer_reduction=1e-10, outer_maxit=50):
-        """FP32 Krylov inside FP64 refinement. inner_reduction defaults to 0.1 for CG (the
-        FP32 residual reduction between FP64 residual replacements) and 1e-4 for BiCGSTAB
-        (the reduction of each restarted inner solve)."""
+                    outer_reduction=1e-10, outer_maxit=50, variant=0):
+        """FP32 Krylov inside FP64 refinement. variant: 0, MIXED_RESTARTED or MIXED_FLEXIBLE
+        (OR'd into the method, see pdelab_gpu_solve_mixed). inner_reduction defaults to 0.1
+        for reliable-update CG (the FP32 residual reduction between FP64 residual
+        replacements), 1e-2 for the flexible outer iteration and 1e-4 for restarted
+        refinement (the reduction of each inner solve)."""
         if inner_reduction is None:
-            inner_reduction = 0.1 if method == SOLVER_CG else 1e-4
+            if variant & MIXED_FLEXIBLE:
+                inner_reduction = 1e-2
+            elif method == SOLVER_CG and not variant & MIXED_RESTARTED:
+                inner_reduction = 0.1
+            else:
+                inner_reduction = 1e-4
         res = MixedResult()
-        self._check(lib().pdelab_gpu_solve_mixed(self.h, method, precond, inner_reduction, inner_maxit,
+        self._check(lib().pdelab_gpu_solve_mixed(self.h, method | variant, precond, inner_reduction, inner_maxit,
                                                  outer_reduction, outer_maxit, self._vec(x), ctypes.byref(res)))
         return res.as_dict()
 

@@ -79,6 +79,10 @@ struct pdelab_gpu_gridoperator {
   double shift = 0.0;
   double prec_shift = 0.0;  // shift for which the cached diagonal and block Jacobi data hold
   double* os[4] = {nullptr, nullptr, nullptr, nullptr};  // one-step work vectors
+  // flexible GMRES of the mixed-precision solve: the Arnoldi basis V (kFgmresRestart + 1
+  // vectors) and the preconditioned directions Z (kFgmresRestart vectors), FP64
+  static constexpr int kFgmresRestart = 10;
+  double* fg[2 * kFgmresRestart + 1] = {nullptr};
   void* d_bjpart = nullptr;  // per-workgroup partials of the block Jacobi application
 };
 

@@ -867,6 +867,114 @@ static int upload_kappa(H* h, const double* kappa_host) {
 }
 
 // ===========================================================================
+// Mixed precision with a flexible outer Krylov iteration: FGMRES(m) (Saad 1993, "A flexible
+// inner-outer preconditioned GMRES algorithm") in FP64 on J d = R(x), right-preconditioned
+// by the FP32 Krylov solve (CG or BiCGSTAB with its own preconditioner) of J32 z = v to
+// inner_reduction from z = 0. v has unit norm, so the FP32 solve needs no scaling. Unlike a
+// restarted refinement, the outer iteration keeps every FP32 correction in its Krylov space:
+// each outer step minimises the FP64 residual over all corrections of the cycle. The Arnoldi
+// basis is orthogonalised by modified Gram-Schmidt in FP64. x -= d at the end of a cycle, and
+// each cycle starts from the true FP64 residual R(x) = J x + R(0), which also decides
+// convergence (||R(x)|| <= outer_reduction ||R(x0)||). outer_iterations counts the FP64
+// outer steps (one FP64 apply and one FP32 solve each).
+static int fgmres_mixed(H* h, int method, int precond, double inner_reduction, int inner_maxit,
+                        double outer_reduction, int outer_maxit, double* x, pdelab_mixed_result* res) {
+  constexpr int m = H::kFgmresRestart;
+  const long long n = h->N;
+  for (double*& v : h->fg)
+    if (!v) HIPCHK(h, hipMalloc(&v, sizeof(double) * n));
+  CHK(ensure_ws<float>(h, 2));
+  double** V = h->fg;      // V[0..m]
+  double** Z = h->fg + m + 1;  // Z[0..m-1]
+  float* r32 = h->w32[0];
+  float* z32 = h->w32[1];
+  const dim3 g(grid_for(n)), b(256);
+  double def0 = -1, def = 0;
+  int outer = 0, inner = 0;
+  bool conv = false;
+  std::vector<double> Hm((m + 1) * m), cs(m), sn(m), gv(m + 1), y(m);
+  for (;;) {
+    // true FP64 residual r = J x + R(0) into V[0]
+    CHK(apply64(h, x, V[0], false, false));
+    hipLaunchKernelGGL(axpy_kernel<double>, g, b, 0, h->stream, n, 1.0, (const double*)h->d_R0, V[0]);
+    CHK(pdelab_gpu_norm(h, V[0], &def));
+    if (!std::isfinite(def)) {
+      res->defect = def;
+      return fail(h, PDELAB_ERR_SOLVER_ABORT, "flexible refinement: residual is infinite or NaN");
+    }
+    if (def0 < 0) def0 = def;
+    if (def <= outer_reduction * def0 || def < 1e-30) { conv = true; break; }
+    if (outer >= outer_maxit) break;
+    hipLaunchKernelGGL(scale_kernel<double>, g, b, 0, h->stream, n, 1.0 / def, V[0]);
+    std::fill(gv.begin(), gv.end(), 0.0);
+    gv[0] = def;
+    const double target = outer_reduction * def0;
+    int kdone = 0;
+    for (int j = 0; j < m && outer < outer_maxit; ++j) {
+      // z_j = J32^{-1} v_j in FP32, then w = J z_j in FP64 (into V[j+1])
+      hipLaunchKernelGGL((convert_kernel<double, float>), g, b, 0, h->stream, n, (const double*)V[j], r32);
+      hipLaunchKernelGGL(fill_kernel<float>, g, b, 0, h->stream, n, 0.0f, z32);
+      HIPCHK(h, hipGetLastError());
+      pdelab_linear_solver_result ires;
+      const int st = krylov<float, float>(h, method, precond, inner_reduction, inner_maxit, false, z32, r32, 2, &ires);
+      inner += (int)ires.iterations;
+      if (st == PDELAB_ERR_SOLVER_ABORT) h->err.clear();  // a breakdown still leaves a usable direction
+      else CHK(st);
+      hipLaunchKernelGGL((convert_kernel<float, double>), g, b, 0, h->stream, n, (const float*)z32, Z[j]);
+      HIPCHK(h, hipGetLastError());
+      double* w = V[j + 1];
+      CHK(apply64(h, Z[j], w, false, false));
+      // modified Gram-Schmidt
+      for (int i = 0; i <= j; ++i) {
+        double hij = 0;
+        CHK(pdelab_gpu_dot(h, w, V[i], &hij));
+        Hm[i * m + j] = hij;
+        hipLaunchKernelGGL(axpy_kernel<double>, g, b, 0, h->stream, n, -hij, (const double*)V[i], w);
+      }
+      double hn = 0;
+      CHK(pdelab_gpu_norm(h, w, &hn));
+      Hm[(j + 1) * m + j] = hn;
+      if (hn > 0) hipLaunchKernelGGL(scale_kernel<double>, g, b, 0, h->stream, n, 1.0 / hn, w);
+      HIPCHK(h, hipGetLastError());
+      // Givens rotations of column j and the residual estimate |g_{j+1}|
+      for (int i = 0; i < j; ++i) {
+        const double a0 = Hm[i * m + j], a1 = Hm[(i + 1) * m + j];
+        Hm[i * m + j] = cs[i] * a0 + sn[i] * a1;
+        Hm[(i + 1) * m + j] = -sn[i] * a0 + cs[i] * a1;
+      }
+      const double a0 = Hm[j * m + j], a1 = Hm[(j + 1) * m + j];
+      const double rr = std::hypot(a0, a1);
+      cs[j] = rr > 0 ? a0 / rr : 1.0;
+      sn[j] = rr > 0 ? a1 / rr : 0.0;
+      Hm[j * m + j] = rr;
+      Hm[(j + 1) * m + j] = 0.0;
+      gv[j + 1] = -sn[j] * gv[j];
+      gv[j] = cs[j] * gv[j];
+      ++outer;
+      kdone = j + 1;
+      if (std::fabs(gv[j + 1]) <= 0.5 * target || hn == 0.0) break;
+    }
+    // y = H^{-1} g (upper triangular), x -= sum_i y_i z_i
+    for (int i = kdone - 1; i >= 0; --i) {
+      double sum = gv[i];
+      for (int k2 = i + 1; k2 < kdone; ++k2) sum -= Hm[i * m + k2] * y[k2];
+      y[i] = Hm[i * m + i] != 0.0 ? sum / Hm[i * m + i] : 0.0;
+    }
+    for (int i = 0; i < kdone; ++i)
+      hipLaunchKernelGGL(axpy_kernel<double>, g, b, 0, h->stream, n, -y[i], (const double*)Z[i], x);
+    HIPCHK(h, hipGetLastError());
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  res->converged = conv ? 1 : 0;
+  res->outer_iterations = outer;
+  res->inner_iterations = inner;
+  res->first_defect = def0;
+  res->defect = def;
+  res->reduction = def0 > 0 ? def / def0 : 0.0;
+  return PDELAB_OK;
+}
+
+// ===========================================================================
 // Mixed-precision CG with reliable updates (residual replacement): one FP32 CG run on
 // J z = R(x) whose recursive residual is replaced by the FP64 residual whenever it has
 // dropped by inner_reduction (or by the reduction still missing). At each replacement
@@ -890,8 +998,11 @@ static int cg_mixed_reliable(H* h, int precond, double inner_reduction, int inne
     if (!std::isfinite(d)) return fail(h, PDELAB_ERR_SOLVER_ABORT, "refinement: residual is infinite or NaN");
     return PDELAB_OK;
   };
-  auto to_f32 = [&]() -> int {
-    hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(n)), dim3(256), 0, h->stream, n,
+  // the FP32 recursion runs on the residual at unit norm: r32 = r / scale with scale the FP64
+  // defect at the last replacement, so its dot products stay far from the FP32 underflow
+  // threshold however small the outer residual has become; x32 holds the correction / scale
+  auto to_f32 = [&](double scale) -> int {
+    hipLaunchKernelGGL(scale_to_f32_kernel, dim3(grid_for(n)), dim3(256), 0, h->stream, n, 1.0 / scale,
                        (const double*)r64, r32);
     HIPCHK(h, hipGetLastError());
     return PDELAB_OK;
@@ -903,22 +1014,24 @@ static int cg_mixed_reliable(H* h, int precond, double inner_reduction, int inne
   int outer = 0, inner = 0;
   bool conv = def0 < 1e-30;
   if (!conv) {
-    CHK(to_f32());
+    double scale = def0;
+    CHK(to_f32(scale));
     hipLaunchKernelGGL(fill_kernel<float>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, 0.0f, x32);
     // p = W^{-1} r; rho -> slot 7
     CHK((prec_dot<float, float>(h, P, r32, p32, 7)));
     for (;;) {
       const double thr = std::max(inner_reduction * def, 0.5 * outer_reduction * def0);
       CgLoopOut lo;
-      CHK((cg_loop<float, float>(h, x32, r32, p32, q32, z32, P, false, std::max(1, inner_maxit - inner), def,
-                                 thr / def, lo)));
+      CHK((cg_loop<float, float>(h, x32, r32, p32, q32, z32, P, false, std::max(1, inner_maxit - inner), 1.0,
+                                 thr / scale, lo)));
       inner += lo.iters;
       if (lo.aborted) {
         res->inner_iterations = inner;
         return fail(h, PDELAB_ERR_SOLVER_ABORT, "mixed CG: FP32 defect is infinite or NaN");
       }
-      // residual replacement: x -= z32, r = R(x) in FP64
-      hipLaunchKernelGGL(sub_f32_kernel, dim3(grid_for(n)), dim3(256), 0, h->stream, n, (const float*)x32, x);
+      // residual replacement: x -= scale x32, r = R(x) in FP64
+      hipLaunchKernelGGL(sub_scaled_f32_kernel, dim3(grid_for(n)), dim3(256), 0, h->stream, n, scale,
+                         (const float*)x32, x);
       hipLaunchKernelGGL(fill_kernel<float>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, 0.0f, x32);
       HIPCHK(h, hipGetLastError());
       CHK(residual64(def));
@@ -928,14 +1041,18 @@ static int cg_mixed_reliable(H* h, int precond, double inner_reduction, int inne
         break;
       }
       if (outer >= outer_maxit || inner >= inner_maxit) break;
-      // continue the direction: p = W^{-1} r + (rho_new / rho_old) p
+      // continue the direction in the new scale: with r32 = r / scale the CG recurrence
+      // p = W^{-1} r + (rho_new / rho_old) p reads p_new = q + (scale_new / scale_old)
+      // (rho32_new / rho32_old) p_old in scaled units
       float sc[8];
       CHK((read_scalars<float>(h, sc, 8)));
       const double rho_old = sc[lo.rho_slot];
-      CHK(to_f32());
+      const double scale_old = scale;
+      scale = def;
+      CHK(to_f32(scale));
       CHK((prec_dot<float, float>(h, P, r32, q32, 7)));
       CHK((read_scalars<float>(h, sc, 8)));
-      const float beta = rho_old != 0 ? (float)(sc[7] / rho_old) : 0.0f;
+      const float beta = rho_old != 0 ? (float)((scale / scale_old) * (sc[7] / rho_old)) : 0.0f;
       // p = q + beta p with q = W^{-1} r
       hipLaunchKernelGGL(cg_pupdate_kernel<float>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, p32,
                          (const float*)q32, (const float*)nullptr, beta);
@@ -1042,6 +1159,8 @@ int pdelab_gpu_destroy(pdelab_gpu_gridoperator* h) {
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (double* p : h->os)
+    if (p) (void)hipFree(p);
+  for (double* p : h->fg)
     if (p) (void)hipFree(p);
   bj_free(h);
   for (int i = 0; i < 8; ++i) {
@@ -1320,7 +1439,20 @@ int pdelab_gpu_solve_mixed(pdelab_gpu_gridoperator* h, int method, int precond, 
   std::memset(res, 0, sizeof(*res));
   if (outer_maxit <= 0) outer_maxit = 50;
   if (inner_maxit <= 0) inner_maxit = 5000;
-  if (method == PDELAB_SOLVER_CG) {
+  const int variant = method & (PDELAB_MIXED_RESTARTED | PDELAB_MIXED_FLEXIBLE);
+  method &= ~(PDELAB_MIXED_RESTARTED | PDELAB_MIXED_FLEXIBLE);
+  if (method != PDELAB_SOLVER_CG && method != PDELAB_SOLVER_BICGSTAB)
+    return fail(h, PDELAB_ERR_INVALID_ARG, "unknown Krylov method");
+  if (variant == (PDELAB_MIXED_RESTARTED | PDELAB_MIXED_FLEXIBLE))
+    return fail(h, PDELAB_ERR_INVALID_ARG, "PDELAB_MIXED_RESTARTED and PDELAB_MIXED_FLEXIBLE exclude each other");
+  if (variant & PDELAB_MIXED_FLEXIBLE) {
+    auto t0 = std::chrono::steady_clock::now();
+    CHK(compute_R0(h));
+    CHK(fgmres_mixed(h, method, precond, inner_reduction, inner_maxit, outer_reduction, outer_maxit, x, res));
+    res->elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return PDELAB_OK;
+  }
+  if (method == PDELAB_SOLVER_CG && !(variant & PDELAB_MIXED_RESTARTED)) {
     auto t0 = std::chrono::steady_clock::now();
     CHK(compute_R0(h));
     CHK(cg_mixed_reliable(h, precond, inner_reduction, inner_maxit, outer_reduction, outer_maxit, x, res));

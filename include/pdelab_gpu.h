@@ -303,6 +303,14 @@ int pdelab_gpu_linear_problem_solve(pdelab_gpu_gridoperator* h, int method, int 
    correction is subtracted from x in FP64 and the residual replaced by the FP64 R(x), with
    the search direction kept (0.1 is a good interval; much smaller ones let the FP32
    recursion drift and stall). outer_iterations counts the FP64 residual replacements. */
+/* Variants of pdelab_gpu_solve_mixed, OR'd into its method argument:
+   PDELAB_MIXED_RESTARTED  CG also as restarted refinement (each FP32 solve to inner_reduction,
+                           a new Krylov space per refinement step) instead of reliable updates;
+   PDELAB_MIXED_FLEXIBLE   flexible GMRES(10) in FP64 as the outer iteration, preconditioned
+                           by the FP32 solve (CG or BiCGSTAB to inner_reduction from zero on a
+                           unit-norm right-hand side); outer_iterations then counts FGMRES steps
+                           and outer_maxit bounds them. */
+enum { PDELAB_MIXED_RESTARTED = 16, PDELAB_MIXED_FLEXIBLE = 32 };
 int pdelab_gpu_solve_mixed(pdelab_gpu_gridoperator* h, int method, int precond,
                            double inner_reduction, int inner_maxit, double outer_reduction,
                            int outer_maxit, double* x, pdelab_mixed_result* res);

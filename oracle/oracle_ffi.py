@@ -34,11 +34,42 @@ _lib = None
 
 
 def build(march=None):
-    """Compiles liboracle.so with make (g++)."""
+    """Compiles the checker build liboracle.so and the timing builds with make (g++)."""
     cmd = ["make", "-s", "-C", HERE]
     if march:
         cmd.append(f"MARCH={march}")
     subprocess.run(cmd, check=True)
+
+
+def _host_has_avx512():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("flags"):
+                    return " avx512f " in line + " "
+    except OSError:
+        pass
+    return False
+
+
+def fast_lib_path():
+    """The oracle build used for the timed CPU baseline: -O3 with FP contraction, for
+    x86-64-v4 (AVX-512) when this host has it, else x86-64-v3."""
+    return os.path.join(HERE, "liboracle_fast_v4.so" if _host_has_avx512() else "liboracle_fast_v3.so")
+
+
+_fast = None
+
+
+def fast_lib():
+    """ctypes handle of the timing build (same entry points as lib())."""
+    global _fast
+    if _fast is None:
+        path = fast_lib_path()
+        if not os.path.exists(path):
+            build()
+        _fast = _bind(ctypes.CDLL(path))
+    return _fast
 
 
 def lib():
@@ -46,45 +77,49 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        L = ctypes.CDLL(LIB)
-        vp = ctypes.c_void_p
-        L.oracle_create.restype = vp
-        L.oracle_create.argtypes = [vp, _dp]
-        L.oracle_destroy.argtypes = [vp]
-        L.oracle_last_error.restype = ctypes.c_char_p
-        L.oracle_last_error.argtypes = [vp]
-        L.oracle_size.restype = ctypes.c_size_t
-        L.oracle_size.argtypes = [vp]
-        L.oracle_set_threads.argtypes = [vp, ctypes.c_int]
-        for fn in ("oracle_set_kappa", "oracle_set_cell_source", "oracle_point_diagonal", "oracle_interpolate",
-                   "oracle_block_diagonal"):
-            getattr(L, fn).argtypes = [vp, _dp]
-        L.oracle_set_tabulated.argtypes = [vp, _dp, _dp]
-        L.oracle_quadrature_points.argtypes = [vp]
-        for fn in ("oracle_residual", "oracle_jacobian_apply", "oracle_jacobian_apply_transpose",
-                   "oracle_mass_apply", "oracle_grad_source"):
-            getattr(L, fn).argtypes = [vp, _dp, _dp]
-        L.oracle_jacobian_apply_twosided.argtypes = [vp, _dp, _dp, ctypes.c_int]
-        L.oracle_residual_twosided.argtypes = [vp, _dp, _dp, ctypes.c_int]
-        L.oracle_jacobian_apply_transpose_twosided.argtypes = [vp, _dp, _dp, ctypes.c_int]
-        L.oracle_grad_logkappa.argtypes = [vp, _dp, _dp, _dp]
-        L.oracle_assemble.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_longlong)]
-        L.oracle_l2_error_sq.restype = ctypes.c_double
-        L.oracle_l2_error_sq.argtypes = [vp, _dp]
-        L.oracle_solve.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _dp,
-                                   ctypes.POINTER(cfgmod.LinearSolverResult), ctypes.c_int]
-        L.oracle_linear_problem_solve.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
-                                                  ctypes.c_int, _dp, ctypes.POINTER(cfgmod.LinearProblemResult),
-                                                  ctypes.c_int]
-        L.oracle_time_apply.restype = ctypes.c_double
-        L.oracle_time_apply.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_int)]
-        L.oracle_tables_1d.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
-        L.oracle_onestep_jacobian_apply.argtypes = [vp, ctypes.c_double, ctypes.c_double, _dp, _dp]
-        L.oracle_onestep_step.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
-                                          ctypes.c_double, ctypes.c_int, _dp, _dp,
-                                          ctypes.POINTER(cfgmod.OneStepResult)]
-        _lib = L
+        _lib = _bind(ctypes.CDLL(LIB))
     return _lib
+
+
+def _bind(L):
+    """Declares the argument and result types of the oracle entry points on L."""
+    vp = ctypes.c_void_p
+    L.oracle_create.restype = vp
+    L.oracle_create.argtypes = [vp, _dp]
+    L.oracle_destroy.argtypes = [vp]
+    L.oracle_last_error.restype = ctypes.c_char_p
+    L.oracle_last_error.argtypes = [vp]
+    L.oracle_size.restype = ctypes.c_size_t
+    L.oracle_size.argtypes = [vp]
+    L.oracle_set_threads.argtypes = [vp, ctypes.c_int]
+    for fn in ("oracle_set_kappa", "oracle_set_cell_source", "oracle_point_diagonal", "oracle_interpolate",
+               "oracle_block_diagonal"):
+        getattr(L, fn).argtypes = [vp, _dp]
+    L.oracle_set_tabulated.argtypes = [vp, _dp, _dp]
+    L.oracle_quadrature_points.argtypes = [vp]
+    for fn in ("oracle_residual", "oracle_jacobian_apply", "oracle_jacobian_apply_transpose",
+               "oracle_mass_apply", "oracle_grad_source"):
+        getattr(L, fn).argtypes = [vp, _dp, _dp]
+    L.oracle_jacobian_apply_twosided.argtypes = [vp, _dp, _dp, ctypes.c_int]
+    L.oracle_residual_twosided.argtypes = [vp, _dp, _dp, ctypes.c_int]
+    L.oracle_jacobian_apply_transpose_twosided.argtypes = [vp, _dp, _dp, ctypes.c_int]
+    L.oracle_grad_logkappa.argtypes = [vp, _dp, _dp, _dp]
+    L.oracle_assemble.argtypes = [vp, _dp, ctypes.POINTER(ctypes.c_longlong)]
+    L.oracle_l2_error_sq.restype = ctypes.c_double
+    L.oracle_l2_error_sq.argtypes = [vp, _dp]
+    L.oracle_solve.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _dp,
+                               ctypes.POINTER(cfgmod.LinearSolverResult), ctypes.c_int]
+    L.oracle_linear_problem_solve.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                              ctypes.c_int, _dp, ctypes.POINTER(cfgmod.LinearProblemResult),
+                                              ctypes.c_int]
+    L.oracle_time_apply.restype = ctypes.c_double
+    L.oracle_time_apply.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_int)]
+    L.oracle_tables_1d.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
+    L.oracle_onestep_jacobian_apply.argtypes = [vp, ctypes.c_double, ctypes.c_double, _dp, _dp]
+    L.oracle_onestep_step.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_int, _dp, _dp,
+                                      ctypes.POINTER(cfgmod.OneStepResult)]
+    return L
 
 
 def _p(a):
@@ -110,7 +145,10 @@ def tables_1d(k, order):
 class Oracle:
     """CPU restatement of PDELab's SIPG grid operator + ISTL solvers."""
 
-    def __init__(self, cfg, kappa=None):
+    def __init__(self, cfg, kappa=None, timing=False):
+        """timing: use the -O3 build with FP contraction (the timed CPU baseline) instead of the
+        checker build."""
+        self._L = fast_lib() if timing else lib()
         self.cfg = cfg
         self.ncells = cfgmod.ncells(cfg)
         self.N = cfgmod.ndofs(cfg)
@@ -120,95 +158,95 @@ class Oracle:
             assert self._kappa.size == self.ncells
             kp = _p(self._kappa)
         # the config may come from the product package's ctypes class: pass its address
-        self.h = lib().oracle_create(ctypes.addressof(cfg), kp)
+        self.h = self._L.oracle_create(ctypes.addressof(cfg), kp)
         if not self.h:
             raise ValueError("oracle_create rejected the config")
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().oracle_destroy(self.h)
+            self._L.oracle_destroy(self.h)
             self.h = None
 
     def _check(self, st):
         if st != 0:
-            raise RuntimeError(lib().oracle_last_error(self.h).decode())
+            raise RuntimeError(self._L.oracle_last_error(self.h).decode())
 
     def set_threads(self, n):
-        lib().oracle_set_threads(self.h, n)
+        self._L.oracle_set_threads(self.h, n)
 
     def set_kappa(self, kappa):
         k = np.ascontiguousarray(kappa, dtype=np.float64)
-        lib().oracle_set_kappa(self.h, _p(k))
+        self._L.oracle_set_kappa(self.h, _p(k))
 
     def set_cell_source(self, s):
         if s is None:
-            lib().oracle_set_cell_source(self.h, None)
+            self._L.oracle_set_cell_source(self.h, None)
         else:
             self._src = np.ascontiguousarray(s, dtype=np.float64)
-            lib().oracle_set_cell_source(self.h, _p(self._src))
+            self._L.oracle_set_cell_source(self.h, _p(self._src))
 
     def set_tabulated(self, fq, gq):
         self._fq = np.ascontiguousarray(fq, dtype=np.float64)
         self._gq = np.ascontiguousarray(gq, dtype=np.float64)
-        lib().oracle_set_tabulated(self.h, _p(self._fq), _p(self._gq))
+        self._L.oracle_set_tabulated(self.h, _p(self._fq), _p(self._gq))
 
     def quadrature_points(self):
-        return lib().oracle_quadrature_points(self.h)
+        return self._L.oracle_quadrature_points(self.h)
 
     def residual(self, x, r=None):
         x = np.ascontiguousarray(x, dtype=np.float64)
         r = np.zeros(self.N) if r is None else r
-        self._check(lib().oracle_residual(self.h, _p(x), _p(r)))
+        self._check(self._L.oracle_residual(self.h, _p(x), _p(r)))
         return r
 
     def residual_twosided(self, x, nthreads=0):
         """R(x) with two-sided face visits, parallel over cells (nthreads <= 0: all)."""
         x = np.ascontiguousarray(x, dtype=np.float64)
         r = np.zeros(self.N)
-        self._check(lib().oracle_residual_twosided(self.h, _p(x), _p(r), nthreads))
+        self._check(self._L.oracle_residual_twosided(self.h, _p(x), _p(r), nthreads))
         return r
 
     def jacobian_apply(self, z, y=None):
         z = np.ascontiguousarray(z, dtype=np.float64)
         y = np.zeros(self.N) if y is None else y
-        self._check(lib().oracle_jacobian_apply(self.h, _p(z), _p(y)))
+        self._check(self._L.oracle_jacobian_apply(self.h, _p(z), _p(y)))
         return y
 
     def jacobian_apply_twosided(self, z, nthreads=0):
         z = np.ascontiguousarray(z, dtype=np.float64)
         y = np.zeros(self.N)
-        self._check(lib().oracle_jacobian_apply_twosided(self.h, _p(z), _p(y), nthreads))
+        self._check(self._L.oracle_jacobian_apply_twosided(self.h, _p(z), _p(y), nthreads))
         return y
 
     def jacobian_apply_transpose(self, z):
         z = np.ascontiguousarray(z, dtype=np.float64)
         y = np.zeros(self.N)
-        self._check(lib().oracle_jacobian_apply_transpose(self.h, _p(z), _p(y)))
+        self._check(self._L.oracle_jacobian_apply_transpose(self.h, _p(z), _p(y)))
         return y
 
     def jacobian_apply_transpose_twosided(self, z, nthreads=0):
         """J^T z from the same local blocks, gathered per column cell, parallel over cells."""
         z = np.ascontiguousarray(z, dtype=np.float64)
         y = np.zeros(self.N)
-        self._check(lib().oracle_jacobian_apply_transpose_twosided(self.h, _p(z), _p(y), nthreads))
+        self._check(self._L.oracle_jacobian_apply_transpose_twosided(self.h, _p(z), _p(y), nthreads))
         return y
 
     def mass_apply(self, z):
         z = np.ascontiguousarray(z, dtype=np.float64)
         y = np.zeros(self.N)
-        self._check(lib().oracle_mass_apply(self.h, _p(z), _p(y)))
+        self._check(self._L.oracle_mass_apply(self.h, _p(z), _p(y)))
         return y
 
     def point_diagonal(self):
         d = np.zeros(self.N)
-        self._check(lib().oracle_point_diagonal(self.h, _p(d)))
+        self._check(self._L.oracle_point_diagonal(self.h, _p(d)))
         return d
 
     def onestep_jacobian_apply(self, a_mass, b_dt, z, y=None):
         """y += b_dt J z + a_mass M z (returns y)."""
         z = np.ascontiguousarray(z, dtype=np.float64)
         y = np.zeros(self.N) if y is None else np.array(y, dtype=np.float64)
-        self._check(lib().oracle_onestep_jacobian_apply(self.h, a_mass, b_dt, _p(z), _p(y)))
+        self._check(self._L.oracle_onestep_jacobian_apply(self.h, a_mass, b_dt, _p(z), _p(y)))
         return y
 
     def onestep_step(self, xold, scheme, dt, xnew=None, method=cfgmod.SOLVER_CG, precond=cfgmod.PREC_JACOBI,
@@ -217,7 +255,7 @@ class Oracle:
         xold = np.ascontiguousarray(xold, dtype=np.float64)
         xnew = np.zeros(self.N) if xnew is None else np.array(xnew, dtype=np.float64)
         res = cfgmod.OneStepResult()
-        self._check(lib().oracle_onestep_step(self.h, scheme, method, precond, dt, reduction, maxit, _p(xold),
+        self._check(self._L.oracle_onestep_step(self.h, scheme, method, precond, dt, reduction, maxit, _p(xold),
                                               _p(xnew), ctypes.byref(res)))
         return xnew, res.as_dict()
 
@@ -225,14 +263,14 @@ class Oracle:
         """Diagonal blocks of J, shape (ncells, nloc, nloc), row-major (BlockDiagonalLocalOperatorWrapper)."""
         nloc = (self.cfg.degree + 1) ** 3
         b = np.zeros(self.ncells * nloc * nloc)
-        self._check(lib().oracle_block_diagonal(self.h, _p(b)))
+        self._check(self._L.oracle_block_diagonal(self.h, _p(b)))
         return b.reshape(self.ncells, nloc, nloc)
 
     def assemble(self):
         nloc = (self.cfg.degree + 1) ** 3
         blocks = np.zeros(self.ncells * 7 * nloc * nloc)
         cols = np.zeros(self.ncells * 7, dtype=np.int64)
-        self._check(lib().oracle_assemble(self.h, _p(blocks), cols.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))))
+        self._check(self._L.oracle_assemble(self.h, _p(blocks), cols.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))))
         return blocks.reshape(self.ncells, 7, nloc, nloc), cols.reshape(self.ncells, 7)
 
     def dense_matrix(self):
@@ -249,24 +287,24 @@ class Oracle:
 
     def interpolate(self):
         x = np.zeros(self.N)
-        self._check(lib().oracle_interpolate(self.h, _p(x)))
+        self._check(self._L.oracle_interpolate(self.h, _p(x)))
         return x
 
     def l2_error_sq(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)
-        return lib().oracle_l2_error_sq(self.h, _p(x))
+        return self._L.oracle_l2_error_sq(self.h, _p(x))
 
     def grad_logkappa(self, u, lam):
         u = np.ascontiguousarray(u, dtype=np.float64)
         lam = np.ascontiguousarray(lam, dtype=np.float64)
         g = np.zeros(self.ncells)
-        self._check(lib().oracle_grad_logkappa(self.h, _p(u), _p(lam), _p(g)))
+        self._check(self._L.oracle_grad_logkappa(self.h, _p(u), _p(lam), _p(g)))
         return g
 
     def grad_source(self, lam):
         lam = np.ascontiguousarray(lam, dtype=np.float64)
         g = np.zeros(self.ncells)
-        self._check(lib().oracle_grad_source(self.h, _p(lam), _p(g)))
+        self._check(self._L.oracle_grad_source(self.h, _p(lam), _p(g)))
         return g
 
     def solve(self, r, z=None, method=cfgmod.SOLVER_CG, precond=cfgmod.PREC_JACOBI, reduction=1e-10,
@@ -274,7 +312,7 @@ class Oracle:
         r = np.array(r, dtype=np.float64)
         z = np.zeros(self.N) if z is None else np.array(z, dtype=np.float64)
         res = cfgmod.LinearSolverResult()
-        self._check(lib().oracle_solve(self.h, method, precond, reduction, maxit, _p(z), _p(r),
+        self._check(self._L.oracle_solve(self.h, method, precond, reduction, maxit, _p(z), _p(r),
                                        ctypes.byref(res), matrix_based))
         return z, r, res.as_dict()
 
@@ -282,11 +320,11 @@ class Oracle:
                              reduction=1e-10, min_defect=1e-99, maxit=5000, matrix_based=0):
         x = np.zeros(self.N) if x is None else np.array(x, dtype=np.float64)
         res = cfgmod.LinearProblemResult()
-        self._check(lib().oracle_linear_problem_solve(self.h, method, precond, reduction, min_defect, maxit,
+        self._check(self._L.oracle_linear_problem_solve(self.h, method, precond, reduction, min_defect, maxit,
                                                       _p(x), ctypes.byref(res), matrix_based))
         return x, res.as_dict()
 
     def time_apply(self, nthreads=1, min_seconds=1.0):
         reps = ctypes.c_int(0)
-        sec = lib().oracle_time_apply(self.h, nthreads, min_seconds, ctypes.byref(reps))
+        sec = self._L.oracle_time_apply(self.h, nthreads, min_seconds, ctypes.byref(reps))
         return sec, reps.value

@@ -365,6 +365,69 @@ def test_mixed_precision_bicgstab_convection(pkg, of):
     assert rel(x, x_ref) <= 1e-6
 
 
+@pytest.mark.parametrize("variant", ["restarted_cg", "flexible_cg", "flexible_cg_block"])
+def test_mixed_precision_variants_cg(pkg, of, variant):
+    """The 64^3 lognormal-kappa problem at 12^3: restarted refinement (each FP32 CG to 1e-4) and
+    flexible GMRES over FP32 CG, against the oracle's residual of the returned solution."""
+    say(f"[gpu] mixed precision {variant} 12^3 Q3 lognormal kappa")
+    n = 12
+    cfg = pkg.make_config(n=n, degree=3, alpha=3.0, kappa_per_cell=True, problem=pkg.PROBLEM_CONSTANT, f0=1.0, g0=0.0)
+    kappa = np.exp(np.random.default_rng(8).normal(size=n ** 3))
+    go, o = pkg.GridOperator(cfg, kappa), of.Oracle(cfg, kappa)
+    o.set_threads(8)
+    flags = pkg.MIXED_RESTARTED if variant == "restarted_cg" else pkg.MIXED_FLEXIBLE
+    precond = pkg.PREC_BLOCK_JACOBI if variant.endswith("block") else pkg.PREC_JACOBI
+    xd = go.vector()
+    res = go.solve_mixed(xd, precond=precond, variant=flags, outer_reduction=1e-10, inner_maxit=20000, outer_maxit=200)
+    assert res["converged"], res
+    x = xd.download()
+    assert np.linalg.norm(o.residual(x)) <= 1.01e-10 * res["first_defect"]
+    x_ref, _ = o.linear_problem_solve(reduction=1e-13, maxit=20000)
+    assert rel(x, x_ref) <= 1e-8
+
+
+@pytest.mark.parametrize("precond", [1, 2], ids=["jacobi", "block_jacobi"])
+def test_mixed_precision_flexible_bicgstab_convection(pkg, of, precond):
+    """Flexible GMRES in FP64 over FP32 BiCGSTAB (to 1e-2 from zero) on the convection-diffusion
+    bench problem at 8^3, to 1e-8: the oracle's residual of the solution and its distance to the
+    oracle's FP64 BiCGSTAB solve to 1e-12."""
+    say(f"[gpu] flexible mixed BiCGSTAB 8^3 Q2 convection precond={precond}")
+    gauss = ((0.3, 0.4, 0.5, 0.1, 1.0), (0.7, 0.2, 0.6, 0.15, -0.5), (0.5, 0.8, 0.3, 0.08, 0.8))
+    cfg = pkg.make_config(n=8, degree=2, alpha=3.0, kappa0=1e-2, b=(1.0, 0.5, 0.25), problem=pkg.PROBLEM_GAUSSIANS,
+                          gaussians=gauss, g0=0.0)
+    go, o = pkg.GridOperator(cfg), of.Oracle(cfg)
+    o.set_threads(8)
+    xd = go.vector()
+    res = go.solve_mixed(xd, method=pkg.SOLVER_BICGSTAB, precond=precond, variant=pkg.MIXED_FLEXIBLE,
+                         outer_reduction=1e-8, inner_maxit=20000, outer_maxit=200)
+    assert res["converged"], res
+    x = xd.download()
+    assert np.linalg.norm(o.residual(x)) <= 1.01e-8 * res["first_defect"]
+    x_ref, _ = o.linear_problem_solve(method=1, reduction=1e-12, maxit=20000)
+    assert rel(x, x_ref) <= 1e-6
+
+
+def test_mixed_precision_cg_small_load(pkg, of):
+    """Reliable-update CG with a load of 1e-9: the FP32 recursion runs on the residual at unit
+    norm, so r.r does not underflow in FP32 and the solve converges to 1e-10 as with f = 1."""
+    say("[gpu] mixed precision CG, f0 = 1e-9")
+    n = 8
+    cfg = pkg.make_config(n=n, degree=2, alpha=3.0, problem=pkg.PROBLEM_CONSTANT, f0=1e-9, g0=0.0)
+    go, o = pkg.GridOperator(cfg), of.Oracle(cfg)
+    xd = go.vector()
+    res = go.solve_mixed(xd, outer_reduction=1e-10, inner_maxit=20000)
+    assert res["converged"], res
+    assert np.linalg.norm(o.residual(xd.download())) <= 1.01e-10 * res["first_defect"]
+
+
+def test_mixed_precision_variant_flags_rejected_together(pkg):
+    go = pkg.GridOperator(pkg.make_config(n=2, degree=1))
+    with pytest.raises(pkg.PdelabError) as e:
+        go.solve_mixed(go.vector(), variant=pkg.MIXED_RESTARTED | pkg.MIXED_FLEXIBLE)
+    assert e.value.status == 1
+    go.close()
+
+
 def test_adjoint_gradient_vs_oracle(pkg, of):
     say("[gpu] grad logkappa 6^3 Q2")
     n = 6
