@@ -496,6 +496,147 @@ __global__ __launch_bounds__(BJShape<N1>::THREADS) void bj_dense_kernel(const BJ
   }
 }
 
+// Dense inverse blocks shared by boundary type (constant coefficients), Q1 and Q2: the cells of
+// one type form a box (interior or boundary range in each direction), and every wave of this
+// kernel takes 64 consecutive cells of one type, one cell per lane, so the block is
+// wave-uniform: its rows are read by scalar loads from the row-major table and enter the
+// FMAs as scalar operands. The wave stages its cells' vectors through LDS (coalesced loads
+// and stores, each lane reading its own cell's column at a stride of NL values, which is odd
+// and so spreads the 32 lanes of a pass over distinct banks), keeps its cell's values in
+// registers and forms row i of D^{-1} u per step.
+struct BJTypes {
+  int first[65];       // first wave of boundary type t (t = tx + 4 ty + 16 tz); first[64] = all waves
+  int lo[3][4];        // first cell index of type t_d in direction d
+  int cnt[3][4];       // cells of type t_d in direction d
+};
+
+template <int N1, typename T, typename A, int MODE>
+__global__ __launch_bounds__(64) void bj_typed_kernel(const BJVec<T, A> v, const T* __restrict__ rm, const BJTypes ty,
+                                                      const BJGeom g) {
+  constexpr int NL = N1 * N1 * N1;
+  if (v.state && v.state[0]) return;  // a device-side Krylov loop has already stopped
+  __shared__ T buf[64 * NL];
+  const int lane = threadIdx.x;
+  const int wv = blockIdx.x;
+  int t = 0;
+  for (int s = 0; s < 64; ++s)
+    if (wv >= ty.first[s] && wv < ty.first[s + 1]) t = s;
+  const int tx = t & 3, tyy = (t >> 2) & 3, tz = t >> 4;
+  const int cx = ty.cnt[0][tx], cy = ty.cnt[1][tyy];
+  const long long cnt = (long long)cx * cy * ty.cnt[2][tz];
+  const long long base = (long long)(wv - ty.first[t]) * 64;
+  // global index of the lane's cell (local index base + lane of the type's box; fewer than 2^31
+  // cells, which bj_prepare checks)
+  int gc = -1;
+  {
+    const long long li = base + lane;
+    if (li < cnt) {
+      const unsigned lu = (unsigned)li;
+      const unsigned q = lu / (unsigned)cx, iz = q / (unsigned)cy;
+      const unsigned ix = lu - q * (unsigned)cx, iy = q - iz * (unsigned)cy;
+      gc = (int)((long long)(ty.lo[0][tx] + (int)ix) + g.stride[1] * (ty.lo[1][tyy] + (int)iy) +
+                 g.stride[2] * (ty.lo[2][tz] + (int)iz));
+    }
+  }
+  T lambda = T(0);
+  if (MODE == BJ_CGR) lambda = (T)*v.lam;
+  A acc0 = A(0), acc1 = A(0);
+  // coalesced staging: element k of local cell cl at buf[cl NL + k]. All loads of a phase are
+  // issued before the first use; lanes past the type's last cell load the first cell of the
+  // wave (which exists) and discard it.
+  const int cfirst = __shfl(gc, 0);
+  auto where = [&](int r, long long& idx) -> bool {
+    const int flat = r * 64 + lane, cl = flat / NL, k = flat - cl * NL;
+    const int c = __shfl(gc, cl);
+    idx = (long long)(c >= 0 ? c : cfirst) * NL + k;
+    return c >= 0;
+  };
+  {
+    T st[NL], sq[MODE == BJ_CGR ? NL : 1];
+#pragma unroll
+    for (int r = 0; r < NL; ++r) {
+      long long idx;
+      where(r, idx);
+      st[r] = v.in[idx];
+      if (MODE == BJ_CGR) sq[r] = v.q[idx];
+    }
+#pragma unroll
+    for (int r = 0; r < NL; ++r) {
+      long long idx;
+      const bool ok = where(r, idx);
+      T u = ok ? st[r] : T(0);
+      if (MODE == BJ_CGR) {
+        u = ok ? u - lambda * sq[r] : T(0);
+        if (ok) v.r[idx] = u;
+        acc0 += (A)u * (A)u;
+      }
+      buf[r * 64 + lane] = u;
+    }
+  }
+  wave_lds_sync();
+  T u[NL];
+#pragma unroll
+  for (int k = 0; k < NL; ++k) u[k] = buf[lane * NL + k];
+  wave_lds_sync();  // every lane holds its column before buf is reused for the results
+  const T* B = rm + (size_t)t * NL * NL;
+#pragma unroll 1
+  for (int i = 0; i < NL; ++i) {
+    T y = T(0);
+#pragma unroll
+    for (int k = 0; k < NL; ++k) y += B[i * NL + k] * u[k];
+    // slot i of the lane's column still holds u_i (read there: a register array indexed by
+    // the loop counter would live in scratch memory)
+    const T ui = buf[lane * NL + i];
+    buf[lane * NL + i] = y;
+    if (MODE == BJ_PREC) acc0 += (A)y * (A)ui;
+    if (MODE == BJ_CGR) acc1 += (A)y * (A)ui;
+  }
+  wave_lds_sync();
+  {
+    T sy[MODE == BJ_ADD ? NL : 1];
+    if (MODE == BJ_ADD) {
+#pragma unroll
+      for (int r = 0; r < NL; ++r) {
+        long long idx;
+        where(r, idx);
+        sy[r] = v.out[idx];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < NL; ++r) {
+      long long idx;
+      if (where(r, idx)) {
+        const T res = buf[r * 64 + lane];
+        if (MODE == BJ_ADD) v.out[idx] = sy[r] + res;
+        else v.out[idx] = res;
+      }
+    }
+  }
+  if (MODE == BJ_PREC) {
+    A s[1] = {acc0};
+    block_sum_n<64, A, 1>(s);
+    if (lane == 0) v.partial[wv] = s[0];
+  } else if (MODE == BJ_CGR) {
+    A s[2] = {acc0, acc1};
+    block_sum_n<64, A, 2>(s);
+    if (lane == 0) {
+      v.partial[2 * wv] = s[0];
+      v.partial[2 * wv + 1] = s[1];
+    }
+  }
+}
+
+// transposed dense blocks (storage j NL + i holds entry (i, j)) -> row-major copies
+template <typename T>
+__global__ __launch_bounds__(256) void bj_rowmajor_kernel(const T* __restrict__ src, T* __restrict__ dst, int nl,
+                                                          long long total) {
+  const long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= total) return;
+  const long long e = x / ((long long)nl * nl);
+  const int ij = (int)(x - e * nl * nl), i = ij / nl, j = ij - i * nl;
+  dst[x] = src[e * nl * nl + (long long)j * nl + i];
+}
+
 // Fast diagonalisation: lane (ix, iy) of a cell holds the z-line u(ix, iy, 0..N1-1). The
 // x and y sweeps exchange lines through the wave's LDS region; the z sweeps stay in
 // registers. D^{-1} = (Vz (x) Vy (x) Vx) diag(1/(lx + ly + lz)) (Vz^T (x) Vy^T (x) Vx^T).

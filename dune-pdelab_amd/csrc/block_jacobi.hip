@@ -48,8 +48,39 @@ long long groups_of(const H* h) {
   return (h->ncells + cpg - 1) / cpg;
 }
 long long bj_groups(const H* h, bool fdm) { return fdm ? groups_of<BJFdmShape>(h) : groups_of<BJShape>(h); }
+
+// Dense blocks shared by boundary type at Q1 and Q2 run bj_typed_kernel: 64 cells of one type per
+// wave. The cells of type t_d in direction d: interior 1 .. n-2, lower {0} and upper {n-1}
+// (n >= 2), both {0} (n = 1), as bj_type classifies them.
+bool typed_dense(const H* h, const BlockJacobiData& b) { return !b.fdm && !b.per_cell && h->n1 <= 3; }
+BJTypes typed_waves(const H* h) {
+  BJTypes ty;
+  std::memset(&ty, 0, sizeof(ty));
+  for (int d = 0; d < 3; ++d) {
+    const int n = h->a64.n[d];
+    ty.lo[d][0] = 1;
+    ty.cnt[d][0] = n >= 3 ? n - 2 : 0;
+    ty.lo[d][1] = 0;
+    ty.cnt[d][1] = n >= 2 ? 1 : 0;
+    ty.lo[d][2] = n - 1;
+    ty.cnt[d][2] = n >= 2 ? 1 : 0;
+    ty.lo[d][3] = 0;
+    ty.cnt[d][3] = n == 1 ? 1 : 0;
+  }
+  int w = 0;
+  for (int t = 0; t < 64; ++t) {
+    ty.first[t] = w;
+    const long long c = (long long)ty.cnt[0][t & 3] * ty.cnt[1][(t >> 2) & 3] * ty.cnt[2][t >> 4];
+    w += (int)((c + 63) / 64);
+  }
+  ty.first[64] = w;
+  return ty;
+}
+
 // partials of the reducing modes: one (or one pair) per workgroup
-long long bj_partials(const H* h, bool fdm) { return bj_groups(h, fdm); }
+long long bj_partials(const H* h, const BlockJacobiData& b) {
+  return typed_dense(h, b) ? typed_waves(h).first[64] : bj_groups(h, b.fdm);
+}
 
 BJGeom geom(const H* h, bool per_cell) {
   BJGeom g;
@@ -103,6 +134,13 @@ int setup_t(H* h, const BlockJacobiData& b, const BJGeom& g, double* o64, float*
   }
 }
 
+template <typename T>
+const T* bj_rowmajor(const BlockJacobiData& b);
+template <>
+const double* bj_rowmajor<double>(const BlockJacobiData& b) { return b.rm64; }
+template <>
+const float* bj_rowmajor<float>(const BlockJacobiData& b) { return b.rm32; }
+
 template <int N1, typename T, typename A, int MODE>
 int launch_n(H* h, const BlockJacobiData& b, const T* data, const BJVec<T, A>& v) {
   const BJGeom g = geom(h, b.per_cell);
@@ -116,7 +154,13 @@ int launch_n(H* h, const BlockJacobiData& b, const T* data, const BJVec<T, A>& v
   else if (b.fdm)
     hipLaunchKernelGGL((bj_fdm_kernel<N1, T, A, MODE, false>), dim3(nb), dim3(BJFdmShape<N1>::THREADS), 0, h->stream, v,
                        data, g);
-  else
+  else if (typed_dense(h, b)) {
+    if constexpr (N1 <= 3) {
+      const BJTypes ty = typed_waves(h);
+      hipLaunchKernelGGL((bj_typed_kernel<N1, T, A, MODE>), dim3((unsigned)ty.first[64]), dim3(64), 0, h->stream, v,
+                         bj_rowmajor<T>(b), ty, g);
+    }
+  } else
     hipLaunchKernelGGL((bj_dense_kernel<N1, T, A, MODE>), dim3(nb), dim3(BJShape<N1>::THREADS), 0, h->stream, v, data,
                        g);
   HIPCHK(h, hipGetLastError());
@@ -149,6 +193,8 @@ void bj_free(H* h) {
   for (auto& b : h->bj) {
     if (b.d64) (void)hipFree(b.d64);
     if (b.d32) (void)hipFree(b.d32);
+    if (b.rm64) (void)hipFree(b.rm64);
+    if (b.rm32) (void)hipFree(b.rm32);
     b = BlockJacobiData();
   }
   if (h->d_bjpart) (void)hipFree(h->d_bjpart);
@@ -166,6 +212,8 @@ int bj_prepare(H* h, bool trans, int precision) {
   if (b.fdm != fdm || b.per_cell != per_cell || b.entries != entries || b.esize != esize) {
     if (b.d64) (void)hipFree(b.d64);
     if (b.d32) (void)hipFree(b.d32);
+    if (b.rm64) (void)hipFree(b.rm64);
+    if (b.rm32) (void)hipFree(b.rm32);
     b = BlockJacobiData();
     b.fdm = fdm;
     b.per_cell = per_cell;
@@ -173,7 +221,9 @@ int bj_prepare(H* h, bool trans, int precision) {
     b.esize = esize;
   }
   if (!h->d_bjpart) {
-    const size_t np = (size_t)std::max(bj_partials(h, true), bj_partials(h, false));
+    // sized for every kernel choice (set_kappa may change the operator later)
+    const size_t np = (size_t)std::max(std::max(bj_groups(h, true), bj_groups(h, false)),
+                                       (long long)typed_waves(h).first[64]);
     HIPCHK(h, hipMalloc(&h->d_bjpart, sizeof(double) * 2 * np));
   }
   const bool need64 = precision == 64 && !b.valid64;
@@ -186,6 +236,20 @@ int bj_prepare(H* h, bool trans, int precision) {
   double* o64 = need64 ? b.d64 : nullptr;
   float* o32 = need32 ? b.d32 : nullptr;
   CHK(trans ? setup_t<true>(h, b, g, o64, o32) : setup_t<false>(h, b, g, o64, o32));
+  if (typed_dense(h, b)) {
+    const int nb = (int)((count + 255) / 256);
+    if (need64) {
+      if (!b.rm64) HIPCHK(h, hipMalloc(&b.rm64, sizeof(double) * count));
+      hipLaunchKernelGGL(bj_rowmajor_kernel<double>, dim3(nb), dim3(256), 0, h->stream, (const double*)b.d64, b.rm64,
+                         (int)nl, (long long)count);
+    }
+    if (need32) {
+      if (!b.rm32) HIPCHK(h, hipMalloc(&b.rm32, sizeof(float) * count));
+      hipLaunchKernelGGL(bj_rowmajor_kernel<float>, dim3(nb), dim3(256), 0, h->stream, (const float*)b.d32, b.rm32,
+                         (int)nl, (long long)count);
+    }
+    HIPCHK(h, hipGetLastError());
+  }
   if (need64) b.valid64 = true;
   if (need32) b.valid32 = true;
   return PDELAB_OK;
@@ -197,7 +261,7 @@ int bj_launch(H* h, bool trans, int mode, BJVec<T, A> v, int* nparts) {
   const BlockJacobiData& b = h->bj[trans ? 1 : 0];
   const T* data = bj_data<T>(b);
   v.partial = static_cast<A*>(h->d_bjpart);
-  if (nparts) *nparts = (int)bj_partials(h, b.fdm);
+  if (nparts) *nparts = (int)bj_partials(h, b);
   if (mode == BJ_PREC) return launch_m<T, A, BJ_PREC>(h, b, data, v);
   if (mode == BJ_ADD) return launch_m<T, A, BJ_ADD>(h, b, data, v);
   return launch_m<T, A, BJ_CGR>(h, b, data, v);

@@ -25,6 +25,9 @@ struct BlockJacobiData {
   long long entries = 0, esize = 0;
   double* d64 = nullptr;
   float* d32 = nullptr;
+  // row-major copies of per-type dense blocks (bj_typed_kernel)
+  double* rm64 = nullptr;
+  float* rm32 = nullptr;
 };
 
 struct pdelab_gpu_gridoperator {
