@@ -5,7 +5,7 @@ the struct size against the C compiler's.
 """
 import ctypes
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 MAX_GAUSSIANS = 16
 
 # ConvectionDiffusionDGMethod / Weights / BoundaryConditions
@@ -53,6 +53,7 @@ class PdelabConfig(ctypes.Structure):
         ("quad", ctypes.c_double * 7),
         ("n_gaussians", ctypes.c_int),
         ("gaussians", (ctypes.c_double * 5) * MAX_GAUSSIANS),
+        ("j0", ctypes.c_double), ("o0", ctypes.c_double),
     ]
 
 
@@ -122,7 +123,7 @@ def make_config(n=4, degree=1, nx=None, ny=None, nz=None, lx=1.0, ly=1.0, lz=1.0
                 method=SIPG, weights=WEIGHTS_ON, alpha=1.0, intorderadd=0,
                 kappa_per_cell=False, kappa0=1.0, b=(0.0, 0.0, 0.0), c=0.0,
                 bctype=(0, 0, 0, 0, 0, 0), problem=PROBLEM_ZERO, f0=0.0, g0=0.0,
-                quad=(0.0,) * 7, gaussians=()):
+                quad=(0.0,) * 7, gaussians=(), j0=0.0, o0=0.0):
     """Config with the reference defaults (SIPG, weightsOn, alpha=1,
     convectiondiffusiondg.hh:85-90), overridable by keyword."""
     cfg = PdelabConfig()
@@ -149,6 +150,7 @@ def make_config(n=4, degree=1, nx=None, ny=None, nz=None, lx=1.0, ly=1.0, lz=1.0
         cfg.quad[i] = quad[i]
     if len(gaussians) > MAX_GAUSSIANS:
         raise ValueError("too many gaussians")
+    cfg.j0, cfg.o0 = j0, o0
     cfg.n_gaussians = len(gaussians)
     for i, gsn in enumerate(gaussians):
         for j in range(5):

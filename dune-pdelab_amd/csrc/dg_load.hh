@@ -14,7 +14,7 @@ enum { PROB_ZERO = 0, PROB_EXP = 1, PROB_CONSTANT = 2, PROB_GAUSSIANS = 3, PROB_
 
 struct LoadArgs {
   int problem;
-  double kappa0, b[3], c, f0, g0, quad[7];
+  double kappa0, b[3], c, f0, g0, quad[7], j0, o0;
   int n_gauss;
   double gauss[16][5];
   int n[3];
@@ -73,6 +73,21 @@ __device__ inline double eval_f(const LoadArgs& a, const double x[3]) {
   }
 }
 
+// Neumann flux j ((b u - A grad u).n = j) and outflow data o (-(A grad u).n = o) of the
+// built-in problems at x on a boundary side with outer normal nsign e_d
+// (convectiondiffusionparameter.hh:121-124, 196-208): the fluxes of the exact solution
+// (A = kappa0 I) for EXP and QUADRATIC, the constants j0 / o0 otherwise.
+__device__ inline double eval_flux(const LoadArgs& a, const double x[3], int d, double nsign, bool neumann) {
+  if (a.problem == PROB_EXP || a.problem == PROB_QUADRATIC) {
+    double gd;
+    if (a.problem == PROB_EXP) gd = -2.0 * (x[d] - 0.5) * exact_u(a, x);
+    else gd = a.quad[1 + d] + 2.0 * a.quad[4 + d] * x[d];
+    const double diff = -a.kappa0 * nsign * gd;
+    return neumann ? nsign * a.b[d] * exact_u(a, x) + diff : diff;
+  }
+  return neumann ? a.j0 : a.o0;
+}
+
 __device__ inline double eval_g(const LoadArgs& a, const double x[3]) {
   switch (a.problem) {
     case PROB_EXP:
@@ -118,14 +133,15 @@ __global__ __launch_bounds__(256) void load_volume_kernel(double* __restrict__ R
   R0[gid] += -(ext[0] * ext[1] * ext[2]) * acc;
 }
 
-// R0 += Dirichlet data terms of residual_boundary_integral (convectiondiffusiondg.hh:
-// 839-877 with u = 0): inflow convection g (b.n) psi, -theta g (A n . grad psi),
-// -penalty g psi. One launch per boundary side (no two threads of a launch write
-// the same entry); one thread per (boundary cell, DoF).
+// R0 += data terms of residual_boundary_integral (u = 0): on Dirichlet sides
+// (convectiondiffusiondg.hh:839-877) inflow convection g (b.n) psi, -theta g (A n . grad psi),
+// -penalty g psi; on Neumann sides (:776-783) j psi; on outflow sides (:810-815) o psi.
+// One launch per boundary side (no two threads of a launch write the same entry); one
+// thread per (boundary cell, DoF). bct is the side's boundary type.
 template <int N1>
 __global__ __launch_bounds__(256) void load_boundary_kernel(double* __restrict__ R0, const double* __restrict__ gtab,
                                                             const double* __restrict__ kappa, const LoadArgs a,
-                                                            int side) {
+                                                            int side, int bct) {
   constexpr int L = N1 * N1, NL = L * N1;
   const int d = side / 2, upper = side % 2;
   const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
@@ -143,6 +159,7 @@ __global__ __launch_bounds__(256) void load_boundary_kernel(double* __restrict__
   double lo[3], ext[3];
   for (int e = 0; e < 3; ++e) { lo[e] = coord[e] * a.h[e]; ext[e] = (coord[e] + 1) * a.h[e] - lo[e]; }
   const int m = a.m;
+  const double nsign = upper ? 1.0 : -1.0;
   double gt = 0.0;
   for (int q2 = 0; q2 < m; ++q2)
     for (int q1 = 0; q1 < m; ++q1) {
@@ -154,18 +171,21 @@ __global__ __launch_bounds__(256) void load_boundary_kernel(double* __restrict__
         x[d] = lo[d] + ext[d] * (upper ? 1.0 : 0.0);
         x[t1] = lo[t1] + ext[t1] * a.qx[q1];
         x[t2] = lo[t2] + ext[t2] * a.qx[q2];
-        g = eval_g(a, x);
+        g = bct == BC_DIRICHLET ? eval_g(a, x) : eval_flux(a, x, d, nsign, bct == BC_NEUMANN);
       }
       gt += g * a.qw[q1] * a.qw[q2] * a.P[q1][ii[t1]] * a.P[q2][ii[t2]];
     }
   const double vol = ext[0] * ext[1] * ext[2];
   const double area = vol / ext[d];
-  const double hF = vol / area;
-  const double kc = kappa ? kappa[c] : a.kappa0;
-  const double nsign = upper ? 1.0 : -1.0;
-  const double nf = nsign * a.b[d];
   const int i = ii[d];
   const double es = upper ? a.e1[i] : a.e0[i];
+  if (bct != BC_DIRICHLET) {  // Neumann flux or outflow data: + int data psi
+    R0[c * NL + li] += area * gt * es;
+    return;
+  }
+  const double hF = vol / area;
+  const double kc = kappa ? kappa[c] : a.kappa0;
+  const double nf = nsign * a.b[d];
   const double ds = upper ? a.d1[i] : a.d0[i];
   const double Pb = (a.alphak / hF) * (a.weightsOn ? kc : 1.0);
   double coef = 0.0;
@@ -199,13 +219,16 @@ struct GradArgs {
   int bc[6];
 };
 
-// g[c] = -lambda^T (dJ/dlog kappa_c) u for weighted SIPG/NIPG/IIPG diffusion
-// (weightsOn). All kappa dependence is through kappa_c (volume, boundary faces) and
-// through w = kappa_lo*kappa_hi/S and H = 2w on interior faces
-// (convectiondiffusiondg.hh:326-333, 346): d w / d log kappa_c = D and
-// d(penalty)/d log kappa_c = 2 alpha k(k+2) D with D = kappa_c kappa_n (kappa_n + 1e-20)/S^2.
+// g[c] = -lambda^T (dJ/dlog kappa_c) u for SIPG/NIPG/IIPG diffusion. All kappa dependence
+// is through kappa_c (volume, boundary faces) and through the face weights:
+// * weightsOn: w = kappa_lo*kappa_hi/S and H = 2w on interior faces
+//   (convectiondiffusiondg.hh:326-333, 346): d w / d log kappa_c = D and
+//   d(penalty)/d log kappa_c = 2 alpha k(k+2) D with D = kappa_c kappa_n (kappa_n + 1e-20)/S^2;
+//   the boundary penalty is proportional to kappa_c (:717-734);
+// * weightsOff: the weight of each side's normal derivative is kappa/2 of that side, the
+//   penalties do not depend on kappa.
 // One thread per cell; every interior face is evaluated by both of its cells.
-template <int N1>
+template <int N1, bool WON>
 __global__ __launch_bounds__(128) void grad_logkappa_kernel(const double* __restrict__ u, const double* __restrict__ lam,
                                                             const double* __restrict__ kappa, double* __restrict__ g,
                                                             const GradArgs a) {
@@ -250,18 +273,25 @@ __global__ __launch_bounds__(128) void grad_logkappa_kernel(const double* __rest
           for (int e = 0; e < N1; ++e) {
             const int ix = D == 0 ? e + N1 * p + L * q : (D == 1 ? p + N1 * e + L * q : p + N1 * q + L * e);
             ju += a.e1[e] * ulo[ix] - a.e0[e] * uhi[ix];
-            gu += a.d1[e] * ulo[ix] + a.d0[e] * uhi[ix];
             jl += a.e1[e] * llo[ix] - a.e0[e] * lhi[ix];
-            gl += a.d1[e] * llo[ix] + a.d0[e] * lhi[ix];
+            // weightsOn: both sides' normal derivatives carry w; weightsOff: only this
+            // cell's (the lower cell's d1 part or the upper cell's d0 part) depends on kappa_c
+            if (WON || upper) { gu += a.d1[e] * ulo[ix]; gl += a.d1[e] * llo[ix]; }
+            if (WON || !upper) { gu += a.d0[e] * uhi[ix]; gl += a.d0[e] * lhi[ix]; }
           }
           Ju[t] = ju; Gu[t] = gu; Jl[t] = jl; Gl[t] = gl;
         }
-        const double S = kc + kn + 1e-20;
-        const double Dw = kc * kn * (kn + 1e-20) / (S * S);
-        // face form with w = 1 and penalty 2*alpha*k(k+2): -G(u)J(l) + theta J(u)G(l) + P J(u)J(l)
-        const double f = -face_mass_dot<N1>(a.M, Jl, Gu) + a.theta * face_mass_dot<N1>(a.M, Gl, Ju) +
-                         2.0 * a.alphak * face_mass_dot<N1>(a.M, Jl, Ju);
-        acc += Dw * Sd * f;
+        if (WON) {
+          const double S = kc + kn + 1e-20;
+          const double Dw = kc * kn * (kn + 1e-20) / (S * S);
+          // face form with w = 1 and penalty 2*alpha*k(k+2): -G(u)J(l) + theta J(u)G(l) + P J(u)J(l)
+          const double f = -face_mass_dot<N1>(a.M, Jl, Gu) + a.theta * face_mass_dot<N1>(a.M, Gl, Ju) +
+                           2.0 * a.alphak * face_mass_dot<N1>(a.M, Jl, Ju);
+          acc += Dw * Sd * f;
+        } else {
+          const double f = -face_mass_dot<N1>(a.M, Jl, Gu) + a.theta * face_mass_dot<N1>(a.M, Gl, Ju);
+          acc += 0.5 * kc * Sd * f;
+        }
       } else if (a.bc[2 * D + side] == BC_DIRICHLET) {
         for (int t = 0; t < L; ++t) {
           const int p = t % N1, q = t / N1;
@@ -273,15 +303,73 @@ __global__ __launch_bounds__(128) void grad_logkappa_kernel(const double* __rest
           }
           Ju[t] = vu; Gu[t] = du; Jl[t] = vl; Gl[t] = dl;
         }
-        // lower: (l0)(du0) + P l0 u0 - theta (dl0)(u0); upper: -(l1)(du1) + P l1 u1 + theta (dl1)(u1)
+        // lower: (l0)(du0) + P l0 u0 - theta (dl0)(u0); upper: -(l1)(du1) + P l1 u1 + theta (dl1)(u1);
+        // the penalty P is proportional to kappa_c with weightsOn only
         const double sg = upper ? -1.0 : 1.0;
-        const double f = sg * face_mass_dot<N1>(a.M, Jl, Gu) + a.alphak * face_mass_dot<N1>(a.M, Jl, Ju) -
-                         sg * a.theta * face_mass_dot<N1>(a.M, Gl, Ju);
+        double f = sg * face_mass_dot<N1>(a.M, Jl, Gu) - sg * a.theta * face_mass_dot<N1>(a.M, Gl, Ju);
+        if (WON) f += a.alphak * face_mass_dot<N1>(a.M, Jl, Ju);
         acc += kc * Sd * f;
       }
     }
   }
   g[c] = -acc;
+}
+
+// g[c] -= lambda_c^T dR0_c / d log kappa_c for the Dirichlet data terms of one boundary side
+// (load_boundary_kernel): -theta g kappa_c (n . grad psi) and, with weightsOn, the penalty
+// term -(alpha k(k+2) / h_F) kappa_c g psi. One launch per Dirichlet side, one thread per
+// boundary cell of the side (a cell on several sides is updated by successive launches).
+template <int N1>
+__global__ __launch_bounds__(128) void grad_logkappa_dirichlet_kernel(const double* __restrict__ lam,
+                                                                      const double* __restrict__ gtab,
+                                                                      const double* __restrict__ kappa,
+                                                                      double* __restrict__ g, const LoadArgs a,
+                                                                      int side) {
+  constexpr int L = N1 * N1, NL = L * N1;
+  const int d = side / 2, upper = side % 2;
+  const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+  const long long nfaces = (long long)a.n[t1] * a.n[t2];
+  const long long fidx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (fidx >= nfaces) return;
+  int coord[3];
+  coord[d] = upper ? a.n[d] - 1 : 0;
+  coord[t1] = (int)(fidx % a.n[t1]);
+  coord[t2] = (int)(fidx / a.n[t1]);
+  const long long c = coord[0] + (long long)a.n[0] * (coord[1] + (long long)a.n[1] * coord[2]);
+  double lo[3], ext[3];
+  for (int e = 0; e < 3; ++e) { lo[e] = coord[e] * a.h[e]; ext[e] = (coord[e] + 1) * a.h[e] - lo[e]; }
+  const int m = a.m;
+  const double vol = ext[0] * ext[1] * ext[2];
+  const double area = vol / ext[d];
+  const double hF = vol / area;
+  const double kc = kappa[c];
+  const double nsign = upper ? 1.0 : -1.0;
+  double acc = 0.0;
+  for (int li = 0; li < NL; ++li) {
+    const int ii[3] = {li % N1, (li / N1) % N1, li / L};
+    double gt = 0.0;
+    for (int q2 = 0; q2 < m; ++q2)
+      for (int q1 = 0; q1 < m; ++q1) {
+        double gv;
+        if (a.problem == PROB_TABULATED) {
+          gv = gtab ? gtab[(a.bface_offset[side] + fidx) * (m * m) + q1 + m * q2] : 0.0;
+        } else {
+          double x[3];
+          x[d] = lo[d] + ext[d] * (upper ? 1.0 : 0.0);
+          x[t1] = lo[t1] + ext[t1] * a.qx[q1];
+          x[t2] = lo[t2] + ext[t2] * a.qx[q2];
+          gv = eval_g(a, x);
+        }
+        gt += gv * a.qw[q1] * a.qw[q2] * a.P[q1][ii[t1]] * a.P[q2][ii[t2]];
+      }
+    const int i = ii[d];
+    const double es = upper ? a.e1[i] : a.e0[i];
+    const double ds = upper ? a.d1[i] : a.d0[i];
+    double coef = -a.theta * kc * nsign * ds / ext[d];
+    if (a.weightsOn) coef += -(a.alphak / hF) * kc * es;
+    acc += lam[c * NL + li] * area * gt * coef;
+  }
+  g[c] -= acc;
 }
 
 // g[c] = sum_i lambda_{c,i} int_c phi_i (derivative of -R w.r.t. a per-cell source)

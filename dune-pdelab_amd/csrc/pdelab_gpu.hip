@@ -265,6 +265,8 @@ static void fill_load_args(H* h) {
   a.c = c.c;
   a.f0 = c.f0;
   a.g0 = c.g0;
+  a.j0 = c.j0;
+  a.o0 = c.o0;
   for (int i = 0; i < 7; ++i) a.quad[i] = c.quad[i];
   a.n_gauss = c.n_gaussians;
   for (int i = 0; i < c.n_gaussians && i < 16; ++i)
@@ -374,20 +376,26 @@ static int compute_R0(H* h) {
     case 4: hipLaunchKernelGGL(load_volume_kernel<4>, dim3(nb), dim3(256), 0, h->stream, h->d_R0, h->d_ftab, h->d_src, h->la); break;
   }
   HIPCHK(h, hipGetLastError());
-  const bool has_g = h->cfg.problem != PDELAB_PROBLEM_ZERO;
-  if (has_g) {
-    for (int s = 0; s < 6; ++s) {
-      if (h->a64.bc[s] != BC_DIRICHLET) continue;
-      const int d = s / 2, t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
-      const long long nf = (long long)h->la.n[t1] * h->la.n[t2] * h->nloc;
-      const int gb = (int)((nf + 255) / 256);
-      switch (h->n1) {
-        case 2: hipLaunchKernelGGL(load_boundary_kernel<2>, dim3(gb), dim3(256), 0, h->stream, h->d_R0, h->d_gtab, h->d_kappa, h->la, s); break;
-        case 3: hipLaunchKernelGGL(load_boundary_kernel<3>, dim3(gb), dim3(256), 0, h->stream, h->d_R0, h->d_gtab, h->d_kappa, h->la, s); break;
-        case 4: hipLaunchKernelGGL(load_boundary_kernel<4>, dim3(gb), dim3(256), 0, h->stream, h->d_R0, h->d_gtab, h->d_kappa, h->la, s); break;
-      }
-      HIPCHK(h, hipGetLastError());
+  // boundary data: g on Dirichlet sides (none for the ZERO problem), j on Neumann and o on
+  // outflow sides (j0 / o0 for the problems without an exact solution)
+  const int prob = h->cfg.problem;
+  const bool exact_fluxes = prob == PDELAB_PROBLEM_EXP || prob == PDELAB_PROBLEM_QUADRATIC || prob == PDELAB_PROBLEM_TABULATED;
+  for (int s = 0; s < 6; ++s) {
+    const int bct = h->a64.bc[s];
+    bool data = false;
+    if (bct == BC_DIRICHLET) data = prob != PDELAB_PROBLEM_ZERO;
+    else if (bct == BC_NEUMANN) data = exact_fluxes || h->cfg.j0 != 0.0;
+    else if (bct == BC_OUTFLOW) data = exact_fluxes || h->cfg.o0 != 0.0;
+    if (!data) continue;
+    const int d = s / 2, t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+    const long long nf = (long long)h->la.n[t1] * h->la.n[t2] * h->nloc;
+    const int gb = (int)((nf + 255) / 256);
+    switch (h->n1) {
+      case 2: hipLaunchKernelGGL(load_boundary_kernel<2>, dim3(gb), dim3(256), 0, h->stream, h->d_R0, h->d_gtab, h->d_kappa, h->la, s, bct); break;
+      case 3: hipLaunchKernelGGL(load_boundary_kernel<3>, dim3(gb), dim3(256), 0, h->stream, h->d_R0, h->d_gtab, h->d_kappa, h->la, s, bct); break;
+      case 4: hipLaunchKernelGGL(load_boundary_kernel<4>, dim3(gb), dim3(256), 0, h->stream, h->d_R0, h->d_gtab, h->d_kappa, h->la, s, bct); break;
     }
+    HIPCHK(h, hipGetLastError());
   }
   h->R0_valid = true;
   return PDELAB_OK;
@@ -1792,22 +1800,37 @@ int pdelab_gpu_onestep_step(pdelab_gpu_gridoperator* h, int scheme, int method, 
 
 int pdelab_gpu_grad_logkappa(pdelab_gpu_gridoperator* h, const double* u, const double* lambda, double* g) {
   if (!h || !u || !lambda || !g) return PDELAB_ERR_INVALID_ARG;
-  if (h->cfg.weights != PDELAB_WEIGHTS_ON)
-    return fail(h, PDELAB_ERR_UNSUPPORTED, "kappa gradient implemented for weightsOn");
-  const int p = h->cfg.problem;
-  const bool g_zero = p == PDELAB_PROBLEM_ZERO || ((p == PDELAB_PROBLEM_CONSTANT || p == PDELAB_PROBLEM_GAUSSIANS) && h->cfg.g0 == 0.0);
-  if (!g_zero) return fail(h, PDELAB_ERR_UNSUPPORTED, "kappa gradient implemented for Dirichlet data g = 0");
   if (!h->d_kappa) {
     std::vector<double> k(h->ncells, h->cfg.kappa0);
     CHK(upload_kappa(h, k.data()));
   }
+  const bool won = h->cfg.weights == PDELAB_WEIGHTS_ON;
   const int nb = (int)((h->ncells + 127) / 128);
+#define GRADL(N1_)                                                                                                    \
+  if (won) hipLaunchKernelGGL((grad_logkappa_kernel<N1_, true>), dim3(nb), dim3(128), 0, h->stream, u, lambda, h->d_kappa, g, h->ga); \
+  else hipLaunchKernelGGL((grad_logkappa_kernel<N1_, false>), dim3(nb), dim3(128), 0, h->stream, u, lambda, h->d_kappa, g, h->ga);
   switch (h->n1) {
-    case 2: hipLaunchKernelGGL(grad_logkappa_kernel<2>, dim3(nb), dim3(128), 0, h->stream, u, lambda, h->d_kappa, g, h->ga); break;
-    case 3: hipLaunchKernelGGL(grad_logkappa_kernel<3>, dim3(nb), dim3(128), 0, h->stream, u, lambda, h->d_kappa, g, h->ga); break;
-    case 4: hipLaunchKernelGGL(grad_logkappa_kernel<4>, dim3(nb), dim3(128), 0, h->stream, u, lambda, h->d_kappa, g, h->ga); break;
+    case 2: GRADL(2); break;
+    case 3: GRADL(3); break;
+    case 4: GRADL(4); break;
   }
+#undef GRADL
   HIPCHK(h, hipGetLastError());
+  // the Dirichlet data terms of the residual scale with kappa_c too (g != 0)
+  if (h->cfg.problem != PDELAB_PROBLEM_ZERO) {
+    for (int s = 0; s < 6; ++s) {
+      if (h->a64.bc[s] != BC_DIRICHLET) continue;
+      const int d = s / 2, t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+      const long long nf = (long long)h->la.n[t1] * h->la.n[t2];
+      const int gb = (int)((nf + 127) / 128);
+      switch (h->n1) {
+        case 2: hipLaunchKernelGGL(grad_logkappa_dirichlet_kernel<2>, dim3(gb), dim3(128), 0, h->stream, lambda, h->d_gtab, h->d_kappa, g, h->la, s); break;
+        case 3: hipLaunchKernelGGL(grad_logkappa_dirichlet_kernel<3>, dim3(gb), dim3(128), 0, h->stream, lambda, h->d_gtab, h->d_kappa, g, h->la, s); break;
+        case 4: hipLaunchKernelGGL(grad_logkappa_dirichlet_kernel<4>, dim3(gb), dim3(128), 0, h->stream, lambda, h->d_gtab, h->d_kappa, g, h->la, s); break;
+      }
+      HIPCHK(h, hipGetLastError());
+    }
+  }
   return PDELAB_OK;
 }
 

@@ -195,6 +195,10 @@ struct ConvectionDiffusionModelProblem {
     auto xg = e.geometry().global(x);
     return std::sqrt(xg[0] * xg[0] + xg[1] * xg[1] + xg[2] * xg[2]);
   }
+  // Neumann flux (b u - A grad u).n = j and outflow data -(A grad u).n = o
+  // (convectiondiffusionparameter.hh:121-124, 196-208), at face-local coordinates
+  double j(const Intersection&, const std::array<double, 2>&) const { return 0.0; }
+  double o(const Intersection&, const std::array<double, 2>&) const { return 0.0; }
 };
 
 struct ConvectionDiffusionDGMethod {
@@ -324,6 +328,7 @@ class GridOperator {
     for (int s = 0; s < 6; ++s) {
       Intersection is{s, detail::element(grid, 0)};
       cfg.bctype[s] = par.bctype(is, {0.5, 0.5});
+      cfg_bctype_[s] = cfg.bctype[s];
     }
     cfg.problem = PDELAB_PROBLEM_TABULATED;
     check(pdelab_gpu_create(&cfg, kappa.data(), &h_), nullptr);
@@ -381,13 +386,19 @@ class GridOperator {
           ijk[t2] = a2;
           size_t c = (size_t)ijk[0] + (size_t)n[0] * ((size_t)ijk[1] + (size_t)n[1] * ijk[2]);
           Element e = detail::element(grid, c);
+          const Intersection is{s, e};
+          // the boundary value of the side's type: g (Dirichlet), j (Neumann), o (outflow)
+          const int bct = cfg_bctype_[s];
           for (int q2 = 0; q2 < m; ++q2)
             for (int q1 = 0; q1 < m; ++q1) {
               std::array<double, 3> xi;
               xi[d] = (s % 2) ? 1.0 : 0.0;
               xi[t1] = qx[q1];
               xi[t2] = qx[q2];
-              gq.push_back(par.g(e, xi));
+              const std::array<double, 2> xf = {qx[q1], qx[q2]};
+              if (bct == ConvectionDiffusionBoundaryConditions::Neumann) gq.push_back(par.j(is, xf));
+              else if (bct == ConvectionDiffusionBoundaryConditions::Outflow) gq.push_back(par.o(is, xf));
+              else gq.push_back(par.g(e, xi));
             }
         }
     }
@@ -397,6 +408,7 @@ class GridOperator {
   const GFSU& gfsu_;
   const GFSV& gfsv_;
   const LOP& lop_;
+  int cfg_bctype_[6] = {0, 0, 0, 0, 0, 0};
   LocalAssembler la_;
   pdelab_gpu_gridoperator* h_ = nullptr;
 };

@@ -78,7 +78,7 @@
 extern "C" {
 #endif
 
-#define PDELAB_GPU_ABI_VERSION 1
+#define PDELAB_GPU_ABI_VERSION 2
 #define PDELAB_MAX_GAUSSIANS 16
 
 /* Status codes. */
@@ -101,9 +101,12 @@ enum { PDELAB_WEIGHTS_ON = 0, PDELAB_WEIGHTS_OFF = 1 };
    R:dune/pdelab/localoperator/convectiondiffusionparameter.hh:111-115 */
 enum { PDELAB_BC_DIRICHLET = 1, PDELAB_BC_NEUMANN = -1, PDELAB_BC_OUTFLOW = -2, PDELAB_BC_NONE = -3 };
 
-/* Source term f and Dirichlet data g of the parameter class
-   (ConvectionDiffusionModelProblem::f / ::g, convectiondiffusionparameter.hh:174-194).
-   The built-in problems are evaluated on the device at the quadrature points. */
+/* Source term f, Dirichlet data g, Neumann flux j and outflow data o of the parameter class
+   (ConvectionDiffusionModelProblem::f / ::g / ::j / ::o, convectiondiffusionparameter.hh:174-208;
+   (b u - A grad u).n = j on Neumann sides, -(A grad u).n = o on outflow sides, :121-124).
+   The built-in problems are evaluated on the device at the quadrature points. The problems
+   with an exact solution u (EXP, QUADRATIC) take its fluxes, j = (b.n) u - kappa0 grad(u).n
+   and o = -kappa0 grad(u).n; the others the constants j0 and o0 of the config. */
 enum {
   PDELAB_PROBLEM_ZERO = 0,      /* f = 0, g = 0 */
   PDELAB_PROBLEM_EXP = 1,       /* u = exp(-|x-(1/2,1/2,1/2)|^2); f = -kappa0*lap(u) + b.grad(u) + c u; g = u */
@@ -146,6 +149,7 @@ typedef struct pdelab_config {
   double quad[7];             /* QUADRATIC: q0, q_x, q_y, q_z, p_x, p_y, p_z */
   int n_gaussians;
   double gaussians[PDELAB_MAX_GAUSSIANS][5]; /* x, y, z, sigma, amplitude */
+  double j0, o0;              /* ZERO / CONSTANT / GAUSSIANS: Neumann flux j and outflow data o */
 } pdelab_config;
 
 /* LinearSolverResult<double>, R:dune/pdelab/backend/solver.hh:29-51 */
@@ -223,7 +227,9 @@ int pdelab_gpu_set_cell_source(pdelab_gpu_gridoperator* h, const double* source_
 /* PDELAB_PROBLEM_TABULATED data, host arrays:
    f_q: per cell, volume quadrature points, x fastest: nx*ny*nz*m^3,
    g_q: per boundary face (side-major: -x,+x,-y,+y,-z,+z; within a side faces
-        lexicographic in the two tangential cell indices), m^2 points each,
+        lexicographic in the two tangential cell indices), m^2 points each: the Dirichlet
+        value g on Dirichlet sides, the Neumann flux j on Neumann sides, the outflow data
+        o on outflow sides,
    with m = (2k + intorderadd)/2 + 1 Gauss points per direction. */
 int pdelab_gpu_set_tabulated_data(pdelab_gpu_gridoperator* h, const double* f_q, const double* g_q);
 int pdelab_gpu_quadrature_points(const pdelab_gpu_gridoperator* h); /* m */
@@ -354,7 +360,10 @@ int pdelab_gpu_l2_error_squared_noalloc(pdelab_gpu_gridoperator* h, const double
 int pdelab_gpu_sample(pdelab_gpu_gridoperator* h, const double* x, int intervals, double* values);
 
 /* ---- adjoint gradients ---------------------------------------------------- */
-/* g[c] = -lambda^T (dJ/dlog kappa_c) u for every cell (device arrays, g has nx*ny*nz). */
+/* g[c] = -lambda^T (dR/dlog kappa_c)(u) for every cell (device arrays, g has nx*ny*nz): the
+   derivative of the residual R(u) = J u + R(0), i.e. of the operator and, with Dirichlet data
+   g != 0, of the data terms that scale with kappa (R:dune/pdelab/localoperator/
+   convectiondiffusiondg.hh:717-734, 839-877); weightsOn and weightsOff. */
 int pdelab_gpu_grad_logkappa(pdelab_gpu_gridoperator* h, const double* u, const double* lambda,
                              double* g);
 /* g[c] = lambda^T (d b / d s_c): the load-vector derivative for a per-cell source s_c. */

@@ -240,6 +240,42 @@ struct Param {
     if (!cell_source.empty()) val += cell_source[cell];
     return val;
   }
+  // gradient of the analytic solution (EXP, QUADRATIC)
+  void grad_exact(const double x[3], double gr[3]) const {
+    if (cfg.problem == PDELAB_PROBLEM_EXP) {
+      const double u = exact(x);
+      for (int d = 0; d < 3; ++d) gr[d] = -2.0 * (x[d] - 0.5) * u;
+    } else if (cfg.problem == PDELAB_PROBLEM_QUADRATIC) {
+      const double* q = cfg.quad;
+      for (int d = 0; d < 3; ++d) gr[d] = q[1 + d] + 2.0 * q[4 + d] * x[d];
+    } else {
+      gr[0] = gr[1] = gr[2] = 0.0;
+    }
+  }
+  // Neumann flux j(x) with (b u - A grad u).n = j and outflow data o(x) with
+  // -(A grad u).n = o (convectiondiffusionparameter.hh:121-124, 196-208): the fluxes of the
+  // analytic solution (A = kappa0 I) for EXP and QUADRATIC, the constants j0 / o0 otherwise,
+  // the tabulated boundary value for TABULATED. n is the unit outer normal.
+  double j(const double x[3], const double n[3], size_t tabq) const {
+    if (cfg.problem == PDELAB_PROBLEM_TABULATED) return g_tab.empty() ? 0.0 : g_tab[tabq];
+    if (cfg.problem == PDELAB_PROBLEM_EXP || cfg.problem == PDELAB_PROBLEM_QUADRATIC) {
+      double gr[3];
+      grad_exact(x, gr);
+      double bn = 0.0, gn = 0.0;
+      for (int d = 0; d < 3; ++d) { bn += cfg.b[d] * n[d]; gn += gr[d] * n[d]; }
+      return bn * exact(x) - cfg.kappa0 * gn;
+    }
+    return cfg.j0;
+  }
+  double o(const double x[3], const double n[3], size_t tabq) const {
+    if (cfg.problem == PDELAB_PROBLEM_TABULATED) return g_tab.empty() ? 0.0 : g_tab[tabq];
+    if (cfg.problem == PDELAB_PROBLEM_EXP || cfg.problem == PDELAB_PROBLEM_QUADRATIC) {
+      double gr[3];
+      grad_exact(x, gr);
+      return -cfg.kappa0 * (gr[0] * n[0] + gr[1] * n[1] + gr[2] * n[2]);
+    }
+    return cfg.o0;
+  }
   // g(x) (ConvectionDiffusionModelProblem::g); tabq = index into g_tab
   double g(const double x[3], size_t tabq) const {
     switch (cfg.problem) {
@@ -639,18 +675,29 @@ struct LocalOperator {
       size_t o = (size_t)side * nf + q;
       const double* phi = &tab.fphi[o * nloc];
       const double factor = tab.fw[q] * bc.factor_area;
+      double xg[3];
+      const double* xi = &tab.fxi[3 * o];
+      for (int e = 0; e < 3; ++e) xg[e] = lo[e] + ext[e] * xi[e];
       if (bct == PDELAB_BC_NEUMANN) {
-        // j = 0 for every problem of this build (model problem j, parameter.hh:196-201)
+        // :776-783, the flux data only
+        if (!jac_apply) {
+          const double jv = param.j(xg, n, bface * nf + q);
+          for (int i = 0; i < nloc; ++i) rs[i] += jv * phi[i] * factor;
+        }
         continue;
       }
       double u = 0.0;
       for (int i = 0; i < nloc; ++i) u += xs[i] * phi[i];
       const double normalflux = b[0] * n[0] + b[1] * n[1] + b[2] * n[2];
       if (bct == PDELAB_BC_OUTFLOW) {
+        // :794-816
         if (normalflux < -1e-30) throw std::runtime_error("Outflow boundary condition on inflow!");
         double term1 = u * normalflux * factor;
         for (int i = 0; i < nloc; ++i) rs[i] += term1 * phi[i];
-        // o = 0 (model problem o, parameter.hh:203-208)
+        if (!jac_apply) {
+          const double ov = param.o(xg, n, bface * nf + q);
+          for (int i = 0; i < nloc; ++i) rs[i] += ov * phi[i] * factor;
+        }
         continue;
       }
       const double* gr = &tab.fgrad[o * nloc * 3];
@@ -659,9 +706,6 @@ struct LocalOperator {
       double gu[3] = {0, 0, 0};
       for (int i = 0; i < nloc; ++i)
         for (int e = 0; e < 3; ++e) gu[e] += xs[i] * tg[3 * i + e];
-      double xg[3];
-      const double* xi = &tab.fxi[3 * o];
-      for (int e = 0; e < 3; ++e) xg[e] = lo[e] + ext[e] * xi[e];
       double g = param.g(xg, bface * nf + q);
       if (jac_apply) g = 0.0;
       double up_s, up_n;

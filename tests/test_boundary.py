@@ -29,10 +29,11 @@ def test_library_exports_every_declared_symbol(pkg):
 def test_struct_layout_matches_c(pkg, tmp_path):
     src = tmp_path / "sz.c"
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "pdelab_gpu.h"\n'
-                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(pdelab_config),'
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(pdelab_config),'
                    ' offsetof(pdelab_config, gaussians), offsetof(pdelab_config, bctype),'
                    ' sizeof(pdelab_linear_solver_result), sizeof(pdelab_linear_problem_result),'
-                   ' sizeof(pdelab_mixed_result), offsetof(pdelab_config, problem)); return 0;}\n')
+                   ' sizeof(pdelab_mixed_result), offsetof(pdelab_config, problem),'
+                   ' offsetof(pdelab_config, j0), offsetof(pdelab_config, o0)); return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     vals = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
@@ -44,6 +45,8 @@ def test_struct_layout_matches_c(pkg, tmp_path):
     assert vals[4] == ctypes.sizeof(C.LinearProblemResult)
     assert vals[5] == ctypes.sizeof(C.MixedResult)
     assert vals[6] == C.PdelabConfig.problem.offset
+    assert vals[7] == C.PdelabConfig.j0.offset
+    assert vals[8] == C.PdelabConfig.o0.offset
 
 
 def test_default_config_matches_reference_defaults(pkg):
@@ -51,7 +54,7 @@ def test_default_config_matches_reference_defaults(pkg):
     (R:dune/pdelab/localoperator/convectiondiffusiondg.hh:85-90)."""
     cfg = pkg.PdelabConfig()
     pkg.lib().pdelab_gpu_default_config(ctypes.byref(cfg))
-    assert cfg.abi_version == 1
+    assert cfg.abi_version == pkg.config.ABI_VERSION == 2
     assert cfg.method == pkg.SIPG and cfg.weights == pkg.WEIGHTS_ON
     assert cfg.alpha == 1.0 and cfg.intorderadd == 0 and cfg.kappa0 == 1.0
     mine = pkg.make_config()

@@ -198,6 +198,15 @@ PROBLEMS = [
     ("gauss_conv_q2", dict(n=4, degree=2, alpha=3.0, b=(1.0, 0.5, 0.25), kappa0=0.01,
                            gaussians=((0.3, 0.4, 0.5, 0.1, 1.0), (0.7, 0.2, 0.6, 0.15, -0.5)), g0=0.2), None, 3),
     ("constant_intorder_q1", dict(n=5, degree=1, alpha=2.0, f0=1.5, g0=-0.5, intorderadd=3), 8, 2),
+    # Neumann flux j and outflow data o: the exact solution's fluxes (exp, quadratic) or the
+    # constants j0 / o0 (convectiondiffusiondg.hh:776-783, 810-815)
+    ("exp_neumann_outflow_q3", dict(nx=3, ny=4, nz=2, degree=3, alpha=3.0, b=(1.0, -0.5, 0.25),
+                                    bctype=(0, -2, -2, -1, 0, -1)), 10, 1),
+    ("quadratic_neumann_q2_var", dict(n=3, degree=2, alpha=3.0, quad=(0.3, 1.0, 2.0, -1.0, 1.0, 1.0, -2.0),
+                                      bctype=(-1, 0, 0, -1, -1, 0)), 11, 4),
+    ("constant_j0_o0_q2", dict(nx=4, ny=2, nz=3, degree=2, alpha=3.0, b=(0.0, 0.0, 2.0), f0=1.0, g0=0.5, j0=-1.5,
+                               o0=0.75, bctype=(-1, 0, 0, -1, 0, -2)), None, 2),
+    ("zero_problem_j0_q1", dict(n=4, degree=1, alpha=3.0, j0=2.0, bctype=(-1, -1, 0, 0, 0, 0)), None, 0),
 ]
 
 
@@ -224,6 +233,26 @@ def test_tabulated_problem(pkg, of):
     m = go.quadrature_points()
     assert m == o.quadrature_points()
     rng = np.random.default_rng(2)
+    nb = 2 * (4 * 2 + 3 * 2 + 3 * 4)
+    fq = rng.normal(size=go.ncells * m ** 3)
+    gq = rng.normal(size=nb * m ** 2)
+    go.set_tabulated(fq, gq)
+    o.set_tabulated(fq, gq)
+    rd = go.vector()
+    x = rng.normal(size=go.N)
+    go.residual(go.vector(x), rd)
+    assert rel(rd.download(), o.residual(x)) <= APPLY_RTOL
+
+
+def test_tabulated_neumann_outflow(pkg, of):
+    """Tabulated boundary data on Neumann and outflow sides: the value of each face is j or o by
+    the side's type."""
+    say("[gpu] tabulated f / g / j / o")
+    cfg = pkg.make_config(nx=3, ny=4, nz=2, degree=2, alpha=3.0, problem=pkg.PROBLEM_TABULATED, b=(0.5, 0.0, 0.0),
+                          bctype=(0, -2, -1, 0, -1, 0))
+    go, o = pkg.GridOperator(cfg), of.Oracle(cfg)
+    m = go.quadrature_points()
+    rng = np.random.default_rng(12)
     nb = 2 * (4 * 2 + 3 * 2 + 3 * 4)
     fq = rng.normal(size=go.ncells * m ** 3)
     gq = rng.normal(size=nb * m ** 2)
@@ -444,6 +473,37 @@ def test_adjoint_gradient_vs_oracle(pkg, of):
     gs = go.cell_vector()
     go.grad_source(go.vector(lam), gs)
     assert rel(gs.download(), o.grad_source(lam)) <= 1e-12
+
+
+GRAD_CASES = [
+    ("weights_off_g0_q2", dict(n=4, degree=2, alpha=3.0, weights=1, problem=2, f0=1.0)),
+    ("exp_g_q1", dict(n=4, degree=1, alpha=3.0, problem=1, bctype=(0, 0, -1, 0, 0, 0))),
+    ("exp_g_weights_off_q3", dict(nx=3, ny=2, nz=3, degree=3, alpha=2.0, weights=1, problem=1,
+                                  bctype=(0, -1, 0, 0, 0, 0))),
+    ("tabulated_g_nipg_q2", dict(nx=3, ny=3, nz=2, degree=2, alpha=3.0, method=0, problem=5)),
+]
+
+
+@pytest.mark.parametrize("name,kw", GRAD_CASES, ids=[c[0] for c in GRAD_CASES])
+def test_grad_logkappa_general_vs_oracle(pkg, of, name, kw):
+    """The kappa gradient with weightsOff and with Dirichlet data g != 0 (the derivative of the
+    residual's data terms included) against the oracle's dual-number derivative."""
+    say(f"[gpu] grad logkappa case={name}")
+    cfg = pkg.make_config(kappa_per_cell=True, **kw)
+    rng = np.random.default_rng(21)
+    kappa = np.exp(rng.normal(0, 0.5, pkg.ncells(cfg)))
+    go, o = pkg.GridOperator(cfg, kappa), of.Oracle(cfg, kappa)
+    if kw.get("problem") == 5:
+        m = go.quadrature_points()
+        nb = 2 * (cfg.nx * cfg.ny + cfg.nx * cfg.nz + cfg.ny * cfg.nz)
+        fq, gq = rng.normal(size=go.ncells * m ** 3), rng.normal(size=nb * m ** 2)
+        go.set_tabulated(fq, gq)
+        o.set_tabulated(fq, gq)
+    u = rng.normal(size=go.N)
+    lam = rng.normal(size=go.N)
+    g = go.cell_vector()
+    go.grad_logkappa(go.vector(u), go.vector(lam), g)
+    assert rel(g.download(), o.grad_logkappa(u, lam)) <= 1e-12
 
 
 @pytest.mark.parametrize("method", [0, 1], ids=["cg", "bicgstab"])

@@ -172,6 +172,51 @@ def test_polynomial_reproduction(of, k):
     assert np.abs(x - u).max() < 1e-9
 
 
+@pytest.mark.parametrize("k", [1, 2])
+def test_polynomial_reproduction_neumann_outflow(of, k):
+    """Neumann and outflow sides with the exact solution's fluxes, (b u - A grad u).n = j and
+    -(A grad u).n = o (convectiondiffusionparameter.hh:121-124): the interpolant of a Q_k
+    polynomial still has zero residual. A wrong sign or a missing convective part of j, or o
+    taken as the total flux, leaves a residual of order one on those faces."""
+    q = (0.3, 1.0, 2.0, -1.0, 0.0, 0.0, 0.0) if k == 1 else (0.3, 1.0, 2.0, -1.0, 1.0, 1.0, -2.0)
+    # +x outflow (b_x > 0), -y and +z Neumann, the rest Dirichlet
+    cfg = of.cfgmod.make_config(nx=3, ny=2, nz=2, degree=k, alpha=3.0, problem=of.cfgmod.PROBLEM_QUADRATIC, quad=q,
+                                b=(1.0, 0.5, -0.25), kappa0=0.7, c=0.2, bctype=(0, -2, -1, 0, 0, -1))
+    o = of.Oracle(cfg)
+    u = o.interpolate()
+    r = o.residual(u)
+    assert np.abs(r).max() < 1e-12
+    x, res = o.linear_problem_solve(method=1, reduction=1e-13, maxit=5000)
+    assert res["solver"]["converged"]
+    assert np.abs(x - u).max() < 1e-8
+
+
+def test_neumann_outflow_constant_data(of):
+    """Constant data j0 / o0: R(x) - R_0(x) is int j0 psi over the Neumann faces plus int o0 psi
+    over the outflow faces, i.e. j0 / o0 times the face integrals of the basis functions."""
+    kw = dict(nx=2, ny=2, nz=3, degree=2, alpha=3.0, b=(1.0, 0.0, 0.0), problem=of.cfgmod.PROBLEM_CONSTANT, f0=0.5,
+              bctype=(0, -2, -1, 0, 0, 0))
+    o0 = of.Oracle(of.cfgmod.make_config(**kw))
+    o1 = of.Oracle(of.cfgmod.make_config(j0=2.0, o0=-3.0, **kw))
+    x = np.random.default_rng(4).normal(size=o0.N)
+    d = o1.residual(x) - o0.residual(x)
+    # face integrals of the basis: the +x (outflow) and -y (Neumann) faces of the unit cube,
+    # 2x2x3 cells, Q2: tensor product of the 1D trace value and the 1D Gauss integrals
+    t = of.tables_1d(2, 4)
+    wint = t["P"].T @ t["w"]  # int_0^1 phi_a
+    h = (0.5, 0.5, 1.0 / 3.0)
+    exp = np.zeros(o0.N)
+    for c in range(12):
+        i, j, l = c % 2, (c // 2) % 2, c // 4
+        for dof in range(27):
+            a, b_, cc = dof % 3, (dof // 3) % 3, dof // 9
+            if i == 1:  # +x outflow face: psi(1, y, z)
+                exp[c * 27 + dof] += -3.0 * t["e1"][a] * wint[b_] * h[1] * wint[cc] * h[2]
+            if j == 0:  # -y Neumann face: psi(x, 0, z)
+                exp[c * 27 + dof] += 2.0 * t["e0"][b_] * wint[a] * h[0] * wint[cc] * h[2]
+    assert np.abs(d - exp).max() <= 1e-13 * np.abs(exp).max()
+
+
 def test_convergence_order(of):
     for k, expect in ((1, 2.0), (2, 3.0)):
         errs = []
@@ -440,6 +485,31 @@ def test_grad_logkappa_vs_finite_differences(of):
     uobs = rng.normal(0, 0.01, o.N)
     J0, u = _objective(o, cfg, np.exp(theta), uobs)
     # adjoint: J^T lambda = M (u - uobs)
+    rhs = o.mass_apply(u - uobs)
+    lam, _, res = o.solve(rhs, reduction=1e-14, maxit=20000, matrix_based=2)
+    g = o.grad_logkappa(u, lam)
+    eps = 1e-5
+    for c in (0, 4, 13, 26):
+        tp = theta.copy(); tp[c] += eps
+        tm = theta.copy(); tm[c] -= eps
+        Jp, _ = _objective(o, cfg, np.exp(tp), uobs)
+        Jm, _ = _objective(o, cfg, np.exp(tm), uobs)
+        fd = (Jp - Jm) / (2 * eps)
+        assert abs(fd - g[c]) <= 1e-6 * max(abs(fd), 1e-12) + 1e-12, (c, fd, g[c])
+
+
+@pytest.mark.parametrize("weights,problem", [(1, 2), (0, 1), (1, 1)], ids=["weightsOff_g0", "weightsOn_exp", "weightsOff_exp"])
+def test_grad_logkappa_vs_finite_differences_general(of, weights, problem):
+    """The dual-number kappa gradient with weightsOff and with Dirichlet data g != 0 (the exp
+    problem: g = u_exact on the Dirichlet sides, a Neumann side with the exact flux), against
+    central finite differences of the solved objective."""
+    cfg = of.cfgmod.make_config(n=3, degree=1, alpha=3.0, kappa_per_cell=True, problem=problem, f0=1.0, g0=0.0,
+                                weights=weights, bctype=(0, 0, -1, 0, 0, 0))
+    rng = np.random.default_rng(5)
+    theta = rng.normal(0, 0.5, 27)
+    o = of.Oracle(cfg, np.exp(theta))
+    uobs = rng.normal(0, 0.01, o.N)
+    J0, u = _objective(o, cfg, np.exp(theta), uobs)
     rhs = o.mass_apply(u - uobs)
     lam, _, res = o.solve(rhs, reduction=1e-14, maxit=20000, matrix_based=2)
     g = o.grad_logkappa(u, lam)
