@@ -21,7 +21,7 @@ import numpy as np
 from . import config as cfgmod
 from .config import (  # noqa: F401  (re-exported)
     BC_DIRICHLET, BC_NEUMANN, BC_NONE, BC_OUTFLOW, IIPG, MIXED_FLEXIBLE, MIXED_RESTARTED, NIPG, PREC_BLOCK_JACOBI,
-    PREC_JACOBI, PREC_NONE,
+    PREC_BLOCK_SOR, PREC_JACOBI, PREC_NONE,
     PROBLEM_CONSTANT, PROBLEM_EXP, PROBLEM_GAUSSIANS, PROBLEM_QUADRATIC, PROBLEM_TABULATED,
     PROBLEM_ZERO, SIPG, SOLVER_BICGSTAB, SOLVER_CG, TS_ALEXANDER2, TS_CRANK_NICOLSON, TS_IMPLICIT_EULER,
     WEIGHTS_OFF, WEIGHTS_ON, LinearProblemResult, LinearSolverResult, MixedResult, OneStepResult, PdelabConfig,
@@ -96,6 +96,8 @@ def lib():
     L.pdelab_gpu_block_matrix_apply.argtypes = [_vp, _vp, _vp, _vp, _vp]
     L.pdelab_gpu_block_jacobi_apply.argtypes = [_vp, _vp, _vp]
     L.pdelab_gpu_block_jacobi_apply_f32.argtypes = [_vp, _vp, _vp]
+    L.pdelab_gpu_block_sor_apply.argtypes = [_vp, _vp, _vp]
+    L.pdelab_gpu_set_sor_relaxation.argtypes = [_vp, ctypes.c_double]
     L.pdelab_gpu_dot.argtypes = [_vp, _vp, _vp, _dp]
     L.pdelab_gpu_norm.argtypes = [_vp, _vp, _dp]
     L.pdelab_gpu_axpy.argtypes = [_vp, ctypes.c_double, _vp, _vp]
@@ -355,6 +357,14 @@ class GridOperator:
             self._check(lib().pdelab_gpu_block_jacobi_apply_f32(self.h, zp, yp))
         else:
             self._check(lib().pdelab_gpu_block_jacobi_apply(self.h, zp, yp))
+
+    def block_sor_apply(self, d, v):
+        """v = W^{-1} d, one forward block-SOR sweep from v = 0 (FP64; v is overwritten)."""
+        self._check(lib().pdelab_gpu_block_sor_apply(self.h, self._vec(d), self._vec(v)))
+
+    def set_sor_relaxation(self, omega):
+        """Relaxation factor omega of the block-SOR preconditioner (default 1, in (0, 2))."""
+        self._check(lib().pdelab_gpu_set_sor_relaxation(self.h, float(omega)))
 
     def mass_apply(self, z, y):
         self._check(lib().pdelab_gpu_mass_apply(self.h, self._vec(z), self._vec(y)))

@@ -4,6 +4,7 @@
 #include <cstring>
 
 #include "handle.hh"
+#include "block_sor.hh"
 #include "jacobian.hh"
 
 namespace {
@@ -199,12 +200,17 @@ void bj_free(H* h) {
   }
   if (h->d_bjpart) (void)hipFree(h->d_bjpart);
   h->d_bjpart = nullptr;
+  if (h->d_sor_order) (void)hipFree(h->d_sor_order);
+  h->d_sor_order = nullptr;
+  h->sor_first.clear();
 }
 
-int bj_prepare(H* h, bool trans, int precision) {
+// Block data of slot 0 (J) or 1 (J^T), or of slot 2 (J, dense blocks only: block SOR).
+static int bj_prepare_slot(H* h, int slot, int precision) {
   if (h->ncells >= (1LL << 31)) return fail(h, PDELAB_ERR_UNSUPPORTED, "block Jacobi: more than 2^31 - 1 cells");
-  BlockJacobiData& b = h->bj[trans ? 1 : 0];
-  const bool fdm = !h->conv && h->cfg.method == PDELAB_SIPG;
+  BlockJacobiData& b = h->bj[slot];
+  const bool trans = slot == 1;
+  const bool fdm = slot != 2 && !h->conv && h->cfg.method == PDELAB_SIPG;
   const bool per_cell = h->vark;
   const long long nl = h->nloc;
   const long long entries = fdm ? (per_cell ? 3 * h->ncells : 12) : (per_cell ? h->ncells : 64);
@@ -254,6 +260,8 @@ int bj_prepare(H* h, bool trans, int precision) {
   if (need32) b.valid32 = true;
   return PDELAB_OK;
 }
+
+int bj_prepare(H* h, bool trans, int precision) { return bj_prepare_slot(h, trans ? 1 : 0, precision); }
 
 template <typename T, typename A>
 int bj_launch(H* h, bool trans, int mode, BJVec<T, A> v, int* nparts) {
@@ -344,3 +352,73 @@ int bj_block_diagonal(H* h, bool trans, double* blocks) {
     default: return run(std::integral_constant<int, 4>());
   }
 }
+
+// ---------------------------------------------------------------------------
+// Block SOR (block_sor.hh): the cells of each hyperplane x + y + z = s in ascending index
+// order, and one launch per hyperplane.
+
+namespace {
+
+template <int N1, typename T, bool CONV, bool VARK>
+int sor_waves(H* h, const T* d, T* v, const T* dinv, const int* stop) {
+  using S = SorShape<N1>;
+  const BJGeom g = geom(h, h->bj[2].per_cell);
+  const T* fc = VARK ? fc_for<T>(h) : nullptr;
+  const DGArgs<T>& a = args_for<T>(h);
+  const LineMats<T>* lm = line_mats<T>(h, false);
+  const int ns = (int)h->sor_first.size() - 1;
+  for (int s = 0; s < ns; ++s) {
+    const int first = h->sor_first[s], count = h->sor_first[s + 1] - first;
+    const unsigned nb = (unsigned)((count + S::CPW - 1) / S::CPW);
+    hipLaunchKernelGGL((block_sor_wave_kernel<N1, T, CONV, VARK>), dim3(nb), dim3(64), 0, h->stream, d, v, fc, a, lm,
+                       dinv, g, (const int*)h->d_sor_order, first, count, (T)h->sor_omega, stop);
+  }
+  HIPCHK(h, hipGetLastError());
+  return PDELAB_OK;
+}
+
+template <int N1, typename T>
+int sor_n(H* h, const T* d, T* v, const T* dinv, const int* stop) {
+  if (h->vark) return h->conv ? sor_waves<N1, T, true, true>(h, d, v, dinv, stop) : sor_waves<N1, T, false, true>(h, d, v, dinv, stop);
+  return h->conv ? sor_waves<N1, T, true, false>(h, d, v, dinv, stop) : sor_waves<N1, T, false, false>(h, d, v, dinv, stop);
+}
+
+}  // namespace
+
+int sor_prepare(H* h, int precision) {
+  if (h->ncells >= (1LL << 31)) return fail(h, PDELAB_ERR_UNSUPPORTED, "block SOR: more than 2^31 - 1 cells");
+  CHK(bj_prepare_slot(h, 2, precision));
+  if (!h->d_sor_order) {
+    const int nx = h->a64.n[0], ny = h->a64.n[1], nz = h->a64.n[2];
+    const int ns = nx + ny + nz - 2;
+    std::vector<int> order;
+    order.reserve((size_t)h->ncells);
+    h->sor_first.assign(1, 0);
+    for (int s = 0; s < ns; ++s) {
+      for (int z = std::max(0, s - (nx - 1) - (ny - 1)); z <= std::min(nz - 1, s); ++z)
+        for (int y = std::max(0, s - z - (nx - 1)); y <= std::min(ny - 1, s - z); ++y) {
+          const int x = s - z - y;
+          order.push_back(x + nx * (y + ny * z));
+        }
+      h->sor_first.push_back((int)order.size());
+    }
+    if ((long long)order.size() != h->ncells) return fail(h, PDELAB_ERR_INVALID_ARG, "block SOR: hyperplane count");
+    HIPCHK(h, hipMalloc(&h->d_sor_order, sizeof(int) * order.size()));
+    HIPCHK(h, hipMemcpy(h->d_sor_order, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
+  }
+  return PDELAB_OK;
+}
+
+template <typename T>
+int sor_launch(H* h, const T* d, T* v, const int* stop) {
+  CHK(sor_prepare(h, sizeof(T) == 8 ? 64 : 32));
+  const T* dinv = bj_data<T>(h->bj[2]);
+  switch (h->n1) {
+    case 2: return sor_n<2, T>(h, d, v, dinv, stop);
+    case 3: return sor_n<3, T>(h, d, v, dinv, stop);
+    default: return sor_n<4, T>(h, d, v, dinv, stop);
+  }
+}
+
+template int sor_launch<double>(H*, const double*, double*, const int*);
+template int sor_launch<float>(H*, const float*, float*, const int*);

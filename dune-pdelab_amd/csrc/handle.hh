@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <string>
+#include <vector>
 
 #include "../../include/pdelab_gpu.h"
 #include "block_jacobi.hh"
@@ -77,7 +78,7 @@ struct pdelab_gpu_gridoperator {
   bool sym[2] = {false, false};  // J / J^T have the structure of line_op_sym
   // apply kernel: 0 the row kernel (x-row chunks), 1 the general kernel (chunks of consecutive cells)
   int apply_variant = 0;
-  BlockJacobiData bj[2];  // [0] J, [1] J^T
+  BlockJacobiData bj[3];  // [0] J, [1] J^T, [2] J with dense blocks (block SOR)
   // reaction shift of the one-step stage operators: the kernels apply J + shift M
   double shift = 0.0;
   double prec_shift = 0.0;  // shift for which the cached diagonal and block Jacobi data hold
@@ -87,6 +88,11 @@ struct pdelab_gpu_gridoperator {
   static constexpr int kFgmresRestart = 10;
   double* fg[2 * kFgmresRestart + 1] = {nullptr};
   void* d_bjpart = nullptr;  // per-workgroup partials of the block Jacobi application
+  // block SOR: cells ordered by hyperplane x + y + z, hyperplane s at
+  // d_sor_order[sor_first[s] .. sor_first[s + 1]); relaxation factor omega
+  int* d_sor_order = nullptr;
+  std::vector<int> sor_first;
+  double sor_omega = 1.0;
 };
 
 using H = pdelab_gpu_gridoperator;
@@ -157,6 +163,20 @@ inline int grid_for(long long n) {
   return (int)std::max(1LL, std::min(b, 4096LL * 8));
 }
 
+// per-cell face coefficients and operator arguments of the precision T
+template <typename T>
+inline const T* fc_for(H* h);
+template <>
+inline const double* fc_for<double>(H* h) { return h->d_fc64; }
+template <>
+inline const float* fc_for<float>(H* h) { return h->d_fc32; }
+template <typename T>
+inline const DGArgs<T>& args_for(H* h);
+template <>
+inline const DGArgs<double>& args_for<double>(H* h) { return h->a64; }
+template <>
+inline const DGArgs<float>& args_for<float>(H* h) { return h->a32; }
+
 template <typename T>
 inline const LineMats<T>* line_mats(H* h, bool trans);
 template <>
@@ -180,6 +200,11 @@ void bj_free(H* h);
 template <typename T, typename A>
 int bj_launch(H* h, bool trans, int mode, BJVec<T, A> v, int* nparts);
 int bj_block_diagonal(H* h, bool trans, double* blocks);
+// Block SOR (block_sor.hh): data set-up (dense blocks of J, hyperplane order) and one
+// sweep v = W^{-1} d; kernels return at once when *stop is set.
+int sor_prepare(H* h, int precision);
+template <typename T>
+int sor_launch(H* h, const T* d, T* v, const int* stop);
 // Assembled Jacobian in the 7-point block pattern (jacobian.hh) and its apply y = A z.
 int jac_assemble(H* h, bool trans, double* blocks, long long* cols);
 int jac_block_apply(H* h, const double* blocks, const long long* cols, const double* z, double* y);

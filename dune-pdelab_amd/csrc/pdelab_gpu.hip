@@ -320,19 +320,6 @@ static int check_outflow(H* h) {
 }
 
 template <typename T>
-static const T* fc_for(H* h);
-template <>
-const double* fc_for<double>(H* h) { return h->d_fc64; }
-template <>
-const float* fc_for<float>(H* h) { return h->d_fc32; }
-template <typename T>
-static const DGArgs<T>& args_for(H* h);
-template <>
-const DGArgs<double>& args_for<double>(H* h) { return h->a64; }
-template <>
-const DGArgs<float>& args_for<float>(H* h) { return h->a32; }
-
-template <typename T>
 static int apply_mode(H* h, const T* z, T* y, bool trans, int mode, T* dotp = nullptr,
                       const int* stop = nullptr) {
   CHK(check_outflow(h));
@@ -516,14 +503,15 @@ const double* dinv_for<double>(H* h, int precond) { return precond == PDELAB_PRE
 template <>
 const float* dinv_for<float>(H* h, int precond) { return precond == PDELAB_PREC_JACOBI ? h->d_dinv32 : nullptr; }
 
-// Preconditioner W of a Krylov solve: none, point Jacobi (dinv) or per-cell block Jacobi
-// of J (or of J^T for the transposed operator).
+// Preconditioner W of a Krylov solve: none, point Jacobi (dinv), per-cell block Jacobi
+// of J (or of J^T for the transposed operator) or one block-SOR sweep of J.
 template <typename T>
 struct Prec {
   int kind = PDELAB_PREC_NONE;
   const T* dinv = nullptr;
   bool trans = false;
   bool block() const { return kind == PDELAB_PREC_BLOCK_JACOBI; }
+  bool sor() const { return kind == PDELAB_PREC_BLOCK_SOR; }
 };
 
 // The cached point diagonal and block Jacobi data belong to the operator with the current
@@ -543,7 +531,10 @@ static int make_prec(H* h, int precond, bool trans, Prec<T>& P) {
   P.trans = trans;
   if (precond == PDELAB_PREC_JACOBI) CHK(compute_diag(h));
   else if (precond == PDELAB_PREC_BLOCK_JACOBI) CHK(bj_prepare(h, trans, sizeof(T) == 8 ? 64 : 32));
-  else if (precond != PDELAB_PREC_NONE) return fail(h, PDELAB_ERR_INVALID_ARG, "unknown preconditioner");
+  else if (precond == PDELAB_PREC_BLOCK_SOR) {
+    if (trans) return fail(h, PDELAB_ERR_UNSUPPORTED, "block SOR: not for the transposed operator");
+    CHK(sor_prepare(h, sizeof(T) == 8 ? 64 : 32));
+  } else if (precond != PDELAB_PREC_NONE) return fail(h, PDELAB_ERR_INVALID_ARG, "unknown preconditioner");
   P.dinv = dinv_for<T>(h, precond);
   return PDELAB_OK;
 }
@@ -746,6 +737,7 @@ static int bicgstab_solve(H* h, T* x, T* r, double reduction, int maxit, int pre
     A* part = partial_buf<A>(h);
     const dim3 rg(kRedBlocks), rb(kRedThreads), fg(1), fb(kFinalThreads);
     auto prec = [&](const T* in, T* out) -> int {
+      if (P.sor()) return sor_launch<T>(h, in, out, st);
       if (P.block()) {
         const BJVec<T, A> bv{in, out, nullptr, nullptr, nullptr, nullptr, st};
         return bj_launch<T, A>(h, P.trans, BJ_PREC, bv, nullptr);
@@ -810,6 +802,8 @@ template <typename T, typename A>
 static int krylov(H* h, int method, int precond, double reduction, int maxit, bool trans, T* z, T* r, int base,
                   pdelab_linear_solver_result* res) {
   if (maxit <= 0) maxit = 5000;
+  if (method == PDELAB_SOLVER_CG && precond == PDELAB_PREC_BLOCK_SOR)
+    return fail(h, PDELAB_ERR_UNSUPPORTED, "block SOR: BiCGSTAB only (the preconditioner is not symmetric)");
   if (method == PDELAB_SOLVER_CG) {
     if constexpr (std::is_same<T, A>::value) return cg_solve<T, A>(h, z, r, reduction, maxit, precond, trans, base, res);
     else return fail(h, PDELAB_ERR_UNSUPPORTED, "CG accumulates in its vector type");
@@ -1366,6 +1360,21 @@ int pdelab_gpu_block_jacobi_apply_f32(pdelab_gpu_gridoperator* h, const float* z
   return bj_launch<float, float>(h, false, BJ_ADD, v, nullptr);
 }
 
+int pdelab_gpu_block_sor_apply(pdelab_gpu_gridoperator* h, const double* d, double* v) {
+  if (!h || !d || !v || d == v) return PDELAB_ERR_INVALID_ARG;
+  CHK(check_outflow(h));
+  sync_prec_cache(h);
+  return sor_launch<double>(h, d, v, nullptr);
+}
+
+int pdelab_gpu_set_sor_relaxation(pdelab_gpu_gridoperator* h, double omega) {
+  if (!h) return PDELAB_ERR_INVALID_ARG;
+  if (!std::isfinite(omega) || omega <= 0.0 || omega >= 2.0)
+    return fail(h, PDELAB_ERR_INVALID_ARG, "block SOR: omega must lie in (0, 2)");
+  h->sor_omega = omega;
+  return PDELAB_OK;
+}
+
 int pdelab_gpu_mass_apply(pdelab_gpu_gridoperator* h, const double* z, double* y) {
   if (!h || !z || !y) return PDELAB_ERR_INVALID_ARG;
   const int nb = (int)((h->N + 255) / 256);
@@ -1453,6 +1462,8 @@ int pdelab_gpu_solve_mixed(pdelab_gpu_gridoperator* h, int method, int precond, 
     return fail(h, PDELAB_ERR_INVALID_ARG, "unknown Krylov method");
   if (variant == (PDELAB_MIXED_RESTARTED | PDELAB_MIXED_FLEXIBLE))
     return fail(h, PDELAB_ERR_INVALID_ARG, "PDELAB_MIXED_RESTARTED and PDELAB_MIXED_FLEXIBLE exclude each other");
+  if (method == PDELAB_SOLVER_CG && precond == PDELAB_PREC_BLOCK_SOR)
+    return fail(h, PDELAB_ERR_UNSUPPORTED, "block SOR: BiCGSTAB only (the preconditioner is not symmetric)");
   if (variant & PDELAB_MIXED_FLEXIBLE) {
     auto t0 = std::chrono::steady_clock::now();
     CHK(compute_R0(h));

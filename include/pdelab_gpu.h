@@ -123,7 +123,15 @@ enum { PDELAB_SOLVER_CG = 0, PDELAB_SOLVER_BICGSTAB = 1 };
    R:dune/pdelab/backend/istl/seqistlsolverbackend.hh:240-243, and the matrix-free
    AssembledBlockJacobiPreconditionerLocalOperator,
    R:dune/pdelab/backend/istl/matrixfree/assembledblockjacobipreconditioner.hh:96-242). */
-enum { PDELAB_PREC_NONE = 0, PDELAB_PREC_JACOBI = 1, PDELAB_PREC_BLOCK_JACOBI = 2 };
+enum { PDELAB_PREC_NONE = 0, PDELAB_PREC_JACOBI = 1, PDELAB_PREC_BLOCK_JACOBI = 2, PDELAB_PREC_BLOCK_SOR = 3 };
+/* PDELAB_PREC_BLOCK_SOR: one forward block-SOR sweep over the cells in their index order from
+   v = 0, v_i = omega D_i^{-1} (d_i - sum_{j<i} A_ij v_j) with the exact inverse D_i^{-1} of the
+   cell's diagonal block (matrix-free BlockSORPreconditionerLocalOperator,
+   R:dune/pdelab/backend/istl/matrixfree/blocksorpreconditioner.hh:38-296, applied by
+   GridOperatorPreconditioner::apply, gridoperatorpreconditioner.hh:78-84, on the zeroed v of
+   dune-istl's BiCGSTAB; the block solve of AssembledBlockJacobiPreconditionerLocalOperator).
+   Not symmetric: BiCGSTAB only, J only (not J^T). omega is set by
+   pdelab_gpu_set_sor_relaxation (default 1). */
 
 typedef struct pdelab_config {
   int abi_version;            /* must be PDELAB_GPU_ABI_VERSION */
@@ -279,6 +287,16 @@ int pdelab_gpu_block_matrix_apply(pdelab_gpu_gridoperator* h, const double* bloc
 int pdelab_gpu_block_jacobi_apply(pdelab_gpu_gridoperator* h, const double* z, double* y);
 /* FP32 form of pdelab_gpu_block_jacobi_apply (inverse blocks rounded to FP32). */
 int pdelab_gpu_block_jacobi_apply_f32(pdelab_gpu_gridoperator* h, const float* z, float* y);
+/* v = W^{-1} d for the block-SOR preconditioner W (PDELAB_PREC_BLOCK_SOR): one forward sweep
+   from v = 0, as BiCGSTAB applies it (GridOperatorPreconditioner::apply over
+   BlockSORPreconditionerLocalOperator, R:dune/pdelab/backend/istl/matrixfree/
+   gridoperatorpreconditioner.hh:78-84, blocksorpreconditioner.hh:221-250). v is overwritten
+   (not accumulated into) and must not alias d. One launch per hyperplane x + y + z of the
+   grid; the dense inverse blocks of J are built once (and again after the next kappa change). */
+int pdelab_gpu_block_sor_apply(pdelab_gpu_gridoperator* h, const double* d, double* v);
+/* Relaxation factor omega of the block-SOR preconditioner (BlockSORPreconditionerLocalOperator's
+   omega, default 1); finite and in (0, 2), else PDELAB_ERR_INVALID_ARG. */
+int pdelab_gpu_set_sor_relaxation(pdelab_gpu_gridoperator* h, double omega);
 /* y += M z, DG mass matrix (ScalarL2 with scaling 1). */
 int pdelab_gpu_mass_apply(pdelab_gpu_gridoperator* h, const double* z, double* y);
 

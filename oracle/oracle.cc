@@ -574,6 +574,7 @@ struct oracle_handle {
   std::vector<double> blocks;
   std::vector<long long> cols;
   int threads = 1;
+  double sor_omega = 1.0;
 };
 
 static int guard(oracle_handle* h, const std::function<void()>& fn) {
@@ -598,6 +599,8 @@ void oracle_destroy(oracle_handle* h) { delete h; }
 const char* oracle_last_error(oracle_handle* h) { return h && h->o ? h->o->err.c_str() : ""; }
 size_t oracle_size(oracle_handle* h) { return h->o->N; }
 void oracle_set_threads(oracle_handle* h, int n) { h->threads = n; }
+void oracle_set_sor_omega(oracle_handle* h, double omega) { h->sor_omega = omega; }
+int oracle_block_sor_apply(oracle_handle* h, const double* d, double* v);
 
 int oracle_set_kappa(oracle_handle* h, const double* kappa) {
   h->o->param.kappa.assign(kappa, kappa + h->o->grid.ncells);
@@ -665,6 +668,73 @@ int oracle_grad_source(oracle_handle* h, const double* lam, double* g) {
 // Krylov solve of J z = r (ISTL backend apply(z, r, reduction)).
 // matrix_based: 0 = OnTheFlyOperator over jacobian_apply, 1 = assembled block matrix
 // (MatrixAdapter), 2 = transposed operator (adjoint).
+// LU factors of the diagonal blocks of J (bt = 0) or J^T (bt = 1), computed once per handle.
+static void block_lu_prepare(oracle_handle* h, int bt) {
+  auto& o = *h->o;
+  const size_t nloc = o.nloc, n2 = nloc * nloc;
+  if (!h->blu[bt].empty()) return;
+  std::vector<double>& B = h->blu[bt];
+  B.resize(o.grid.ncells * n2);
+  h->bpiv[bt].resize(o.N);
+  o.block_diagonal(B.data());
+  for (size_t c = 0; c < o.grid.ncells; ++c) {
+    double* A = &B[c * n2];
+    if (bt)
+      for (size_t i = 0; i < nloc; ++i)
+        for (size_t j = i + 1; j < nloc; ++j) std::swap(A[i * nloc + j], A[j * nloc + i]);
+    lu_factor(nloc, A, &h->bpiv[bt][c * nloc]);
+  }
+}
+
+// Block SOR needs the off-diagonal blocks A_ij (the assembled matrix) and the LU factors of
+// the diagonal blocks.
+static void sor_prepare(oracle_handle* h) {
+  auto& o = *h->o;
+  const size_t n2 = o.nloc * o.nloc;
+  if (h->blocks.empty()) {
+    h->blocks.resize(o.grid.ncells * 7 * n2);
+    h->cols.resize(o.grid.ncells * 7);
+    o.assemble(h->blocks.data(), h->cols.data());
+  }
+  block_lu_prepare(h, 0);
+}
+
+// One forward block-SOR sweep from v = 0, R:dune/pdelab/backend/istl/matrixfree/
+// blocksorpreconditioner.hh:19-27 (the update a_i = d_i - sum_{j<i} A_ij v_j^(k) -
+// sum_{j>i} A_ij v_j^(k-1), D_i b_i = a_i, v_i = (1 - omega) v_i + omega b_i; the cell
+// update at :221-250) with v^(k-1) = 0: GridOperatorPreconditioner::apply
+// (gridoperatorpreconditioner.hh:78-84) on the zeroed v of dune-istl's BiCGSTAB. Cells in
+// index order; only the lower-index face neighbours contribute.
+static void sor_sweep(oracle_handle* h, const double* d, double* v) {
+  auto& o = *h->o;
+  const size_t nloc = o.nloc, n2 = nloc * nloc;
+  std::vector<double> a(nloc), b(nloc);
+  for (size_t c = 0; c < o.grid.ncells; ++c) {
+    for (size_t i = 0; i < nloc; ++i) a[i] = d[c * nloc + i];
+    for (int sl = 1; sl < 7; ++sl) {
+      const long long cc = h->cols[c * 7 + sl];
+      if (cc < 0 || (size_t)cc >= c) continue;
+      const double* B = &h->blocks[(c * 7 + sl) * n2];
+      const double* vn = v + (size_t)cc * nloc;
+      for (size_t i = 0; i < nloc; ++i) {
+        double s = 0.0;
+        for (size_t j = 0; j < nloc; ++j) s += B[i * nloc + j] * vn[j];
+        a[i] -= s;
+      }
+    }
+    lu_solve(nloc, &h->blu[0][c * n2], &h->bpiv[0][c * nloc], a.data(), b.data());
+    for (size_t i = 0; i < nloc; ++i) v[c * nloc + i] = h->sor_omega * b[i];
+  }
+}
+
+// v = the block-SOR preconditioner applied to d (one sweep from v = 0)
+int oracle_block_sor_apply(oracle_handle* h, const double* d, double* v) {
+  return guard(h, [&] {
+    sor_prepare(h);
+    sor_sweep(h, d, v);
+  });
+}
+
 int oracle_solve(oracle_handle* h, int method, int precond, double reduction, int maxit, double* z, double* r,
                  pdelab_linear_solver_result* res, int matrix_based) {
   return guard(h, [&] {
@@ -676,19 +746,12 @@ int oracle_solve(oracle_handle* h, int method, int precond, double reduction, in
       o.point_diagonal(h->diag.data());
     }
     const int bt = matrix_based == 2 ? 1 : 0;  // blocks of J^T for the transposed operator
-    if (precond == PDELAB_PREC_BLOCK_JACOBI && h->blu[bt].empty()) {
-      std::vector<double>& B = h->blu[bt];
-      B.resize(o.grid.ncells * n2);
-      h->bpiv[bt].resize(n);
-      o.block_diagonal(B.data());
-      for (size_t c = 0; c < o.grid.ncells; ++c) {
-        double* A = &B[c * n2];
-        if (bt)
-          for (size_t i = 0; i < nloc; ++i)
-            for (size_t j = i + 1; j < nloc; ++j) std::swap(A[i * nloc + j], A[j * nloc + i]);
-        lu_factor(nloc, A, &h->bpiv[bt][c * nloc]);
-      }
-    }
+    const bool sor = precond == PDELAB_PREC_BLOCK_SOR;
+    if (sor && matrix_based == 2) throw std::runtime_error("block SOR: not for the transposed operator");
+    if (sor && method != PDELAB_SOLVER_BICGSTAB)
+      throw std::runtime_error("block SOR: BiCGSTAB only (the preconditioner is not symmetric)");
+    if (sor) sor_prepare(h);
+    else if (precond == PDELAB_PREC_BLOCK_JACOBI) block_lu_prepare(h, bt);
     oracle::OpFn A;
     if (matrix_based == 1) {
       if (h->blocks.empty()) {
@@ -728,6 +791,8 @@ int oracle_solve(oracle_handle* h, int method, int precond, double reduction, in
       else if (precond == PDELAB_PREC_BLOCK_JACOBI)
         for (size_t c = 0; c < o.grid.ncells; ++c)
           lu_solve(nloc, &h->blu[bt][c * n2], &h->bpiv[bt][c * nloc], d + c * nloc, v + c * nloc);
+      else if (sor)
+        sor_sweep(h, d, v);
       else
         std::copy(d, d + n, v);
     };

@@ -92,6 +92,8 @@ def _bind(L):
     L.oracle_size.restype = ctypes.c_size_t
     L.oracle_size.argtypes = [vp]
     L.oracle_set_threads.argtypes = [vp, ctypes.c_int]
+    L.oracle_set_sor_omega.argtypes = [vp, ctypes.c_double]
+    L.oracle_block_sor_apply.argtypes = [vp, _dp, _dp]
     for fn in ("oracle_set_kappa", "oracle_set_cell_source", "oracle_point_diagonal", "oracle_interpolate",
                "oracle_block_diagonal"):
         getattr(L, fn).argtypes = [vp, _dp]
@@ -173,6 +175,10 @@ class Oracle:
 
     def set_threads(self, n):
         self._L.oracle_set_threads(self.h, n)
+
+    def set_sor_omega(self, omega):
+        """Relaxation factor of the block-SOR preconditioner (default 1)."""
+        self._L.oracle_set_sor_omega(self.h, omega)
 
     def set_kappa(self, kappa):
         k = np.ascontiguousarray(kappa, dtype=np.float64)
@@ -306,6 +312,13 @@ class Oracle:
         g = np.zeros(self.ncells)
         self._check(self._L.oracle_grad_source(self.h, _p(lam), _p(g)))
         return g
+
+    def block_sor_apply(self, d):
+        """One forward block-SOR sweep from v = 0 applied to d (the preconditioner)."""
+        d = np.ascontiguousarray(d, dtype=np.float64)
+        v = np.zeros(self.N)
+        self._check(self._L.oracle_block_sor_apply(self.h, _p(d), _p(v)))
+        return v
 
     def solve(self, r, z=None, method=cfgmod.SOLVER_CG, precond=cfgmod.PREC_JACOBI, reduction=1e-10,
               maxit=5000, matrix_based=0):
