@@ -354,8 +354,7 @@ int bj_block_diagonal(H* h, bool trans, double* blocks) {
 }
 
 // ---------------------------------------------------------------------------
-// Block SOR (block_sor.hh): the cells of each hyperplane x + y + z = s in ascending index
-// order, and one launch per hyperplane.
+// Block SOR (block_sor.hh): one launch per hyperplane x + y + z = s.
 
 namespace {
 
@@ -389,17 +388,15 @@ int sor_prepare(H* h, int precision) {
   if (h->ncells >= (1LL << 31)) return fail(h, PDELAB_ERR_UNSUPPORTED, "block SOR: more than 2^31 - 1 cells");
   CHK(bj_prepare_slot(h, 2, precision));
   if (!h->d_sor_order) {
+    // hyperplane s: z ascending, then y ascending (x = s - z - y), i.e. ascending cell index
     const int nx = h->a64.n[0], ny = h->a64.n[1], nz = h->a64.n[2];
-    const int ns = nx + ny + nz - 2;
     std::vector<int> order;
     order.reserve((size_t)h->ncells);
     h->sor_first.assign(1, 0);
-    for (int s = 0; s < ns; ++s) {
+    for (int s = 0; s <= nx + ny + nz - 3; ++s) {
       for (int z = std::max(0, s - (nx - 1) - (ny - 1)); z <= std::min(nz - 1, s); ++z)
-        for (int y = std::max(0, s - z - (nx - 1)); y <= std::min(ny - 1, s - z); ++y) {
-          const int x = s - z - y;
-          order.push_back(x + nx * (y + ny * z));
-        }
+        for (int y = std::max(0, s - z - (nx - 1)); y <= std::min(ny - 1, s - z); ++y)
+          order.push_back((s - z - y) + nx * (y + ny * z));
       h->sor_first.push_back((int)order.size());
     }
     if ((long long)order.size() != h->ncells) return fail(h, PDELAB_ERR_INVALID_ARG, "block SOR: hyperplane count");

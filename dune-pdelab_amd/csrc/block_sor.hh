@@ -39,15 +39,16 @@ struct SorShape {
 
 // Hyperplane s of the sweep: order[first .. first + count) holds its cells (int32 indices,
 // ascending). dinv: the transposed dense inverse blocks D_e^{-1} (element [j][i] of entry e
-// at e NL^2 + j NL + i), e the cell (per_cell) or its boundary type.
+// at e NL^2 + j NL + i), e the cell (per_cell) or its boundary type. Each lane issues its
+// row of the block together with the neighbour loads, and the stop flag is read in the same
+// batch, so a wave waits for memory twice (the cell index, then everything else).
 template <int N1, typename T, bool CONV, bool VARK>
 __global__ __launch_bounds__(64) void block_sor_wave_kernel(const T* __restrict__ d, T* __restrict__ v,
                                                             const T* __restrict__ fc, const DGArgs<T> a,
                                                             const LineMats<T>* __restrict__ lm,
                                                             const T* __restrict__ dinv, const BJGeom g,
-                                                            const int* __restrict__ order, int first, int count,
-                                                            T omega, const int* __restrict__ stop) {
-  if (stop && *stop) return;  // a device-side Krylov loop has already stopped
+                                                            const int* __restrict__ order, int first,
+                                                            int count, T omega, const int* __restrict__ stop) {
   using S = SorShape<N1>;
   constexpr int L = S::L, NL = S::NL, LPC = S::LPC, CPW = S::CPW;
   __shared__ T B0[CPW][NL], B1[CPW][NL], B2[CPW][NL];
@@ -56,28 +57,34 @@ __global__ __launch_bounds__(64) void block_sor_wave_kernel(const T* __restrict_
   const int k = blockIdx.x * CPW + cw;
   const bool cell_ok = k < count;
   const long long c = cell_ok ? order[first + k] : 0;
-  int coord[3];
-  coord[0] = (int)(c % a.n[0]);
-  coord[1] = (int)((c / a.n[0]) % a.n[1]);
-  coord[2] = (int)(c / ((long long)a.n[0] * a.n[1]));
+  const int x = (int)(c % a.n[0]), y = (int)((c / a.n[0]) % a.n[1]), z = (int)(c / ((long long)a.n[0] * a.n[1]));
+  const int coord[3] = {x, y, z};
   const bool line_lane = cell_ok && l < L;
+  const bool dof_lane = cell_ok && l < NL;
   const int p = l % N1, q = l / N1;
   T* b0 = B0[cw];
   T* b1 = B1[cw];
   T* b2 = B2[cw];
-  T dz[N1], sx[N1], sy[N1], sz[N1];
+  bool lo[3];
+#pragma unroll
+  for (int dd = 0; dd < 3; ++dd) lo[dd] = coord[dd] > 0;
+  // ---- all global loads
+  T dz[N1], ux[N1], uy[N1], uz[N1], drow[NL];
+  T kc = a.kappa0;
+  FaceCoef<T> flo[3];
+  const FaceCoef<T> fnone{a.wI_lo, a.wI_hi, a.PI};
+#pragma unroll
+  for (int dd = 0; dd < 3; ++dd) flo[dd] = fnone;
   if (line_lane) {
-    bool lo[3];
+    const T* vc = v + c * NL;
+    const T* dc = d + c * NL;
 #pragma unroll
-    for (int dd = 0; dd < 3; ++dd) lo[dd] = coord[dd] > 0;
-    T zero[N1], ul[N1];
-#pragma unroll
-    for (int e = 0; e < N1; ++e) zero[e] = T(0);
-    T kc = a.kappa0;
-    FaceCoef<T> flo[3];
-    const FaceCoef<T> fnone{a.wI_lo, a.wI_hi, a.PI};
-#pragma unroll
-    for (int dd = 0; dd < 3; ++dd) flo[dd] = fnone;
+    for (int e = 0; e < N1; ++e) {
+      dz[e] = dc[line_index<N1, 2>(p, q, e)];
+      ux[e] = lo[0] ? vc[line_index<N1, 0>(p, q, e) - a.stride[0] * NL] : T(0);
+      uy[e] = lo[1] ? vc[line_index<N1, 1>(p, q, e) - a.stride[1] * NL] : T(0);
+      uz[e] = lo[2] ? vc[line_index<N1, 2>(p, q, e) - a.stride[2] * NL] : T(0);
+    }
     if (VARK) {
       const T* fb = fc + c * FC;
       kc = fb[0];
@@ -85,31 +92,30 @@ __global__ __launch_bounds__(64) void block_sor_wave_kernel(const T* __restrict_
       for (int dd = 0; dd < 3; ++dd)
         if (lo[dd]) flo[dd] = FaceCoef<T>{fb[1 + 3 * dd], fb[2 + 3 * dd], fb[3 + 3 * dd]};
     }
-    const T* vc = v + c * NL;
-    const T* dc = d + c * NL;
+  }
+  if (dof_lane) {
+    long long e;
+    if (g.per_cell) e = c;
+    else e = bj_type(x, g.n[0]) + 4 * bj_type(y, g.n[1]) + 16 * bj_type(z, g.n[2]);
+    const T* DT = dinv + e * NL * NL + l;
 #pragma unroll
-    for (int e = 0; e < N1; ++e) dz[e] = dc[line_index<N1, 2>(p, q, e)];
-    // the lower neighbour of each direction: its line through the lane's (p, q)
+    for (int j = 0; j < NL; ++j) drow[j] = DT[j * NL];
+  }
+  const bool stopped = stop && *stop;  // a device-side Krylov loop has already stopped
+  // ---- line terms of the lower neighbours (own and upper values zero)
+  if (line_lane) {
+    T zero[N1], sx[N1], sy[N1], sz[N1];
 #pragma unroll
-    for (int e = 0; e < N1; ++e) ul[e] = lo[0] ? vc[line_index<N1, 0>(p, q, e) - a.stride[0] * NL] : T(0);
-    if (lo[0]) line_apply<N1, 0, T, false, CONV, false, VARK>(a, lm, zero, ul, zero, true, false, kc, flo[0], fnone, sx);
-#pragma unroll
-    for (int e = 0; e < N1; ++e) ul[e] = lo[1] ? vc[line_index<N1, 1>(p, q, e) - a.stride[1] * NL] : T(0);
-    if (lo[1]) line_apply<N1, 1, T, false, CONV, false, VARK>(a, lm, zero, ul, zero, true, false, kc, flo[1], fnone, sy);
-#pragma unroll
-    for (int e = 0; e < N1; ++e) ul[e] = lo[2] ? vc[line_index<N1, 2>(p, q, e) - a.stride[2] * NL] : T(0);
-    if (lo[2]) line_apply<N1, 2, T, false, CONV, false, VARK>(a, lm, zero, ul, zero, true, false, kc, flo[2], fnone, sz);
-#pragma unroll
-    for (int e = 0; e < N1; ++e) {
-      if (!lo[0]) sx[e] = T(0);
-      if (!lo[1]) sy[e] = T(0);
-      if (!lo[2]) sz[e] = T(0);
-    }
-    // sum of the three directions in the y layout, through the rotated cell slots
+    for (int e = 0; e < N1; ++e) zero[e] = sx[e] = sy[e] = sz[e] = T(0);
+    if (lo[0]) line_apply<N1, 0, T, false, CONV, false, VARK>(a, lm, zero, ux, zero, true, false, kc, flo[0], fnone, sx);
+    if (lo[1]) line_apply<N1, 1, T, false, CONV, false, VARK>(a, lm, zero, uy, zero, true, false, kc, flo[1], fnone, sy);
+    if (lo[2]) line_apply<N1, 2, T, false, CONV, false, VARK>(a, lm, zero, uz, zero, true, false, kc, flo[2], fnone, sz);
+    // the three directions summed in the y layout through the rotated cell slots
 #pragma unroll
     for (int e = 0; e < N1; ++e) {
       b0[lds_index<N1, 2>(p, q, e)] = sz[e];
       b1[lds_index<N1, 0>(p, q, e)] = sx[e];
+      uy[e] = sy[e];
     }
   }
   __syncthreads();
@@ -119,7 +125,7 @@ __global__ __launch_bounds__(64) void block_sor_wave_kernel(const T* __restrict_
 #pragma unroll
     for (int e = 0; e < N1; ++e) {
       const int sl = lds_index<N1, 1>(p, q, e);
-      w[e] = sy[e] + b0[sl] + b1[sl];
+      w[e] = uy[e] + b0[sl] + b1[sl];
     }
 #pragma unroll
     for (int i = 0; i < N1; ++i) {
@@ -147,7 +153,7 @@ __global__ __launch_bounds__(64) void block_sor_wave_kernel(const T* __restrict_
     T w[N1];
 #pragma unroll
     for (int e = 0; e < N1; ++e) w[e] = b0[lds_index<N1, 2>(p, q, e)];
-    // r = d - coupling, stored in natural DoF order for the block solve
+    // r = d - coupling, in natural DoF order for the block solve
 #pragma unroll
     for (int i = 0; i < N1; ++i) {
       T acc = T(0);
@@ -157,14 +163,10 @@ __global__ __launch_bounds__(64) void block_sor_wave_kernel(const T* __restrict_
     }
   }
   __syncthreads();
-  if (cell_ok && l < NL) {
-    long long e;
-    if (g.per_cell) e = c;
-    else e = bj_type(coord[0], g.n[0]) + 4 * bj_type(coord[1], g.n[1]) + 16 * bj_type(coord[2], g.n[2]);
-    const T* DT = dinv + e * NL * NL + l;
+  if (dof_lane && !stopped) {
     T res = T(0);
-#pragma unroll 9
-    for (int j = 0; j < NL; ++j) res += DT[j * NL] * b1[j];
+#pragma unroll
+    for (int j = 0; j < NL; ++j) res += drow[j] * b1[j];
     v[c * NL + l] = omega * res;
   }
 }

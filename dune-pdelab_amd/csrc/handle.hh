@@ -200,8 +200,8 @@ void bj_free(H* h);
 template <typename T, typename A>
 int bj_launch(H* h, bool trans, int mode, BJVec<T, A> v, int* nparts);
 int bj_block_diagonal(H* h, bool trans, double* blocks);
-// Block SOR (block_sor.hh): data set-up (dense blocks of J, hyperplane order) and one
-// sweep v = W^{-1} d; kernels return at once when *stop is set.
+// Block SOR (block_sor.hh): data set-up (dense blocks of J) and one sweep v = W^{-1} d;
+// the kernels store nothing when *stop is set.
 int sor_prepare(H* h, int precision);
 template <typename T>
 int sor_launch(H* h, const T* d, T* v, const int* stop);
