@@ -21,7 +21,7 @@ import numpy as np
 from . import config as cfgmod
 from .config import (  # noqa: F401  (re-exported)
     BC_DIRICHLET, BC_NEUMANN, BC_NONE, BC_OUTFLOW, IIPG, MIXED_FLEXIBLE, MIXED_RESTARTED, NIPG, PREC_BLOCK_JACOBI,
-    PREC_BLOCK_SOR, PREC_JACOBI, PREC_NONE,
+    PREC_BLOCK_SOR, PREC_ITERATIVE_BLOCK_JACOBI, PREC_JACOBI, PREC_NONE,
     PROBLEM_CONSTANT, PROBLEM_EXP, PROBLEM_GAUSSIANS, PROBLEM_QUADRATIC, PROBLEM_TABULATED,
     PROBLEM_ZERO, SIPG, SOLVER_BICGSTAB, SOLVER_CG, TS_ALEXANDER2, TS_CRANK_NICOLSON, TS_IMPLICIT_EULER,
     WEIGHTS_OFF, WEIGHTS_ON, LinearProblemResult, LinearSolverResult, MixedResult, OneStepResult, PdelabConfig,
@@ -98,6 +98,8 @@ def lib():
     L.pdelab_gpu_block_jacobi_apply_f32.argtypes = [_vp, _vp, _vp]
     L.pdelab_gpu_block_sor_apply.argtypes = [_vp, _vp, _vp]
     L.pdelab_gpu_set_sor_relaxation.argtypes = [_vp, ctypes.c_double]
+    L.pdelab_gpu_iterative_block_jacobi_apply.argtypes = [_vp, _vp, _vp, ctypes.POINTER(ctypes.c_ulonglong)]
+    L.pdelab_gpu_set_block_solver_options.argtypes = [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double]
     L.pdelab_gpu_dot.argtypes = [_vp, _vp, _vp, _dp]
     L.pdelab_gpu_norm.argtypes = [_vp, _vp, _dp]
     L.pdelab_gpu_axpy.argtypes = [_vp, ctypes.c_double, _vp, _vp]
@@ -361,6 +363,18 @@ class GridOperator:
     def block_sor_apply(self, d, v):
         """v = W^{-1} d, one forward block-SOR sweep from v = 0 (FP64; v is overwritten)."""
         self._check(lib().pdelab_gpu_block_sor_apply(self.h, self._vec(d), self._vec(v)))
+
+    def iterative_block_jacobi_apply(self, d, v):
+        """v = W^{-1} d with per-cell BiCGSTAB block solves (FP64; v is overwritten). Returns the
+        block solvers' iteration count summed over the cells."""
+        its = ctypes.c_ulonglong(0)
+        self._check(lib().pdelab_gpu_iterative_block_jacobi_apply(self.h, self._vec(d), self._vec(v), ctypes.byref(its)))
+        return its.value
+
+    def set_block_solver_options(self, reduction=1e-5, maxiter=100, precondition=True, weight=1.0):
+        """BlockSolverOptions of the iterative block Jacobi preconditioner."""
+        self._check(lib().pdelab_gpu_set_block_solver_options(self.h, float(reduction), int(maxiter),
+                                                               1 if precondition else 0, float(weight)))
 
     def set_sor_relaxation(self, omega):
         """Relaxation factor omega of the block-SOR preconditioner (default 1, in (0, 2))."""

@@ -17,7 +17,7 @@ BC_DIRICHLET, BC_NEUMANN, BC_OUTFLOW, BC_NONE = 1, -1, -2, -3
 
 PROBLEM_ZERO, PROBLEM_EXP, PROBLEM_CONSTANT, PROBLEM_GAUSSIANS, PROBLEM_QUADRATIC, PROBLEM_TABULATED = range(6)
 SOLVER_CG, SOLVER_BICGSTAB = 0, 1
-PREC_NONE, PREC_JACOBI, PREC_BLOCK_JACOBI, PREC_BLOCK_SOR = 0, 1, 2, 3
+PREC_NONE, PREC_JACOBI, PREC_BLOCK_JACOBI, PREC_BLOCK_SOR, PREC_ITERATIVE_BLOCK_JACOBI = 0, 1, 2, 3, 4
 # variants of the mixed-precision solve, OR'd into its method (PDELAB_MIXED_RESTARTED / _FLEXIBLE)
 MIXED_RESTARTED, MIXED_FLEXIBLE = 16, 32
 

@@ -328,7 +328,8 @@ __global__ __launch_bounds__(256) void bj_dense_setup_kernel(const DGArgs<double
         const int lx = l % N1, ly = (l / N1) % N1, lz = l / (N1 * N1);
         s += a.M[ix][lx] * a.M[iy][ly] * a.M[iz][lz] * K[l][j];
       }
-      out64[e * NL * NL + ij] = s;
+      if (out64) out64[e * NL * NL + ij] = s;
+      if (out32) out32[e * NL * NL + ij] = (float)s;
     }
     return;
   }
@@ -828,6 +829,120 @@ __global__ __launch_bounds__(BJFdmShape<N1>::THREADS) void bj_fdm_kernel(const B
         v.partial[2 * blockIdx.x + 1] = r[1];
       }
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Iterative block Jacobi (IterativeBlockJacobiPreconditionerLocalOperator,
+// R:dune/pdelab/backend/istl/matrixfree/iterativeblockjacobipreconditioner.hh:267-450):
+// out_c = y with D_c y = in_c solved by dune-istl's BiCGSTAB from y = 0 (half steps counted,
+// stop at def < reduction def0 or def < 1e-30, breakdown checks with EPSILON 1e-80),
+// preconditioned by weight * diag(D_c)^{-1} (LocalPointJacobiPreconditioner) or not at all.
+// One wave per cell, lane l holding DoF l and row l of D_c in registers; the dot products are
+// wave sums broadcast from lane 0, so every lane takes the same branches.
+
+enum { IBJ_OK = 0, IBJ_RHO = 1, IBJ_OMEGA = 2, IBJ_H = 3, IBJ_NAN = 4 };
+
+struct IBJOptions {
+  double reduction;
+  int maxit;
+  int precondition;
+  double weight;
+};
+
+template <typename T>
+__device__ __forceinline__ T ibj_wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return __shfl(v, 0, 64);
+}
+
+// D: the diagonal blocks, row-major (entry e at e NL^2), e the cell or its boundary type.
+// err: the first failing block's code is kept (atomicCAS from 0); iters: sum of the block
+// solvers' iteration counts (half steps rounded up), when non-null.
+template <int N1, typename T>
+__global__ __launch_bounds__(64) void bj_iter_kernel(const T* __restrict__ in, T* __restrict__ out,
+                                                     const T* __restrict__ D, const BJGeom g, const IBJOptions opt,
+                                                     int* __restrict__ err, unsigned long long* __restrict__ iters,
+                                                     const int* __restrict__ stop) {
+  constexpr int NL = N1 * N1 * N1;
+  if (stop && *stop) return;  // a device-side Krylov loop has already stopped
+  __shared__ T xs[64];
+  const long long c = blockIdx.x;
+  const int l = threadIdx.x;
+  const bool act = l < NL;
+  long long e;
+  if (g.per_cell) e = c;
+  else
+    e = bj_type(bj_coord(g, c, 0), g.n[0]) + 4 * bj_type(bj_coord(g, c, 1), g.n[1]) +
+        16 * bj_type(bj_coord(g, c, 2), g.n[2]);
+  T drow[NL];
+  const T* Dr = D + e * NL * NL + (long long)(act ? l : 0) * NL;
+#pragma unroll
+  for (int j = 0; j < NL; ++j) drow[j] = act ? Dr[j] : T(0);
+  const T dinv = act ? T(opt.weight) / Dr[l] : T(0);
+  // y = D x for the wave's cell (x staged through LDS)
+  auto matvec = [&](T xl) -> T {
+    __syncthreads();
+    xs[l] = xl;
+    __syncthreads();
+    T acc = T(0);
+#pragma unroll
+    for (int j = 0; j < NL; ++j) acc += drow[j] * xs[j];
+    return act ? acc : T(0);
+  };
+  auto prec = [&](T vl) -> T { return opt.precondition ? dinv * vl : vl; };
+  const T EPS = T(1e-80);
+  T x = T(0);
+  T r = act ? in[c * NL + l] : T(0);  // r = b - A 0
+  const T rt = r;
+  T p = T(0), v = T(0), t = T(0), y = T(0);
+  T norm = sqrt(ibj_wave_sum(r * r));
+  const T def0 = norm;
+  double it_last = 0.0;
+  int code = IBJ_OK;
+  auto converged = [&](T d) { return d < def0 * T(opt.reduction) || d < T(1e-30); };
+  if (!(norm == norm) || isinf((double)norm)) code = IBJ_NAN;
+  else if (!converged(norm)) {
+    T rho = T(1), alpha = T(1), omega = T(1), rho_new, beta, h;
+    for (double it = 0.5; it < opt.maxit; it += 0.5) {
+      rho_new = ibj_wave_sum(rt * r);
+      if (fabs(rho) <= EPS) { code = IBJ_RHO; break; }
+      if (fabs(omega) <= EPS) { code = IBJ_OMEGA; break; }
+      if (it < 1) p = r;
+      else {
+        beta = (norm == T(0)) ? T(0) : (rho_new / rho) * (alpha / omega);
+        p = beta * (p - omega * v) + r;
+      }
+      y = prec(p);
+      v = matvec(y);
+      h = ibj_wave_sum(rt * v);
+      if (fabs(h) < EPS) { code = IBJ_H; break; }
+      alpha = (norm == T(0)) ? T(0) : rho_new / h;
+      x += alpha * y;
+      r -= alpha * v;
+      norm = sqrt(ibj_wave_sum(r * r));
+      it_last = it;
+      if (!(norm == norm) || isinf((double)norm)) { code = IBJ_NAN; break; }
+      if (converged(norm)) break;
+      it += 0.5;
+      y = prec(r);
+      t = matvec(y);
+      const T tr = ibj_wave_sum(t * r), tt = ibj_wave_sum(t * t);
+      omega = (norm == T(0)) ? T(0) : tr / tt;
+      x += omega * y;
+      r -= omega * t;
+      rho = rho_new;
+      norm = sqrt(ibj_wave_sum(r * r));
+      it_last = it;
+      if (!(norm == norm) || isinf((double)norm)) { code = IBJ_NAN; break; }
+      if (converged(norm)) break;
+    }
+  }
+  if (act) out[c * NL + l] = x;
+  if (l == 0) {
+    if (code != IBJ_OK) atomicCAS(err, 0, code);
+    if (iters) atomicAdd(iters, (unsigned long long)ceil(it_last));
   }
 }
 

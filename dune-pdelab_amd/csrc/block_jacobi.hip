@@ -200,6 +200,8 @@ void bj_free(H* h) {
   }
   if (h->d_bjpart) (void)hipFree(h->d_bjpart);
   h->d_bjpart = nullptr;
+  if (h->d_ibj_stat) (void)hipFree(h->d_ibj_stat);
+  h->d_ibj_stat = nullptr;
   if (h->d_sor_order) (void)hipFree(h->d_sor_order);
   h->d_sor_order = nullptr;
   h->sor_first.clear();
@@ -419,3 +421,67 @@ int sor_launch(H* h, const T* d, T* v, const int* stop) {
 
 template int sor_launch<double>(H*, const double*, double*, const int*);
 template int sor_launch<float>(H*, const float*, float*, const int*);
+
+// ---------------------------------------------------------------------------
+// Iterative block Jacobi (block_jacobi.hh, bj_iter_kernel): the diagonal blocks D_c of J,
+// row-major, per boundary type (constant coefficients) or per cell (cell-wise kappa).
+
+int ibj_prepare(H* h, int precision) {
+  if (h->ncells >= (1LL << 31)) return fail(h, PDELAB_ERR_UNSUPPORTED, "iterative block Jacobi: more than 2^31 - 1 cells");
+  BlockJacobiData& b = h->bj[3];
+  const bool per_cell = h->vark;
+  const long long nl = h->nloc, entries = per_cell ? h->ncells : 64;
+  if (b.per_cell != per_cell || b.entries != entries || b.esize != nl * nl) {
+    if (b.d64) (void)hipFree(b.d64);
+    if (b.d32) (void)hipFree(b.d32);
+    b = BlockJacobiData();
+    b.per_cell = per_cell;
+    b.entries = entries;
+    b.esize = nl * nl;
+  }
+  if (!h->d_ibj_stat) HIPCHK(h, hipMalloc(&h->d_ibj_stat, 16));
+  const bool need64 = precision == 64 && !b.valid64;
+  const bool need32 = precision == 32 && !b.valid32;
+  if (!need64 && !need32) return PDELAB_OK;
+  const size_t count = (size_t)(entries * nl * nl);
+  if (need64 && !b.d64) HIPCHK(h, hipMalloc(&b.d64, sizeof(double) * count));
+  if (need32 && !b.d32) HIPCHK(h, hipMalloc(&b.d32, sizeof(float) * count));
+  const BJGeom g = geom(h, per_cell);
+  double* o64 = need64 ? b.d64 : nullptr;
+  float* o32 = need32 ? b.d32 : nullptr;
+  auto run = [&](auto n1tag) -> int {
+    constexpr int N1 = decltype(n1tag)::value;
+    if (h->conv)
+      return h->vark ? dense_setup<N1, false, true, true>(h, g, entries, 1, o64, o32)
+                     : dense_setup<N1, false, true, false>(h, g, entries, 1, o64, o32);
+    return h->vark ? dense_setup<N1, false, false, true>(h, g, entries, 1, o64, o32)
+                   : dense_setup<N1, false, false, false>(h, g, entries, 1, o64, o32);
+  };
+  switch (h->n1) {
+    case 2: CHK(run(std::integral_constant<int, 2>())); break;
+    case 3: CHK(run(std::integral_constant<int, 3>())); break;
+    default: CHK(run(std::integral_constant<int, 4>())); break;
+  }
+  if (need64) b.valid64 = true;
+  if (need32) b.valid32 = true;
+  return PDELAB_OK;
+}
+
+template <typename T>
+int ibj_launch(H* h, const T* d, T* v, int* err, unsigned long long* iters, const int* stop) {
+  CHK(ibj_prepare(h, sizeof(T) == 8 ? 64 : 32));
+  const BlockJacobiData& b = h->bj[3];
+  const BJGeom g = geom(h, b.per_cell);
+  const T* D = bj_data<T>(b);
+  const dim3 grid((unsigned)h->ncells), block(64);
+  switch (h->n1) {
+    case 2: hipLaunchKernelGGL((bj_iter_kernel<2, T>), grid, block, 0, h->stream, d, v, D, g, h->ibj_opt, err, iters, stop); break;
+    case 3: hipLaunchKernelGGL((bj_iter_kernel<3, T>), grid, block, 0, h->stream, d, v, D, g, h->ibj_opt, err, iters, stop); break;
+    default: hipLaunchKernelGGL((bj_iter_kernel<4, T>), grid, block, 0, h->stream, d, v, D, g, h->ibj_opt, err, iters, stop); break;
+  }
+  HIPCHK(h, hipGetLastError());
+  return PDELAB_OK;
+}
+
+template int ibj_launch<double>(H*, const double*, double*, int*, unsigned long long*, const int*);
+template int ibj_launch<float>(H*, const float*, float*, int*, unsigned long long*, const int*);

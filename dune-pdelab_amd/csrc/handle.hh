@@ -78,7 +78,11 @@ struct pdelab_gpu_gridoperator {
   bool sym[2] = {false, false};  // J / J^T have the structure of line_op_sym
   // apply kernel: 0 the row kernel (x-row chunks), 1 the general kernel (chunks of consecutive cells)
   int apply_variant = 0;
-  BlockJacobiData bj[3];  // [0] J, [1] J^T, [2] J with dense blocks (block SOR)
+  // [0] J, [1] J^T, [2] J with dense inverse blocks (block SOR), [3] the diagonal blocks of J
+  // themselves (iterative block Jacobi)
+  BlockJacobiData bj[4];
+  IBJOptions ibj_opt{1e-5, 100, 1, 1.0};
+  void* d_ibj_stat = nullptr;  // int error code at 0, unsigned long long iteration sum at 8
   // reaction shift of the one-step stage operators: the kernels apply J + shift M
   double shift = 0.0;
   double prec_shift = 0.0;  // shift for which the cached diagonal and block Jacobi data hold
@@ -205,6 +209,12 @@ int bj_block_diagonal(H* h, bool trans, double* blocks);
 int sor_prepare(H* h, int precision);
 template <typename T>
 int sor_launch(H* h, const T* d, T* v, const int* stop);
+// Iterative block Jacobi (bj_iter_kernel): the diagonal blocks of J, and one application
+// v = W^{-1} d; the block solvers' error code goes to *err (kept at the first nonzero), their
+// iteration sum is added to *iters when non-null.
+int ibj_prepare(H* h, int precision);
+template <typename T>
+int ibj_launch(H* h, const T* d, T* v, int* err, unsigned long long* iters, const int* stop);
 // Assembled Jacobian in the 7-point block pattern (jacobian.hh) and its apply y = A z.
 int jac_assemble(H* h, bool trans, double* blocks, long long* cols);
 int jac_block_apply(H* h, const double* blocks, const long long* cols, const double* z, double* y);

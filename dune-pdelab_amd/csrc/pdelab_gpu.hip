@@ -504,7 +504,8 @@ template <>
 const float* dinv_for<float>(H* h, int precond) { return precond == PDELAB_PREC_JACOBI ? h->d_dinv32 : nullptr; }
 
 // Preconditioner W of a Krylov solve: none, point Jacobi (dinv), per-cell block Jacobi
-// of J (or of J^T for the transposed operator) or one block-SOR sweep of J.
+// of J (or of J^T for the transposed operator), one block-SOR sweep of J, or block Jacobi of
+// J with iteratively solved blocks.
 template <typename T>
 struct Prec {
   int kind = PDELAB_PREC_NONE;
@@ -512,6 +513,7 @@ struct Prec {
   bool trans = false;
   bool block() const { return kind == PDELAB_PREC_BLOCK_JACOBI; }
   bool sor() const { return kind == PDELAB_PREC_BLOCK_SOR; }
+  bool ibj() const { return kind == PDELAB_PREC_ITERATIVE_BLOCK_JACOBI; }
 };
 
 // The cached point diagonal and block Jacobi data belong to the operator with the current
@@ -534,6 +536,9 @@ static int make_prec(H* h, int precond, bool trans, Prec<T>& P) {
   else if (precond == PDELAB_PREC_BLOCK_SOR) {
     if (trans) return fail(h, PDELAB_ERR_UNSUPPORTED, "block SOR: not for the transposed operator");
     CHK(sor_prepare(h, sizeof(T) == 8 ? 64 : 32));
+  } else if (precond == PDELAB_PREC_ITERATIVE_BLOCK_JACOBI) {
+    if (trans) return fail(h, PDELAB_ERR_UNSUPPORTED, "iterative block Jacobi: not for the transposed operator");
+    CHK(ibj_prepare(h, sizeof(T) == 8 ? 64 : 32));
   } else if (precond != PDELAB_PREC_NONE) return fail(h, PDELAB_ERR_INVALID_ARG, "unknown preconditioner");
   P.dinv = dinv_for<T>(h, precond);
   return PDELAB_OK;
@@ -736,8 +741,10 @@ static int bicgstab_solve(H* h, T* x, T* r, double reduction, int maxit, int pre
     HIPCHK(h, hipMemcpyAsync(ds, h->h_dstate, sizeof(double) * 4, hipMemcpyHostToDevice, h->stream));
     A* part = partial_buf<A>(h);
     const dim3 rg(kRedBlocks), rb(kRedThreads), fg(1), fb(kFinalThreads);
+    if (P.ibj()) HIPCHK(h, hipMemsetAsync(h->d_ibj_stat, 0, 16, h->stream));
     auto prec = [&](const T* in, T* out) -> int {
       if (P.sor()) return sor_launch<T>(h, in, out, st);
+      if (P.ibj()) return ibj_launch<T>(h, in, out, static_cast<int*>(h->d_ibj_stat), nullptr, st);
       if (P.block()) {
         const BJVec<T, A> bv{in, out, nullptr, nullptr, nullptr, nullptr, st};
         return bj_launch<T, A>(h, P.trans, BJ_PREC, bv, nullptr);
@@ -786,6 +793,13 @@ static int bicgstab_solve(H* h, T* x, T* r, double reduction, int maxit, int pre
     iter.i = 0.5 * h->h_state[2];
     iter.aborted = code == BICG_NAN;
     iter.converged = h->h_state[0] != 0 && code == BICG_RUNNING;
+    if (P.ibj()) {
+      int ierr = 0;
+      HIPCHK(h, hipMemcpy(&ierr, h->d_ibj_stat, sizeof(int), hipMemcpyDeviceToHost));
+      if (ierr != 0)
+        status = fail(h, PDELAB_ERR_SOLVER_ABORT, "iterative block Jacobi: breakdown in a block solve (code " +
+                                                      std::to_string(ierr) + ")");
+    }
     if (code == BICG_RHO) status = fail(h, PDELAB_ERR_SOLVER_ABORT, "breakdown in BiCGSTAB - rho");
     else if (code == BICG_OMEGA) status = fail(h, PDELAB_ERR_SOLVER_ABORT, "breakdown in BiCGSTAB - omega");
     else if (code == BICG_H) status = fail(h, PDELAB_ERR_SOLVER_ABORT, "abs(h) < EPSILON in BiCGSTAB");
@@ -804,6 +818,8 @@ static int krylov(H* h, int method, int precond, double reduction, int maxit, bo
   if (maxit <= 0) maxit = 5000;
   if (method == PDELAB_SOLVER_CG && precond == PDELAB_PREC_BLOCK_SOR)
     return fail(h, PDELAB_ERR_UNSUPPORTED, "block SOR: BiCGSTAB only (the preconditioner is not symmetric)");
+  if (method == PDELAB_SOLVER_CG && precond == PDELAB_PREC_ITERATIVE_BLOCK_JACOBI)
+    return fail(h, PDELAB_ERR_UNSUPPORTED, "iterative block Jacobi: BiCGSTAB only (the preconditioner is an inner iteration)");
   if (method == PDELAB_SOLVER_CG) {
     if constexpr (std::is_same<T, A>::value) return cg_solve<T, A>(h, z, r, reduction, maxit, precond, trans, base, res);
     else return fail(h, PDELAB_ERR_UNSUPPORTED, "CG accumulates in its vector type");
@@ -1367,6 +1383,37 @@ int pdelab_gpu_block_sor_apply(pdelab_gpu_gridoperator* h, const double* d, doub
   return sor_launch<double>(h, d, v, nullptr);
 }
 
+int pdelab_gpu_iterative_block_jacobi_apply(pdelab_gpu_gridoperator* h, const double* d, double* v,
+                                            unsigned long long* iterations) {
+  if (!h || !d || !v || d == v) return PDELAB_ERR_INVALID_ARG;
+  CHK(check_outflow(h));
+  sync_prec_cache(h);
+  CHK(ibj_prepare(h, 64));
+  HIPCHK(h, hipMemsetAsync(h->d_ibj_stat, 0, 16, h->stream));
+  CHK(ibj_launch<double>(h, d, v, static_cast<int*>(h->d_ibj_stat),
+                         reinterpret_cast<unsigned long long*>(static_cast<char*>(h->d_ibj_stat) + 8), nullptr));
+  struct {
+    int err, pad;
+    unsigned long long its;
+  } st;
+  HIPCHK(h, hipMemcpyAsync(&st, h->d_ibj_stat, 16, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (iterations) *iterations = st.its;
+  if (st.err != 0)
+    return fail(h, PDELAB_ERR_SOLVER_ABORT,
+                "iterative block Jacobi: breakdown in a block solve (code " + std::to_string(st.err) + ")");
+  return PDELAB_OK;
+}
+
+int pdelab_gpu_set_block_solver_options(pdelab_gpu_gridoperator* h, double reduction, int maxiter, int precondition,
+                                        double weight) {
+  if (!h) return PDELAB_ERR_INVALID_ARG;
+  if (!std::isfinite(reduction) || reduction <= 0.0 || maxiter < 1 || !std::isfinite(weight) || weight == 0.0)
+    return fail(h, PDELAB_ERR_INVALID_ARG, "block solver options: reduction > 0, maxiter >= 1, weight finite and nonzero");
+  h->ibj_opt = IBJOptions{reduction, maxiter, precondition != 0 ? 1 : 0, weight};
+  return PDELAB_OK;
+}
+
 int pdelab_gpu_set_sor_relaxation(pdelab_gpu_gridoperator* h, double omega) {
   if (!h) return PDELAB_ERR_INVALID_ARG;
   if (!std::isfinite(omega) || omega <= 0.0 || omega >= 2.0)
@@ -1464,6 +1511,8 @@ int pdelab_gpu_solve_mixed(pdelab_gpu_gridoperator* h, int method, int precond, 
     return fail(h, PDELAB_ERR_INVALID_ARG, "PDELAB_MIXED_RESTARTED and PDELAB_MIXED_FLEXIBLE exclude each other");
   if (method == PDELAB_SOLVER_CG && precond == PDELAB_PREC_BLOCK_SOR)
     return fail(h, PDELAB_ERR_UNSUPPORTED, "block SOR: BiCGSTAB only (the preconditioner is not symmetric)");
+  if (method == PDELAB_SOLVER_CG && precond == PDELAB_PREC_ITERATIVE_BLOCK_JACOBI)
+    return fail(h, PDELAB_ERR_UNSUPPORTED, "iterative block Jacobi: BiCGSTAB only (the preconditioner is an inner iteration)");
   if (variant & PDELAB_MIXED_FLEXIBLE) {
     auto t0 = std::chrono::steady_clock::now();
     CHK(compute_R0(h));

@@ -123,7 +123,13 @@ enum { PDELAB_SOLVER_CG = 0, PDELAB_SOLVER_BICGSTAB = 1 };
    R:dune/pdelab/backend/istl/seqistlsolverbackend.hh:240-243, and the matrix-free
    AssembledBlockJacobiPreconditionerLocalOperator,
    R:dune/pdelab/backend/istl/matrixfree/assembledblockjacobipreconditioner.hh:96-242). */
-enum { PDELAB_PREC_NONE = 0, PDELAB_PREC_JACOBI = 1, PDELAB_PREC_BLOCK_JACOBI = 2, PDELAB_PREC_BLOCK_SOR = 3 };
+enum {
+  PDELAB_PREC_NONE = 0,
+  PDELAB_PREC_JACOBI = 1,
+  PDELAB_PREC_BLOCK_JACOBI = 2,
+  PDELAB_PREC_BLOCK_SOR = 3,
+  PDELAB_PREC_ITERATIVE_BLOCK_JACOBI = 4
+};
 /* PDELAB_PREC_BLOCK_SOR: one forward block-SOR sweep over the cells in their index order from
    v = 0, v_i = omega D_i^{-1} (d_i - sum_{j<i} A_ij v_j) with the exact inverse D_i^{-1} of the
    cell's diagonal block (matrix-free BlockSORPreconditionerLocalOperator,
@@ -131,7 +137,13 @@ enum { PDELAB_PREC_NONE = 0, PDELAB_PREC_JACOBI = 1, PDELAB_PREC_BLOCK_JACOBI = 
    GridOperatorPreconditioner::apply, gridoperatorpreconditioner.hh:78-84, on the zeroed v of
    dune-istl's BiCGSTAB; the block solve of AssembledBlockJacobiPreconditionerLocalOperator).
    Not symmetric: BiCGSTAB only, J only (not J^T). omega is set by
-   pdelab_gpu_set_sor_relaxation (default 1). */
+   pdelab_gpu_set_sor_relaxation (default 1).
+   PDELAB_PREC_ITERATIVE_BLOCK_JACOBI: block Jacobi whose blocks are solved iteratively, per cell
+   dune-istl's BiCGSTAB on D_c y = d_c from y = 0, preconditioned by weight * diag(D_c)^{-1}, to
+   the residual reduction and iteration limit of BlockSolverOptions (1e-5, 100 by default;
+   IterativeBlockJacobiPreconditionerLocalOperator,
+   R:dune/pdelab/backend/istl/matrixfree/iterativeblockjacobipreconditioner.hh:198-450).
+   An inner iteration: BiCGSTAB only, J only. Options: pdelab_gpu_set_block_solver_options. */
 
 typedef struct pdelab_config {
   int abi_version;            /* must be PDELAB_GPU_ABI_VERSION */
@@ -294,6 +306,21 @@ int pdelab_gpu_block_jacobi_apply_f32(pdelab_gpu_gridoperator* h, const float* z
    (not accumulated into) and must not alias d. One launch per hyperplane x + y + z of the
    grid; the dense inverse blocks of J are built once (and again after the next kappa change). */
 int pdelab_gpu_block_sor_apply(pdelab_gpu_gridoperator* h, const double* d, double* v);
+/* v = W^{-1} d for the iterative block Jacobi preconditioner W (PDELAB_PREC_ITERATIVE_BLOCK_JACOBI):
+   per cell BiCGSTAB on D_c v_c = d_c from v_c = 0 (GridOperatorPreconditioner::apply over
+   IterativeBlockJacobiPreconditionerLocalOperator, R:dune/pdelab/backend/istl/matrixfree/
+   iterativeblockjacobipreconditioner.hh:355-390). v is overwritten and must not alias d.
+   *iterations (when non-null) receives the block solvers' iteration count summed over the cells
+   (SolverStatistics, solverstatistics.hh). A breakdown of a block solve returns
+   PDELAB_ERR_SOLVER_ABORT, as the ISTL solver throws. */
+int pdelab_gpu_iterative_block_jacobi_apply(pdelab_gpu_gridoperator* h, const double* d, double* v,
+                                            unsigned long long* iterations);
+/* BlockSolverOptions of the iterative block Jacobi preconditioner
+   (iterativeblockjacobipreconditioner.hh:198-231): residual reduction (default 1e-5), iteration
+   limit (100), point-Jacobi preconditioning of the block solves (1) and its weight (1).
+   PDELAB_ERR_INVALID_ARG unless reduction > 0, maxiter >= 1 and weight finite and nonzero. */
+int pdelab_gpu_set_block_solver_options(pdelab_gpu_gridoperator* h, double reduction, int maxiter, int precondition,
+                                        double weight);
 /* Relaxation factor omega of the block-SOR preconditioner (BlockSORPreconditionerLocalOperator's
    omega, default 1); finite and in (0, 2), else PDELAB_ERR_INVALID_ARG. */
 int pdelab_gpu_set_sor_relaxation(pdelab_gpu_gridoperator* h, double omega);

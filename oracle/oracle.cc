@@ -575,6 +575,11 @@ struct oracle_handle {
   std::vector<long long> cols;
   int threads = 1;
   double sor_omega = 1.0;
+  // iterative block Jacobi: the diagonal blocks of J (row-major) and the block-solver options
+  std::vector<double> bdiag;
+  double ibj_reduction = 1e-5, ibj_weight = 1.0;
+  int ibj_maxit = 100, ibj_precondition = 1;
+  unsigned long long ibj_iterations = 0;  // block-solver iterations of the last application
 };
 
 static int guard(oracle_handle* h, const std::function<void()>& fn) {
@@ -600,6 +605,14 @@ const char* oracle_last_error(oracle_handle* h) { return h && h->o ? h->o->err.c
 size_t oracle_size(oracle_handle* h) { return h->o->N; }
 void oracle_set_threads(oracle_handle* h, int n) { h->threads = n; }
 void oracle_set_sor_omega(oracle_handle* h, double omega) { h->sor_omega = omega; }
+void oracle_set_block_solver(oracle_handle* h, double reduction, int maxit, int precondition, double weight) {
+  h->ibj_reduction = reduction;
+  h->ibj_maxit = maxit;
+  h->ibj_precondition = precondition;
+  h->ibj_weight = weight;
+}
+unsigned long long oracle_block_solver_iterations(oracle_handle* h) { return h->ibj_iterations; }
+int oracle_iterative_block_jacobi_apply(oracle_handle* h, const double* d, double* v);
 int oracle_block_sor_apply(oracle_handle* h, const double* d, double* v);
 
 int oracle_set_kappa(oracle_handle* h, const double* kappa) {
@@ -727,6 +740,55 @@ static void sor_sweep(oracle_handle* h, const double* d, double* v) {
   }
 }
 
+// The iterative block Jacobi preconditioner, R:dune/pdelab/backend/istl/matrixfree/
+// iterativeblockjacobipreconditioner.hh:267-450 (IterativeBlockJacobiPreconditionerLocalOperator,
+// linear case :355-390): per cell, dune-istl's BiCGSTAB on D_c y = d_c from y = 0 (the
+// BlockDiagonalOperator, :84-150), preconditioned by weight * diag(D_c)^{-1}
+// (LocalPointJacobiPreconditioner, :28-80; the identity when precondition is off), to the
+// residual reduction and iteration limit of BlockSolverOptions (:198-231, defaults 1e-5 and
+// 100). v_c = y (GridOperatorPreconditioner::apply on a zeroed v). A breakdown of a block
+// solve throws, as the ISTL solver does.
+static void ibj_prepare(oracle_handle* h) {
+  auto& o = *h->o;
+  if (!h->bdiag.empty()) return;
+  h->bdiag.resize(o.grid.ncells * o.nloc * o.nloc);
+  o.block_diagonal(h->bdiag.data());
+}
+
+static void ibj_apply(oracle_handle* h, const double* d, double* v) {
+  auto& o = *h->o;
+  const size_t nloc = o.nloc, n2 = nloc * nloc;
+  std::vector<double> b(nloc), x(nloc);
+  h->ibj_iterations = 0;
+  for (size_t c = 0; c < o.grid.ncells; ++c) {
+    const double* D = &h->bdiag[c * n2];
+    oracle::OpFn A = [&](const double* in, double* out) {
+      for (size_t i = 0; i < nloc; ++i) {
+        double s = 0.0;
+        for (size_t j = 0; j < nloc; ++j) s += D[i * nloc + j] * in[j];
+        out[i] = s;
+      }
+    };
+    oracle::OpFn P = [&](const double* in, double* out) {
+      for (size_t i = 0; i < nloc; ++i)
+        out[i] = h->ibj_precondition ? h->ibj_weight * in[i] * (1.0 / D[i * nloc + i]) : in[i];
+    };
+    std::copy(d + c * nloc, d + (c + 1) * nloc, b.begin());
+    std::fill(x.begin(), x.end(), 0.0);
+    pdelab_linear_solver_result r;
+    oracle::bicgstab(nloc, A, P, h->ibj_reduction, h->ibj_maxit, x.data(), b.data(), &r);
+    h->ibj_iterations += r.iterations;
+    std::copy(x.begin(), x.end(), v + c * nloc);
+  }
+}
+
+int oracle_iterative_block_jacobi_apply(oracle_handle* h, const double* d, double* v) {
+  return guard(h, [&] {
+    ibj_prepare(h);
+    ibj_apply(h, d, v);
+  });
+}
+
 // v = the block-SOR preconditioner applied to d (one sweep from v = 0)
 int oracle_block_sor_apply(oracle_handle* h, const double* d, double* v) {
   return guard(h, [&] {
@@ -750,7 +812,12 @@ int oracle_solve(oracle_handle* h, int method, int precond, double reduction, in
     if (sor && matrix_based == 2) throw std::runtime_error("block SOR: not for the transposed operator");
     if (sor && method != PDELAB_SOLVER_BICGSTAB)
       throw std::runtime_error("block SOR: BiCGSTAB only (the preconditioner is not symmetric)");
+    const bool ibj = precond == PDELAB_PREC_ITERATIVE_BLOCK_JACOBI;
+    if (ibj && matrix_based == 2) throw std::runtime_error("iterative block Jacobi: not for the transposed operator");
+    if (ibj && method != PDELAB_SOLVER_BICGSTAB)
+      throw std::runtime_error("iterative block Jacobi: BiCGSTAB only (the preconditioner is an inner iteration)");
     if (sor) sor_prepare(h);
+    else if (ibj) ibj_prepare(h);
     else if (precond == PDELAB_PREC_BLOCK_JACOBI) block_lu_prepare(h, bt);
     oracle::OpFn A;
     if (matrix_based == 1) {
@@ -793,6 +860,8 @@ int oracle_solve(oracle_handle* h, int method, int precond, double reduction, in
           lu_solve(nloc, &h->blu[bt][c * n2], &h->bpiv[bt][c * nloc], d + c * nloc, v + c * nloc);
       else if (sor)
         sor_sweep(h, d, v);
+      else if (ibj)
+        ibj_apply(h, d, v);
       else
         std::copy(d, d + n, v);
     };

@@ -94,6 +94,10 @@ def _bind(L):
     L.oracle_set_threads.argtypes = [vp, ctypes.c_int]
     L.oracle_set_sor_omega.argtypes = [vp, ctypes.c_double]
     L.oracle_block_sor_apply.argtypes = [vp, _dp, _dp]
+    L.oracle_iterative_block_jacobi_apply.argtypes = [vp, _dp, _dp]
+    L.oracle_set_block_solver.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+    L.oracle_block_solver_iterations.restype = ctypes.c_ulonglong
+    L.oracle_block_solver_iterations.argtypes = [vp]
     for fn in ("oracle_set_kappa", "oracle_set_cell_source", "oracle_point_diagonal", "oracle_interpolate",
                "oracle_block_diagonal"):
         getattr(L, fn).argtypes = [vp, _dp]
@@ -319,6 +323,18 @@ class Oracle:
         v = np.zeros(self.N)
         self._check(self._L.oracle_block_sor_apply(self.h, _p(d), _p(v)))
         return v
+
+    def set_block_solver(self, reduction=1e-5, maxit=100, precondition=True, weight=1.0):
+        """BlockSolverOptions of the iterative block Jacobi preconditioner."""
+        self._L.oracle_set_block_solver(self.h, reduction, maxit, 1 if precondition else 0, weight)
+
+    def iterative_block_jacobi_apply(self, d):
+        """v = the iterative block Jacobi preconditioner applied to d; also returns the block
+        solvers' iteration count (sum over the cells of the half-step count rounded up)."""
+        d = np.ascontiguousarray(d, dtype=np.float64)
+        v = np.zeros(self.N)
+        self._check(self._L.oracle_iterative_block_jacobi_apply(self.h, _p(d), _p(v)))
+        return v, int(self._L.oracle_block_solver_iterations(self.h))
 
     def solve(self, r, z=None, method=cfgmod.SOLVER_CG, precond=cfgmod.PREC_JACOBI, reduction=1e-10,
               maxit=5000, matrix_based=0):

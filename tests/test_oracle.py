@@ -487,6 +487,76 @@ def test_block_sor_rejects_cg_and_transpose(of):
         o.solve(rhs.copy(), method=1, precond=of.cfgmod.PREC_BLOCK_SOR, matrix_based=2)
 
 
+IBJ_CASES = [
+    ("q2_var", dict(n=3, degree=2, alpha=3.0), 4),
+    ("q3_conv", dict(nx=2, ny=3, nz=2, degree=3, alpha=3.0, kappa0=0.05, b=(1.0, -0.5, 0.25)), None),
+    ("q1_nipg", dict(n=3, degree=1, method=0, alpha=2.0), 6),
+]
+
+
+@pytest.mark.parametrize("name,kw,seed", IBJ_CASES, ids=[c[0] for c in IBJ_CASES])
+def test_iterative_block_jacobi_vs_textbook_block_solves(of, name, kw, seed):
+    """The oracle's iterative block Jacobi (iterativeblockjacobipreconditioner.hh:355-390: per
+    cell BiCGSTAB on the diagonal block, point-Jacobi preconditioned, reduction 1e-5, at most
+    100 iterations) against the numpy textbook BiCGSTAB on the assembled blocks with the
+    preconditioner weight / diag: the same vector up to where a block's last half step falls
+    (1e-5 relative), the same iteration count to 2 %, every block's residual reduced by 1e-5.
+    Driven to a 1e-14 reduction it reproduces the exact block solve."""
+    cfg, o = _oracle_case(of, kw, seed)
+    blocks = o.block_diagonal()
+    nloc = blocks.shape[-1]
+    d = np.random.default_rng(51).normal(size=o.N)
+    for weight, precondition in ((1.0, True), (0.7, True), (1.0, False)):
+        o.set_block_solver(1e-5, 100, precondition, weight)
+        v, its = o.iterative_block_jacobi_apply(d)
+        want, its_np = [], 0
+        for c in range(blocks.shape[0]):
+            D = blocks[c]
+            dinv = weight / np.diag(D)
+            pre = (lambda r, dinv=dinv: dinv * r) if precondition else (lambda r: r.copy())
+            x, i = _numpy_bicgstab(D, d[c * nloc:(c + 1) * nloc], pre, 1e-5)
+            assert i > 0
+            want.append(x)
+            its_np += i
+        want = np.concatenate(want)
+        # a block whose defect ends next to the 1e-5 threshold may stop one half step apart
+        # under different rounding, which moves its solution by about the threshold; without
+        # the point-Jacobi scaling the Q3 blocks (condition ~1e4) take long enough for
+        # BiCGSTAB's rounding to move the iterates by 1e-5 relative
+        tol, cnt = (1e-5, 50) if precondition else (1e-4, 10)
+        assert np.abs(v - want).max() <= tol * np.abs(want).max(), (weight, precondition)
+        assert abs(its - its_np) <= max(1, its_np // cnt), (its, its_np)
+        res = np.linalg.norm((blocks @ v.reshape(-1, nloc, 1))[..., 0] - d.reshape(-1, nloc), axis=1)
+        assert (res <= 1.0001e-5 * np.linalg.norm(d.reshape(-1, nloc), axis=1)).all()
+    exact = np.linalg.solve(blocks, d.reshape(-1, nloc, 1))[..., 0].ravel()
+    o.set_block_solver(1e-5, 100, True, 1.0)
+    v, _ = o.iterative_block_jacobi_apply(d)
+    e = np.abs(v - exact).max() / np.abs(exact).max()
+    assert 1e-12 < e < 1e-2  # an inexact block solve
+    o.set_block_solver(1e-14, 1000, True, 1.0)
+    v, _ = o.iterative_block_jacobi_apply(d)
+    assert np.abs(v - exact).max() <= 1e-9 * np.abs(exact).max()
+
+
+def test_iterative_block_jacobi_bicgstab(of):
+    """matrix_free_linear.cc:295-330: BiCGSTAB with the iterative block Jacobi preconditioner
+    converges to the solver tolerance, in about as many iterations as with the exact block
+    inverse; CG and the transposed operator are refused."""
+    cfg = of.cfgmod.make_config(n=4, degree=2, alpha=3.0, kappa0=0.05, b=(1.0, 0.5, 0.25))
+    o = of.Oracle(cfg)
+    A = o.dense_matrix()
+    rhs = np.random.default_rng(61).normal(size=o.N)
+    z, _, res = o.solve(rhs.copy(), method=1, precond=of.cfgmod.PREC_ITERATIVE_BLOCK_JACOBI, reduction=1e-10)
+    assert res["converged"]
+    assert np.linalg.norm(A @ z - rhs) <= 1e-10 * np.linalg.norm(rhs)
+    _, _, rb = o.solve(rhs.copy(), method=1, precond=of.cfgmod.PREC_BLOCK_JACOBI, reduction=1e-10)
+    assert abs(res["iterations"] - rb["iterations"]) <= max(3, 0.3 * rb["iterations"])
+    with pytest.raises(RuntimeError, match="BiCGSTAB only"):
+        o.solve(rhs.copy(), method=0, precond=of.cfgmod.PREC_ITERATIVE_BLOCK_JACOBI)
+    with pytest.raises(RuntimeError, match="transposed"):
+        o.solve(rhs.copy(), method=1, precond=of.cfgmod.PREC_ITERATIVE_BLOCK_JACOBI, matrix_based=2)
+
+
 @pytest.mark.parametrize("precond", [0, 1], ids=["none", "point_jacobi"])
 @pytest.mark.parametrize("k", [1, 2])
 def test_bicgstab_vs_numpy_textbook(of, precond, k):
