@@ -399,6 +399,64 @@ def profile_summary(d, n, k):
     return "\n".join(md)
 
 
+FP64_PEAK = 78.6e12  # MI355X dense FP64 (vector and matrix), FLOP/s
+APPLY_WORKLOADS = {  # grid size (threads) of the apply launches of this benchmark -> workload
+    "2097152": "64^3 Q3", "4194304": "64^3 Q3", "995328": "96^3 Q2", "7962624": "96^3 Q2"}
+
+
+def counter_summary(d):
+    """Markdown table of the apply kernels' counters from the rocprofv3 passes kept under `d`
+    (d/trace: --kernel-trace --stats; d/pmc*: --pmc passes; see the profiles/ summary for the
+    counter sets). Per kernel: median duration, achieved HBM fraction (16 B/DoF FP64, 8 B/DoF
+    FP32 over 8 TB/s), VALU busy (SQ_INSTS_VALU x 4 cycles over the 1024 SIMDs' cycles; GRBM_GUI_ACTIVE
+    is summed over the 8 XCDs), FP64 utilisation ((2 FMA + ADD + MUL)_F64 x 64 lanes per wave
+    instruction / duration / 78.6 TFLOP/s), LDS busy and bank-conflict cycles (per CU), SALU busy,
+    instructions per wave, fraction of wave cycles waiting, mean resident waves per CU, HBM
+    traffic over algorithmic bytes (FETCH_SIZE x 2 and WRITE_SIZE x 1, the axpy calibration)."""
+    import collections
+    import re
+    agg = collections.defaultdict(lambda: collections.defaultdict(list))
+    meta = {}
+    for r in _csv_rows(d, "_counter_collection.csv"):
+        key = (r["Kernel_Name"], r["Grid_Size"])
+        agg[key][r["Counter_Name"]].append(float(r["Counter_Value"]))
+        meta[key] = r["VGPR_Count"]
+    dur = collections.defaultdict(list)
+    for r in _csv_rows(os.path.join(d, "trace"), "_kernel_trace.csv"):
+        dur[r["Kernel_Name"]].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+
+    def short(k):
+        m = re.search(r"dg_apply\w*kernel<[^(]*>", k)
+        return m.group(0) if m else None
+
+    out = ["| kernel | workload | us | HBM frac | VALU busy | FP64 util | LDS busy | LDS conflict | SALU busy "
+           "| VALU/wave | LDS/wave | VMEM/wave | wait | waves/CU | traffic/alg |",
+           "|---|---|---|---|---|---|---|---|---|---|---|---|---|---|---|"]
+    for key, cs in sorted(agg.items(), key=lambda kv: short(kv[0][0]) or ""):
+        s = short(key[0])
+        if not s or key[1] not in APPLY_WORKLOADS:
+            continue
+        c = {k: sum(v) / len(v) for k, v in cs.items()}
+        cyc = c["GRBM_GUI_ACTIVE"] / 8.0
+        ts = sorted(dur.get(key[0], []))
+        us = ts[len(ts) // 2] if ts else float("nan")
+        wl = APPLY_WORKLOADS[key[1]]
+        n, k = (64, 3) if "Q3" in wl else (96, 2)
+        N = n ** 3 * (k + 1) ** 3
+        bpd = 16.0 if "double" in s else 8.0
+        waves = c["SQ_WAVES"]
+        fp64 = 2 * c["SQ_INSTS_VALU_FMA_F64"] + c["SQ_INSTS_VALU_ADD_F64"] + c["SQ_INSTS_VALU_MUL_F64"]
+        traffic = (2.0 * c["FETCH_SIZE"] + c["WRITE_SIZE"]) * 1024.0 / (bpd * N)
+        out.append(
+            f"| `{s}` | {wl} | {us:.1f} | {bpd * N / (us * 1e-6) / 8e12:.2f} | "
+            f"{c['SQ_INSTS_VALU'] * 4 / 1024 / cyc:.2f} | {fp64 * 64 / (us * 1e-6) / FP64_PEAK:.2f} | "
+            f"{c['SQ_LDS_IDX_ACTIVE'] / 256 / cyc:.2f} | {c['SQ_LDS_BANK_CONFLICT'] / 256 / cyc:.2f} | "
+            f"{c['SQ_INSTS_SALU'] / 1024 / cyc:.2f} | {c['SQ_INSTS_VALU'] / waves:.0f} | {c['SQ_INSTS_LDS'] / waves:.0f} | "
+            f"{(c['SQ_INSTS_VMEM_RD'] + c['SQ_INSTS_VMEM_WR']) / waves:.0f} | {c['SQ_WAIT_ANY'] / c['SQ_WAVE_CYCLES']:.2f} | "
+            f"{c['SQ_WAVE_CYCLES'] / cyc / 256:.1f} | {traffic:.3f} |")
+    return "\n".join(out)
+
+
 def measured_traffic(n, k):
     """HBM bytes per apply launch from the committed rocprofv3 counter runs (or None)."""
     try:
@@ -426,9 +484,14 @@ def main():
                     help="mixed-precision CG: FP32 residual reduction between FP64 residual replacements")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run 5 axpy launches (reads 16 B/DoF, writes 8 B/DoF) to scale HBM counters")
+    ap.add_argument("--counter-summary", metavar="DIR",
+                    help="print the counter table of the apply kernels from rocprofv3 passes under DIR and exit")
     ap.add_argument("--profile-summary", metavar="DIR",
                     help="print the summary of the rocprofv3 runs of this benchmark kept under DIR and exit")
     args = ap.parse_args()
+    if args.counter_summary:
+        print(counter_summary(args.counter_summary))
+        return
     if args.profile_summary:
         print(profile_summary(args.profile_summary, args.n, args.degree))
         return
