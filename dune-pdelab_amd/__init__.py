@@ -183,6 +183,8 @@ class GridOperator:
         kp = None
         if kappa is not None:
             self._kappa = np.ascontiguousarray(kappa, dtype=np.float64)
+            if cfg.kappa_per_cell and self._kappa.size != self._kappa_size(cfg):
+                raise ValueError(f"kappa needs {self._kappa_size(cfg)} values, got {self._kappa.size}")
             kp = self._kappa.ctypes.data_as(_dp)
         h = _vp()
         st = L.pdelab_gpu_create(ctypes.byref(cfg), kp, ctypes.byref(h))
@@ -249,8 +251,15 @@ class GridOperator:
         self._check(lib().pdelab_gpu_synchronize(self.h))
 
     # -- parameters
+    @staticmethod
+    def _kappa_size(cfg):
+        # one value per cell, or a_x, a_y, a_z per cell for kappa_per_cell == 2
+        return cfgmod.ncells(cfg) * (3 if cfg.kappa_per_cell == 2 else 1)
+
     def set_kappa(self, kappa):
         k = np.ascontiguousarray(kappa, dtype=np.float64)
+        if k.size != self._kappa_size(self.cfg):
+            raise ValueError(f"kappa needs {self._kappa_size(self.cfg)} values, got {k.size}")
         self._check(lib().pdelab_gpu_set_kappa(self.h, k.ctypes.data_as(_dp)))
 
     def set_log_kappa_device(self, logk):

@@ -137,7 +137,8 @@ def make_config(n=4, degree=1, nx=None, ny=None, nz=None, lx=1.0, ly=1.0, lz=1.0
     cfg.weights = weights
     cfg.alpha = alpha
     cfg.intorderadd = intorderadd
-    cfg.kappa_per_cell = 1 if kappa_per_cell else 0
+    # False/True, or 2: a diagonal tensor diag(a_x, a_y, a_z) per cell (three values per cell)
+    cfg.kappa_per_cell = 2 if kappa_per_cell == 2 else (1 if kappa_per_cell else 0)
     cfg.kappa0 = kappa0
     for i in range(3):
         cfg.b[i] = b[i]

@@ -100,7 +100,8 @@ __device__ inline void bj_line(const DGArgs<double>& a, int d, int t, double kc,
   else bj_line_d<N1, 2, TRANS, CONV, VARK>(a, t, kc, flo, fhi, L);
 }
 
-// Cell coefficients of direction d for line_op: the cell's kappa and its two faces.
+// Cell coefficients of direction d for line_op: the cell's coefficient in direction d and its
+// two faces normal to d.
 __device__ inline void bj_cell_coeffs(const DGArgs<double>& a, const double* __restrict__ fc, const BJGeom& g,
                                       long long c, int d, int coord, double& kc, FaceCoef<double>& flo,
                                       FaceCoef<double>& fhi) {
@@ -111,12 +112,12 @@ __device__ inline void bj_cell_coeffs(const DGArgs<double>& a, const double* __r
     return;
   }
   const double* f = fc + c * FC;
-  kc = f[0];
-  flo = FaceCoef<double>{f[1 + 3 * d], f[2 + 3 * d], f[3 + 3 * d]};
+  kc = f[d];
+  flo = FaceCoef<double>{f[FCF + 3 * d], f[FCF + 1 + 3 * d], f[FCF + 2 + 3 * d]};
   fhi = FaceCoef<double>{0.0, 0.0, 0.0};
   if (coord < g.n[d] - 1) {
     const double* fu = fc + (c + g.stride[d]) * FC;
-    fhi = FaceCoef<double>{fu[1 + 3 * d], fu[2 + 3 * d], fu[3 + 3 * d]};
+    fhi = FaceCoef<double>{fu[FCF + 3 * d], fu[FCF + 1 + 3 * d], fu[FCF + 2 + 3 * d]};
   }
 }
 

@@ -70,7 +70,7 @@ __global__ __launch_bounds__(64) void block_sor_wave_kernel(const T* __restrict_
   for (int dd = 0; dd < 3; ++dd) lo[dd] = coord[dd] > 0;
   // ---- all global loads
   T dz[N1], ux[N1], uy[N1], uz[N1], drow[NL];
-  T kc = a.kappa0;
+  T kc[3] = {a.kappa0, a.kappa0, a.kappa0};
   FaceCoef<T> flo[3];
   const FaceCoef<T> fnone{a.wI_lo, a.wI_hi, a.PI};
 #pragma unroll
@@ -87,10 +87,11 @@ __global__ __launch_bounds__(64) void block_sor_wave_kernel(const T* __restrict_
     }
     if (VARK) {
       const T* fb = fc + c * FC;
-      kc = fb[0];
 #pragma unroll
-      for (int dd = 0; dd < 3; ++dd)
-        if (lo[dd]) flo[dd] = FaceCoef<T>{fb[1 + 3 * dd], fb[2 + 3 * dd], fb[3 + 3 * dd]};
+      for (int dd = 0; dd < 3; ++dd) {
+        kc[dd] = fb[dd];
+        if (lo[dd]) flo[dd] = FaceCoef<T>{fb[FCF + 3 * dd], fb[FCF + 1 + 3 * dd], fb[FCF + 2 + 3 * dd]};
+      }
     }
   }
   if (dof_lane) {
@@ -107,9 +108,9 @@ __global__ __launch_bounds__(64) void block_sor_wave_kernel(const T* __restrict_
     T zero[N1], sx[N1], sy[N1], sz[N1];
 #pragma unroll
     for (int e = 0; e < N1; ++e) zero[e] = sx[e] = sy[e] = sz[e] = T(0);
-    if (lo[0]) line_apply<N1, 0, T, false, CONV, false, VARK>(a, lm, zero, ux, zero, true, false, kc, flo[0], fnone, sx);
-    if (lo[1]) line_apply<N1, 1, T, false, CONV, false, VARK>(a, lm, zero, uy, zero, true, false, kc, flo[1], fnone, sy);
-    if (lo[2]) line_apply<N1, 2, T, false, CONV, false, VARK>(a, lm, zero, uz, zero, true, false, kc, flo[2], fnone, sz);
+    if (lo[0]) line_apply<N1, 0, T, false, CONV, false, VARK>(a, lm, zero, ux, zero, true, false, kc[0], flo[0], fnone, sx);
+    if (lo[1]) line_apply<N1, 1, T, false, CONV, false, VARK>(a, lm, zero, uy, zero, true, false, kc[1], flo[1], fnone, sy);
+    if (lo[2]) line_apply<N1, 2, T, false, CONV, false, VARK>(a, lm, zero, uz, zero, true, false, kc[2], flo[2], fnone, sz);
     // the three directions summed in the y layout through the rotated cell slots
 #pragma unroll
     for (int e = 0; e < N1; ++e) {

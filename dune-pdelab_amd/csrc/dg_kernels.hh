@@ -53,7 +53,14 @@ struct DGArgs {
   long long ncells;
   int weightsOn;
   int bc[6];
+  int aniso;  // cell-wise diagonal tensor A = diag(a_x, a_y, a_z): kappa holds 3 values per cell
 };
+
+// Index of the diffusion coefficient of cell c in direction d: kappa holds one value per
+// cell (A = kappa I) or three (A = diag(a_x, a_y, a_z)).
+__host__ __device__ __forceinline__ long long kappa_index(int aniso, long long c, int d) {
+  return aniso ? 3 * c + d : c;
+}
 
 // Maps the hardware block index to a logical block so that each of the 8 XCDs
 // (blocks are dispatched round-robin over XCDs) works on one contiguous range of
@@ -89,10 +96,12 @@ struct FaceCoef {
   T wlo, whi, P;
 };
 
-// Per-cell face coefficients for cell-wise kappa, FC values per cell: kappa of the cell,
-// then (wlo, whi, P) of its lower face in x, y and z (zero where that face is a boundary
-// face); the upper face of a cell is the lower face of its upper neighbour.
-constexpr int FC = 10;
+// Per-cell face coefficients for cell-wise kappa, FC values per cell: the cell's diffusion
+// coefficient in x, y and z (a_d = n.A n for the faces normal to direction d; all three equal
+// for A = kappa I), then (wlo, whi, P) of its lower face in x, y and z (zero where that face is
+// a boundary face); the upper face of a cell is the lower face of its upper neighbour.
+constexpr int FC = 12;
+constexpr int FCF = 3;  // offset of the face coefficients
 
 template <typename T>
 __global__ __launch_bounds__(256) void face_coeff_kernel(const double* __restrict__ kappa, T* __restrict__ fc,
@@ -100,16 +109,16 @@ __global__ __launch_bounds__(256) void face_coeff_kernel(const double* __restric
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.ncells) return;
   const int coord[3] = {(int)(c % a.n[0]), (int)((c / a.n[0]) % a.n[1]), (int)(c / ((long long)a.n[0] * a.n[1]))};
-  const double kc = kappa[c];
   T* f = fc + c * FC;
-  f[0] = (T)kc;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
+    const double kc = kappa[kappa_index(a.aniso, c, d)];
+    f[d] = (T)kc;
     double wlo = 0, whi = 0, P = 0;
-    if (coord[d] > 0) face_coeffs<double>(a, kappa[c - a.stride[d]], kc, wlo, whi, P);
-    f[1 + 3 * d] = (T)wlo;
-    f[2 + 3 * d] = (T)whi;
-    f[3 + 3 * d] = (T)P;
+    if (coord[d] > 0) face_coeffs<double>(a, kappa[kappa_index(a.aniso, c - a.stride[d], d)], kc, wlo, whi, P);
+    f[FCF + 3 * d] = (T)wlo;
+    f[FCF + 1 + 3 * d] = (T)whi;
+    f[FCF + 2 + 3 * d] = (T)P;
   }
 }
 
@@ -441,7 +450,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
     hi[d] = active && coord[d] < a.n[d] - 1;
   }
   T uz[N1], zl[N1], zr[N1], yl[N1], yr[N1], xe[N1];
-  T kc = a.kappa0;
+  T kc[3] = {a.kappa0, a.kappa0, a.kappa0};
   FaceCoef<T> flo[3], fhi[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) flo[d] = fhi[d] = FaceCoef<T>{a.wI_lo, a.wI_hi, a.PI};
@@ -458,13 +467,13 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
     if (ld_xe) load_line<N1, 2, T>(zc + (first && lo[0] ? -NL : NL), off, p, q, xe);
     if (VARK) {
       const T* fb = fc + (c0 + lc) * FC;
-      kc = fb[0];
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        if (lo[d]) flo[d] = FaceCoef<T>{fb[1 + 3 * d], fb[2 + 3 * d], fb[3 + 3 * d]};
+        kc[d] = fb[d];
+        if (lo[d]) flo[d] = FaceCoef<T>{fb[FCF + 3 * d], fb[FCF + 1 + 3 * d], fb[FCF + 2 + 3 * d]};
         if (hi[d]) {
           const T* fn = fb + a.stride[d] * FC;
-          fhi[d] = FaceCoef<T>{fn[1 + 3 * d], fn[2 + 3 * d], fn[3 + 3 * d]};
+          fhi[d] = FaceCoef<T>{fn[FCF + 3 * d], fn[FCF + 1 + 3 * d], fn[FCF + 2 + 3 * d]};
         }
       }
     }
@@ -485,7 +494,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
   __syncthreads();
   T sx[N1], sy[N1], sz[N1];
   if (active) {
-    line_apply<N1, 2, T, TRANS, CONV, REACT, VARK>(a, lm, uz, zl, zr, lo[2], hi[2], kc, flo[2], fhi[2], sz);
+    line_apply<N1, 2, T, TRANS, CONV, REACT, VARK>(a, lm, uz, zl, zr, lo[2], hi[2], kc[2], flo[2], fhi[2], sz);
     T u[N1], ul[N1], ur[N1];
 #pragma unroll
     for (int e = 0; e < N1; ++e) {
@@ -494,7 +503,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
       ul[e] = lo[1] ? YMl[sl] : T(0);
       ur[e] = hi[1] ? YPl[sl] : T(0);
     }
-    line_apply<N1, 1, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, lo[1], hi[1], kc, flo[1], fhi[1], sy);
+    line_apply<N1, 1, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, lo[1], hi[1], kc[1], flo[1], fhi[1], sy);
     // x-neighbours: staged cell lc-1 / lc+1 (U[0] and U[CPB+1] hold the end cells)
     const T* xlo = Ul - NLP;
     const T* xhi = Ul + NLP;
@@ -505,7 +514,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_kernel(const
       ul[e] = lo[0] ? xlo[sl] : T(0);
       ur[e] = hi[0] ? xhi[sl] : T(0);
     }
-    line_apply<N1, 0, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, lo[0], hi[0], kc, flo[0], fhi[0], sx);
+    line_apply<N1, 0, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, lo[0], hi[0], kc[0], flo[0], fhi[0], sx);
   }
   __syncthreads();  // all reads of the staged cells done
   T pq = T(0);
@@ -638,7 +647,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
   const int off = lc * NL;  // lane's cell block relative to c0
   const T* zc = z + c0 * NL;
   T yl[N1], yr[N1], xe[N1];
-  T kc = a.kappa0;
+  T kc[3] = {a.kappa0, a.kappa0, a.kappa0};
   FaceCoef<T> flo[3], fhi[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) flo[d] = fhi[d] = FaceCoef<T>{a.wI_lo, a.wI_hi, a.PI};
@@ -666,16 +675,14 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
   // staging barrier (fewer registers live across the loads and the staging)
   const T* fb = fc + (c0 + lc) * FC;
   auto load_faces = [&](int d) {
-    if (lo[d]) flo[d] = FaceCoef<T>{fb[1 + 3 * d], fb[2 + 3 * d], fb[3 + 3 * d]};
+    kc[d] = fb[d];
+    if (lo[d]) flo[d] = FaceCoef<T>{fb[FCF + 3 * d], fb[FCF + 1 + 3 * d], fb[FCF + 2 + 3 * d]};
     if (hi[d]) {
       const T* fn = fb + a.stride[d] * FC;
-      fhi[d] = FaceCoef<T>{fn[1 + 3 * d], fn[2 + 3 * d], fn[3 + 3 * d]};
+      fhi[d] = FaceCoef<T>{fn[FCF + 3 * d], fn[FCF + 1 + 3 * d], fn[FCF + 2 + 3 * d]};
     }
   };
-  if (VARK) {
-    kc = fb[0];
-    load_faces(2);
-  }
+  if (VARK) load_faces(2);
   // slot of element e of the lane's x-, y- and z-line, relative to the cell region
   // (an x-neighbour, of the other parity, keeps the same element at the own slot ^ XP)
   constexpr int XP = XS ? 16 : 0;
@@ -710,7 +717,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
   {
     {
       T sz[N1];
-      line_apply<N1, 2, T, TRANS, CONV, REACT, VARK>(a, lm, uz, zl, zr, lo[2], hi[2], kc, flo[2], fhi[2], sz);
+      line_apply<N1, 2, T, TRANS, CONV, REACT, VARK>(a, lm, uz, zl, zr, lo[2], hi[2], kc[2], flo[2], fhi[2], sz);
 #pragma unroll
       for (int e = 0; e < N1; ++e) LS[YMO + az[e]] = sz[e];
     }
@@ -721,7 +728,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
       ul[e] = lo[1] ? yl[e] : T(0);
       ur[e] = hi[1] ? yr[e] : T(0);
     }
-    line_apply<N1, 1, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, lo[1], hi[1], kc, flo[1], fhi[1], sy);
+    line_apply<N1, 1, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, lo[1], hi[1], kc[1], flo[1], fhi[1], sy);
     // x-neighbours: the staged cells lc-1 and lc+1 (the end cells for the chunk ends)
 #pragma unroll
     for (int e = 0; e < N1; ++e) {
@@ -731,7 +738,7 @@ __global__ __launch_bounds__(ApplyShape<N1>::THREADS) void dg_apply_rowreg_kerne
       ur[e] = hi[0] ? LS[UO + NLP + xn] : T(0);
     }
     T sx[N1];
-    line_apply<N1, 0, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, lo[0], hi[0], kc, flo[0], fhi[0], sx);
+    line_apply<N1, 0, T, TRANS, CONV, REACT, VARK>(a, lm, u, ul, ur, lo[0], hi[0], kc[0], flo[0], fhi[0], sx);
 #pragma unroll
     for (int e = 0; e < N1; ++e) LS[YPO + ax[e]] = sx[e];
   }
@@ -808,10 +815,10 @@ __global__ __launch_bounds__(256) void dg_diag_kernel(T* __restrict__ d, const T
   const int li = (int)(gid % NL);
   const int ii[3] = {li % N1, (li / N1) % N1, li / L};
   const int coord[3] = {(int)(c % a.n[0]), (int)((c / a.n[0]) % a.n[1]), (int)(c / ((long long)a.n[0] * a.n[1]))};
-  const T kc = VARK ? kappa[c] : a.kappa0;
   const T theta = a.theta;
   T acc = T(0);
   for (int D = 0; D < 3; ++D) {
+    const T kc = VARK ? kappa[kappa_index(a.aniso, c, D)] : a.kappa0;
     const int i = ii[D];
     const T Sd = a.S[D], Sc = a.Sc[D], bd = a.b[D];
     const T e0 = (i == 0) ? T(1) : T(0), e1 = (i == N1 - 1) ? T(1) : T(0);
@@ -819,7 +826,7 @@ __global__ __launch_bounds__(256) void dg_diag_kernel(T* __restrict__ d, const T
     T dd = Sd * kc * a.Kdiag[i] - Sc * bd * a.Gdiag[i];
     if (coord[D] > 0) {
       T wlo, whi, P;
-      if (VARK) face_coeffs<T>(a, kappa[c - a.stride[D]], kc, wlo, whi, P);
+      if (VARK) face_coeffs<T>(a, kappa[kappa_index(a.aniso, c - a.stride[D], D)], kc, wlo, whi, P);
       else { wlo = a.wI_lo; whi = a.wI_hi; P = a.PI; }
       dd += Sd * (whi * e0d0 + P * e0 - theta * whi * e0d0);
       if (bd <= T(0)) dd += Sc * (-bd) * e0;
@@ -835,7 +842,7 @@ __global__ __launch_bounds__(256) void dg_diag_kernel(T* __restrict__ d, const T
     }
     if (coord[D] < a.n[D] - 1) {
       T wlo, whi, P;
-      if (VARK) face_coeffs<T>(a, kc, kappa[c + a.stride[D]], wlo, whi, P);
+      if (VARK) face_coeffs<T>(a, kc, kappa[kappa_index(a.aniso, c + a.stride[D], D)], wlo, whi, P);
       else { wlo = a.wI_lo; whi = a.wI_hi; P = a.PI; }
       dd += Sd * (-wlo * e1d1 + P * e1 + theta * wlo * e1d1);
       if (bd > T(0)) dd += Sc * bd * e1;

@@ -27,6 +27,7 @@ struct LoadArgs {
   int weightsOn;
   int bc[6];
   long long bface_offset[7];
+  int aniso;  // kappa holds a_x, a_y, a_z per cell (kappa_index)
 };
 
 __device__ inline double exact_u(const LoadArgs& a, const double x[3]) {
@@ -184,7 +185,7 @@ __global__ __launch_bounds__(256) void load_boundary_kernel(double* __restrict__
     return;
   }
   const double hF = vol / area;
-  const double kc = kappa ? kappa[c] : a.kappa0;
+  const double kc = kappa ? kappa[kappa_index(a.aniso, c, d)] : a.kappa0;
   const double nf = nsign * a.b[d];
   const double ds = upper ? a.d1[i] : a.d0[i];
   const double Pb = (a.alphak / hF) * (a.weightsOn ? kc : 1.0);

@@ -79,6 +79,7 @@ static void fill_args(H* h, DGArgs<T>& a) {
   a.alphak = (T)(c.alpha * (double)(k * (k + 3 - 1)));
   a.kappa0 = (T)c.kappa0;
   a.weightsOn = c.weights == PDELAB_WEIGHTS_ON ? 1 : 0;
+  a.aniso = c.kappa_per_cell == 2 ? 1 : 0;
   // constant-kappa face coefficients, same formulas as face_coeffs()
   {
     const T klo = (T)c.kappa0, khi = (T)c.kappa0;
@@ -288,6 +289,7 @@ static void fill_load_args(H* h) {
   a.theta = (double)h->a64.theta;
   a.alphak = (double)h->a64.alphak;
   a.weightsOn = h->a64.weightsOn;
+  a.aniso = h->a64.aniso;
   for (int s = 0; s < 6; ++s) a.bc[s] = h->a64.bc[s];
   a.bface_offset[0] = 0;
   for (int s = 0; s < 6; ++s) {
@@ -846,6 +848,7 @@ static int validate(const pdelab_config* c, std::string& msg) {
       return PDELAB_ERR_INVALID_ARG;
     }
   }
+  if (c->kappa_per_cell < 0 || c->kappa_per_cell > 2) { msg = "kappa_per_cell must be 0, 1 or 2"; return PDELAB_ERR_INVALID_ARG; }
   if (c->kappa_per_cell == 0 && !(c->kappa0 > 0)) { msg = "kappa0 must be > 0"; return PDELAB_ERR_INVALID_ARG; }
   return PDELAB_OK;
 }
@@ -866,7 +869,8 @@ static int update_face_coeffs(H* h) {
 }
 
 static int upload_kappa(H* h, const double* kappa_host) {
-  const size_t nc = (size_t)h->ncells;
+  // one value per cell, or a_x, a_y, a_z per cell (kappa_per_cell == 2)
+  const size_t nc = (size_t)h->ncells * (h->a64.aniso ? 3 : 1);
   for (size_t i = 0; i < nc; ++i)
     if (!(kappa_host[i] > 0) || !std::isfinite(kappa_host[i]))
       return fail(h, PDELAB_ERR_INVALID_ARG, "kappa must be finite and > 0");
@@ -1211,6 +1215,7 @@ int pdelab_gpu_set_kappa(pdelab_gpu_gridoperator* h, const double* kappa_host) {
 
 int pdelab_gpu_set_log_kappa_device(pdelab_gpu_gridoperator* h, const double* logk) {
   if (!h || !logk) return PDELAB_ERR_INVALID_ARG;
+  if (h->a64.aniso) return fail(h, PDELAB_ERR_UNSUPPORTED, "log kappa: isotropic A = kappa I only");
   const long long nc = h->ncells;
   if (!h->d_kappa) HIPCHK(h, hipMalloc(&h->d_kappa, sizeof(double) * nc));
   if (!h->d_kappa32) HIPCHK(h, hipMalloc(&h->d_kappa32, sizeof(float) * nc));
@@ -1860,6 +1865,7 @@ int pdelab_gpu_onestep_step(pdelab_gpu_gridoperator* h, int scheme, int method, 
 
 int pdelab_gpu_grad_logkappa(pdelab_gpu_gridoperator* h, const double* u, const double* lambda, double* g) {
   if (!h || !u || !lambda || !g) return PDELAB_ERR_INVALID_ARG;
+  if (h->a64.aniso) return fail(h, PDELAB_ERR_UNSUPPORTED, "kappa gradient: isotropic A = kappa I only");
   if (!h->d_kappa) {
     std::vector<double> k(h->ncells, h->cfg.kappa0);
     CHK(upload_kappa(h, k.data()));

@@ -28,6 +28,7 @@
 #include <cstring>
 #include <memory>
 #include <stdexcept>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -309,20 +310,33 @@ class GridOperator {
     cfg.weights = lop.weights();
     cfg.alpha = lop.alpha();
     cfg.intorderadd = lop.intorderadd();
-    // A = kappa * I evaluated at cell centres (permeabilityIsConstantPerCell)
-    std::vector<double> kappa(nc);
+    // A evaluated at cell centres (permeabilityIsConstantPerCell): a scalar kappa (A = kappa I)
+    // or a 3x3 tensor (PermTensorType, convectiondiffusionparameter.hh:74-84), which must be
+    // diagonal on this path (it then keeps the Kronecker structure)
     const std::array<double, 3> centre = {0.5, 0.5, 0.5};
     Element e0 = detail::element(grid, 0);
+    using AType = decltype(par.A(e0, centre));
+    constexpr bool scalarA = std::is_arithmetic<AType>::value;
+    std::vector<double> kappa(nc * (scalarA ? 1 : 3));
     auto b0 = par.b(e0, centre);
     double c0 = par.c(e0, centre);
     for (size_t c = 0; c < nc; ++c) {
       Element e = detail::element(grid, c);
-      kappa[c] = par.A(e, centre);
+      if constexpr (scalarA) {
+        kappa[c] = par.A(e, centre);
+      } else {
+        const auto A = par.A(e, centre);
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j)
+            if (i != j && A[i][j] != 0.0)
+              throw NotImplemented("GridOperator: the diffusion tensor must be diagonal on this path");
+        for (int d = 0; d < 3; ++d) kappa[3 * c + d] = A[d][d];
+      }
       auto bc = par.b(e, centre);
       if (bc != b0 || par.c(e, centre) != c0)
         throw NotImplemented("GridOperator: b and c must be constant over the grid on this path");
     }
-    cfg.kappa_per_cell = 1;
+    cfg.kappa_per_cell = scalarA ? 1 : 2;
     for (int d = 0; d < 3; ++d) cfg.b[d] = b0[d];
     cfg.c = c0;
     for (int s = 0; s < 6; ++s) {

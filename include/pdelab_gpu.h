@@ -158,7 +158,11 @@ typedef struct pdelab_config {
   double alpha;
   int intorderadd;            /* >= 0; affects only the load vector (f, g) quadrature */
   /* parameter class: A = kappa * I (constant per cell), b constant, c constant */
-  int kappa_per_cell;         /* 0: kappa = kappa0 everywhere; 1: per-cell array given at create */
+  int kappa_per_cell;         /* 0: A = kappa0 I everywhere; 1: A = kappa_c I, one value per cell given at
+                                 create; 2: A = diag(a_x, a_y, a_z) per cell, three values per cell
+                                 (the diffusion tensor of ConvectionDiffusionParameterInterface::A,
+                                 R:dune/pdelab/localoperator/convectiondiffusionparameter.hh:74-84,
+                                 constant per cell and diagonal: it keeps the Kronecker structure) */
   double kappa0;
   double b[3];
   double c;
@@ -225,8 +229,9 @@ typedef struct pdelab_gpu_gridoperator pdelab_gpu_gridoperator;
    (convectiondiffusionparameter.hh:134-209), unit cube, 4^3 cells, k=1. */
 void pdelab_gpu_default_config(pdelab_config* cfg);
 
-/* Creates a grid operator on the current HIP device. kappa_host (nx*ny*nz doubles,
-   host memory) is read when cfg->kappa_per_cell != 0, else ignored. */
+/* Creates a grid operator on the current HIP device. kappa_host (host memory: nx*ny*nz doubles
+   for kappa_per_cell == 1, 3*nx*ny*nz doubles a_x, a_y, a_z cell by cell for kappa_per_cell == 2;
+   every value finite and > 0) is read when cfg->kappa_per_cell != 0, else ignored. */
 int pdelab_gpu_create(const pdelab_config* cfg, const double* kappa_host,
                       pdelab_gpu_gridoperator** out);
 int pdelab_gpu_destroy(pdelab_gpu_gridoperator* h);
@@ -238,9 +243,11 @@ size_t pdelab_gpu_num_cells(const pdelab_gpu_gridoperator* h);
 /* isLinear of the wrapped local operator (always 1 for ConvectionDiffusionDG,
    convectiondiffusiondg.hh inherits LocalOperatorDefaultFlags::isLinear = true). */
 int pdelab_gpu_is_linear(const pdelab_gpu_gridoperator* h);
-/* Replaces the per-cell kappa (host array of nx*ny*nz doubles). */
+/* Replaces the per-cell kappa (host array laid out as at create: nx*ny*nz doubles, or three per
+   cell for kappa_per_cell == 2). */
 int pdelab_gpu_set_kappa(pdelab_gpu_gridoperator* h, const double* kappa_host);
-/* Same from a device array of log(kappa) (kappa = exp(log_kappa), computed on the device). */
+/* Same from a device array of log(kappa) (kappa = exp(log_kappa), computed on the device);
+   A = kappa I only (PDELAB_ERR_UNSUPPORTED for kappa_per_cell == 2, as for the kappa gradient). */
 int pdelab_gpu_set_log_kappa_device(pdelab_gpu_gridoperator* h, const double* log_kappa_dev);
 /* Per-cell additive source s_c (f -> f + s_c on cell c); host array or NULL to clear. */
 int pdelab_gpu_set_cell_source(pdelab_gpu_gridoperator* h, const double* source_host);

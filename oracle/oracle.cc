@@ -33,8 +33,9 @@ struct Oracle {
 
   Oracle(const pdelab_config& c, const double* kappa)
       : cfg(c), grid(c), param(make_param(c)), tab(c.degree, 2 * c.degree + c.intorderadd), lop(grid, param, tab) {
-    param.kappa.assign(grid.ncells, c.kappa0);
-    if (c.kappa_per_cell && kappa) param.kappa.assign(kappa, kappa + grid.ncells);
+    param.aniso = c.kappa_per_cell == 2;
+    param.kappa.assign(grid.ncells * (param.aniso ? 3 : 1), c.kappa0);
+    if (c.kappa_per_cell && kappa) param.kappa.assign(kappa, kappa + grid.ncells * (param.aniso ? 3 : 1));
     nloc = tab.nloc;
     N = grid.ncells * nloc;
   }
@@ -616,7 +617,7 @@ int oracle_iterative_block_jacobi_apply(oracle_handle* h, const double* d, doubl
 int oracle_block_sor_apply(oracle_handle* h, const double* d, double* v);
 
 int oracle_set_kappa(oracle_handle* h, const double* kappa) {
-  h->o->param.kappa.assign(kappa, kappa + h->o->grid.ncells);
+  h->o->param.kappa.assign(kappa, kappa + h->o->grid.ncells * (h->o->param.aniso ? 3 : 1));
   h->diag.clear();
   h->blocks.clear();
   for (int t = 0; t < 2; ++t) { h->blu[t].clear(); h->bpiv[t].clear(); }

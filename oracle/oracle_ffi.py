@@ -161,7 +161,7 @@ class Oracle:
         kp = None
         if kappa is not None:
             self._kappa = np.ascontiguousarray(kappa, dtype=np.float64)
-            assert self._kappa.size == self.ncells
+            assert self._kappa.size == self.ncells * (3 if cfg.kappa_per_cell == 2 else 1)
             kp = _p(self._kappa)
         # the config may come from the product package's ctypes class: pass its address
         self.h = self._L.oracle_create(ctypes.addressof(cfg), kp)

@@ -177,12 +177,15 @@ struct Grid {
 };
 
 // ---------------------------------------------------------------------------
-// Parameter class: A = kappa*I constant per cell
+// Parameter class: A = kappa*I, or a diagonal A = diag(a_x, a_y, a_z), constant per cell
 // (permeabilityIsConstantPerCell, convectiondiffusionparameter.hh:28-62,142-157),
 // b and c constant, f / g / bctype per the problem selector of pdelab_config.
 struct Param {
   pdelab_config cfg;
-  std::vector<double> kappa;        // per cell
+  std::vector<double> kappa;        // per cell, or a_x, a_y, a_z per cell (aniso)
+  bool aniso = false;
+  // n.A n for the faces normal to direction d (and A's diagonal entry d)
+  double kappa_d(size_t c, int d) const { return aniso ? kappa[3 * c + d] : kappa[c]; }
   std::vector<double> cell_source;  // per cell, may be empty
   std::vector<double> f_tab;        // TABULATED: ncells * m^3
   std::vector<double> g_tab;        // TABULATED: boundary faces * m^2
@@ -398,7 +401,11 @@ struct LocalOperator {
     if (method == PDELAB_IIPG) theta = 0.0;
   }
 
-  RF kap(size_t c) const;  // A(cell, center) = kappa_c * I
+  // entry d of the diagonal tensor A(cell, center) (kappa_c for A = kappa_c I). The faces are
+  // normal to a coordinate direction d, so A n = a_d n there and n.A n = a_d: the reference's
+  // A_s.mv(n_F, An_F_s), delta_s = An_F_s . n_F and An_F_s . grad u (convectiondiffusiondg.hh:316-333,
+  // 418-424) reduce to the scalar a_d below.
+  RF kap(size_t c, int d) const;
 
   // alpha_volume, convectiondiffusiondg.hh:106-188 (also jacobian_apply_volume :192-195)
   void alpha_volume(size_t c, const double* x, RF* r) const {
@@ -407,7 +414,7 @@ struct LocalOperator {
     grid.cell_box(c, lo, ext);
     const double det = ext[0] * ext[1] * ext[2];
     const double jit[3] = {1.0 / ext[0], 1.0 / ext[1], 1.0 / ext[2]};
-    RF A = kap(c);
+    const RF A[3] = {kap(c, 0), kap(c, 1), kap(c, 2)};
     const double* b = param.cfg.b;
     const double cc = param.cfg.c;
     std::vector<double> gradphi(3 * nloc);
@@ -422,7 +429,7 @@ struct LocalOperator {
       double gradu[3] = {0, 0, 0};
       for (int i = 0; i < nloc; ++i)
         for (int d = 0; d < 3; ++d) gradu[d] += x[i] * gradphi[3 * i + d];
-      RF Agradu[3] = {A * gradu[0], A * gradu[1], A * gradu[2]};
+      RF Agradu[3] = {A[0] * gradu[0], A[1] * gradu[1], A[2] * gradu[2]};
       const double factor = tab.vw[q] * det;
       for (int i = 0; i < nloc; ++i) {
         const double* gp = &gradphi[3 * i];
@@ -440,7 +447,7 @@ struct LocalOperator {
     grid.cell_box(c, lo, ext);
     const double det = ext[0] * ext[1] * ext[2];
     const double jit[3] = {1.0 / ext[0], 1.0 / ext[1], 1.0 / ext[2]};
-    RF A = kap(c);
+    const RF A[3] = {kap(c, 0), kap(c, 1), kap(c, 2)};
     const double* b = param.cfg.b;
     const double cc = param.cfg.c;
     std::vector<double> gradphi(3 * nloc);
@@ -453,7 +460,7 @@ struct LocalOperator {
       const double factor = tab.vw[q] * det;
       for (int j = 0; j < nloc; ++j) {
         const double* gj = &gradphi[3 * j];
-        RF Agj[3] = {A * gj[0], A * gj[1], A * gj[2]};
+        RF Agj[3] = {A[0] * gj[0], A[1] * gj[1], A[2] * gj[2]};
         for (int i = 0; i < nloc; ++i) {
           const double* gi = &gradphi[3 * i];
           RF ag = Agj[0] * gi[0] + Agj[1] * gi[1] + Agj[2] * gi[2];
@@ -484,8 +491,8 @@ struct LocalOperator {
     fc.hF = std::min(vol_in, vol_out) / area;
     for (int e = 0; e < 3; ++e) fc.nF[e] = 0.0;
     fc.nF[d] = (fin % 2) ? 1.0 : -1.0;
-    fc.As = kap(cin);
-    fc.An = kap(cout);
+    fc.As = kap(cin, d);
+    fc.An = kap(cout, d);
     // A n, delta = n.A n (A = kappa I): :316-333
     RF An_s[3], An_n[3];
     for (int e = 0; e < 3; ++e) { An_s[e] = fc.As * fc.nF[e]; An_n[e] = fc.An * fc.nF[e]; }
@@ -642,7 +649,7 @@ struct LocalOperator {
     bc.hF = vol / area;  // :717
     for (int e = 0; e < 3; ++e) bc.nF[e] = 0.0;
     bc.nF[d] = (side % 2) ? 1.0 : -1.0;
-    bc.As = kap(c);
+    bc.As = kap(c, d);
     RF An[3] = {bc.As * bc.nF[0], bc.As * bc.nF[1], bc.As * bc.nF[2]};
     if (weights == PDELAB_WEIGHTS_ON)
       bc.harmonic = An[0] * bc.nF[0] + An[1] * bc.nF[1] + An[2] * bc.nF[2];
@@ -780,10 +787,10 @@ struct LocalOperator {
 };
 
 template <>
-inline double LocalOperator<double>::kap(size_t c) const { return param.kappa[c]; }
+inline double LocalOperator<double>::kap(size_t c, int d) const { return param.kappa_d(c, d); }
 template <>
-inline Dual LocalOperator<Dual>::kap(size_t c) const {
-  double k = param.kappa[c];
+inline Dual LocalOperator<Dual>::kap(size_t c, int d) const {
+  double k = param.kappa_d(c, d);
   return Dual(k, c == seed ? k : 0.0);  // d kappa / d log kappa = kappa
 }
 

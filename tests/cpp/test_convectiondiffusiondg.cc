@@ -37,6 +37,15 @@ struct LognormalProblem : ConvectionDiffusionModelProblem {
   double g(const Element&, const std::array<double, 3>&) const { return 0.0; }
 };
 
+// The same field as a diagonal diffusion tensor (PermTensorType); a full tensor is refused.
+struct TensorProblem : LognormalProblem {
+  bool full = false;
+  std::array<std::array<double, 3>, 3> A(const Element& e, const std::array<double, 3>&) const {
+    const double k = kappa[e.index];
+    return {{{k, full ? 0.1 : 0.0, 0.0}, {0.0, k, 0.0}, {0.0, 0.0, k}}};
+  }
+};
+
 int main() {
   int fails = 0;
   try {
@@ -173,6 +182,36 @@ int main() {
     double vn = 0;
     for (double e : vh) vn += e * e;
     if (!(vn > 0) || !std::isfinite(vn)) ++fails;
+
+    // --- the lognormal field given as the diagonal tensor diag(k, k, k): the same solve, bit for bit
+    {
+      TensorProblem tp;
+      tp.kappa = lp.kappa;
+      using LOPT = ConvectionDiffusionDG<TensorProblem, FEM3>;
+      LOPT lopt(tp, ConvectionDiffusionDGMethod::SIPG, ConvectionDiffusionDGWeights::weightsOn, 3.0);
+      using GOT = GridOperator<GridFunctionSpace<FEM3>, GridFunctionSpace<FEM3>, LOPT>;
+      GOT got(gfs3, gfs3, lopt);
+      auto xt = got.makeVector();
+      ISTLBackend_SEQ_MatrixFree_CG_Jac<GOT> lst(got, 20000, 0);
+      StationaryLinearProblemSolver<GOT, decltype(lst), GOT::Domain> st(got, lst, xt, 1e-12);
+      st.apply();
+      auto t = xt.download();
+      double dmax = 0;
+      for (size_t i = 0; i < t.size(); ++i) dmax = std::max(dmax, std::fabs(t[i] - b[i]));
+      std::printf("diagonal tensor vs scalar kappa: max difference %g, iterations %d and %d\n", dmax,
+                  (int)lst.result().iterations, (int)ls3.result().iterations);
+      if (dmax != 0.0 || lst.result().iterations != ls3.result().iterations) ++fails;
+      tp.full = true;
+      LOPT lopf(tp, ConvectionDiffusionDGMethod::SIPG, ConvectionDiffusionDGWeights::weightsOn, 3.0);
+      bool refused = false;
+      try {
+        GOT gof(gfs3, gfs3, lopf);
+      } catch (const NotImplemented&) {
+        refused = true;
+      }
+      std::printf("full tensor refused: %d\n", (int)refused);
+      if (!refused) ++fails;
+    }
   } catch (const std::exception& e) {
     std::fprintf(stderr, "error: %s\n", e.what());
     return 1;

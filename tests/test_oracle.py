@@ -191,6 +191,63 @@ def test_polynomial_reproduction_neumann_outflow(of, k):
     assert np.abs(x - u).max() < 1e-8
 
 
+def _aniso_flux_continuous(nx, ny, nz, seed):
+    """A = diag(a_x, a_y, a_z) per cell with a_x depending on the cell's (y, z) index only, a_y
+    on (x, z) and a_z on (x, y): the normal flux a_d du/dx_d of a separable u is then continuous
+    across every face, so the interpolant of a quadratic u solves the discrete problem."""
+    rng = np.random.default_rng(seed)
+    ax = np.exp(rng.normal(size=(nz, ny)))
+    ay = np.exp(rng.normal(size=(nz, nx)))
+    az = np.exp(rng.normal(size=(ny, nx)))
+    a = np.zeros((nz, ny, nx, 3))
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                a[k, j, i] = (ax[k, j], ay[k, i], az[j, i])
+    return a.reshape(-1, 3)
+
+
+@pytest.mark.parametrize("k,method", [(2, 1), (3, 1), (2, 0)], ids=["q2_sipg", "q3_sipg", "q2_nipg"])
+def test_anisotropic_polynomial_reproduction(of, k, method):
+    """A cell-wise diagonal tensor A = diag(a_x, a_y, a_z) (kappa_per_cell = 2): the interpolant of
+    u = q0 + q1 x + q2 y + q3 z + q4 x^2 + q5 y^2 + q6 z^2 has zero residual when
+    f = -sum_d a_d 2 q_{4+d} + b.grad u + c u (the built-in quadratic f with kappa0 = 1, plus a
+    per-cell source for the difference) and a_d is continuous across the faces normal to d.
+    A coefficient taken from the wrong direction in the volume, face or boundary terms leaves a
+    residual of order one (the three a_d differ by factors of e)."""
+    q = (0.3, 1.0, 2.0, -1.0, 1.0, 1.5, -2.0)
+    nx, ny, nz = 3, 2, 4
+    a = _aniso_flux_continuous(nx, ny, nz, 17)
+    cfg = of.cfgmod.make_config(nx=nx, ny=ny, nz=nz, degree=k, method=method, alpha=4.0, kappa_per_cell=2,
+                                problem=of.cfgmod.PROBLEM_QUADRATIC, quad=q, b=(0.5, -0.25, 0.75), c=0.3)
+    o = of.Oracle(cfg, a.ravel())
+    src = -2.0 * ((a - 1.0) @ np.array(q[4:]))
+    o.set_cell_source(src)
+    u = o.interpolate()
+    r = o.residual(u)
+    assert np.abs(r).max() < 1e-12
+    # the same with the directions of A permuted is no solution
+    o2 = of.Oracle(cfg, a[:, [1, 2, 0]].ravel())
+    o2.set_cell_source(src)
+    assert np.abs(o2.residual(u)).max() > 1e-3
+
+
+def test_anisotropic_equals_isotropic_and_sipg_symmetric(of):
+    """A = diag(kappa, kappa, kappa) is the same operator as A = kappa I (residual and Jacobian);
+    SIPG with a cell-wise diagonal A is symmetric positive definite."""
+    kw = dict(nx=3, ny=2, nz=2, degree=2, alpha=3.0, b=(0.3, 0.0, -0.2), problem=of.cfgmod.PROBLEM_EXP)
+    kappa = np.exp(np.random.default_rng(5).normal(size=12))
+    o1 = of.Oracle(of.cfgmod.make_config(kappa_per_cell=True, **kw), kappa)
+    o3 = of.Oracle(of.cfgmod.make_config(kappa_per_cell=2, **kw), np.repeat(kappa, 3))
+    x = np.random.default_rng(6).normal(size=o1.N)
+    assert np.abs(o3.residual(x) - o1.residual(x)).max() <= 1e-14 * np.abs(o1.residual(x)).max()
+    assert np.abs(o3.dense_matrix() - o1.dense_matrix()).max() == 0.0
+    a = np.exp(np.random.default_rng(7).normal(size=(12, 3)))
+    A = of.Oracle(of.cfgmod.make_config(nx=3, ny=2, nz=2, degree=2, alpha=3.0, kappa_per_cell=2), a.ravel()).dense_matrix()
+    assert np.abs(A - A.T).max() <= 1e-14 * np.abs(A).max()
+    assert np.linalg.eigvalsh(0.5 * (A + A.T)).min() > 0
+
+
 def test_neumann_outflow_constant_data(of):
     """Constant data j0 / o0: R(x) - R_0(x) is int j0 psi over the Neumann faces plus int o0 psi
     over the outflow faces, i.e. j0 / o0 times the face integrals of the basis functions."""
