@@ -370,12 +370,12 @@ class GridOperator:
             self._check(lib().pdelab_gpu_block_jacobi_apply(self.h, zp, yp))
 
     def block_sor_apply(self, d, v):
-        """v = W^{-1} d, one forward block-SOR sweep from v = 0 (FP64; v is overwritten)."""
+        """One block-SOR step on v in place (FP64); on v = 0 the preconditioner v = W^{-1} d."""
         self._check(lib().pdelab_gpu_block_sor_apply(self.h, self._vec(d), self._vec(v)))
 
     def iterative_block_jacobi_apply(self, d, v):
-        """v = W^{-1} d with per-cell BiCGSTAB block solves (FP64; v is overwritten). Returns the
-        block solvers' iteration count summed over the cells."""
+        """v += W^{-1} d with per-cell BiCGSTAB block solves (FP64). Returns the block solvers'
+        iteration count summed over the cells."""
         its = ctypes.c_ulonglong(0)
         self._check(lib().pdelab_gpu_iterative_block_jacobi_apply(self.h, self._vec(d), self._vec(v), ctypes.byref(its)))
         return its.value

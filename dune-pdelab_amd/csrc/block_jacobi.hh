@@ -861,7 +861,8 @@ __device__ __forceinline__ T ibj_wave_sum(T v) {
 // D: the diagonal blocks, row-major (entry e at e NL^2), e the cell or its boundary type.
 // err: the first failing block's code is kept (atomicCAS from 0); iters: sum of the block
 // solvers' iteration counts (half steps rounded up), when non-null.
-template <int N1, typename T>
+// ADD: out += y (GridOperatorPreconditioner::apply accumulates); else out = y.
+template <int N1, typename T, bool ADD = false>
 __global__ __launch_bounds__(64) void bj_iter_kernel(const T* __restrict__ in, T* __restrict__ out,
                                                      const T* __restrict__ D, const BJGeom g, const IBJOptions opt,
                                                      int* __restrict__ err, unsigned long long* __restrict__ iters,
@@ -940,7 +941,7 @@ __global__ __launch_bounds__(64) void bj_iter_kernel(const T* __restrict__ in, T
       if (converged(norm)) break;
     }
   }
-  if (act) out[c * NL + l] = x;
+  if (act) out[c * NL + l] = ADD ? out[c * NL + l] + x : x;
   if (l == 0) {
     if (code != IBJ_OK) atomicCAS(err, 0, code);
     if (iters) atomicAdd(iters, (unsigned long long)ceil(it_last));

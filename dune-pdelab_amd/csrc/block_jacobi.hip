@@ -360,7 +360,7 @@ int bj_block_diagonal(H* h, bool trans, double* blocks) {
 
 namespace {
 
-template <int N1, typename T, bool CONV, bool VARK>
+template <int N1, typename T, bool CONV, bool VARK, bool INPLACE>
 int sor_waves(H* h, const T* d, T* v, const T* dinv, const int* stop) {
   using S = SorShape<N1>;
   const BJGeom g = geom(h, h->bj[2].per_cell);
@@ -371,17 +371,25 @@ int sor_waves(H* h, const T* d, T* v, const T* dinv, const int* stop) {
   for (int s = 0; s < ns; ++s) {
     const int first = h->sor_first[s], count = h->sor_first[s + 1] - first;
     const unsigned nb = (unsigned)((count + S::CPW - 1) / S::CPW);
-    hipLaunchKernelGGL((block_sor_wave_kernel<N1, T, CONV, VARK>), dim3(nb), dim3(64), 0, h->stream, d, v, fc, a, lm,
+    hipLaunchKernelGGL((block_sor_wave_kernel<N1, T, CONV, VARK, INPLACE>), dim3(nb), dim3(64), 0, h->stream, d, v, fc, a, lm,
                        dinv, g, (const int*)h->d_sor_order, first, count, (T)h->sor_omega, stop);
   }
   HIPCHK(h, hipGetLastError());
   return PDELAB_OK;
 }
 
+template <int N1, typename T, bool INPLACE>
+int sor_i(H* h, const T* d, T* v, const T* dinv, const int* stop) {
+  if (h->vark)
+    return h->conv ? sor_waves<N1, T, true, true, INPLACE>(h, d, v, dinv, stop)
+                   : sor_waves<N1, T, false, true, INPLACE>(h, d, v, dinv, stop);
+  return h->conv ? sor_waves<N1, T, true, false, INPLACE>(h, d, v, dinv, stop)
+                 : sor_waves<N1, T, false, false, INPLACE>(h, d, v, dinv, stop);
+}
+
 template <int N1, typename T>
-int sor_n(H* h, const T* d, T* v, const T* dinv, const int* stop) {
-  if (h->vark) return h->conv ? sor_waves<N1, T, true, true>(h, d, v, dinv, stop) : sor_waves<N1, T, false, true>(h, d, v, dinv, stop);
-  return h->conv ? sor_waves<N1, T, true, false>(h, d, v, dinv, stop) : sor_waves<N1, T, false, false>(h, d, v, dinv, stop);
+int sor_n(H* h, const T* d, T* v, const T* dinv, const int* stop, bool inplace) {
+  return inplace ? sor_i<N1, T, true>(h, d, v, dinv, stop) : sor_i<N1, T, false>(h, d, v, dinv, stop);
 }
 
 }  // namespace
@@ -409,18 +417,18 @@ int sor_prepare(H* h, int precision) {
 }
 
 template <typename T>
-int sor_launch(H* h, const T* d, T* v, const int* stop) {
+int sor_launch(H* h, const T* d, T* v, const int* stop, bool inplace) {
   CHK(sor_prepare(h, sizeof(T) == 8 ? 64 : 32));
   const T* dinv = bj_data<T>(h->bj[2]);
   switch (h->n1) {
-    case 2: return sor_n<2, T>(h, d, v, dinv, stop);
-    case 3: return sor_n<3, T>(h, d, v, dinv, stop);
-    default: return sor_n<4, T>(h, d, v, dinv, stop);
+    case 2: return sor_n<2, T>(h, d, v, dinv, stop, inplace);
+    case 3: return sor_n<3, T>(h, d, v, dinv, stop, inplace);
+    default: return sor_n<4, T>(h, d, v, dinv, stop, inplace);
   }
 }
 
-template int sor_launch<double>(H*, const double*, double*, const int*);
-template int sor_launch<float>(H*, const float*, float*, const int*);
+template int sor_launch<double>(H*, const double*, double*, const int*, bool);
+template int sor_launch<float>(H*, const float*, float*, const int*, bool);
 
 // ---------------------------------------------------------------------------
 // Iterative block Jacobi (block_jacobi.hh, bj_iter_kernel): the diagonal blocks D_c of J,
@@ -467,21 +475,30 @@ int ibj_prepare(H* h, int precision) {
   return PDELAB_OK;
 }
 
+template <int N1, typename T>
+static void ibj_kernel(H* h, const T* d, T* v, const T* D, const BJGeom& g, int* err, unsigned long long* iters,
+                       const int* stop, bool add) {
+  const dim3 grid((unsigned)h->ncells), block(64);
+  if (add)
+    hipLaunchKernelGGL((bj_iter_kernel<N1, T, true>), grid, block, 0, h->stream, d, v, D, g, h->ibj_opt, err, iters, stop);
+  else
+    hipLaunchKernelGGL((bj_iter_kernel<N1, T, false>), grid, block, 0, h->stream, d, v, D, g, h->ibj_opt, err, iters, stop);
+}
+
 template <typename T>
-int ibj_launch(H* h, const T* d, T* v, int* err, unsigned long long* iters, const int* stop) {
+int ibj_launch(H* h, const T* d, T* v, int* err, unsigned long long* iters, const int* stop, bool add) {
   CHK(ibj_prepare(h, sizeof(T) == 8 ? 64 : 32));
   const BlockJacobiData& b = h->bj[3];
   const BJGeom g = geom(h, b.per_cell);
   const T* D = bj_data<T>(b);
-  const dim3 grid((unsigned)h->ncells), block(64);
   switch (h->n1) {
-    case 2: hipLaunchKernelGGL((bj_iter_kernel<2, T>), grid, block, 0, h->stream, d, v, D, g, h->ibj_opt, err, iters, stop); break;
-    case 3: hipLaunchKernelGGL((bj_iter_kernel<3, T>), grid, block, 0, h->stream, d, v, D, g, h->ibj_opt, err, iters, stop); break;
-    default: hipLaunchKernelGGL((bj_iter_kernel<4, T>), grid, block, 0, h->stream, d, v, D, g, h->ibj_opt, err, iters, stop); break;
+    case 2: ibj_kernel<2, T>(h, d, v, D, g, err, iters, stop, add); break;
+    case 3: ibj_kernel<3, T>(h, d, v, D, g, err, iters, stop, add); break;
+    default: ibj_kernel<4, T>(h, d, v, D, g, err, iters, stop, add); break;
   }
   HIPCHK(h, hipGetLastError());
   return PDELAB_OK;
 }
 
-template int ibj_launch<double>(H*, const double*, double*, int*, unsigned long long*, const int*);
-template int ibj_launch<float>(H*, const float*, float*, int*, unsigned long long*, const int*);
+template int ibj_launch<double>(H*, const double*, double*, int*, unsigned long long*, const int*, bool);
+template int ibj_launch<float>(H*, const float*, float*, int*, unsigned long long*, const int*, bool);

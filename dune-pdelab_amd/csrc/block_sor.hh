@@ -42,7 +42,10 @@ struct SorShape {
 // at e NL^2 + j NL + i), e the cell (per_cell) or its boundary type. Each lane issues its
 // row of the block together with the neighbour loads, and the stop flag is read in the same
 // batch, so a wave waits for memory twice (the cell index, then everything else).
-template <int N1, typename T, bool CONV, bool VARK>
+// INPLACE: one SOR step from the v given (GridOperatorPreconditioner::apply on any v): the upper
+// neighbours' lines (not yet updated: they lie on the next hyperplane) enter the coupling too,
+// and v_c = (1 - omega) v_c + omega D_c^{-1} r; else the sweep from v = 0 that the solvers use.
+template <int N1, typename T, bool CONV, bool VARK, bool INPLACE = false>
 __global__ __launch_bounds__(64) void block_sor_wave_kernel(const T* __restrict__ d, T* __restrict__ v,
                                                             const T* __restrict__ fc, const DGArgs<T> a,
                                                             const LineMats<T>* __restrict__ lm,
@@ -70,11 +73,16 @@ __global__ __launch_bounds__(64) void block_sor_wave_kernel(const T* __restrict_
   for (int dd = 0; dd < 3; ++dd) lo[dd] = coord[dd] > 0;
   // ---- all global loads
   T dz[N1], ux[N1], uy[N1], uz[N1], drow[NL];
+  T wx[N1], wy[N1], wz[N1];  // INPLACE: the upper neighbours' lines
+  bool hi[3];
+#pragma unroll
+  for (int dd = 0; dd < 3; ++dd) hi[dd] = INPLACE && coord[dd] < a.n[dd] - 1;
   T kc[3] = {a.kappa0, a.kappa0, a.kappa0};
-  FaceCoef<T> flo[3];
+  FaceCoef<T> flo[3], fhi[3];
   const FaceCoef<T> fnone{a.wI_lo, a.wI_hi, a.PI};
 #pragma unroll
-  for (int dd = 0; dd < 3; ++dd) flo[dd] = fnone;
+  for (int dd = 0; dd < 3; ++dd) flo[dd] = fhi[dd] = fnone;
+  T vold = T(0);
   if (line_lane) {
     const T* vc = v + c * NL;
     const T* dc = d + c * NL;
@@ -84,6 +92,9 @@ __global__ __launch_bounds__(64) void block_sor_wave_kernel(const T* __restrict_
       ux[e] = lo[0] ? vc[line_index<N1, 0>(p, q, e) - a.stride[0] * NL] : T(0);
       uy[e] = lo[1] ? vc[line_index<N1, 1>(p, q, e) - a.stride[1] * NL] : T(0);
       uz[e] = lo[2] ? vc[line_index<N1, 2>(p, q, e) - a.stride[2] * NL] : T(0);
+      wx[e] = hi[0] ? vc[line_index<N1, 0>(p, q, e) + a.stride[0] * NL] : T(0);
+      wy[e] = hi[1] ? vc[line_index<N1, 1>(p, q, e) + a.stride[1] * NL] : T(0);
+      wz[e] = hi[2] ? vc[line_index<N1, 2>(p, q, e) + a.stride[2] * NL] : T(0);
     }
     if (VARK) {
       const T* fb = fc + c * FC;
@@ -91,6 +102,10 @@ __global__ __launch_bounds__(64) void block_sor_wave_kernel(const T* __restrict_
       for (int dd = 0; dd < 3; ++dd) {
         kc[dd] = fb[dd];
         if (lo[dd]) flo[dd] = FaceCoef<T>{fb[FCF + 3 * dd], fb[FCF + 1 + 3 * dd], fb[FCF + 2 + 3 * dd]};
+        if (hi[dd]) {
+          const T* fn = fb + a.stride[dd] * FC;
+          fhi[dd] = FaceCoef<T>{fn[FCF + 3 * dd], fn[FCF + 1 + 3 * dd], fn[FCF + 2 + 3 * dd]};
+        }
       }
     }
   }
@@ -101,16 +116,20 @@ __global__ __launch_bounds__(64) void block_sor_wave_kernel(const T* __restrict_
     const T* DT = dinv + e * NL * NL + l;
 #pragma unroll
     for (int j = 0; j < NL; ++j) drow[j] = DT[j * NL];
+    if (INPLACE) vold = v[c * NL + l];
   }
   const bool stopped = stop && *stop;  // a device-side Krylov loop has already stopped
-  // ---- line terms of the lower neighbours (own and upper values zero)
+  // ---- line terms of the neighbours (own values zero; upper values zero unless INPLACE)
   if (line_lane) {
     T zero[N1], sx[N1], sy[N1], sz[N1];
 #pragma unroll
     for (int e = 0; e < N1; ++e) zero[e] = sx[e] = sy[e] = sz[e] = T(0);
-    if (lo[0]) line_apply<N1, 0, T, false, CONV, false, VARK>(a, lm, zero, ux, zero, true, false, kc[0], flo[0], fnone, sx);
-    if (lo[1]) line_apply<N1, 1, T, false, CONV, false, VARK>(a, lm, zero, uy, zero, true, false, kc[1], flo[1], fnone, sy);
-    if (lo[2]) line_apply<N1, 2, T, false, CONV, false, VARK>(a, lm, zero, uz, zero, true, false, kc[2], flo[2], fnone, sz);
+    if (lo[0] || hi[0])
+      line_apply<N1, 0, T, false, CONV, false, VARK>(a, lm, zero, ux, wx, lo[0], hi[0], kc[0], flo[0], fhi[0], sx);
+    if (lo[1] || hi[1])
+      line_apply<N1, 1, T, false, CONV, false, VARK>(a, lm, zero, uy, wy, lo[1], hi[1], kc[1], flo[1], fhi[1], sy);
+    if (lo[2] || hi[2])
+      line_apply<N1, 2, T, false, CONV, false, VARK>(a, lm, zero, uz, wz, lo[2], hi[2], kc[2], flo[2], fhi[2], sz);
     // the three directions summed in the y layout through the rotated cell slots
 #pragma unroll
     for (int e = 0; e < N1; ++e) {
@@ -168,7 +187,7 @@ __global__ __launch_bounds__(64) void block_sor_wave_kernel(const T* __restrict_
     T res = T(0);
 #pragma unroll
     for (int j = 0; j < NL; ++j) res += drow[j] * b1[j];
-    v[c * NL + l] = omega * res;
+    v[c * NL + l] = INPLACE ? (T(1) - omega) * vold + omega * res : omega * res;
   }
 }
 

@@ -204,17 +204,18 @@ void bj_free(H* h);
 template <typename T, typename A>
 int bj_launch(H* h, bool trans, int mode, BJVec<T, A> v, int* nparts);
 int bj_block_diagonal(H* h, bool trans, double* blocks);
-// Block SOR (block_sor.hh): data set-up (dense blocks of J) and one sweep v = W^{-1} d;
-// the kernels store nothing when *stop is set.
+// Block SOR (block_sor.hh): data set-up (dense blocks of J) and one sweep, from v = 0
+// (v = W^{-1} d, the solvers) or in place from the v given (inplace); the kernels store
+// nothing when *stop is set.
 int sor_prepare(H* h, int precision);
 template <typename T>
-int sor_launch(H* h, const T* d, T* v, const int* stop);
+int sor_launch(H* h, const T* d, T* v, const int* stop, bool inplace);
 // Iterative block Jacobi (bj_iter_kernel): the diagonal blocks of J, and one application
-// v = W^{-1} d; the block solvers' error code goes to *err (kept at the first nonzero), their
-// iteration sum is added to *iters when non-null.
+// v = W^{-1} d (v += W^{-1} d with add); the block solvers' error code goes to *err (kept at
+// the first nonzero), their iteration sum is added to *iters when non-null.
 int ibj_prepare(H* h, int precision);
 template <typename T>
-int ibj_launch(H* h, const T* d, T* v, int* err, unsigned long long* iters, const int* stop);
+int ibj_launch(H* h, const T* d, T* v, int* err, unsigned long long* iters, const int* stop, bool add);
 // Assembled Jacobian in the 7-point block pattern (jacobian.hh) and its apply y = A z.
 int jac_assemble(H* h, bool trans, double* blocks, long long* cols);
 int jac_block_apply(H* h, const double* blocks, const long long* cols, const double* z, double* y);

@@ -745,8 +745,8 @@ static int bicgstab_solve(H* h, T* x, T* r, double reduction, int maxit, int pre
     const dim3 rg(kRedBlocks), rb(kRedThreads), fg(1), fb(kFinalThreads);
     if (P.ibj()) HIPCHK(h, hipMemsetAsync(h->d_ibj_stat, 0, 16, h->stream));
     auto prec = [&](const T* in, T* out) -> int {
-      if (P.sor()) return sor_launch<T>(h, in, out, st);
-      if (P.ibj()) return ibj_launch<T>(h, in, out, static_cast<int*>(h->d_ibj_stat), nullptr, st);
+      if (P.sor()) return sor_launch<T>(h, in, out, st, false);
+      if (P.ibj()) return ibj_launch<T>(h, in, out, static_cast<int*>(h->d_ibj_stat), nullptr, st, false);
       if (P.block()) {
         const BJVec<T, A> bv{in, out, nullptr, nullptr, nullptr, nullptr, st};
         return bj_launch<T, A>(h, P.trans, BJ_PREC, bv, nullptr);
@@ -1385,7 +1385,7 @@ int pdelab_gpu_block_sor_apply(pdelab_gpu_gridoperator* h, const double* d, doub
   if (!h || !d || !v || d == v) return PDELAB_ERR_INVALID_ARG;
   CHK(check_outflow(h));
   sync_prec_cache(h);
-  return sor_launch<double>(h, d, v, nullptr);
+  return sor_launch<double>(h, d, v, nullptr, true);
 }
 
 int pdelab_gpu_iterative_block_jacobi_apply(pdelab_gpu_gridoperator* h, const double* d, double* v,
@@ -1396,7 +1396,7 @@ int pdelab_gpu_iterative_block_jacobi_apply(pdelab_gpu_gridoperator* h, const do
   CHK(ibj_prepare(h, 64));
   HIPCHK(h, hipMemsetAsync(h->d_ibj_stat, 0, 16, h->stream));
   CHK(ibj_launch<double>(h, d, v, static_cast<int*>(h->d_ibj_stat),
-                         reinterpret_cast<unsigned long long*>(static_cast<char*>(h->d_ibj_stat) + 8), nullptr));
+                         reinterpret_cast<unsigned long long*>(static_cast<char*>(h->d_ibj_stat) + 8), nullptr, true));
   struct {
     int err, pad;
     unsigned long long its;

@@ -499,10 +499,29 @@ class OnTheFlyOperator {
 
 enum class Krylov { CG = PDELAB_SOLVER_CG, BiCGSTAB = PDELAB_SOLVER_BICGSTAB };
 
-// Preconditioner of the device Krylov backends: point Jacobi (Blocking::none) or per-cell
+// Preconditioner of the device Krylov backends: point Jacobi (Blocking::none), per-cell
 // block Jacobi with exact block inverses (Blocking::fixed SeqJac; the matrix-free
-// AssembledBlockJacobiPreconditionerLocalOperator).
-enum class Preconditioner { Jacobi = PDELAB_PREC_JACOBI, BlockJacobi = PDELAB_PREC_BLOCK_JACOBI };
+// AssembledBlockJacobiPreconditionerLocalOperator), one block-SOR sweep
+// (BlockSORPreconditionerLocalOperator) or block Jacobi with iteratively solved blocks
+// (IterativeBlockJacobiPreconditionerLocalOperator); the last two with BiCGSTAB only.
+enum class Preconditioner {
+  Jacobi = PDELAB_PREC_JACOBI,
+  BlockJacobi = PDELAB_PREC_BLOCK_JACOBI,
+  BlockSOR = PDELAB_PREC_BLOCK_SOR,
+  IterativeBlockJacobi = PDELAB_PREC_ITERATIVE_BLOCK_JACOBI
+};
+
+// BlockSolverOptions, R:dune/pdelab/backend/istl/matrixfree/iterativeblockjacobipreconditioner.hh:198-231
+struct BlockSolverOptions {
+  BlockSolverOptions(const double resreduction = 1e-5, const size_t maxiter = 100, const bool precondition = true,
+                     const double weight = 1.0, const int verbose = 0)
+      : _resreduction(resreduction), _maxiter(maxiter), _precondition(precondition), _weight(weight), _verbose(verbose) {}
+  double _resreduction;
+  size_t _maxiter;
+  bool _precondition;
+  double _weight;
+  int _verbose;
+};
 
 // GridOperatorPreconditioner over a grid operator of
 // AssembledBlockJacobiPreconditionerLocalOperator<BlockDiagonalLocalOperatorWrapper<LOP>>
@@ -519,6 +538,59 @@ class AssembledBlockJacobiPreconditioner {
   explicit AssembledBlockJacobiPreconditioner(const GO& go) : go_(go) {}
   void pre(V&, W&) {}
   void apply(V& v, const W& d) { check(pdelab_gpu_block_jacobi_apply(go_.handle(), d.data(), v.data()), go_.handle()); }
+  void post(V&) {}
+
+ private:
+  const GO& go_;
+};
+
+// GridOperatorPreconditioner over IterativeBlockJacobiPreconditionerLocalOperator
+// (iterativeblockjacobipreconditioner.hh:267-450): apply(v, d) adds y with D_c y_c = d_c solved per
+// cell by BiCGSTAB to the block-solver options. The options are the grid operator's: they also
+// apply to its solves with Preconditioner::IterativeBlockJacobi. iterations() is the block
+// solvers' iteration count of the last apply, summed over the cells (SolverStatistics).
+template <class GO>
+class IterativeBlockJacobiPreconditioner {
+  using V = typename GO::Traits::Domain;
+  using W = typename GO::Traits::Range;
+
+ public:
+  static constexpr bool isLinear = true;
+  explicit IterativeBlockJacobiPreconditioner(const GO& go, BlockSolverOptions options = BlockSolverOptions())
+      : go_(go) {
+    check(pdelab_gpu_set_block_solver_options(go_.handle(), options._resreduction, (int)options._maxiter,
+                                              options._precondition ? 1 : 0, options._weight),
+          go_.handle());
+  }
+  void pre(V&, W&) {}
+  void apply(V& v, const W& d) {
+    check(pdelab_gpu_iterative_block_jacobi_apply(go_.handle(), d.data(), v.data(), &iterations_), go_.handle());
+  }
+  void post(V&) {}
+  unsigned long long iterations() const { return iterations_; }
+
+ private:
+  const GO& go_;
+  unsigned long long iterations_ = 0;
+};
+
+// GridOperatorPreconditioner over BlockSORPreconditionerLocalOperator
+// (blocksorpreconditioner.hh:38-296) with exact block solves: apply(v, d) is one SOR step on v in
+// place, v_c = (1 - omega) v_c + omega D_c^{-1} (d_c - sum_{j != c} A_cj v_j) with the cells in index
+// order (the preconditioner of BiCGSTAB on its zeroed v). omega is the grid operator's: it also
+// applies to its solves with Preconditioner::BlockSOR.
+template <class GO>
+class BlockSORPreconditioner {
+  using V = typename GO::Traits::Domain;
+  using W = typename GO::Traits::Range;
+
+ public:
+  static constexpr bool isLinear = true;
+  explicit BlockSORPreconditioner(const GO& go, const double omega = 1.0) : go_(go) {
+    check(pdelab_gpu_set_sor_relaxation(go_.handle(), omega), go_.handle());
+  }
+  void pre(V&, W&) {}
+  void apply(V& v, const W& d) { check(pdelab_gpu_block_sor_apply(go_.handle(), d.data(), v.data()), go_.handle()); }
   void post(V&) {}
 
  private:
@@ -565,6 +637,14 @@ using ISTLBackend_SEQ_MatrixFree_CG_BlockJac = ISTLBackend_SEQ_MatrixFree_Jac<GO
 template <class GO>
 using ISTLBackend_SEQ_MatrixFree_BCGS_BlockJac =
     ISTLBackend_SEQ_MatrixFree_Jac<GO, Krylov::BiCGSTAB, Preconditioner::BlockJacobi>;
+// BiCGSTAB with the matrix-free block SOR and iterative block Jacobi preconditioners
+// (matrix_free_linear.cc:185-250, 295-330: ISTLBackend_SEQ_MatrixFree_Base<GO, PrecGO, BiCGSTABSolver>)
+template <class GO>
+using ISTLBackend_SEQ_MatrixFree_BCGS_BlockSOR =
+    ISTLBackend_SEQ_MatrixFree_Jac<GO, Krylov::BiCGSTAB, Preconditioner::BlockSOR>;
+template <class GO>
+using ISTLBackend_SEQ_MatrixFree_BCGS_IterativeBlockJac =
+    ISTLBackend_SEQ_MatrixFree_Jac<GO, Krylov::BiCGSTAB, Preconditioner::IterativeBlockJacobi>;
 
 // Mixed precision backend: FP32 inner Krylov inside FP64 refinement (CG: reliable
 // updates, inner_reduction = FP32 residual reduction between FP64 residual replacements;

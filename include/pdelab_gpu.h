@@ -306,17 +306,19 @@ int pdelab_gpu_block_matrix_apply(pdelab_gpu_gridoperator* h, const double* bloc
 int pdelab_gpu_block_jacobi_apply(pdelab_gpu_gridoperator* h, const double* z, double* y);
 /* FP32 form of pdelab_gpu_block_jacobi_apply (inverse blocks rounded to FP32). */
 int pdelab_gpu_block_jacobi_apply_f32(pdelab_gpu_gridoperator* h, const float* z, float* y);
-/* v = W^{-1} d for the block-SOR preconditioner W (PDELAB_PREC_BLOCK_SOR): one forward sweep
-   from v = 0, as BiCGSTAB applies it (GridOperatorPreconditioner::apply over
-   BlockSORPreconditionerLocalOperator, R:dune/pdelab/backend/istl/matrixfree/
-   gridoperatorpreconditioner.hh:78-84, blocksorpreconditioner.hh:221-250). v is overwritten
-   (not accumulated into) and must not alias d. One launch per hyperplane x + y + z of the
-   grid; the dense inverse blocks of J are built once (and again after the next kappa change). */
+/* One block-SOR step on v in place, GridOperatorPreconditioner::apply(v, d) over
+   BlockSORPreconditionerLocalOperator (R:dune/pdelab/backend/istl/matrixfree/
+   gridoperatorpreconditioner.hh:78-84, blocksorpreconditioner.hh:221-250): cells in index
+   order, v_c = (1 - omega) v_c + omega D_c^{-1} (d_c - sum_{j<c} A_cj v_j(new) -
+   sum_{j>c} A_cj v_j(old)). On v = 0 this is the preconditioner W^{-1} d that BiCGSTAB applies
+   with PDELAB_PREC_BLOCK_SOR (dune-istl zeroes v before apply). v must not alias d. One launch
+   per hyperplane x + y + z of the grid; the dense inverse blocks of J are built once (and again
+   after the next kappa change). */
 int pdelab_gpu_block_sor_apply(pdelab_gpu_gridoperator* h, const double* d, double* v);
-/* v = W^{-1} d for the iterative block Jacobi preconditioner W (PDELAB_PREC_ITERATIVE_BLOCK_JACOBI):
-   per cell BiCGSTAB on D_c v_c = d_c from v_c = 0 (GridOperatorPreconditioner::apply over
-   IterativeBlockJacobiPreconditionerLocalOperator, R:dune/pdelab/backend/istl/matrixfree/
-   iterativeblockjacobipreconditioner.hh:355-390). v is overwritten and must not alias d.
+/* v += W^{-1} d for the iterative block Jacobi preconditioner W (PDELAB_PREC_ITERATIVE_BLOCK_JACOBI):
+   per cell BiCGSTAB on D_c y_c = d_c from y_c = 0, added to v_c (GridOperatorPreconditioner::apply
+   over IterativeBlockJacobiPreconditionerLocalOperator, R:dune/pdelab/backend/istl/matrixfree/
+   iterativeblockjacobipreconditioner.hh:355-390, which adds y_tmp to y). v must not alias d.
    *iterations (when non-null) receives the block solvers' iteration count summed over the cells
    (SolverStatistics, solverstatistics.hh). A breakdown of a block solve returns
    PDELAB_ERR_SOLVER_ABORT, as the ISTL solver throws. */

@@ -614,7 +614,7 @@ void oracle_set_block_solver(oracle_handle* h, double reduction, int maxit, int 
 }
 unsigned long long oracle_block_solver_iterations(oracle_handle* h) { return h->ibj_iterations; }
 int oracle_iterative_block_jacobi_apply(oracle_handle* h, const double* d, double* v);
-int oracle_block_sor_apply(oracle_handle* h, const double* d, double* v);
+int oracle_block_sor_apply(oracle_handle* h, const double* d, double* v, int inplace);
 
 int oracle_set_kappa(oracle_handle* h, const double* kappa) {
   h->o->param.kappa.assign(kappa, kappa + h->o->grid.ncells * (h->o->param.aniso ? 3 : 1));
@@ -713,21 +713,21 @@ static void sor_prepare(oracle_handle* h) {
   block_lu_prepare(h, 0);
 }
 
-// One forward block-SOR sweep from v = 0, R:dune/pdelab/backend/istl/matrixfree/
-// blocksorpreconditioner.hh:19-27 (the update a_i = d_i - sum_{j<i} A_ij v_j^(k) -
-// sum_{j>i} A_ij v_j^(k-1), D_i b_i = a_i, v_i = (1 - omega) v_i + omega b_i; the cell
-// update at :221-250) with v^(k-1) = 0: GridOperatorPreconditioner::apply
-// (gridoperatorpreconditioner.hh:78-84) on the zeroed v of dune-istl's BiCGSTAB. Cells in
-// index order; only the lower-index face neighbours contribute.
-static void sor_sweep(oracle_handle* h, const double* d, double* v) {
+// One forward block-SOR step, R:dune/pdelab/backend/istl/matrixfree/blocksorpreconditioner.hh:19-27
+// (the cell update at :221-250): cells in index order, a_i = d_i - sum_{j<i} A_ij v_j^(k) -
+// sum_{j>i} A_ij v_j^(k-1), D_i b_i = a_i, v_i = (1 - omega) v_i + omega b_i, in place on v
+// (GridOperatorPreconditioner::apply, gridoperatorpreconditioner.hh:78-84). from_zero: v^(k-1)
+// = 0, as on the zeroed v of dune-istl's BiCGSTAB (only the lower neighbours contribute).
+static void sor_sweep(oracle_handle* h, const double* d, double* v, bool from_zero = true) {
   auto& o = *h->o;
   const size_t nloc = o.nloc, n2 = nloc * nloc;
   std::vector<double> a(nloc), b(nloc);
+  if (from_zero) std::fill(v, v + o.N, 0.0);
   for (size_t c = 0; c < o.grid.ncells; ++c) {
     for (size_t i = 0; i < nloc; ++i) a[i] = d[c * nloc + i];
     for (int sl = 1; sl < 7; ++sl) {
       const long long cc = h->cols[c * 7 + sl];
-      if (cc < 0 || (size_t)cc >= c) continue;
+      if (cc < 0) continue;
       const double* B = &h->blocks[(c * 7 + sl) * n2];
       const double* vn = v + (size_t)cc * nloc;
       for (size_t i = 0; i < nloc; ++i) {
@@ -737,7 +737,7 @@ static void sor_sweep(oracle_handle* h, const double* d, double* v) {
       }
     }
     lu_solve(nloc, &h->blu[0][c * n2], &h->bpiv[0][c * nloc], a.data(), b.data());
-    for (size_t i = 0; i < nloc; ++i) v[c * nloc + i] = h->sor_omega * b[i];
+    for (size_t i = 0; i < nloc; ++i) v[c * nloc + i] = (1.0 - h->sor_omega) * v[c * nloc + i] + h->sor_omega * b[i];
   }
 }
 
@@ -790,11 +790,12 @@ int oracle_iterative_block_jacobi_apply(oracle_handle* h, const double* d, doubl
   });
 }
 
-// v = the block-SOR preconditioner applied to d (one sweep from v = 0)
-int oracle_block_sor_apply(oracle_handle* h, const double* d, double* v) {
+// v = the block-SOR preconditioner applied to d: one sweep from v = 0 (inplace = 0), or one SOR
+// step from the v given (inplace = 1)
+int oracle_block_sor_apply(oracle_handle* h, const double* d, double* v, int inplace) {
   return guard(h, [&] {
     sor_prepare(h);
-    sor_sweep(h, d, v);
+    sor_sweep(h, d, v, inplace == 0);
   });
 }
 

@@ -93,7 +93,7 @@ def _bind(L):
     L.oracle_size.argtypes = [vp]
     L.oracle_set_threads.argtypes = [vp, ctypes.c_int]
     L.oracle_set_sor_omega.argtypes = [vp, ctypes.c_double]
-    L.oracle_block_sor_apply.argtypes = [vp, _dp, _dp]
+    L.oracle_block_sor_apply.argtypes = [vp, _dp, _dp, ctypes.c_int]
     L.oracle_iterative_block_jacobi_apply.argtypes = [vp, _dp, _dp]
     L.oracle_set_block_solver.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double]
     L.oracle_block_solver_iterations.restype = ctypes.c_ulonglong
@@ -317,12 +317,13 @@ class Oracle:
         self._check(self._L.oracle_grad_source(self.h, _p(lam), _p(g)))
         return g
 
-    def block_sor_apply(self, d):
-        """One forward block-SOR sweep from v = 0 applied to d (the preconditioner)."""
+    def block_sor_apply(self, d, v=None):
+        """One forward block-SOR sweep from v = 0 applied to d (the preconditioner), or, with v
+        given, one SOR step from it (returned as a new array)."""
         d = np.ascontiguousarray(d, dtype=np.float64)
-        v = np.zeros(self.N)
-        self._check(self._L.oracle_block_sor_apply(self.h, _p(d), _p(v)))
-        return v
+        out = np.zeros(self.N) if v is None else np.array(v, dtype=np.float64)
+        self._check(self._L.oracle_block_sor_apply(self.h, _p(d), _p(out), 0 if v is None else 1))
+        return out
 
     def set_block_solver(self, reduction=1e-5, maxit=100, precondition=True, weight=1.0):
         """BlockSolverOptions of the iterative block Jacobi preconditioner."""

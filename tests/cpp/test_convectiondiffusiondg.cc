@@ -183,6 +183,43 @@ int main() {
     for (double e : vh) vn += e * e;
     if (!(vn > 0) || !std::isfinite(vn)) ++fails;
 
+    // --- BiCGSTAB with the matrix-free block SOR and the iterative block Jacobi (the two
+    // preconditioners of matrix_free_linear.cc): the same solution
+    {
+      auto xs = go3.makeVector();
+      ISTLBackend_SEQ_MatrixFree_BCGS_BlockSOR<GO3> lss(go3, 20000, 0);
+      StationaryLinearProblemSolver<GO3, decltype(lss), GO3::Domain> ss(go3, lss, xs, 1e-12);
+      ss.apply();
+      auto xi = go3.makeVector();
+      ISTLBackend_SEQ_MatrixFree_BCGS_IterativeBlockJac<GO3> lsi(go3, 20000, 0);
+      StationaryLinearProblemSolver<GO3, decltype(lsi), GO3::Domain> si(go3, lsi, xi, 1e-12);
+      si.apply();
+      auto hs = xs.download(), hi = xi.download();
+      double es = 0, ei = 0;
+      for (size_t i = 0; i < hs.size(); ++i) {
+        es += (hs[i] - b[i]) * (hs[i] - b[i]);
+        ei += (hi[i] - b[i]) * (hi[i] - b[i]);
+      }
+      std::printf("BiCGSTAB block SOR: %d iterations, rel. difference %g; iterative block Jacobi: %d, %g\n",
+                  (int)lss.result().iterations, std::sqrt(es / den), (int)lsi.result().iterations, std::sqrt(ei / den));
+      if (std::sqrt(es / den) > 1e-8 || std::sqrt(ei / den) > 1e-8 || !lss.result().converged || !lsi.result().converged)
+        ++fails;
+      // the preconditioner objects: SOR on v = 0 and the accumulating iterative block Jacobi
+      BlockSORPreconditioner<GO3> sor(go3, 1.0);
+      IterativeBlockJacobiPreconditioner<GO3> ibj(go3, BlockSolverOptions(1e-5, 100));
+      auto vs = go3.makeVector(), vi = go3.makeVector();
+      vs = 0.0;
+      vi = 0.0;
+      sor.apply(vs, d);
+      ibj.apply(vi, d);
+      auto a1 = vs.download(), a2 = vi.download();
+      double n1 = 0, n2 = 0;
+      for (size_t i = 0; i < a1.size(); ++i) { n1 += a1[i] * a1[i]; n2 += a2[i] * a2[i]; }
+      std::printf("preconditioner objects: |W_sor^-1 d| %g, |W_ibj^-1 d| %g, block iterations %llu\n", std::sqrt(n1),
+                  std::sqrt(n2), ibj.iterations());
+      if (!(n1 > 0) || !(n2 > 0) || !std::isfinite(n1 + n2) || ibj.iterations() == 0) ++fails;
+    }
+
     // --- the lognormal field given as the diagonal tensor diag(k, k, k): the same solve, bit for bit
     {
       TensorProblem tp;

@@ -106,8 +106,10 @@ def test_aniso_blocks_and_preconditioners(pkg, of, name, kw):
     assert rel(bdiag.download().reshape(-1, nloc, nloc), o.block_diagonal()) <= 1e-12
     d = np.random.default_rng(74).uniform(-1, 1, go.N)
     v = go.vector()
+    v.zero()
     go.block_sor_apply(go.vector(d), v)
     assert rel(v.download(), o.block_sor_apply(d)) <= 1e-10
+    v.zero()
     its = go.iterative_block_jacobi_apply(go.vector(d), v)
     want, its_ref = o.iterative_block_jacobi_apply(d)
     assert rel(v.download(), want) <= 1e-4

@@ -1,7 +1,8 @@
 """GPU parity of the block-SOR preconditioner (through the C-ABI).
 
-* pdelab_gpu_block_sor_apply (one forward sweep from v = 0, one launch per hyperplane
-  x + y + z) against the oracle's sequential sweep over the cells in index order, on every
+* pdelab_gpu_block_sor_apply (one forward sweep, one launch per hyperplane x + y + z), from
+  v = 0 and in place from a given v, against the oracle's sequential sweep over the cells in
+  index order, on every
   operator variant of the apply parity cases (constant and cell-wise kappa, SIPG / NIPG /
   IIPG, weightsOff, convection with reaction, outflow and Neumann sides, a single cell, a
   line of cells) and two relaxation factors: within 1e-10 relative (the dense inverse
@@ -30,14 +31,18 @@ def test_block_sor_apply_vs_oracle(pkg, of, name, kw, seed):
     cfg, kappa, go, o = make(pkg, of, kw, seed)
     rng = np.random.default_rng(23)
     d = rng.uniform(-1, 1, go.N)
-    dd = go.vector(d)
-    v = go.vector(rng.normal(size=go.N))  # overwritten, not accumulated into
+    v0 = rng.normal(size=go.N)
+    dd, v = go.vector(d), go.vector()
     for omega in (1.0, 1.35):
         go.set_sor_relaxation(omega)
         o.set_sor_omega(omega)
-        go.block_sor_apply(dd, v)
-        want = o.block_sor_apply(d)
-        e = rel(v.download(), want)
+        v.zero()
+        go.block_sor_apply(dd, v)  # on v = 0: the preconditioner the solvers apply
+        e = rel(v.download(), o.block_sor_apply(d))
+        assert e <= 1e-10, (omega, e)
+        v.upload(v0)
+        go.block_sor_apply(dd, v)  # one SOR step from v0, in place
+        e = rel(v.download(), o.block_sor_apply(d, v0))
         assert e <= 1e-10, (omega, e)
     go.close()
 

@@ -1,6 +1,6 @@
 """GPU parity of the iterative block Jacobi preconditioner (through the C-ABI).
 
-* pdelab_gpu_iterative_block_jacobi_apply (one wave per cell running dune-istl's BiCGSTAB on
+* pdelab_gpu_iterative_block_jacobi_apply (v += W^{-1} d; one wave per cell running dune-istl's BiCGSTAB on
   the cell's diagonal block, point-Jacobi preconditioned) against the oracle's per-cell solves
   on every operator variant of the apply parity cases: within 1e-4 relative (a block whose
   defect ends next to the 1e-5 threshold may stop one half step apart under different rounding,
@@ -27,14 +27,15 @@ def test_iterative_block_jacobi_apply_vs_oracle(pkg, of, name, kw, seed):
     cfg, kappa, go, o = make(pkg, of, kw, seed)
     d = np.random.default_rng(29).uniform(-1, 1, go.N)
     dd, v = go.vector(d), go.vector(np.ones(go.N))
-    its = go.iterative_block_jacobi_apply(dd, v)
+    its = go.iterative_block_jacobi_apply(dd, v)  # accumulates, as GridOperatorPreconditioner::apply
     want, its_ref = o.iterative_block_jacobi_apply(d)
-    e = rel(v.download(), want)
+    e = rel(v.download() - 1.0, want)
     say(f"  rel err {e:.2e}, block iterations {its} (oracle {its_ref})")
     assert e <= 1e-4
     assert abs(its - its_ref) <= max(2, its_ref // 50)
     go.set_block_solver_options(1e-11, 500, True, 1.0)
     o.set_block_solver(1e-11, 500, True, 1.0)
+    v.zero()
     go.iterative_block_jacobi_apply(dd, v)
     want, _ = o.iterative_block_jacobi_apply(d)
     assert rel(v.download(), want) <= 1e-7

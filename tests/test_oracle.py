@@ -497,7 +497,8 @@ def test_block_sor_apply_vs_lower_triangular_solve(of, name, kw, seed, omega):
     """The oracle's block-SOR sweep (blocksorpreconditioner.hh:19-27 from v = 0) against
     numpy.linalg.solve with M = D / omega + L built from the dense matrix: the same vector.
     A sweep over the cells in any other order, one that dropped the lower neighbours or used
-    the upper ones, or that misplaced omega, fails this."""
+    the upper ones, or that misplaced omega, fails this. One step from a given v is the
+    textbook SOR iteration (D / omega + L) v' = d - U v + (1 / omega - 1) D v."""
     cfg, o = _oracle_case(of, kw, seed)
     o.set_sor_omega(omega)
     A = o.dense_matrix()
@@ -506,6 +507,14 @@ def test_block_sor_apply_vs_lower_triangular_solve(of, name, kw, seed, omega):
     v = o.block_sor_apply(d)
     v_np = np.linalg.solve(_sor_matrix(A, nloc, omega), d)
     assert np.abs(v - v_np).max() <= 1e-11 * np.abs(v_np).max()
+    # one SOR step from a given v: (D / omega + L) v' = d - U v + (1 / omega - 1) D v
+    v0 = np.random.default_rng(32).normal(size=o.N)
+    cell = np.arange(o.N) // nloc
+    U = np.where(cell[:, None] < cell[None, :], A, 0.0)
+    D = np.where(cell[:, None] == cell[None, :], A, 0.0)
+    v1 = o.block_sor_apply(d, v0)
+    v1_np = np.linalg.solve(_sor_matrix(A, nloc, omega), d - U @ v0 + (1.0 / omega - 1.0) * (D @ v0))
+    assert np.abs(v1 - v1_np).max() <= 1e-10 * np.abs(v1_np).max()
 
 
 def test_block_sor_bicgstab_vs_numpy_textbook(of):
