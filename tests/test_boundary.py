@@ -74,6 +74,20 @@ def test_create_rejects_bad_configs_before_touching_a_device(pkg):
     cfg = pkg.make_config()
     cfg.abi_version = 99
     assert L.pdelab_gpu_create(ctypes.byref(cfg), None, ctypes.byref(h)) == 1
+    cfg = pkg.make_config()
+    cfg.kappa_per_cell = 3  # 0, 1 or 2 only
+    assert L.pdelab_gpu_create(ctypes.byref(cfg), None, ctypes.byref(h)) == 1
+    assert b"kappa_per_cell" in L.pdelab_gpu_last_error(None)
+
+
+def test_kappa_layouts_checked_on_the_host(pkg):
+    """kappa_per_cell = 2 (a diagonal tensor per cell) takes three values per cell; the binding
+    checks the length before the library reads the array."""
+    cfg = pkg.make_config(n=2, kappa_per_cell=2)
+    assert cfg.kappa_per_cell == 2 and pkg.make_config(n=2, kappa_per_cell=True).kappa_per_cell == 1
+    assert pkg.GridOperator._kappa_size(cfg) == 24
+    with pytest.raises(ValueError, match="24"):
+        pkg.GridOperator(cfg, [1.0] * 8)
 
 
 def test_null_handles_are_invalid_arguments(pkg):
