@@ -496,8 +496,8 @@ def main():
         print(profile_summary(args.profile_summary, args.n, args.degree))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
-              "(the domain-decomposition design is described in DESIGN.md).", file=sys.stderr)
+        print("bench.py: this library runs on one GPU only; a multi-GPU run (domain decomposition "
+              "over ranks) is not implemented.", file=sys.stderr)
         sys.exit(2)
 
     pkg = import_package()
