@@ -25,7 +25,7 @@ from .config import (  # noqa: F401  (re-exported)
     PROBLEM_CONSTANT, PROBLEM_EXP, PROBLEM_GAUSSIANS, PROBLEM_QUADRATIC, PROBLEM_TABULATED,
     PROBLEM_ZERO, SIPG, SOLVER_BICGSTAB, SOLVER_CG, TS_ALEXANDER2, TS_CRANK_NICOLSON, TS_IMPLICIT_EULER,
     WEIGHTS_OFF, WEIGHTS_ON, LinearProblemResult, LinearSolverResult, MixedResult, OneStepResult, PdelabConfig,
-    make_config, ncells, ndofs)
+    make_config, ncells, ndofs, quadrature_points, tabulate_velocity)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "lib", "libpdelab_gpu.so")
@@ -72,6 +72,7 @@ def lib():
     L.pdelab_gpu_set_log_kappa_device.argtypes = [_vp, _vp]
     L.pdelab_gpu_set_cell_source.argtypes = [_vp, _dp]
     L.pdelab_gpu_set_tabulated_data.argtypes = [_vp, _dp, _dp]
+    L.pdelab_gpu_set_velocity_field.argtypes = [_vp, _dp, _dp]
     L.pdelab_gpu_malloc.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.c_size_t]
     L.pdelab_gpu_free.argtypes = [_vp, _vp]
     for fn in ("pdelab_gpu_memcpy_htod", "pdelab_gpu_memcpy_dtoh", "pdelab_gpu_memcpy_dtod"):
@@ -279,6 +280,22 @@ class GridOperator:
 
     def quadrature_points(self):
         return lib().pdelab_gpu_quadrature_points(self.h)
+
+    def set_velocity_field(self, b_vol, b_face=None):
+        """Velocity field b(x) at the quadrature points (layouts of tabulate_velocity); None
+        restores the constant b of the configuration."""
+        if b_vol is None:
+            self._check(lib().pdelab_gpu_set_velocity_field(self.h, None, None))
+            return
+        m = self.quadrature_points()
+        c = self.cfg
+        nv = ncells(c) * m ** 3 * 3
+        nf = ((c.nx + 1) * c.ny * c.nz + c.nx * (c.ny + 1) * c.nz + c.nx * c.ny * (c.nz + 1)) * m * m
+        bv = np.ascontiguousarray(b_vol, dtype=np.float64).ravel()
+        bf = np.ascontiguousarray(b_face, dtype=np.float64).ravel()
+        if bv.size != nv or bf.size != nf:
+            raise ValueError(f"velocity field: b_vol needs {nv} values and b_face {nf}")
+        self._check(lib().pdelab_gpu_set_velocity_field(self.h, bv.ctypes.data_as(_dp), bf.ctypes.data_as(_dp)))
 
     # -- GridOperator
     def residual(self, x, r):

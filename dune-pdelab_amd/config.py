@@ -5,6 +5,8 @@ the struct size against the C compiler's.
 """
 import ctypes
 
+import numpy as np
+
 ABI_VERSION = 2
 MAX_GAUSSIANS = 16
 
@@ -165,3 +167,53 @@ def ncells(cfg):
 
 def ndofs(cfg):
     return ncells(cfg) * (cfg.degree + 1) ** 3
+
+
+def quadrature_points(cfg):
+    """Gauss points per direction of the operator's quadrature (order 2k + intorderadd)."""
+    return (2 * cfg.degree + cfg.intorderadd) // 2 + 1
+
+
+def tabulate_velocity(cfg, b):
+    """The arrays pdelab_gpu_set_velocity_field takes, from a velocity field b(x, y, z) that
+    accepts numpy arrays and returns (b_x, b_y, b_z):
+    b_vol  per cell (x fastest), the m^3 volume Gauss points (x fastest), (b_x, b_y, b_z);
+    b_face per face normal to x, then y, then z (within a direction lexicographic in the
+           face position, x fastest, the position along the normal running 0..n), the m^2
+           face points (the lower tangential direction fastest), the normal component b_d.
+    The points are the Gauss-Legendre points of quadratureRule(geo, 2k + intorderadd) mapped
+    to each cell. b is evaluated at the physical point, so on an interior face both cells
+    see the same value (a continuous field)."""
+    m = quadrature_points(cfg)
+    xi, _ = np.polynomial.legendre.leggauss(m)
+    xi = (xi + 1.0) / 2.0
+    n = (cfg.nx, cfg.ny, cfg.nz)
+    h = (cfg.lx / cfg.nx, cfg.ly / cfg.ny, cfg.lz / cfg.nz)
+    # volume: cell (i, j, k), point (a, b, c) -> coordinates
+    ci = [np.arange(n[d]) for d in range(3)]
+    X = (ci[0][None, None, :, None, None, None] + xi[None, None, None, None, None, :]) * h[0]
+    Y = (ci[1][None, :, None, None, None, None] + xi[None, None, None, None, :, None]) * h[1]
+    Z = (ci[2][:, None, None, None, None, None] + xi[None, None, None, :, None, None]) * h[2]
+    shape = (n[2], n[1], n[0], m, m, m)
+    bv = [np.broadcast_to(np.asarray(v, dtype=np.float64), shape)
+          for v in np.broadcast_arrays(*b(*np.broadcast_arrays(X, Y, Z)))]
+    b_vol = np.stack(bv, axis=-1).reshape(-1)
+    faces = []
+    for d in range(3):
+        t1, t2 = (1, 2) if d == 0 else ((0, 2) if d == 1 else (0, 1))
+        npos = [n[e] + (1 if e == d else 0) for e in range(3)]
+        pos = [np.arange(npos[e]) for e in range(3)]
+        coord = [None, None, None]
+        # face (z, y, x) positions, then points (q2 along t2, q1 along t1)
+        for e in range(3):
+            p = pos[e].astype(np.float64).reshape([-1 if a == 2 - e else 1 for a in range(3)] + [1, 1])
+            if e == d:
+                coord[e] = p * h[e]
+            elif e == t1:
+                coord[e] = (p + xi.reshape(1, 1, 1, 1, m)) * h[e]
+            else:
+                coord[e] = (p + xi.reshape(1, 1, 1, m, 1)) * h[e]
+        bd = b(*np.broadcast_arrays(*coord))[d]
+        faces.append(np.broadcast_to(np.asarray(bd, dtype=np.float64),
+                                     (npos[2], npos[1], npos[0], m, m)).reshape(-1))
+    return b_vol, np.concatenate(faces)

@@ -44,14 +44,14 @@ __device__ inline double exact_u(const LoadArgs& a, const double x[3]) {
 }
 
 // Source term f of the built-in problems (ConvectionDiffusionModelProblem::f,
-// R:dune/pdelab/localoperator/convectiondiffusionparameter.hh:174-179)
-__device__ inline double eval_f(const LoadArgs& a, const double x[3]) {
+// R:dune/pdelab/localoperator/convectiondiffusionparameter.hh:174-179); b is the velocity at x
+__device__ inline double eval_f(const LoadArgs& a, const double x[3], const double b[3]) {
   switch (a.problem) {
     case PROB_EXP: {
       double r2 = 0, bg = 0;
       for (int d = 0; d < 3; ++d) r2 += (x[d] - 0.5) * (x[d] - 0.5);
       const double u = exp(-r2);
-      for (int d = 0; d < 3; ++d) bg += a.b[d] * (-2.0 * (x[d] - 0.5) * u);
+      for (int d = 0; d < 3; ++d) bg += b[d] * (-2.0 * (x[d] - 0.5) * u);
       return a.kappa0 * (6.0 - 4.0 * r2) * u + bg + a.c * u;
     }
     case PROB_CONSTANT: return a.f0;
@@ -68,7 +68,7 @@ __device__ inline double eval_f(const LoadArgs& a, const double x[3]) {
       const double* q = a.quad;
       const double lap = 2.0 * (q[4] + q[5] + q[6]);
       const double grad[3] = {q[1] + 2 * q[4] * x[0], q[2] + 2 * q[5] * x[1], q[3] + 2 * q[6] * x[2]};
-      return -a.kappa0 * lap + a.b[0] * grad[0] + a.b[1] * grad[1] + a.b[2] * grad[2] + a.c * exact_u(a, x);
+      return -a.kappa0 * lap + b[0] * grad[0] + b[1] * grad[1] + b[2] * grad[2] + a.c * exact_u(a, x);
     }
     default: return 0.0;
   }
@@ -77,14 +77,14 @@ __device__ inline double eval_f(const LoadArgs& a, const double x[3]) {
 // Neumann flux j ((b u - A grad u).n = j) and outflow data o (-(A grad u).n = o) of the
 // built-in problems at x on a boundary side with outer normal nsign e_d
 // (convectiondiffusionparameter.hh:121-124, 196-208): the fluxes of the exact solution
-// (A = kappa0 I) for EXP and QUADRATIC, the constants j0 / o0 otherwise.
-__device__ inline double eval_flux(const LoadArgs& a, const double x[3], int d, double nsign, bool neumann) {
+// (A = kappa0 I) for EXP and QUADRATIC, the constants j0 / o0 otherwise. bn = b . n at x.
+__device__ inline double eval_flux(const LoadArgs& a, const double x[3], int d, double nsign, double bn, bool neumann) {
   if (a.problem == PROB_EXP || a.problem == PROB_QUADRATIC) {
     double gd;
     if (a.problem == PROB_EXP) gd = -2.0 * (x[d] - 0.5) * exact_u(a, x);
     else gd = a.quad[1 + d] + 2.0 * a.quad[4 + d] * x[d];
     const double diff = -a.kappa0 * nsign * gd;
-    return neumann ? nsign * a.b[d] * exact_u(a, x) + diff : diff;
+    return neumann ? bn * exact_u(a, x) + diff : diff;
   }
   return neumann ? a.j0 : a.o0;
 }
@@ -100,10 +100,12 @@ __device__ inline double eval_g(const LoadArgs& a, const double x[3]) {
 }
 
 // R0 += lambda_volume contribution -int f phi_i (convectiondiffusiondg.hh:1048-1075),
-// sum-factorised over the m^3 Gauss points. One thread per DoF.
+// sum-factorised over the m^3 Gauss points. One thread per DoF. bvt: the tabulated velocity
+// field (per cell, component d of the m^3 points at d m^3), or null for the constant b.
 template <int N1>
 __global__ __launch_bounds__(256) void load_volume_kernel(double* __restrict__ R0, const double* __restrict__ ftab,
-                                                          const double* __restrict__ src, const LoadArgs a) {
+                                                          const double* __restrict__ src, const LoadArgs a,
+                                                          const double* __restrict__ bvt) {
   constexpr int L = N1 * N1, NL = L * N1;
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= a.ncells * NL) return;
@@ -125,7 +127,14 @@ __global__ __launch_bounds__(256) void load_volume_kernel(double* __restrict__ R
           f = ftab ? ftab[c * (long long)(m * m * m) + q0 + m * (q1 + m * q2)] : 0.0;
         } else {
           const double x[3] = {lo[0] + ext[0] * a.qx[q0], lo[1] + ext[1] * a.qx[q1], lo[2] + ext[2] * a.qx[q2]};
-          f = eval_f(a, x);
+          if (bvt) {
+            const long long m3 = (long long)m * m * m, q = q0 + m * (q1 + m * q2);
+            const double* bc = bvt + c * 3 * m3 + q;
+            const double bq[3] = {bc[0], bc[m3], bc[2 * m3]};
+            f = eval_f(a, x, bq);
+          } else {
+            f = eval_f(a, x, a.b);
+          }
         }
         f += s;
         acc += f * a.qw[q0] * a.P[q0][i0] * w12;
@@ -138,11 +147,13 @@ __global__ __launch_bounds__(256) void load_volume_kernel(double* __restrict__ R
 // (convectiondiffusiondg.hh:839-877) inflow convection g (b.n) psi, -theta g (A n . grad psi),
 // -penalty g psi; on Neumann sides (:776-783) j psi; on outflow sides (:810-815) o psi.
 // One launch per boundary side (no two threads of a launch write the same entry); one
-// thread per (boundary cell, DoF). bct is the side's boundary type.
+// thread per (boundary cell, DoF). bct is the side's boundary type. bft: the tabulated
+// velocity field (per cell and face f = 2 d + upper, b_d at the m^2 points), or null for the
+// constant b; with it the upwind choice of the inflow term is made per point.
 template <int N1>
 __global__ __launch_bounds__(256) void load_boundary_kernel(double* __restrict__ R0, const double* __restrict__ gtab,
                                                             const double* __restrict__ kappa, const LoadArgs a,
-                                                            int side, int bct) {
+                                                            int side, int bct, const double* __restrict__ bft) {
   constexpr int L = N1 * N1, NL = L * N1;
   const int d = side / 2, upper = side % 2;
   const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
@@ -161,9 +172,11 @@ __global__ __launch_bounds__(256) void load_boundary_kernel(double* __restrict__
   for (int e = 0; e < 3; ++e) { lo[e] = coord[e] * a.h[e]; ext[e] = (coord[e] + 1) * a.h[e] - lo[e]; }
   const int m = a.m;
   const double nsign = upper ? 1.0 : -1.0;
-  double gt = 0.0;
+  const double* bfc = bft ? bft + (c * 6 + side) * (long long)(m * m) : nullptr;
+  double gt = 0.0, gconv = 0.0;
   for (int q2 = 0; q2 < m; ++q2)
     for (int q1 = 0; q1 < m; ++q1) {
+      const double bn = bfc ? nsign * bfc[q1 + m * q2] : nsign * a.b[d];
       double g;
       if (a.problem == PROB_TABULATED) {
         g = gtab ? gtab[(a.bface_offset[side] + fidx) * (m * m) + q1 + m * q2] : 0.0;
@@ -172,9 +185,11 @@ __global__ __launch_bounds__(256) void load_boundary_kernel(double* __restrict__
         x[d] = lo[d] + ext[d] * (upper ? 1.0 : 0.0);
         x[t1] = lo[t1] + ext[t1] * a.qx[q1];
         x[t2] = lo[t2] + ext[t2] * a.qx[q2];
-        g = bct == BC_DIRICHLET ? eval_g(a, x) : eval_flux(a, x, d, nsign, bct == BC_NEUMANN);
+        g = bct == BC_DIRICHLET ? eval_g(a, x) : eval_flux(a, x, d, nsign, bn, bct == BC_NEUMANN);
       }
-      gt += g * a.qw[q1] * a.qw[q2] * a.P[q1][ii[t1]] * a.P[q2][ii[t2]];
+      const double gw = g * a.qw[q1] * a.qw[q2] * a.P[q1][ii[t1]] * a.P[q2][ii[t2]];
+      gt += gw;
+      if (bn < 0.0) gconv += bn * gw;  // inflow: omega_n = 1 picks g (:861-868)
     }
   const double vol = ext[0] * ext[1] * ext[2];
   const double area = vol / ext[d];
@@ -190,10 +205,11 @@ __global__ __launch_bounds__(256) void load_boundary_kernel(double* __restrict__
   const double ds = upper ? a.d1[i] : a.d0[i];
   const double Pb = (a.alphak / hF) * (a.weightsOn ? kc : 1.0);
   double coef = 0.0;
-  if (nf < 0.0) coef += nf * es;
+  if (!bfc && nf < 0.0) coef += nf * es;
   coef += -a.theta * kc * nsign * ds / ext[d];
   coef += -Pb * es;
-  R0[c * NL + li] += area * gt * coef;
+  if (bfc) R0[c * NL + li] += area * (gt * coef + gconv * es);
+  else R0[c * NL + li] += area * gt * coef;
 }
 
 // Tangential-mass contraction sum_{t,t'} (M (x) M)[t][t'] A[t] B[t'] of two face arrays.

@@ -14,6 +14,7 @@
 #include "dg_kernels.hh"
 #include "dg_load.hh"
 #include "dg_tables.hh"
+#include "dg_velocity.hh"
 #include "output.hh"
 
 using namespace pdelab_amd;
@@ -97,6 +98,15 @@ struct pdelab_gpu_gridoperator {
   int* d_sor_order = nullptr;
   std::vector<int> sor_first;
   double sor_omega = 1.0;
+  // velocity field b(x) tabulated at the quadrature points (replaces the constant b of cfg):
+  // device layouts of dg_velocity.hh, FP64 and FP32
+  bool varb = false;
+  double* d_bv64 = nullptr;
+  double* d_bf64 = nullptr;
+  float* d_bv32 = nullptr;
+  float* d_bf32 = nullptr;
+  VelArgs<double> va64;
+  VelArgs<float> va32;
 };
 
 using H = pdelab_gpu_gridoperator;
@@ -219,5 +229,13 @@ int ibj_launch(H* h, const T* d, T* v, int* err, unsigned long long* iters, cons
 // Assembled Jacobian in the 7-point block pattern (jacobian.hh) and its apply y = A z.
 int jac_assemble(H* h, bool trans, double* blocks, long long* cols);
 int jac_block_apply(H* h, const double* blocks, const long long* cols, const double* z, double* y);
+// Velocity field (velocity.hip): set or clear the tabulated b(x) (host arrays in the layout of
+// pdelab_gpu_set_velocity_field), add its convection terms y += C z (nothing stored when
+// *stop is set) and their diagonal, free the data.
+int vel_set(H* h, const double* b_vol, const double* b_face);
+template <typename T>
+int vel_launch(H* h, const T* z, T* y, const int* stop);
+int vel_diag(H* h, double* diag);
+void vel_free(H* h);
 
 #endif

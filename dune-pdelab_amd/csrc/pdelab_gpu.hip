@@ -311,7 +311,15 @@ static void fill_load_args(H* h) {
 
 // ---------------------------------------------------------------------------
 // kernel dispatch
+// The cell blocks (block preconditioners, block diagonal, assembled Jacobian) are built for
+// the constant b only.
+static int block_data_ok(H* h) {
+  if (h->varb) return fail(h, PDELAB_ERR_UNSUPPORTED, "velocity field: the cell blocks are not built for it");
+  return PDELAB_OK;
+}
+
 static int check_outflow(H* h) {
+  if (h->varb) return PDELAB_OK;  // a velocity field is checked when it is set
   for (int s = 0; s < 6; ++s) {
     if (h->a64.bc[s] != BC_OUTFLOW) continue;
     const int d = s / 2;
@@ -325,12 +333,19 @@ template <typename T>
 static int apply_mode(H* h, const T* z, T* y, bool trans, int mode, T* dotp = nullptr,
                       const int* stop = nullptr) {
   CHK(check_outflow(h));
-  switch (h->n1) {
-    case 2: return launch_apply_n<2, T>(h, z, y, fc_for<T>(h), args_for<T>(h), trans, mode, dotp, stop);
-    case 3: return launch_apply_n<3, T>(h, z, y, fc_for<T>(h), args_for<T>(h), trans, mode, dotp, stop);
-    case 4: return launch_apply_n<4, T>(h, z, y, fc_for<T>(h), args_for<T>(h), trans, mode, dotp, stop);
+  if (h->varb) {
+    if (trans) return fail(h, PDELAB_ERR_UNSUPPORTED, "velocity field: the transposed operator is not built");
+    if (mode == 2) return fail(h, PDELAB_ERR_UNSUPPORTED, "velocity field: CG (non-symmetric operator), use BiCGSTAB");
   }
-  return fail(h, PDELAB_ERR_UNSUPPORTED, "degree");
+  int st = PDELAB_ERR_UNSUPPORTED;
+  switch (h->n1) {
+    case 2: st = launch_apply_n<2, T>(h, z, y, fc_for<T>(h), args_for<T>(h), trans, mode, dotp, stop); break;
+    case 3: st = launch_apply_n<3, T>(h, z, y, fc_for<T>(h), args_for<T>(h), trans, mode, dotp, stop); break;
+    case 4: st = launch_apply_n<4, T>(h, z, y, fc_for<T>(h), args_for<T>(h), trans, mode, dotp, stop); break;
+    default: return fail(h, PDELAB_ERR_UNSUPPORTED, "degree");
+  }
+  CHK(st);
+  return vel_launch<T>(h, z, y, stop);  // the velocity field's convection terms, if set
 }
 static int apply64(H* h, const double* z, double* y, bool trans, bool accum) {
   return apply_mode<double>(h, z, y, trans, accum ? 1 : 0);
@@ -359,10 +374,12 @@ static int compute_R0(H* h) {
   HIPCHK(h, hipMemsetAsync(h->d_R0, 0, sizeof(double) * h->N, h->stream));
   const long long n = h->N;
   const int nb = (int)((n + 255) / 256);
+  const double* bvt = h->varb ? h->d_bv64 : nullptr;  // velocity field at the volume / face points
+  const double* bft = h->varb ? h->d_bf64 : nullptr;
   switch (h->n1) {
-    case 2: hipLaunchKernelGGL(load_volume_kernel<2>, dim3(nb), dim3(256), 0, h->stream, h->d_R0, h->d_ftab, h->d_src, h->la); break;
-    case 3: hipLaunchKernelGGL(load_volume_kernel<3>, dim3(nb), dim3(256), 0, h->stream, h->d_R0, h->d_ftab, h->d_src, h->la); break;
-    case 4: hipLaunchKernelGGL(load_volume_kernel<4>, dim3(nb), dim3(256), 0, h->stream, h->d_R0, h->d_ftab, h->d_src, h->la); break;
+    case 2: hipLaunchKernelGGL(load_volume_kernel<2>, dim3(nb), dim3(256), 0, h->stream, h->d_R0, h->d_ftab, h->d_src, h->la, bvt); break;
+    case 3: hipLaunchKernelGGL(load_volume_kernel<3>, dim3(nb), dim3(256), 0, h->stream, h->d_R0, h->d_ftab, h->d_src, h->la, bvt); break;
+    case 4: hipLaunchKernelGGL(load_volume_kernel<4>, dim3(nb), dim3(256), 0, h->stream, h->d_R0, h->d_ftab, h->d_src, h->la, bvt); break;
   }
   HIPCHK(h, hipGetLastError());
   // boundary data: g on Dirichlet sides (none for the ZERO problem), j on Neumann and o on
@@ -380,9 +397,9 @@ static int compute_R0(H* h) {
     const long long nf = (long long)h->la.n[t1] * h->la.n[t2] * h->nloc;
     const int gb = (int)((nf + 255) / 256);
     switch (h->n1) {
-      case 2: hipLaunchKernelGGL(load_boundary_kernel<2>, dim3(gb), dim3(256), 0, h->stream, h->d_R0, h->d_gtab, h->d_kappa, h->la, s, bct); break;
-      case 3: hipLaunchKernelGGL(load_boundary_kernel<3>, dim3(gb), dim3(256), 0, h->stream, h->d_R0, h->d_gtab, h->d_kappa, h->la, s, bct); break;
-      case 4: hipLaunchKernelGGL(load_boundary_kernel<4>, dim3(gb), dim3(256), 0, h->stream, h->d_R0, h->d_gtab, h->d_kappa, h->la, s, bct); break;
+      case 2: hipLaunchKernelGGL(load_boundary_kernel<2>, dim3(gb), dim3(256), 0, h->stream, h->d_R0, h->d_gtab, h->d_kappa, h->la, s, bct, bft); break;
+      case 3: hipLaunchKernelGGL(load_boundary_kernel<3>, dim3(gb), dim3(256), 0, h->stream, h->d_R0, h->d_gtab, h->d_kappa, h->la, s, bct, bft); break;
+      case 4: hipLaunchKernelGGL(load_boundary_kernel<4>, dim3(gb), dim3(256), 0, h->stream, h->d_R0, h->d_gtab, h->d_kappa, h->la, s, bct, bft); break;
     }
     HIPCHK(h, hipGetLastError());
   }
@@ -408,6 +425,7 @@ static int compute_diag(H* h) {
   }
 #undef DIAGL
   HIPCHK(h, hipGetLastError());
+  CHK(vel_diag(h, h->d_diag));
   hipLaunchKernelGGL(invert_kernel<double>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, h->d_diag, h->d_dinv);
   hipLaunchKernelGGL(invert_kernel<float>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, h->d_diag, h->d_dinv32);
   HIPCHK(h, hipGetLastError());
@@ -530,6 +548,7 @@ static void sync_prec_cache(H* h) {
 // Checks the preconditioner and prepares its data (point diagonal or block data).
 template <typename T>
 static int make_prec(H* h, int precond, bool trans, Prec<T>& P) {
+  if (precond >= PDELAB_PREC_BLOCK_JACOBI && precond <= PDELAB_PREC_ITERATIVE_BLOCK_JACOBI) CHK(block_data_ok(h));
   sync_prec_cache(h);
   P.kind = precond;
   P.trans = trans;
@@ -1185,6 +1204,7 @@ int pdelab_gpu_destroy(pdelab_gpu_gridoperator* h) {
   for (double* p : h->fg)
     if (p) (void)hipFree(p);
   bj_free(h);
+  vel_free(h);
   for (int i = 0; i < 8; ++i) {
     if (h->w64[i]) (void)hipFree(h->w64[i]);
     if (h->w32[i]) (void)hipFree(h->w32[i]);
@@ -1207,6 +1227,11 @@ size_t pdelab_gpu_size(const pdelab_gpu_gridoperator* h) { return h ? (size_t)h-
 size_t pdelab_gpu_num_cells(const pdelab_gpu_gridoperator* h) { return h ? (size_t)h->ncells : 0; }
 int pdelab_gpu_is_linear(const pdelab_gpu_gridoperator* h) { return h ? 1 : 0; }
 int pdelab_gpu_quadrature_points(const pdelab_gpu_gridoperator* h) { return h ? h->tab.m : 0; }
+
+int pdelab_gpu_set_velocity_field(pdelab_gpu_gridoperator* h, const double* b_vol, const double* b_face) {
+  if (!h) return PDELAB_ERR_INVALID_ARG;
+  return vel_set(h, b_vol, b_face);
+}
 
 int pdelab_gpu_set_kappa(pdelab_gpu_gridoperator* h, const double* kappa_host) {
   if (!h || !kappa_host) return PDELAB_ERR_INVALID_ARG;
@@ -1350,12 +1375,14 @@ int pdelab_gpu_point_diagonal(pdelab_gpu_gridoperator* h, double* d) {
 int pdelab_gpu_block_diagonal(pdelab_gpu_gridoperator* h, int transpose, double* blocks) {
   if (!h || !blocks) return PDELAB_ERR_INVALID_ARG;
   CHK(check_outflow(h));
+  CHK(block_data_ok(h));
   return bj_block_diagonal(h, transpose != 0, blocks);
 }
 
 int pdelab_gpu_jacobian(pdelab_gpu_gridoperator* h, int transpose, double* blocks, long long* cols) {
   if (!h || !blocks || !cols) return PDELAB_ERR_INVALID_ARG;
   CHK(check_outflow(h));
+  CHK(block_data_ok(h));
   return jac_assemble(h, transpose != 0, blocks, cols);
 }
 
@@ -1368,6 +1395,7 @@ int pdelab_gpu_block_matrix_apply(pdelab_gpu_gridoperator* h, const double* bloc
 int pdelab_gpu_block_jacobi_apply(pdelab_gpu_gridoperator* h, const double* z, double* y) {
   if (!h || !z || !y) return PDELAB_ERR_INVALID_ARG;
   CHK(check_outflow(h));
+  CHK(block_data_ok(h));
   sync_prec_cache(h);
   const BJVec<double, double> v{z, y, nullptr, nullptr, nullptr, nullptr, nullptr};
   return bj_launch<double, double>(h, false, BJ_ADD, v, nullptr);
@@ -1376,6 +1404,7 @@ int pdelab_gpu_block_jacobi_apply(pdelab_gpu_gridoperator* h, const double* z, d
 int pdelab_gpu_block_jacobi_apply_f32(pdelab_gpu_gridoperator* h, const float* z, float* y) {
   if (!h || !z || !y) return PDELAB_ERR_INVALID_ARG;
   CHK(check_outflow(h));
+  CHK(block_data_ok(h));
   sync_prec_cache(h);
   const BJVec<float, float> v{z, y, nullptr, nullptr, nullptr, nullptr, nullptr};
   return bj_launch<float, float>(h, false, BJ_ADD, v, nullptr);
@@ -1384,6 +1413,7 @@ int pdelab_gpu_block_jacobi_apply_f32(pdelab_gpu_gridoperator* h, const float* z
 int pdelab_gpu_block_sor_apply(pdelab_gpu_gridoperator* h, const double* d, double* v) {
   if (!h || !d || !v || d == v) return PDELAB_ERR_INVALID_ARG;
   CHK(check_outflow(h));
+  CHK(block_data_ok(h));
   sync_prec_cache(h);
   return sor_launch<double>(h, d, v, nullptr, true);
 }
@@ -1392,6 +1422,7 @@ int pdelab_gpu_iterative_block_jacobi_apply(pdelab_gpu_gridoperator* h, const do
                                             unsigned long long* iterations) {
   if (!h || !d || !v || d == v) return PDELAB_ERR_INVALID_ARG;
   CHK(check_outflow(h));
+  CHK(block_data_ok(h));
   sync_prec_cache(h);
   CHK(ibj_prepare(h, 64));
   HIPCHK(h, hipMemsetAsync(h->d_ibj_stat, 0, 16, h->stream));

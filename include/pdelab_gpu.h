@@ -260,6 +260,26 @@ int pdelab_gpu_set_cell_source(pdelab_gpu_gridoperator* h, const double* source_
    with m = (2k + intorderadd)/2 + 1 Gauss points per direction. */
 int pdelab_gpu_set_tabulated_data(pdelab_gpu_gridoperator* h, const double* f_q, const double* g_q);
 int pdelab_gpu_quadrature_points(const pdelab_gpu_gridoperator* h); /* m */
+/* Velocity field b(x) of the convection term, tabulated at the quadrature points where
+   ConvectionDiffusionDG evaluates param.b (convectiondiffusiondg.hh:178 volume, :426 skeleton,
+   :797 boundary; ConvectionDiffusionModelProblem::b, convectiondiffusionparameter.hh:159-165).
+   Host arrays, with m = pdelab_gpu_quadrature_points:
+   b_vol:  per cell (lexicographic, x fastest), the m^3 volume Gauss points (x fastest),
+           (b_x, b_y, b_z) per point: nx*ny*nz*m^3*3 doubles;
+   b_face: the component b_d along +e_d at the m^2 Gauss points of every face normal to d
+           (lower tangential direction fastest): the faces normal to x, then y, then z, each
+           direction's faces lexicographic in their position (x fastest; the position along
+           d runs 0..n_d): ((nx+1)*ny*nz + nx*(ny+1)*nz + nx*ny*(nz+1))*m^2 doubles. On an
+           interior face the reference evaluates b from the cell it visits the face from,
+           the one with the higher index (default/assembler.hh:181).
+   The field replaces the configuration's constant b, which must be zero, in the apply, the
+   residual and its load vector (the Dirichlet inflow term, and the manufactured f and j of
+   PDELAB_PROBLEM_EXP / _QUADRATIC), the point diagonal and the FP32 apply. b_vol = NULL
+   restores the constant b. Outflow sides must see no inflow (b . n >= -1e-30 at every point,
+   else PDELAB_ERR_OUTFLOW_ON_INFLOW). Requires intorderadd 0..3. With a field set, the
+   transposed operator, CG, the cell blocks (block diagonal, block Jacobi, block SOR, the
+   iterative block Jacobi) and the assembled Jacobian return PDELAB_ERR_UNSUPPORTED. */
+int pdelab_gpu_set_velocity_field(pdelab_gpu_gridoperator* h, const double* b_vol, const double* b_face);
 
 /* ---- device memory helpers (ordered on the handle's stream) --------------- */
 int pdelab_gpu_malloc(pdelab_gpu_gridoperator* h, void** dptr, size_t bytes);

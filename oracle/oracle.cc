@@ -637,6 +637,18 @@ int oracle_set_tabulated(oracle_handle* h, const double* fq, const double* gq) {
   return 0;
 }
 int oracle_quadrature_points(oracle_handle* h) { return h->o->tab.m; }
+// Velocity field b(x) at the quadrature points (layouts of Param::b_vol / b_face); NULL
+// b_vol restores the constant b of the configuration.
+int oracle_set_velocity(oracle_handle* h, const double* bv, const double* bf) {
+  auto& o = *h->o;
+  if (bv && !bf) return 1;
+  o.param.set_velocity(bv, bf, o.grid.n, o.tab.m, o.grid.ncells);
+  h->diag.clear();
+  h->blocks.clear();
+  h->bdiag.clear();
+  for (int t = 0; t < 2; ++t) { h->blu[t].clear(); h->bpiv[t].clear(); }
+  return 0;
+}
 
 int oracle_residual(oracle_handle* h, const double* x, double* r) {
   return guard(h, [&] { h->o->residual(x, r); });

@@ -103,6 +103,7 @@ def _bind(L):
         getattr(L, fn).argtypes = [vp, _dp]
     L.oracle_set_tabulated.argtypes = [vp, _dp, _dp]
     L.oracle_quadrature_points.argtypes = [vp]
+    L.oracle_set_velocity.argtypes = [vp, _dp, _dp]
     for fn in ("oracle_residual", "oracle_jacobian_apply", "oracle_jacobian_apply_transpose",
                "oracle_mass_apply", "oracle_grad_source"):
         getattr(L, fn).argtypes = [vp, _dp, _dp]
@@ -202,6 +203,15 @@ class Oracle:
 
     def quadrature_points(self):
         return self._L.oracle_quadrature_points(self.h)
+
+    def set_velocity_field(self, b_vol, b_face):
+        """b(x) at the quadrature points (None: the constant b of the configuration)."""
+        if b_vol is None:
+            self._check(self._L.oracle_set_velocity(self.h, None, None))
+            return
+        self._bv = np.ascontiguousarray(b_vol, dtype=np.float64)
+        self._bf = np.ascontiguousarray(b_face, dtype=np.float64)
+        self._check(self._L.oracle_set_velocity(self.h, _p(self._bv), _p(self._bf)))
 
     def residual(self, x, r=None):
         x = np.ascontiguousarray(x, dtype=np.float64)

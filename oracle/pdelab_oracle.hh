@@ -179,7 +179,8 @@ struct Grid {
 // ---------------------------------------------------------------------------
 // Parameter class: A = kappa*I, or a diagonal A = diag(a_x, a_y, a_z), constant per cell
 // (permeabilityIsConstantPerCell, convectiondiffusionparameter.hh:28-62,142-157),
-// b and c constant, f / g / bctype per the problem selector of pdelab_config.
+// c constant, b constant or a velocity field tabulated at the quadrature points,
+// f / g / bctype per the problem selector of pdelab_config.
 struct Param {
   pdelab_config cfg;
   std::vector<double> kappa;        // per cell, or a_x, a_y, a_z per cell (aniso)
@@ -189,6 +190,47 @@ struct Param {
   std::vector<double> cell_source;  // per cell, may be empty
   std::vector<double> f_tab;        // TABULATED: ncells * m^3
   std::vector<double> g_tab;        // TABULATED: boundary faces * m^2
+  // Velocity field b(x) at the quadrature points (empty: the constant cfg.b), the values
+  // param.b(cell, x) returns at the points where convectiondiffusiondg.hh evaluates it
+  // (:178 volume, :426 skeleton, :797 boundary):
+  //   b_vol:  per cell, the m^3 volume points (x fastest), (b_x, b_y, b_z) per point;
+  //   b_face: per face, the m^2 face points (tangential t1 < t2, t1 fastest), the component
+  //           b_d along +e_d of a face normal to d; faces of direction 0, then 1, then 2,
+  //           within a direction lexicographic in the face position (x fastest; the
+  //           position along d runs 0..n_d). On an interior face the skeleton evaluates b from
+  //           the inside cell, which the assembler's visit rule makes the higher-index cell.
+  std::vector<double> b_vol, b_face;
+  size_t bface_dir[4] = {0, 0, 0, 0};
+  int grid_n[3] = {0, 0, 0};
+  int bm = 0;  // points per direction of the tabulation
+  bool varb() const { return !b_vol.empty(); }
+  void set_velocity(const double* bv, const double* bf, const int n[3], int m, size_t ncells) {
+    if (!bv) { b_vol.clear(); b_face.clear(); return; }
+    for (int d = 0; d < 3; ++d) grid_n[d] = n[d];
+    bm = m;
+    bface_dir[0] = 0;
+    for (int d = 0; d < 3; ++d) {
+      size_t nfd = 1;
+      for (int e = 0; e < 3; ++e) nfd *= (size_t)(n[e] + (e == d ? 1 : 0));
+      bface_dir[d + 1] = bface_dir[d] + nfd;
+    }
+    b_vol.assign(bv, bv + ncells * (size_t)m * m * m * 3);
+    b_face.assign(bf, bf + bface_dir[3] * (size_t)m * m);
+  }
+  // b at volume point q of cell c (tabq = c m^3 + q), or the constant b
+  const double* b_at(size_t tabq) const { return varb() ? &b_vol[3 * tabq] : cfg.b; }
+  // b . n at face point q of the face of cell `cell` on side `side` (n = the side's outer
+  // normal of that cell)
+  double b_normal(const int cell[3], int side, int q) const {
+    const int d = side / 2;
+    const double ns = (side % 2) ? 1.0 : -1.0;
+    if (!varb()) return ns * cfg.b[d];
+    int pos[3] = {cell[0], cell[1], cell[2]};
+    pos[d] += side % 2;
+    const size_t e0 = (size_t)grid_n[0] + (d == 0), e1 = (size_t)grid_n[1] + (d == 1);
+    const size_t f = bface_dir[d] + (size_t)pos[0] + e0 * ((size_t)pos[1] + e1 * (size_t)pos[2]);
+    return ns * b_face[f * (size_t)bm * bm + q];
+  }
   int bctype(int side) const {
     int b = cfg.bctype[side];
     return b == 0 ? PDELAB_BC_DIRICHLET : b;
@@ -215,7 +257,8 @@ struct Param {
         double r2 = 0, bg = 0;
         for (int d = 0; d < 3; ++d) r2 += (x[d] - 0.5) * (x[d] - 0.5);
         double u = std::exp(-r2);
-        for (int d = 0; d < 3; ++d) bg += cfg.b[d] * (-2.0 * (x[d] - 0.5) * u);
+        const double* bb = b_at(tabq);
+        for (int d = 0; d < 3; ++d) bg += bb[d] * (-2.0 * (x[d] - 0.5) * u);
         val = cfg.kappa0 * (6.0 - 4.0 * r2) * u + bg + cfg.c * u;
         break;
       }
@@ -233,7 +276,8 @@ struct Param {
         const double* q = cfg.quad;
         double lap = 2.0 * (q[4] + q[5] + q[6]);
         double grad[3] = {q[1] + 2 * q[4] * x[0], q[2] + 2 * q[5] * x[1], q[3] + 2 * q[6] * x[2]};
-        double bg = cfg.b[0] * grad[0] + cfg.b[1] * grad[1] + cfg.b[2] * grad[2];
+        const double* bb = b_at(tabq);
+        double bg = bb[0] * grad[0] + bb[1] * grad[1] + bb[2] * grad[2];
         val = -cfg.kappa0 * lap + bg + cfg.c * exact(x);
         break;
       }
@@ -258,14 +302,14 @@ struct Param {
   // Neumann flux j(x) with (b u - A grad u).n = j and outflow data o(x) with
   // -(A grad u).n = o (convectiondiffusionparameter.hh:121-124, 196-208): the fluxes of the
   // analytic solution (A = kappa0 I) for EXP and QUADRATIC, the constants j0 / o0 otherwise,
-  // the tabulated boundary value for TABULATED. n is the unit outer normal.
-  double j(const double x[3], const double n[3], size_t tabq) const {
+  // the tabulated boundary value for TABULATED. n is the unit outer normal, bn = b.n at x.
+  double j(const double x[3], const double n[3], size_t tabq, double bn) const {
     if (cfg.problem == PDELAB_PROBLEM_TABULATED) return g_tab.empty() ? 0.0 : g_tab[tabq];
     if (cfg.problem == PDELAB_PROBLEM_EXP || cfg.problem == PDELAB_PROBLEM_QUADRATIC) {
       double gr[3];
       grad_exact(x, gr);
-      double bn = 0.0, gn = 0.0;
-      for (int d = 0; d < 3; ++d) { bn += cfg.b[d] * n[d]; gn += gr[d] * n[d]; }
+      double gn = 0.0;
+      for (int d = 0; d < 3; ++d) gn += gr[d] * n[d];
       return bn * exact(x) - cfg.kappa0 * gn;
     }
     return cfg.j0;
@@ -415,11 +459,11 @@ struct LocalOperator {
     const double det = ext[0] * ext[1] * ext[2];
     const double jit[3] = {1.0 / ext[0], 1.0 / ext[1], 1.0 / ext[2]};
     const RF A[3] = {kap(c, 0), kap(c, 1), kap(c, 2)};
-    const double* b = param.cfg.b;
     const double cc = param.cfg.c;
     std::vector<double> gradphi(3 * nloc);
     const int nq = tab.m * tab.m * tab.m;
     for (int q = 0; q < nq; ++q) {
+      const double* b = param.b_at(c * (size_t)nq + q);  // :178
       const double* phi = &tab.vphi[(size_t)q * nloc];
       const double* js = &tab.vgrad[(size_t)q * nloc * 3];
       double u = 0.0;
@@ -448,11 +492,11 @@ struct LocalOperator {
     const double det = ext[0] * ext[1] * ext[2];
     const double jit[3] = {1.0 / ext[0], 1.0 / ext[1], 1.0 / ext[2]};
     const RF A[3] = {kap(c, 0), kap(c, 1), kap(c, 2)};
-    const double* b = param.cfg.b;
     const double cc = param.cfg.c;
     std::vector<double> gradphi(3 * nloc);
     const int nq = tab.m * tab.m * tab.m;
     for (int q = 0; q < nq; ++q) {
+      const double* b = param.b_at(c * (size_t)nq + q);  // :255
       const double* phi = &tab.vphi[(size_t)q * nloc];
       const double* js = &tab.vgrad[(size_t)q * nloc * 3];
       for (int i = 0; i < nloc; ++i)
@@ -523,9 +567,10 @@ struct LocalOperator {
     grid.cell_box(cout, lo, eo);
     const double jis[3] = {1.0 / ei[0], 1.0 / ei[1], 1.0 / ei[2]};
     const double jin[3] = {1.0 / eo[0], 1.0 / eo[1], 1.0 / eo[2]};
-    const double* b = param.cfg.b;
     const double* n = fc.nF;
     const int nf = tab.m * tab.m;
+    int cijk[3];
+    grid.coords(cin, cijk);
     std::vector<double> tg_s(3 * nloc), tg_n(3 * nloc);
     for (int q = 0; q < nf; ++q) {
       size_t os = (size_t)fin * nf + q, on = (size_t)fout * nf + q;
@@ -542,7 +587,7 @@ struct LocalOperator {
       for (int i = 0; i < nloc; ++i)
         for (int e = 0; e < 3; ++e) { gu_s[e] += xs[i] * tg_s[3 * i + e]; gu_n[e] += xn[i] * tg_n[3 * i + e]; }
       // upwinding, :425-438
-      const double normalflux = b[0] * n[0] + b[1] * n[1] + b[2] * n[2];
+      const double normalflux = param.b_normal(cijk, fin, q);
       double up_s, up_n;
       if (normalflux >= 0.0) { up_s = 1.0; up_n = 0.0; } else { up_s = 0.0; up_n = 1.0; }
       const double factor = tab.fw[q] * fc.factor_area;
@@ -580,9 +625,10 @@ struct LocalOperator {
     grid.cell_box(cout, lo, eo);
     const double jis[3] = {1.0 / ei[0], 1.0 / ei[1], 1.0 / ei[2]};
     const double jin[3] = {1.0 / eo[0], 1.0 / eo[1], 1.0 / eo[2]};
-    const double* b = param.cfg.b;
     const double* n = fc.nF;
     const int nf = tab.m * tab.m;
+    int cijk[3];
+    grid.coords(cin, cijk);
     std::vector<double> ngs(nloc), ngn(nloc);  // n . grad phi (transformed)
     for (int q = 0; q < nf; ++q) {
       size_t os = (size_t)fin * nf + q, on = (size_t)fout * nf + q;
@@ -594,7 +640,7 @@ struct LocalOperator {
         ngs[i] = n[0] * jis[0] * gs[3 * i] + n[1] * jis[1] * gs[3 * i + 1] + n[2] * jis[2] * gs[3 * i + 2];
         ngn[i] = n[0] * jin[0] * gn[3 * i] + n[1] * jin[1] * gn[3 * i + 1] + n[2] * jin[2] * gn[3 * i + 2];
       }
-      const double normalflux = b[0] * n[0] + b[1] * n[1] + b[2] * n[2];
+      const double normalflux = param.b_normal(cijk, fin, q);  // :613
       double up_s, up_n;
       if (normalflux >= 0.0) { up_s = 1.0; up_n = 0.0; } else { up_s = 0.0; up_n = 1.0; }
       const double factor = tab.fw[q] * fc.factor_area;
@@ -668,7 +714,6 @@ struct LocalOperator {
     double lo[3], ext[3];
     grid.cell_box(c, lo, ext);
     const double ji[3] = {1.0 / ext[0], 1.0 / ext[1], 1.0 / ext[2]};
-    const double* b = param.cfg.b;
     const double* n = bc.nF;
     const int bct = param.bctype(side);
     const int nf = tab.m * tab.m;
@@ -685,17 +730,17 @@ struct LocalOperator {
       double xg[3];
       const double* xi = &tab.fxi[3 * o];
       for (int e = 0; e < 3; ++e) xg[e] = lo[e] + ext[e] * xi[e];
+      const double normalflux = param.b_normal(ijk, side, q);  // :797
       if (bct == PDELAB_BC_NEUMANN) {
         // :776-783, the flux data only
         if (!jac_apply) {
-          const double jv = param.j(xg, n, bface * nf + q);
+          const double jv = param.j(xg, n, bface * nf + q, normalflux);
           for (int i = 0; i < nloc; ++i) rs[i] += jv * phi[i] * factor;
         }
         continue;
       }
       double u = 0.0;
       for (int i = 0; i < nloc; ++i) u += xs[i] * phi[i];
-      const double normalflux = b[0] * n[0] + b[1] * n[1] + b[2] * n[2];
       if (bct == PDELAB_BC_OUTFLOW) {
         // :794-816
         if (normalflux < -1e-30) throw std::runtime_error("Outflow boundary condition on inflow!");
@@ -738,17 +783,18 @@ struct LocalOperator {
     double lo[3], ext[3];
     grid.cell_box(c, lo, ext);
     const double ji[3] = {1.0 / ext[0], 1.0 / ext[1], 1.0 / ext[2]};
-    const double* b = param.cfg.b;
     const double* n = bc.nF;
     const int bct = param.bctype(side);
     const int nf = tab.m * tab.m;
+    int ijk[3];
+    grid.coords(c, ijk);
     std::vector<double> ng(nloc);
     for (int q = 0; q < nf; ++q) {
       if (bct == PDELAB_BC_NONE || bct == PDELAB_BC_NEUMANN) continue;
       size_t o = (size_t)side * nf + q;
       const double* phi = &tab.fphi[o * nloc];
       const double factor = tab.fw[q] * bc.factor_area;
-      const double normalflux = b[0] * n[0] + b[1] * n[1] + b[2] * n[2];
+      const double normalflux = param.b_normal(ijk, side, q);  // :995
       if (bct == PDELAB_BC_OUTFLOW) {
         if (normalflux < -1e-30) throw std::runtime_error("Outflow boundary condition on inflow!");
         for (int j = 0; j < nloc; ++j)
