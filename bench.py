@@ -12,8 +12,9 @@ The JSON line also carries:
                 two warm-up launches (the GPU clock drops under sustained FP64 load);
   roofline_q2, apply_variants
                 the same HIP-event measurement for the 96^3 Q2 operators (constant kappa,
-                cell-wise lognormal kappa, convection) and the 64^3 Q3 cell-wise-kappa
-                operator, FP64 and FP32;
+                cell-wise lognormal kappa, convection with a constant b and with a velocity
+                field b(x) tabulated at the quadrature points) and the 64^3 Q3
+                cell-wise-kappa operator, FP64 and FP32;
   cpu_baseline  the oracle (CPU restatement of the reference quadrature-loop
                 algorithm, its -O3 build with FP contraction) timed on this host's cores on
                 the same apply, and on one core;
@@ -149,12 +150,19 @@ def cpu_baseline(cfg, min_seconds, gpu_cg_iterations=None, checks=None):
     return out
 
 
+def swirl(x, y, z):
+    """A divergence-free velocity field of the magnitude of the constant b = (1, 0.5, 0.25)."""
+    return (1.0 - 0.5 * (y - 0.5), 0.5 + 0.5 * (x - 0.5), 0.25 + 0.0 * z)
+
+
 APPLY_VARIANTS = [
-    # (label, grid cells per direction, degree, config keywords, cell-wise lognormal kappa)
-    ("q2_96_constant_kappa", 96, 2, dict(problem=1), False),
-    ("q2_96_lognormal_kappa", 96, 2, dict(problem=2, f0=1.0), True),
-    ("q2_96_convection", 96, 2, dict(kappa0=1e-2, b=(1.0, 0.5, 0.25)), False),
-    ("q3_64_lognormal_kappa", 64, 3, dict(problem=2, f0=1.0), True),
+    # (label, grid cells per direction, degree, config keywords, cell-wise lognormal kappa,
+    #  velocity field tabulated at the quadrature points or None)
+    ("q2_96_constant_kappa", 96, 2, dict(problem=1), False, None),
+    ("q2_96_lognormal_kappa", 96, 2, dict(problem=2, f0=1.0), True, None),
+    ("q2_96_convection", 96, 2, dict(kappa0=1e-2, b=(1.0, 0.5, 0.25)), False, None),
+    ("q2_96_velocity_field", 96, 2, dict(kappa0=1e-2), False, swirl),
+    ("q3_64_lognormal_kappa", 64, 3, dict(problem=2, f0=1.0), True, None),
 ]
 
 
@@ -162,21 +170,31 @@ def apply_variants_leg(pkg, reps=20):
     """FP64 and FP32 applies y = J z of the other operators on the path: the Q2 operators of the
     16^3 reference run, the adjoint and the convection-diffusion workloads at 96^3 (constant
     kappa, cell-wise lognormal kappa, convection), and the 64^3 Q3 cell-wise-kappa operator of the
-    mixed-precision solve. HIP-event average over `reps` launches after 3 warm-up launches.
-    Algorithmic bytes: 16 B/DoF FP64 (8 B/DoF FP32), plus 8 B (4 B) per cell for cell-wise kappa."""
+    mixed-precision solve, and the 96^3 Q2 convection-diffusion operator with a velocity field
+    b(x) tabulated at the quadrature points (two kernels: the apply with b = 0, then the
+    convection terms). HIP-event average over `reps` launches after 3 warm-up launches.
+    Algorithmic bytes: 16 B/DoF FP64 (8 B/DoF FP32), plus 8 B (4 B) per cell for cell-wise
+    kappa, plus the field (3 m^3 + 6 m^2 values per cell) for the velocity field."""
     out = {}
-    for label, n, k, kw, cellwise in APPLY_VARIANTS:
+    for label, n, k, kw, cellwise, field in APPLY_VARIANTS:
         cfg = pkg.make_config(n=n, degree=k, alpha=3.0, kappa_per_cell=cellwise, **kw)
         go = pkg.GridOperator(cfg, lognormal_kappa(n ** 3, 4242) if cellwise else None)
         N, nc = go.N, go.ncells
+        fvals = 0
+        if field is not None:
+            bv, bf = pkg.tabulate_velocity(cfg, field)
+            go.set_velocity_field(bv, bf)
+            m = pkg.quadrature_points(cfg)
+            fvals = 3 * m ** 3 + 6 * m ** 2
+            del bv, bf
         z = go.vector(splitmix_uniform(N, 777))
         y = go.vector()
         ms64 = go.time_apply(z, y, warmup=3, reps=reps)
         z32 = go.vector(splitmix_uniform(N, 777).astype(np.float32), dtype=np.float32)
         y32 = go.vector(dtype=np.float32)
         ms32 = go.time_apply(z32, y32, warmup=3, reps=reps)
-        b64 = 16.0 * N + (8.0 * nc if cellwise else 0.0)
-        b32 = 8.0 * N + (4.0 * nc if cellwise else 0.0)
+        b64 = 16.0 * N + (8.0 * nc if cellwise else 0.0) + 8.0 * fvals * nc
+        b32 = 8.0 * N + (4.0 * nc if cellwise else 0.0) + 4.0 * fvals * nc
         out[label] = {"dofs": N, "fp64_us": ms64 * 1e3, "fp64_GBs": b64 / (ms64 * 1e-3) / 1e9,
                       "fp64_frac": b64 / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS, "fp64_MDoFs": N / (ms64 * 1e-3) / 1e6,
                       "fp32_us": ms32 * 1e3, "fp32_GBs": b32 / (ms32 * 1e-3) / 1e9,

@@ -49,157 +49,192 @@ __device__ inline int dof3(const int (&i)[3]) {
   return i[0] + N1 * (i[1] + N1 * i[2]);
 }
 
-// y_c += (convection terms of cell c)(z). One wave per cell; `stop` as in the apply kernels
-// (a device-side Krylov loop that has stopped makes the launch a no-op).
+// Index of the DoF of face f's layer at tangential position (p1, p2).
+template <int N1>
+__device__ inline int face_dof(int f, int p1, int p2) {
+  const int d = f / 2;
+  int idx[3];
+  idx[d] = (f % 2) ? N1 - 1 : 0;
+  idx[d == 0 ? 1 : 0] = p1;
+  idx[d == 2 ? 1 : 2] = p2;
+  return dof3<N1>(idx);
+}
+
+// y_c += (convection terms of cell c)(z). One wave per cell. Every global load (the cell's
+// values, the six neighbour layers facing it, the field at the volume and face points) is
+// issued before the first barrier; the six faces are evaluated alongside the volume
+// contractions, so the cell takes six barriers. `stop` as in the apply kernels (a device-side
+// Krylov loop that has stopped makes the launch a no-op).
 template <int N1, int M, typename T>
 __global__ __launch_bounds__(64) void conv_var_kernel(const T* __restrict__ z, T* __restrict__ y,
                                                       const T* __restrict__ bv, const T* __restrict__ bf,
                                                       const VelArgs<T> a, const int* __restrict__ stop) {
   constexpr int L = N1 * N1, NL = L * N1, M2 = M * M, M3 = M2 * M;
-  __shared__ T zs[NL], r[NL];
+  __shared__ T zs[NL];
+  __shared__ T zt[6][L];        // neighbour layers (zero without a neighbour)
+  __shared__ T V[3][M3];        // the field at the volume points, then V_d
+  __shared__ T bq[6][M2];       // b_d at the face points
   __shared__ T A1[M * L], A2[M2 * N1];
-  __shared__ T V[3][M3];
   __shared__ T W[3][N1 * M2];
   __shared__ T S[2][L * M];
-  __shared__ T F[2][M * N1], fv[M2];
+  __shared__ T F[6][2][M * N1];  // traces interpolated along t1 (own, neighbour)
+  __shared__ T fv[6][M2];        // upwinded flux at the face points
+  __shared__ T G[6][N1 * M];     // flux tested along t1
+  __shared__ T rf[6][L];         // face contributions to the layer DoFs
+  __shared__ T tB[M][N1], tD[M][N1], tw[M];  // the 1D tables (lane-indexed reads from LDS)
   if (stop && *stop) return;
   const long long c = blockIdx.x;
   const int t = threadIdx.x;
   const int coord[3] = {(int)(c % a.n[0]), (int)((c / a.n[0]) % a.n[1]), (int)(c / ((long long)a.n[0] * a.n[1]))};
+  // face f takes part when it has a neighbour or a Dirichlet / outflow side (Neumann and
+  // None sides have no term in the apply); the same for every thread
+  unsigned interior = 0, active = 0;
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    const int d = f / 2;
+    const bool in = (f % 2) ? coord[d] < a.n[d] - 1 : coord[d] > 0;
+    if (in) interior |= 1u << f;
+    if (in || a.bc[f] == BC_DIRICHLET || a.bc[f] == BC_OUTFLOW) active |= 1u << f;
+  }
+  // ---- all global loads
+  for (int o = t; o < M * N1; o += 64) {
+    tB[o / N1][o % N1] = a.B[o / N1][o % N1];
+    tD[o / N1][o % N1] = a.Dq[o / N1][o % N1];
+  }
+  if (t < M) tw[t] = a.w[t];
   for (int o = t; o < NL; o += 64) zs[o] = z[c * NL + o];
+  for (int o = t; o < 6 * L; o += 64) {
+    const int f = o / L, pq = o % L;
+    T v = T(0);
+    if (interior & (1u << f)) {
+      const int d = f / 2;
+      const long long nb = c + ((f % 2) ? a.stride[d] : -a.stride[d]);
+      v = z[nb * NL + face_dof<N1>(f ^ 1, pq % N1, pq / N1)];
+    }
+    zt[f][pq] = v;
+  }
+  const T* bc = bv + c * 3 * M3;
+  for (int o = t; o < 3 * M3; o += 64) V[o / M3][o % M3] = bc[o];
+  const T* bfc = bf + c * 6 * M2;
+  for (int o = t; o < 6 * M2; o += 64) bq[o / M2][o % M2] = bfc[o];
   __syncthreads();
-  // ---- volume: interpolate to the points, x then y then z
+  // ---- phase 1: volume interpolation along x; face traces along t1
   for (int o = t; o < M * L; o += 64) {
     const int q0 = o % M, jk = o / M;
     T acc = T(0);
 #pragma unroll
-    for (int e = 0; e < N1; ++e) acc += a.B[q0][e] * zs[e + N1 * jk];
+    for (int e = 0; e < N1; ++e) acc += tB[q0][e] * zs[e + N1 * jk];
     A1[o] = acc;
   }
+  for (int o = t; o < 6 * M * N1; o += 64) {
+    const int f = o / (M * N1), r = o % (M * N1), q1 = r % M, p2 = r / M;
+    T as = T(0), an = T(0);
+#pragma unroll
+    for (int p1 = 0; p1 < N1; ++p1) {
+      as += tB[q1][p1] * zs[face_dof<N1>(f, p1, p2)];
+      an += tB[q1][p1] * zt[f][p1 + N1 * p2];
+    }
+    F[f][0][r] = as;
+    F[f][1][r] = an;
+  }
   __syncthreads();
+  // ---- phase 2: along y; the upwinded flux at the face points
   for (int o = t; o < M2 * N1; o += 64) {
     const int q0 = o % M, q1 = (o / M) % M, k = o / M2;
     T acc = T(0);
 #pragma unroll
-    for (int e = 0; e < N1; ++e) acc += a.B[q1][e] * A1[q0 + M * (e + N1 * k)];
+    for (int e = 0; e < N1; ++e) acc += tB[q1][e] * A1[q0 + M * (e + N1 * k)];
     A2[o] = acc;
   }
+  for (int o = t; o < 6 * M2; o += 64) {
+    const int f = o / M2, q = o % M2, q1 = q % M, q2 = q / M;
+    T v = T(0);
+    if (active & (1u << f)) {
+      T us = T(0), un = T(0);
+#pragma unroll
+      for (int p2 = 0; p2 < N1; ++p2) {
+        us += tB[q2][p2] * F[f][0][q1 + M * p2];
+        un += tB[q2][p2] * F[f][1][q1 + M * p2];
+      }
+      // (b . n) times the point's weight; a Dirichlet side has un = g = 0 in the apply, an
+      // outflow side takes the inside trace whatever the sign
+      const T cf = tw[q1] * tw[q2] * a.area[f / 2] * ((f % 2) ? bq[f][q] : -bq[f][q]);
+      const bool outflow = !(interior & (1u << f)) && a.bc[f] == BC_OUTFLOW;
+      v = (outflow || cf >= T(0)) ? cf * us : cf * un;
+    }
+    fv[f][q] = v;
+  }
   __syncthreads();
-  const T* bc = bv + c * 3 * M3;
+  // ---- phase 3: along z and the pointwise factors; the flux tested along t1
   for (int o = t; o < M3; o += 64) {
     const int q0 = o % M, q1 = (o / M) % M, q2 = o / M2;
     T u = T(0);
 #pragma unroll
-    for (int e = 0; e < N1; ++e) u += a.B[q2][e] * A2[q0 + M * (q1 + M * e)];
-    const T wu = a.w[q0] * a.w[q1] * a.w[q2] * u;
+    for (int e = 0; e < N1; ++e) u += tB[q2][e] * A2[q0 + M * (q1 + M * e)];
+    const T wu = tw[q0] * tw[q1] * tw[q2] * u;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) V[d][o] = a.vs[d] * bc[d * M3 + o] * wu;
+    for (int d = 0; d < 3; ++d) V[d][o] = a.vs[d] * V[d][o] * wu;
+  }
+  for (int o = t; o < 6 * N1 * M; o += 64) {
+    const int f = o / (N1 * M), r = o % (N1 * M), p1 = r % N1, q2 = r / N1;
+    T acc = T(0);
+#pragma unroll
+    for (int q1 = 0; q1 < M; ++q1) acc += tB[q1][p1] * fv[f][q1 + M * q2];
+    G[f][r] = acc;
   }
   __syncthreads();
-  // ---- volume: test with the gradients, x then y then z
+  // ---- phase 4: volume test along x; the flux tested along t2
   for (int o = t; o < N1 * M2; o += 64) {
     const int i = o % N1, q12 = o / N1;
     T w0 = T(0), w1 = T(0), w2 = T(0);
 #pragma unroll
     for (int q0 = 0; q0 < M; ++q0) {
-      w0 += a.Dq[q0][i] * V[0][q0 + M * q12];
-      w1 += a.B[q0][i] * V[1][q0 + M * q12];
-      w2 += a.B[q0][i] * V[2][q0 + M * q12];
+      w0 += tD[q0][i] * V[0][q0 + M * q12];
+      w1 += tB[q0][i] * V[1][q0 + M * q12];
+      w2 += tB[q0][i] * V[2][q0 + M * q12];
     }
     W[0][o] = w0;
     W[1][o] = w1;
     W[2][o] = w2;
   }
+  for (int o = t; o < 6 * L; o += 64) {
+    const int f = o / L, pq = o % L, p1 = pq % N1, p2 = pq / N1;
+    T acc = T(0);
+#pragma unroll
+    for (int q2 = 0; q2 < M; ++q2) acc += tB[q2][p2] * G[f][p1 + N1 * q2];
+    rf[f][pq] = acc;
+  }
   __syncthreads();
+  // ---- phase 5: along y
   for (int o = t; o < L * M; o += 64) {
     const int i = o % N1, j = (o / N1) % N1, q2 = o / L;
     T s0 = T(0), s1 = T(0);
 #pragma unroll
     for (int q1 = 0; q1 < M; ++q1) {
       const int wi = i + N1 * (q1 + M * q2);
-      s0 += a.B[q1][j] * W[0][wi] + a.Dq[q1][j] * W[1][wi];
-      s1 += a.B[q1][j] * W[2][wi];
+      s0 += tB[q1][j] * W[0][wi] + tD[q1][j] * W[1][wi];
+      s1 += tB[q1][j] * W[2][wi];
     }
     S[0][o] = s0;
     S[1][o] = s1;
   }
   __syncthreads();
+  // ---- phase 6: along z, plus the faces whose layer holds the DoF
   for (int o = t; o < NL; o += 64) {
     const int ij = o % L, k = o / L;
+    const int ii[3] = {o % N1, (o / N1) % N1, k};
     T acc = T(0);
 #pragma unroll
-    for (int q2 = 0; q2 < M; ++q2) acc += a.B[q2][k] * S[0][ij + L * q2] + a.Dq[q2][k] * S[1][ij + L * q2];
-    r[o] = acc;
+    for (int q2 = 0; q2 < M; ++q2) acc += tB[q2][k] * S[0][ij + L * q2] + tD[q2][k] * S[1][ij + L * q2];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+      const int d = f / 2;
+      if (ii[d] != ((f % 2) ? N1 - 1 : 0)) continue;
+      const int p1 = ii[d == 0 ? 1 : 0], p2 = ii[d == 2 ? 1 : 2];
+      acc += rf[f][p1 + N1 * p2];
+    }
+    y[c * NL + o] += acc;
   }
-  __syncthreads();
-  // ---- faces (the same for every thread of the block: the branches below are uniform)
-  for (int f = 0; f < 6; ++f) {
-    const int d = f / 2, up = f % 2;
-    const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
-    const bool interior = up ? coord[d] < a.n[d] - 1 : coord[d] > 0;
-    const int bct = interior ? 0 : a.bc[f];
-    if (!interior && bct != BC_DIRICHLET && bct != BC_OUTFLOW) continue;  // Neumann, None: no apply term
-    const long long nb = interior ? c + (up ? a.stride[d] : -a.stride[d]) : c;
-    const T* zn = z + nb * NL;
-    // traces along t1 interpolated: F[s][q1 + M p2] = sum_p1 B[q1][p1] u_s(p1, p2)
-    for (int o = t; o < M * N1; o += 64) {
-      const int q1 = o % M, p2 = o / M;
-      T as = T(0), an = T(0);
-#pragma unroll
-      for (int p1 = 0; p1 < N1; ++p1) {
-        int idx[3];
-        idx[d] = up ? N1 - 1 : 0;
-        idx[t1] = p1;
-        idx[t2] = p2;
-        as += a.B[q1][p1] * zs[dof3<N1>(idx)];
-        if (interior) {
-          idx[d] = up ? 0 : N1 - 1;
-          an += a.B[q1][p1] * zn[dof3<N1>(idx)];
-        }
-      }
-      F[0][o] = as;
-      F[1][o] = an;
-    }
-    __syncthreads();
-    const T* bfc = bf + (c * 6 + f) * M2;
-    const T ns = up ? T(1) : T(-1);
-    for (int o = t; o < M2; o += 64) {
-      const int q1 = o % M, q2 = o / M;
-      T us = T(0), un = T(0);
-#pragma unroll
-      for (int p2 = 0; p2 < N1; ++p2) {
-        us += a.B[q2][p2] * F[0][q1 + M * p2];
-        un += a.B[q2][p2] * F[1][q1 + M * p2];
-      }
-      const T cf = a.w[q1] * a.w[q2] * a.area[d] * (ns * bfc[o]);  // (b . n) times the point's weight
-      T v;
-      if (bct == BC_OUTFLOW) v = cf * us;
-      else v = cf >= T(0) ? cf * us : cf * un;  // a Dirichlet side has un = g = 0 in the apply
-      fv[o] = v;
-    }
-    __syncthreads();
-    for (int o = t; o < N1 * M; o += 64) {
-      const int p1 = o % N1, q2 = o / N1;
-      T acc = T(0);
-#pragma unroll
-      for (int q1 = 0; q1 < M; ++q1) acc += a.B[q1][p1] * fv[q1 + M * q2];
-      F[0][o] = acc;
-    }
-    __syncthreads();
-    for (int o = t; o < L; o += 64) {
-      const int p1 = o % N1, p2 = o / N1;
-      T acc = T(0);
-#pragma unroll
-      for (int q2 = 0; q2 < M; ++q2) acc += a.B[q2][p2] * F[0][p1 + N1 * q2];
-      int idx[3];
-      idx[d] = up ? N1 - 1 : 0;
-      idx[t1] = p1;
-      idx[t2] = p2;
-      r[dof3<N1>(idx)] += acc;
-    }
-    __syncthreads();
-  }
-  for (int o = t; o < NL; o += 64) y[c * NL + o] += r[o];
 }
 
 // diag += the convection terms' diagonal entries (the coefficient of z_i in the row of

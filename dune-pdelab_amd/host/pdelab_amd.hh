@@ -332,12 +332,14 @@ class GridOperator {
               throw NotImplemented("GridOperator: the diffusion tensor must be diagonal on this path");
         for (int d = 0; d < 3; ++d) kappa[3 * c + d] = A[d][d];
       }
-      auto bc = par.b(e, centre);
-      if (bc != b0 || par.c(e, centre) != c0)
-        throw NotImplemented("GridOperator: b and c must be constant over the grid on this path");
+      if (par.c(e, centre) != c0) throw NotImplemented("GridOperator: c must be constant over the grid on this path");
     }
     cfg.kappa_per_cell = scalarA ? 1 : 2;
-    for (int d = 0; d < 3; ++d) cfg.b[d] = b0[d];
+    // b at every quadrature point (convectiondiffusiondg.hh:178, 426, 797): a constant b is
+    // passed in the configuration, a varying one as the tabulated velocity field
+    tabulate_velocity(grid, par, 2 * cfg.degree + cfg.intorderadd);
+    const bool bconst = velocity_constant(b0);
+    for (int d = 0; d < 3; ++d) cfg.b[d] = bconst ? b0[d] : 0.0;
     cfg.c = c0;
     for (int s = 0; s < 6; ++s) {
       Intersection is{s, detail::element(grid, 0)};
@@ -346,6 +348,9 @@ class GridOperator {
     }
     cfg.problem = PDELAB_PROBLEM_TABULATED;
     check(pdelab_gpu_create(&cfg, kappa.data(), &h_), nullptr);
+    if (!bconst) check(pdelab_gpu_set_velocity_field(h_, bvol_.data(), bface_.data()), h_);
+    std::vector<double>().swap(bvol_);
+    std::vector<double>().swap(bface_);
     tabulate(grid, par);
   }
   ~GridOperator() {
@@ -374,6 +379,65 @@ class GridOperator {
   }
 
  private:
+  // b(e, x) at the volume and face Gauss points of the rule of the given order, in the
+  // layouts of pdelab_gpu_set_velocity_field: an interior face is evaluated from the cell
+  // the reference's assembler visits it from, the one with the higher index (its lower face,
+  // default/assembler.hh:181), a boundary face from its only cell.
+  template <class P>
+  void tabulate_velocity(const StructuredGrid& grid, const P& par, int order) {
+    const int m = order / 2 + 1;
+    std::vector<double> qx, qw;
+    detail::gauss01(m, qx, qw);
+    const size_t nc = grid.size();
+    bvol_.resize(nc * m * m * m * 3);
+    for (size_t c = 0; c < nc; ++c) {
+      Element e = detail::element(grid, c);
+      for (int q2 = 0; q2 < m; ++q2)
+        for (int q1 = 0; q1 < m; ++q1)
+          for (int q0 = 0; q0 < m; ++q0) {
+            const auto b = par.b(e, {qx[q0], qx[q1], qx[q2]});
+            const size_t o = 3 * (c * m * m * m + q0 + m * (q1 + m * q2));
+            for (int d = 0; d < 3; ++d) bvol_[o + d] = b[d];
+          }
+    }
+    bface_.clear();
+    const int* n = grid.cells.data();
+    for (int d = 0; d < 3; ++d) {
+      const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+      int ext[3] = {n[0], n[1], n[2]};
+      ext[d] += 1;
+      face_count_[d] = (size_t)ext[0] * ext[1] * ext[2] * m * m;
+      for (int p2 = 0; p2 < ext[2]; ++p2)
+        for (int p1 = 0; p1 < ext[1]; ++p1)
+          for (int p0 = 0; p0 < ext[0]; ++p0) {
+            int ijk[3] = {p0, p1, p2};
+            const bool top = ijk[d] == n[d];  // the upper boundary face: its cell lies below
+            if (top) ijk[d] -= 1;
+            const size_t c = (size_t)ijk[0] + (size_t)n[0] * ((size_t)ijk[1] + (size_t)n[1] * ijk[2]);
+            Element e = detail::element(grid, c);
+            for (int q2 = 0; q2 < m; ++q2)
+              for (int q1 = 0; q1 < m; ++q1) {
+                std::array<double, 3> xi;
+                xi[d] = top ? 1.0 : 0.0;
+                xi[t1] = qx[q1];
+                xi[t2] = qx[q2];
+                bface_.push_back(par.b(e, xi)[d]);
+              }
+          }
+    }
+  }
+  // whether every tabulated value equals b0 (then the constant-coefficient path applies)
+  template <class B>
+  bool velocity_constant(const B& b0) const {
+    for (size_t i = 0; i < bvol_.size(); ++i)
+      if (bvol_[i] != b0[i % 3]) return false;
+    size_t o = 0;
+    for (int d = 0; d < 3; ++d)
+      for (size_t i = 0; i < face_count_[d]; ++i, ++o)
+        if (bface_[o] != b0[d]) return false;
+    return true;
+  }
+
   template <class P>
   void tabulate(const StructuredGrid& grid, const P& par) {
     const int m = pdelab_gpu_quadrature_points(h_);
@@ -423,6 +487,8 @@ class GridOperator {
   const GFSV& gfsv_;
   const LOP& lop_;
   int cfg_bctype_[6] = {0, 0, 0, 0, 0, 0};
+  std::vector<double> bvol_, bface_;  // the tabulated velocity field, until the handle has it
+  size_t face_count_[3] = {0, 0, 0};  // values of bface_ per face direction
   LocalAssembler la_;
   pdelab_gpu_gridoperator* h_ = nullptr;
 };

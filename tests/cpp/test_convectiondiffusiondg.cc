@@ -46,6 +46,30 @@ struct TensorProblem : LognormalProblem {
   }
 };
 
+// u = 0.3 + x + 2y - z + x^2 + 1.5 y^2 - 2 z^2 with the divergence-free linear velocity
+// b = (0.5 - (y - 1/2) + 0.3 z, 0.25 + (x - 1/2) - 0.2 z, 0.2 + 0.5 (x - 1/2) - 0.3 (y - 1/2)),
+// kappa 0.7: f = -kappa lap u + b.grad u, g = u. Every integral of the Q2 scheme is exact, so
+// the discrete solution is the interpolant of u: the tabulated velocity field end to end.
+struct VelocityFieldProblem : ConvectionDiffusionModelProblem {
+  static double u(const std::array<double, 3>& x) {
+    return 0.3 + x[0] + 2 * x[1] - x[2] + x[0] * x[0] + 1.5 * x[1] * x[1] - 2 * x[2] * x[2];
+  }
+  static std::array<double, 3> bfield(const std::array<double, 3>& x) {
+    return {0.5 - (x[1] - 0.5) + 0.3 * x[2], 0.25 + (x[0] - 0.5) - 0.2 * x[2], 0.2 + 0.5 * (x[0] - 0.5) - 0.3 * (x[1] - 0.5)};
+  }
+  double A(const Element&, const std::array<double, 3>&) const { return 0.7; }
+  std::array<double, 3> b(const Element& e, const std::array<double, 3>& xl) const {
+    return bfield(e.geometry().global(xl));
+  }
+  double f(const Element& e, const std::array<double, 3>& xl) const {
+    const auto x = e.geometry().global(xl);
+    const auto bb = bfield(x);
+    const double gu[3] = {1 + 2 * x[0], 2 + 3 * x[1], -1 - 4 * x[2]};
+    return -0.7 * (2 + 3 - 4) + bb[0] * gu[0] + bb[1] * gu[1] + bb[2] * gu[2];
+  }
+  double g(const Element& e, const std::array<double, 3>& xl) const { return u(e.geometry().global(xl)); }
+};
+
 int main() {
   int fails = 0;
   try {
@@ -248,6 +272,28 @@ int main() {
       }
       std::printf("full tensor refused: %d\n", (int)refused);
       if (!refused) ++fails;
+    }
+
+    // --- a velocity field b(x) given by the parameter class, tabulated at the quadrature points
+    {
+      StructuredGrid gridv({1.0, 1.0, 1.0}, {6, 5, 4});
+      using FEM2 = QkDGLocalFiniteElementMap<double, double, 2, 3>;
+      FEM2 fem2;
+      GridFunctionSpace<FEM2> gfsv(gridv, fem2);
+      VelocityFieldProblem vp;
+      using LOPV = ConvectionDiffusionDG<VelocityFieldProblem, FEM2>;
+      LOPV lopv(vp, ConvectionDiffusionDGMethod::SIPG, ConvectionDiffusionDGWeights::weightsOn, 4.0);
+      using GOV = GridOperator<GridFunctionSpace<FEM2>, GridFunctionSpace<FEM2>, LOPV>;
+      GOV gov(gfsv, gfsv, lopv);
+      auto xv = gov.makeVector();
+      xv = 0.0;
+      ISTLBackend_SEQ_MatrixFree_BCGS_Jac<GOV> lsv(gov, 5000, 0);
+      StationaryLinearProblemSolver<GOV, decltype(lsv), GOV::Domain> sv(gov, lsv, xv, 1e-13);
+      sv.apply();
+      const double ev = l2_error_squared(gfsv, xv.download(), VelocityFieldProblem::u, 10);
+      std::printf("velocity field b(x), Q2 BiCGSTAB: l2errorsquared %g (iterations %u, converged %d)\n", ev,
+                  sv.result().iterations, (int)sv.result().converged);
+      if (!(ev < 1e-18) || !sv.result().converged) ++fails;
     }
   } catch (const std::exception& e) {
     std::fprintf(stderr, "error: %s\n", e.what());
