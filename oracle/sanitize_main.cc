@@ -20,6 +20,7 @@ int oracle_set_kappa(oracle_handle* h, const double* kappa);
 int oracle_set_cell_source(oracle_handle* h, const double* s);
 int oracle_set_tabulated(oracle_handle* h, const double* fq, const double* gq);
 int oracle_quadrature_points(oracle_handle* h);
+int oracle_set_velocity(oracle_handle* h, const double* bv, const double* bf);
 int oracle_residual(oracle_handle* h, const double* x, double* r);
 int oracle_residual_twosided(oracle_handle* h, const double* x, double* r, int nthreads);
 int oracle_jacobian_apply(oracle_handle* h, const double* z, double* y);
@@ -161,6 +162,25 @@ int main() {
     std::vector<double> fq(8 * m * m * m, 1.0), gq(6 * 4 * m * m, 0.5), x(oracle_size(h), 0.0), r(oracle_size(h));
     check(oracle_set_tabulated(h, fq.data(), gq.data()), h, "set_tabulated");
     check(oracle_residual(h, x.data(), r.data()), h, "residual tabulated");
+    oracle_destroy(h);
+  }
+  // a velocity field at the quadrature points (3 x 2 x 2 cells, Q2, outflow on +x)
+  {
+    pdelab_config c = base(3, 2, 2, 2);
+    c.bctype[1] = PDELAB_BC_OUTFLOW;
+    c.bctype[2] = PDELAB_BC_NEUMANN;
+    oracle_handle* h = oracle_create(&c, nullptr);
+    const size_t m = oracle_quadrature_points(h), nf = 4 * 2 * 2 + 3 * 3 * 2 + 3 * 2 * 3;
+    std::vector<double> bv(12 * m * m * m * 3), bf(nf * m * m, 0.75), x(oracle_size(h)), y(oracle_size(h)),
+        d(oracle_size(h));
+    for (size_t i = 0; i < bv.size(); ++i) bv[i] = 0.5 + 0.25 * std::sin(0.1 * (double)i);
+    for (size_t i = 0; i < x.size(); ++i) x[i] = std::cos(0.3 * (double)i);
+    check(oracle_set_velocity(h, bv.data(), bf.data()), h, "set_velocity");
+    check(oracle_residual(h, x.data(), y.data()), h, "residual with a velocity field");
+    check(oracle_jacobian_apply(h, x.data(), y.data()), h, "apply with a velocity field");
+    check(oracle_jacobian_apply_twosided(h, x.data(), y.data(), 2), h, "two-sided apply with a velocity field");
+    check(oracle_point_diagonal(h, d.data()), h, "point diagonal with a velocity field");
+    check(oracle_set_velocity(h, nullptr, nullptr), h, "clear the velocity field");
     oracle_destroy(h);
   }
   if (failures) {
