@@ -60,12 +60,18 @@ __device__ inline int face_dof(int f, int p1, int p2) {
   return dof3<N1>(idx);
 }
 
-// y_c += (convection terms of cell c)(z). One wave per cell. Every global load (the cell's
-// values, the six neighbour layers facing it, the field at the volume and face points) is
-// issued before the first barrier; the six faces are evaluated alongside the volume
-// contractions, so the cell takes six barriers. `stop` as in the apply kernels (a device-side
-// Krylov loop that has stopped makes the launch a no-op).
-template <int N1, int M, typename T>
+// y_c += (convection terms of cell c)(z), or with TRANS their transpose. One wave per cell.
+// Every global load (the cell's values, the six neighbour layers facing it, the field at the
+// volume and face points) is issued before the first barrier; the six faces are evaluated
+// alongside the volume contractions, so the cell takes six barriers. `stop` as in the apply
+// kernels (a device-side Krylov loop that has stopped makes the launch a no-op).
+//
+// The transpose: the volume term -int phi_j b . grad z, i.e. the gradient of z interpolated to
+// the points (D in direction d, B elsewhere), P = -w |K| sum_d b_d (dz/dx_d), tested with
+// B (x) B (x) B; a face, whose forward entries are [cf >= 0] cf psi_s psi_s (own column) and
+// [cf > 0] (-cf) psi_n psi_s (the neighbour's row), gives [cf >= 0] cf (z_s - z_n) at each
+// point (z_n = 0 on a Dirichlet side; an outflow side always takes cf z_s).
+template <int N1, int M, typename T, bool TRANS = false>
 __global__ __launch_bounds__(64) void conv_var_kernel(const T* __restrict__ z, T* __restrict__ y,
                                                       const T* __restrict__ bv, const T* __restrict__ bf,
                                                       const VelArgs<T> a, const int* __restrict__ stop) {
@@ -74,7 +80,7 @@ __global__ __launch_bounds__(64) void conv_var_kernel(const T* __restrict__ z, T
   __shared__ T zt[6][L];        // neighbour layers (zero without a neighbour)
   __shared__ T V[3][M3];        // the field at the volume points, then V_d
   __shared__ T bq[6][M2];       // b_d at the face points
-  __shared__ T A1[M * L], A2[M2 * N1];
+  __shared__ T A1[TRANS ? 2 : 1][M * L], A2[TRANS ? 3 : 1][M2 * N1];
   __shared__ T W[3][N1 * M2];
   __shared__ T S[2][L * M];
   __shared__ T F[6][2][M * N1];  // traces interpolated along t1 (own, neighbour)
@@ -121,10 +127,14 @@ __global__ __launch_bounds__(64) void conv_var_kernel(const T* __restrict__ z, T
   // ---- phase 1: volume interpolation along x; face traces along t1
   for (int o = t; o < M * L; o += 64) {
     const int q0 = o % M, jk = o / M;
-    T acc = T(0);
+    T acc = T(0), accd = T(0);
 #pragma unroll
-    for (int e = 0; e < N1; ++e) acc += tB[q0][e] * zs[e + N1 * jk];
-    A1[o] = acc;
+    for (int e = 0; e < N1; ++e) {
+      acc += tB[q0][e] * zs[e + N1 * jk];
+      if (TRANS) accd += tD[q0][e] * zs[e + N1 * jk];
+    }
+    A1[0][o] = acc;
+    if (TRANS) A1[1][o] = accd;
   }
   for (int o = t; o < 6 * M * N1; o += 64) {
     const int f = o / (M * N1), r = o % (M * N1), q1 = r % M, p2 = r / M;
@@ -141,10 +151,21 @@ __global__ __launch_bounds__(64) void conv_var_kernel(const T* __restrict__ z, T
   // ---- phase 2: along y; the upwinded flux at the face points
   for (int o = t; o < M2 * N1; o += 64) {
     const int q0 = o % M, q1 = (o / M) % M, k = o / M2;
-    T acc = T(0);
+    T acc = T(0), g0 = T(0), g1 = T(0);
 #pragma unroll
-    for (int e = 0; e < N1; ++e) acc += tB[q1][e] * A1[q0 + M * (e + N1 * k)];
-    A2[o] = acc;
+    for (int e = 0; e < N1; ++e) {
+      const int ai = q0 + M * (e + N1 * k);
+      acc += tB[q1][e] * A1[0][ai];
+      if (TRANS) {
+        g0 += tB[q1][e] * A1[1][ai];  // d/dx
+        g1 += tD[q1][e] * A1[0][ai];  // d/dy
+      }
+    }
+    A2[0][o] = acc;
+    if (TRANS) {
+      A2[1][o] = g0;
+      A2[2][o] = g1;
+    }
   }
   for (int o = t; o < 6 * M2; o += 64) {
     const int f = o / M2, q = o % M2, q1 = q % M, q2 = q / M;
@@ -160,7 +181,8 @@ __global__ __launch_bounds__(64) void conv_var_kernel(const T* __restrict__ z, T
       // outflow side takes the inside trace whatever the sign
       const T cf = tw[q1] * tw[q2] * a.area[f / 2] * ((f % 2) ? bq[f][q] : -bq[f][q]);
       const bool outflow = !(interior & (1u << f)) && a.bc[f] == BC_OUTFLOW;
-      v = (outflow || cf >= T(0)) ? cf * us : cf * un;
+      if (TRANS) v = outflow ? cf * us : (cf >= T(0) ? cf * (us - un) : T(0));
+      else v = (outflow || cf >= T(0)) ? cf * us : cf * un;
     }
     fv[f][q] = v;
   }
@@ -168,12 +190,25 @@ __global__ __launch_bounds__(64) void conv_var_kernel(const T* __restrict__ z, T
   // ---- phase 3: along z and the pointwise factors; the flux tested along t1
   for (int o = t; o < M3; o += 64) {
     const int q0 = o % M, q1 = (o / M) % M, q2 = o / M2;
-    T u = T(0);
+    const T wq = tw[q0] * tw[q1] * tw[q2];
+    if (TRANS) {
+      T gx = T(0), gy = T(0), gz = T(0);
 #pragma unroll
-    for (int e = 0; e < N1; ++e) u += tB[q2][e] * A2[q0 + M * (q1 + M * e)];
-    const T wu = tw[q0] * tw[q1] * tw[q2] * u;
+      for (int e = 0; e < N1; ++e) {
+        const int ai = q0 + M * (q1 + M * e);
+        gx += tB[q2][e] * A2[1][ai];
+        gy += tB[q2][e] * A2[2][ai];
+        gz += tD[q2][e] * A2[0][ai];
+      }
+      V[0][o] = wq * (a.vs[0] * V[0][o] * gx + a.vs[1] * V[1][o] * gy + a.vs[2] * V[2][o] * gz);
+    } else {
+      T u = T(0);
 #pragma unroll
-    for (int d = 0; d < 3; ++d) V[d][o] = a.vs[d] * V[d][o] * wu;
+      for (int e = 0; e < N1; ++e) u += tB[q2][e] * A2[0][q0 + M * (q1 + M * e)];
+      const T wu = wq * u;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) V[d][o] = a.vs[d] * V[d][o] * wu;
+    }
   }
   for (int o = t; o < 6 * N1 * M; o += 64) {
     const int f = o / (N1 * M), r = o % (N1 * M), p1 = r % N1, q2 = r / N1;
@@ -189,13 +224,19 @@ __global__ __launch_bounds__(64) void conv_var_kernel(const T* __restrict__ z, T
     T w0 = T(0), w1 = T(0), w2 = T(0);
 #pragma unroll
     for (int q0 = 0; q0 < M; ++q0) {
-      w0 += tD[q0][i] * V[0][q0 + M * q12];
-      w1 += tB[q0][i] * V[1][q0 + M * q12];
-      w2 += tB[q0][i] * V[2][q0 + M * q12];
+      if (TRANS) {
+        w0 += tB[q0][i] * V[0][q0 + M * q12];
+      } else {
+        w0 += tD[q0][i] * V[0][q0 + M * q12];
+        w1 += tB[q0][i] * V[1][q0 + M * q12];
+        w2 += tB[q0][i] * V[2][q0 + M * q12];
+      }
     }
     W[0][o] = w0;
-    W[1][o] = w1;
-    W[2][o] = w2;
+    if (!TRANS) {
+      W[1][o] = w1;
+      W[2][o] = w2;
+    }
   }
   for (int o = t; o < 6 * L; o += 64) {
     const int f = o / L, pq = o % L, p1 = pq % N1, p2 = pq / N1;
@@ -212,8 +253,12 @@ __global__ __launch_bounds__(64) void conv_var_kernel(const T* __restrict__ z, T
 #pragma unroll
     for (int q1 = 0; q1 < M; ++q1) {
       const int wi = i + N1 * (q1 + M * q2);
-      s0 += tB[q1][j] * W[0][wi] + tD[q1][j] * W[1][wi];
-      s1 += tB[q1][j] * W[2][wi];
+      if (TRANS) {
+        s0 += tB[q1][j] * W[0][wi];
+      } else {
+        s0 += tB[q1][j] * W[0][wi] + tD[q1][j] * W[1][wi];
+        s1 += tB[q1][j] * W[2][wi];
+      }
     }
     S[0][o] = s0;
     S[1][o] = s1;
@@ -225,7 +270,10 @@ __global__ __launch_bounds__(64) void conv_var_kernel(const T* __restrict__ z, T
     const int ii[3] = {o % N1, (o / N1) % N1, k};
     T acc = T(0);
 #pragma unroll
-    for (int q2 = 0; q2 < M; ++q2) acc += tB[q2][k] * S[0][ij + L * q2] + tD[q2][k] * S[1][ij + L * q2];
+    for (int q2 = 0; q2 < M; ++q2) {
+      acc += tB[q2][k] * S[0][ij + L * q2];
+      if (!TRANS) acc += tD[q2][k] * S[1][ij + L * q2];
+    }
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
       const int d = f / 2;

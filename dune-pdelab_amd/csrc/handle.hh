@@ -230,11 +230,11 @@ int ibj_launch(H* h, const T* d, T* v, int* err, unsigned long long* iters, cons
 int jac_assemble(H* h, bool trans, double* blocks, long long* cols);
 int jac_block_apply(H* h, const double* blocks, const long long* cols, const double* z, double* y);
 // Velocity field (velocity.hip): set or clear the tabulated b(x) (host arrays in the layout of
-// pdelab_gpu_set_velocity_field), add its convection terms y += C z (nothing stored when
-// *stop is set) and their diagonal, free the data.
+// pdelab_gpu_set_velocity_field), add its convection terms y += C z or y += C^T z (nothing
+// stored when *stop is set) and their diagonal, free the data.
 int vel_set(H* h, const double* b_vol, const double* b_face);
 template <typename T>
-int vel_launch(H* h, const T* z, T* y, const int* stop);
+int vel_launch(H* h, const T* z, T* y, const int* stop, bool trans);
 int vel_diag(H* h, double* diag);
 void vel_free(H* h);
 

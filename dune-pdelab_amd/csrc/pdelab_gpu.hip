@@ -333,10 +333,8 @@ template <typename T>
 static int apply_mode(H* h, const T* z, T* y, bool trans, int mode, T* dotp = nullptr,
                       const int* stop = nullptr) {
   CHK(check_outflow(h));
-  if (h->varb) {
-    if (trans) return fail(h, PDELAB_ERR_UNSUPPORTED, "velocity field: the transposed operator is not built");
-    if (mode == 2) return fail(h, PDELAB_ERR_UNSUPPORTED, "velocity field: CG (non-symmetric operator), use BiCGSTAB");
-  }
+  if (h->varb && mode == 2)
+    return fail(h, PDELAB_ERR_UNSUPPORTED, "velocity field: CG (non-symmetric operator), use BiCGSTAB");
   int st = PDELAB_ERR_UNSUPPORTED;
   switch (h->n1) {
     case 2: st = launch_apply_n<2, T>(h, z, y, fc_for<T>(h), args_for<T>(h), trans, mode, dotp, stop); break;
@@ -345,7 +343,7 @@ static int apply_mode(H* h, const T* z, T* y, bool trans, int mode, T* dotp = nu
     default: return fail(h, PDELAB_ERR_UNSUPPORTED, "degree");
   }
   CHK(st);
-  return vel_launch<T>(h, z, y, stop);  // the velocity field's convection terms, if set
+  return vel_launch<T>(h, z, y, stop, trans);  // the velocity field's convection terms, if set
 }
 static int apply64(H* h, const double* z, double* y, bool trans, bool accum) {
   return apply_mode<double>(h, z, y, trans, accum ? 1 : 0);

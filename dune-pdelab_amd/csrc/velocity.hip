@@ -137,26 +137,30 @@ template <>
 const VelArgs<float>& va_for<float>(const H* h) { return h->va32; }
 
 template <int N1, int M, typename T>
-static void vel_launch_nm(H* h, const T* z, T* y, const int* stop) {
-  hipLaunchKernelGGL((conv_var_kernel<N1, M, T>), dim3((unsigned)h->ncells), dim3(64), 0, h->stream, z, y,
-                     bv_for<T>(h), bf_for<T>(h), va_for<T>(h), stop);
+static void vel_launch_nm(H* h, const T* z, T* y, const int* stop, bool trans) {
+  if (trans)
+    hipLaunchKernelGGL((conv_var_kernel<N1, M, T, true>), dim3((unsigned)h->ncells), dim3(64), 0, h->stream, z, y,
+                       bv_for<T>(h), bf_for<T>(h), va_for<T>(h), stop);
+  else
+    hipLaunchKernelGGL((conv_var_kernel<N1, M, T, false>), dim3((unsigned)h->ncells), dim3(64), 0, h->stream, z, y,
+                       bv_for<T>(h), bf_for<T>(h), va_for<T>(h), stop);
 }
 
 template <typename T>
-int vel_launch(H* h, const T* z, T* y, const int* stop) {
+int vel_launch(H* h, const T* z, T* y, const int* stop, bool trans) {
   if (!h->varb) return PDELAB_OK;
   const bool m1 = h->tab.m == h->n1;
   switch (h->n1) {
-    case 2: m1 ? vel_launch_nm<2, 2, T>(h, z, y, stop) : vel_launch_nm<2, 3, T>(h, z, y, stop); break;
-    case 3: m1 ? vel_launch_nm<3, 3, T>(h, z, y, stop) : vel_launch_nm<3, 4, T>(h, z, y, stop); break;
-    case 4: m1 ? vel_launch_nm<4, 4, T>(h, z, y, stop) : vel_launch_nm<4, 5, T>(h, z, y, stop); break;
+    case 2: m1 ? vel_launch_nm<2, 2, T>(h, z, y, stop, trans) : vel_launch_nm<2, 3, T>(h, z, y, stop, trans); break;
+    case 3: m1 ? vel_launch_nm<3, 3, T>(h, z, y, stop, trans) : vel_launch_nm<3, 4, T>(h, z, y, stop, trans); break;
+    case 4: m1 ? vel_launch_nm<4, 4, T>(h, z, y, stop, trans) : vel_launch_nm<4, 5, T>(h, z, y, stop, trans); break;
     default: return fail(h, PDELAB_ERR_UNSUPPORTED, "degree");
   }
   HIPCHK(h, hipGetLastError());
   return PDELAB_OK;
 }
-template int vel_launch<double>(H*, const double*, double*, const int*);
-template int vel_launch<float>(H*, const float*, float*, const int*);
+template int vel_launch<double>(H*, const double*, double*, const int*, bool);
+template int vel_launch<float>(H*, const float*, float*, const int*, bool);
 
 template <int N1, int M>
 static void vel_diag_nm(H* h, double* diag) {
