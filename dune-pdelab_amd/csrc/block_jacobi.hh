@@ -25,6 +25,7 @@
 #include "block_reduce.hh"
 #include "stream_io.hh"
 #include "dg_kernels.hh"
+#include "dg_velocity.hh"
 
 namespace pdelab_amd {
 
@@ -275,10 +276,16 @@ __global__ __launch_bounds__(256) void bj_fdm_setup_kernel(const DGArgs<double> 
 // Dense blocks, one workgroup per entry (boundary-type triple tx + 4 ty + 16 tz, or cell).
 // out_mode 0: D^{-1} transposed (DT[j][i] = D^{-1}[i][j]) into out64/out32;
 // out_mode 1: the block D itself, row-major, into out64 (per cell only).
-template <int N1, bool TRANS, bool CONV, bool VARK>
+// VARB: a velocity field b(x) at the quadrature points (va, bv, bf; per cell): the block is
+// D = (M (x) M (x) M) K + C_c with C_c the cell block of its convection terms
+// (conv_block_entry, transposed for J^T), formed in place of K before the inversion.
+template <int N1, bool TRANS, bool CONV, bool VARK, bool VARB = false>
 __global__ __launch_bounds__(256) void bj_dense_setup_kernel(const DGArgs<double> a, const double* __restrict__ fc,
                                                              const BJGeom g, const BJMass ch, int out_mode,
-                                                             double* __restrict__ out64, float* __restrict__ out32) {
+                                                             double* __restrict__ out64, float* __restrict__ out32,
+                                                             const VelArgs<double>* __restrict__ va,
+                                                             const double* __restrict__ bv,
+                                                             const double* __restrict__ bf) {
   constexpr int NL = N1 * N1 * N1;
   __shared__ double Ls[3][N1][N1];
   __shared__ double K[NL][NL + 1];
@@ -319,7 +326,33 @@ __global__ __launch_bounds__(256) void bj_dense_setup_kernel(const DGArgs<double
     K[i][j] = s;
   }
   __syncthreads();
-  if (out_mode == 1) {
+  if constexpr (VARB) {
+    // K <- D = (M (x) M (x) M) K + C_c, one column at a time through fcol
+    const int coord[3] = {bj_coord(g, c, 0), bj_coord(g, c, 1), bj_coord(g, c, 2)};
+    for (int j = 0; j < NL; ++j) {
+      for (int i = tid; i < NL; i += blockDim.x) {
+        const int ix = i % N1, iy = (i / N1) % N1, iz = i / (N1 * N1);
+        double s = 0.0;
+        for (int l = 0; l < NL; ++l) {
+          const int lx = l % N1, ly = (l / N1) % N1, lz = l / (N1 * N1);
+          s += a.M[ix][lx] * a.M[iy][ly] * a.M[iz][lz] * K[l][j];
+        }
+        fcol[i] = s + (TRANS ? conv_block_entry(*va, N1, bv, bf, c, coord, j, i)
+                             : conv_block_entry(*va, N1, bv, bf, c, coord, i, j));
+      }
+      __syncthreads();
+      for (int i = tid; i < NL; i += blockDim.x) K[i][j] = fcol[i];
+      __syncthreads();
+    }
+    if (out_mode == 1) {
+      for (int ij = tid; ij < NL * NL; ij += blockDim.x) {
+        const int i = ij / NL, j = ij - i * NL;
+        if (out64) out64[e * NL * NL + ij] = K[i][j];
+        if (out32) out32[e * NL * NL + ij] = (float)K[i][j];
+      }
+      return;
+    }
+  } else if (out_mode == 1) {
     // D = (M (x) M (x) M) K
     for (int ij = tid; ij < NL * NL; ij += blockDim.x) {
       const int i = ij / NL, j = ij - i * NL;
@@ -384,14 +417,18 @@ __global__ __launch_bounds__(256) void bj_dense_setup_kernel(const DGArgs<double
       }
     __syncthreads();
   }
-  // D^{-1} = K^{-1} (M^{-1} (x) M^{-1} (x) M^{-1}), written transposed
+  // D^{-1} = K^{-1} (M^{-1} (x) M^{-1} (x) M^{-1}) (VARB: K^{-1} is D^{-1}), written transposed
   for (int ij = tid; ij < NL * NL; ij += blockDim.x) {
     const int j = ij / NL, i = ij - j * NL;
     const int jx = j % N1, jy = (j / N1) % N1, jz = j / (N1 * N1);
     double s = 0.0;
-    for (int l = 0; l < NL; ++l) {
-      const int lx = l % N1, ly = (l / N1) % N1, lz = l / (N1 * N1);
-      s += K[i][l] * ch.Minv[lx][jx] * ch.Minv[ly][jy] * ch.Minv[lz][jz];
+    if (VARB) {
+      s = K[i][j];
+    } else {
+      for (int l = 0; l < NL; ++l) {
+        const int lx = l % N1, ly = (l / N1) % N1, lz = l / (N1 * N1);
+        s += K[i][l] * ch.Minv[lx][jx] * ch.Minv[ly][jy] * ch.Minv[lz][jz];
+      }
     }
     if (out64) out64[e * NL * NL + ij] = s;
     if (out32) out32[e * NL * NL + ij] = (float)s;

@@ -106,9 +106,20 @@ int fdm_setup(H* h, const BJGeom& g, long long entries, double* o64, float* o32)
 
 template <int N1, bool TRANS, bool CONV, bool VARK>
 int dense_setup(H* h, const BJGeom& g, long long entries, int out_mode, double* o64, float* o32) {
-  // one workgroup per block
+  // one workgroup per block; with a velocity field (constant b zero, blocks per cell) the
+  // cell blocks add its convection terms
+  if constexpr (!CONV) {
+    if (h->varb) {
+      hipLaunchKernelGGL((bj_dense_setup_kernel<N1, TRANS, false, VARK, true>), dim3((unsigned)entries), dim3(256), 0,
+                         h->stream, h->a64, (const double*)h->d_fc64, g, mass_data(h), out_mode, o64, o32,
+                         (const VelArgs<double>*)h->d_va64, (const double*)h->d_bv64, (const double*)h->d_bf64);
+      HIPCHK(h, hipGetLastError());
+      return PDELAB_OK;
+    }
+  }
   hipLaunchKernelGGL((bj_dense_setup_kernel<N1, TRANS, CONV, VARK>), dim3((unsigned)entries), dim3(256), 0, h->stream,
-                     h->a64, (const double*)h->d_fc64, g, mass_data(h), out_mode, o64, o32);
+                     h->a64, (const double*)h->d_fc64, g, mass_data(h), out_mode, o64, o32,
+                     (const VelArgs<double>*)nullptr, (const double*)nullptr, (const double*)nullptr);
   HIPCHK(h, hipGetLastError());
   return PDELAB_OK;
 }
@@ -212,8 +223,8 @@ static int bj_prepare_slot(H* h, int slot, int precision) {
   if (h->ncells >= (1LL << 31)) return fail(h, PDELAB_ERR_UNSUPPORTED, "block Jacobi: more than 2^31 - 1 cells");
   BlockJacobiData& b = h->bj[slot];
   const bool trans = slot == 1;
-  const bool fdm = slot != 2 && !h->conv && h->cfg.method == PDELAB_SIPG;
-  const bool per_cell = h->vark;
+  const bool fdm = slot != 2 && !h->conv && !h->varb && h->cfg.method == PDELAB_SIPG;
+  const bool per_cell = h->vark || h->varb;
   const long long nl = h->nloc;
   const long long entries = fdm ? (per_cell ? 3 * h->ncells : 12) : (per_cell ? h->ncells : 64);
   const long long esize = fdm ? (long long)(h->n1 * h->n1 + h->n1) : nl * nl;
@@ -437,7 +448,7 @@ template int sor_launch<float>(H*, const float*, float*, const int*, bool);
 int ibj_prepare(H* h, int precision) {
   if (h->ncells >= (1LL << 31)) return fail(h, PDELAB_ERR_UNSUPPORTED, "iterative block Jacobi: more than 2^31 - 1 cells");
   BlockJacobiData& b = h->bj[3];
-  const bool per_cell = h->vark;
+  const bool per_cell = h->vark || h->varb;
   const long long nl = h->nloc, entries = per_cell ? h->ncells : 64;
   if (b.per_cell != per_cell || b.entries != entries || b.esize != nl * nl) {
     if (b.d64) (void)hipFree(b.d64);

@@ -38,6 +38,7 @@ struct VelArgs {
   T vs[3];              // -|K| / h_d
   T area[3];            // |K| / h_d, the area of a face normal to d
   int bc[6];
+  int m;                // Gauss points per direction
 };
 
 // Device layout of the tabulated field (filled by the library from the caller's arrays):
@@ -283,6 +284,47 @@ __global__ __launch_bounds__(64) void conv_var_kernel(const T* __restrict__ z, T
     }
     y[c * NL + o] += acc;
   }
+}
+
+// Entry (i, j) (row i: test function, column j: trial function) of the cell block of the
+// convection terms of cell c: the volume term and the faces' inside-trace parts (the
+// neighbour-trace parts couple to other cells). The point diagonal is entry (i, i).
+__device__ inline double conv_block_entry(const VelArgs<double>& a, int n1, const double* __restrict__ bv,
+                                          const double* __restrict__ bf, long long c, const int (&coord)[3], int i,
+                                          int j) {
+  const int m = a.m, m2 = m * m, m3 = m2 * m, l2 = n1 * n1;
+  const int ii[3] = {i % n1, (i / n1) % n1, i / l2}, jj[3] = {j % n1, (j / n1) % n1, j / l2};
+  const double* bc = bv + c * 3 * m3;
+  double acc = 0.0;
+  for (int q2 = 0; q2 < m; ++q2)
+    for (int q1 = 0; q1 < m; ++q1)
+      for (int q0 = 0; q0 < m; ++q0) {
+        const int q = q0 + m * (q1 + m * q2);
+        const double b0 = a.B[q0][ii[0]], b1 = a.B[q1][ii[1]], b2 = a.B[q2][ii[2]];
+        const double phij = a.B[q0][jj[0]] * a.B[q1][jj[1]] * a.B[q2][jj[2]];
+        const double w = a.w[q0] * a.w[q1] * a.w[q2];
+        const double g = a.vs[0] * bc[q] * a.Dq[q0][ii[0]] * b1 * b2 +
+                         a.vs[1] * bc[m3 + q] * b0 * a.Dq[q1][ii[1]] * b2 +
+                         a.vs[2] * bc[2 * m3 + q] * b0 * b1 * a.Dq[q2][ii[2]];
+        acc += w * phij * g;
+      }
+  for (int f = 0; f < 6; ++f) {
+    const int d = f / 2, up = f % 2, layer = up ? n1 - 1 : 0;
+    if (ii[d] != layer || jj[d] != layer) continue;
+    const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+    const bool interior = up ? coord[d] < a.n[d] - 1 : coord[d] > 0;
+    const int bct = interior ? 0 : a.bc[f];
+    if (!interior && bct != BC_DIRICHLET && bct != BC_OUTFLOW) continue;
+    const double* bfc = bf + (c * 6 + f) * m2;
+    const double ns = up ? 1.0 : -1.0;
+    for (int q2 = 0; q2 < m; ++q2)
+      for (int q1 = 0; q1 < m; ++q1) {
+        const double psi2 = a.B[q1][ii[t1]] * a.B[q2][ii[t2]] * a.B[q1][jj[t1]] * a.B[q2][jj[t2]];
+        const double cf = a.w[q1] * a.w[q2] * a.area[d] * (ns * bfc[q1 + m * q2]);
+        if (bct == BC_OUTFLOW || cf >= 0.0) acc += cf * psi2;
+      }
+  }
+  return acc;
 }
 
 // diag += the convection terms' diagonal entries (the coefficient of z_i in the row of

@@ -107,6 +107,7 @@ struct pdelab_gpu_gridoperator {
   float* d_bf32 = nullptr;
   VelArgs<double> va64;
   VelArgs<float> va32;
+  VelArgs<double>* d_va64 = nullptr;  // device copy for the cell-block set-up
 };
 
 using H = pdelab_gpu_gridoperator;

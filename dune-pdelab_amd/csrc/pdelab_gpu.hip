@@ -311,10 +311,11 @@ static void fill_load_args(H* h) {
 
 // ---------------------------------------------------------------------------
 // kernel dispatch
-// The cell blocks (block preconditioners, block diagonal, assembled Jacobian) are built for
-// the constant b only.
+// Block SOR (its neighbour couplings) and the assembled Jacobian (its off-diagonal blocks) are
+// built for the constant b only; the cell blocks themselves take a velocity field.
 static int block_data_ok(H* h) {
-  if (h->varb) return fail(h, PDELAB_ERR_UNSUPPORTED, "velocity field: the cell blocks are not built for it");
+  if (h->varb)
+    return fail(h, PDELAB_ERR_UNSUPPORTED, "velocity field: block SOR and the assembled Jacobian are not built for it");
   return PDELAB_OK;
 }
 
@@ -546,7 +547,7 @@ static void sync_prec_cache(H* h) {
 // Checks the preconditioner and prepares its data (point diagonal or block data).
 template <typename T>
 static int make_prec(H* h, int precond, bool trans, Prec<T>& P) {
-  if (precond >= PDELAB_PREC_BLOCK_JACOBI && precond <= PDELAB_PREC_ITERATIVE_BLOCK_JACOBI) CHK(block_data_ok(h));
+  if (precond == PDELAB_PREC_BLOCK_SOR) CHK(block_data_ok(h));
   sync_prec_cache(h);
   P.kind = precond;
   P.trans = trans;
@@ -1373,7 +1374,6 @@ int pdelab_gpu_point_diagonal(pdelab_gpu_gridoperator* h, double* d) {
 int pdelab_gpu_block_diagonal(pdelab_gpu_gridoperator* h, int transpose, double* blocks) {
   if (!h || !blocks) return PDELAB_ERR_INVALID_ARG;
   CHK(check_outflow(h));
-  CHK(block_data_ok(h));
   return bj_block_diagonal(h, transpose != 0, blocks);
 }
 
@@ -1393,7 +1393,6 @@ int pdelab_gpu_block_matrix_apply(pdelab_gpu_gridoperator* h, const double* bloc
 int pdelab_gpu_block_jacobi_apply(pdelab_gpu_gridoperator* h, const double* z, double* y) {
   if (!h || !z || !y) return PDELAB_ERR_INVALID_ARG;
   CHK(check_outflow(h));
-  CHK(block_data_ok(h));
   sync_prec_cache(h);
   const BJVec<double, double> v{z, y, nullptr, nullptr, nullptr, nullptr, nullptr};
   return bj_launch<double, double>(h, false, BJ_ADD, v, nullptr);
@@ -1402,7 +1401,6 @@ int pdelab_gpu_block_jacobi_apply(pdelab_gpu_gridoperator* h, const double* z, d
 int pdelab_gpu_block_jacobi_apply_f32(pdelab_gpu_gridoperator* h, const float* z, float* y) {
   if (!h || !z || !y) return PDELAB_ERR_INVALID_ARG;
   CHK(check_outflow(h));
-  CHK(block_data_ok(h));
   sync_prec_cache(h);
   const BJVec<float, float> v{z, y, nullptr, nullptr, nullptr, nullptr, nullptr};
   return bj_launch<float, float>(h, false, BJ_ADD, v, nullptr);
@@ -1420,7 +1418,6 @@ int pdelab_gpu_iterative_block_jacobi_apply(pdelab_gpu_gridoperator* h, const do
                                             unsigned long long* iterations) {
   if (!h || !d || !v || d == v) return PDELAB_ERR_INVALID_ARG;
   CHK(check_outflow(h));
-  CHK(block_data_ok(h));
   sync_prec_cache(h);
   CHK(ibj_prepare(h, 64));
   HIPCHK(h, hipMemsetAsync(h->d_ibj_stat, 0, 16, h->stream));

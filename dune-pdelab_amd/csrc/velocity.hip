@@ -29,14 +29,16 @@ static void fill_vel_args(const H* h, VelArgs<T>& a) {
     a.area[d] = (T)(V / hh[d]);
   }
   for (int s = 0; s < 6; ++s) a.bc[s] = h->a64.bc[s];
+  a.m = t.m;
 }
 
 void vel_free(H* h) {
-  void* p[] = {h->d_bv64, h->d_bf64, h->d_bv32, h->d_bf32};
+  void* p[] = {h->d_bv64, h->d_bf64, h->d_bv32, h->d_bf32, h->d_va64};
   for (void* q : p)
     if (q) (void)hipFree(q);
   h->d_bv64 = h->d_bf64 = nullptr;
   h->d_bv32 = h->d_bf32 = nullptr;
+  h->d_va64 = nullptr;
   h->varb = false;
 }
 
@@ -49,6 +51,7 @@ int vel_set(H* h, const double* bvol, const double* bface) {
     vel_free(h);
     h->R0_valid = false;
     h->diag_valid = false;
+    bj_invalidate(h);
     return PDELAB_OK;
   }
   if (!bface) return fail(h, PDELAB_ERR_INVALID_ARG, "velocity field: b_face is required with b_vol");
@@ -111,9 +114,12 @@ int vel_set(H* h, const double* bvol, const double* bface) {
   HIPCHK(h, hipMemcpy(h->d_bf32, bf32.data(), sizeof(float) * bf.size(), hipMemcpyHostToDevice));
   fill_vel_args<double>(h, h->va64);
   fill_vel_args<float>(h, h->va32);
+  HIPCHK(h, hipMalloc(&h->d_va64, sizeof(VelArgs<double>)));
+  HIPCHK(h, hipMemcpy(h->d_va64, &h->va64, sizeof(VelArgs<double>), hipMemcpyHostToDevice));
   h->varb = true;
   h->R0_valid = false;
   h->diag_valid = false;
+  bj_invalidate(h);  // the cell blocks take the field
   return PDELAB_OK;
 }
 
