@@ -316,10 +316,21 @@ int jac_assemble(H* h, bool trans, double* blocks, long long* cols) {
   if (h->ncells >= (1LL << 31)) return fail(h, PDELAB_ERR_UNSUPPORTED, "jacobian: more than 2^31 - 1 cells");
   const BJGeom g = geom(h, true);
   switch (h->n1) {
-    case 2: return trans ? jac_launch_t<2, true>(h, g, blocks, cols) : jac_launch_t<2, false>(h, g, blocks, cols);
-    case 3: return trans ? jac_launch_t<3, true>(h, g, blocks, cols) : jac_launch_t<3, false>(h, g, blocks, cols);
-    default: return trans ? jac_launch_t<4, true>(h, g, blocks, cols) : jac_launch_t<4, false>(h, g, blocks, cols);
+    case 2: CHK(trans ? jac_launch_t<2, true>(h, g, blocks, cols) : jac_launch_t<2, false>(h, g, blocks, cols)); break;
+    case 3: CHK(trans ? jac_launch_t<3, true>(h, g, blocks, cols) : jac_launch_t<3, false>(h, g, blocks, cols)); break;
+    default: CHK(trans ? jac_launch_t<4, true>(h, g, blocks, cols) : jac_launch_t<4, false>(h, g, blocks, cols)); break;
   }
+  if (h->varb) {  // the velocity field's convection terms on top of the b = 0 blocks
+    const VelArgs<double>* va = h->d_va64;
+    if (trans)
+      hipLaunchKernelGGL(conv_var_jac_kernel<true>, dim3((unsigned)h->ncells), dim3(256), 0, h->stream, va, h->n1,
+                         (const double*)h->d_bv64, (const double*)h->d_bf64, blocks, (const long long*)cols);
+    else
+      hipLaunchKernelGGL(conv_var_jac_kernel<false>, dim3((unsigned)h->ncells), dim3(256), 0, h->stream, va, h->n1,
+                         (const double*)h->d_bv64, (const double*)h->d_bf64, blocks, (const long long*)cols);
+    HIPCHK(h, hipGetLastError());
+  }
+  return PDELAB_OK;
 }
 
 int jac_block_apply(H* h, const double* blocks, const long long* cols, const double* z, double* y) {

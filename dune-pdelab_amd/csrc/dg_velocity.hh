@@ -327,6 +327,51 @@ __device__ inline double conv_block_entry(const VelArgs<double>& a, int n1, cons
   return acc;
 }
 
+// Entry (i, j) of the block coupling cell c to its neighbour n across face f (row i: c's test
+// function, column j: n's trial function): the upwind term [cf < 0] cf psi_i psi_j at each face
+// point, cf = w area (b . n_c). With trans, the entry of J^T's block (c, n), i.e. of
+// J's block (n, c) transposed: [cf > 0] (-cf) psi_i psi_j.
+__device__ inline double conv_coupling_entry(const VelArgs<double>& a, int n1, const double* __restrict__ bf,
+                                             long long c, int f, int i, int j, bool trans) {
+  const int d = f / 2, up = f % 2, l2 = n1 * n1, m = a.m, m2 = m * m;
+  const int ii[3] = {i % n1, (i / n1) % n1, i / l2}, jj[3] = {j % n1, (j / n1) % n1, j / l2};
+  if (ii[d] != (up ? n1 - 1 : 0) || jj[d] != (up ? 0 : n1 - 1)) return 0.0;
+  const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+  const double* bfc = bf + (c * 6 + f) * m2;
+  const double ns = up ? 1.0 : -1.0;
+  double acc = 0.0;
+  for (int q2 = 0; q2 < m; ++q2)
+    for (int q1 = 0; q1 < m; ++q1) {
+      const double psi2 = a.B[q1][ii[t1]] * a.B[q2][ii[t2]] * a.B[q1][jj[t1]] * a.B[q2][jj[t2]];
+      const double cf = a.w[q1] * a.w[q2] * a.area[d] * (ns * bfc[q1 + m * q2]);
+      if (trans ? cf > 0.0 : cf < 0.0) acc += (trans ? -cf : cf) * psi2;
+    }
+  return acc;
+}
+
+// Adds a velocity field's convection terms to the block row of cell c of the assembled
+// Jacobian (seven n_loc x n_loc blocks, row-major; slot 0 the cell, slot 1 + f the neighbour
+// across face f, cols < 0 where there is none): C_c to slot 0, the upwind coupling to the
+// others (TRANS: the blocks of J^T). One workgroup per cell.
+template <bool TRANS>
+__global__ __launch_bounds__(256) void conv_var_jac_kernel(const VelArgs<double>* __restrict__ va, int n1,
+                                                           const double* __restrict__ bv, const double* __restrict__ bf,
+                                                           double* __restrict__ blocks, const long long* __restrict__ cols) {
+  const VelArgs<double>& a = *va;
+  const long long c = blockIdx.x;
+  const int nl = n1 * n1 * n1, nl2 = nl * nl;
+  const int coord[3] = {(int)(c % a.n[0]), (int)((c / a.n[0]) % a.n[1]), (int)(c / ((long long)a.n[0] * a.n[1]))};
+  double* B = blocks + c * 7 * nl2;
+  for (int k = threadIdx.x; k < 7 * nl2; k += blockDim.x) {
+    const int slot = k / nl2, ij = k % nl2, i = ij / nl, j = ij % nl;
+    if (slot == 0) {
+      B[k] += TRANS ? conv_block_entry(a, n1, bv, bf, c, coord, j, i) : conv_block_entry(a, n1, bv, bf, c, coord, i, j);
+    } else if (cols[c * 7 + slot] >= 0) {
+      B[k] += conv_coupling_entry(a, n1, bf, c, slot - 1, i, j, TRANS);
+    }
+  }
+}
+
 // diag += the convection terms' diagonal entries (the coefficient of z_i in the row of
 // DoF i); one thread per DoF.
 template <int N1, int M>

@@ -311,11 +311,10 @@ static void fill_load_args(H* h) {
 
 // ---------------------------------------------------------------------------
 // kernel dispatch
-// Block SOR (its neighbour couplings) and the assembled Jacobian (its off-diagonal blocks) are
-// built for the constant b only; the cell blocks themselves take a velocity field.
+// Block SOR (its neighbour couplings) is built for the constant b only; the cell blocks and
+// the assembled Jacobian take a velocity field.
 static int block_data_ok(H* h) {
-  if (h->varb)
-    return fail(h, PDELAB_ERR_UNSUPPORTED, "velocity field: block SOR and the assembled Jacobian are not built for it");
+  if (h->varb) return fail(h, PDELAB_ERR_UNSUPPORTED, "velocity field: block SOR is not built for it");
   return PDELAB_OK;
 }
 
@@ -1380,7 +1379,6 @@ int pdelab_gpu_block_diagonal(pdelab_gpu_gridoperator* h, int transpose, double*
 int pdelab_gpu_jacobian(pdelab_gpu_gridoperator* h, int transpose, double* blocks, long long* cols) {
   if (!h || !blocks || !cols) return PDELAB_ERR_INVALID_ARG;
   CHK(check_outflow(h));
-  CHK(block_data_ok(h));
   return jac_assemble(h, transpose != 0, blocks, cols);
 }
 
