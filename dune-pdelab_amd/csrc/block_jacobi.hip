@@ -393,7 +393,9 @@ int sor_waves(H* h, const T* d, T* v, const T* dinv, const int* stop) {
   for (int s = 0; s < ns; ++s) {
     const int first = h->sor_first[s], count = h->sor_first[s + 1] - first;
     const unsigned nb = (unsigned)((count + S::CPW - 1) / S::CPW);
-    hipLaunchKernelGGL((block_sor_wave_kernel<N1, T, CONV, VARK, INPLACE>), dim3(nb), dim3(64), 0, h->stream, d, v, fc, a, lm,
+    const T* ds = d;
+    if (h->varb) CHK(vel_sor_rhs<T>(h, d, v, first, count, INPLACE, &ds));  // the field's couplings
+    hipLaunchKernelGGL((block_sor_wave_kernel<N1, T, CONV, VARK, INPLACE>), dim3(nb), dim3(64), 0, h->stream, ds, v, fc, a, lm,
                        dinv, g, (const int*)h->d_sor_order, first, count, (T)h->sor_omega, stop);
   }
   HIPCHK(h, hipGetLastError());

@@ -372,6 +372,47 @@ __global__ __launch_bounds__(256) void conv_var_jac_kernel(const VelArgs<double>
   }
 }
 
+// Block SOR with a velocity field: the convection terms' couplings of the cells of one
+// hyperplane (order[first .. first + count)) to their neighbours, moved to the right-hand side:
+// dtmp_c = d_c - sum_f sum_q [cf < 0] cf psi_c(q) v_n(q), over the lower neighbours (from
+// v = 0: the upper ones are still zero) or, INPLACE, over all of them (the upper ones hold the
+// old values). The sweep kernel then runs on dtmp. One thread per DoF of the hyperplane.
+template <int N1, int M, typename T, bool INPLACE>
+__global__ __launch_bounds__(256) void conv_var_sor_kernel(const T* __restrict__ d, const T* __restrict__ v,
+                                                           T* __restrict__ dtmp, const T* __restrict__ bf,
+                                                           const VelArgs<T> a, const int* __restrict__ order,
+                                                           int first, int count) {
+  constexpr int L = N1 * N1, NL = L * N1, M2 = M * M;
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long long)count * NL) return;
+  const long long c = order[first + gid / NL];
+  const int li = (int)(gid % NL);
+  const int ii[3] = {li % N1, (li / N1) % N1, li / L};
+  const int coord[3] = {(int)(c % a.n[0]), (int)((c / a.n[0]) % a.n[1]), (int)(c / ((long long)a.n[0] * a.n[1]))};
+  T acc = T(0);
+  for (int f = 0; f < 6; ++f) {
+    const int dd = f / 2, up = f % 2;
+    if (!INPLACE && up) continue;
+    if (ii[dd] != (up ? N1 - 1 : 0)) continue;
+    if (up ? coord[dd] >= a.n[dd] - 1 : coord[dd] == 0) continue;  // no neighbour
+    const long long nb = c + (up ? a.stride[dd] : -a.stride[dd]);
+    const int t1 = dd == 0 ? 1 : 0, t2 = dd == 2 ? 1 : 2;
+    const T* bfc = bf + (c * 6 + f) * M2;
+    const T* vn = v + nb * NL;
+    const T ns = up ? T(1) : T(-1);
+    for (int q2 = 0; q2 < M; ++q2)
+      for (int q1 = 0; q1 < M; ++q1) {
+        const T cf = a.w[q1] * a.w[q2] * a.area[dd] * (ns * bfc[q1 + M * q2]);
+        if (!(cf < T(0))) continue;
+        T un = T(0);
+        for (int p2 = 0; p2 < N1; ++p2)
+          for (int p1 = 0; p1 < N1; ++p1) un += a.B[q1][p1] * a.B[q2][p2] * vn[face_dof<N1>(f ^ 1, p1, p2)];
+        acc += cf * a.B[q1][ii[t1]] * a.B[q2][ii[t2]] * un;
+      }
+  }
+  dtmp[c * NL + li] = d[c * NL + li] - acc;
+}
+
 // diag += the convection terms' diagonal entries (the coefficient of z_i in the row of
 // DoF i); one thread per DoF.
 template <int N1, int M>

@@ -108,6 +108,8 @@ struct pdelab_gpu_gridoperator {
   VelArgs<double> va64;
   VelArgs<float> va32;
   VelArgs<double>* d_va64 = nullptr;  // device copy for the cell-block set-up
+  double* d_sortmp64 = nullptr;       // block SOR with a field: the right-hand side less the couplings
+  float* d_sortmp32 = nullptr;
 };
 
 using H = pdelab_gpu_gridoperator;
@@ -238,5 +240,9 @@ template <typename T>
 int vel_launch(H* h, const T* z, T* y, const int* stop, bool trans);
 int vel_diag(H* h, double* diag);
 void vel_free(H* h);
+// Block SOR with a velocity field: *out = d less the convection couplings of the cells of
+// hyperplane d_sor_order[first .. first + count) (lower neighbours; all with inplace).
+template <typename T>
+int vel_sor_rhs(H* h, const T* d, const T* v, int first, int count, bool inplace, const T** out);
 
 #endif

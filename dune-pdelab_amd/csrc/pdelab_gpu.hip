@@ -311,13 +311,6 @@ static void fill_load_args(H* h) {
 
 // ---------------------------------------------------------------------------
 // kernel dispatch
-// Block SOR (its neighbour couplings) is built for the constant b only; the cell blocks and
-// the assembled Jacobian take a velocity field.
-static int block_data_ok(H* h) {
-  if (h->varb) return fail(h, PDELAB_ERR_UNSUPPORTED, "velocity field: block SOR is not built for it");
-  return PDELAB_OK;
-}
-
 static int check_outflow(H* h) {
   if (h->varb) return PDELAB_OK;  // a velocity field is checked when it is set
   for (int s = 0; s < 6; ++s) {
@@ -546,7 +539,6 @@ static void sync_prec_cache(H* h) {
 // Checks the preconditioner and prepares its data (point diagonal or block data).
 template <typename T>
 static int make_prec(H* h, int precond, bool trans, Prec<T>& P) {
-  if (precond == PDELAB_PREC_BLOCK_SOR) CHK(block_data_ok(h));
   sync_prec_cache(h);
   P.kind = precond;
   P.trans = trans;
@@ -1407,7 +1399,6 @@ int pdelab_gpu_block_jacobi_apply_f32(pdelab_gpu_gridoperator* h, const float* z
 int pdelab_gpu_block_sor_apply(pdelab_gpu_gridoperator* h, const double* d, double* v) {
   if (!h || !d || !v || d == v) return PDELAB_ERR_INVALID_ARG;
   CHK(check_outflow(h));
-  CHK(block_data_ok(h));
   sync_prec_cache(h);
   return sor_launch<double>(h, d, v, nullptr, true);
 }

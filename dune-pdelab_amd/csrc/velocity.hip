@@ -33,12 +33,14 @@ static void fill_vel_args(const H* h, VelArgs<T>& a) {
 }
 
 void vel_free(H* h) {
-  void* p[] = {h->d_bv64, h->d_bf64, h->d_bv32, h->d_bf32, h->d_va64};
+  void* p[] = {h->d_bv64, h->d_bf64, h->d_bv32, h->d_bf32, h->d_va64, h->d_sortmp64, h->d_sortmp32};
   for (void* q : p)
     if (q) (void)hipFree(q);
   h->d_bv64 = h->d_bf64 = nullptr;
   h->d_bv32 = h->d_bf32 = nullptr;
   h->d_va64 = nullptr;
+  h->d_sortmp64 = nullptr;
+  h->d_sortmp32 = nullptr;
   h->varb = false;
 }
 
@@ -141,6 +143,44 @@ template <>
 const VelArgs<double>& va_for<double>(const H* h) { return h->va64; }
 template <>
 const VelArgs<float>& va_for<float>(const H* h) { return h->va32; }
+
+// Block SOR with a field: dtmp = d less the convection couplings of hyperplane
+// order[first .. first + count) (conv_var_sor_kernel); returns dtmp for the sweep kernel.
+template <int N1, int M, typename T, bool INPLACE>
+static void vel_sor_nm(H* h, const T* d, const T* v, T* dtmp, int first, int count) {
+  const long long n = (long long)count * N1 * N1 * N1;
+  hipLaunchKernelGGL((conv_var_sor_kernel<N1, M, T, INPLACE>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     h->stream, d, v, dtmp, bf_for<T>(h), va_for<T>(h), (const int*)h->d_sor_order, first, count);
+}
+
+template <typename T>
+static T*& sortmp_for(H* h);
+template <>
+double*& sortmp_for<double>(H* h) { return h->d_sortmp64; }
+template <>
+float*& sortmp_for<float>(H* h) { return h->d_sortmp32; }
+
+template <typename T>
+int vel_sor_rhs(H* h, const T* d, const T* v, int first, int count, bool inplace, const T** out) {
+  T*& tmp = sortmp_for<T>(h);
+  if (!tmp) HIPCHK(h, hipMalloc(&tmp, sizeof(T) * h->N));
+  const bool m1 = h->tab.m == h->n1;
+#define VSOR(N1_, M_)                                                                  \
+  (inplace ? vel_sor_nm<N1_, M_, T, true>(h, d, v, tmp, first, count)                  \
+           : vel_sor_nm<N1_, M_, T, false>(h, d, v, tmp, first, count))
+  switch (h->n1) {
+    case 2: m1 ? VSOR(2, 2) : VSOR(2, 3); break;
+    case 3: m1 ? VSOR(3, 3) : VSOR(3, 4); break;
+    case 4: m1 ? VSOR(4, 4) : VSOR(4, 5); break;
+    default: return fail(h, PDELAB_ERR_UNSUPPORTED, "degree");
+  }
+#undef VSOR
+  HIPCHK(h, hipGetLastError());
+  *out = tmp;
+  return PDELAB_OK;
+}
+template int vel_sor_rhs<double>(H*, const double*, const double*, int, int, bool, const double**);
+template int vel_sor_rhs<float>(H*, const float*, const float*, int, int, bool, const float**);
 
 template <int N1, int M, typename T>
 static void vel_launch_nm(H* h, const T* z, T* y, const int* stop, bool trans) {

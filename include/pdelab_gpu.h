@@ -277,9 +277,9 @@ int pdelab_gpu_quadrature_points(const pdelab_gpu_gridoperator* h); /* m */
    PDELAB_PROBLEM_EXP / _QUADRATIC), the point diagonal and the FP32 apply. b_vol = NULL
    restores the constant b. Outflow sides must see no inflow (b . n >= -1e-30 at every point,
    else PDELAB_ERR_OUTFLOW_ON_INFLOW). Requires intorderadd 0..3. The transposed operator
-   (and with it the adjoint solves), the block diagonal, block Jacobi and the iterative block
-   Jacobi (dense blocks per cell) and the assembled Jacobian take the field too. With a field
-   set, CG and block SOR return PDELAB_ERR_UNSUPPORTED. */
+   (and with it the adjoint solves), the block diagonal, the block preconditioners (dense
+   blocks per cell) and the assembled Jacobian take the field too. With a field set, CG
+   returns PDELAB_ERR_UNSUPPORTED (the operator is not symmetric). */
 int pdelab_gpu_set_velocity_field(pdelab_gpu_gridoperator* h, const double* b_vol, const double* b_face);
 
 /* ---- device memory helpers (ordered on the handle's stream) --------------- */

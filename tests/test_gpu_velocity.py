@@ -15,11 +15,12 @@
   transposed solve J^T z = r checked with the oracle's J^T;
 * the cell blocks with the field: the block diagonal of J and J^T against the oracle's
   (1e-12), the assembled Jacobian of J and J^T against the oracle's assembly (1e-12) and its
-  block-matrix apply against the matrix-free apply, BiCGSTAB with block Jacobi and with the iterative block Jacobi against the oracle's
+  block-matrix apply against the matrix-free apply, one block-SOR step (from zero and in place)
+  against the oracle's, the mixed-precision BiCGSTAB with the FP32 sweep, BiCGSTAB with block Jacobi and with the iterative block Jacobi against the oracle's
   solves, one iterative block Jacobi application against the oracle's;
 * a 24^3 Q2 apply against the oracle's two-sided apply;
 * the rejections: the configuration's constant b not zero, wrong array sizes, inflow on an
-  outflow side, CG, block SOR;
+  outflow side, CG;
   clearing the field restores the b = 0 operator.
 """
 import numpy as np
@@ -181,7 +182,19 @@ def test_velocity_cell_blocks(pkg, of, name, kw, seed, field):
                 want[c, 1 + f] = ref_blocks[cn, 1 + (f ^ 1)].T
     assert rel(jt.download().reshape(ref_blocks.shape), want) <= 1e-12
     d = np.random.default_rng(74).uniform(-1, 1, go.N)
-    v = go.vector()
+    v0 = np.random.default_rng(75).normal(size=go.N)
+    dd, v = go.vector(d), go.vector()
+    for omega in (1.0, 1.3):
+        go.set_sor_relaxation(omega)
+        o.set_sor_omega(omega)
+        v.zero()
+        go.block_sor_apply(dd, v)
+        assert rel(v.download(), o.block_sor_apply(d)) <= 1e-10
+        v.upload(v0)
+        go.block_sor_apply(dd, v)
+        assert rel(v.download(), o.block_sor_apply(d, v0)) <= 1e-10
+    go.set_sor_relaxation(1.0)
+    o.set_sor_omega(1.0)
     v.zero()
     its = go.iterative_block_jacobi_apply(go.vector(d), v)
     want, its_ref = o.iterative_block_jacobi_apply(d)
@@ -192,7 +205,7 @@ def test_velocity_cell_blocks(pkg, of, name, kw, seed, field):
     # follows where rounding stops each inexact block solve
     go.set_block_solver_options(reduction=1e-10, maxiter=200)
     o.set_block_solver(1e-10, 200, 1, 1.0)
-    for prec in (pkg.PREC_BLOCK_JACOBI, pkg.PREC_ITERATIVE_BLOCK_JACOBI):
+    for prec in (pkg.PREC_BLOCK_JACOBI, pkg.PREC_BLOCK_SOR, pkg.PREC_ITERATIVE_BLOCK_JACOBI):
         xd = go.vector()
         res = go.linear_problem_solve(xd, method=1, precond=prec, reduction=1e-10)
         x_ref, res_ref = o.linear_problem_solve(method=1, precond=prec, reduction=1e-10)
@@ -201,6 +214,11 @@ def test_velocity_cell_blocks(pkg, of, name, kw, seed, field):
         say(f"  precond {prec}: iterations {it} (oracle {it_ref})")
         assert abs(it - it_ref) <= max(2, 0.15 * it_ref)
         assert np.linalg.norm(o.residual(xd.download())) <= 1.01e-10 * res["first_defect"]
+    # FP32 block SOR inside the mixed-precision BiCGSTAB
+    xm = go.vector()
+    rm = go.solve_mixed(xm, method=1, precond=pkg.PREC_BLOCK_SOR, inner_reduction=1e-4, outer_reduction=1e-10)
+    assert rm["converged"], rm
+    assert np.linalg.norm(o.residual(xm.download())) <= 1.01e-10 * np.linalg.norm(o.residual(np.zeros(go.N)))
     go.close()
 
 
@@ -236,10 +254,6 @@ def test_velocity_rejections(pkg, of):
     go.set_velocity_field(bv, bf)
     with pytest.raises(RuntimeError, match="CG"):
         go.linear_problem_solve(go.vector(), method=0)
-    with pytest.raises(RuntimeError, match="block SOR"):
-        go.linear_problem_solve(go.vector(), method=1, precond=pkg.PREC_BLOCK_SOR)
-    with pytest.raises(RuntimeError, match="block SOR"):
-        go.block_sor_apply(go.vector(np.ones(go.N)), go.vector())
     # cleared: the b = 0 operator again
     go.set_velocity_field(None)
     x = np.random.default_rng(2).normal(size=go.N)
