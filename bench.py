@@ -419,7 +419,8 @@ def profile_summary(d, n, k):
 
 FP64_PEAK = 78.6e12  # MI355X dense FP64 (vector and matrix), FLOP/s
 APPLY_WORKLOADS = {  # grid size (threads) of the apply launches of this benchmark -> workload
-    "2097152": "64^3 Q3", "4194304": "64^3 Q3", "995328": "96^3 Q2", "7962624": "96^3 Q2"}
+    "2097152": "64^3 Q3", "4194304": "64^3 Q3", "995328": "96^3 Q2", "7962624": "96^3 Q2",
+    "56623104": "96^3 Q2"}  # the last: conv_var_kernel, one 64-lane wave per cell
 
 
 def counter_summary(d):
@@ -430,7 +431,10 @@ def counter_summary(d):
     is summed over the 8 XCDs), FP64 utilisation ((2 FMA + ADD + MUL)_F64 x 64 lanes per wave
     instruction / duration / 78.6 TFLOP/s), LDS busy and bank-conflict cycles (per CU), SALU busy,
     instructions per wave, fraction of wave cycles waiting, mean resident waves per CU, HBM
-    traffic over algorithmic bytes (FETCH_SIZE x 2 and WRITE_SIZE x 1, the axpy calibration)."""
+    traffic over algorithmic bytes (FETCH_SIZE x 2 and WRITE_SIZE x 1, the axpy calibration).
+    conv_var_kernel (the velocity field's convection terms, y += C z) is measured against its own
+    algorithmic bytes: z read, y read and written (24 B/DoF FP64), and the field at the 27 volume
+    and 54 face points of a Q2 cell (135 values per cell)."""
     import collections
     import re
     agg = collections.defaultdict(lambda: collections.defaultdict(list))
@@ -444,7 +448,7 @@ def counter_summary(d):
         dur[r["Kernel_Name"]].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
 
     def short(k):
-        m = re.search(r"dg_apply\w*kernel<[^(]*>", k)
+        m = re.search(r"(dg_apply\w*kernel|conv_var_kernel)<[^(]*>", k)
         return m.group(0) if m else None
 
     out = ["| kernel | workload | us | HBM frac | VALU busy | FP64 util | LDS busy | LDS conflict | SALU busy "
@@ -462,6 +466,9 @@ def counter_summary(d):
         n, k = (64, 3) if "Q3" in wl else (96, 2)
         N = n ** 3 * (k + 1) ** 3
         bpd = 16.0 if "double" in s else 8.0
+        if s.startswith("conv_var_kernel"):
+            es = 8.0 if "double" in s else 4.0
+            bpd = 3 * es + 135 * es / (k + 1) ** 3
         waves = c["SQ_WAVES"]
         fp64 = 2 * c["SQ_INSTS_VALU_FMA_F64"] + c["SQ_INSTS_VALU_ADD_F64"] + c["SQ_INSTS_VALU_MUL_F64"]
         traffic = (2.0 * c["FETCH_SIZE"] + c["WRITE_SIZE"]) * 1024.0 / (bpd * N)
