@@ -430,7 +430,9 @@ def counter_summary(d):
     FP32 over 8 TB/s), VALU busy (SQ_INSTS_VALU x 4 cycles over the 1024 SIMDs' cycles; GRBM_GUI_ACTIVE
     is summed over the 8 XCDs), FP64 utilisation ((2 FMA + ADD + MUL)_F64 x 64 lanes per wave
     instruction / duration / 78.6 TFLOP/s), LDS busy and bank-conflict cycles (per CU), SALU busy,
-    instructions per wave, fraction of wave cycles waiting, mean resident waves per CU, HBM
+    instructions per wave, fraction of wave cycles waiting, mean resident waves per CU
+    (SQ_WAVE_CYCLES counts quad-cycles: 4 SQ_WAVE_CYCLES / cycles / 256, rocprofv3's own
+    OccupancyPercent formula), HBM
     traffic over algorithmic bytes (FETCH_SIZE x 2 and WRITE_SIZE x 1, the axpy calibration).
     conv_var_kernel (the velocity field's convection terms, y += C z) is measured against its own
     algorithmic bytes: z read, y read and written (24 B/DoF FP64), and the field at the 27 volume
@@ -478,7 +480,7 @@ def counter_summary(d):
             f"{c['SQ_LDS_IDX_ACTIVE'] / 256 / cyc:.2f} | {c['SQ_LDS_BANK_CONFLICT'] / 256 / cyc:.2f} | "
             f"{c['SQ_INSTS_SALU'] / 1024 / cyc:.2f} | {c['SQ_INSTS_VALU'] / waves:.0f} | {c['SQ_INSTS_LDS'] / waves:.0f} | "
             f"{(c['SQ_INSTS_VMEM_RD'] + c['SQ_INSTS_VMEM_WR']) / waves:.0f} | {c['SQ_WAIT_ANY'] / c['SQ_WAVE_CYCLES']:.2f} | "
-            f"{c['SQ_WAVE_CYCLES'] / cyc / 256:.1f} | {traffic:.3f} |")
+            f"{4.0 * c['SQ_WAVE_CYCLES'] / cyc / 256:.1f} | {traffic:.3f} |")
     return "\n".join(out)
 
 
