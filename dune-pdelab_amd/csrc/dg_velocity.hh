@@ -286,6 +286,284 @@ __global__ __launch_bounds__(64) void conv_var_kernel(const T* __restrict__ z, T
   }
 }
 
+// The same terms for the rule with N1 points per direction (intorderadd 0 or 1), the common
+// case. Every volume phase then has the N1^3 items (i0, i1, i2) and every face phase the
+// 6 N1^2 items (f, r1, r2), so each lane decomposes its items once and keeps the table rows
+// and columns it needs, and its values of the field, in registers: the phases read LDS only
+// for the data they exchange.
+template <int N1, typename T, bool TRANS = false, bool REGTAB = (sizeof(T) == 4)>
+__global__ __launch_bounds__(64) void conv_var_sq_kernel(const T* __restrict__ z, T* __restrict__ y,
+                                                         const T* __restrict__ bv, const T* __restrict__ bf,
+                                                         const VelArgs<T> a, const int* __restrict__ stop) {
+  constexpr int L = N1 * N1, NL = L * N1;
+  constexpr int FI = (6 * L + 63) / 64;  // face items per lane
+  __shared__ T zs[NL];
+  __shared__ T zt[6][L];  // neighbour layers (zero without a neighbour)
+  __shared__ T V[3][NL];
+  __shared__ T A1[TRANS ? 2 : 1][NL], A2[TRANS ? 3 : 1][NL];
+  __shared__ T W[3][NL];
+  __shared__ T S[2][NL];
+  __shared__ T F[6][2][L];
+  __shared__ T fv[6][L];
+  __shared__ T G[6][L];
+  __shared__ T rf[6][L];
+  __shared__ T tB[N1][N1], tD[N1][N1];  // the tables when they are not held in registers
+  if (stop && *stop) return;
+  const long long c = blockIdx.x;
+  const int t = threadIdx.x;
+  const int coord[3] = {(int)(c % a.n[0]), (int)((c / a.n[0]) % a.n[1]), (int)(c / ((long long)a.n[0] * a.n[1]))};
+  // ---- the lane's volume item and its table rows (B[i_k][.]) and columns (B[.][i_k])
+  const bool vl = t < NL;
+  const int iv = vl ? t : 0;
+  const int ii[3] = {iv % N1, (iv / N1) % N1, iv / L};
+  // REGTAB (FP32): the rows and columns in registers; FP64 reads them from LDS at the
+  // lane's row / column with compile-time offsets (registers would halve the occupancy)
+  T Br[3][N1], Dr[3][N1], Bc[3][N1], Dc[3][N1];
+  if constexpr (REGTAB) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int e = 0; e < N1; ++e) {
+        Br[k][e] = a.B[ii[k]][e];
+        Dr[k][e] = a.Dq[ii[k]][e];
+        Bc[k][e] = a.B[e][ii[k]];
+        Dc[k][e] = a.Dq[e][ii[k]];
+      }
+  } else if (t < L) {
+    tB[t / N1][t % N1] = a.B[t / N1][t % N1];
+    tD[t / N1][t % N1] = a.Dq[t / N1][t % N1];
+  }
+#define VBR(k_, e_) (REGTAB ? Br[k_][e_] : tB[ii[k_]][e_])
+#define VDR(k_, e_) (REGTAB ? Dr[k_][e_] : tD[ii[k_]][e_])
+#define VBC(k_, e_) (REGTAB ? Bc[k_][e_] : tB[e_][ii[k_]])
+#define VDC(k_, e_) (REGTAB ? Dc[k_][e_] : tD[e_][ii[k_]])
+  const T wq = a.w[ii[0]] * a.w[ii[1]] * a.w[ii[2]];
+  // ---- the lane's face items (f, r1, r2)
+  int fo[FI], fr1[FI], fr2[FI], fbo[FI], fbn[FI], fs1[FI], fs2[FI];
+  bool fin[FI], fact[FI], fout[FI];
+  T fcw[FI], fR1[FI][N1], fR2[FI][N1], fC1[FI][N1], fC2[FI][N1];
+#pragma unroll
+  for (int k = 0; k < FI; ++k) {
+    const int o = t + 64 * k;
+    const bool ok = o < 6 * L;
+    const int oo = ok ? o : 0, f = oo / L, r = oo % L, d = f / 2, up = f % 2;
+    fo[k] = ok ? o : -1;
+    fr1[k] = r % N1;
+    fr2[k] = r / N1;
+    const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+    const int str[3] = {1, N1, L};
+    fbo[k] = (up ? N1 - 1 : 0) * str[d];  // this cell's layer on face f
+    fbn[k] = (up ? 0 : N1 - 1) * str[d];  // the neighbour's layer facing it
+    fs1[k] = str[t1];
+    fs2[k] = str[t2];
+    fin[k] = up ? coord[d] < a.n[d] - 1 : coord[d] > 0;
+    fout[k] = !fin[k] && a.bc[f] == BC_OUTFLOW;
+    fact[k] = ok && (fin[k] || a.bc[f] == BC_DIRICHLET || fout[k]);
+    fcw[k] = a.w[fr1[k]] * a.w[fr2[k]] * a.area[d] * (up ? T(1) : T(-1));
+    if constexpr (REGTAB) {
+#pragma unroll
+      for (int e = 0; e < N1; ++e) {
+        fR1[k][e] = a.B[fr1[k]][e];
+        fR2[k][e] = a.B[fr2[k]][e];
+        fC1[k][e] = a.B[e][fr1[k]];
+        fC2[k][e] = a.B[e][fr2[k]];
+      }
+    }
+  }
+#define FR1(k_, e_) (REGTAB ? fR1[k_][e_] : tB[fr1[k_]][e_])
+#define FR2(k_, e_) (REGTAB ? fR2[k_][e_] : tB[fr2[k_]][e_])
+#define FC1(k_, e_) (REGTAB ? fC1[k_][e_] : tB[e_][fr1[k_]])
+#define FC2(k_, e_) (REGTAB ? fC2[k_][e_] : tB[e_][fr2[k_]])
+  // ---- all global loads: the cell, the neighbour layers, the field (into registers)
+  if (vl) zs[t] = z[c * NL + t];
+  T bvr[3] = {T(0), T(0), T(0)};
+  if (vl)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) bvr[d] = bv[c * 3 * NL + d * NL + t];
+  T bfr[FI];
+#pragma unroll
+  for (int k = 0; k < FI; ++k) {
+    bfr[k] = T(0);
+    if (fo[k] < 0) continue;
+    const int f = fo[k] / L, r = fo[k] % L;
+    bfr[k] = bf[c * 6 * L + fo[k]];
+    T v = T(0);
+    if (fin[k]) {
+      const int d = f / 2;
+      const long long nb = c + ((f % 2) ? a.stride[d] : -a.stride[d]);
+      v = z[nb * NL + fbn[k] + fr1[k] * fs1[k] + fr2[k] * fs2[k]];
+    }
+    zt[f][r] = v;
+  }
+  __syncthreads();
+  // ---- phase 1: along x; face traces along t1
+  if (vl) {
+    const int base = N1 * (ii[1] + N1 * ii[2]);
+    T acc = T(0), accd = T(0);
+#pragma unroll
+    for (int e = 0; e < N1; ++e) {
+      acc += VBR(0, e) * zs[e + base];
+      if (TRANS) accd += VDR(0, e) * zs[e + base];
+    }
+    A1[0][t] = acc;
+    if (TRANS) A1[1][t] = accd;
+  }
+#pragma unroll
+  for (int k = 0; k < FI; ++k) {
+    if (fo[k] < 0) continue;
+    const int f = fo[k] / L;
+    T as = T(0), an = T(0);
+#pragma unroll
+    for (int p1 = 0; p1 < N1; ++p1) {
+      as += FR1(k, p1) * zs[fbo[k] + p1 * fs1[k] + fr2[k] * fs2[k]];
+      an += FR1(k, p1) * zt[f][p1 + N1 * fr2[k]];
+    }
+    F[f][0][fo[k] % L] = as;
+    F[f][1][fo[k] % L] = an;
+  }
+  __syncthreads();
+  // ---- phase 2: along y; the upwinded flux at the face points
+  if (vl) {
+    T acc = T(0), g0 = T(0), g1 = T(0);
+#pragma unroll
+    for (int e = 0; e < N1; ++e) {
+      const int ai = ii[0] + N1 * (e + N1 * ii[2]);
+      acc += VBR(1, e) * A1[0][ai];
+      if (TRANS) {
+        g0 += VBR(1, e) * A1[1][ai];
+        g1 += VDR(1, e) * A1[0][ai];
+      }
+    }
+    A2[0][t] = acc;
+    if (TRANS) {
+      A2[1][t] = g0;
+      A2[2][t] = g1;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < FI; ++k) {
+    if (fo[k] < 0) continue;
+    const int f = fo[k] / L;
+    T v = T(0);
+    if (fact[k]) {
+      T us = T(0), un = T(0);
+#pragma unroll
+      for (int p2 = 0; p2 < N1; ++p2) {
+        us += FR2(k, p2) * F[f][0][fr1[k] + N1 * p2];
+        un += FR2(k, p2) * F[f][1][fr1[k] + N1 * p2];
+      }
+      const T cf = fcw[k] * bfr[k];
+      if (TRANS) v = fout[k] ? cf * us : (cf >= T(0) ? cf * (us - un) : T(0));
+      else v = (fout[k] || cf >= T(0)) ? cf * us : cf * un;
+    }
+    fv[f][fo[k] % L] = v;
+  }
+  __syncthreads();
+  // ---- phase 3: along z and the pointwise factors; the flux tested along t1
+  if (vl) {
+    if (TRANS) {
+      T gx = T(0), gy = T(0), gz = T(0);
+#pragma unroll
+      for (int e = 0; e < N1; ++e) {
+        const int ai = ii[0] + N1 * (ii[1] + N1 * e);
+        gx += VBR(2, e) * A2[1][ai];
+        gy += VBR(2, e) * A2[2][ai];
+        gz += VDR(2, e) * A2[0][ai];
+      }
+      V[0][t] = wq * (a.vs[0] * bvr[0] * gx + a.vs[1] * bvr[1] * gy + a.vs[2] * bvr[2] * gz);
+    } else {
+      T u = T(0);
+#pragma unroll
+      for (int e = 0; e < N1; ++e) u += VBR(2, e) * A2[0][ii[0] + N1 * (ii[1] + N1 * e)];
+      const T wu = wq * u;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) V[d][t] = a.vs[d] * bvr[d] * wu;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < FI; ++k) {
+    if (fo[k] < 0) continue;
+    const int f = fo[k] / L;
+    T acc = T(0);
+#pragma unroll
+    for (int q1 = 0; q1 < N1; ++q1) acc += FC1(k, q1) * fv[f][q1 + N1 * fr2[k]];
+    G[f][fo[k] % L] = acc;
+  }
+  __syncthreads();
+  // ---- phase 4: volume test along x; the flux tested along t2
+  if (vl) {
+    const int base = N1 * (ii[1] + N1 * ii[2]);
+    T w0 = T(0), w1 = T(0), w2 = T(0);
+#pragma unroll
+    for (int q0 = 0; q0 < N1; ++q0) {
+      if (TRANS) {
+        w0 += VBC(0, q0) * V[0][q0 + base];
+      } else {
+        w0 += VDC(0, q0) * V[0][q0 + base];
+        w1 += VBC(0, q0) * V[1][q0 + base];
+        w2 += VBC(0, q0) * V[2][q0 + base];
+      }
+    }
+    W[0][t] = w0;
+    if (!TRANS) {
+      W[1][t] = w1;
+      W[2][t] = w2;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < FI; ++k) {
+    if (fo[k] < 0) continue;
+    const int f = fo[k] / L;
+    T acc = T(0);
+#pragma unroll
+    for (int q2 = 0; q2 < N1; ++q2) acc += FC2(k, q2) * G[f][fr1[k] + N1 * q2];
+    rf[f][fo[k] % L] = acc;
+  }
+  __syncthreads();
+  // ---- phase 5: along y
+  if (vl) {
+    T s0 = T(0), s1 = T(0);
+#pragma unroll
+    for (int q1 = 0; q1 < N1; ++q1) {
+      const int wi = ii[0] + N1 * (q1 + N1 * ii[2]);
+      if (TRANS) {
+        s0 += VBC(1, q1) * W[0][wi];
+      } else {
+        s0 += VBC(1, q1) * W[0][wi] + VDC(1, q1) * W[1][wi];
+        s1 += VBC(1, q1) * W[2][wi];
+      }
+    }
+    S[0][t] = s0;
+    if (!TRANS) S[1][t] = s1;
+  }
+  __syncthreads();
+  // ---- phase 6: along z, plus the faces whose layer holds the DoF
+  if (vl) {
+    const int ij = ii[0] + N1 * ii[1];
+    T acc = T(0);
+#pragma unroll
+    for (int q2 = 0; q2 < N1; ++q2) {
+      acc += VBC(2, q2) * S[0][ij + L * q2];
+      if (!TRANS) acc += VDC(2, q2) * S[1][ij + L * q2];
+    }
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+      const int d = f / 2;
+      if (ii[d] != ((f % 2) ? N1 - 1 : 0)) continue;
+      acc += rf[f][ii[d == 0 ? 1 : 0] + N1 * ii[d == 2 ? 1 : 2]];
+    }
+    y[c * NL + t] += acc;
+  }
+}
+#undef VBR
+#undef VDR
+#undef VBC
+#undef VDC
+#undef FR1
+#undef FR2
+#undef FC1
+#undef FC2
+
 // Entry (i, j) (row i: test function, column j: trial function) of the cell block of the
 // convection terms of cell c: the volume term and the faces' inside-trace parts (the
 // neighbour-trace parts couple to other cells). The point diagonal is entry (i, i).

@@ -182,14 +182,26 @@ int vel_sor_rhs(H* h, const T* d, const T* v, int first, int count, bool inplace
 template int vel_sor_rhs<double>(H*, const double*, const double*, int, int, bool, const double**);
 template int vel_sor_rhs<float>(H*, const float*, const float*, int, int, bool, const float**);
 
+// N1 points per direction: conv_var_sq_kernel (per-lane items and tables in registers);
+// N1 + 1: the general conv_var_kernel.
 template <int N1, int M, typename T>
 static void vel_launch_nm(H* h, const T* z, T* y, const int* stop, bool trans) {
-  if (trans)
-    hipLaunchKernelGGL((conv_var_kernel<N1, M, T, true>), dim3((unsigned)h->ncells), dim3(64), 0, h->stream, z, y,
-                       bv_for<T>(h), bf_for<T>(h), va_for<T>(h), stop);
-  else
-    hipLaunchKernelGGL((conv_var_kernel<N1, M, T, false>), dim3((unsigned)h->ncells), dim3(64), 0, h->stream, z, y,
-                       bv_for<T>(h), bf_for<T>(h), va_for<T>(h), stop);
+  const dim3 grid((unsigned)h->ncells), block(64);
+  if constexpr (M == N1) {
+    if (trans)
+      hipLaunchKernelGGL((conv_var_sq_kernel<N1, T, true>), grid, block, 0, h->stream, z, y, bv_for<T>(h), bf_for<T>(h),
+                         va_for<T>(h), stop);
+    else
+      hipLaunchKernelGGL((conv_var_sq_kernel<N1, T, false>), grid, block, 0, h->stream, z, y, bv_for<T>(h), bf_for<T>(h),
+                         va_for<T>(h), stop);
+  } else {
+    if (trans)
+      hipLaunchKernelGGL((conv_var_kernel<N1, M, T, true>), grid, block, 0, h->stream, z, y, bv_for<T>(h), bf_for<T>(h),
+                         va_for<T>(h), stop);
+    else
+      hipLaunchKernelGGL((conv_var_kernel<N1, M, T, false>), grid, block, 0, h->stream, z, y, bv_for<T>(h), bf_for<T>(h),
+                         va_for<T>(h), stop);
+  }
 }
 
 template <typename T>
