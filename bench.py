@@ -222,8 +222,8 @@ def convection_leg(pkg, checks):
     out["fp64_bicgstab"] = {"seconds": t64, "iterations": r64["solver"]["iterations"],
                             "converged": bool(r64["solver"]["converged"]), "reduction": r64["solver"]["reduction"]}
     xm = go.vector()
-    log("bench: 96^3 Q2 mixed-precision BiCGSTAB")
-    rm = go.solve_mixed(xm, method=pkg.SOLVER_BICGSTAB, inner_reduction=1e-4, outer_reduction=1e-8,
+    log("bench: 96^3 Q2 mixed-precision BiCGSTAB (reliable updates)")
+    rm = go.solve_mixed(xm, method=pkg.SOLVER_BICGSTAB, inner_reduction=1e-2, outer_reduction=1e-8,
                         inner_maxit=20000)
     a64, am = x64.download(), xm.download()
     checks["convection"] = {"cfg": cfg, "first_defect": rm["first_defect"], "target": 1e-8,
@@ -233,7 +233,18 @@ def convection_leg(pkg, checks):
                              "inner_iterations": rm["inner_iterations"], "converged": bool(rm["converged"]),
                              "reduction": rm["reduction"],
                              "rel_diff_vs_fp64_gpu": float(np.linalg.norm(am - a64) / np.linalg.norm(a64)),
-                             "speedup_vs_fp64": t64 / rm["elapsed"] if rm["elapsed"] > 0 else None}
+                             "speedup_vs_fp64": t64 / rm["elapsed"] if rm["elapsed"] > 0 else None,
+                             "note": "one FP32 BiCGSTAB run, FP64 residual replaced at each 1e-2 reduction"}
+    xr = go.vector()
+    log("bench: 96^3 Q2 mixed-precision BiCGSTAB (restarted refinement)")
+    rr = go.solve_mixed(xr, method=pkg.SOLVER_BICGSTAB | pkg.MIXED_RESTARTED, inner_reduction=1e-4,
+                        outer_reduction=1e-8, inner_maxit=20000)
+    out["mixed_bicgstab_restarted"] = {"seconds": rr["elapsed"], "outer": rr["outer_iterations"],
+                                       "inner_iterations": rr["inner_iterations"], "converged": bool(rr["converged"]),
+                                       "reduction": rr["reduction"],
+                                       "speedup_vs_fp64": t64 / rr["elapsed"] if rr["elapsed"] > 0 else None,
+                                       "note": "each FP32 BiCGSTAB solve to 1e-4 from zero"}
+    del xr
     # dJ/ds for J = 1/2 u^T M u: J^T lambda = M u, dJ/ds_c = lambda^T (int_c phi)
     log("bench: 96^3 Q2 adjoint source gradient")
     t0 = time.perf_counter()

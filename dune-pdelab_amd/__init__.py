@@ -440,17 +440,17 @@ class GridOperator:
     def solve_mixed(self, x, method=SOLVER_CG, precond=PREC_JACOBI, inner_reduction=None, inner_maxit=5000,
                     outer_reduction=1e-10, outer_maxit=50, variant=0):
         """FP32 Krylov inside FP64 refinement. variant: 0, MIXED_RESTARTED or MIXED_FLEXIBLE
-        (OR'd into the method, see pdelab_gpu_solve_mixed). inner_reduction defaults to 0.1
-        for reliable-update CG (the FP32 residual reduction between FP64 residual
-        replacements), 1e-2 for the flexible outer iteration and 1e-4 for restarted
-        refinement (the reduction of each inner solve)."""
+        (OR'd into the method, see pdelab_gpu_solve_mixed). inner_reduction defaults to the
+        FP32 residual reduction between FP64 residual replacements with reliable updates, 0.1
+        for CG and 1e-2 for BiCGSTAB, to 1e-2 for the flexible outer iteration and to 1e-4 for
+        restarted refinement (the reduction of each inner solve)."""
         if inner_reduction is None:
             if variant & MIXED_FLEXIBLE:
                 inner_reduction = 1e-2
-            elif method == SOLVER_CG and not variant & MIXED_RESTARTED:
-                inner_reduction = 0.1
-            else:
+            elif variant & MIXED_RESTARTED:
                 inner_reduction = 1e-4
+            else:
+                inner_reduction = 0.1 if method == SOLVER_CG else 1e-2
         res = MixedResult()
         self._check(lib().pdelab_gpu_solve_mixed(self.h, method | variant, precond, inner_reduction, inner_maxit,
                                                  outer_reduction, outer_maxit, self._vec(x), ctypes.byref(res)))

@@ -1099,6 +1099,198 @@ static int cg_mixed_reliable(H* h, int precond, double inner_reduction, int inne
   return PDELAB_OK;
 }
 
+// Mixed-precision BiCGSTAB with reliable updates: one FP32 BiCGSTAB run on J z = R(x) whose
+// recursive residual is replaced by the FP64 residual whenever it has dropped by
+// inner_reduction (or by the reduction still missing), as cg_mixed_reliable does for CG.
+// The FP32 vectors hold the residual at unit norm (r32 = r / scale); at a replacement the
+// correction is subtracted from x in FP64, R(x) is recomputed and rescaled, and the recurrence
+// goes on: p, v, the shadow residual and rho_old stay in the old scale, which cancels in
+// beta = (rho_new / rho_old)(alpha / omega) applied to p - omega v. A run stopped after the
+// first half step resumes with its second half (omega is scale-free); after a full step
+// rho_new = rt.r is taken from the replaced residual. A breakdown restarts the recurrence
+// with the shadow residual set to the current residual.
+static int bicgstab_mixed_reliable(H* h, int precond, double inner_reduction, int inner_maxit, double outer_reduction,
+                                   int outer_maxit, double* x, pdelab_mixed_result* res) {
+  using T = float;
+  using A = float;
+  const long long n = h->N;
+  Prec<T> P;
+  CHK(make_prec<T>(h, precond, false, P));
+  CHK(ensure_ws<double>(h, 6));
+  CHK(ensure_ws<T>(h, 7));
+  double* r64 = h->w64[5];
+  T *r = h->w32[0], *xc = h->w32[1], *p = h->w32[2], *v = h->w32[3], *t = h->w32[4], *y = h->w32[5],
+    *rt = h->w32[6];
+  if (!h->d_bscal) HIPCHK(h, hipMalloc(&h->d_bscal, sizeof(double) * 8));
+  int* st = h->d_state;
+  double* ds = h->d_dstate;
+  double* bs = h->d_bscal;
+  A* part = partial_buf<A>(h);
+  const dim3 rg(kRedBlocks), rb(kRedThreads), fg(1), fb(kFinalThreads);
+  auto residual64 = [&](double& d) -> int {
+    CHK(apply64(h, x, r64, false, false));
+    hipLaunchKernelGGL(axpy_kernel<double>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, 1.0,
+                       (const double*)h->d_R0, r64);
+    CHK(pdelab_gpu_norm(h, r64, &d));
+    if (!std::isfinite(d)) return fail(h, PDELAB_ERR_SOLVER_ABORT, "refinement: residual is infinite or NaN");
+    return PDELAB_OK;
+  };
+  auto to_f32 = [&](double scale) -> int {
+    hipLaunchKernelGGL(scale_to_f32_kernel, dim3(grid_for(n)), dim3(256), 0, h->stream, n, 1.0 / scale,
+                       (const double*)r64, r);
+    HIPCHK(h, hipGetLastError());
+    return PDELAB_OK;
+  };
+  auto prec = [&](const T* in, T* out) -> int {
+    if (P.sor()) return sor_launch<T>(h, in, out, st, false);
+    if (P.ibj()) return ibj_launch<T>(h, in, out, static_cast<int*>(h->d_ibj_stat), nullptr, st, false);
+    if (P.block()) {
+      const BJVec<T, A> bv{in, out, nullptr, nullptr, nullptr, nullptr, st};
+      return bj_launch<T, A>(h, false, BJ_PREC, bv, nullptr);
+    }
+    hipLaunchKernelGGL(prec_stop_kernel<T>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, P.dinv, in, out,
+                       (const int*)st);
+    HIPCHK(h, hipGetLastError());
+    return PDELAB_OK;
+  };
+  // a fresh recurrence: rt = r, rho_new = rt.r, p = r at the first step
+  auto start = [&]() -> int {
+    HIPCHK(h, hipMemcpyAsync(rt, r, sizeof(T) * n, hipMemcpyDeviceToDevice, h->stream));
+    CHK((dot_dev<T, A>(h, r, r, n, 0)));
+    A sc[1];
+    CHK((read_scalars<A>(h, sc, 1)));
+    hipLaunchKernelGGL(bicg_init_kernel, dim3(1), dim3(64), 0, h->stream, bs, (double)sc[0]);
+    HIPCHK(h, hipGetLastError());
+    return PDELAB_OK;
+  };
+  if (P.ibj()) HIPCHK(h, hipMemsetAsync(h->d_ibj_stat, 0, 16, h->stream));
+  double def0 = 0, def = 0;
+  CHK(residual64(def0));
+  def = def0;
+  res->first_defect = def0;
+  int outer = 0, it = 0;
+  long long halves = 0;
+  bool conv = def0 < 1e-30;
+  if (!conv) {
+    double scale = def0;
+    CHK(to_f32(scale));
+    hipLaunchKernelGGL(fill_kernel<T>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, T(0), xc);
+    CHK(start());
+    bool first = true, resume_second = false;
+    for (;;) {
+      // one run of the FP32 recurrence, stopped by the device when its (unit-scaled) defect
+      // falls below thr / scale
+      const double thr = std::max(inner_reduction * def, 0.5 * outer_reduction * def0);
+      h->h_state[0] = h->h_state[1] = h->h_state[2] = h->h_state[3] = 0;
+      h->h_dstate[0] = 1.0;
+      h->h_dstate[1] = 1.0;
+      h->h_dstate[2] = thr / scale;
+      h->h_dstate[3] = 0.0;
+      HIPCHK(h, hipMemcpyAsync(st, h->h_state, sizeof(int) * 4, hipMemcpyHostToDevice, h->stream));
+      HIPCHK(h, hipMemcpyAsync(ds, h->h_dstate, sizeof(double) * 4, hipMemcpyHostToDevice, h->stream));
+      const bool resumed = resume_second;
+      bool stopped = false;
+      int seg = 0;
+      while (!stopped && it < inner_maxit) {
+        ++it;
+        ++seg;
+        if (!resume_second) {
+          hipLaunchKernelGGL(bicg_p_dev_kernel<T>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, p, (const T*)r,
+                             (const T*)v, (const double*)bs, (const int*)st, first ? 1 : 0);
+          CHK(prec(p, y));
+          CHK(apply_mode<T>(h, y, v, false, 0, nullptr, st));
+          hipLaunchKernelGGL((dot_partial_kernel<T, A>), rg, rb, 0, h->stream, (const T*)rt, (const T*)v, n, part);
+          hipLaunchKernelGGL((bicg_scalar_kernel<A, 0>), fg, fb, 0, h->stream, (const A*)part, kRedBlocks, bs, st, ds,
+                             2 * seg - 1);
+          hipLaunchKernelGGL((bicg_xr_kernel<T, A, 1>), rg, rb, 0, h->stream, n, xc, (const T*)y, r, (const T*)v,
+                             (const T*)nullptr, (const double*)bs, kBsAlpha, part, (const int*)st);
+          hipLaunchKernelGGL((bicg_scalar_kernel<A, 1>), fg, fb, 0, h->stream, (const A*)part, kRedBlocks, bs, st, ds,
+                             2 * seg - 1);
+          first = false;
+        }
+        resume_second = false;
+        CHK(prec(r, y));
+        CHK(apply_mode<T>(h, y, t, false, 0, nullptr, st));
+        hipLaunchKernelGGL((dot2_kernel<T, A>), rg, rb, 0, h->stream, n, (const T*)t, (const T*)r, part);
+        hipLaunchKernelGGL((bicg_scalar_kernel<A, 2>), fg, fb, 0, h->stream, (const A*)part, kRedBlocks, bs, st, ds,
+                           2 * seg);
+        hipLaunchKernelGGL((bicg_xr_kernel<T, A, 2>), rg, rb, 0, h->stream, n, xc, (const T*)y, r, (const T*)t,
+                           (const T*)rt, (const double*)bs, kBsOmega, part, (const int*)st);
+        hipLaunchKernelGGL((bicg_scalar_kernel<A, 3>), fg, fb, 0, h->stream, (const A*)part, kRedBlocks, bs, st, ds,
+                           2 * seg);
+        HIPCHK(h, hipGetLastError());
+        if (seg % kBicgCheckEvery == 0 || it >= inner_maxit) {
+          HIPCHK(h, hipMemcpyAsync(h->h_state, st, sizeof(int) * 4, hipMemcpyDeviceToHost, h->stream));
+          HIPCHK(h, hipStreamSynchronize(h->stream));
+          stopped = h->h_state[0] != 0;
+        }
+      }
+      HIPCHK(h, hipMemcpyAsync(h->h_state, st, sizeof(int) * 4, hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(h, hipStreamSynchronize(h->stream));
+      stopped = h->h_state[0] != 0;
+      const int code = h->h_state[1];
+      const int half = stopped ? h->h_state[2] : 2 * seg;  // the half step at which the run stopped
+      halves += half - (resumed ? 1 : 0);
+      if (stopped) it -= seg - (half + 1) / 2;  // iterations queued after the stop did nothing
+      if (code == BICG_NAN) {
+        res->inner_iterations = it;
+        return fail(h, PDELAB_ERR_SOLVER_ABORT, "mixed BiCGSTAB: FP32 defect is infinite or NaN");
+      }
+      // residual replacement: x -= scale xc, r = R(x) in FP64
+      hipLaunchKernelGGL(sub_scaled_f32_kernel, dim3(grid_for(n)), dim3(256), 0, h->stream, n, scale, (const T*)xc,
+                         x);
+      hipLaunchKernelGGL(fill_kernel<T>, dim3(grid_for(n)), dim3(256), 0, h->stream, n, T(0), xc);
+      HIPCHK(h, hipGetLastError());
+      CHK(residual64(def));
+      ++outer;
+      if (def <= outer_reduction * def0 || def < 1e-30) {
+        conv = true;
+        break;
+      }
+      if (outer >= outer_maxit || it >= inner_maxit) break;
+      scale = def;
+      CHK(to_f32(scale));
+      if (code != BICG_RUNNING) {  // a breakdown of the FP32 recurrence: start it again
+        CHK(start());
+        first = true;
+        continue;
+      }
+      if (half % 2 == 1) {  // stopped after the first half step: its second half follows
+        resume_second = true;
+        continue;
+      }
+      // stopped after a full step: rho_new from the replaced residual, then beta
+      CHK((dot_dev<T, A>(h, rt, r, n, 0)));
+      A sc[1];
+      CHK((read_scalars<A>(h, sc, 1)));
+      double b[5];
+      HIPCHK(h, hipMemcpy(b, bs, sizeof(b), hipMemcpyDeviceToHost));
+      b[kBsRhoNew] = (double)sc[0];
+      if (std::fabs(b[kBsRho]) <= kBicgEps || std::fabs(b[kBsOmega]) <= kBicgEps) {
+        CHK(start());
+        first = true;
+        continue;
+      }
+      b[kBsBeta] = (b[kBsRhoNew] / b[kBsRho]) * (b[kBsAlpha] / b[kBsOmega]);
+      HIPCHK(h, hipMemcpy(bs, b, sizeof(b), hipMemcpyHostToDevice));
+    }
+  }
+  if (P.ibj()) {
+    int ierr = 0;
+    HIPCHK(h, hipMemcpy(&ierr, h->d_ibj_stat, sizeof(int), hipMemcpyDeviceToHost));
+    if (ierr != 0)
+      return fail(h, PDELAB_ERR_SOLVER_ABORT,
+                  "iterative block Jacobi: breakdown in a block solve (code " + std::to_string(ierr) + ")");
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  res->converged = conv ? 1 : 0;
+  res->outer_iterations = outer;
+  res->inner_iterations = (int)((halves + 1) / 2);
+  res->defect = def;
+  res->reduction = def0 > 0 ? def / def0 : 0.0;
+  return PDELAB_OK;
+}
+
 extern "C" {
 
 void pdelab_gpu_default_config(pdelab_config* cfg) {
@@ -1540,10 +1732,13 @@ int pdelab_gpu_solve_mixed(pdelab_gpu_gridoperator* h, int method, int precond, 
     res->elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return PDELAB_OK;
   }
-  if (method == PDELAB_SOLVER_CG && !(variant & PDELAB_MIXED_RESTARTED)) {
+  if (!(variant & PDELAB_MIXED_RESTARTED)) {
     auto t0 = std::chrono::steady_clock::now();
     CHK(compute_R0(h));
-    CHK(cg_mixed_reliable(h, precond, inner_reduction, inner_maxit, outer_reduction, outer_maxit, x, res));
+    if (method == PDELAB_SOLVER_CG)
+      CHK(cg_mixed_reliable(h, precond, inner_reduction, inner_maxit, outer_reduction, outer_maxit, x, res));
+    else
+      CHK(bicgstab_mixed_reliable(h, precond, inner_reduction, inner_maxit, outer_reduction, outer_maxit, x, res));
     res->elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return PDELAB_OK;
   }

@@ -378,15 +378,17 @@ int pdelab_gpu_linear_problem_solve(pdelab_gpu_gridoperator* h, int method, int 
                                     double reduction, double min_defect, int maxit, double* x,
                                     pdelab_linear_problem_result* res);
 /* Mixed precision: FP32 Krylov inside FP64 refinement until ||R(x)|| <= outer_reduction *
-   ||R(x0)||. BiCGSTAB: x <- x - J32^{-1} R(x), each inner solve to inner_reduction (or, when
-   weaker, to a quarter of the reduction still missing). CG: one FP32 CG run with reliable
-   updates; whenever its recursive residual has dropped by inner_reduction, the FP32
-   correction is subtracted from x in FP64 and the residual replaced by the FP64 R(x), with
-   the search direction kept (0.1 is a good interval; much smaller ones let the FP32
-   recursion drift and stall). outer_iterations counts the FP64 residual replacements. */
+   ||R(x0)||: one FP32 CG or BiCGSTAB run with reliable updates; whenever its recursive
+   residual has dropped by inner_reduction, the FP32 correction is subtracted from x in FP64
+   and the residual replaced by the FP64 R(x), with the recurrence kept (CG's search direction;
+   BiCGSTAB's direction, shadow residual and rho). 0.1 is a good interval for CG, 0.01 for
+   BiCGSTAB; much smaller ones let the FP32 recursion drift and stall. outer_iterations counts
+   the FP64 residual replacements. */
 /* Variants of pdelab_gpu_solve_mixed, OR'd into its method argument:
-   PDELAB_MIXED_RESTARTED  CG also as restarted refinement (each FP32 solve to inner_reduction,
-                           a new Krylov space per refinement step) instead of reliable updates;
+   PDELAB_MIXED_RESTARTED  restarted refinement instead of reliable updates:
+                           x <- x - J32^{-1} R(x), each FP32 solve to inner_reduction (or, when
+                           weaker, to a quarter of the reduction still missing), a new Krylov
+                           space per refinement step;
    PDELAB_MIXED_FLEXIBLE   flexible GMRES(10) in FP64 as the outer iteration, preconditioned
                            by the FP32 solve (CG or BiCGSTAB to inner_reduction from zero on a
                            unit-norm right-hand side); outer_iterations then counts FGMRES steps
