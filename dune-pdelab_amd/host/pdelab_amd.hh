@@ -712,14 +712,14 @@ template <class GO>
 using ISTLBackend_SEQ_MatrixFree_BCGS_IterativeBlockJac =
     ISTLBackend_SEQ_MatrixFree_Jac<GO, Krylov::BiCGSTAB, Preconditioner::IterativeBlockJacobi>;
 
-// Mixed precision backend: FP32 inner Krylov inside FP64 refinement (CG: reliable
-// updates, inner_reduction = FP32 residual reduction between FP64 residual replacements;
-// BiCGSTAB: restarted inner solves to inner_reduction); used in place of the linear
-// problem solver (the refinement owns the residual evaluations).
+// Mixed precision backend: FP32 inner Krylov inside FP64 refinement with reliable updates
+// (inner_reduction = FP32 residual reduction between FP64 residual replacements, 0.1 for CG,
+// 1e-2 for BiCGSTAB); used in place of the linear problem solver (the refinement owns the
+// residual evaluations).
 template <class GO, Krylov method, Preconditioner prec = Preconditioner::Jacobi>
 class MixedPrecisionSolver {
  public:
-  MixedPrecisionSolver(const GO& go, double inner_reduction = method == Krylov::CG ? 0.1 : 1e-4,
+  MixedPrecisionSolver(const GO& go, double inner_reduction = method == Krylov::CG ? 0.1 : 1e-2,
                        unsigned inner_maxit = 20000)
       : go_(go), inner_(inner_reduction), inner_maxit_(inner_maxit) {}
   pdelab_mixed_result apply(typename GO::Traits::Domain& x, double reduction, int outer_maxit = 50) {
