@@ -445,7 +445,7 @@ def counter_summary(d):
     (SQ_WAVE_CYCLES counts quad-cycles: 4 SQ_WAVE_CYCLES / cycles / 256, rocprofv3's own
     OccupancyPercent formula), HBM
     traffic over algorithmic bytes (FETCH_SIZE x 2 and WRITE_SIZE x 1, the axpy calibration).
-    conv_var_kernel (the velocity field's convection terms, y += C z) is measured against its own
+    conv_var_*kernel (the velocity field's convection terms, y += C z) is measured against its own
     algorithmic bytes: z read, y read and written (24 B/DoF FP64), and the field at the 27 volume
     and 54 face points of a Q2 cell (135 values per cell)."""
     import collections
@@ -461,7 +461,7 @@ def counter_summary(d):
         dur[r["Kernel_Name"]].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
 
     def short(k):
-        m = re.search(r"(dg_apply\w*kernel|conv_var_kernel)<[^(]*>", k)
+        m = re.search(r"(dg_apply\w*kernel|conv_var\w*kernel)<[^(]*>", k)
         return m.group(0) if m else None
 
     out = ["| kernel | workload | us | HBM frac | VALU busy | FP64 util | LDS busy | LDS conflict | SALU busy "
@@ -479,7 +479,7 @@ def counter_summary(d):
         n, k = (64, 3) if "Q3" in wl else (96, 2)
         N = n ** 3 * (k + 1) ** 3
         bpd = 16.0 if "double" in s else 8.0
-        if s.startswith("conv_var_kernel"):
+        if s.startswith("conv_var"):
             es = 8.0 if "double" in s else 4.0
             bpd = 3 * es + 135 * es / (k + 1) ** 3
         waves = c["SQ_WAVES"]
