@@ -18,6 +18,8 @@
   block-matrix apply against the matrix-free apply, one block-SOR step (from zero and in place)
   against the oracle's, the mixed-precision BiCGSTAB with the FP32 sweep, BiCGSTAB with block Jacobi and with the iterative block Jacobi against the oracle's
   solves, one iterative block Jacobi application against the oracle's;
+* the one-step stage operator a M + b dt J and two Alexander2 steps (BiCGSTAB, point and block
+  Jacobi) against the oracle's;
 * a 24^3 Q2 apply against the oracle's two-sided apply;
 * the rejections: the configuration's constant b not zero, wrong array sizes, inflow on an
   outflow side, CG;
@@ -219,6 +221,35 @@ def test_velocity_cell_blocks(pkg, of, name, kw, seed, field):
     rm = go.solve_mixed(xm, method=1, precond=pkg.PREC_BLOCK_SOR, inner_reduction=1e-4, outer_reduction=1e-10)
     assert rm["converged"], rm
     assert np.linalg.norm(o.residual(xm.download())) <= 1.01e-10 * np.linalg.norm(o.residual(np.zeros(go.N)))
+    go.close()
+
+
+def test_velocity_onestep(pkg, of):
+    """The instationary stage operator a M + b dt J with the field, and two Alexander2 steps
+    (BiCGSTAB with point and with block Jacobi), against the oracle's."""
+    say("[gpu] velocity field one-step operator and Alexander2 steps")
+    kw = dict(n=5, degree=2, alpha=3.0, kappa0=0.05, problem=2, f0=1.0)
+    cfg, go, o = make_field(pkg, of, kw, None, "divfree")
+    o.set_threads(8)
+    rng = np.random.default_rng(43)
+    z, y0 = rng.uniform(-1, 1, go.N), rng.uniform(-1, 1, go.N)
+    for a_mass, b_dt in ((1.0, 0.05), (0.0, 2.0), (1.0, 1e-4)):
+        yd = go.vector(y0)
+        go.onestep_jacobian_apply(a_mass, b_dt, go.vector(z), yd)
+        assert rel(yd.download(), o.onestep_jacobian_apply(a_mass, b_dt, z, y0.copy())) <= 1e-12, (a_mass, b_dt)
+    x0 = rng.normal(size=go.N) * 0.1
+    for precond in (pkg.PREC_JACOBI, pkg.PREC_BLOCK_JACOBI):
+        xo, xn = go.vector(x0), go.vector(x0)
+        x_ref = x0.copy()
+        for _ in range(2):
+            res = go.onestep_step(xo, xn, scheme=pkg.TS_ALEXANDER2, dt=0.05, method=pkg.SOLVER_BICGSTAB,
+                                  precond=precond, reduction=1e-12)
+            assert res["converged"] and res["stages"] == 2
+            xo.copy_from(xn)
+            x_ref, rr = o.onestep_step(x_ref, pkg.TS_ALEXANDER2, 0.05, xnew=x_ref, method=pkg.SOLVER_BICGSTAB,
+                                       precond=precond, reduction=1e-12)
+            assert rr["converged"]
+        assert rel(xn.download(), x_ref) <= 1e-8
     go.close()
 
 
