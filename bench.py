@@ -515,6 +515,8 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-convection", action="store_true", help="skip the 96^3 Q2 convection-diffusion leg")
+    ap.add_argument("--convection-only", action="store_true",
+                    help="of the solve legs run only the 96^3 Q2 convection-diffusion one (for its profile)")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the applies of the Q2 and cell-wise-kappa operators (apply_variants)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -624,7 +626,12 @@ def main():
         out["apply_variants"] = variants
 
     checks = {}  # solutions whose oracle residual the cpu_baseline leg evaluates
-    if not args.no_solve:
+    if args.convection_only and not args.no_solve:
+        z.free()
+        y.free()
+        go.close()
+        out["convection"] = convection_leg(pkg, checks)
+    elif not args.no_solve:
         solve = {}
         # plain FP64 CG/Jacobi on the manufactured problem (64^3 Q3)
         x = go.vector()
